@@ -1,0 +1,155 @@
+/*
+ * zr_mpm.h -- C-ABI of the MI355X MPM time-step engine.
+ *
+ * The engine replaces the CPU/TBB MPM step of penn-graphics-research/ziran2019
+ * behind its own simulation interface.  Each entry point names the reference
+ * virtual or function it stands in for (file:line in the reference tree).
+ * Plain pointers and sizes only; no exceptions cross this boundary; every call
+ * is stream-ordered on the engine's own HIP stream and returns when its results
+ * are visible to the host.  One host thread per engine.
+ *
+ * Host data layout (the layout of the reference's per-attribute arrays,
+ * DataArray<T>::array, Lib/Ziran/CS/DataStructure/DataArray.h:43-53):
+ *   vectors  : 3 doubles per particle, particle-major (Vector<double,3>)
+ *   matrices : 9 doubles per particle, COLUMN-MAJOR (Matrix<double,3,3>)
+ * The engine keeps its own structure-of-arrays copy in HBM.
+ */
+#ifndef ZR_MPM_H
+#define ZR_MPM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: 0 = OK, > 0 control signals, < 0 errors */
+enum {
+  ZR_OK = 0,
+  /* a particle moved more than one cell in the step: the reference throws
+     RestartException (MpmSimulationBase.cpp:983-1002, SimulationBase.h:21-29) */
+  ZR_CFL_RESTART = 1,
+  ZR_ERR_ARG = -1,
+  ZR_ERR_HIP = -2,
+  /* a particle's base node left [0, 2^grid_log2 - 8] on some axis */
+  ZR_ERR_DOMAIN = -3,
+  ZR_ERR_STATE = -4
+};
+
+enum { ZR_FP64 = 0 };
+enum { ZR_TRANSFER_MLS = 0, ZR_TRANSFER_APIC = 1 };
+
+/* collision object types, AnalyticCollisionObject::COLLISION_OBJECT_TYPE
+   (Lib/Ziran/Math/Geometry/CollisionObject.h:47-52) */
+enum { ZR_STICKY = 1, ZR_SLIP = 2, ZR_SEPARATE = 3 };
+enum { ZR_HALF_SPACE = 0, ZR_SPHERE = 1 };
+
+typedef struct zr_config {
+  double dx;            /* MpmSimulationBase::dx */
+  double gravity[3];    /* MpmSimulationBase::gravity */
+  double cfl;           /* MpmSimulationBase::cfl (used by the CFL flags) */
+  int32_t grid_log2;    /* node coordinates live in [0, 2^grid_log2); 6..10 */
+  int32_t precision;    /* ZR_FP64 */
+  int32_t transfer;     /* ZR_TRANSFER_MLS (mls_mpm = true) */
+  int32_t reserved;
+} zr_config;
+
+/* An analytic collision object with a constant-velocity translation:
+   AnalyticCollisionObject + HalfSpace / Sphere level sets
+   (CollisionObject.cpp:48-105,340-405; AnalyticLevelSet.cpp:259-287,435-450). */
+typedef struct zr_collider {
+  int32_t shape;        /* ZR_HALF_SPACE or ZR_SPHERE */
+  int32_t type;         /* ZR_STICKY, ZR_SLIP or ZR_SEPARATE */
+  double friction;
+  double origin[3];     /* half-space point / sphere centre at time 0 */
+  double normal[3];     /* half-space outward normal */
+  double radius;        /* sphere radius */
+  double velocity[3];   /* dbdt */
+} zr_collider;
+
+/* Host views of the particle attributes (Particles m/X/V plus the F-based
+   force helper attributes, Particles.h:8-45, FBasedMpmForceHelper.h:17-60). */
+typedef struct zr_particles {
+  int64_t n;
+  double* X;       /* 3n  */
+  double* V;       /* 3n  */
+  double* C;       /* 9n, APIC/MLS affine matrix (C_name<TM>()) */
+  double* F;       /* 9n, deformation gradient */
+  double* mass;    /* n   */
+  double* vol;     /* n, element measure V0 */
+  double* mu;      /* n, per-particle constitutive model (fixed-corotated) */
+  double* lambda;  /* n   */
+} zr_particles;
+
+typedef struct zr_step_stats {
+  int64_t num_groups;   /* particle pages (particle_group.size()) */
+  int64_t num_pages;    /* touched SPGrid pages (page_map->Get_Blocks().second) */
+  int32_t cfl_flags;    /* bit0: > 1 cell, bit1: > cfl/2 cell */
+  int32_t reserved;
+} zr_step_stats;
+
+typedef struct zr_engine zr_engine;
+
+/* MpmSimulationBase constructor + setDx (MpmSimulationBase.cpp:33-72) */
+int zr_create(const zr_config* cfg, zr_engine** out);
+void zr_destroy(zr_engine* e);
+/* message of the last failed call on this engine */
+const char* zr_last_error(const zr_engine* e);
+
+/* addCollisionObject (MpmSimulationBase.cpp:84-88); replaces the whole set */
+int zr_set_colliders(zr_engine* e, const zr_collider* c, int32_t n);
+/* SimulationBase::step.time, used for moving colliders */
+int zr_set_time(zr_engine* e, double t);
+double zr_get_time(const zr_engine* e);
+
+/* Upload / download of the particle attributes: Scene/DataManager state
+   (DataManager.h:67-346).  Download fills X, V, C, F (NULL pointers skip). */
+int zr_upload_particles(zr_engine* e, const zr_particles* p);
+int zr_download_particles(zr_engine* e, zr_particles* p);
+
+/* MpmSimulationBase::sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1032-1103) */
+int zr_sort_particles_and_pollute_grid(zr_engine* e, zr_step_stats* st);
+/* forcesUpdateParticleState + particlesToGrid, MLS with force, then getNumNodes
+   and velocity normalisation (MpmSimulationBase.cpp:437-561,681-709) */
+int zr_particles_to_grid(zr_engine* e);
+/* gridVelocityExplicitUpdate + collider updateState(t+dt) + trialCollision
+   (MpmSimulationBase.cpp:267-271,298-312,818-865) */
+int zr_grid_velocity_explicit_update(zr_engine* e, double dt);
+/* gridToParticles (MLS/APIC) + evolveStrain (MpmSimulationBase.cpp:879-1016);
+   returns ZR_CFL_RESTART when a particle moved more than one cell */
+int zr_grid_to_particles(zr_engine* e, double dt, zr_step_stats* st);
+/* MpmSimulationBase::advanceOneTimeStep, symplectic branch
+   (MpmSimulationBase.cpp:256-281); advances the engine time by dt */
+int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st);
+
+/* Introspection for parity tests (reference state after the named call):
+   sizes in bytes via zr_debug_size, contents via zr_debug_dump. */
+enum {
+  ZR_DUMP_SORTER = 0,       /* u64[n]  particle_sorter after the sort */
+  ZR_DUMP_ORDER = 1,        /* i32[n]  particle_order */
+  ZR_DUMP_BASE_OFFSET = 2,  /* u64[n]  particle_base_offset (by particle) */
+  ZR_DUMP_GROUP_FIRST = 3,  /* i64[G]  particle_group[g].first */
+  ZR_DUMP_GROUP_LAST = 4,   /* i64[G]  particle_group[g].second */
+  ZR_DUMP_BLOCK_OFFSET = 5, /* u64[G]  block_offset (page code) */
+  ZR_DUMP_PAGES = 6,        /* u64[P]  page_map block offsets, first-touch order */
+  ZR_DUMP_GRID_M = 7,       /* f64[32P] node mass, page-list order x in-page order */
+  ZR_DUMP_GRID_V = 8,       /* f64[3*32P] normalised node velocity */
+  ZR_DUMP_GRID_F = 9,       /* f64[3*32P] rasterised force (new_v after P2G) */
+  ZR_DUMP_GRID_VSTAR = 10,  /* f64[3*32P] new_v after the grid update */
+  ZR_DUMP_NODE_IDX = 11     /* i64[32P] GridState::idx (getNumNodes numbering) */
+};
+int64_t zr_debug_size(zr_engine* e, int32_t what);
+int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes);
+
+/* Kernel timing with HIP events recorded on the engine's stream around each
+   launch.  Instrumentation for measurement; it stands in for the reference's
+   ZIRAN_TIMER scoped timers (Lib/Ziran/CS/Util/Timer.cpp:34-95), which time
+   host code.  Enabling resets the accumulated times. */
+enum { ZR_PROF_P2G = 0, ZR_PROF_GRID = 1, ZR_PROF_G2P = 2, ZR_PROF_SORT = 3, ZR_PROF_COUNT = 4 };
+int zr_profile_enable(zr_engine* e, int32_t on);
+int zr_profile_read(zr_engine* e, int32_t which, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

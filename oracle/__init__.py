@@ -1,0 +1,264 @@
+"""FP64 CPU oracle for the MPM time step -- TEST INFRASTRUCTURE ONLY.
+
+ctypes loader for ``oracle/_build/libzr_oracle.so`` (built by
+``oracle/Makefile``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package; the product path
+(``ziran2019_amd``) never does.
+
+The C sources restate penn-graphics-research/ziran2019 (file:line citations in
+each function).  Arrays follow the C-ABI conventions: vectors are 3 doubles per
+particle, matrices 9 doubles per particle in column-major (Eigen) order.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libzr_oracle.so")
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+_p = ctypes.c_void_p
+
+
+class ZroCollider(ctypes.Structure):
+    _fields_ = [
+        ("shape", ctypes.c_int32),
+        ("type", ctypes.c_int32),
+        ("friction", _d),
+        ("origin", _d * 3),
+        ("normal", _d * 3),
+        ("radius", _d),
+        ("velocity", _d * 3),
+    ]
+
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.zro_linear_offset.restype = _u64
+        L.zro_linear_offset.argtypes = [_i, _i, _i]
+        L.zro_linear_to_coord.argtypes = [_u64, _p]
+        L.zro_packed_add.restype = _u64
+        L.zro_packed_add.argtypes = [_u64, _u64]
+        L.zro_bspline_weights.argtypes = [_d, _p, _p, _p]
+        L.zro_svd3.restype = _i
+        L.zro_svd3.argtypes = [_p, _p, _p, _p]
+        L.zro_polar3.argtypes = [_p, _p, _p]
+        L.zro_fcr_psi.restype = _d
+        L.zro_fcr_psi.argtypes = [_d, _d, _p]
+        for f in ("zro_fcr_first_piola", "zro_fcr_kirchhoff"):
+            getattr(L, f).argtypes = [_d, _d, _p, _p]
+        L.zro_fcr_first_piola_differential.argtypes = [_d, _d, _p, _p, _p]
+        L.zro_lame.argtypes = [_d, _d, _p, _p]
+        L.zro_create.restype = _p
+        L.zro_create.argtypes = [_d, _p, _d, _i]
+        L.zro_destroy.argtypes = [_p]
+        L.zro_set_particles.argtypes = [_p, _i64] + [_p] * 8
+        L.zro_get_particles.argtypes = [_p, _p, _p, _p, _p]
+        L.zro_add_collider.restype = _i
+        L.zro_add_collider.argtypes = [_p, ctypes.POINTER(ZroCollider)]
+        L.zro_set_time.argtypes = [_p, _d]
+        L.zro_sort_and_pollute.argtypes = [_p]
+        L.zro_num_groups.restype = _i64
+        L.zro_num_groups.argtypes = [_p]
+        L.zro_num_pages.restype = _i64
+        L.zro_num_pages.argtypes = [_p]
+        L.zro_get_sort.argtypes = [_p, _p, _p, _p]
+        L.zro_get_groups.argtypes = [_p, _p, _p, _p]
+        L.zro_get_pages.argtypes = [_p, _p]
+        L.zro_update_stress.argtypes = [_p]
+        L.zro_p2g_mls.restype = _i64
+        L.zro_p2g_mls.argtypes = [_p]
+        L.zro_grid_update.argtypes = [_p, _d]
+        L.zro_g2p_mls.restype = _i
+        L.zro_g2p_mls.argtypes = [_p, _d]
+        L.zro_step_explicit.restype = _i
+        L.zro_step_explicit.argtypes = [_p, _d]
+        L.zro_get_grid.argtypes = [_p, _p, _p, _p, _p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _f64(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.size == n, (a.shape, n)
+    return a
+
+
+# ---------------------------------------------------------------- functions
+def linear_offset(i, j, k):
+    return int(lib().zro_linear_offset(int(i), int(j), int(k)))
+
+
+def linear_to_coord(off):
+    out = np.zeros(3, dtype=np.int32)
+    lib().zro_linear_to_coord(ctypes.c_uint64(off), _ptr(out))
+    return tuple(int(v) for v in out)
+
+
+def packed_add(a, b):
+    return int(lib().zro_packed_add(ctypes.c_uint64(a), ctypes.c_uint64(b)))
+
+
+def bspline_weights(x):
+    base = np.zeros(1, dtype=np.int32)
+    w = np.zeros(3)
+    dw = np.zeros(3)
+    lib().zro_bspline_weights(float(x), _ptr(base), _ptr(w), _ptr(dw))
+    return int(base[0]), w, dw
+
+
+def _cm(M):
+    """3x3 numpy matrix -> column-major 9-vector."""
+    return np.ascontiguousarray(np.asarray(M, dtype=np.float64).T.reshape(9))
+
+
+def _from_cm(a):
+    return np.asarray(a, dtype=np.float64).reshape(3, 3).T.copy()
+
+
+def svd3(F):
+    U = np.zeros(9)
+    s = np.zeros(3)
+    V = np.zeros(9)
+    n = lib().zro_svd3(_ptr(_cm(F)), _ptr(U), _ptr(s), _ptr(V))
+    return _from_cm(U), s, _from_cm(V), n
+
+
+def polar3(F):
+    R = np.zeros(9)
+    S = np.zeros(9)
+    lib().zro_polar3(_ptr(_cm(F)), _ptr(R), _ptr(S))
+    return _from_cm(R), _from_cm(S)
+
+
+def lame(E, nu):
+    mu = np.zeros(1)
+    lam = np.zeros(1)
+    lib().zro_lame(float(E), float(nu), _ptr(mu), _ptr(lam))
+    return float(mu[0]), float(lam[0])
+
+
+def fcr_psi(mu, lam, F):
+    return float(lib().zro_fcr_psi(mu, lam, _ptr(_cm(F))))
+
+
+def fcr_first_piola(mu, lam, F):
+    P = np.zeros(9)
+    lib().zro_fcr_first_piola(mu, lam, _ptr(_cm(F)), _ptr(P))
+    return _from_cm(P)
+
+
+def fcr_kirchhoff(mu, lam, F):
+    t = np.zeros(9)
+    lib().zro_fcr_kirchhoff(mu, lam, _ptr(_cm(F)), _ptr(t))
+    return _from_cm(t)
+
+
+def fcr_first_piola_differential(mu, lam, F, dF):
+    dP = np.zeros(9)
+    lib().zro_fcr_first_piola_differential(mu, lam, _ptr(_cm(F)), _ptr(_cm(dF)), _ptr(dP))
+    return _from_cm(dP)
+
+
+# ---------------------------------------------------------------- simulation
+class OracleSim:
+    """FP64 CPU MPM simulation (explicit MLS step), restating
+    MpmSimulationBase<double,3> (Lib/MPM/MpmSimulationBase.cpp)."""
+
+    def __init__(self, dx, gravity=(0.0, -9.8, 0.0), cfl=0.6, nthreads=1):
+        g = np.asarray(gravity, dtype=np.float64)
+        self._g = g
+        self.h = lib().zro_create(float(dx), _ptr(g), float(cfl), int(nthreads))
+        self.dx = dx
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().zro_destroy(self.h)
+            self.h = None
+
+    def set_particles(self, X, V, C, F, mass, vol, mu, lam):
+        n = len(mass)
+        self.n = n
+        arrs = [_f64(X, 3 * n), _f64(V, 3 * n), _f64(C, 9 * n), _f64(F, 9 * n),
+                _f64(mass, n), _f64(vol, n), _f64(mu, n), _f64(lam, n)]
+        lib().zro_set_particles(self.h, n, *[_ptr(a) for a in arrs])
+
+    def add_collider(self, shape, type, origin, normal=(0, 1, 0), radius=0.0,
+                     velocity=(0, 0, 0), friction=0.0):
+        c = ZroCollider(int(shape), int(type), float(friction), (_d * 3)(*origin),
+                        (_d * 3)(*normal), float(radius), (_d * 3)(*velocity))
+        assert lib().zro_add_collider(self.h, ctypes.byref(c)) == 0
+
+    def set_time(self, t):
+        lib().zro_set_time(self.h, float(t))
+
+    def get_particles(self):
+        n = self.n
+        X = np.zeros(3 * n); V = np.zeros(3 * n); C = np.zeros(9 * n); F = np.zeros(9 * n)
+        lib().zro_get_particles(self.h, _ptr(X), _ptr(V), _ptr(C), _ptr(F))
+        return X.reshape(n, 3), V.reshape(n, 3), C.reshape(n, 9), F.reshape(n, 9)
+
+    def sort_and_pollute(self):
+        lib().zro_sort_and_pollute(self.h)
+
+    def sort_result(self):
+        n = self.n
+        sorter = np.zeros(n, dtype=np.uint64)
+        order = np.zeros(n, dtype=np.int32)
+        base = np.zeros(n, dtype=np.uint64)
+        lib().zro_get_sort(self.h, _ptr(sorter), _ptr(order), _ptr(base))
+        ng = lib().zro_num_groups(self.h)
+        first = np.zeros(ng, dtype=np.int64)
+        last = np.zeros(ng, dtype=np.int64)
+        blk = np.zeros(ng, dtype=np.uint64)
+        lib().zro_get_groups(self.h, _ptr(first), _ptr(last), _ptr(blk))
+        npg = lib().zro_num_pages(self.h)
+        pages = np.zeros(npg, dtype=np.uint64)
+        lib().zro_get_pages(self.h, _ptr(pages))
+        return dict(sorter=sorter, order=order, base_offset=base, group_first=first,
+                    group_last=last, block_offset=blk, pages=pages)
+
+    def update_stress(self):
+        lib().zro_update_stress(self.h)
+
+    def p2g(self):
+        return int(lib().zro_p2g_mls(self.h))
+
+    def grid_update(self, dt):
+        lib().zro_grid_update(self.h, float(dt))
+
+    def g2p(self, dt):
+        return int(lib().zro_g2p_mls(self.h, float(dt)))
+
+    def step(self, dt):
+        return int(lib().zro_step_explicit(self.h, float(dt)))
+
+    def grid(self):
+        npg = lib().zro_num_pages(self.h)
+        k = npg * 32
+        m = np.zeros(k); v = np.zeros(3 * k); nv = np.zeros(3 * k)
+        idx = np.zeros(k, dtype=np.int64)
+        lib().zro_get_grid(self.h, _ptr(m), _ptr(v), _ptr(nv), _ptr(idx))
+        return dict(m=m, v=v.reshape(k, 3), new_v=nv.reshape(k, 3), idx=idx)

@@ -1,0 +1,98 @@
+/*
+ * zr_oracle.h -- FP64 CPU oracle for the MPM time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (penn-graphics-research/ziran2019) used as the checker for the
+ * HIP engine.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product path never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Pinning (what
+ * checks each function against something other than its own formula) lives in
+ * tests/test_oracle_*.py; see the per-function notes there.
+ *
+ * Conventions (shared with the C-ABI in include/zr_mpm.h):
+ *   - vectors are 3 doubles per particle, particle-major (Eigen StdVector
+ *     layout of Vector<double,3>);
+ *   - 3x3 matrices are 9 doubles per particle in COLUMN-MAJOR order (Eigen's
+ *     default storage of Matrix<double,3,3>): m[c*3+r] = M(r,c).
+ */
+#ifndef ZR_ORACLE_H
+#define ZR_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- SPGrid (Lib/SPGrid/Core/SPGrid_Mask.h, GridState<double,3>) ---- */
+uint64_t zro_linear_offset(int i, int j, int k);
+void zro_linear_to_coord(uint64_t off, int* ijk);
+uint64_t zro_packed_add(uint64_t a, uint64_t b);
+
+/* ---- quadratic B-spline (Lib/Ziran/Math/Splines/BSplines.h:55-81) ---- */
+void zro_bspline_weights(double x, int* base, double* w3, double* dw3);
+
+/* ---- 3x3 SVD (Lib/Ziran/Math/Linear/ImplicitQRSVD.h:355-533) ---- */
+int zro_svd3(const double* F, double* U, double* sigma, double* V);
+void zro_polar3(const double* F, double* R, double* S);
+
+/* ---- fixed-corotated model (textbook; no counterpart in the reference) --- */
+double zro_fcr_psi(double mu, double lam, const double* F);
+void zro_fcr_first_piola(double mu, double lam, const double* F, double* P);
+void zro_fcr_kirchhoff(double mu, double lam, const double* F, double* tau);
+void zro_fcr_first_piola_differential(double mu, double lam, const double* F,
+                                      const double* dF, double* dP);
+void zro_lame(double E, double nu, double* mu, double* lam);
+
+/* ---- simulation state ---- */
+typedef struct zro_sim zro_sim;
+
+/* collider description, identical field meaning to zr_collider */
+typedef struct {
+  int32_t shape;        /* 0 half-space, 1 sphere */
+  int32_t type;         /* 1 STICKY, 2 SLIP, 3 SEPARATE (CollisionObject.h:47-52) */
+  double friction;
+  double origin[3];     /* half-space point / sphere centre at t=0 */
+  double normal[3];     /* half-space outward normal (normalised here) */
+  double radius;
+  double velocity[3];   /* constant translation velocity (dbdt) */
+} zro_collider;
+
+zro_sim* zro_create(double dx, const double* gravity, double cfl, int nthreads);
+void zro_destroy(zro_sim* s);
+void zro_set_particles(zro_sim* s, int64_t n, const double* X, const double* V,
+                       const double* C, const double* F, const double* mass,
+                       const double* vol, const double* mu, const double* lam);
+void zro_get_particles(const zro_sim* s, double* X, double* V, double* C, double* F);
+int zro_add_collider(zro_sim* s, const zro_collider* c);
+void zro_set_time(zro_sim* s, double t);
+
+/* sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1032-1103) */
+void zro_sort_and_pollute(zro_sim* s);
+int64_t zro_num_groups(const zro_sim* s);
+int64_t zro_num_pages(const zro_sim* s);
+void zro_get_sort(const zro_sim* s, uint64_t* sorter, int32_t* order, uint64_t* base_offset);
+void zro_get_groups(const zro_sim* s, int64_t* first, int64_t* last, uint64_t* block_offset);
+void zro_get_pages(const zro_sim* s, uint64_t* page_offsets);
+
+/* forcesUpdateParticleState + particlesToGrid (MLS) + getNumNodes +
+   normalisation: MpmSimulationBase.cpp:437-561,681-709 */
+void zro_update_stress(zro_sim* s);
+int64_t zro_p2g_mls(zro_sim* s);
+/* gridVelocityExplicitUpdate + collider update + trialCollision
+   (MpmSimulationBase.cpp:298-312,818-865) */
+void zro_grid_update(zro_sim* s, double dt);
+/* gridToParticles MLS/APIC + evolveStrain (MpmSimulationBase.cpp:906-1016) */
+int zro_g2p_mls(zro_sim* s, double dt);
+/* the whole symplectic step (MpmSimulationBase.cpp:256-281) */
+int zro_step_explicit(zro_sim* s, double dt);
+
+/* grid fields, per node in page-list order x 32 elements:
+   m[npages*32], v[npages*32*3] (normalised), new_v (force, then v*),
+   idx[npages*32] (node numbering, -1 for m==0) */
+void zro_get_grid(const zro_sim* s, double* m, double* v, double* new_v, int64_t* idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

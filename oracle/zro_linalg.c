@@ -1,0 +1,471 @@
+/*
+ * zro_linalg.c -- oracle (TEST INFRASTRUCTURE ONLY): 3x3 SVD by implicit-
+ * shifted QR and the fixed-corotated constitutive model, FP64.
+ *
+ * The SVD restates the method of Gast, Fu, Jiang, Teran, "Implicit-shifted
+ * Symmetric QR Singular Value Decomposition of 3x3 Matrices" (UCLA tech report
+ * 2016), in the form the reference uses:
+ *   Lib/Ziran/Math/Linear/Givens.h:32-271          (Givens, zeroChase, makeUpperBidiag)
+ *   Lib/Ziran/Math/Linear/ImplicitQRSVD.h:45-163   (2x2 polar / SVD)
+ *   Lib/Ziran/Math/Linear/ImplicitQRSVD.h:220-345  (wilkinsonShift, process, flipSign, sort)
+ *   Lib/Ziran/Math/Linear/ImplicitQRSVD.h:355-533  (3x3 driver, tolerances)
+ * Output convention (same as the reference): F = U diag(sigma) V^T with U, V
+ * rotations (det +1), |sigma0| >= |sigma1| >= |sigma2|, sigma0, sigma1 >= 0,
+ * only sigma2 may be negative.
+ * Deviation: the QR sweep loop is capped at 64 iterations (the reference loop
+ * is unbounded); convergence takes far fewer for finite input.
+ *
+ * Fixed-corotated (Stomakhin et al. 2012; Jiang et al. 2016 MPM course notes):
+ *   psi = mu |F - R|^2 + lam/2 (J-1)^2
+ *   P   = 2 mu (F - R) + lam (J-1) J F^-T
+ *   dP  = 2 mu (dF - dR) + lam (JF^-T : dF) JF^-T + lam (J-1) d(JF^-T)
+ *   with U^T dR V = skew part W, W_ij = (M_ij - M_ji)/(s_i + s_j), M = U^T dF V.
+ * The reference has no fixed-corotated model in Lib/ (only a commented-out
+ * CorotatedElasticity in Projects/admm/AdmmFormula.h:57-693); Lame parameters
+ * follow NeoHookeanBorden::setLameParameters (NeoHookeanBorden.cpp:16-21).
+ */
+#include <math.h>
+#include <float.h>
+#include <string.h>
+#include "zr_oracle.h"
+
+typedef double M3[3][3];
+
+/* column-major 9-array <-> row-major [3][3] */
+static void from_cm(const double* a, M3 m)
+{
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) m[r][c] = a[c * 3 + r];
+}
+static void to_cm(M3 m, double* a)
+{
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) a[c * 3 + r] = m[r][c];
+}
+static void m_ident(M3 m)
+{
+  memset(m, 0, sizeof(M3));
+  m[0][0] = m[1][1] = m[2][2] = 1.0;
+}
+
+/* ---------------- Givens rotations (Givens.h:32-180) ---------------- */
+typedef struct { int i, k; double c, s; } givens;
+
+static givens gv_make(int i, int k) { givens g = { i, k, 1.0, 0.0 }; return g; }
+/* (c -s; s c)(a; b) = (*; 0)  -- Givens.h:73-87 */
+static void gv_compute(givens* g, double a, double b)
+{
+  double d = a * a + b * b;
+  g->c = 1.0; g->s = 0.0;
+  double sq = sqrt(d);
+  if (sq != 0.0) { double t = 1.0 / sq; g->c = a * t; g->s = -b * t; }
+}
+/* (c -s; s c)(a; b) = (0; *)  -- Givens.h:94-107 */
+static void gv_compute_unconv(givens* g, double a, double b)
+{
+  double d = a * a + b * b;
+  g->c = 0.0; g->s = 1.0;
+  double sq = sqrt(d);
+  if (sq != 0.0) { double t = 1.0 / sq; g->s = a * t; g->c = b * t; }
+}
+/* G^T A on rows i,k -- Givens.h:129-139 */
+static void gv_row(const givens* g, M3 a)
+{
+  for (int j = 0; j < 3; ++j) {
+    double t1 = a[g->i][j], t2 = a[g->k][j];
+    a[g->i][j] = g->c * t1 - g->s * t2;
+    a[g->k][j] = g->s * t1 + g->c * t2;
+  }
+}
+/* A G on columns i,k -- Givens.h:148-158 */
+static void gv_col(const givens* g, M3 a)
+{
+  for (int j = 0; j < 3; ++j) {
+    double t1 = a[j][g->i], t2 = a[j][g->k];
+    a[j][g->i] = g->c * t1 - g->s * t2;
+    a[j][g->k] = g->s * t1 + g->c * t2;
+  }
+}
+
+/* zero chasing of a 3x3 with one bulge at (1,0): Givens.h:196-239 */
+static void zero_chase(M3 h, M3 u, M3 v)
+{
+  givens r1 = gv_make(0, 1);
+  gv_compute(&r1, h[0][0], h[1][0]);
+  givens r2 = gv_make(1, 2);
+  if (h[1][0] != 0.0)
+    gv_compute(&r2, h[0][0] * h[0][1] + h[1][0] * h[1][1], h[0][0] * h[0][2] + h[1][0] * h[1][2]);
+  else
+    gv_compute(&r2, h[0][1], h[0][2]);
+  gv_row(&r1, h);
+  gv_col(&r2, h);
+  gv_col(&r2, v);
+  givens r3 = gv_make(1, 2);
+  gv_compute(&r3, h[1][1], h[2][1]);
+  gv_row(&r3, h);
+  gv_col(&r1, u);
+  gv_col(&r3, u);
+}
+
+/* Givens.h:251-271 */
+static void make_upper_bidiag(M3 h, M3 u, M3 v)
+{
+  m_ident(u);
+  m_ident(v);
+  givens r = gv_make(1, 2);
+  gv_compute(&r, h[1][0], h[2][0]);
+  gv_row(&r, h);
+  gv_col(&r, u);
+  zero_chase(h, u, v);
+}
+
+/* ---------------- 2x2 polar / SVD (ImplicitQRSVD.h:45-163) ----------- */
+/* a = [a00 a01; a10 a11]; returns rotation (as c,s) and symmetric s2 */
+static void polar2(double a00, double a01, double a10, double a11, givens* r, double s2[2][2])
+{
+  double x0 = a00 + a11, x1 = a10 - a01;
+  double den = sqrt(x0 * x0 + x1 * x1);
+  r->c = 1.0; r->s = 0.0;
+  if (den != 0.0) { r->c = x0 / den; r->s = -x1 / den; }
+  /* S = G^T A on the 2x2 */
+  s2[0][0] = r->c * a00 - r->s * a10;
+  s2[0][1] = r->c * a01 - r->s * a11;
+  s2[1][0] = r->s * a00 + r->c * a10;
+  s2[1][1] = r->s * a01 + r->c * a11;
+}
+
+static void svd2(double a00, double a01, double a10, double a11, givens* u, double* sig, givens* v)
+{
+  double s2[2][2];
+  polar2(a00, a01, a10, a11, u, s2);
+  double x = s2[0][0], y = s2[0][1], z = s2[1][1];
+  double y2 = y * y, cosine, sine;
+  if (y2 == 0.0) {
+    cosine = 1.0; sine = 0.0;
+    sig[0] = x; sig[1] = z;
+  } else {
+    double tau = 0.5 * (x - z);
+    double w = sqrt(tau * tau + y2);
+    double t = (tau > 0.0) ? y / (tau + w) : y / (tau - w);
+    cosine = 1.0 / sqrt(t * t + 1.0);
+    sine = -t * cosine;
+    double c2 = cosine * cosine, csy = 2.0 * cosine * sine * y, sn2 = sine * sine;
+    sig[0] = c2 * x - csy + sn2 * z;
+    sig[1] = sn2 * x + csy + c2 * z;
+  }
+  if (sig[0] < sig[1]) {
+    double t = sig[0]; sig[0] = sig[1]; sig[1] = t;
+    v->c = -sine; v->s = cosine;
+  } else {
+    v->c = cosine; v->s = sine;
+  }
+  /* U *= V (compose two rotations of the same plane): Givens.h:164-170 */
+  double nc = u->c * v->c - u->s * v->s;
+  double ns = u->s * v->c + u->c * v->s;
+  u->c = nc; u->s = ns;
+}
+
+/* ImplicitQRSVD.h:220-232 */
+static double wilkinson_shift(double a1, double b1, double a2)
+{
+  double d = 0.5 * (a1 - a2);
+  double bs = b1 * b1;
+  return a2 - copysign(bs / (fabs(d) + sqrt(d * d + bs)), d);
+}
+
+/* process<t>: 2x2 SVD of the block at (t,t): ImplicitQRSVD.h:237-252 */
+static void process_block(int t, M3 b, M3 u, double* sig, M3 v)
+{
+  int other = (t == 1) ? 0 : 2;
+  sig[other] = b[other][other];
+  givens gu = gv_make(t, t + 1), gvv = gv_make(t, t + 1);
+  svd2(b[t][t], b[t][t + 1], b[t + 1][t], b[t + 1][t + 1], &gu, sig + t, &gvv);
+  gv_col(&gu, u);
+  gv_col(&gvv, v);
+}
+
+static void flip_sign(int i, M3 u, double* sig)
+{
+  sig[i] = -sig[i];
+  for (int r = 0; r < 3; ++r) u[r][i] = -u[r][i];
+}
+static void swap_cols(M3 a, int i, int k)
+{
+  for (int r = 0; r < 3; ++r) { double t = a[r][i]; a[r][i] = a[r][k]; a[r][k] = t; }
+}
+static void neg_col(M3 a, int i)
+{
+  for (int r = 0; r < 3; ++r) a[r][i] = -a[r][i];
+}
+static void swap_d(double* a, double* b) { double t = *a; *a = *b; *b = t; }
+
+/* sort<0>: ImplicitQRSVD.h:257-294 */
+static void sort0(M3 u, double* sig, M3 v)
+{
+  if (fabs(sig[1]) >= fabs(sig[2])) {
+    if (sig[1] < 0) { flip_sign(1, u, sig); flip_sign(2, u, sig); }
+    return;
+  }
+  if (sig[2] < 0) { flip_sign(1, u, sig); flip_sign(2, u, sig); }
+  swap_d(&sig[1], &sig[2]); swap_cols(u, 1, 2); swap_cols(v, 1, 2);
+  if (sig[1] > sig[0]) {
+    swap_d(&sig[0], &sig[1]); swap_cols(u, 0, 1); swap_cols(v, 0, 1);
+  } else {
+    neg_col(u, 2); neg_col(v, 2);
+  }
+}
+
+/* sort<1>: ImplicitQRSVD.h:299-334 */
+static void sort1(M3 u, double* sig, M3 v)
+{
+  if (fabs(sig[0]) >= sig[1]) {
+    if (sig[0] < 0) { flip_sign(0, u, sig); flip_sign(2, u, sig); }
+    return;
+  }
+  swap_d(&sig[0], &sig[1]); swap_cols(u, 0, 1); swap_cols(v, 0, 1);
+  if (fabs(sig[1]) < fabs(sig[2])) {
+    swap_d(&sig[1], &sig[2]); swap_cols(u, 1, 2); swap_cols(v, 1, 2);
+  } else {
+    neg_col(u, 1); neg_col(v, 1);
+  }
+  if (sig[1] < 0) { flip_sign(1, u, sig); flip_sign(2, u, sig); }
+}
+
+#define ZRO_SVD_MAX_SWEEPS 64
+
+/* ImplicitQRSVD.h:355-517 with tol = 1024 eps (ImplicitQRSVD.h:526-533) */
+static int svd3_rm(M3 a, M3 u, double* sig, M3 v)
+{
+  double tol = 1024.0 * DBL_EPSILON;
+  M3 b;
+  memcpy(b, a, sizeof(M3));
+  make_upper_bidiag(b, u, v);
+  int count = 0;
+  double a1 = b[0][0], b1 = b[0][1], a2 = b[1][1], a3 = b[2][2], b2 = b[1][2];
+  double g1 = a1 * b1, g2 = a2 * b2;
+  double scale = 0.5 * sqrt(a1 * a1 + a2 * a2 + a3 * a3 + b1 * b1 + b2 * b2);
+  tol *= (scale > 1.0 ? scale : 1.0);
+  while (fabs(b2) > tol && fabs(b1) > tol && fabs(a1) > tol && fabs(a2) > tol &&
+         fabs(a3) > tol && count < ZRO_SVD_MAX_SWEEPS) {
+    double mu = wilkinson_shift(a2 * a2 + b1 * b1, g2, a3 * a3 + b2 * b2);
+    givens r = gv_make(0, 1);
+    gv_compute(&r, a1 * a1 - mu, g1);
+    gv_col(&r, b);
+    gv_col(&r, v);
+    zero_chase(b, u, v);
+    a1 = b[0][0]; b1 = b[0][1]; a2 = b[1][1]; a3 = b[2][2]; b2 = b[1][2];
+    g1 = a1 * b1; g2 = a2 * b2;
+    count++;
+  }
+  if (fabs(b2) <= tol) {
+    process_block(0, b, u, sig, v);
+    sort0(u, sig, v);
+  } else if (fabs(b1) <= tol) {
+    process_block(1, b, u, sig, v);
+    sort1(u, sig, v);
+  } else if (fabs(a2) <= tol) {
+    givens r1 = gv_make(1, 2);
+    gv_compute_unconv(&r1, b[1][2], b[2][2]);
+    gv_row(&r1, b);
+    gv_col(&r1, u);
+    process_block(0, b, u, sig, v);
+    sort0(u, sig, v);
+  } else if (fabs(a3) <= tol) {
+    givens r1 = gv_make(1, 2);
+    gv_compute(&r1, b[1][1], b[1][2]);
+    gv_col(&r1, b);
+    gv_col(&r1, v);
+    givens r2 = gv_make(0, 2);
+    gv_compute(&r2, b[0][0], b[0][2]);
+    gv_col(&r2, b);
+    gv_col(&r2, v);
+    process_block(0, b, u, sig, v);
+    sort0(u, sig, v);
+  } else if (fabs(a1) <= tol) {
+    givens r1 = gv_make(0, 1);
+    gv_compute_unconv(&r1, b[0][1], b[1][1]);
+    gv_row(&r1, b);
+    gv_col(&r1, u);
+    givens r2 = gv_make(0, 2);
+    gv_compute_unconv(&r2, b[0][2], b[2][2]);
+    gv_row(&r2, b);
+    gv_col(&r2, u);
+    process_block(1, b, u, sig, v);
+    sort1(u, sig, v);
+  } else {
+    /* only reachable when the sweep cap was hit: treat as converged */
+    process_block(0, b, u, sig, v);
+    sort0(u, sig, v);
+  }
+  return count;
+}
+
+int zro_svd3(const double* F, double* U, double* sigma, double* V)
+{
+  M3 a, u, v;
+  from_cm(F, a);
+  int c = svd3_rm(a, u, sigma, v);
+  to_cm(u, U);
+  to_cm(v, V);
+  return c;
+}
+
+/* ImplicitQRSVD.h:546-558 */
+void zro_polar3(const double* F, double* R, double* S)
+{
+  M3 a, u, v, r, s;
+  double sig[3];
+  from_cm(F, a);
+  svd3_rm(a, u, sig, v);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0, accs = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        acc += u[i][k] * v[j][k];
+        accs += v[i][k] * sig[k] * v[j][k];
+      }
+      r[i][j] = acc;
+      s[i][j] = accs;
+    }
+  to_cm(r, R);
+  to_cm(s, S);
+}
+
+/* ---------------- fixed-corotated ---------------- */
+void zro_lame(double E, double nu, double* mu, double* lam)
+{
+  *lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  *mu = E / (2.0 * (1.0 + nu));
+}
+
+static double det3(M3 f)
+{
+  return f[0][0] * (f[1][1] * f[2][2] - f[1][2] * f[2][1])
+       - f[0][1] * (f[1][0] * f[2][2] - f[1][2] * f[2][0])
+       + f[0][2] * (f[1][0] * f[2][1] - f[1][1] * f[2][0]);
+}
+
+/* cofactor matrix J F^-T (DenseExt.h cofactorMatrix) */
+static void cofactor3(M3 f, M3 c)
+{
+  c[0][0] = f[1][1] * f[2][2] - f[1][2] * f[2][1];
+  c[0][1] = f[1][2] * f[2][0] - f[1][0] * f[2][2];
+  c[0][2] = f[1][0] * f[2][1] - f[1][1] * f[2][0];
+  c[1][0] = f[0][2] * f[2][1] - f[0][1] * f[2][2];
+  c[1][1] = f[0][0] * f[2][2] - f[0][2] * f[2][0];
+  c[1][2] = f[0][1] * f[2][0] - f[0][0] * f[2][1];
+  c[2][0] = f[0][1] * f[1][2] - f[0][2] * f[1][1];
+  c[2][1] = f[0][2] * f[1][0] - f[0][0] * f[1][2];
+  c[2][2] = f[0][0] * f[1][1] - f[0][1] * f[1][0];
+}
+
+/* differential of the cofactor matrix along d (bilinear in f, d) */
+static void cofactor3_diff(M3 f, M3 d, M3 c)
+{
+#define CD(a, b, e, g) (d[a][b] * f[e][g] + f[a][b] * d[e][g])
+  c[0][0] = CD(1, 1, 2, 2) - CD(1, 2, 2, 1);
+  c[0][1] = CD(1, 2, 2, 0) - CD(1, 0, 2, 2);
+  c[0][2] = CD(1, 0, 2, 1) - CD(1, 1, 2, 0);
+  c[1][0] = CD(0, 2, 2, 1) - CD(0, 1, 2, 2);
+  c[1][1] = CD(0, 0, 2, 2) - CD(0, 2, 2, 0);
+  c[1][2] = CD(0, 1, 2, 0) - CD(0, 0, 2, 1);
+  c[2][0] = CD(0, 1, 1, 2) - CD(0, 2, 1, 1);
+  c[2][1] = CD(0, 2, 1, 0) - CD(0, 0, 1, 2);
+  c[2][2] = CD(0, 0, 1, 1) - CD(0, 1, 1, 0);
+#undef CD
+}
+
+static void rotation_of(M3 f, M3 r, M3 u, double* sig, M3 v)
+{
+  svd3_rm(f, u, sig, v);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r[i][j] = u[i][0] * v[j][0] + u[i][1] * v[j][1] + u[i][2] * v[j][2];
+}
+
+double zro_fcr_psi(double mu, double lam, const double* F)
+{
+  M3 f, u, v;
+  double sig[3];
+  from_cm(F, f);
+  svd3_rm(f, u, sig, v);
+  double J = det3(f);
+  double a = (sig[0] - 1.0) * (sig[0] - 1.0) + (sig[1] - 1.0) * (sig[1] - 1.0) + (sig[2] - 1.0) * (sig[2] - 1.0);
+  return mu * a + 0.5 * lam * (J - 1.0) * (J - 1.0);
+}
+
+void zro_fcr_first_piola(double mu, double lam, const double* F, double* P)
+{
+  M3 f, r, u, v, c, p;
+  double sig[3];
+  from_cm(F, f);
+  rotation_of(f, r, u, sig, v);
+  cofactor3(f, c);
+  double J = det3(f);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      p[i][j] = 2.0 * mu * (f[i][j] - r[i][j]) + lam * (J - 1.0) * c[i][j];
+  to_cm(p, P);
+}
+
+void zro_fcr_kirchhoff(double mu, double lam, const double* F, double* tau)
+{
+  M3 f, r, u, v, t;
+  double sig[3];
+  from_cm(F, f);
+  rotation_of(f, r, u, sig, v);
+  double J = det3(f);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k) acc += (f[i][k] - r[i][k]) * f[j][k];
+      t[i][j] = 2.0 * mu * acc + (i == j ? lam * (J - 1.0) * J : 0.0);
+    }
+  to_cm(t, tau);
+}
+
+void zro_fcr_first_piola_differential(double mu, double lam, const double* F,
+                                      const double* dF, double* dP)
+{
+  M3 f, d, r, u, v, c, dc, m, w, dr, p;
+  double sig[3];
+  from_cm(F, f);
+  from_cm(dF, d);
+  rotation_of(f, r, u, sig, v);
+  cofactor3(f, c);
+  cofactor3_diff(f, d, dc);
+  double J = det3(f);
+  /* M = U^T dF V */
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) acc += u[k][i] * d[k][l] * v[l][j];
+      m[i][j] = acc;
+    }
+  /* skew W */
+  for (int i = 0; i < 3; ++i) w[i][i] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = i + 1; j < 3; ++j) {
+      double den = sig[i] + sig[j];
+      if (fabs(den) < 1e-12) den = (den < 0 ? -1e-12 : 1e-12);
+      double x = (m[i][j] - m[j][i]) / den;
+      w[i][j] = x;
+      w[j][i] = -x;
+    }
+  /* dR = U W V^T */
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) acc += u[i][k] * w[k][l] * v[j][l];
+      dr[i][j] = acc;
+    }
+  double cd = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) cd += c[i][j] * d[i][j];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      p[i][j] = 2.0 * mu * (d[i][j] - dr[i][j]) + lam * cd * c[i][j] + lam * (J - 1.0) * dc[i][j];
+  to_cm(p, dP);
+}

@@ -1,0 +1,632 @@
+/*
+ * zro_mpm.c -- oracle (TEST INFRASTRUCTURE ONLY): SPGrid indexing, particle
+ * sort + grid pollution, quadratic B-splines and the explicit MLS-MPM step,
+ * FP64, restated from the reference:
+ *   Lib/SPGrid/Core/SPGrid_Mask.h:22-305, SPGrid_Utilities.{h:80-358,cpp:43-62}
+ *   Lib/SPGrid/Core/SPGrid_Page_Map.h:53-97      (first-touch block list)
+ *   Lib/MPM/MpmGrid.h:14-34,55-78,123-271        (GridState, weights, iteration)
+ *   Lib/Ziran/Math/Splines/BSplines.h:10-81, Lib/Ziran/Math/MathTools.h:15-25
+ *   Lib/MPM/MpmSimulationBase.cpp:256-312,437-561,681-709,818-865,906-1103
+ *   Lib/MPM/Force/FBasedMpmForceHelper.cpp:44-64,95-110
+ *   Lib/Ziran/Math/Geometry/CollisionObject.cpp:48-105,340-405
+ *   Lib/Ziran/Math/Geometry/AnalyticLevelSet.cpp:111-119,259-287,435-450
+ * P2G runs the reference's 8-colour page schedule with OpenMP
+ * (MpmSimulationBase.h:252-265, MpmSimulationBase.cpp:499-502).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "zr_oracle.h"
+
+/* ---------------- SPGrid masks for GridState<double,3> ----------------
+ * sizeof(GridState<double,3>) = 128 B -> data_bits = 7, block_bits = 5,
+ * block (x,y,z) bits = (1,2,2), lob = 1 (SPGrid_Mask.h:27-47,76-123):
+ *   page_xmask = (0x4924..24 << 1) & ~0xfff, element_xmask = 1 << 11
+ *   page_ymask = (0x2492..92 << 1) & ~0xfff, element_ymask = 3 << 9
+ *   page_zmask = (0x9249..49 << 1) & ~0xfff, element_zmask = 3 << 7       */
+#define ZRO_XMASK 0x9249249249249800ULL
+#define ZRO_YMASK 0x4924924924924600ULL
+#define ZRO_ZMASK 0x2492492492492180ULL
+#define ZRO_WMASK (ZRO_XMASK | ZRO_YMASK | ZRO_ZMASK)
+
+/* Bit_Spread_u64 / pdep (SPGrid_Utilities.h:80-358) */
+static uint64_t spread(uint64_t data, uint64_t mask)
+{
+  uint64_t r = 0;
+  for (uint64_t bit = 1; bit; bit <<= 1)
+    if (mask & bit) { r |= (data & 1ULL) ? bit : 0; data >>= 1; }
+  return r;
+}
+/* Bit_Pack (SPGrid_Utilities.cpp:43-62) */
+static int64_t pack(uint64_t data, uint64_t mask)
+{
+  uint64_t r = 0;
+  int count = 0;
+  for (uint64_t bit = 1; bit; bit <<= 1)
+    if (mask & bit) r |= (data & bit) >> count;
+    else count++;
+  return (int64_t)r;
+}
+
+/* Linear_Offset (SPGrid_Mask.h:150-174); coordinates are sign-extended */
+uint64_t zro_linear_offset(int i, int j, int k)
+{
+  return spread((uint64_t)(int64_t)i, ZRO_XMASK) | spread((uint64_t)(int64_t)j, ZRO_YMASK) |
+         spread((uint64_t)(int64_t)k, ZRO_ZMASK);
+}
+void zro_linear_to_coord(uint64_t off, int* ijk)
+{
+  ijk[0] = (int)pack(off, ZRO_XMASK);
+  ijk[1] = (int)pack(off, ZRO_YMASK);
+  ijk[2] = (int)pack(off, ZRO_ZMASK);
+}
+/* Packed_Add (SPGrid_Mask.h:237-245) */
+uint64_t zro_packed_add(uint64_t i, uint64_t j)
+{
+  uint64_t x = ((i | ~ZRO_XMASK) + (j & ZRO_XMASK)) & ZRO_XMASK;
+  uint64_t y = ((i | ~ZRO_YMASK) + (j & ZRO_YMASK)) & ZRO_YMASK;
+  uint64_t z = ((i | ~ZRO_ZMASK) + (j & ZRO_ZMASK)) & ZRO_ZMASK;
+  uint64_t w = ((i | ZRO_WMASK) + (j & ~ZRO_WMASK)) & ~ZRO_WMASK;
+  return x | y | z | w;
+}
+
+/* MathTools.h:21-25 */
+static int int_floor(double x)
+{
+  int i = (int)x;
+  return i - (i > x);
+}
+
+/* BSplines.h:16-20,55-81 (quadratic) */
+void zro_bspline_weights(double x, int* base, double* w, double* dw)
+{
+  int b = int_floor(x - 0.5);
+  double d0 = x - b;
+  double z = 1.5 - d0;
+  w[0] = 0.5 * z * z;
+  double d1 = d0 - 1.0;
+  w[1] = 0.75 - d1 * d1;
+  double d2 = 1.0 - d1;
+  double zz = 1.5 - d2;
+  w[2] = 0.5 * zz * zz;
+  if (dw) { dw[0] = -z; dw[1] = -2.0 * d1; dw[2] = zz; }
+  *base = b;
+}
+
+/* ---------------- simulation state ---------------- */
+typedef struct {
+  double v[3];
+  double m;
+  double new_v[3];
+  int64_t idx;
+} zro_node;
+
+#define ZRO_MAX_COLLIDERS 16
+
+struct zro_sim {
+  double dx, one_over_dx, D_inverse, gravity[3], cfl, time;
+  int nthreads;
+  int64_t n;
+  double *X, *V, *C, *F, *mass, *vol, *mu, *lam;
+  double *vtau, *gradV;
+  uint64_t* sorter;
+  int32_t* order;
+  uint64_t* base_offset;
+  int64_t ngroups;
+  int64_t *gfirst, *glast;
+  uint64_t* block_offset;
+  int64_t npages, pages_cap;
+  uint64_t* pages;             /* page base offsets, first-touch order */
+  uint64_t* hkeys;             /* hash: page code + 1 (0 = empty) */
+  int64_t* hvals;
+  int64_t hcap;
+  zro_node* nodes;             /* npages * 32 */
+  int64_t num_nodes;
+  char* explicit_collided;     /* per node slot */
+  int ncol;
+  zro_collider col[ZRO_MAX_COLLIDERS];
+};
+
+zro_sim* zro_create(double dx, const double* gravity, double cfl, int nthreads)
+{
+  zro_sim* s = (zro_sim*)calloc(1, sizeof(zro_sim));
+  s->dx = dx;
+  s->one_over_dx = 1.0 / dx;
+  s->D_inverse = 4.0 / (dx * dx); /* MpmSimulationBase.cpp:114-116 */
+  for (int d = 0; d < 3; ++d) s->gravity[d] = gravity ? gravity[d] : 0.0;
+  s->cfl = cfl;
+  s->nthreads = nthreads > 0 ? nthreads : 1;
+  return s;
+}
+
+static void free_particles(zro_sim* s)
+{
+  free(s->X); free(s->V); free(s->C); free(s->F); free(s->mass); free(s->vol);
+  free(s->mu); free(s->lam); free(s->vtau); free(s->gradV); free(s->sorter);
+  free(s->order); free(s->base_offset); free(s->gfirst); free(s->glast);
+  free(s->block_offset);
+}
+
+void zro_destroy(zro_sim* s)
+{
+  if (!s) return;
+  free_particles(s);
+  free(s->pages); free(s->hkeys); free(s->hvals); free(s->nodes);
+  free(s->explicit_collided);
+  free(s);
+}
+
+static double* dup(const double* a, int64_t n)
+{
+  double* r = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  if (a) memcpy(r, a, sizeof(double) * (size_t)n);
+  else memset(r, 0, sizeof(double) * (size_t)(n > 0 ? n : 1));
+  return r;
+}
+
+void zro_set_particles(zro_sim* s, int64_t n, const double* X, const double* V,
+                       const double* C, const double* F, const double* mass,
+                       const double* vol, const double* mu, const double* lam)
+{
+  free_particles(s);
+  s->n = n;
+  s->X = dup(X, 3 * n); s->V = dup(V, 3 * n); s->C = dup(C, 9 * n); s->F = dup(F, 9 * n);
+  s->mass = dup(mass, n); s->vol = dup(vol, n); s->mu = dup(mu, n); s->lam = dup(lam, n);
+  s->vtau = dup(NULL, 9 * n); s->gradV = dup(NULL, 9 * n);
+  size_t nn = (size_t)(n > 0 ? n : 1);
+  s->sorter = (uint64_t*)malloc(sizeof(uint64_t) * nn);
+  s->order = (int32_t*)malloc(sizeof(int32_t) * nn);
+  s->base_offset = (uint64_t*)malloc(sizeof(uint64_t) * nn);
+  s->gfirst = (int64_t*)malloc(sizeof(int64_t) * nn);
+  s->glast = (int64_t*)malloc(sizeof(int64_t) * nn);
+  s->block_offset = (uint64_t*)malloc(sizeof(uint64_t) * nn);
+  s->ngroups = 0;
+}
+
+void zro_get_particles(const zro_sim* s, double* X, double* V, double* C, double* F)
+{
+  if (X) memcpy(X, s->X, sizeof(double) * 3 * (size_t)s->n);
+  if (V) memcpy(V, s->V, sizeof(double) * 3 * (size_t)s->n);
+  if (C) memcpy(C, s->C, sizeof(double) * 9 * (size_t)s->n);
+  if (F) memcpy(F, s->F, sizeof(double) * 9 * (size_t)s->n);
+}
+
+int zro_add_collider(zro_sim* s, const zro_collider* c)
+{
+  if (s->ncol >= ZRO_MAX_COLLIDERS) return -1;
+  zro_collider k = *c;
+  if (k.shape == 0) { /* HalfSpace normalises its normal (AnalyticLevelSet.cpp:262-266) */
+    double l = sqrt(k.normal[0] * k.normal[0] + k.normal[1] * k.normal[1] + k.normal[2] * k.normal[2]);
+    for (int d = 0; d < 3; ++d) k.normal[d] /= l;
+  }
+  s->col[s->ncol++] = k;
+  return 0;
+}
+
+void zro_set_time(zro_sim* s, double t) { s->time = t; }
+
+/* ---------------- page hash ---------------- */
+static int64_t hash_find(const zro_sim* s, uint64_t code)
+{
+  uint64_t key = code + 1, h = (code * 0x9E3779B97F4A7C15ULL) & (uint64_t)(s->hcap - 1);
+  while (s->hkeys[h]) {
+    if (s->hkeys[h] == key) return s->hvals[h];
+    h = (h + 1) & (uint64_t)(s->hcap - 1);
+  }
+  return -1;
+}
+/* SPGrid_Page_Map::Set_Page (SPGrid_Page_Map.h:61-70): first touch appends */
+static void set_page(zro_sim* s, uint64_t offset)
+{
+  uint64_t code = offset >> 12;
+  uint64_t key = code + 1, h = (code * 0x9E3779B97F4A7C15ULL) & (uint64_t)(s->hcap - 1);
+  while (s->hkeys[h]) {
+    if (s->hkeys[h] == key) return;
+    h = (h + 1) & (uint64_t)(s->hcap - 1);
+  }
+  s->hkeys[h] = key;
+  s->hvals[h] = s->npages;
+  s->pages[s->npages++] = (offset >> 12) << 12;
+}
+
+static int cmp_u64(const void* a, const void* b)
+{
+  uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? -1 : (x > y);
+}
+
+/* MpmSimulationBase.cpp:1032-1103 */
+void zro_sort_and_pollute(zro_sim* s)
+{
+  const int index_bits = 32 - 5; /* 32 - block_bits */
+  int64_t n = s->n;
+  double one_over_dx = s->one_over_dx;
+#pragma omp parallel for num_threads(s->nthreads)
+  for (int64_t i = 0; i < n; ++i) {
+    int b[3];
+    for (int d = 0; d < 3; ++d) b[d] = int_floor(s->X[3 * i + d] * one_over_dx - 0.5);
+    uint64_t off = zro_linear_offset(b[0], b[1], b[2]);
+    s->sorter[i] = ((off >> 7) << index_bits) + (uint64_t)i;
+  }
+  qsort(s->sorter, (size_t)n, sizeof(uint64_t), cmp_u64);
+
+  s->ngroups = 0;
+  int64_t last = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (i == n - 1 || (s->sorter[i] >> 32) != (s->sorter[i + 1] >> 32)) {
+      s->gfirst[s->ngroups] = last;
+      s->glast[s->ngroups] = i;
+      s->block_offset[s->ngroups] = s->sorter[i] >> 32;
+      s->ngroups++;
+      last = i + 1;
+    }
+
+  /* page map: worst case 8 pages per group */
+  int64_t need = 8 * (s->ngroups > 0 ? s->ngroups : 1);
+  if (need > s->pages_cap) {
+    free(s->pages); free(s->hkeys); free(s->hvals);
+    s->pages_cap = need;
+    s->pages = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)need);
+    s->hcap = 1;
+    while (s->hcap < 2 * need) s->hcap <<= 1;
+    s->hkeys = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)s->hcap);
+    s->hvals = (int64_t*)malloc(sizeof(int64_t) * (size_t)s->hcap);
+  }
+  memset(s->hkeys, 0, sizeof(uint64_t) * (size_t)s->hcap);
+  s->npages = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    s->order[i] = (int32_t)(s->sorter[i] & ((1ULL << index_bits) - 1));
+    uint64_t offset = (s->sorter[i] >> index_bits) << 7;
+    s->base_offset[s->order[i]] = offset;
+    if (i == n - 1 || (s->sorter[i] >> 32) != (s->sorter[i + 1] >> 32)) {
+      set_page(s, offset);
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+          for (int c = 0; c < 2; ++c)
+            set_page(s, zro_packed_add(offset, zro_linear_offset(2 * a, 4 * b, 4 * c)));
+    }
+  }
+  free(s->nodes);
+  free(s->explicit_collided);
+  s->nodes = (zro_node*)calloc((size_t)(s->npages * 32 + 1), sizeof(zro_node));
+  s->explicit_collided = (char*)calloc((size_t)(s->npages * 32 + 1), 1);
+  for (int64_t k = 0; k < s->npages * 32; ++k) s->nodes[k].idx = -1;
+  s->num_nodes = 0;
+}
+
+int64_t zro_num_groups(const zro_sim* s) { return s->ngroups; }
+int64_t zro_num_pages(const zro_sim* s) { return s->npages; }
+void zro_get_sort(const zro_sim* s, uint64_t* sorter, int32_t* order, uint64_t* base_offset)
+{
+  if (sorter) memcpy(sorter, s->sorter, sizeof(uint64_t) * (size_t)s->n);
+  if (order) memcpy(order, s->order, sizeof(int32_t) * (size_t)s->n);
+  if (base_offset) memcpy(base_offset, s->base_offset, sizeof(uint64_t) * (size_t)s->n);
+}
+void zro_get_groups(const zro_sim* s, int64_t* first, int64_t* last, uint64_t* block_offset)
+{
+  if (first) memcpy(first, s->gfirst, sizeof(int64_t) * (size_t)s->ngroups);
+  if (last) memcpy(last, s->glast, sizeof(int64_t) * (size_t)s->ngroups);
+  if (block_offset) memcpy(block_offset, s->block_offset, sizeof(uint64_t) * (size_t)s->ngroups);
+}
+void zro_get_pages(const zro_sim* s, uint64_t* pages)
+{
+  memcpy(pages, s->pages, sizeof(uint64_t) * (size_t)s->npages);
+}
+
+/* node storage lookup for a node coordinate (grid_array(offset)) */
+static zro_node* node_at(zro_sim* s, int i, int j, int k, int64_t* slot_out)
+{
+  uint64_t off = zro_linear_offset(i, j, k);
+  int64_t p = hash_find(s, off >> 12);
+  if (p < 0) return NULL;
+  int64_t slot = p * 32 + (int64_t)((off >> 7) & 31);
+  if (slot_out) *slot_out = slot;
+  return &s->nodes[slot];
+}
+
+/* ---------------- stress: FBasedMpmForceHelper::updateState ---------------- */
+void zro_update_stress(zro_sim* s)
+{
+#pragma omp parallel for num_threads(s->nthreads)
+  for (int64_t p = 0; p < s->n; ++p) {
+    double tau[9];
+    zro_fcr_kirchhoff(s->mu[p], s->lam[p], s->F + 9 * p, tau);
+    for (int c = 0; c < 9; ++c) s->vtau[9 * p + c] = s->vol[p] * tau[c];
+  }
+}
+
+/* ---------------- P2G (MLS, APIC) ---------------- */
+static void p2g_particle(zro_sim* s, int i)
+{
+  const double* Xp = s->X + 3 * i;
+  double mass = s->mass[i];
+  double mom[3], Cm[3][3], st[3][3];
+  for (int d = 0; d < 3; ++d) mom[d] = mass * s->V[3 * i + d];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) {
+      Cm[r][c] = mass * s->C[9 * i + c * 3 + r];
+      st[r][c] = s->vtau[9 * i + c * 3 + r];
+    }
+  int base[3];
+  double w[3][3];
+  for (int d = 0; d < 3; ++d) zro_bspline_weights(Xp[d] * s->one_over_dx, &base[d], w[d], NULL);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double wab = w[0][a] * w[1][b];
+      for (int c = 0; c < 3; ++c) {
+        double wt = wab * w[2][c];
+        int nc[3] = { base[0] + a, base[1] + b, base[2] + c };
+        zro_node* g = node_at(s, nc[0], nc[1], nc[2], NULL);
+        double dxp[3];
+        for (int d = 0; d < 3; ++d) dxp[d] = (double)nc[d] * s->dx - Xp[d];
+        g->m += mass * wt;
+        for (int r = 0; r < 3; ++r) {
+          double vd = Cm[r][0] * dxp[0] + Cm[r][1] * dxp[1] + Cm[r][2] * dxp[2] + mom[r];
+          g->v[r] += vd * wt;
+          double fd = st[r][0] * dxp[0] + st[r][1] * dxp[1] + st[r][2] * dxp[2];
+          g->new_v[r] -= fd * (wt * s->D_inverse);
+        }
+      }
+    }
+}
+
+/* getNumNodes (MpmGrid.h:123-136) + normalisation (MpmSimulationBase.cpp:474-485) */
+static void number_and_normalise(zro_sim* s)
+{
+  int64_t total = 0;
+  for (int64_t k = 0; k < s->npages * 32; ++k)
+    if (s->nodes[k].m != 0.0) s->nodes[k].idx = total++;
+  s->num_nodes = total;
+  for (int64_t k = 0; k < s->npages * 32; ++k) {
+    zro_node* g = &s->nodes[k];
+    if (g->idx < 0) continue;
+    if (g->m != 0.0) { for (int d = 0; d < 3; ++d) g->v[d] /= g->m; }
+    else { for (int d = 0; d < 3; ++d) g->v[d] = 0.0; }
+  }
+}
+
+int64_t zro_p2g_mls(zro_sim* s)
+{
+  for (int color = 0; color < 8; ++color) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(s->nthreads)
+    for (int64_t g = 0; g < s->ngroups; ++g) {
+      if ((int)(s->block_offset[g] & 7) != color) continue;
+      for (int64_t idx = s->gfirst[g]; idx <= s->glast[g]; ++idx) p2g_particle(s, s->order[idx]);
+    }
+  }
+  number_and_normalise(s);
+  return s->num_nodes;
+}
+
+/* ---------------- colliders ---------------- */
+static void collider_b(const zro_sim* s, const zro_collider* c, double t, double* b)
+{
+  (void)s;
+  for (int d = 0; d < 3; ++d) b[d] = c->velocity[d] * t;
+}
+
+/* level-set queryInside in material space (AnalyticLevelSet.cpp:111-119, 435-450) */
+static int ls_query_inside(const zro_collider* c, const double* X, double* phi, double* n)
+{
+  if (c->shape == 0) {
+    double sd = 0.0;
+    for (int d = 0; d < 3; ++d) sd += c->normal[d] * (X[d] - c->origin[d]);
+    *phi = sd;
+    if (sd <= 0.0) { for (int d = 0; d < 3; ++d) n[d] = c->normal[d]; return 1; }
+    return 0;
+  } else {
+    double tc[3], d2 = 0.0;
+    for (int d = 0; d < 3; ++d) { tc[d] = X[d] - c->origin[d]; d2 += tc[d] * tc[d]; }
+    double r2 = c->radius * c->radius;
+    if (d2 < r2) {
+      double dist = sqrt(d2);
+      *phi = dist - c->radius;
+      if (dist < 1e-7) { n[0] = 1.0; n[1] = 0.0; n[2] = 0.0; }
+      else for (int d = 0; d < 3; ++d) n[d] = tc[d] / dist;
+      return 1;
+    }
+    return 0;
+  }
+}
+
+/* AnalyticCollisionObject::detectAndResolveCollision (CollisionObject.cpp:340-405),
+   translation-only transforms (R = I, s = 1, omega = 0) */
+static int detect_and_resolve(const zro_collider* c, const double* b, const double* x, double* v)
+{
+  double X[3], phi, N[3];
+  for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
+  if (!ls_query_inside(c, X, &phi, N)) return 0;
+  const double* vo = c->velocity;
+  for (int d = 0; d < 3; ++d) v[d] -= vo[d];
+  if (c->type == 1) {
+    v[0] = v[1] = v[2] = 0.0;
+  } else if (c->type == 2 || c->type == 3) {
+    double dot = v[0] * N[0] + v[1] * N[1] + v[2] * N[2];
+    if (c->type == 2 || dot < 0) {
+      for (int d = 0; d < 3; ++d) v[d] -= N[d] * dot;
+      if (c->friction != 0.0 && dot < 0) {
+        double vn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (-dot * c->friction < vn) {
+          for (int d = 0; d < 3; ++d) v[d] += v[d] / vn * dot * c->friction;
+        } else {
+          v[0] = v[1] = v[2] = 0.0;
+        }
+      }
+    }
+  }
+  for (int d = 0; d < 3; ++d) v[d] += vo[d];
+  return 1;
+}
+
+/* multiObjectCollision (CollisionObject.cpp:66-105); the normal basis is only
+   needed by the implicit path */
+static int multi_object_collision(const zro_sim* s, double t, const double* x, double* v)
+{
+  int any = 0, nslip = 0;
+  double basis[3][3];
+  for (int k = 0; k < s->ncol; ++k) {
+    double b[3], n[3] = { 0, 0, 0 };
+    collider_b(s, &s->col[k], t, b);
+    int hit = detect_and_resolve(&s->col[k], b, x, v);
+    any = hit || any;
+    if (hit) {
+      if (s->col[k].type == 1) break;
+      double X[3], phi;
+      for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
+      ls_query_inside(&s->col[k], X, &phi, n);
+      for (int c2 = 0; c2 < nslip; ++c2) {
+        double dot = basis[c2][0] * n[0] + basis[c2][1] * n[1] + basis[c2][2] * n[2];
+        for (int d = 0; d < 3; ++d) n[d] -= dot * basis[c2][d];
+      }
+      double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      if (l != 0.0) {
+        for (int d = 0; d < 3; ++d) basis[nslip][d] = n[d] / l;
+        if (++nslip == 3) break;
+      }
+    }
+  }
+  return any;
+}
+
+/* trialCollision (CollisionObject.cpp:48-64) */
+static void trial_collision(const zro_sim* s, double t, const double* x, double dt, double* v)
+{
+  for (int k = 0; k < s->ncol; ++k) {
+    double b[3], X[3], phi, n[3];
+    collider_b(s, &s->col[k], t, b);
+    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - b[d];
+    if (ls_query_inside(&s->col[k], X, &phi, n))
+      for (int d = 0; d < 3; ++d) v[d] -= (phi / dt) * n[d];
+  }
+}
+
+/* gridVelocityExplicitUpdate (MpmSimulationBase.cpp:818-865), collider
+   updateState(t+dt) and trialCollision (MpmSimulationBase.cpp:267-271,298-312) */
+void zro_grid_update(zro_sim* s, double dt)
+{
+  double dvg[3];
+  for (int d = 0; d < 3; ++d) dvg[d] = s->gravity[d] * dt;
+#pragma omp parallel for num_threads(s->nthreads)
+  for (int64_t p = 0; p < s->npages; ++p) {
+    int base[3];
+    zro_linear_to_coord(s->pages[p], base);
+    for (int e = 0; e < 32; ++e) {
+      int64_t slot = p * 32 + e;
+      zro_node* g = &s->nodes[slot];
+      if (g->idx < 0) continue;
+      double xi[3] = { (double)(base[0] + ((e >> 4) & 1)) * s->dx, (double)(base[1] + ((e >> 2) & 3)) * s->dx,
+                       (double)(base[2] + (e & 3)) * s->dx };
+      double dt_over_mi = dt / g->m, vi[3];
+      for (int d = 0; d < 3; ++d) vi[d] = g->v[d] + dvg[d] + g->new_v[d] * dt_over_mi;
+      s->explicit_collided[slot] = (char)multi_object_collision(s, s->time, xi, vi);
+      for (int d = 0; d < 3; ++d) g->new_v[d] = vi[d];
+    }
+  }
+  /* trial collision with colliders at t + dt */
+#pragma omp parallel for num_threads(s->nthreads)
+  for (int64_t p = 0; p < s->npages; ++p) {
+    int base[3];
+    zro_linear_to_coord(s->pages[p], base);
+    for (int e = 0; e < 32; ++e) {
+      int64_t slot = p * 32 + e;
+      zro_node* g = &s->nodes[slot];
+      if (g->idx < 0 || s->explicit_collided[slot]) continue;
+      double xi[3] = { (double)(base[0] + ((e >> 4) & 1)) * s->dx, (double)(base[1] + ((e >> 2) & 3)) * s->dx,
+                       (double)(base[2] + (e & 3)) * s->dx };
+      trial_collision(s, s->time + dt, xi, dt, g->new_v);
+    }
+  }
+}
+
+/* gridToParticlesHelper<APIC,deg2,MLS> (MpmSimulationBase.cpp:906-1016) +
+   evolveStrain (FBasedMpmForceHelper.cpp:95-110).  Returns bit0 = a particle
+   moved more than one cell, bit1 = more than half a cell scaled by cfl. */
+int zro_g2p_mls(zro_sim* s, double dt)
+{
+  int flags = 0;
+  double dx2 = s->dx * s->dx;
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : flags) num_threads(s->nthreads)
+  for (int64_t g = 0; g < s->ngroups; ++g) {
+    for (int64_t idx = s->gfirst[g]; idx <= s->glast[g]; ++idx) {
+      int i = s->order[idx];
+      double* Xp = s->X + 3 * i;
+      int base[3];
+      double w[3][3];
+      for (int d = 0; d < 3; ++d) zro_bspline_weights(Xp[d] * s->one_over_dx, &base[d], w[d], NULL);
+      double picV[3] = { 0, 0, 0 }, B[3][3] = { { 0 } };
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          double wab = w[0][a] * w[1][b];
+          for (int c = 0; c < 3; ++c) {
+            double wt = wab * w[2][c];
+            int nc[3] = { base[0] + a, base[1] + b, base[2] + c };
+            zro_node* gn = node_at(s, nc[0], nc[1], nc[2], NULL);
+            double dxp[3];
+            for (int d = 0; d < 3; ++d) dxp[d] = (double)nc[d] * s->dx - Xp[d];
+            for (int r = 0; r < 3; ++r) {
+              picV[r] += wt * gn->new_v[r];
+              for (int cc = 0; cc < 3; ++cc) B[r][cc] += wt * gn->new_v[r] * dxp[cc];
+            }
+          }
+        }
+      for (int d = 0; d < 3; ++d) s->V[3 * i + d] = picV[d];
+      double Cn[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          Cn[r][c] = B[r][c] * s->D_inverse;
+          s->C[9 * i + c * 3 + r] = Cn[r][c];
+          s->gradV[9 * i + c * 3 + r] = Cn[r][c];
+        }
+      double inc2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        double inc = dt * picV[d];
+        Xp[d] += inc;
+        inc2 += inc * inc;
+      }
+      if (inc2 > dx2) flags |= 1;
+      if (inc2 > dx2 * 0.25 * (s->cfl * s->cfl)) flags |= 2;
+      /* F = (I + dt gradV) F */
+      double Fo[3][3], Fn[3][3];
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) Fo[r][c] = s->F[9 * i + c * 3 + r];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double acc = Fo[r][c];
+          for (int k = 0; k < 3; ++k) acc += dt * Cn[r][k] * Fo[k][c];
+          Fn[r][c] = acc;
+        }
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) s->F[9 * i + c * 3 + r] = Fn[r][c];
+    }
+  }
+  return flags;
+}
+
+/* advanceOneTimeStep, symplectic branch (MpmSimulationBase.cpp:256-281) */
+int zro_step_explicit(zro_sim* s, double dt)
+{
+  zro_sort_and_pollute(s);
+  zro_update_stress(s);
+  zro_p2g_mls(s);
+  zro_grid_update(s, dt);
+  int flags = zro_g2p_mls(s, dt);
+  s->time += dt;
+  return flags;
+}
+
+void zro_get_grid(const zro_sim* s, double* m, double* v, double* new_v, int64_t* idx)
+{
+  for (int64_t k = 0; k < s->npages * 32; ++k) {
+    const zro_node* g = &s->nodes[k];
+    if (m) m[k] = g->m;
+    for (int d = 0; d < 3; ++d) {
+      if (v) v[3 * k + d] = g->v[d];
+      if (new_v) new_v[3 * k + d] = g->new_v[d];
+    }
+    if (idx) idx[k] = g->idx;
+  }
+}

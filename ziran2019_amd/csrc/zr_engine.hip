@@ -1,0 +1,1229 @@
+// zr_engine.hip -- MI355X (gfx950) MPM time-step engine behind the C-ABI of
+// include/zr_mpm.h.
+//
+// Step structure (one HIP stream, FP64 plain path):
+//   bin      : per particle base node -> SPGrid page code; per-page counts via
+//              atomics on a dense page table; occupancy bitmap
+//   groups   : bitmap popcount scan -> sorted list of particle pages (groups)
+//   scatter  : particles into their group segment, then an in-LDS bitonic sort
+//              of each segment by (cell, index): the reference's sorted keys
+//   pages    : 8 neighbour pages per group, first-touch order by atomicMin on
+//              the touch index -> page slots (the reference's block list)
+//   P2G      : one workgroup per group scatters its particles into per-wave
+//              LDS tiles covering the 4x6x6 stencil footprint of the page,
+//              then writes the tile to a staging buffer (no global atomics)
+//   grid     : one thread per node of each slot sums the <= 8 overlapping
+//              tiles in a fixed order, normalises, applies gravity/forces and
+//              colliders (+ trial collision)
+//   G2P      : one workgroup per group loads v* of the footprint into LDS and
+//              gathers per particle: v, C, x, F update, CFL flags
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/zr_mpm.h"
+#include "zr_math.hpp"
+#include "zr_scan.hpp"
+
+using namespace zr;
+
+namespace {
+
+constexpr int kTileX = 4, kTileY = 6, kTileZ = 6;
+constexpr int kTileNodes = kTileX * kTileY * kTileZ;  // 144
+constexpr int kP2GFields = 7;                          // m, mv[3], f[3]
+constexpr int kStageStride = kP2GFields * kTileNodes;  // doubles per group
+constexpr int kMaxColliders = 8;
+constexpr int kSortLds = 2048;  // particles per page sorted in LDS
+constexpr uint32_t kNone = 0xffffffffu;
+
+struct ColliderSet {
+  int32_t n;
+  zr_collider c[kMaxColliders];
+};
+
+enum { CNT_GROUPS = 0, CNT_SLOTS = 1, CNT_DOMAIN = 2, CNT_CFL = 3, CNT_TOTAL = 4, CNT_N = 8 };
+
+// ---------------------------------------------------------------------------
+// collisions: AnalyticCollisionObject with translation-only motion
+// (CollisionObject.cpp:48-105,232-246,340-405; AnalyticLevelSet.cpp:111-119,
+// 259-287,435-450)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool ls_query_inside(const zr_collider& c, const double X[3], double& phi, double n[3])
+{
+  if (c.shape == ZR_HALF_SPACE) {
+    double sd = c.normal[0] * (X[0] - c.origin[0]) + c.normal[1] * (X[1] - c.origin[1]) +
+                c.normal[2] * (X[2] - c.origin[2]);
+    phi = sd;
+    if (sd <= 0.0) {
+      n[0] = c.normal[0];
+      n[1] = c.normal[1];
+      n[2] = c.normal[2];
+      return true;
+    }
+    return false;
+  }
+  double tc[3] = { X[0] - c.origin[0], X[1] - c.origin[1], X[2] - c.origin[2] };
+  double d2 = tc[0] * tc[0] + tc[1] * tc[1] + tc[2] * tc[2];
+  if (d2 < c.radius * c.radius) {
+    double dist = sqrt(d2);
+    phi = dist - c.radius;
+    if (dist < 1e-7) {
+      n[0] = 1.0;
+      n[1] = 0.0;
+      n[2] = 0.0;
+    } else {
+      n[0] = tc[0] / dist;
+      n[1] = tc[1] / dist;
+      n[2] = tc[2] / dist;
+    }
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool detect_and_resolve(const zr_collider& c, double t, const double x[3], double v[3])
+{
+  double X[3], phi, N[3];
+  for (int d = 0; d < 3; ++d) X[d] = x[d] - c.velocity[d] * t;
+  if (!ls_query_inside(c, X, phi, N)) return false;
+  for (int d = 0; d < 3; ++d) v[d] -= c.velocity[d];
+  if (c.type == ZR_STICKY) {
+    v[0] = v[1] = v[2] = 0.0;
+  } else {
+    double dot = v[0] * N[0] + v[1] * N[1] + v[2] * N[2];
+    if (c.type == ZR_SLIP || dot < 0.0) {
+      for (int d = 0; d < 3; ++d) v[d] -= N[d] * dot;
+      if (c.friction != 0.0 && dot < 0.0) {
+        double vn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (-dot * c.friction < vn) {
+          for (int d = 0; d < 3; ++d) v[d] += v[d] / vn * dot * c.friction;
+        } else {
+          v[0] = v[1] = v[2] = 0.0;
+        }
+      }
+    }
+  }
+  for (int d = 0; d < 3; ++d) v[d] += c.velocity[d];
+  return true;
+}
+
+__device__ __forceinline__ bool multi_object_collision(const ColliderSet& cs, double t, const double x[3], double v[3])
+{
+  bool any = false;
+  for (int k = 0; k < cs.n; ++k) {
+    bool hit = detect_and_resolve(cs.c[k], t, x, v);
+    any = any || hit;
+    if (hit && cs.c[k].type == ZR_STICKY) break;
+  }
+  return any;
+}
+
+__device__ __forceinline__ void trial_collision(const ColliderSet& cs, double t, const double x[3], double dt, double v[3])
+{
+  for (int k = 0; k < cs.n; ++k) {
+    double X[3], phi, n[3];
+    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - cs.c[k].velocity[d] * t;
+    if (ls_query_inside(cs.c[k], X, phi, n))
+      for (int d = 0; d < 3; ++d) v[d] -= (phi / dt) * n[d];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// binning and sort (MpmSimulationBase.cpp:1032-1103)
+// ---------------------------------------------------------------------------
+__global__ void k_reset_tables(const uint32_t* group_page, uint32_t ngroups, const uint32_t* slot_page, uint32_t nslots,
+                               uint32_t* page_count, int32_t* page_group, uint32_t* page_bits, uint32_t* page_touch,
+                               int32_t* page_slot)
+{
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ngroups) {
+    uint32_t p = group_page[t];
+    page_count[p] = 0u;
+    page_group[p] = -1;
+    page_bits[p >> 5] = 0u;
+  }
+  if (t < nslots) {
+    uint32_t p = slot_page[t];
+    page_touch[p] = kNone;
+    page_slot[p] = -1;
+  }
+}
+
+__global__ void k_bin(const double* __restrict__ x, int64_t stride, uint32_t n, double inv_dx, int lim,
+                      uint32_t* page_count, uint32_t* page_bits, uint32_t* p_page, uint32_t* p_re, uint32_t* counters)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int b[3];
+  bool bad = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b[d] = int_floor(x[d * stride + i] * inv_dx - 0.5);
+    if (b[d] < 0 || b[d] > lim) {
+      bad = true;
+      b[d] = b[d] < 0 ? 0 : lim;
+    }
+  }
+  if (bad) atomicOr(&counters[CNT_DOMAIN], 1u);
+  uint32_t pc = page_encode((uint32_t)b[0] >> 1, (uint32_t)b[1] >> 2, (uint32_t)b[2] >> 2);
+  uint32_t e = elem_of(b[0], b[1], b[2]);
+  uint32_t r = atomicAdd(&page_count[pc], 1u);
+  if (r == 0u) atomicOr(&page_bits[pc >> 5], 1u << (pc & 31u));
+  p_page[i] = pc;
+  p_re[i] = (r << 5) | e;
+}
+
+__global__ void k_list_groups(const uint32_t* page_bits, uint32_t nwords, const uint32_t* word_base,
+                              const uint32_t* page_count, uint32_t* group_page, int32_t* page_group,
+                              uint32_t* group_count)
+{
+  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwords) return;
+  uint32_t bits = page_bits[w];
+  uint32_t g = word_base[w];
+  while (bits) {
+    uint32_t b = (uint32_t)__builtin_ctz(bits);
+    bits &= bits - 1u;
+    uint32_t pc = w * 32u + b;
+    group_page[g] = pc;
+    page_group[pc] = (int32_t)g;
+    group_count[g] = page_count[pc];
+    ++g;
+  }
+}
+
+__global__ void k_scatter(uint32_t n, const uint32_t* p_page, const uint32_t* p_re, const int32_t* page_group,
+                          const uint32_t* group_start, uint64_t* keys)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t re = p_re[i];
+  int32_t g = page_group[p_page[i]];
+  uint32_t dst = group_start[g] + (re >> 5);
+  keys[dst] = ((uint64_t)(re & 31u) << 32) | (uint64_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_sort_groups(const uint64_t* keys, const uint32_t* group_start, uint32_t* order)
+{
+  __shared__ uint64_t a[kSortLds];
+  const uint32_t g = blockIdx.x;
+  const uint32_t start = group_start[g];
+  const uint32_t n = group_start[g + 1] - start;
+  if (n > (uint32_t)kSortLds) return;
+  uint32_t P = 1;
+  while (P < n) P <<= 1;
+  for (uint32_t k = threadIdx.x; k < P; k += blockDim.x) a[k] = (k < n) ? keys[start + k] : ~0ull;
+  __syncthreads();
+  for (uint32_t size = 2; size <= P; size <<= 1)
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t k = threadIdx.x; k < P; k += blockDim.x) {
+        uint32_t l = k ^ stride;
+        if (l > k) {
+          bool up = (k & size) == 0;
+          uint64_t x = a[k], y = a[l];
+          if ((x > y) == up) {
+            a[k] = y;
+            a[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) order[start + k] = (uint32_t)(a[k] & 0xffffffffull);
+}
+
+// pages holding more than kSortLds particles: rank sort (keys are unique)
+__global__ __launch_bounds__(256) void k_sort_groups_big(const uint64_t* keys, const uint32_t* group_start, uint32_t* order)
+{
+  const uint32_t g = blockIdx.x;
+  const uint32_t start = group_start[g];
+  const uint32_t n = group_start[g + 1] - start;
+  if (n <= (uint32_t)kSortLds) return;
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+    uint64_t key = keys[start + k];
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < n; ++j) rank += keys[start + j] < key ? 1u : 0u;
+    order[start + rank] = (uint32_t)(key & 0xffffffffull);
+  }
+}
+
+// SPGrid_Page_Map::Set_Page over the 2x2x2 page neighbourhood of each group in
+// the reference loop order i(x), j(y), k(z) (MpmSimulationBase.cpp:1081-1090)
+__global__ void k_touch(uint32_t ngroups, const uint32_t* group_page, uint32_t* touch_page, uint32_t* page_touch)
+{
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8u * ngroups) return;
+  uint32_t g = t >> 3, o = t & 7u;
+  uint32_t px, py, pz;
+  page_decode(group_page[g], px, py, pz);
+  uint32_t q = page_encode(px + (o >> 2), py + ((o >> 1) & 1u), pz + (o & 1u));
+  touch_page[t] = q;
+  atomicMin(&page_touch[q], t);
+}
+
+__global__ void k_assign_slots(uint32_t ntouch, const uint32_t* touch_page, const uint32_t* page_touch,
+                               const uint32_t* touch_slot, uint32_t* slot_page, int32_t* page_slot)
+{
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntouch) return;
+  uint32_t q = touch_page[t];
+  if (page_touch[q] != t) return;
+  uint32_t s = touch_slot[t];
+  slot_page[s] = q;
+  page_slot[q] = (int32_t)s;
+}
+
+// ---------------------------------------------------------------------------
+// P2G, MLS with force (MpmSimulationBase.cpp:488-561) with the fixed-corotated
+// Kirchhoff stress of FBasedMpmForceHelper::updateState fused in
+// (FBasedMpmForceHelper.cpp:44-64).
+// ---------------------------------------------------------------------------
+struct ParticlesView {
+  double* x;
+  double* v;
+  double* C;
+  double* F;
+  const double* m;
+  const double* vol;
+  const double* mu;
+  const double* lam;
+  int64_t stride;
+};
+
+__global__ __launch_bounds__(256) void k_p2g_mls(ParticlesView P, const uint32_t* __restrict__ order,
+                                                 const uint32_t* __restrict__ group_start,
+                                                 const uint32_t* __restrict__ group_page, double* __restrict__ stage,
+                                                 double dx, double inv_dx, double Dinv)
+{
+  __shared__ double tile[4 * kStageStride];
+  const int tid = threadIdx.x;
+  const uint32_t g = blockIdx.x;
+  for (int k = tid; k < 4 * kStageStride; k += 256) tile[k] = 0.0;
+  const uint32_t start = group_start[g];
+  const uint32_t n = group_start[g + 1] - start;
+  uint32_t px, py, pz;
+  page_decode(group_page[g], px, py, pz);
+  const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
+  __syncthreads();
+  double* wt = tile + (tid >> 6) * kStageStride;
+  // lanes of a wave take particles 8 apart so that they mostly sit in
+  // different cells and their LDS adds rarely collide
+  const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
+  const int64_t S = P.stride;
+  for (uint32_t j0 = 0; j0 < n; j0 += 256) {
+    const uint32_t j = j0 + perm;
+    if (j >= n) continue;
+    const uint32_t i = order[start + j];
+    double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
+    const double mass = P.m[i];
+    double mom[3] = { mass * P.v[i], mass * P.v[S + i], mass * P.v[2 * S + i] };
+    M3<double> f, tau;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
+    fcr_kirchhoff(P.mu[i], P.lam[i], f, tau);
+    const double vol = P.vol[i];
+    double mC[3][3], st[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        mC[r][c] = mass * P.C[(c * 3 + r) * S + i];
+        st[r][c] = vol * tau.a[r][c];
+      }
+    double w[3][3];
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) b[d] = bspline_w(xp[d] * inv_dx, w[d]);
+    const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
+    if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double dx0 = (double)(b[0] + a) * dx - xp[0];
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) {
+        const double wab = w[0][a] * w[1][bb];
+        const double dx1 = (double)(b[1] + bb) * dx - xp[1];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double wt_ = wab * w[2][c];
+          const double dx2 = (double)(b[2] + c) * dx - xp[2];
+          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+          const double fw = wt_ * Dinv;
+          atomicAdd(&wt[t], mass * wt_);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            double vd = mC[r][0] * dx0 + mC[r][1] * dx1 + mC[r][2] * dx2 + mom[r];
+            double fd = st[r][0] * dx0 + st[r][1] * dx1 + st[r][2] * dx2;
+            atomicAdd(&wt[(1 + r) * kTileNodes + t], vd * wt_);
+            atomicAdd(&wt[(4 + r) * kTileNodes + t], -(fd * fw));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double* out = stage + (size_t)g * kStageStride;
+  for (int k = tid; k < kStageStride; k += 256)
+    out[k] = tile[k] + tile[kStageStride + k] + tile[2 * kStageStride + k] + tile[3 * kStageStride + k];
+}
+
+// ---------------------------------------------------------------------------
+// grid: sum of the overlapping staged tiles, normalisation, explicit update
+// (MpmSimulationBase.cpp:474-485,818-865) and trial collision (:298-312)
+// ---------------------------------------------------------------------------
+struct GridView {
+  double* m;
+  double* v;   // [3][stride] normalised velocity
+  double* f;   // [3][stride] rasterised force
+  double* vs;  // [3][stride] new_v after the update
+  int64_t stride;
+};
+
+enum { GRID_COMBINE = 0, GRID_COMBINE_UPDATE = 1, GRID_UPDATE = 2 };
+
+__global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const uint32_t* __restrict__ slot_page,
+                                              const int32_t* __restrict__ page_group,
+                                              const double* __restrict__ stage, GridView G, double dx, double dt,
+                                              double gx, double gy, double gz, double time, ColliderSet cs)
+{
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  if (k >= nslots * 32u) return;
+  const uint32_t s = k >> 5, e = k & 31u;
+  uint32_t qx, qy, qz;
+  page_decode(slot_page[s], qx, qy, qz);
+  const int ex = (int)(e >> 4), ey = (int)((e >> 2) & 3u), ez = (int)(e & 3u);
+  double mass, v[3], f[3];
+  const int64_t S = G.stride;
+  if (mode != GRID_UPDATE) {
+    double acc[kP2GFields];
+#pragma unroll
+    for (int q = 0; q < kP2GFields; ++q) acc[q] = 0.0;
+    for (int a = 0; a < 2; ++a) {
+      if (a > (int)qx) break;
+      const int tx = ex + 2 * a;
+      for (int b = 0; b < 2; ++b) {
+        const int ty = ey + 4 * b;
+        if (ty >= kTileY || b > (int)qy) break;
+        for (int c = 0; c < 2; ++c) {
+          const int tz = ez + 4 * c;
+          if (tz >= kTileZ || c > (int)qz) break;
+          const int32_t src = page_group[page_encode(qx - a, qy - b, qz - c)];
+          if (src < 0) continue;
+          const double* base = stage + (size_t)src * kStageStride + (tx * kTileY + ty) * kTileZ + tz;
+#pragma unroll
+          for (int q = 0; q < kP2GFields; ++q) acc[q] += base[q * kTileNodes];
+        }
+      }
+    }
+    mass = acc[0];
+    for (int d = 0; d < 3; ++d) {
+      v[d] = mass != 0.0 ? acc[1 + d] / mass : 0.0;
+      f[d] = acc[4 + d];
+    }
+    G.m[k] = mass;
+    for (int d = 0; d < 3; ++d) {
+      G.v[d * S + k] = v[d];
+      G.f[d * S + k] = f[d];
+    }
+    if (mode == GRID_COMBINE) return;
+  } else {
+    mass = G.m[k];
+    for (int d = 0; d < 3; ++d) {
+      v[d] = G.v[d * S + k];
+      f[d] = G.f[d * S + k];
+    }
+  }
+  double vi[3] = { 0.0, 0.0, 0.0 };
+  if (mass != 0.0) {
+    const double xi[3] = { (double)(2 * (int)qx + ex) * dx, (double)(4 * (int)qy + ey) * dx,
+                           (double)(4 * (int)qz + ez) * dx };
+    const double dt_over_mi = dt / mass;
+    vi[0] = v[0] + gx * dt + f[0] * dt_over_mi;
+    vi[1] = v[1] + gy * dt + f[1] * dt_over_mi;
+    vi[2] = v[2] + gz * dt + f[2] * dt_over_mi;
+    bool collided = multi_object_collision(cs, time, xi, vi);
+    if (!collided) trial_collision(cs, time + dt, xi, dt, vi);
+  }
+  for (int d = 0; d < 3; ++d) G.vs[d * S + k] = vi[d];
+}
+
+// ---------------------------------------------------------------------------
+// G2P, MLS/APIC (MpmSimulationBase.cpp:906-1016) + evolveStrain
+// (FBasedMpmForceHelper.cpp:95-110)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_g2p_mls(ParticlesView P, const uint32_t* __restrict__ order,
+                                                 const uint32_t* __restrict__ group_start,
+                                                 const uint32_t* __restrict__ group_page,
+                                                 const int32_t* __restrict__ page_slot, const double* __restrict__ gvs,
+                                                 int64_t gstride, double dx, double inv_dx, double Dinv, double dt,
+                                                 double cfl, uint32_t* counters)
+{
+  __shared__ double tv[3 * kTileNodes];
+  __shared__ int32_t nslot[8];
+  const int tid = threadIdx.x;
+  const uint32_t g = blockIdx.x;
+  const uint32_t start = group_start[g];
+  const uint32_t n = group_start[g + 1] - start;
+  uint32_t px, py, pz;
+  page_decode(group_page[g], px, py, pz);
+  if (tid < 8) nslot[tid] = page_slot[page_encode(px + (tid >> 2), py + ((tid >> 1) & 1), pz + (tid & 1))];
+  __syncthreads();
+  for (int t = tid; t < kTileNodes; t += 256) {
+    const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
+    const int32_t sl = nslot[((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2)];
+    const int64_t node = (int64_t)sl * 32 + elem_of(tx, ty, tz);
+    for (int d = 0; d < 3; ++d) tv[d * kTileNodes + t] = sl >= 0 ? gvs[d * gstride + node] : 0.0;
+  }
+  __syncthreads();
+  const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
+  const int64_t S = P.stride;
+  const double dx2 = dx * dx;
+  uint32_t flags = 0;
+  for (uint32_t j = tid; j < n; j += 256) {
+    const uint32_t i = order[start + j];
+    double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
+    double w[3][3];
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) b[d] = bspline_w(xp[d] * inv_dx, w[d]);
+    const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
+    if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
+    double pv[3] = { 0.0, 0.0, 0.0 }, B[3][3] = { { 0.0 } };
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double dx0 = (double)(b[0] + a) * dx - xp[0];
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) {
+        const double wab = w[0][a] * w[1][bb];
+        const double dx1 = (double)(b[1] + bb) * dx - xp[1];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double wt_ = wab * w[2][c];
+          const double dx2c = (double)(b[2] + c) * dx - xp[2];
+          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const double gv = tv[r * kTileNodes + t];
+            pv[r] += wt_ * gv;
+            B[r][0] += wt_ * gv * dx0;
+            B[r][1] += wt_ * gv * dx1;
+            B[r][2] += wt_ * gv * dx2c;
+          }
+        }
+      }
+    }
+    double Cn[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Cn[r][c] = B[r][c] * Dinv;
+        P.C[(c * 3 + r) * S + i] = Cn[r][c];
+      }
+    double inc2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      P.v[d * S + i] = pv[d];
+      const double inc = dt * pv[d];
+      P.x[d * S + i] = xp[d] + inc;
+      inc2 += inc * inc;
+    }
+    if (inc2 > dx2) flags |= 1u;
+    if (inc2 > dx2 * 0.25 * (cfl * cfl)) flags |= 2u;
+    double Fo[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) Fo[r][c] = P.F[(c * 3 + r) * S + i];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double acc = Fo[r][c];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc += dt * Cn[r][k] * Fo[k][c];
+        P.F[(c * 3 + r) * S + i] = acc;
+      }
+  }
+  if (flags) atomicOr(&counters[CNT_CFL], flags);
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion and introspection
+// ---------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, uint32_t n, int ncomp, int64_t stride)
+{
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)n * ncomp) return;
+  uint32_t i = (uint32_t)(t / ncomp);
+  int c = (int)(t % ncomp);
+  soa[c * stride + i] = aos[t];
+}
+__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, uint32_t n, int ncomp, int64_t stride)
+{
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)n * ncomp) return;
+  uint32_t i = (uint32_t)(t / ncomp);
+  int c = (int)(t % ncomp);
+  aos[t] = soa[c * stride + i];
+}
+__global__ void k_dump_sorter(uint32_t n, const uint32_t* order, const uint32_t* p_page, const uint32_t* p_re, uint64_t* out)
+{
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t i = order[k];
+  uint64_t cell = ((uint64_t)p_page[i] << 5) | (p_re[i] & 31u);
+  out[k] = (cell << 27) + i;
+}
+__global__ void k_dump_base_offset(uint32_t n, const uint32_t* p_page, const uint32_t* p_re, uint64_t* out)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t cell = ((uint64_t)p_page[i] << 5) | (p_re[i] & 31u);
+  out[i] = cell << 7;
+}
+__global__ void k_node_idx(uint32_t nn, const double* m, const uint32_t* scan, int64_t* idx)
+{
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  idx[k] = m[k] != 0.0 ? (int64_t)scan[k] : -1;
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+// ===========================================================================
+// engine
+// ===========================================================================
+struct zr_engine {
+  zr_config cfg{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  double time = 0.0;
+  std::vector<zr_collider> colliders;
+
+  uint32_t L = 0, lim = 0, ndense = 0, nwords = 0;
+  uint32_t* page_count = nullptr;
+  int32_t* page_group = nullptr;
+  uint32_t* page_bits = nullptr;
+  uint32_t* page_touch = nullptr;
+  int32_t* page_slot = nullptr;
+
+  int64_t n = 0;
+  double *x = nullptr, *v = nullptr, *C = nullptr, *F = nullptr;
+  double *m = nullptr, *vol = nullptr, *mu = nullptr, *lam = nullptr;
+  uint32_t *p_page = nullptr, *p_re = nullptr, *order = nullptr;
+  uint64_t* keys = nullptr;
+  uint32_t *group_page = nullptr, *group_count = nullptr, *group_start = nullptr;
+  uint32_t *touch_page = nullptr, *touch_slot = nullptr;
+  uint32_t* slot_page = nullptr;
+  uint32_t *word_base = nullptr;
+
+  uint32_t A = 0, S = 0;  // groups, slots of the current sort
+  bool sorted = false, rasterised = false, updated = false;
+
+  double* stage = nullptr;
+  size_t stage_cap = 0;  // groups
+  double *gm = nullptr, *gv = nullptr, *gf = nullptr, *gvs = nullptr;
+  size_t grid_cap = 0;  // nodes
+
+  uint32_t* scan_tmp = nullptr;
+  size_t scan_cap = 0;
+  uint32_t* dcount = nullptr;
+  uint32_t* hcount = nullptr;
+
+  // kernel timing: pairs of events per launch, per kernel class
+  bool prof = false;
+  std::vector<hipEvent_t> prof_ev[ZR_PROF_COUNT];
+  size_t prof_used[ZR_PROF_COUNT] = { 0, 0, 0, 0 };
+};
+
+namespace {
+
+int fail(zr_engine* e, int code, const std::string& msg)
+{
+  if (e) e->err = msg;
+  return code;
+}
+
+#define ZR_HIP(e, call)                                                                     \
+  do {                                                                                      \
+    hipError_t _st = (call);                                                                \
+    if (_st != hipSuccess)                                                                  \
+      return fail((e), ZR_ERR_HIP, std::string(#call ": ") + hipGetErrorString(_st));       \
+  } while (0)
+
+#define ZR_LAUNCH_CHECK(e) ZR_HIP(e, hipGetLastError())
+
+template <class T>
+void dfree(T*& p)
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int ensure_scan(zr_engine* e, uint32_t n)
+{
+  size_t nt = (n + kScanTile - 1) / kScanTile + 1;
+  if (nt <= e->scan_cap) return ZR_OK;
+  dfree(e->scan_tmp);
+  ZR_HIP(e, hipMalloc(&e->scan_tmp, nt * sizeof(uint32_t)));
+  e->scan_cap = nt;
+  return ZR_OK;
+}
+
+template <class Ld>
+int run_scan(zr_engine* e, Ld ld, uint32_t n, uint32_t* out, uint32_t* total_dev)
+{
+  int rc = ensure_scan(e, n);
+  if (rc) return rc;
+  if (n == 0) {
+    ZR_HIP(e, hipMemsetAsync(total_dev, 0, sizeof(uint32_t), e->stream));
+    return ZR_OK;
+  }
+  uint32_t nt = (n + kScanTile - 1) / kScanTile;
+  k_scan_reduce<Ld><<<nt, kScanBlock, 0, e->stream>>>(ld, n, e->scan_tmp);
+  k_scan_tile_sums<<<1, 1024, 0, e->stream>>>(e->scan_tmp, nt, total_dev);
+  k_scan_down<Ld><<<nt, kScanBlock, 0, e->stream>>>(ld, n, e->scan_tmp, out);
+  ZR_LAUNCH_CHECK(e);
+  return ZR_OK;
+}
+
+// records one event of a start/stop pair for kernel class k when timing is on
+int prof_mark(zr_engine* e, int k)
+{
+  if (!e->prof) return ZR_OK;
+  auto& ev = e->prof_ev[k];
+  if (e->prof_used[k] == ev.size()) {
+    hipEvent_t h;
+    ZR_HIP(e, hipEventCreate(&h));
+    ev.push_back(h);
+  }
+  ZR_HIP(e, hipEventRecord(ev[e->prof_used[k]++], e->stream));
+  return ZR_OK;
+}
+
+int sync_counts(zr_engine* e)
+{
+  ZR_HIP(e, hipMemcpyAsync(e->hcount, e->dcount, CNT_N * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+void free_particles(zr_engine* e)
+{
+  dfree(e->x); dfree(e->v); dfree(e->C); dfree(e->F); dfree(e->m); dfree(e->vol); dfree(e->mu); dfree(e->lam);
+  dfree(e->p_page); dfree(e->p_re); dfree(e->order); dfree(e->keys);
+  dfree(e->group_page); dfree(e->group_count); dfree(e->group_start);
+  dfree(e->touch_page); dfree(e->touch_slot); dfree(e->slot_page);
+  e->n = 0;
+}
+
+ColliderSet collider_set(const zr_engine* e)
+{
+  ColliderSet cs;
+  memset(&cs, 0, sizeof(cs));
+  cs.n = (int32_t)e->colliders.size();
+  for (int k = 0; k < cs.n; ++k) cs.c[k] = e->colliders[k];
+  return cs;
+}
+
+ParticlesView pview(zr_engine* e)
+{
+  return ParticlesView{ e->x, e->v, e->C, e->F, e->m, e->vol, e->mu, e->lam, e->n };
+}
+
+int ensure_grid(zr_engine* e)
+{
+  size_t nodes = (size_t)e->S * 32;
+  if (nodes > e->grid_cap) {
+    dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
+    size_t cap = nodes + nodes / 4 + 1024;
+    ZR_HIP(e, hipMalloc(&e->gm, cap * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->gv, 3 * cap * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->gf, 3 * cap * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->gvs, 3 * cap * sizeof(double)));
+    e->grid_cap = cap;
+  }
+  if ((size_t)e->A > e->stage_cap) {
+    dfree(e->stage);
+    size_t cap = (size_t)e->A + e->A / 4 + 64;
+    ZR_HIP(e, hipMalloc(&e->stage, cap * kStageStride * sizeof(double)));
+    e->stage_cap = cap;
+  }
+  return ZR_OK;
+}
+
+int do_sort(zr_engine* e, zr_step_stats* st)
+{
+  if (e->n <= 0) return fail(e, ZR_ERR_STATE, "no particles uploaded");
+  const uint32_t n = (uint32_t)e->n;
+  hipStream_t s = e->stream;
+  // clear the dense tables touched by the previous sort
+  uint32_t rst = e->A > e->S ? e->A : e->S;
+  if (rst)
+    k_reset_tables<<<blocks_for(rst, 256), 256, 0, s>>>(e->group_page, e->A, e->slot_page, e->S, e->page_count,
+                                                       e->page_group, e->page_bits, e->page_touch, e->page_slot);
+  e->sorted = e->rasterised = e->updated = false;
+  prof_mark(e, ZR_PROF_SORT);
+  ZR_HIP(e, hipMemsetAsync(e->dcount, 0, CNT_N * sizeof(uint32_t), s));
+  k_bin<<<blocks_for(n, 256), 256, 0, s>>>(e->x, e->n, n, 1.0 / e->cfg.dx, (int)e->lim, e->page_count, e->page_bits,
+                                           e->p_page, e->p_re, e->dcount);
+  ZR_LAUNCH_CHECK(e);
+  int rc = run_scan(e, LdPopc{ e->page_bits }, e->nwords, e->word_base, e->dcount + CNT_GROUPS);
+  if (rc) return rc;
+  k_list_groups<<<blocks_for(e->nwords, 256), 256, 0, s>>>(e->page_bits, e->nwords, e->word_base, e->page_count,
+                                                           e->group_page, e->page_group, e->group_count);
+  ZR_LAUNCH_CHECK(e);
+  prof_mark(e, ZR_PROF_SORT);
+  rc = sync_counts(e);
+  if (rc) return rc;
+  e->A = e->hcount[CNT_GROUPS];
+  e->S = 0;
+  if (e->hcount[CNT_DOMAIN]) {
+    // tables hold clamped entries; keep A so the next sort clears them
+    return fail(e, ZR_ERR_DOMAIN, "particle base node outside [0, 2^grid_log2 - 8]");
+  }
+  const uint32_t A = e->A;
+  prof_mark(e, ZR_PROF_SORT);
+  rc = run_scan(e, LdArray{ e->group_count }, A, e->group_start, e->dcount + CNT_TOTAL);
+  if (rc) return rc;
+  ZR_HIP(e, hipMemcpyAsync(e->group_start + A, e->dcount + CNT_TOTAL, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  k_scatter<<<blocks_for(n, 256), 256, 0, s>>>(n, e->p_page, e->p_re, e->page_group, e->group_start, e->keys);
+  k_sort_groups<<<A, 256, 0, s>>>(e->keys, e->group_start, e->order);
+  k_sort_groups_big<<<A, 256, 0, s>>>(e->keys, e->group_start, e->order);
+  k_touch<<<blocks_for(8ull * A, 256), 256, 0, s>>>(A, e->group_page, e->touch_page, e->page_touch);
+  ZR_LAUNCH_CHECK(e);
+  rc = run_scan(e, LdFirstTouch{ e->touch_page, e->page_touch }, 8u * A, e->touch_slot, e->dcount + CNT_SLOTS);
+  if (rc) return rc;
+  k_assign_slots<<<blocks_for(8ull * A, 256), 256, 0, s>>>(8u * A, e->touch_page, e->page_touch, e->touch_slot,
+                                                          e->slot_page, e->page_slot);
+  ZR_LAUNCH_CHECK(e);
+  prof_mark(e, ZR_PROF_SORT);
+  rc = sync_counts(e);
+  if (rc) return rc;
+  e->S = e->hcount[CNT_SLOTS];
+  rc = ensure_grid(e);
+  if (rc) return rc;
+  e->sorted = true;
+  if (st) {
+    st->num_groups = A;
+    st->num_pages = e->S;
+  }
+  return ZR_OK;
+}
+
+int do_p2g(zr_engine* e)
+{
+  if (!e->sorted) return fail(e, ZR_ERR_STATE, "particlesToGrid before sortParticlesAndPolluteGrid");
+  const double dx = e->cfg.dx;
+  prof_mark(e, ZR_PROF_P2G);
+  k_p2g_mls<<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx,
+                                         4.0 / (dx * dx));
+  ZR_LAUNCH_CHECK(e);
+  prof_mark(e, ZR_PROF_P2G);
+  e->rasterised = true;
+  return ZR_OK;
+}
+
+int do_grid(zr_engine* e, int mode, double dt)
+{
+  GridView G{ e->gm, e->gv, e->gf, e->gvs, (int64_t)e->grid_cap };
+  ColliderSet cs = collider_set(e);
+  prof_mark(e, ZR_PROF_GRID);
+  if (e->S)
+    k_grid<<<blocks_for((uint64_t)e->S * 32, 256), 256, 0, e->stream>>>(
+        mode, e->S, e->slot_page, e->page_group, e->stage, G, e->cfg.dx, dt, e->cfg.gravity[0], e->cfg.gravity[1],
+        e->cfg.gravity[2], e->time, cs);
+  ZR_LAUNCH_CHECK(e);
+  prof_mark(e, ZR_PROF_GRID);
+  if (mode != GRID_COMBINE) e->updated = true;
+  return ZR_OK;
+}
+
+int do_g2p(zr_engine* e, double dt, zr_step_stats* st)
+{
+  if (!e->updated) return fail(e, ZR_ERR_STATE, "gridToParticles before the grid update");
+  const double dx = e->cfg.dx;
+  ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_CFL, 0, sizeof(uint32_t), e->stream));
+  prof_mark(e, ZR_PROF_G2P);
+  k_g2p_mls<<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot, e->gvs,
+                                         (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt, e->cfg.cfl,
+                                         e->dcount);
+  ZR_LAUNCH_CHECK(e);
+  prof_mark(e, ZR_PROF_G2P);
+  int rc = sync_counts(e);
+  if (rc) return rc;
+  uint32_t flags = e->hcount[CNT_CFL];
+  // particles moved: the sort is stale until the next one
+  e->sorted = e->rasterised = e->updated = false;
+  if (st) st->cfl_flags = (int32_t)flags;
+  return (flags & 1u) ? ZR_CFL_RESTART : ZR_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int zr_create(const zr_config* cfg, zr_engine** out)
+{
+  if (!cfg || !out) return ZR_ERR_ARG;
+  *out = nullptr;
+  if (!(cfg->dx > 0.0) || cfg->grid_log2 < 6 || cfg->grid_log2 > 10 || cfg->precision != ZR_FP64 ||
+      cfg->transfer != ZR_TRANSFER_MLS)
+    return ZR_ERR_ARG;
+  zr_engine* e = new zr_engine();
+  e->cfg = *cfg;
+  e->L = (uint32_t)cfg->grid_log2;
+  e->lim = (1u << e->L) - 8u;
+  e->ndense = 1u << (3 * e->L - 5);
+  e->nwords = (e->ndense + 31u) / 32u;
+  hipError_t st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (st != hipSuccess) {
+    delete e;
+    return ZR_ERR_HIP;
+  }
+  bool ok = hipMalloc(&e->page_count, e->ndense * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&e->page_group, e->ndense * sizeof(int32_t)) == hipSuccess &&
+            hipMalloc(&e->page_touch, e->ndense * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&e->page_slot, e->ndense * sizeof(int32_t)) == hipSuccess &&
+            hipMalloc(&e->page_bits, e->nwords * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&e->word_base, (e->nwords + 1) * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&e->dcount, CNT_N * sizeof(uint32_t)) == hipSuccess &&
+            hipHostMalloc(&e->hcount, CNT_N * sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipMemset(e->page_count, 0, e->ndense * sizeof(uint32_t)) == hipSuccess &&
+       hipMemset(e->page_group, 0xff, e->ndense * sizeof(int32_t)) == hipSuccess &&
+       hipMemset(e->page_touch, 0xff, e->ndense * sizeof(uint32_t)) == hipSuccess &&
+       hipMemset(e->page_slot, 0xff, e->ndense * sizeof(int32_t)) == hipSuccess &&
+       hipMemset(e->page_bits, 0, e->nwords * sizeof(uint32_t)) == hipSuccess &&
+       hipDeviceSynchronize() == hipSuccess;
+  if (!ok) {
+    zr_destroy(e);
+    return ZR_ERR_HIP;
+  }
+  *out = e;
+  return ZR_OK;
+}
+
+void zr_destroy(zr_engine* e)
+{
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  free_particles(e);
+  dfree(e->page_count); dfree(e->page_group); dfree(e->page_touch); dfree(e->page_slot); dfree(e->page_bits);
+  dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
+  dfree(e->scan_tmp); dfree(e->dcount);
+  for (auto& v : e->prof_ev)
+    for (auto h : v) (void)hipEventDestroy(h);
+  if (e->hcount) (void)hipHostFree(e->hcount);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char* zr_last_error(const zr_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int zr_set_colliders(zr_engine* e, const zr_collider* c, int32_t n)
+{
+  if (!e || n < 0 || n > kMaxColliders || (n > 0 && !c)) return fail(e, ZR_ERR_ARG, "bad collider set");
+  e->colliders.assign(c, c + n);
+  for (auto& k : e->colliders) {
+    if (k.shape == ZR_HALF_SPACE) {  // HalfSpace normalises its normal (AnalyticLevelSet.cpp:262-266)
+      double l = sqrt(k.normal[0] * k.normal[0] + k.normal[1] * k.normal[1] + k.normal[2] * k.normal[2]);
+      if (!(l > 0.0)) return fail(e, ZR_ERR_ARG, "half-space normal is zero");
+      for (int d = 0; d < 3; ++d) k.normal[d] /= l;
+    } else if (k.shape != ZR_SPHERE) {
+      return fail(e, ZR_ERR_ARG, "unknown collider shape");
+    }
+    if (k.type < ZR_STICKY || k.type > ZR_SEPARATE) return fail(e, ZR_ERR_ARG, "unknown collider type");
+  }
+  return ZR_OK;
+}
+
+int zr_set_time(zr_engine* e, double t)
+{
+  if (!e) return ZR_ERR_ARG;
+  e->time = t;
+  return ZR_OK;
+}
+double zr_get_time(const zr_engine* e) { return e ? e->time : 0.0; }
+
+int zr_upload_particles(zr_engine* e, const zr_particles* p)
+{
+  if (!e || !p || p->n < 0) return fail(e, ZR_ERR_ARG, "bad particle view");
+  if (p->n >= (1ll << 27)) return fail(e, ZR_ERR_ARG, "particle count must be below 2^27");
+  if (p->n > 0 && (!p->X || !p->V || !p->C || !p->F || !p->mass || !p->vol || !p->mu || !p->lambda))
+    return fail(e, ZR_ERR_ARG, "all particle attributes are required");
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  // reset tables owned by the previous particle set
+  if (e->A || e->S) {
+    uint32_t rst = e->A > e->S ? e->A : e->S;
+    k_reset_tables<<<blocks_for(rst, 256), 256, 0, e->stream>>>(e->group_page, e->A, e->slot_page, e->S,
+                                                                e->page_count, e->page_group, e->page_bits,
+                                                                e->page_touch, e->page_slot);
+    ZR_LAUNCH_CHECK(e);
+    ZR_HIP(e, hipStreamSynchronize(e->stream));
+    e->A = e->S = 0;
+  }
+  free_particles(e);
+  e->sorted = e->rasterised = e->updated = false;
+  const int64_t n = p->n;
+  e->n = n;
+  if (n == 0) return ZR_OK;
+  size_t N = (size_t)n;
+  ZR_HIP(e, hipMalloc(&e->x, 3 * N * sizeof(double)));
+  ZR_HIP(e, hipMalloc(&e->v, 3 * N * sizeof(double)));
+  ZR_HIP(e, hipMalloc(&e->C, 9 * N * sizeof(double)));
+  ZR_HIP(e, hipMalloc(&e->F, 9 * N * sizeof(double)));
+  ZR_HIP(e, hipMalloc(&e->m, N * sizeof(double)));
+  ZR_HIP(e, hipMalloc(&e->vol, N * sizeof(double)));
+  ZR_HIP(e, hipMalloc(&e->mu, N * sizeof(double)));
+  ZR_HIP(e, hipMalloc(&e->lam, N * sizeof(double)));
+  ZR_HIP(e, hipMalloc(&e->p_page, N * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->p_re, N * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->order, N * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->keys, N * sizeof(uint64_t)));
+  ZR_HIP(e, hipMalloc(&e->group_page, N * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->group_count, N * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->group_start, (N + 1) * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->touch_page, 8 * N * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->touch_slot, 8 * N * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->slot_page, 8 * N * sizeof(uint32_t)));
+  double* tmp = nullptr;
+  ZR_HIP(e, hipMalloc(&tmp, 9 * N * sizeof(double)));
+  struct Item {
+    const double* src;
+    double* dst;
+    int nc;
+  } items[] = { { p->X, e->x, 3 }, { p->V, e->v, 3 }, { p->C, e->C, 9 }, { p->F, e->F, 9 },
+                { p->mass, e->m, 1 }, { p->vol, e->vol, 1 }, { p->mu, e->mu, 1 }, { p->lambda, e->lam, 1 } };
+  for (auto& it : items) {
+    hipError_t st = hipMemcpyAsync(tmp, it.src, it.nc * N * sizeof(double), hipMemcpyHostToDevice, e->stream);
+    if (st == hipSuccess) {
+      k_aos_to_soa<<<blocks_for((uint64_t)N * it.nc, 256), 256, 0, e->stream>>>(tmp, it.dst, (uint32_t)N, it.nc, n);
+      st = hipGetLastError();
+    }
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) {
+      (void)hipFree(tmp);
+      return fail(e, ZR_ERR_HIP, std::string("upload: ") + hipGetErrorString(st));
+    }
+  }
+  ZR_HIP(e, hipFree(tmp));
+  return ZR_OK;
+}
+
+int zr_download_particles(zr_engine* e, zr_particles* p)
+{
+  if (!e || !p) return fail(e, ZR_ERR_ARG, "bad particle view");
+  if (p->n != e->n) return fail(e, ZR_ERR_ARG, "particle count mismatch");
+  if (e->n == 0) return ZR_OK;
+  size_t N = (size_t)e->n;
+  double* tmp = nullptr;
+  ZR_HIP(e, hipMalloc(&tmp, 9 * N * sizeof(double)));
+  struct Item {
+    double* dst;
+    const double* src;
+    int nc;
+  } items[] = { { p->X, e->x, 3 }, { p->V, e->v, 3 }, { p->C, e->C, 9 }, { p->F, e->F, 9 },
+                { p->mass, e->m, 1 }, { p->vol, e->vol, 1 }, { p->mu, e->mu, 1 }, { p->lambda, e->lam, 1 } };
+  for (auto& it : items) {
+    if (!it.dst) continue;
+    k_soa_to_aos<<<blocks_for((uint64_t)N * it.nc, 256), 256, 0, e->stream>>>(it.src, tmp, (uint32_t)N, it.nc, e->n);
+    hipError_t st = hipGetLastError();
+    if (st == hipSuccess) st = hipMemcpyAsync(it.dst, tmp, it.nc * N * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) {
+      (void)hipFree(tmp);
+      return fail(e, ZR_ERR_HIP, std::string("download: ") + hipGetErrorString(st));
+    }
+  }
+  ZR_HIP(e, hipFree(tmp));
+  return ZR_OK;
+}
+
+int zr_sort_particles_and_pollute_grid(zr_engine* e, zr_step_stats* st)
+{
+  if (!e) return ZR_ERR_ARG;
+  return do_sort(e, st);
+}
+
+int zr_particles_to_grid(zr_engine* e)
+{
+  if (!e) return ZR_ERR_ARG;
+  int rc = do_p2g(e);
+  if (rc) return rc;
+  rc = do_grid(e, GRID_COMBINE, 0.0);
+  if (rc) return rc;
+  e->updated = false;
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+int zr_grid_velocity_explicit_update(zr_engine* e, double dt)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!e->rasterised) return fail(e, ZR_ERR_STATE, "grid update before particlesToGrid");
+  int rc = do_grid(e, GRID_UPDATE, dt);
+  if (rc) return rc;
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+int zr_grid_to_particles(zr_engine* e, double dt, zr_step_stats* st)
+{
+  if (!e) return ZR_ERR_ARG;
+  return do_g2p(e, dt, st);
+}
+
+int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!(dt > 0.0)) return fail(e, ZR_ERR_ARG, "dt must be positive");
+  int rc = do_sort(e, st);
+  if (rc) return rc;
+  rc = do_p2g(e);
+  if (rc) return rc;
+  rc = do_grid(e, GRID_COMBINE_UPDATE, dt);
+  if (rc) return rc;
+  rc = do_g2p(e, dt, st);
+  if (rc < 0) return rc;
+  e->time += dt;
+  return rc;
+}
+
+int64_t zr_debug_size(zr_engine* e, int32_t what)
+{
+  if (!e) return -1;
+  switch (what) {
+    case ZR_DUMP_SORTER: return e->n * 8;
+    case ZR_DUMP_ORDER: return e->n * 4;
+    case ZR_DUMP_BASE_OFFSET: return e->n * 8;
+    case ZR_DUMP_GROUP_FIRST:
+    case ZR_DUMP_GROUP_LAST:
+    case ZR_DUMP_BLOCK_OFFSET: return (int64_t)e->A * 8;
+    case ZR_DUMP_PAGES: return (int64_t)e->S * 8;
+    case ZR_DUMP_GRID_M: return (int64_t)e->S * 32 * 8;
+    case ZR_DUMP_GRID_V:
+    case ZR_DUMP_GRID_F:
+    case ZR_DUMP_GRID_VSTAR: return (int64_t)e->S * 32 * 3 * 8;
+    case ZR_DUMP_NODE_IDX: return (int64_t)e->S * 32 * 8;
+    default: return -1;
+  }
+}
+
+int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes)
+{
+  if (!e) return ZR_ERR_ARG;
+  int64_t need = zr_debug_size(e, what);
+  if (need < 0 || (size_t)need != bytes || (need > 0 && !host)) return fail(e, ZR_ERR_ARG, "bad dump request");
+  if (need == 0) return ZR_OK;
+  hipStream_t s = e->stream;
+  const uint32_t n = (uint32_t)e->n, A = e->A, nn = e->S * 32;
+  ZR_HIP(e, hipStreamSynchronize(s));
+  if (what == ZR_DUMP_SORTER || what == ZR_DUMP_BASE_OFFSET) {
+    uint64_t* tmp = nullptr;
+    ZR_HIP(e, hipMalloc(&tmp, n * sizeof(uint64_t)));
+    if (what == ZR_DUMP_SORTER)
+      k_dump_sorter<<<blocks_for(n, 256), 256, 0, s>>>(n, e->order, e->p_page, e->p_re, tmp);
+    else
+      k_dump_base_offset<<<blocks_for(n, 256), 256, 0, s>>>(n, e->p_page, e->p_re, tmp);
+    hipError_t st = hipMemcpyAsync(host, tmp, bytes, hipMemcpyDeviceToHost, s);
+    if (st == hipSuccess) st = hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    if (st != hipSuccess) return fail(e, ZR_ERR_HIP, hipGetErrorString(st));
+    return ZR_OK;
+  }
+  if (what == ZR_DUMP_ORDER) {
+    ZR_HIP(e, hipMemcpy(host, e->order, bytes, hipMemcpyDeviceToHost));
+    return ZR_OK;
+  }
+  if (what == ZR_DUMP_GROUP_FIRST || what == ZR_DUMP_GROUP_LAST || what == ZR_DUMP_BLOCK_OFFSET ||
+      what == ZR_DUMP_PAGES) {
+    const uint32_t* src = what == ZR_DUMP_PAGES ? e->slot_page : (what == ZR_DUMP_BLOCK_OFFSET ? e->group_page : e->group_start);
+    size_t cnt = what == ZR_DUMP_PAGES ? e->S : (what == ZR_DUMP_BLOCK_OFFSET ? A : A + 1);
+    std::vector<uint32_t> h(cnt);
+    ZR_HIP(e, hipMemcpy(h.data(), src, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int64_t* o64 = (int64_t*)host;
+    uint64_t* u64 = (uint64_t*)host;
+    if (what == ZR_DUMP_GROUP_FIRST)
+      for (uint32_t g = 0; g < A; ++g) o64[g] = h[g];
+    else if (what == ZR_DUMP_GROUP_LAST)
+      for (uint32_t g = 0; g < A; ++g) o64[g] = (int64_t)h[g + 1] - 1;
+    else if (what == ZR_DUMP_BLOCK_OFFSET)
+      for (uint32_t g = 0; g < A; ++g) u64[g] = h[g];
+    else
+      for (uint32_t k = 0; k < e->S; ++k) u64[k] = (uint64_t)h[k] << 12;
+    return ZR_OK;
+  }
+  if (what == ZR_DUMP_GRID_M) {
+    ZR_HIP(e, hipMemcpy(host, e->gm, bytes, hipMemcpyDeviceToHost));
+    return ZR_OK;
+  }
+  if (what == ZR_DUMP_GRID_V || what == ZR_DUMP_GRID_F || what == ZR_DUMP_GRID_VSTAR) {
+    const double* src = what == ZR_DUMP_GRID_V ? e->gv : (what == ZR_DUMP_GRID_F ? e->gf : e->gvs);
+    std::vector<double> h(3 * (size_t)nn);
+    for (int d = 0; d < 3; ++d)
+      ZR_HIP(e, hipMemcpy(h.data() + (size_t)d * nn, src + (size_t)d * e->grid_cap, nn * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    double* o = (double*)host;
+    for (uint32_t k = 0; k < nn; ++k)
+      for (int d = 0; d < 3; ++d) o[3 * (size_t)k + d] = h[(size_t)d * nn + k];
+    return ZR_OK;
+  }
+  if (what == ZR_DUMP_NODE_IDX) {
+    uint32_t* sc = nullptr;
+    int64_t* idx = nullptr;
+    ZR_HIP(e, hipMalloc(&sc, (size_t)nn * sizeof(uint32_t)));
+    ZR_HIP(e, hipMalloc(&idx, (size_t)nn * sizeof(int64_t)));
+    int rc = run_scan(e, LdNonzeroMass{ e->gm }, nn, sc, e->dcount + CNT_TOTAL);
+    if (rc == ZR_OK) {
+      k_node_idx<<<blocks_for(nn, 256), 256, 0, s>>>(nn, e->gm, sc, idx);
+      hipError_t st = hipGetLastError();
+      if (st == hipSuccess) st = hipMemcpyAsync(host, idx, bytes, hipMemcpyDeviceToHost, s);
+      if (st == hipSuccess) st = hipStreamSynchronize(s);
+      if (st != hipSuccess) rc = fail(e, ZR_ERR_HIP, hipGetErrorString(st));
+    }
+    (void)hipFree(sc);
+    (void)hipFree(idx);
+    return rc;
+  }
+  return fail(e, ZR_ERR_ARG, "unknown dump");
+}
+
+int zr_profile_enable(zr_engine* e, int32_t on)
+{
+  if (!e) return ZR_ERR_ARG;
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  e->prof = on != 0;
+  for (auto& u : e->prof_used) u = 0;
+  return ZR_OK;
+}
+
+int zr_profile_read(zr_engine* e, int32_t which, double* total_ms, int64_t* launches)
+{
+  if (!e || which < 0 || which >= ZR_PROF_COUNT) return fail(e, ZR_ERR_ARG, "bad profile slot");
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  double t = 0.0;
+  size_t pairs = e->prof_used[which] / 2;
+  for (size_t k = 0; k < pairs; ++k) {
+    float ms = 0.f;
+    ZR_HIP(e, hipEventElapsedTime(&ms, e->prof_ev[which][2 * k], e->prof_ev[which][2 * k + 1]));
+    t += ms;
+  }
+  if (total_ms) *total_ms = t;
+  if (launches) *launches = (int64_t)pairs;
+  return ZR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,457 @@
+// zr_math.hpp -- per-particle math for the gfx950 kernels: SPGrid page codes,
+// quadratic B-spline weights, the 3x3 implicit-QR SVD and the fixed-corotated
+// constitutive model.  All index arguments are compile-time so the 3x3
+// matrices live in VGPRs (no scratch).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ZR_HD __host__ __device__ __forceinline__
+
+namespace zr {
+
+// ---------------------------------------------------------------------------
+// SPGrid addressing for GridState<double,3> (128 B nodes, 2x4x4 pages).
+// Lib/SPGrid/Core/SPGrid_Mask.h:22-174 with data_bits = 7, block bits (1,2,2):
+// a node offset is (page_code << 12) | (elem << 7), elem = x0<<4 | y<<2 | z
+// (low x bit, two low y bits, two low z bits) and the page code interleaves
+// page coordinates (px = x>>1, pz = z>>2, py = y>>2) as bits x,z,y,x,z,y,...
+// ---------------------------------------------------------------------------
+ZR_HD uint32_t spread3(uint32_t v)
+{
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+ZR_HD uint32_t compact3(uint32_t v)
+{
+  v &= 0x09249249u;
+  v = (v ^ (v >> 2)) & 0x030C30C3u;
+  v = (v ^ (v >> 4)) & 0x0300F00Fu;
+  v = (v ^ (v >> 8)) & 0xFF0000FFu;
+  v = (v ^ (v >> 16)) & 0x000003FFu;
+  return v;
+}
+ZR_HD uint32_t page_encode(uint32_t px, uint32_t py, uint32_t pz)
+{
+  return spread3(px) | (spread3(pz) << 1) | (spread3(py) << 2);
+}
+ZR_HD void page_decode(uint32_t code, uint32_t& px, uint32_t& py, uint32_t& pz)
+{
+  px = compact3(code);
+  pz = compact3(code >> 1);
+  py = compact3(code >> 2);
+}
+ZR_HD uint32_t elem_of(int x, int y, int z)
+{
+  return (uint32_t)(((x & 1) << 4) | ((y & 3) << 2) | (z & 3));
+}
+
+// MathTools.h:21-25
+ZR_HD int int_floor(double x)
+{
+  int i = (int)x;
+  return i - (i > x);
+}
+ZR_HD int int_floor(float x)
+{
+  int i = (int)x;
+  return i - (i > x);
+}
+
+// Quadratic B-spline (BSplines.h:16-20,55-81); x is in index space.
+template <class T>
+ZR_HD int bspline_w(T x, T w[3])
+{
+  int b = int_floor(x - (T)0.5);
+  T d0 = x - (T)b;
+  T z = (T)1.5 - d0;
+  w[0] = (T)0.5 * z * z;
+  T d1 = d0 - (T)1;
+  w[1] = (T)0.75 - d1 * d1;
+  T d2 = (T)1 - d1;
+  T zz = (T)1.5 - d2;
+  w[2] = (T)0.5 * zz * zz;
+  return b;
+}
+template <class T>
+ZR_HD int bspline_wdw(T x, T w[3], T dw[3])
+{
+  int b = int_floor(x - (T)0.5);
+  T d0 = x - (T)b;
+  T z = (T)1.5 - d0;
+  w[0] = (T)0.5 * z * z;
+  T d1 = d0 - (T)1;
+  w[1] = (T)0.75 - d1 * d1;
+  T d2 = (T)1 - d1;
+  T zz = (T)1.5 - d2;
+  w[2] = (T)0.5 * zz * zz;
+  dw[0] = -z;
+  dw[1] = (T)-2 * d1;
+  dw[2] = zz;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 matrix in registers, row-major a[r][c].
+// ---------------------------------------------------------------------------
+template <class T>
+struct M3 {
+  T a[3][3];
+};
+
+template <class T>
+ZR_HD void set_identity(M3<T>& m)
+{
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m.a[r][c] = (r == c) ? (T)1 : (T)0;
+}
+
+template <class T>
+ZR_HD T det3(const M3<T>& f)
+{
+  return f.a[0][0] * (f.a[1][1] * f.a[2][2] - f.a[1][2] * f.a[2][1]) -
+         f.a[0][1] * (f.a[1][0] * f.a[2][2] - f.a[1][2] * f.a[2][0]) +
+         f.a[0][2] * (f.a[1][0] * f.a[2][1] - f.a[1][1] * f.a[2][0]);
+}
+
+// ---------------------------------------------------------------------------
+// Implicit-shifted QR SVD of a 3x3 (Gast, Fu, Jiang, Teran 2016), in the form
+// of Lib/Ziran/Math/Linear/{Givens.h:32-271, ImplicitQRSVD.h:45-533}.
+// Output: F = U diag(s) V^T, U and V rotations, |s0|>=|s1|>=|s2|, only s2 < 0.
+// ---------------------------------------------------------------------------
+template <class T>
+struct Givens {
+  T c, s;
+};
+
+template <class T>
+ZR_HD Givens<T> gv_compute(T a, T b)
+{
+  Givens<T> g{ (T)1, (T)0 };
+  T sq = sqrt(a * a + b * b);
+  if (sq != (T)0) {
+    T t = (T)1 / sq;
+    g.c = a * t;
+    g.s = -b * t;
+  }
+  return g;
+}
+template <class T>
+ZR_HD Givens<T> gv_compute_unconv(T a, T b)
+{
+  Givens<T> g{ (T)0, (T)1 };
+  T sq = sqrt(a * a + b * b);
+  if (sq != (T)0) {
+    T t = (T)1 / sq;
+    g.s = a * t;
+    g.c = b * t;
+  }
+  return g;
+}
+template <int I, int K, class T>
+ZR_HD void gv_row(const Givens<T>& g, M3<T>& m)
+{
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    T t1 = m.a[I][j], t2 = m.a[K][j];
+    m.a[I][j] = g.c * t1 - g.s * t2;
+    m.a[K][j] = g.s * t1 + g.c * t2;
+  }
+}
+template <int I, int K, class T>
+ZR_HD void gv_col(const Givens<T>& g, M3<T>& m)
+{
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    T t1 = m.a[j][I], t2 = m.a[j][K];
+    m.a[j][I] = g.c * t1 - g.s * t2;
+    m.a[j][K] = g.s * t1 + g.c * t2;
+  }
+}
+
+template <class T>
+ZR_HD void zero_chase(M3<T>& h, M3<T>& u, M3<T>& v)
+{
+  Givens<T> r1 = gv_compute(h.a[0][0], h.a[1][0]);
+  Givens<T> r2;
+  if (h.a[1][0] != (T)0)
+    r2 = gv_compute(h.a[0][0] * h.a[0][1] + h.a[1][0] * h.a[1][1], h.a[0][0] * h.a[0][2] + h.a[1][0] * h.a[1][2]);
+  else
+    r2 = gv_compute(h.a[0][1], h.a[0][2]);
+  gv_row<0, 1>(r1, h);
+  gv_col<1, 2>(r2, h);
+  gv_col<1, 2>(r2, v);
+  Givens<T> r3 = gv_compute(h.a[1][1], h.a[2][1]);
+  gv_row<1, 2>(r3, h);
+  gv_col<0, 1>(r1, u);
+  gv_col<1, 2>(r3, u);
+}
+
+template <class T>
+ZR_HD void svd2(T a00, T a01, T a10, T a11, Givens<T>& u, T& s0, T& s1, Givens<T>& v)
+{
+  T x0 = a00 + a11, x1 = a10 - a01;
+  T den = sqrt(x0 * x0 + x1 * x1);
+  u.c = (T)1;
+  u.s = (T)0;
+  if (den != (T)0) {
+    u.c = x0 / den;
+    u.s = -x1 / den;
+  }
+  T sx = u.c * a00 - u.s * a10;
+  T sy = u.c * a01 - u.s * a11;
+  T sz = u.s * a01 + u.c * a11;
+  T cosine, sine;
+  if (sy * sy == (T)0) {
+    cosine = (T)1;
+    sine = (T)0;
+    s0 = sx;
+    s1 = sz;
+  } else {
+    T tau = (T)0.5 * (sx - sz);
+    T w = sqrt(tau * tau + sy * sy);
+    T t = (tau > (T)0) ? sy / (tau + w) : sy / (tau - w);
+    cosine = (T)1 / sqrt(t * t + (T)1);
+    sine = -t * cosine;
+    T c2 = cosine * cosine, csy = (T)2 * cosine * sine * sy, sn2 = sine * sine;
+    s0 = c2 * sx - csy + sn2 * sz;
+    s1 = sn2 * sx + csy + c2 * sz;
+  }
+  if (s0 < s1) {
+    T t = s0;
+    s0 = s1;
+    s1 = t;
+    v.c = -sine;
+    v.s = cosine;
+  } else {
+    v.c = cosine;
+    v.s = sine;
+  }
+  T nc = u.c * v.c - u.s * v.s;
+  T ns = u.s * v.c + u.c * v.s;
+  u.c = nc;
+  u.s = ns;
+}
+
+template <class T>
+ZR_HD T wilkinson_shift(T a1, T b1, T a2)
+{
+  T d = (T)0.5 * (a1 - a2);
+  T bs = b1 * b1;
+  return a2 - copysign(bs / (fabs(d) + sqrt(d * d + bs)), d);
+}
+
+template <int TT, class T>
+ZR_HD void process_block(M3<T>& b, M3<T>& u, T s[3], M3<T>& v)
+{
+  constexpr int other = (TT == 1) ? 0 : 2;
+  s[other] = b.a[other][other];
+  Givens<T> gu, gvv;
+  svd2(b.a[TT][TT], b.a[TT][TT + 1], b.a[TT + 1][TT], b.a[TT + 1][TT + 1], gu, s[TT], s[TT + 1], gvv);
+  gv_col<TT, TT + 1>(gu, u);
+  gv_col<TT, TT + 1>(gvv, v);
+}
+
+template <int I, class T>
+ZR_HD void neg_col(M3<T>& m)
+{
+#pragma unroll
+  for (int r = 0; r < 3; ++r) m.a[r][I] = -m.a[r][I];
+}
+template <int I, int K, class T>
+ZR_HD void swap_col(M3<T>& m)
+{
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    T t = m.a[r][I];
+    m.a[r][I] = m.a[r][K];
+    m.a[r][K] = t;
+  }
+}
+template <int I, class T>
+ZR_HD void flip_sign(M3<T>& u, T s[3])
+{
+  s[I] = -s[I];
+  neg_col<I>(u);
+}
+template <class T>
+ZR_HD void swp(T& a, T& b)
+{
+  T t = a;
+  a = b;
+  b = t;
+}
+
+template <class T>
+ZR_HD void sort0(M3<T>& u, T s[3], M3<T>& v)
+{
+  if (fabs(s[1]) >= fabs(s[2])) {
+    if (s[1] < (T)0) {
+      flip_sign<1>(u, s);
+      flip_sign<2>(u, s);
+    }
+    return;
+  }
+  if (s[2] < (T)0) {
+    flip_sign<1>(u, s);
+    flip_sign<2>(u, s);
+  }
+  swp(s[1], s[2]);
+  swap_col<1, 2>(u);
+  swap_col<1, 2>(v);
+  if (s[1] > s[0]) {
+    swp(s[0], s[1]);
+    swap_col<0, 1>(u);
+    swap_col<0, 1>(v);
+  } else {
+    neg_col<2>(u);
+    neg_col<2>(v);
+  }
+}
+
+template <class T>
+ZR_HD void sort1(M3<T>& u, T s[3], M3<T>& v)
+{
+  if (fabs(s[0]) >= s[1]) {
+    if (s[0] < (T)0) {
+      flip_sign<0>(u, s);
+      flip_sign<2>(u, s);
+    }
+    return;
+  }
+  swp(s[0], s[1]);
+  swap_col<0, 1>(u);
+  swap_col<0, 1>(v);
+  if (fabs(s[1]) < fabs(s[2])) {
+    swp(s[1], s[2]);
+    swap_col<1, 2>(u);
+    swap_col<1, 2>(v);
+  } else {
+    neg_col<1>(u);
+    neg_col<1>(v);
+  }
+  if (s[1] < (T)0) {
+    flip_sign<1>(u, s);
+    flip_sign<2>(u, s);
+  }
+}
+
+template <class T>
+struct SvdTol;
+template <>
+struct SvdTol<double> {
+  static constexpr double value = 1024.0 * 2.220446049250313e-16;  // ImplicitQRSVD.h:526-533
+};
+template <>
+struct SvdTol<float> {
+  static constexpr float value = 128.0f * 1.1920928955078125e-07f;  // ImplicitQRSVD.h:518-524
+};
+
+constexpr int kSvdMaxSweeps = 64;
+
+template <class T>
+ZR_HD int svd3(const M3<T>& a, M3<T>& u, T s[3], M3<T>& v)
+{
+  T tol = SvdTol<T>::value;
+  M3<T> b = a;
+  set_identity(u);
+  set_identity(v);
+  {
+    Givens<T> r = gv_compute(b.a[1][0], b.a[2][0]);
+    gv_row<1, 2>(r, b);
+    gv_col<1, 2>(r, u);
+    zero_chase(b, u, v);
+  }
+  int count = 0;
+  T a1 = b.a[0][0], b1 = b.a[0][1], a2 = b.a[1][1], a3 = b.a[2][2], b2 = b.a[1][2];
+  T g1 = a1 * b1, g2 = a2 * b2;
+  T scale = (T)0.5 * sqrt(a1 * a1 + a2 * a2 + a3 * a3 + b1 * b1 + b2 * b2);
+  tol *= (scale > (T)1 ? scale : (T)1);
+  while (fabs(b2) > tol && fabs(b1) > tol && fabs(a1) > tol && fabs(a2) > tol && fabs(a3) > tol &&
+         count < kSvdMaxSweeps) {
+    T mu = wilkinson_shift(a2 * a2 + b1 * b1, g2, a3 * a3 + b2 * b2);
+    Givens<T> r = gv_compute(a1 * a1 - mu, g1);
+    gv_col<0, 1>(r, b);
+    gv_col<0, 1>(r, v);
+    zero_chase(b, u, v);
+    a1 = b.a[0][0];
+    b1 = b.a[0][1];
+    a2 = b.a[1][1];
+    a3 = b.a[2][2];
+    b2 = b.a[1][2];
+    g1 = a1 * b1;
+    g2 = a2 * b2;
+    count++;
+  }
+  if (fabs(b2) <= tol) {
+    process_block<0>(b, u, s, v);
+    sort0(u, s, v);
+  } else if (fabs(b1) <= tol) {
+    process_block<1>(b, u, s, v);
+    sort1(u, s, v);
+  } else if (fabs(a2) <= tol) {
+    Givens<T> r1 = gv_compute_unconv(b.a[1][2], b.a[2][2]);
+    gv_row<1, 2>(r1, b);
+    gv_col<1, 2>(r1, u);
+    process_block<0>(b, u, s, v);
+    sort0(u, s, v);
+  } else if (fabs(a3) <= tol) {
+    Givens<T> r1 = gv_compute(b.a[1][1], b.a[1][2]);
+    gv_col<1, 2>(r1, b);
+    gv_col<1, 2>(r1, v);
+    Givens<T> r2 = gv_compute(b.a[0][0], b.a[0][2]);
+    gv_col<0, 2>(r2, b);
+    gv_col<0, 2>(r2, v);
+    process_block<0>(b, u, s, v);
+    sort0(u, s, v);
+  } else if (fabs(a1) <= tol) {
+    Givens<T> r1 = gv_compute_unconv(b.a[0][1], b.a[1][1]);
+    gv_row<0, 1>(r1, b);
+    gv_col<0, 1>(r1, u);
+    Givens<T> r2 = gv_compute_unconv(b.a[0][2], b.a[2][2]);
+    gv_row<0, 2>(r2, b);
+    gv_col<0, 2>(r2, u);
+    process_block<1>(b, u, s, v);
+    sort1(u, s, v);
+  } else {
+    process_block<0>(b, u, s, v);
+    sort0(u, s, v);
+  }
+  return count;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-corotated elasticity (Stomakhin et al. 2012): Kirchhoff stress
+//   tau = P F^T = 2 mu (F - R) F^T + lam (J - 1) J I,   R = U V^T.
+// ---------------------------------------------------------------------------
+template <class T>
+ZR_HD void fcr_kirchhoff(T mu, T lam, const M3<T>& f, M3<T>& tau)
+{
+  M3<T> u, v;
+  T s[3];
+  svd3(f, u, s, v);
+  M3<T> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.a[i][j] = u.a[i][0] * v.a[j][0] + u.a[i][1] * v.a[j][1] + u.a[i][2] * v.a[j][2];
+  T J = det3(f);
+  T vol_term = lam * (J - (T)1) * J;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      T acc = (T)0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc += (f.a[i][k] - r.a[i][k]) * f.a[j][k];
+      tau.a[i][j] = (T)2 * mu * acc + (i == j ? vol_term : (T)0);
+    }
+}
+
+}  // namespace zr

@@ -1,0 +1,247 @@
+"""Python binding over the C-ABI of ``libzr_mpm.so`` (include/zr_mpm.h).
+
+``MpmEngine`` mirrors the step API of the reference's
+``MpmSimulationBase<double,3>`` (Lib/MPM/MpmSimulationBase.h:143-232):
+``sortParticlesAndPolluteGrid``, ``particlesToGrid``,
+``gridVelocityExplicitUpdate``, ``gridToParticles`` and
+``advanceOneTimeStep``.  A CFL violation surfaces as ``RestartException``
+like the reference's (Lib/Ziran/Sim/SimulationBase.h:21-29).
+
+The native library is required: importing works without it, but creating an
+engine raises if ``ziran2019_amd/lib/libzr_mpm.so`` is missing or no GPU is
+visible.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libzr_mpm.so")
+
+ZR_OK = 0
+ZR_CFL_RESTART = 1
+ZR_ERR_ARG = -1
+ZR_ERR_HIP = -2
+ZR_ERR_DOMAIN = -3
+ZR_ERR_STATE = -4
+
+ZR_STICKY, ZR_SLIP, ZR_SEPARATE = 1, 2, 3
+ZR_HALF_SPACE, ZR_SPHERE = 0, 1
+
+DUMP = dict(sorter=0, order=1, base_offset=2, group_first=3, group_last=4, block_offset=5, pages=6,
+            grid_m=7, grid_v=8, grid_f=9, grid_vstar=10, node_idx=11)
+_DUMP_DTYPE = dict(sorter=np.uint64, order=np.int32, base_offset=np.uint64, group_first=np.int64,
+                   group_last=np.int64, block_offset=np.uint64, pages=np.uint64, grid_m=np.float64,
+                   grid_v=np.float64, grid_f=np.float64, grid_vstar=np.float64, node_idx=np.int64)
+
+# every symbol include/zr_mpm.h declares
+EXPORTS = (
+    "zr_create", "zr_destroy", "zr_last_error", "zr_set_colliders", "zr_set_time", "zr_get_time",
+    "zr_upload_particles", "zr_download_particles", "zr_sort_particles_and_pollute_grid",
+    "zr_particles_to_grid", "zr_grid_velocity_explicit_update", "zr_grid_to_particles",
+    "zr_advance_one_time_step", "zr_debug_size", "zr_debug_dump", "zr_profile_enable", "zr_profile_read",
+)
+PROF = dict(p2g=0, grid=1, g2p=2, sort=3)
+
+
+class RestartException(RuntimeError):
+    """A particle travelled more than one grid cell in the step
+    (MpmSimulationBase.cpp:983-1002)."""
+
+
+class ZrError(RuntimeError):
+    pass
+
+
+class ZrConfig(ctypes.Structure):
+    _fields_ = [("dx", ctypes.c_double), ("gravity", ctypes.c_double * 3), ("cfl", ctypes.c_double),
+                ("grid_log2", ctypes.c_int32), ("precision", ctypes.c_int32), ("transfer", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class ZrCollider(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int32), ("type", ctypes.c_int32), ("friction", ctypes.c_double),
+                ("origin", ctypes.c_double * 3), ("normal", ctypes.c_double * 3), ("radius", ctypes.c_double),
+                ("velocity", ctypes.c_double * 3)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class ZrParticles(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("X", _dp), ("V", _dp), ("C", _dp), ("F", _dp), ("mass", _dp),
+                ("vol", _dp), ("mu", _dp), ("lam", _dp)]
+
+
+class ZrStepStats(ctypes.Structure):
+    _fields_ = [("num_groups", ctypes.c_int64), ("num_pages", ctypes.c_int64), ("cfl_flags", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load ``libzr_mpm.so``; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ZrError(f"{path} is missing: build it with `python -m ziran2019_amd.build`")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.zr_create.argtypes = [ctypes.POINTER(ZrConfig), ctypes.POINTER(vp)]
+    L.zr_create.restype = ctypes.c_int
+    L.zr_destroy.argtypes = [vp]
+    L.zr_destroy.restype = None
+    L.zr_last_error.argtypes = [vp]
+    L.zr_last_error.restype = ctypes.c_char_p
+    L.zr_set_colliders.argtypes = [vp, ctypes.POINTER(ZrCollider), ctypes.c_int32]
+    L.zr_set_time.argtypes = [vp, ctypes.c_double]
+    L.zr_get_time.argtypes = [vp]
+    L.zr_get_time.restype = ctypes.c_double
+    L.zr_upload_particles.argtypes = [vp, ctypes.POINTER(ZrParticles)]
+    L.zr_download_particles.argtypes = [vp, ctypes.POINTER(ZrParticles)]
+    L.zr_sort_particles_and_pollute_grid.argtypes = [vp, ctypes.POINTER(ZrStepStats)]
+    L.zr_particles_to_grid.argtypes = [vp]
+    L.zr_grid_velocity_explicit_update.argtypes = [vp, ctypes.c_double]
+    L.zr_grid_to_particles.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
+    L.zr_advance_one_time_step.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
+    L.zr_debug_size.argtypes = [vp, ctypes.c_int32]
+    L.zr_debug_size.restype = ctypes.c_int64
+    L.zr_debug_dump.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_size_t]
+    L.zr_profile_enable.argtypes = [vp, ctypes.c_int32]
+    L.zr_profile_read.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_int64)]
+    _lib = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def grid_log2_for(dx, extent=1.0):
+    """Smallest power-of-two node range that holds [0, extent] plus the
+    8-node margin the engine keeps."""
+    need = int(np.ceil(extent / dx)) + 9
+    L = 6
+    while (1 << L) < need:
+        L += 1
+    return L
+
+
+class MpmEngine:
+    """One GPU MPM simulation (FP64 plain path, MLS transfers)."""
+
+    def __init__(self, dx, gravity=(0.0, -9.8, 0.0), cfl=0.6, grid_log2=None):
+        self.L = load_library()
+        cfg = ZrConfig()
+        cfg.dx = float(dx)
+        cfg.gravity = (ctypes.c_double * 3)(*gravity)
+        cfg.cfl = float(cfl)
+        cfg.grid_log2 = int(grid_log2 if grid_log2 is not None else grid_log2_for(dx))
+        cfg.precision = 0
+        cfg.transfer = 0
+        h = ctypes.c_void_p()
+        rc = self.L.zr_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != ZR_OK:
+            raise ZrError(f"zr_create failed with status {rc}")
+        self.h = h
+        self.dx = dx
+        self.n = 0
+        self.stats = ZrStepStats()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.zr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc == ZR_CFL_RESTART:
+            raise RestartException(f"{what}: particle travelling more than a grid cell detected")
+        if rc != ZR_OK:
+            msg = self.L.zr_last_error(self.h).decode()
+            raise ZrError(f"{what} failed ({rc}): {msg}")
+
+    # ---------------------------------------------------------- scene setup
+    def set_colliders(self, colliders):
+        arr = (ZrCollider * max(1, len(colliders)))()
+        for k, c in enumerate(colliders):
+            arr[k].shape = int(c.get("shape", 0))
+            arr[k].type = int(c.get("type", ZR_STICKY))
+            arr[k].friction = float(c.get("friction", 0.0))
+            arr[k].origin = (ctypes.c_double * 3)(*c.get("origin", (0, 0, 0)))
+            arr[k].normal = (ctypes.c_double * 3)(*c.get("normal", (0, 1, 0)))
+            arr[k].radius = float(c.get("radius", 0.0))
+            arr[k].velocity = (ctypes.c_double * 3)(*c.get("velocity", (0, 0, 0)))
+        self._check(self.L.zr_set_colliders(self.h, arr, len(colliders)), "zr_set_colliders")
+
+    def set_time(self, t):
+        self._check(self.L.zr_set_time(self.h, float(t)), "zr_set_time")
+
+    @property
+    def time(self):
+        return float(self.L.zr_get_time(self.h))
+
+    def upload(self, scene):
+        n = len(scene["mass"])
+        keep = {}
+        for key, nc in (("X", 3), ("V", 3), ("C", 9), ("F", 9), ("mass", 1), ("vol", 1), ("mu", 1), ("lam", 1)):
+            a = np.ascontiguousarray(np.asarray(scene[key], dtype=np.float64).reshape(n * nc))
+            keep[key] = a
+        p = ZrParticles(n, *[_dptr(keep[k]) for k in ("X", "V", "C", "F", "mass", "vol", "mu", "lam")])
+        self._check(self.L.zr_upload_particles(self.h, ctypes.byref(p)), "zr_upload_particles")
+        self.n = n
+
+    def download(self):
+        n = self.n
+        X = np.zeros(3 * n); V = np.zeros(3 * n); C = np.zeros(9 * n); F = np.zeros(9 * n)
+        p = ZrParticles(n, _dptr(X), _dptr(V), _dptr(C), _dptr(F), None, None, None, None)
+        self._check(self.L.zr_download_particles(self.h, ctypes.byref(p)), "zr_download_particles")
+        return dict(X=X.reshape(n, 3), V=V.reshape(n, 3), C=C.reshape(n, 9), F=F.reshape(n, 9))
+
+    # ---------------------------------------------------------- step API
+    def sortParticlesAndPolluteGrid(self):
+        self._check(self.L.zr_sort_particles_and_pollute_grid(self.h, ctypes.byref(self.stats)),
+                    "sortParticlesAndPolluteGrid")
+
+    def particlesToGrid(self):
+        self._check(self.L.zr_particles_to_grid(self.h), "particlesToGrid")
+
+    def gridVelocityExplicitUpdate(self, dt):
+        self._check(self.L.zr_grid_velocity_explicit_update(self.h, float(dt)), "gridVelocityExplicitUpdate")
+
+    def gridToParticles(self, dt):
+        self._check(self.L.zr_grid_to_particles(self.h, float(dt), ctypes.byref(self.stats)), "gridToParticles")
+
+    def advanceOneTimeStep(self, dt):
+        self._check(self.L.zr_advance_one_time_step(self.h, float(dt), ctypes.byref(self.stats)),
+                    "advanceOneTimeStep")
+
+    # ---------------------------------------------------------- timing
+    def profile(self, on=True):
+        self._check(self.L.zr_profile_enable(self.h, int(bool(on))), "zr_profile_enable")
+
+    def kernel_time(self, which):
+        """(total milliseconds, launches) of a kernel class since profiling was enabled."""
+        t = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self.L.zr_profile_read(self.h, PROF[which], ctypes.byref(t), ctypes.byref(n)), "zr_profile_read")
+        return t.value, n.value
+
+    # ---------------------------------------------------------- introspection
+    def dump(self, what):
+        code = DUMP[what]
+        size = int(self.L.zr_debug_size(self.h, code))
+        if size < 0:
+            raise ZrError(f"bad dump {what}")
+        out = np.zeros(size // np.dtype(_DUMP_DTYPE[what]).itemsize, dtype=_DUMP_DTYPE[what])
+        self._check(self.L.zr_debug_dump(self.h, code, out.ctypes.data_as(ctypes.c_void_p), size), "zr_debug_dump")
+        if what in ("grid_v", "grid_f", "grid_vstar"):
+            out = out.reshape(-1, 3)
+        return out

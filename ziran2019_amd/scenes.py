@@ -1,0 +1,134 @@
+"""Seeded synthetic particle scenes for the MPM time step.
+
+The reference samples with Poisson disks inside level sets
+(Lib/MPM/MpmInitializationHelper.h:187-202) using std::mt19937, whose
+uniform_real_distribution is implementation-defined.  The workloads here use a
+counter-based splitmix64 stream instead, so the same seed gives the same bits
+on every machine.  Particle attributes follow what the reference creates per
+particle: X, V, C (APIC affine matrix), F, mass, element measure V0, and a
+per-particle constitutive model (mu, lambda) -- MpmParticleHandleBase.cpp:457,
+FBasedMpmForceHelper.h:17-164.
+
+Particles are emitted in the engine's sort order (page Morton order, then
+cell, then index) so that the storage order of the first step is coherent.
+"""
+import numpy as np
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(counter, seed):
+    """k-th output of the splitmix64 generator seeded with ``seed``."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.asarray(counter, dtype=np.uint64) + np.uint64(1)) * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(counter, seed):
+    """Uniform doubles in [0, 1) with 53 random bits."""
+    return (splitmix64(counter, seed) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def lame(E, nu):
+    """Lame parameters, NeoHookeanBorden::setLameParameters
+    (Lib/Ziran/Physics/ConstitutiveModel/NeoHookeanBorden.cpp:16-21)."""
+    E = np.asarray(E, dtype=np.float64)
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = E / (2.0 * (1.0 + nu))
+    return mu, lam
+
+
+# ---- SPGrid page/cell code of a node (GridState<double,3> masks) ---------
+def _spread3(v):
+    v = np.asarray(v, dtype=np.uint64) & np.uint64(0x3FF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x030000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x0300F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x030C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x09249249)
+    return v
+
+
+def cell_code(bx, by, bz):
+    """Linear_Offset(b) >> 7 for base nodes with 0 <= b < 1024
+    (SPGrid_Mask.h:150-174 with the GridState<double,3> masks)."""
+    bx = np.asarray(bx, dtype=np.int64).astype(np.uint64)
+    by = np.asarray(by, dtype=np.int64).astype(np.uint64)
+    bz = np.asarray(bz, dtype=np.int64).astype(np.uint64)
+    page = _spread3(bx >> np.uint64(1)) | (_spread3(bz >> np.uint64(2)) << np.uint64(1)) | \
+        (_spread3(by >> np.uint64(2)) << np.uint64(2))
+    e = ((bx & np.uint64(1)) << np.uint64(4)) | ((by & np.uint64(3)) << np.uint64(2)) | (bz & np.uint64(3))
+    return (page << np.uint64(5)) | e
+
+
+def base_nodes(X, dx):
+    """int_floor(X * (1/dx) - 0.5) per axis (MpmSimulationBase.cpp:1046-1049)."""
+    y = np.asarray(X, dtype=np.float64) * (1.0 / dx) - 0.5
+    t = np.trunc(y)
+    return (t - (t > y)).astype(np.int64)
+
+
+def morton_order(X, dx):
+    b = base_nodes(X, dx)
+    code = cell_code(b[:, 0], b[:, 1], b[:, 2])
+    return np.lexsort((np.arange(len(code)), code))
+
+
+def _particles_in_cells(cells, ppc, dx, seed, stream):
+    """``ppc`` particles per cell, each uniform in its cell."""
+    ncell = len(cells)
+    k = np.arange(ncell * ppc * 3, dtype=np.uint64) + np.uint64(stream) * np.uint64(1 << 40)
+    u = uniform01(k, seed).reshape(ncell, ppc, 3)
+    X = (cells[:, None, :].astype(np.float64) + u) * dx
+    return X.reshape(-1, 3)
+
+
+def _finish(X, dx, rho, ppc, mu, lam, order=True):
+    n = len(X)
+    if order:
+        perm = morton_order(X, dx)
+        X = X[perm]
+        mu = np.broadcast_to(mu, (n,))[perm] if np.ndim(mu) else mu
+        lam = np.broadcast_to(lam, (n,))[perm] if np.ndim(lam) else lam
+    vol = dx ** 3 / ppc
+    F = np.tile(np.eye(3).reshape(1, 9), (n, 1))
+    return dict(
+        X=np.ascontiguousarray(X),
+        V=np.zeros((n, 3)),
+        C=np.zeros((n, 9)),
+        F=F,
+        mass=np.full(n, rho * vol),
+        vol=np.full(n, vol),
+        mu=np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (n,))),
+        lam=np.ascontiguousarray(np.broadcast_to(np.asarray(lam, dtype=np.float64), (n,))),
+    )
+
+
+def cube_scene(dx=1.0 / 128, lo_cell=51, ncells=26, ppc=8, E=1e4, nu=0.3, rho=1000.0, seed=20190701):
+    """Plain explicit MLS-MPM workload: a cube of ``ncells``^3 cells starting at
+    cell index ``lo_cell`` (26^3 cells at dx=1/128 from cell 51: the side-0.2
+    cube centred at 0.5), ``ppc`` particles per cell jittered uniformly in-cell,
+    fixed-corotated material.  Default size: 26^3 * 8 = 140,608 particles."""
+    r = np.arange(lo_cell, lo_cell + ncells)
+    cells = np.stack(np.meshgrid(r, r, r, indexing="ij"), axis=-1).reshape(-1, 3)
+    X = _particles_in_cells(cells, ppc, dx, seed, 0)
+    mu, lam = lame(E, nu)
+    out = _finish(X, dx, rho, ppc, mu, lam)
+    out.update(dx=dx, gravity=(0.0, -9.8, 0.0),
+               colliders=[dict(shape=0, type=1, origin=(0.0, 0.1, 0.0), normal=(0.0, 1.0, 0.0))])
+    return out
+
+
+def cube_scene_small(seed=7, ncells=6, lo_cell=20, dx=1.0 / 64, E=1e4, v0=(0.3, -0.2, 0.1), spin=2.0):
+    """A small cube with a nonzero initial velocity field (translation plus a
+    rigid spin) used by parity tests."""
+    s = cube_scene(dx=dx, lo_cell=lo_cell, ncells=ncells, ppc=8, E=E, seed=seed)
+    X = s["X"]
+    c = X.mean(axis=0)
+    r = X - c
+    w = np.array([0.0, spin, 0.5 * spin])
+    s["V"] = np.asarray(v0)[None, :] + np.cross(w[None, :], r)
+    return s
