@@ -24,6 +24,24 @@ _u64 = ctypes.c_uint64
 _p = ctypes.c_void_p
 
 
+class ZroImplicitOpts(ctypes.Structure):
+    _fields_ = [("newton_max_iterations", ctypes.c_int32), ("newton_tolerance", _d),
+                ("cg_max_iterations", ctypes.c_int32), ("cg_tolerance", _d), ("cg_relative_tolerance", _d)]
+
+
+class ZroImplicitStats(ctypes.Structure):
+    _fields_ = [("newton_iterations", ctypes.c_int32), ("cg_iterations", ctypes.c_int32),
+                ("residual_norm", _d * 8), ("num_residuals", ctypes.c_int32), ("converged", ctypes.c_int32)]
+
+
+def implicit_opts(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=20,
+                  cg_tolerance=16 * 2.220446049250313e-16, cg_relative_tolerance=0.0):
+    """Defaults of the reference: NewtonsMethod(1e-5, 3) (MpmSimulationBase.cpp:51),
+    ConjugateGradient(20) with LinearSolver::tolerance 16 eps (LinearSolver.h:26)."""
+    return ZroImplicitOpts(int(newton_max_iterations), float(newton_tolerance), int(cg_max_iterations),
+                           float(cg_tolerance), float(cg_relative_tolerance))
+
+
 class ZroCollider(ctypes.Structure):
     _fields_ = [
         ("shape", ctypes.c_int32),
@@ -91,6 +109,21 @@ def lib():
         L.zro_step_explicit.restype = _i
         L.zro_step_explicit.argtypes = [_p, _d]
         L.zro_get_grid.argtypes = [_p, _p, _p, _p, _p]
+        L.zro_p2g_apic.restype = _i64
+        L.zro_p2g_apic.argtypes = [_p]
+        L.zro_backward_euler.restype = _i
+        L.zro_backward_euler.argtypes = [_p, _d, ctypes.POINTER(ZroImplicitOpts), ctypes.POINTER(ZroImplicitStats)]
+        L.zro_g2p_apic.restype = _i
+        L.zro_g2p_apic.argtypes = [_p, _d]
+        L.zro_step_implicit.restype = _i
+        L.zro_step_implicit.argtypes = [_p, _d, ctypes.POINTER(ZroImplicitOpts), ctypes.POINTER(ZroImplicitStats)]
+        L.zro_implicit_multiply.argtypes = [_p, _p, _p]
+        L.zro_implicit_residual.argtypes = [_p, _p, _p]
+        L.zro_implicit_energy.restype = _d
+        L.zro_implicit_energy.argtypes = [_p, _p]
+        L.zro_num_node_slots.restype = _i64
+        L.zro_num_node_slots.argtypes = [_p]
+        L.zro_get_implicit.argtypes = [_p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -254,6 +287,51 @@ class OracleSim:
 
     def step(self, dt):
         return int(lib().zro_step_explicit(self.h, float(dt)))
+
+    # ---- implicit backward Euler
+    def p2g_apic(self):
+        return int(lib().zro_p2g_apic(self.h))
+
+    def backward_euler(self, dt, opts=None):
+        st = ZroImplicitStats()
+        o = opts if opts is not None else implicit_opts()
+        lib().zro_backward_euler(self.h, float(dt), ctypes.byref(o), ctypes.byref(st))
+        return st
+
+    def g2p_apic(self, dt):
+        return int(lib().zro_g2p_apic(self.h, float(dt)))
+
+    def step_implicit(self, dt, opts=None):
+        st = ZroImplicitStats()
+        o = opts if opts is not None else implicit_opts()
+        flags = int(lib().zro_step_implicit(self.h, float(dt), ctypes.byref(o), ctypes.byref(st)))
+        return flags, st
+
+    def num_node_slots(self):
+        return int(lib().zro_num_node_slots(self.h))
+
+    def implicit_multiply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        b = np.zeros_like(x)
+        lib().zro_implicit_multiply(self.h, _ptr(x), _ptr(b))
+        return b.reshape(-1, 3)
+
+    def implicit_residual(self, dv):
+        dv = np.ascontiguousarray(dv, dtype=np.float64).reshape(-1)
+        r = np.zeros_like(dv)
+        lib().zro_implicit_residual(self.h, _ptr(dv), _ptr(r))
+        return r.reshape(-1, 3)
+
+    def implicit_energy(self, dv):
+        dv = np.ascontiguousarray(dv, dtype=np.float64).reshape(-1)
+        return float(lib().zro_implicit_energy(self.h, _ptr(dv)))
+
+    def implicit_state(self):
+        k = self.num_node_slots()
+        dv = np.zeros(3 * k); vn = np.zeros(3 * k)
+        col = np.zeros(k, dtype=np.int8)
+        lib().zro_get_implicit(self.h, _ptr(dv), _ptr(vn), _ptr(col))
+        return dict(dv=dv.reshape(k, 3), vn=vn.reshape(k, 3), collided=col.astype(bool))
 
     def grid(self):
         npg = lib().zro_num_pages(self.h)

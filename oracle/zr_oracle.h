@@ -87,6 +87,43 @@ int zro_g2p_mls(zro_sim* s, double dt);
 /* the whole symplectic step (MpmSimulationBase.cpp:256-281) */
 int zro_step_explicit(zro_sim* s, double dt);
 
+/* ---- implicit backward Euler (APIC transfers) ---- */
+typedef struct {
+  int32_t newton_max_iterations;   /* NewtonsMethod max_iterations (MpmSimulationBase.cpp:51: 3) */
+  double newton_tolerance;         /* NewtonsMethod tolerance (1e-5) */
+  int32_t cg_max_iterations;       /* ConjugateGradient(20) (BackwardEuler.h:90-91) */
+  double cg_tolerance;             /* LinearSolver::tolerance, default 16 eps (LinearSolver.h:26) */
+  double cg_relative_tolerance;    /* <= 0: Newton-adaptive min(0.5, sqrt(max(|r|, tol))) */
+} zro_implicit_opts;
+
+typedef struct {
+  int32_t newton_iterations;       /* Newton iterations that computed a step */
+  int32_t cg_iterations;           /* total CG iterations */
+  double residual_norm[8];         /* mass-weighted residual norm per Newton iteration */
+  int32_t num_residuals;
+  int32_t converged;
+} zro_implicit_stats;
+
+/* particlesToGridHelper<APIC> (MpmSimulationBase.cpp:563-608) + normalisation */
+int64_t zro_p2g_apic(zro_sim* s);
+/* backwardEulerStep (MpmSimulationBase.cpp:610-657) */
+int zro_backward_euler(zro_sim* s, double dt, const zro_implicit_opts* o, zro_implicit_stats* st);
+/* gridToParticlesHelper<APIC, deg2, !MLS> + evolveStrain */
+int zro_g2p_apic(zro_sim* s, double dt);
+/* advanceOneTimeStep, implicit branch (MpmSimulationBase.cpp:282-295) */
+int zro_step_implicit(zro_sim* s, double dt, const zro_implicit_opts* o, zro_implicit_stats* st);
+/* matrix-free operator of the last backward-Euler solve at the final Newton
+   iterate, on node vectors in page-list x element order (3 per node):
+   b = M x + dt^2 sum_p V0 dP(grad x_p Fn) Fn^T grad w_ip   (BackwardEuler.h:322-351) */
+void zro_implicit_multiply(zro_sim* s, const double* x, double* b);
+/* residual at dv: dt m g + dt f(dv) - m dv, projected (BackwardEuler.h:101-124) */
+void zro_implicit_residual(zro_sim* s, const double* dv, double* r);
+/* objective energy whose negative gradient is the unprojected residual */
+double zro_implicit_energy(zro_sim* s, const double* dv);
+int64_t zro_num_node_slots(const zro_sim* s);
+/* dv (Newton result), vn and collision flags of the last solve, per node slot */
+void zro_get_implicit(const zro_sim* s, double* dv, double* vn, char* collided);
+
 /* grid fields, per node in page-list order x 32 elements:
    m[npages*32], v[npages*32*3] (normalised), new_v (force, then v*),
    idx[npages*32] (node numbering, -1 for m==0) */

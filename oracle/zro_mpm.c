@@ -127,6 +127,11 @@ struct zro_sim {
   zro_node* nodes;             /* npages * 32 */
   int64_t num_nodes;
   char* explicit_collided;     /* per node slot */
+  /* implicit state: Fn backup, node vectors (3 per node slot), projections */
+  double* Fn;
+  double *vn, *dv, *proj;      /* proj: 9 per node, P = I - K K^T (identity when free) */
+  char* collided;
+  double be_dt;
   int ncol;
   zro_collider col[ZRO_MAX_COLLIDERS];
 };
@@ -146,7 +151,8 @@ zro_sim* zro_create(double dx, const double* gravity, double cfl, int nthreads)
 static void free_particles(zro_sim* s)
 {
   free(s->X); free(s->V); free(s->C); free(s->F); free(s->mass); free(s->vol);
-  free(s->mu); free(s->lam); free(s->vtau); free(s->gradV); free(s->sorter);
+  free(s->mu); free(s->lam); free(s->vtau); free(s->gradV); free(s->sorter); free(s->Fn);
+  s->Fn = NULL;
   free(s->order); free(s->base_offset); free(s->gfirst); free(s->glast);
   free(s->block_offset);
 }
@@ -157,6 +163,7 @@ void zro_destroy(zro_sim* s)
   free_particles(s);
   free(s->pages); free(s->hkeys); free(s->hvals); free(s->nodes);
   free(s->explicit_collided);
+  free(s->vn); free(s->dv); free(s->proj); free(s->collided);
   free(s);
 }
 
@@ -629,4 +636,495 @@ void zro_get_grid(const zro_sim* s, double* m, double* v, double* new_v, int64_t
     }
     if (idx) idx[k] = g->idx;
   }
+}
+
+/* ====================================================================== */
+/* implicit backward Euler (APIC transfers)                               */
+/* ====================================================================== */
+
+/* weights and weight gradients in iterateKernel order (MpmGrid.h:231-268) */
+typedef struct {
+  int base[3];
+  double w[3][3], dw[3][3];
+} zro_spline;
+
+static void spline_of(const zro_sim* s, const double* Xp, zro_spline* sp)
+{
+  for (int d = 0; d < 3; ++d) zro_bspline_weights(Xp[d] * s->one_over_dx, &sp->base[d], sp->w[d], sp->dw[d]);
+}
+
+/* w_abc and grad w_abc exactly as iterateKernel forms them */
+static double spline_node(const zro_sim* s, const zro_spline* sp, int a, int b, int c, double* gw)
+{
+  double wi = sp->w[0][a], wj = sp->w[1][b], wk = sp->w[2][c];
+  double dwidxi = s->one_over_dx * sp->dw[0][a];
+  double wij = wi * wj;
+  double dwijdxi = dwidxi * wj;
+  double dwijdxj = wi * s->one_over_dx * sp->dw[1][b];
+  gw[0] = dwijdxi * wk;
+  gw[1] = dwijdxj * wk;
+  gw[2] = wij * s->one_over_dx * sp->dw[2][c];
+  return wij * wk;
+}
+
+static int64_t node_slot(zro_sim* s, int i, int j, int k)
+{
+  int64_t slot = -1;
+  node_at(s, i, j, k, &slot);
+  return slot;
+}
+
+static void p2g_apic_particle(zro_sim* s, int i)
+{
+  const double* Xp = s->X + 3 * i;
+  double mass = s->mass[i], mom[3], Cm[3][3];
+  for (int d = 0; d < 3; ++d) mom[d] = mass * s->V[3 * i + d];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) Cm[r][c] = mass * s->C[9 * i + c * 3 + r];
+  zro_spline sp;
+  spline_of(s, Xp, &sp);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        double gw[3];
+        double wt = spline_node(s, &sp, a, b, c, gw);
+        int nc[3] = { sp.base[0] + a, sp.base[1] + b, sp.base[2] + c };
+        zro_node* g = node_at(s, nc[0], nc[1], nc[2], NULL);
+        double dxp[3];
+        for (int d = 0; d < 3; ++d) dxp[d] = (double)nc[d] * s->dx - Xp[d];
+        g->m += mass * wt;
+        for (int r = 0; r < 3; ++r)
+          g->v[r] += (Cm[r][0] * dxp[0] + Cm[r][1] * dxp[1] + Cm[r][2] * dxp[2] + mom[r]) * wt;
+      }
+}
+
+int64_t zro_p2g_apic(zro_sim* s)
+{
+  for (int color = 0; color < 8; ++color) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(s->nthreads)
+    for (int64_t g = 0; g < s->ngroups; ++g) {
+      if ((int)(s->block_offset[g] & 7) != color) continue;
+      for (int64_t idx = s->gfirst[g]; idx <= s->glast[g]; ++idx) p2g_apic_particle(s, s->order[idx]);
+    }
+  }
+  number_and_normalise(s);
+  return s->num_nodes;
+}
+
+/* multiObjectCollision with its normal basis (CollisionObject.cpp:66-105);
+   returns the projection P = I - K K^T of CollisionNode (CollisionObject.h:17-40) */
+static int collision_projection(const zro_sim* s, double t, const double* x, double* v, double P[3][3])
+{
+  int any = 0, nslip = 0, sticky = 0;
+  double K[3][3] = { { 0 } };
+  for (int k = 0; k < s->ncol; ++k) {
+    double b[3];
+    collider_b(s, &s->col[k], t, b);
+    int hit = detect_and_resolve(&s->col[k], b, x, v);
+    any = hit || any;
+    if (!hit) continue;
+    if (s->col[k].type == 1) { sticky = 1; break; }
+    double X[3], phi, n[3];
+    for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
+    ls_query_inside(&s->col[k], X, &phi, n);
+    for (int c2 = 0; c2 < nslip; ++c2) {
+      double dot = K[c2][0] * n[0] + K[c2][1] * n[1] + K[c2][2] * n[2];
+      for (int d = 0; d < 3; ++d) n[d] -= dot * K[c2][d];
+    }
+    double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l != 0.0) {
+      for (int d = 0; d < 3; ++d) K[nslip][d] = n[d] / l;
+      if (++nslip == 3) break;
+    }
+  }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double kk = 0.0;
+      for (int q = 0; q < nslip; ++q) kk += K[q][r] * K[q][c];
+      P[r][c] = sticky ? 0.0 : ((r == c ? 1.0 : 0.0) - kk);
+    }
+  return any;
+}
+
+static void project_vec(const zro_sim* s, double* v)
+{
+  int64_t nn = s->npages * 32;
+  for (int64_t k = 0; k < nn; ++k) {
+    if (!s->collided[k]) continue;
+    const double* P = s->proj + 9 * k;
+    double a[3] = { v[3 * k], v[3 * k + 1], v[3 * k + 2] };
+    for (int r = 0; r < 3; ++r) v[3 * k + r] = P[3 * r] * a[0] + P[3 * r + 1] * a[1] + P[3 * r + 2] * a[2];
+  }
+}
+
+static int node_active(const zro_sim* s, int64_t k) { return s->nodes[k].idx >= 0; }
+
+static double dot_active(const zro_sim* s, const double* a, const double* b)
+{
+  double acc = 0.0;
+  int64_t nn = s->npages * 32;
+  for (int64_t k = 0; k < nn; ++k)
+    if (node_active(s, k)) acc += a[3 * k] * b[3 * k] + a[3 * k + 1] * b[3 * k + 1] + a[3 * k + 2] * b[3 * k + 2];
+  return acc;
+}
+
+/* computeNorm (BackwardEuler.h:127-140) */
+static double mass_norm(const zro_sim* s, const double* r)
+{
+  double acc = 0.0;
+  int64_t nn = s->npages * 32;
+  for (int64_t k = 0; k < nn; ++k)
+    if (node_active(s, k))
+      acc += (r[3 * k] * r[3 * k] + r[3 * k + 1] * r[3 * k + 1] + r[3 * k + 2] * r[3 * k + 2]) / s->nodes[k].m;
+  return sqrt(acc);
+}
+
+/* grad f at particle i for a node field f: sum f_i grad w_ip^T (MpmForceBase.cpp:212-244) */
+static void particle_gradient(zro_sim* s, int i, const double* f, double G[3][3])
+{
+  zro_spline sp;
+  spline_of(s, s->X + 3 * i, &sp);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) G[r][c] = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        double gw[3];
+        spline_node(s, &sp, a, b, c, gw);
+        int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+        for (int r = 0; r < 3; ++r)
+          for (int q = 0; q < 3; ++q) G[r][q] += f[3 * k + r] * gw[q];
+      }
+}
+
+/* out_i -= scale * S grad w_ip  (rasterizeForceToTVStack<false>, MpmForceBase.cpp:100-152) */
+static void particle_scatter(zro_sim* s, int i, double S[3][3], double scale, double* out)
+{
+  zro_spline sp;
+  spline_of(s, s->X + 3 * i, &sp);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        double gw[3];
+        spline_node(s, &sp, a, b, c, gw);
+        int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+        for (int r = 0; r < 3; ++r) out[3 * k + r] -= scale * (S[r][0] * gw[0] + S[r][1] * gw[1] + S[r][2] * gw[2]);
+      }
+}
+
+static void mat_from_cm(const double* a, double m[3][3])
+{
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) m[r][c] = a[c * 3 + r];
+}
+static void mat_to_cm(double m[3][3], double* a)
+{
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) a[c * 3 + r] = m[r][c];
+}
+
+/* updateState: F = (I + dt grad(vn + dv)) Fn, vPFnT = V0 P(F) Fn^T
+   (MpmForceBase.cpp:305-344, FBasedMpmForceHelper.cpp:68-91) */
+static void implicit_update_state(zro_sim* s, const double* dv)
+{
+  int64_t nn = s->npages * 32;
+  double* vel = (double*)malloc(sizeof(double) * 3 * (size_t)nn);
+  for (int64_t k = 0; k < 3 * nn; ++k) vel[k] = s->vn[k] + dv[k];
+  double dt = s->be_dt;
+#pragma omp parallel for num_threads(s->nthreads)
+  for (int64_t i = 0; i < s->n; ++i) {
+    double G[3][3], Fn[3][3], F[3][3], P[3][3], vp[3][3];
+    particle_gradient(s, (int)i, vel, G);
+    mat_from_cm(s->Fn + 9 * i, Fn);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double acc = Fn[r][c];
+        for (int k = 0; k < 3; ++k) acc += dt * G[r][k] * Fn[k][c];
+        F[r][c] = acc;
+      }
+    mat_to_cm(F, s->F + 9 * i);
+    double Pcm[9];
+    zro_fcr_first_piola(s->mu[i], s->lam[i], s->F + 9 * i, Pcm);
+    mat_from_cm(Pcm, P);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; ++k) acc += P[r][k] * Fn[c][k];
+        vp[r][c] = s->vol[i] * acc;
+      }
+    mat_to_cm(vp, s->vtau + 9 * i);
+  }
+  free(vel);
+}
+
+static void scatter_colored(zro_sim* s, const double* stress_cm, double scale, double* out)
+{
+  for (int color = 0; color < 8; ++color) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(s->nthreads)
+    for (int64_t g = 0; g < s->ngroups; ++g) {
+      if ((int)(s->block_offset[g] & 7) != color) continue;
+      for (int64_t idx = s->gfirst[g]; idx <= s->glast[g]; ++idx) {
+        int i = s->order[idx];
+        double S[3][3];
+        mat_from_cm(stress_cm + 9 * (int64_t)i, S);
+        particle_scatter(s, i, S, scale, out);
+      }
+    }
+  }
+}
+
+/* computeResidual (BackwardEuler.h:101-124) with MassLumpedInertia
+   (Inertia.cpp:33-43) at the state left by implicit_update_state */
+static void implicit_residual_at_state(zro_sim* s, const double* dv, double* r)
+{
+  int64_t nn = s->npages * 32;
+  double dt = s->be_dt;
+  for (int64_t k = 0; k < nn; ++k)
+    for (int d = 0; d < 3; ++d) r[3 * k + d] = node_active(s, k) ? dt * s->gravity[d] * s->nodes[k].m : 0.0;
+  scatter_colored(s, s->vtau, dt, r);
+  for (int64_t k = 0; k < nn; ++k)
+    for (int d = 0; d < 3; ++d) {
+      if (node_active(s, k)) r[3 * k + d] -= s->nodes[k].m * dv[3 * k + d];
+      else r[3 * k + d] = 0.0;
+    }
+  project_vec(s, r);
+}
+
+void zro_implicit_residual(zro_sim* s, const double* dv, double* r)
+{
+  implicit_update_state(s, dv);
+  implicit_residual_at_state(s, dv, r);
+}
+
+/* multiply (BackwardEuler.h:322-351): M x + dt^2 sum V0 dP(grad x Fn) Fn^T grad w */
+void zro_implicit_multiply(zro_sim* s, const double* x, double* b)
+{
+  int64_t nn = s->npages * 32;
+  double dt = s->be_dt;
+  double* ds = (double*)malloc(sizeof(double) * 9 * (size_t)(s->n > 0 ? s->n : 1));
+#pragma omp parallel for num_threads(s->nthreads)
+  for (int64_t i = 0; i < s->n; ++i) {
+    double G[3][3], Fn[3][3], dF[3][3], dP[3][3], out[3][3], dFcm[9], dPcm[9];
+    particle_gradient(s, (int)i, x, G);
+    mat_from_cm(s->Fn + 9 * i, Fn);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; ++k) acc += G[r][k] * Fn[k][c];
+        dF[r][c] = acc;
+      }
+    mat_to_cm(dF, dFcm);
+    zro_fcr_first_piola_differential(s->mu[i], s->lam[i], s->F + 9 * i, dFcm, dPcm);
+    mat_from_cm(dPcm, dP);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; ++k) acc += dP[r][k] * Fn[c][k];
+        out[r][c] = acc * s->vol[i];
+      }
+    mat_to_cm(out, ds + 9 * i);
+  }
+  for (int64_t k = 0; k < nn; ++k)
+    for (int d = 0; d < 3; ++d) b[3 * k + d] = node_active(s, k) ? s->nodes[k].m * x[3 * k + d] : 0.0;
+  scatter_colored(s, ds, -dt * dt, b);
+  free(ds);
+}
+
+/* ConjugateGradient::solve (ConjugateGradient.h:99-147), mass-lumped
+   preconditioner out = in / m (MpmSimulationBase.cpp:617-623) */
+static int cg_solve(zro_sim* s, double* x, const double* b, double rel_tol, const zro_implicit_opts* o)
+{
+  int64_t nn = s->npages * 32, n3 = 3 * nn;
+  double* r = (double*)calloc((size_t)n3, sizeof(double));
+  double* q = (double*)calloc((size_t)n3, sizeof(double));
+  double* p = (double*)calloc((size_t)n3, sizeof(double));
+  double* t = (double*)calloc((size_t)n3, sizeof(double));
+  for (int64_t k = 0; k < n3; ++k) x[k] = 0.0;
+  for (int64_t k = 0; k < n3; ++k) r[k] = b[k];
+  project_vec(s, r);
+#define PRECOND(dst, src)                                                               \
+  for (int64_t k = 0; k < nn; ++k)                                                      \
+    for (int d = 0; d < 3; ++d)                                                         \
+      dst[3 * k + d] = node_active(s, k) ? src[3 * k + d] / s->nodes[k].m : 0.0;
+  PRECOND(q, r);
+  for (int64_t k = 0; k < n3; ++k) p[k] = q[k];
+  double zTr = dot_active(s, r, q);
+  double rnorm = sqrt(zTr);
+  double local_tol = rel_tol * rnorm < o->cg_tolerance ? rel_tol * rnorm : o->cg_tolerance;
+  int cnt;
+  for (cnt = 0; cnt < o->cg_max_iterations; ++cnt) {
+    if (rnorm < local_tol) break;
+    zro_implicit_multiply(s, p, t);
+    project_vec(s, t);
+    double alpha = zTr / dot_active(s, t, p);
+    for (int64_t k = 0; k < n3; ++k) {
+      x[k] += p[k] * alpha;
+      r[k] -= t[k] * alpha;
+    }
+    PRECOND(q, r);
+    double last = zTr;
+    zTr = dot_active(s, q, r);
+    double beta = zTr / last;
+    for (int64_t k = 0; k < n3; ++k) p[k] = q[k] + beta * p[k];
+    rnorm = sqrt(zTr);
+  }
+#undef PRECOND
+  free(r); free(q); free(p); free(t);
+  return cnt;
+}
+
+int zro_backward_euler(zro_sim* s, double dt, const zro_implicit_opts* o, zro_implicit_stats* st)
+{
+  int64_t nn = s->npages * 32;
+  s->be_dt = dt;
+  free(s->vn); free(s->dv); free(s->proj); free(s->collided);
+  s->vn = (double*)calloc((size_t)(3 * nn + 1), sizeof(double));
+  s->dv = (double*)calloc((size_t)(3 * nn + 1), sizeof(double));
+  s->proj = (double*)calloc((size_t)(9 * nn + 1), sizeof(double));
+  s->collided = (char*)calloc((size_t)(nn + 1), 1);
+  if (!s->Fn) s->Fn = (double*)malloc(sizeof(double) * 9 * (size_t)(s->n > 0 ? s->n : 1));
+  /* buildInitialDvAndVnForNewton (MpmSimulationBase.cpp:1105-1136) */
+  for (int64_t p = 0; p < s->npages; ++p) {
+    int base[3];
+    zro_linear_to_coord(s->pages[p], base);
+    for (int e = 0; e < 32; ++e) {
+      int64_t k = p * 32 + e;
+      if (!node_active(s, k)) continue;
+      double xi[3] = { (double)(base[0] + ((e >> 4) & 1)) * s->dx, (double)(base[1] + ((e >> 2) & 3)) * s->dx,
+                       (double)(base[2] + (e & 3)) * s->dx };
+      double vi[3] = { s->nodes[k].v[0], s->nodes[k].v[1], s->nodes[k].v[2] };
+      double P[3][3];
+      int hit = collision_projection(s, s->time, xi, vi, P);
+      s->collided[k] = (char)hit;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) s->proj[9 * k + 3 * r + c] = P[r][c];
+      for (int d = 0; d < 3; ++d) {
+        s->dv[3 * k + d] = hit ? vi[d] - s->nodes[k].v[d] : s->gravity[d] * dt;
+        s->vn[3 * k + d] = s->nodes[k].v[d];
+      }
+    }
+  }
+  memcpy(s->Fn, s->F, sizeof(double) * 9 * (size_t)s->n); /* backupStrain */
+  zro_implicit_stats local;
+  memset(&local, 0, sizeof(local));
+  double* r = (double*)calloc((size_t)(3 * nn + 1), sizeof(double));
+  double* step = (double*)calloc((size_t)(3 * nn + 1), sizeof(double));
+  /* NewtonsMethod::solve (NewtonsMethod.h:42-66) */
+  for (int it = 0; it < o->newton_max_iterations; ++it) {
+    implicit_update_state(s, s->dv);
+    implicit_residual_at_state(s, s->dv, r);
+    double norm = mass_norm(s, r);
+    if (local.num_residuals < 8) local.residual_norm[local.num_residuals++] = norm;
+    if (norm < o->newton_tolerance) { local.converged = 1; break; }
+    double rel = o->cg_relative_tolerance > 0 ? o->cg_relative_tolerance
+               : fmin(0.5, sqrt(fmax(norm, o->newton_tolerance)));
+    local.cg_iterations += cg_solve(s, step, r, rel, o);
+    for (int64_t k = 0; k < 3 * nn; ++k) s->dv[k] += step[k];
+    local.newton_iterations++;
+  }
+  free(r); free(step);
+  memcpy(s->F, s->Fn, sizeof(double) * 9 * (size_t)s->n); /* restoreStrain */
+  /* constructNewVelocityFromNewtonResult (MpmSimulationBase.cpp:867-877) */
+  for (int64_t k = 0; k < nn; ++k)
+    for (int d = 0; d < 3; ++d)
+      s->nodes[k].new_v[d] = node_active(s, k) ? s->nodes[k].v[d] + s->dv[3 * k + d] : 0.0;
+  if (st) *st = local;
+  return local.converged;
+}
+
+/* gridToParticlesHelper<APIC_blend_RPIC, deg 2, !MLS> (MpmSimulationBase.cpp:941-956)
+   + evolveStrain: gradV = sum v* grad w^T, C = B D^-1 */
+int zro_g2p_apic(zro_sim* s, double dt)
+{
+  int flags = 0;
+  double dx2 = s->dx * s->dx;
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : flags) num_threads(s->nthreads)
+  for (int64_t g = 0; g < s->ngroups; ++g) {
+    for (int64_t idx = s->gfirst[g]; idx <= s->glast[g]; ++idx) {
+      int i = s->order[idx];
+      double* Xp = s->X + 3 * i;
+      zro_spline sp;
+      spline_of(s, Xp, &sp);
+      double picV[3] = { 0, 0, 0 }, B[3][3] = { { 0 } }, G[3][3] = { { 0 } };
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int c = 0; c < 3; ++c) {
+            double gw[3];
+            double wt = spline_node(s, &sp, a, b, c, gw);
+            int nc[3] = { sp.base[0] + a, sp.base[1] + b, sp.base[2] + c };
+            zro_node* gn = node_at(s, nc[0], nc[1], nc[2], NULL);
+            double dxp[3];
+            for (int d = 0; d < 3; ++d) dxp[d] = (double)nc[d] * s->dx - Xp[d];
+            for (int r = 0; r < 3; ++r) {
+              picV[r] += wt * gn->new_v[r];
+              for (int cc = 0; cc < 3; ++cc) {
+                B[r][cc] += wt * gn->new_v[r] * dxp[cc];
+                G[r][cc] += gn->new_v[r] * gw[cc];
+              }
+            }
+          }
+      for (int d = 0; d < 3; ++d) s->V[3 * i + d] = picV[d];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          s->C[9 * i + c * 3 + r] = B[r][c] * s->D_inverse;
+          s->gradV[9 * i + c * 3 + r] = G[r][c];
+        }
+      double inc2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        double inc = dt * picV[d];
+        Xp[d] += inc;
+        inc2 += inc * inc;
+      }
+      if (inc2 > dx2) flags |= 1;
+      if (inc2 > dx2 * 0.25 * (s->cfl * s->cfl)) flags |= 2;
+      double Fo[3][3], Fnew[3][3];
+      mat_from_cm(s->F + 9 * (int64_t)i, Fo);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double acc = Fo[r][c];
+          for (int k = 0; k < 3; ++k) acc += dt * G[r][k] * Fo[k][c];
+          Fnew[r][c] = acc;
+        }
+      mat_to_cm(Fnew, s->F + 9 * (int64_t)i);
+    }
+  }
+  return flags;
+}
+
+int zro_step_implicit(zro_sim* s, double dt, const zro_implicit_opts* o, zro_implicit_stats* st)
+{
+  zro_sort_and_pollute(s);
+  zro_p2g_apic(s);
+  zro_backward_euler(s, dt, o, st);
+  int flags = zro_g2p_apic(s, dt);
+  s->time += dt;
+  return flags;
+}
+
+/* total energy of the backward-Euler objective at dv (BackwardEuler.h:183-197,
+   Inertia.cpp:13-31): sum_i m |dv_i|^2 / 2 - dt sum_i m g.dv_i + sum_p V0 psi(F_p(dv)).
+   Its negative gradient is the residual (before projection). */
+double zro_implicit_energy(zro_sim* s, const double* dv)
+{
+  implicit_update_state(s, dv);
+  double e = 0.0;
+  int64_t nn = s->npages * 32;
+  for (int64_t k = 0; k < nn; ++k) {
+    if (!node_active(s, k)) continue;
+    double m = s->nodes[k].m;
+    for (int d = 0; d < 3; ++d) e += 0.5 * m * dv[3 * k + d] * dv[3 * k + d] - s->be_dt * m * s->gravity[d] * dv[3 * k + d];
+  }
+  double pe = 0.0;
+#pragma omp parallel for reduction(+ : pe) num_threads(s->nthreads)
+  for (int64_t i = 0; i < s->n; ++i) pe += s->vol[i] * zro_fcr_psi(s->mu[i], s->lam[i], s->F + 9 * i);
+  return e + pe;
+}
+
+int64_t zro_num_node_slots(const zro_sim* s) { return s->npages * 32; }
+void zro_get_implicit(const zro_sim* s, double* dv, double* vn, char* collided)
+{
+  int64_t nn = s->npages * 32;
+  if (dv && s->dv) memcpy(dv, s->dv, sizeof(double) * 3 * (size_t)nn);
+  if (vn && s->vn) memcpy(vn, s->vn, sizeof(double) * 3 * (size_t)nn);
+  if (collided && s->collided) memcpy(collided, s->collided, (size_t)nn);
 }

@@ -157,23 +157,42 @@ __global__ void k_reset_tables(const uint32_t* group_page, uint32_t ngroups, con
 __global__ void k_bin(const double* __restrict__ x, int64_t stride, uint32_t n, double inv_dx, int lim,
                       uint32_t* page_count, uint32_t* page_bits, uint32_t* p_page, uint32_t* p_re, uint32_t* counters)
 {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int b[3];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  const int lane = threadIdx.x & 63;
+  int b[3] = { 0, 0, 0 };
   bool bad = false;
+  if (valid) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    b[d] = int_floor(x[d * stride + i] * inv_dx - 0.5);
-    if (b[d] < 0 || b[d] > lim) {
-      bad = true;
-      b[d] = b[d] < 0 ? 0 : lim;
+    for (int d = 0; d < 3; ++d) {
+      b[d] = int_floor(x[d * stride + i] * inv_dx - 0.5);
+      if (b[d] < 0 || b[d] > lim) {
+        bad = true;
+        b[d] = b[d] < 0 ? 0 : lim;
+      }
     }
   }
   if (bad) atomicOr(&counters[CNT_DOMAIN], 1u);
-  uint32_t pc = page_encode((uint32_t)b[0] >> 1, (uint32_t)b[1] >> 2, (uint32_t)b[2] >> 2);
-  uint32_t e = elem_of(b[0], b[1], b[2]);
-  uint32_t r = atomicAdd(&page_count[pc], 1u);
-  if (r == 0u) atomicOr(&page_bits[pc >> 5], 1u << (pc & 31u));
+  const uint32_t pc = page_encode((uint32_t)b[0] >> 1, (uint32_t)b[1] >> 2, (uint32_t)b[2] >> 2);
+  const uint32_t e = elem_of(b[0], b[1], b[2]);
+  // wave-aggregated counting: one atomic per distinct page in the wavefront;
+  // lanes of a page get consecutive ranks in lane order
+  uint64_t active = __ballot(valid);
+  uint32_t r = 0;
+  while (active) {
+    const int leader = __ffsll((long long)active) - 1;
+    const uint32_t lpc = __shfl(pc, leader, 64);
+    const uint64_t m = __ballot(pc == lpc) & active;
+    uint32_t base = 0;
+    if (lane == leader) {
+      base = atomicAdd(&page_count[lpc], (uint32_t)__popcll(m));
+      if (base == 0u) atomicOr(&page_bits[lpc >> 5], 1u << (lpc & 31u));
+    }
+    base = __shfl(base, leader, 64);
+    if ((m >> lane) & 1ull) r = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    active &= ~m;
+  }
+  if (!valid) return;
   p_page[i] = pc;
   p_re[i] = (r << 5) | e;
 }
