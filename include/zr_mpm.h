@@ -88,6 +88,30 @@ typedef struct zr_step_stats {
   int32_t reserved;
 } zr_step_stats;
 
+/* Options of the implicit backward-Euler step.  Reference defaults:
+   NewtonsMethod(tol 1e-5, 3 iterations) (MpmSimulationBase.cpp:51),
+   ConjugateGradient(20) (BackwardEuler.h:90-91) with LinearSolver::tolerance
+   16 eps (LinearSolver.h:26) and the Newton-adaptive relative tolerance
+   min(0.5, sqrt(max(|r|, tol))) (NewtonsMethod.h:61). */
+typedef struct zr_implicit_opts {
+  int32_t newton_max_iterations;
+  int32_t cg_max_iterations;
+  double newton_tolerance;
+  double cg_tolerance;           /* absolute cap of the CG local tolerance */
+  double cg_relative_tolerance;  /* <= 0: Newton-adaptive (reference behaviour) */
+  int32_t precision;             /* ZR_FP64 */
+  int32_t cg_check_interval;     /* CG iterations between host polls of the
+                                    device-side termination flag (0: 8) */
+} zr_implicit_opts;
+
+typedef struct zr_implicit_stats {
+  int32_t newton_iterations;     /* Newton iterations that computed a step */
+  int32_t cg_iterations;         /* total CG iterations of the step */
+  double residual_norm[8];       /* mass-weighted residual norm per Newton iteration */
+  int32_t num_residuals;
+  int32_t converged;             /* residual fell below newton_tolerance */
+} zr_implicit_stats;
+
 typedef struct zr_engine zr_engine;
 
 /* MpmSimulationBase constructor + setDx (MpmSimulationBase.cpp:33-72) */
@@ -122,6 +146,28 @@ int zr_grid_to_particles(zr_engine* e, double dt, zr_step_stats* st);
    (MpmSimulationBase.cpp:256-281); advances the engine time by dt */
 int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st);
 
+/* particlesToGrid of the implicit path: APIC mass and momentum only
+   (particlesToGridHelper<APIC>, MpmSimulationBase.cpp:563-608) + getNumNodes +
+   normalisation (MpmSimulationBase.cpp:474-485) */
+int zr_particles_to_grid_apic(zr_engine* e);
+/* MpmSimulationBase::backwardEulerStep (MpmSimulationBase.cpp:610-657):
+   buildInitialDvAndVnForNewton, Newton with matrix-free preconditioned CG on
+   BackwardEulerLagrangianForceObjective (BackwardEuler.h:101-351),
+   constructNewVelocityFromNewtonResult */
+int zr_backward_euler_step(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* st);
+/* gridToParticles with APIC (grad v = sum v grad w^T) + evolveStrain
+   (MpmSimulationBase.cpp:906-1016 with !mls_mpm) */
+int zr_grid_to_particles_apic(zr_engine* e, double dt, zr_step_stats* st);
+/* advanceOneTimeStep, implicit branch (MpmSimulationBase.cpp:282-295) */
+int zr_advance_one_time_step_implicit(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* ist,
+                                      zr_step_stats* st);
+/* At the state of the last backward-Euler solve, on host node vectors in
+   page-list x element order (3 doubles per node, 32 nodes per page):
+   residual at dv (BackwardEuler.h:101-124; also moves the Newton state to dv)
+   and the unprojected operator b = A x (BackwardEuler.h:322-351). */
+int zr_implicit_residual(zr_engine* e, const double* dv, double* r);
+int zr_implicit_multiply(zr_engine* e, const double* x, double* b);
+
 /* Introspection for parity tests (reference state after the named call):
    sizes in bytes via zr_debug_size, contents via zr_debug_dump. */
 enum {
@@ -136,7 +182,8 @@ enum {
   ZR_DUMP_GRID_V = 8,       /* f64[3*32P] normalised node velocity */
   ZR_DUMP_GRID_F = 9,       /* f64[3*32P] rasterised force (new_v after P2G) */
   ZR_DUMP_GRID_VSTAR = 10,  /* f64[3*32P] new_v after the grid update */
-  ZR_DUMP_NODE_IDX = 11     /* i64[32P] GridState::idx (getNumNodes numbering) */
+  ZR_DUMP_NODE_IDX = 11,    /* i64[32P] GridState::idx (getNumNodes numbering) */
+  ZR_DUMP_DV = 12           /* f64[3*32P] Newton result dv of the last backward-Euler step */
 };
 int64_t zr_debug_size(zr_engine* e, int32_t what);
 int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes);
@@ -145,7 +192,15 @@ int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes);
    launch.  Instrumentation for measurement; it stands in for the reference's
    ZIRAN_TIMER scoped timers (Lib/Ziran/CS/Util/Timer.cpp:34-95), which time
    host code.  Enabling resets the accumulated times. */
-enum { ZR_PROF_P2G = 0, ZR_PROF_GRID = 1, ZR_PROF_G2P = 2, ZR_PROF_SORT = 3, ZR_PROF_COUNT = 4 };
+enum {
+  ZR_PROF_P2G = 0,
+  ZR_PROF_GRID = 1,
+  ZR_PROF_G2P = 2,
+  ZR_PROF_SORT = 3,
+  ZR_PROF_STATE = 4, /* Newton state: G2P gather + SVD + stress scatter */
+  ZR_PROF_HVP = 5,   /* operator apply, particle part */
+  ZR_PROF_COUNT = 6
+};
 int zr_profile_enable(zr_engine* e, int32_t on);
 int zr_profile_read(zr_engine* e, int32_t which, double* total_ms, int64_t* launches);
 

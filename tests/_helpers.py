@@ -1,0 +1,38 @@
+"""Shared fixtures of the GPU parity tests: seeded scenes and an engine/oracle
+pair holding the same particles and colliders."""
+import numpy as np
+
+import oracle
+from ziran2019_amd import scenes
+from ziran2019_amd.engine import MpmEngine
+
+
+def make_pair(s, gravity=(0.0, -9.8, 0.0), colliders=(), nthreads=8):
+    eng = MpmEngine(dx=s["dx"], gravity=gravity)
+    eng.upload(s)
+    eng.set_colliders(list(colliders))
+    ora = oracle.OracleSim(dx=s["dx"], gravity=gravity, nthreads=nthreads)
+    ora.set_particles(s["X"], s["V"], s["C"], s["F"], s["mass"], s["vol"], s["mu"], s["lam"])
+    for c in colliders:
+        ora.add_collider(**c)
+    return eng, ora
+
+
+def perturbed(seed, n_cells=6, shuffle=True, F_noise=0.05, C_noise=0.5, E=1e4):
+    s = scenes.cube_scene_small(seed=seed, ncells=n_cells, E=E)
+    rng = np.random.default_rng(seed)
+    n = len(s["mass"])
+    F = np.tile(np.eye(3), (n, 1, 1)) + F_noise * rng.normal(size=(n, 3, 3))
+    s["F"] = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(n, 9))
+    s["C"] = C_noise * rng.normal(size=(n, 9))
+    s["mu"] = s["mu"] * np.exp(0.3 * rng.normal(size=n))
+    if shuffle:
+        p = rng.permutation(n)
+        for k in ("X", "V", "C", "F", "mass", "vol", "mu", "lam"):
+            s[k] = np.ascontiguousarray(np.asarray(s[k])[p])
+    return s
+
+
+def rel_err(a, b):
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / scale

@@ -314,22 +314,27 @@ struct ParticlesView {
   int64_t stride;
 };
 
-__global__ __launch_bounds__(256) void k_p2g_mls(ParticlesView P, const uint32_t* __restrict__ order,
-                                                 const uint32_t* __restrict__ group_start,
-                                                 const uint32_t* __restrict__ group_page, double* __restrict__ stage,
-                                                 double dx, double inv_dx, double Dinv)
+// FORCE = true: MLS with force (explicit).  FORCE = false: APIC mass and
+// momentum only, particlesToGridHelper<APIC> (MpmSimulationBase.cpp:563-608).
+template <bool FORCE>
+__global__ __launch_bounds__(256) void k_p2g(ParticlesView P, const uint32_t* __restrict__ order,
+                                             const uint32_t* __restrict__ group_start,
+                                             const uint32_t* __restrict__ group_page, double* __restrict__ stage,
+                                             double dx, double inv_dx, double Dinv)
 {
-  __shared__ double tile[4 * kStageStride];
+  constexpr int NF = FORCE ? kP2GFields : 4;
+  constexpr int SS = NF * kTileNodes;
+  __shared__ double tile[4 * SS];
   const int tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
-  for (int k = tid; k < 4 * kStageStride; k += 256) tile[k] = 0.0;
+  for (int k = tid; k < 4 * SS; k += 256) tile[k] = 0.0;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
   uint32_t px, py, pz;
   page_decode(group_page[g], px, py, pz);
   const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
   __syncthreads();
-  double* wt = tile + (tid >> 6) * kStageStride;
+  double* wt = tile + (tid >> 6) * SS;
   // lanes of a wave take particles 8 apart so that they mostly sit in
   // different cells and their LDS adds rarely collide
   const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
@@ -341,21 +346,24 @@ __global__ __launch_bounds__(256) void k_p2g_mls(ParticlesView P, const uint32_t
     double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
     const double mass = P.m[i];
     double mom[3] = { mass * P.v[i], mass * P.v[S + i], mass * P.v[2 * S + i] };
-    M3<double> f, tau;
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
-    fcr_kirchhoff(P.mu[i], P.lam[i], f, tau);
-    const double vol = P.vol[i];
     double mC[3][3], st[3][3];
+    if constexpr (FORCE) {
+      M3<double> f, tau;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
+      fcr_kirchhoff(P.mu[i], P.lam[i], f, tau);
+      const double vol = P.vol[i];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) st[r][c] = vol * tau.a[r][c];
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        mC[r][c] = mass * P.C[(c * 3 + r) * S + i];
-        st[r][c] = vol * tau.a[r][c];
-      }
+      for (int r = 0; r < 3; ++r) mC[r][c] = mass * P.C[(c * 3 + r) * S + i];
     double w[3][3];
     int b[3];
 #pragma unroll
@@ -379,18 +387,19 @@ __global__ __launch_bounds__(256) void k_p2g_mls(ParticlesView P, const uint32_t
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
             double vd = mC[r][0] * dx0 + mC[r][1] * dx1 + mC[r][2] * dx2 + mom[r];
-            double fd = st[r][0] * dx0 + st[r][1] * dx1 + st[r][2] * dx2;
             atomicAdd(&wt[(1 + r) * kTileNodes + t], vd * wt_);
-            atomicAdd(&wt[(4 + r) * kTileNodes + t], -(fd * fw));
+            if constexpr (FORCE) {
+              double fd = st[r][0] * dx0 + st[r][1] * dx1 + st[r][2] * dx2;
+              atomicAdd(&wt[(4 + r) * kTileNodes + t], -(fd * fw));
+            }
           }
         }
       }
     }
   }
   __syncthreads();
-  double* out = stage + (size_t)g * kStageStride;
-  for (int k = tid; k < kStageStride; k += 256)
-    out[k] = tile[k] + tile[kStageStride + k] + tile[2 * kStageStride + k] + tile[3 * kStageStride + k];
+  double* out = stage + (size_t)g * SS;
+  for (int k = tid; k < SS; k += 256) out[k] = tile[k] + tile[SS + k] + tile[2 * SS + k] + tile[3 * SS + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -407,6 +416,7 @@ struct GridView {
 
 enum { GRID_COMBINE = 0, GRID_COMBINE_UPDATE = 1, GRID_UPDATE = 2 };
 
+template <int NF>
 __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const uint32_t* __restrict__ slot_page,
                                               const int32_t* __restrict__ page_group,
                                               const double* __restrict__ stage, GridView G, double dx, double dt,
@@ -424,6 +434,7 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
     double acc[kP2GFields];
 #pragma unroll
     for (int q = 0; q < kP2GFields; ++q) acc[q] = 0.0;
+    constexpr int SS = NF * kTileNodes;
     for (int a = 0; a < 2; ++a) {
       if (a > (int)qx) break;
       const int tx = ex + 2 * a;
@@ -435,9 +446,9 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
           if (tz >= kTileZ || c > (int)qz) break;
           const int32_t src = page_group[page_encode(qx - a, qy - b, qz - c)];
           if (src < 0) continue;
-          const double* base = stage + (size_t)src * kStageStride + (tx * kTileY + ty) * kTileZ + tz;
+          const double* base = stage + (size_t)src * SS + (tx * kTileY + ty) * kTileZ + tz;
 #pragma unroll
-          for (int q = 0; q < kP2GFields; ++q) acc[q] += base[q * kTileNodes];
+          for (int q = 0; q < NF; ++q) acc[q] += base[q * kTileNodes];
         }
       }
     }
@@ -473,11 +484,16 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
   for (int d = 0; d < 3; ++d) G.vs[d * S + k] = vi[d];
 }
 
+#include "zr_implicit.hpp"
+
 // ---------------------------------------------------------------------------
 // G2P, MLS/APIC (MpmSimulationBase.cpp:906-1016) + evolveStrain
 // (FBasedMpmForceHelper.cpp:95-110)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_g2p_mls(ParticlesView P, const uint32_t* __restrict__ order,
+// MLS: grad v = C.  APIC (implicit path): grad v = sum v* grad w^T
+// (MpmSimulationBase.cpp:941-956 with !USE_MLS_MPM).
+template <bool MLS>
+__global__ __launch_bounds__(256) void k_g2p(ParticlesView P, const uint32_t* __restrict__ order,
                                                  const uint32_t* __restrict__ group_start,
                                                  const uint32_t* __restrict__ group_page,
                                                  const int32_t* __restrict__ page_slot, const double* __restrict__ gvs,
@@ -508,13 +524,13 @@ __global__ __launch_bounds__(256) void k_g2p_mls(ParticlesView P, const uint32_t
   for (uint32_t j = tid; j < n; j += 256) {
     const uint32_t i = order[start + j];
     double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
-    double w[3][3];
+    double w[3][3], dw[3][3];
     int b[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) b[d] = bspline_w(xp[d] * inv_dx, w[d]);
+    for (int d = 0; d < 3; ++d) b[d] = bspline_wdw(xp[d] * inv_dx, w[d], dw[d]);
     const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
     if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
-    double pv[3] = { 0.0, 0.0, 0.0 }, B[3][3] = { { 0.0 } };
+    double pv[3] = { 0.0, 0.0, 0.0 }, B[3][3] = { { 0.0 } }, Gv[3][3] = { { 0.0 } };
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const double dx0 = (double)(b[0] + a) * dx - xp[0];
@@ -527,6 +543,8 @@ __global__ __launch_bounds__(256) void k_g2p_mls(ParticlesView P, const uint32_t
           const double wt_ = wab * w[2][c];
           const double dx2c = (double)(b[2] + c) * dx - xp[2];
           const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+          double gw[3] = { 0.0, 0.0, 0.0 };
+          if constexpr (!MLS) wgrad(w, dw, inv_dx, a, bb, c, gw);
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
             const double gv = tv[r * kTileNodes + t];
@@ -534,6 +552,11 @@ __global__ __launch_bounds__(256) void k_g2p_mls(ParticlesView P, const uint32_t
             B[r][0] += wt_ * gv * dx0;
             B[r][1] += wt_ * gv * dx1;
             B[r][2] += wt_ * gv * dx2c;
+            if constexpr (!MLS) {
+              Gv[r][0] += gv * gw[0];
+              Gv[r][1] += gv * gw[1];
+              Gv[r][2] += gv * gw[2];
+            }
           }
         }
       }
@@ -567,7 +590,7 @@ __global__ __launch_bounds__(256) void k_g2p_mls(ParticlesView P, const uint32_t
       for (int c = 0; c < 3; ++c) {
         double acc = Fo[r][c];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) acc += dt * Cn[r][k] * Fo[k][c];
+        for (int k = 0; k < 3; ++k) acc += dt * (MLS ? Cn[r][k] : Gv[r][k]) * Fo[k][c];
         P.F[(c * 3 + r) * S + i] = acc;
       }
   }
@@ -659,10 +682,27 @@ struct zr_engine {
   uint32_t* dcount = nullptr;
   uint32_t* hcount = nullptr;
 
+  // implicit backward Euler: node vectors [3][grid_cap], reduction partials,
+  // device scalars/flags (+ pinned mirrors), per-particle SVD cache
+  bool mls_rasterised = true;  // kind of the last P2G (MLS with force or APIC)
+  double *ivn = nullptr, *idv = nullptr, *ir = nullptr, *ix = nullptr, *irc = nullptr, *iq = nullptr,
+         *ip = nullptr, *iap = nullptr;
+  size_t imp_cap = 0;
+  double* partial = nullptr;
+  size_t partial_cap = 0;
+  double* sc = nullptr;
+  int32_t* ci = nullptr;
+  double* hsc = nullptr;
+  int32_t* hci = nullptr;
+  double *cU = nullptr, *cV = nullptr, *cS = nullptr;
+  int64_t cache_n = 0;
+  bool be_ready = false;
+  double be_dt = 0.0;
+
   // kernel timing: pairs of events per launch, per kernel class
   bool prof = false;
   std::vector<hipEvent_t> prof_ev[ZR_PROF_COUNT];
-  size_t prof_used[ZR_PROF_COUNT] = { 0, 0, 0, 0 };
+  size_t prof_used[ZR_PROF_COUNT] = { 0, 0, 0, 0, 0, 0 };
 };
 
 namespace {
@@ -840,16 +880,22 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   return ZR_OK;
 }
 
-int do_p2g(zr_engine* e)
+int do_p2g(zr_engine* e, bool mls)
 {
   if (!e->sorted) return fail(e, ZR_ERR_STATE, "particlesToGrid before sortParticlesAndPolluteGrid");
   const double dx = e->cfg.dx;
   prof_mark(e, ZR_PROF_P2G);
-  k_p2g_mls<<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx,
-                                         4.0 / (dx * dx));
+  if (mls)
+    k_p2g<true><<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
+                                             1.0 / dx, 4.0 / (dx * dx));
+  else
+    k_p2g<false><<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
+                                              1.0 / dx, 4.0 / (dx * dx));
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_P2G);
   e->rasterised = true;
+  e->mls_rasterised = mls;
+  e->be_ready = false;
   return ZR_OK;
 }
 
@@ -858,34 +904,206 @@ int do_grid(zr_engine* e, int mode, double dt)
   GridView G{ e->gm, e->gv, e->gf, e->gvs, (int64_t)e->grid_cap };
   ColliderSet cs = collider_set(e);
   prof_mark(e, ZR_PROF_GRID);
-  if (e->S)
-    k_grid<<<blocks_for((uint64_t)e->S * 32, 256), 256, 0, e->stream>>>(
-        mode, e->S, e->slot_page, e->page_group, e->stage, G, e->cfg.dx, dt, e->cfg.gravity[0], e->cfg.gravity[1],
-        e->cfg.gravity[2], e->time, cs);
+  if (e->S) {
+    const uint32_t nb = blocks_for((uint64_t)e->S * 32, 256);
+    if (e->mls_rasterised)
+      k_grid<kP2GFields><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->page_group, e->stage, G, e->cfg.dx,
+                                                    dt, e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2],
+                                                    e->time, cs);
+    else
+      k_grid<4><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->page_group, e->stage, G, e->cfg.dx, dt,
+                                           e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs);
+  }
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_GRID);
   if (mode != GRID_COMBINE) e->updated = true;
   return ZR_OK;
 }
 
-int do_g2p(zr_engine* e, double dt, zr_step_stats* st)
+int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
 {
   if (!e->updated) return fail(e, ZR_ERR_STATE, "gridToParticles before the grid update");
   const double dx = e->cfg.dx;
   ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_CFL, 0, sizeof(uint32_t), e->stream));
   prof_mark(e, ZR_PROF_G2P);
-  k_g2p_mls<<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot, e->gvs,
-                                         (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt, e->cfg.cfl,
-                                         e->dcount);
+  if (mls)
+    k_g2p<true><<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot, e->gvs,
+                                             (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt, e->cfg.cfl,
+                                             e->dcount);
+  else
+    k_g2p<false><<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot,
+                                              e->gvs, (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt,
+                                              e->cfg.cfl, e->dcount);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_G2P);
   int rc = sync_counts(e);
   if (rc) return rc;
   uint32_t flags = e->hcount[CNT_CFL];
   // particles moved: the sort is stale until the next one
-  e->sorted = e->rasterised = e->updated = false;
+  e->sorted = e->rasterised = e->updated = e->be_ready = false;
   if (st) st->cfl_flags = (int32_t)flags;
   return (flags & 1u) ? ZR_CFL_RESTART : ZR_OK;
+}
+
+// ---------------------------------------------------------------- implicit
+int ensure_implicit(zr_engine* e)
+{
+  if (e->imp_cap < e->grid_cap) {
+    double** arrs[] = { &e->ivn, &e->idv, &e->ir, &e->ix, &e->irc, &e->iq, &e->ip, &e->iap };
+    for (auto a : arrs) {
+      dfree(*a);
+      ZR_HIP(e, hipMalloc(a, 3 * e->grid_cap * sizeof(double)));
+    }
+    e->imp_cap = e->grid_cap;
+  }
+  size_t nb = (e->grid_cap + 255) / 256 + 1;
+  if (nb > e->partial_cap) {
+    dfree(e->partial);
+    ZR_HIP(e, hipMalloc(&e->partial, nb * sizeof(double)));
+    e->partial_cap = nb;
+  }
+  if (!e->sc) {
+    ZR_HIP(e, hipMalloc(&e->sc, SC_N * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->ci, CI_N * sizeof(int32_t)));
+    ZR_HIP(e, hipHostMalloc(&e->hsc, SC_N * sizeof(double)));
+    ZR_HIP(e, hipHostMalloc(&e->hci, CI_N * sizeof(int32_t)));
+  }
+  if (e->cache_n != e->n) {
+    dfree(e->cU); dfree(e->cV); dfree(e->cS);
+    ZR_HIP(e, hipMalloc(&e->cU, 9 * (size_t)e->n * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->cV, 9 * (size_t)e->n * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->cS, 3 * (size_t)e->n * sizeof(double)));
+    e->cache_n = e->n;
+  }
+  return ZR_OK;
+}
+
+NodeVec3 nv(zr_engine* e, double* p) { return NodeVec3{ p, (int64_t)e->grid_cap }; }
+
+int read_scalars(zr_engine* e)
+{
+  ZR_HIP(e, hipMemcpyAsync(e->hsc, e->sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+// Newton state at the current dv + residual + its squared mass norm (device)
+int be_state_residual(zr_engine* e)
+{
+  const double dx = e->cfg.dx, dt = e->be_dt;
+  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
+  ColliderSet cs = collider_set(e);
+  ImpCache cache{ e->cU, e->cV, e->cS };
+  prof_mark(e, ZR_PROF_STATE);
+  k_be_state<<<e->A, 256, 0, e->stream>>>(pview(e), cache, e->order, e->group_start, e->group_page, e->page_slot,
+                                          e->ivn, e->idv, (int64_t)e->grid_cap, dx, 1.0 / dx, dt, e->stage);
+  ZR_LAUNCH_CHECK(e);
+  prof_mark(e, ZR_PROF_STATE);
+  k_be_residual<<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->page_group, e->stage, e->gm, nv(e, e->idv),
+                                           nv(e, e->ir), dx, dt, e->cfg.gravity[0], e->cfg.gravity[1],
+                                           e->cfg.gravity[2], e->time, cs, e->partial);
+  k_reduce<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, SC_NORM2, nullptr);
+  ZR_LAUNCH_CHECK(e);
+  return ZR_OK;
+}
+
+// A p -> iap (projected when requested); the CG flags gate it when given
+int be_apply(zr_engine* e, const double* pvec, double* out, int project, const int32_t* ci, int red_slot)
+{
+  const double dx = e->cfg.dx, dt = e->be_dt;
+  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
+  ColliderSet cs = collider_set(e);
+  ImpCache cache{ e->cU, e->cV, e->cS };
+  prof_mark(e, ZR_PROF_HVP);
+  k_hvp<<<e->A, 256, 0, e->stream>>>(pview(e), cache, e->order, e->group_start, e->group_page, e->page_slot, pvec,
+                                     (int64_t)e->grid_cap, 1.0 / dx, e->stage, ci);
+  ZR_LAUNCH_CHECK(e);
+  prof_mark(e, ZR_PROF_HVP);
+  k_cg_ap<<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->page_group, e->stage, e->gm,
+                                     NodeVec3{ const_cast<double*>(pvec), (int64_t)e->grid_cap }, nv(e, out), dx, dt,
+                                     e->time, cs, project, e->partial, ci);
+  if (red_slot >= 0) k_reduce<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, red_slot, ci);
+  ZR_LAUNCH_CHECK(e);
+  return ZR_OK;
+}
+
+// ConjugateGradient::solve (ConjugateGradient.h:99-147) on A ix = ir with the
+// mass-lumped preconditioner; termination is decided on the device and the
+// host polls the flag every check_interval iterations
+int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
+{
+  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
+  hipStream_t s = e->stream;
+  k_cg_init<<<nb, 256, 0, s>>>(nn, e->gm, nv(e, e->ir), nv(e, e->ix), nv(e, e->irc), nv(e, e->iq), nv(e, e->ip),
+                               e->partial);
+  k_reduce<<<1, 256, 0, s>>>(e->partial, nb, e->sc, SC_ZTR, nullptr);
+  k_cg_setup<<<1, 64, 0, s>>>(e->sc, e->ci, rel, o->cg_tolerance);
+  ZR_LAUNCH_CHECK(e);
+  const int check = o->cg_check_interval > 0 ? o->cg_check_interval : 8;
+  for (int k = 0; k < o->cg_max_iterations; ++k) {
+    int rc = be_apply(e, e->ip, e->iap, 1, e->ci, SC_PAP);
+    if (rc) return rc;
+    k_cg_update<<<nb, 256, 0, s>>>(nn, e->gm, nv(e, e->ix), nv(e, e->irc), nv(e, e->iq), nv(e, e->ip),
+                                   nv(e, e->iap), e->sc, e->partial, e->ci);
+    k_reduce<<<1, 256, 0, s>>>(e->partial, nb, e->sc, SC_ZTR_NEW, e->ci);
+    k_cg_post<<<1, 64, 0, s>>>(e->sc, e->ci);
+    k_cg_p<<<nb, 256, 0, s>>>(nn, nv(e, e->iq), nv(e, e->ip), e->sc, e->ci);
+    ZR_LAUNCH_CHECK(e);
+    if ((k + 1) % check == 0 && k + 1 < o->cg_max_iterations) {
+      ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      ZR_HIP(e, hipStreamSynchronize(s));
+      if (e->hci[CI_DONE]) break;
+    }
+  }
+  return ZR_OK;
+}
+
+int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* st)
+{
+  if (!e->rasterised || e->mls_rasterised)
+    return fail(e, ZR_ERR_STATE, "backwardEulerStep needs the APIC particlesToGrid of this step");
+  int rc = ensure_implicit(e);
+  if (rc) return rc;
+  zr_implicit_stats local;
+  memset(&local, 0, sizeof(local));
+  e->be_dt = dt;
+  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
+  hipStream_t s = e->stream;
+  ColliderSet cs = collider_set(e);
+  k_be_init<<<nb, 256, 0, s>>>(nn, e->slot_page, e->gm, e->gv, nv(e, e->ivn), nv(e, e->idv), e->cfg.dx, dt,
+                               e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs);
+  ZR_HIP(e, hipMemsetAsync(e->ci, 0, CI_N * sizeof(int32_t), s));
+  ZR_LAUNCH_CHECK(e);
+  e->be_ready = true;
+  // NewtonsMethod::solve (NewtonsMethod.h:42-66)
+  for (int it = 0; it < o->newton_max_iterations; ++it) {
+    rc = be_state_residual(e);
+    if (rc) return rc;
+    rc = read_scalars(e);
+    if (rc) return rc;
+    const double norm = sqrt(e->hsc[SC_NORM2]);
+    if (local.num_residuals < 8) local.residual_norm[local.num_residuals++] = norm;
+    if (norm < o->newton_tolerance) {
+      local.converged = 1;
+      break;
+    }
+    const double rel = o->cg_relative_tolerance > 0.0 ? o->cg_relative_tolerance
+                                                      : fmin(0.5, sqrt(fmax(norm, o->newton_tolerance)));
+    rc = be_cg(e, rel, o);
+    if (rc) return rc;
+    k_axpy3<<<nb, 256, 0, s>>>(nn, nv(e, e->idv), nv(e, e->ix));
+    ZR_LAUNCH_CHECK(e);
+    local.newton_iterations++;
+  }
+  k_be_finish<<<nb, 256, 0, s>>>(nn, e->gm, e->gv, nv(e, e->idv), e->gvs);
+  ZR_LAUNCH_CHECK(e);
+  rc = read_scalars(e);
+  if (rc) return rc;
+  local.cg_iterations = e->hci[CI_ITERS];
+  e->updated = true;
+  if (st) *st = local;
+  return ZR_OK;
 }
 
 }  // namespace
@@ -943,6 +1161,10 @@ void zr_destroy(zr_engine* e)
   dfree(e->page_count); dfree(e->page_group); dfree(e->page_touch); dfree(e->page_slot); dfree(e->page_bits);
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->dcount);
+  dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
+  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->cU); dfree(e->cV); dfree(e->cS);
+  if (e->hsc) (void)hipHostFree(e->hsc);
+  if (e->hci) (void)hipHostFree(e->hci);
   for (auto& v : e->prof_ev)
     for (auto h : v) (void)hipEventDestroy(h);
   if (e->hcount) (void)hipHostFree(e->hcount);
@@ -1080,7 +1302,7 @@ int zr_sort_particles_and_pollute_grid(zr_engine* e, zr_step_stats* st)
 int zr_particles_to_grid(zr_engine* e)
 {
   if (!e) return ZR_ERR_ARG;
-  int rc = do_p2g(e);
+  int rc = do_p2g(e, true);
   if (rc) return rc;
   rc = do_grid(e, GRID_COMBINE, 0.0);
   if (rc) return rc;
@@ -1102,7 +1324,7 @@ int zr_grid_velocity_explicit_update(zr_engine* e, double dt)
 int zr_grid_to_particles(zr_engine* e, double dt, zr_step_stats* st)
 {
   if (!e) return ZR_ERR_ARG;
-  return do_g2p(e, dt, st);
+  return do_g2p(e, dt, st, true);
 }
 
 int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st)
@@ -1111,14 +1333,110 @@ int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st)
   if (!(dt > 0.0)) return fail(e, ZR_ERR_ARG, "dt must be positive");
   int rc = do_sort(e, st);
   if (rc) return rc;
-  rc = do_p2g(e);
+  rc = do_p2g(e, true);
   if (rc) return rc;
   rc = do_grid(e, GRID_COMBINE_UPDATE, dt);
   if (rc) return rc;
-  rc = do_g2p(e, dt, st);
+  rc = do_g2p(e, dt, st, true);
   if (rc < 0) return rc;
   e->time += dt;
   return rc;
+}
+
+int zr_particles_to_grid_apic(zr_engine* e)
+{
+  if (!e) return ZR_ERR_ARG;
+  int rc = do_p2g(e, false);
+  if (rc) return rc;
+  rc = do_grid(e, GRID_COMBINE, 0.0);
+  if (rc) return rc;
+  e->updated = false;
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+static bool valid_opts(const zr_implicit_opts* o)
+{
+  return o && o->newton_max_iterations >= 0 && o->cg_max_iterations >= 0 && o->precision == ZR_FP64;
+}
+
+int zr_backward_euler_step(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* st)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!(dt > 0.0) || !valid_opts(o)) return fail(e, ZR_ERR_ARG, "bad dt or implicit options");
+  return do_be(e, dt, o, st);
+}
+
+int zr_grid_to_particles_apic(zr_engine* e, double dt, zr_step_stats* st)
+{
+  if (!e) return ZR_ERR_ARG;
+  return do_g2p(e, dt, st, false);
+}
+
+int zr_advance_one_time_step_implicit(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* ist,
+                                      zr_step_stats* st)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!(dt > 0.0) || !valid_opts(o)) return fail(e, ZR_ERR_ARG, "bad dt or implicit options");
+  int rc = do_sort(e, st);
+  if (rc) return rc;
+  rc = do_p2g(e, false);
+  if (rc) return rc;
+  rc = do_grid(e, GRID_COMBINE, 0.0);
+  if (rc) return rc;
+  rc = do_be(e, dt, o, ist);
+  if (rc) return rc;
+  rc = do_g2p(e, dt, st, false);
+  if (rc < 0) return rc;
+  e->time += dt;
+  return rc;
+}
+
+// host node-major (3 per node) <-> device [3][grid_cap]
+static int upload_nodevec(zr_engine* e, const double* host, double* dev)
+{
+  const size_t nn = (size_t)e->S * 32;
+  std::vector<double> h(3 * nn);
+  for (size_t k = 0; k < nn; ++k)
+    for (int d = 0; d < 3; ++d) h[d * nn + k] = host[3 * k + d];
+  for (int d = 0; d < 3; ++d)
+    ZR_HIP(e, hipMemcpy(dev + d * e->grid_cap, h.data() + d * nn, nn * sizeof(double), hipMemcpyHostToDevice));
+  return ZR_OK;
+}
+static int download_nodevec(zr_engine* e, const double* dev, double* host)
+{
+  const size_t nn = (size_t)e->S * 32;
+  std::vector<double> h(3 * nn);
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  for (int d = 0; d < 3; ++d)
+    ZR_HIP(e, hipMemcpy(h.data() + d * nn, dev + d * e->grid_cap, nn * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < nn; ++k)
+    for (int d = 0; d < 3; ++d) host[3 * k + d] = h[d * nn + k];
+  return ZR_OK;
+}
+
+int zr_implicit_residual(zr_engine* e, const double* dv, double* r)
+{
+  if (!e || !dv || !r) return ZR_ERR_ARG;
+  if (!e->be_ready) return fail(e, ZR_ERR_STATE, "no backward-Euler state");
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  int rc = upload_nodevec(e, dv, e->idv);
+  if (rc) return rc;
+  rc = be_state_residual(e);
+  if (rc) return rc;
+  return download_nodevec(e, e->ir, r);
+}
+
+int zr_implicit_multiply(zr_engine* e, const double* x, double* b)
+{
+  if (!e || !x || !b) return ZR_ERR_ARG;
+  if (!e->be_ready) return fail(e, ZR_ERR_STATE, "no backward-Euler state");
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  int rc = upload_nodevec(e, x, e->ip);
+  if (rc) return rc;
+  rc = be_apply(e, e->ip, e->iap, 0, nullptr, -1);
+  if (rc) return rc;
+  return download_nodevec(e, e->iap, b);
 }
 
 int64_t zr_debug_size(zr_engine* e, int32_t what)
@@ -1137,6 +1455,7 @@ int64_t zr_debug_size(zr_engine* e, int32_t what)
     case ZR_DUMP_GRID_F:
     case ZR_DUMP_GRID_VSTAR: return (int64_t)e->S * 32 * 3 * 8;
     case ZR_DUMP_NODE_IDX: return (int64_t)e->S * 32 * 8;
+    case ZR_DUMP_DV: return e->be_ready ? (int64_t)e->S * 32 * 3 * 8 : 0;
     default: return -1;
   }
 }
@@ -1189,8 +1508,11 @@ int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes)
     ZR_HIP(e, hipMemcpy(host, e->gm, bytes, hipMemcpyDeviceToHost));
     return ZR_OK;
   }
-  if (what == ZR_DUMP_GRID_V || what == ZR_DUMP_GRID_F || what == ZR_DUMP_GRID_VSTAR) {
-    const double* src = what == ZR_DUMP_GRID_V ? e->gv : (what == ZR_DUMP_GRID_F ? e->gf : e->gvs);
+  if (what == ZR_DUMP_GRID_V || what == ZR_DUMP_GRID_F || what == ZR_DUMP_GRID_VSTAR || what == ZR_DUMP_DV) {
+    const double* src = what == ZR_DUMP_GRID_V ? e->gv
+                        : what == ZR_DUMP_GRID_F ? e->gf
+                        : what == ZR_DUMP_DV     ? e->idv
+                                                 : e->gvs;
     std::vector<double> h(3 * (size_t)nn);
     for (int d = 0; d < 3; ++d)
       ZR_HIP(e, hipMemcpy(h.data() + (size_t)d * nn, src + (size_t)d * e->grid_cap, nn * sizeof(double),

@@ -30,10 +30,10 @@ ZR_STICKY, ZR_SLIP, ZR_SEPARATE = 1, 2, 3
 ZR_HALF_SPACE, ZR_SPHERE = 0, 1
 
 DUMP = dict(sorter=0, order=1, base_offset=2, group_first=3, group_last=4, block_offset=5, pages=6,
-            grid_m=7, grid_v=8, grid_f=9, grid_vstar=10, node_idx=11)
+            grid_m=7, grid_v=8, grid_f=9, grid_vstar=10, node_idx=11, dv=12)
 _DUMP_DTYPE = dict(sorter=np.uint64, order=np.int32, base_offset=np.uint64, group_first=np.int64,
                    group_last=np.int64, block_offset=np.uint64, pages=np.uint64, grid_m=np.float64,
-                   grid_v=np.float64, grid_f=np.float64, grid_vstar=np.float64, node_idx=np.int64)
+                   grid_v=np.float64, grid_f=np.float64, grid_vstar=np.float64, node_idx=np.int64, dv=np.float64)
 
 # every symbol include/zr_mpm.h declares
 EXPORTS = (
@@ -41,8 +41,10 @@ EXPORTS = (
     "zr_upload_particles", "zr_download_particles", "zr_sort_particles_and_pollute_grid",
     "zr_particles_to_grid", "zr_grid_velocity_explicit_update", "zr_grid_to_particles",
     "zr_advance_one_time_step", "zr_debug_size", "zr_debug_dump", "zr_profile_enable", "zr_profile_read",
+    "zr_particles_to_grid_apic", "zr_backward_euler_step", "zr_grid_to_particles_apic",
+    "zr_advance_one_time_step_implicit", "zr_implicit_residual", "zr_implicit_multiply",
 )
-PROF = dict(p2g=0, grid=1, g2p=2, sort=3)
+PROF = dict(p2g=0, grid=1, g2p=2, sort=3, state=4, hvp=5)
 
 
 class RestartException(RuntimeError):
@@ -67,6 +69,29 @@ class ZrCollider(ctypes.Structure):
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
+
+
+class ZrImplicitOpts(ctypes.Structure):
+    _fields_ = [("newton_max_iterations", ctypes.c_int32), ("cg_max_iterations", ctypes.c_int32),
+                ("newton_tolerance", ctypes.c_double), ("cg_tolerance", ctypes.c_double),
+                ("cg_relative_tolerance", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("cg_check_interval", ctypes.c_int32)]
+
+
+class ZrImplicitStats(ctypes.Structure):
+    _fields_ = [("newton_iterations", ctypes.c_int32), ("cg_iterations", ctypes.c_int32),
+                ("residual_norm", ctypes.c_double * 8), ("num_residuals", ctypes.c_int32),
+                ("converged", ctypes.c_int32)]
+
+
+def implicit_opts(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=20,
+                  cg_tolerance=16 * 2.220446049250313e-16, cg_relative_tolerance=0.0, precision=0,
+                  cg_check_interval=8):
+    """Reference defaults: NewtonsMethod(1e-5, 3) (MpmSimulationBase.cpp:51),
+    ConjugateGradient(20) (BackwardEuler.h:90-91), LinearSolver::tolerance
+    16 eps (LinearSolver.h:26), Newton-adaptive relative tolerance."""
+    return ZrImplicitOpts(int(newton_max_iterations), int(cg_max_iterations), float(newton_tolerance),
+                          float(cg_tolerance), float(cg_relative_tolerance), int(precision), int(cg_check_interval))
 
 
 class ZrParticles(ctypes.Structure):
@@ -111,6 +136,14 @@ def load_library(path=LIB_PATH):
     L.zr_debug_size.argtypes = [vp, ctypes.c_int32]
     L.zr_debug_size.restype = ctypes.c_int64
     L.zr_debug_dump.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_size_t]
+    L.zr_particles_to_grid_apic.argtypes = [vp]
+    L.zr_backward_euler_step.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrImplicitOpts),
+                                         ctypes.POINTER(ZrImplicitStats)]
+    L.zr_grid_to_particles_apic.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
+    L.zr_advance_one_time_step_implicit.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrImplicitOpts),
+                                                    ctypes.POINTER(ZrImplicitStats), ctypes.POINTER(ZrStepStats)]
+    L.zr_implicit_residual.argtypes = [vp, vp, vp]
+    L.zr_implicit_multiply.argtypes = [vp, vp, vp]
     L.zr_profile_enable.argtypes = [vp, ctypes.c_int32]
     L.zr_profile_read.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_int64)]
@@ -222,6 +255,42 @@ class MpmEngine:
     def advanceOneTimeStep(self, dt):
         self._check(self.L.zr_advance_one_time_step(self.h, float(dt), ctypes.byref(self.stats)),
                     "advanceOneTimeStep")
+
+    # ---------------------------------------------------------- implicit
+    def particlesToGridApic(self):
+        self._check(self.L.zr_particles_to_grid_apic(self.h), "particlesToGrid (APIC)")
+
+    def backwardEulerStep(self, dt, opts=None):
+        st = ZrImplicitStats()
+        o = opts if opts is not None else implicit_opts()
+        self._check(self.L.zr_backward_euler_step(self.h, float(dt), ctypes.byref(o), ctypes.byref(st)),
+                    "backwardEulerStep")
+        return st
+
+    def gridToParticlesApic(self, dt):
+        self._check(self.L.zr_grid_to_particles_apic(self.h, float(dt), ctypes.byref(self.stats)),
+                    "gridToParticles (APIC)")
+
+    def advanceOneTimeStepImplicit(self, dt, opts=None):
+        st = ZrImplicitStats()
+        o = opts if opts is not None else implicit_opts()
+        self._check(self.L.zr_advance_one_time_step_implicit(self.h, float(dt), ctypes.byref(o), ctypes.byref(st),
+                                                             ctypes.byref(self.stats)), "advanceOneTimeStep (implicit)")
+        return st
+
+    def implicit_residual(self, dv):
+        dv = np.ascontiguousarray(dv, dtype=np.float64).reshape(-1)
+        r = np.zeros_like(dv)
+        self._check(self.L.zr_implicit_residual(self.h, dv.ctypes.data_as(ctypes.c_void_p),
+                                                r.ctypes.data_as(ctypes.c_void_p)), "zr_implicit_residual")
+        return r.reshape(-1, 3)
+
+    def implicit_multiply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        b = np.zeros_like(x)
+        self._check(self.L.zr_implicit_multiply(self.h, x.ctypes.data_as(ctypes.c_void_p),
+                                                b.ctypes.data_as(ctypes.c_void_p)), "zr_implicit_multiply")
+        return b.reshape(-1, 3)
 
     # ---------------------------------------------------------- timing
     def profile(self, on=True):
