@@ -1,12 +1,16 @@
 """Benchmark of the MPM time step on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload explicit-cube]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload implicit-boxes|explicit-cube]
 
-One step = MpmSimulationBase::advanceOneTimeStep on the whole particle set
-(sort + page allocation, P2G with fused stress, grid update, G2P with F
-update), inputs resident in HBM.  Prints one JSON line with particle-steps/s,
-the roofline position of the dominant kernel (HIP-event kernel durations over
-the timed region) and a CPU baseline (the FP64 C oracle on the host cores).
+One step = MpmSimulationBase::advanceOneTimeStep on the whole particle set,
+inputs resident in HBM:
+  implicit-boxes (default; the metric's workload): 64 seeded boxes, ~0.97M
+    particles at dx = 1/256, dt = 2e-3, backward Euler with Newton (tol 1e-5,
+    <= 3 iterations) and mass-preconditioned CG (relative tolerance 1e-6)
+  explicit-cube: 140,608 particles, dx = 1/128, dt = 1e-4, explicit MLS
+Prints one JSON line with particle-steps/s, the roofline position of the
+dominant kernel (HIP-event kernel durations over the timed region) and a CPU
+baseline (the FP64 C oracle on the host cores, on a bounded sample).
 """
 import argparse
 import json
@@ -20,6 +24,7 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s mixed-FP32 vs FP64 at fixed error; P2G/G2P HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 F64 = 8
+F32 = 4
 
 
 def algorithmic_bytes(kernel, n, g):
@@ -34,7 +39,49 @@ def algorithmic_bytes(kernel, n, g):
         return n * (28 * F64 + 4) + g * 7 * F64
     if kernel == "g2p":
         return n * (36 * F64 + 4) + g * 3 * F64
+    if kernel == "hvp":
+        # FP32 operator data in sorted order: U9 Q9 Sigma3 k2 frac3 (26 values) + cell 4 B,
+        # p3 per node (FP32)
+        return n * (26 * F32 + 4) + g * 3 * F32
+    if kernel == "hvp64":
+        return n * (26 * F64 + 4) + g * 3 * F64
+    if kernel == "state":
+        # reads x3 Fn9 mu1 lambda1 V0 1 (15 values) + index and (vn + dv) 6 per node,
+        # writes the FP32 operator data (26 values + cell)
+        return n * (15 * F64 + 4 + 26 * F32 + 4) + g * 6 * F64
     raise ValueError(kernel)
+
+
+IMPLICIT_OPTS = dict(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=500, cg_tolerance=1e30,
+                     cg_relative_tolerance=1e-6)
+
+
+def cpu_baseline_implicit(scene, dt, threads, nboxes=8):
+    """One implicit oracle step on the particles of the first ``nboxes`` boxes."""
+    import numpy as np
+
+    import oracle
+    from ziran2019_amd import scenes
+    dx = scene["dx"]
+    b = scenes.base_nodes(scene["X"], dx)
+    keep = np.zeros(len(b), dtype=bool)
+    for lo, hi in scene["boxes"][:nboxes]:
+        c = np.floor(scene["X"] / dx).astype(np.int64)
+        keep |= np.all((c >= lo) & (c < hi), axis=1)
+    sub = {k: (np.ascontiguousarray(np.asarray(v)[keep]) if isinstance(v, np.ndarray) and len(v) == len(b) else v)
+           for k, v in scene.items()}
+    sim = oracle.OracleSim(dx=dx, gravity=scene["gravity"], nthreads=threads)
+    sim.set_particles(sub["X"], sub["V"], sub["C"], sub["F"], sub["mass"], sub["vol"], sub["mu"], sub["lam"])
+    for c in scene["colliders"]:
+        sim.add_collider(**c)
+    t0 = time.perf_counter()
+    flags, st = sim.step_implicit(dt, oracle.implicit_opts(**IMPLICIT_OPTS))
+    el = time.perf_counter() - t0
+    n = int(keep.sum())
+    return dict(value=n / el, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"FP64 C oracle (oracle/zro_mpm.c, OpenMP) implicit step on the {n} particles of the first "
+                       f"{nboxes} of the 64 boxes, same solver settings ({st.newton_iterations} Newton / "
+                       f"{st.cg_iterations} CG iterations), {el:.1f} s")
 
 
 def cpu_baseline(scene, dt, seconds, threads):
@@ -59,20 +106,7 @@ def cpu_baseline(scene, dt, seconds, threads):
                        f"{steps} steps after one untimed step, {el:.1f} s")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="explicit-cube", choices=["explicit-cube"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the step would shard)", file=sys.stderr)
-        sys.exit(2)
-
+def run_explicit(args):
     import numpy as np
     from ziran2019_amd import scenes
     from ziran2019_amd.engine import MpmEngine
@@ -92,32 +126,115 @@ def main():
     t1 = time.perf_counter()
     kt = {k: eng.kernel_time(k) for k in ("p2g", "grid", "g2p", "sort")}
     eng.profile(False)
-    elapsed = t1 - t0
-    # active grid nodes of the final state
     eng.sortParticlesAndPolluteGrid()
     eng.particlesToGrid()
     g_nodes = int(np.count_nonzero(eng.dump("grid_m")))
     eng.close()
+    workload = ("explicit MLS-MPM, fixed-corotated cube 26^3 cells x 8 ppc, dx=1/128, dt=1e-4, sticky ground, "
+                "FP64 plain path")
+    base = None if args.no_cpu_baseline else cpu_baseline(scene, dt, args.cpu_seconds, min(16, os.cpu_count() or 1))
+    return dict(n=n, dt=dt, dx=scene["dx"], elapsed=t1 - t0, kt=kt, g_nodes=g_nodes, roof_kernels=("p2g", "g2p"),
+                workload=workload, base=base, extra={}, dtype="f64")
+
+
+def _implicit_run(scene, opts, steps, warmup, profile):
+    from ziran2019_amd.engine import MpmEngine
+    eng = MpmEngine(dx=scene["dx"], gravity=scene["gravity"])
+    eng.upload(scene)
+    eng.set_colliders(scene["colliders"])
+    for _ in range(warmup):
+        eng.advanceOneTimeStepImplicit(2e-3, opts)
+    if profile:
+        eng.profile(True)
+    newton = cg = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = eng.advanceOneTimeStepImplicit(2e-3, opts)
+        newton += st.newton_iterations
+        cg += st.cg_iterations
+    t1 = time.perf_counter()
+    kt = {k: eng.kernel_time(k) for k in ("p2g", "grid", "g2p", "sort", "state", "hvp")} if profile else None
+    eng.profile(False)
+    final = eng.download()
+    return eng, dict(elapsed=t1 - t0, kt=kt, newton=newton / steps, cg=cg / steps, final=final)
+
+
+def run_implicit(args):
+    import numpy as np
+    from ziran2019_amd import scenes
+    from ziran2019_amd.engine import implicit_opts
+
+    scene = scenes.boxes_scene()
+    dt = 2e-3
+    n = len(scene["mass"])
+    # plain FP64 path, then the FP32-bulk / FP64-refined form from the same state
+    eng64, r64 = _implicit_run(scene, implicit_opts(**IMPLICIT_OPTS), args.steps, args.warmup, profile=False)
+    eng64.close()
+    eng, r32 = _implicit_run(scene, implicit_opts(precision=1, **IMPLICIT_OPTS), args.steps, args.warmup,
+                             profile=True)
+    eng.sortParticlesAndPolluteGrid()
+    eng.particlesToGridApic()
+    g_nodes = int(np.count_nonzero(eng.dump("grid_m")))
+    eng.close()
+    X64, X32 = r64["final"]["X"], r32["final"]["X"]
+    disp = np.abs(X64 - scene["X"]).max()
+    err = dict(max_abs_position_diff=float(np.abs(X32 - X64).max()),
+               rel_to_displacement=float(np.abs(X32 - X64).max() / disp),
+               rel_to_extent=float(np.abs(X32 - X64).max() / np.abs(X64).max()),
+               velocity_rel=float(np.abs(r32["final"]["V"] - r64["final"]["V"]).max() /
+                                  np.abs(r64["final"]["V"]).max()),
+               steps=args.steps + args.warmup)
+    workload = (f"implicit backward Euler, 64 seeded fixed-corotated boxes (E lognormal(ln 5e4, 0.5)), {n} particles "
+                "at 8 ppc, dx=1/256, dt=2e-3, Newton tol 1e-5 (<=3 its) + mass-preconditioned CG rel tol 1e-6; "
+                "FP32 operator and Krylov vectors inside the FP64 Newton residual")
+    base = None if args.no_cpu_baseline else cpu_baseline_implicit(scene, dt, min(16, os.cpu_count() or 1))
+    fp64_value = n * args.steps / r64["elapsed"]
+    extra = dict(newton_iterations_per_step=r32["newton"], cg_iterations_per_step=r32["cg"],
+                 fp64_plain=dict(value=fp64_value, ms_per_step=r64["elapsed"] / args.steps * 1e3,
+                                 newton_iterations_per_step=r64["newton"], cg_iterations_per_step=r64["cg"]),
+                 speedup_vs_fp64=(n * args.steps / r32["elapsed"]) / fp64_value,
+                 fp32_vs_fp64_error=err)
+    return dict(n=n, dt=dt, dx=scene["dx"], elapsed=r32["elapsed"], kt=r32["kt"], g_nodes=g_nodes,
+                roof_kernels=("hvp", "state"), workload=workload, base=base, extra=extra, dtype="f32-bulk/f64-refined")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="implicit-boxes", choices=["implicit-boxes", "explicit-cube"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the step would shard)", file=sys.stderr)
+        sys.exit(2)
+    implicit = args.workload == "implicit-boxes"
+    if args.steps is None:
+        args.steps = 10 if implicit else 1000
+    if args.warmup is None:
+        args.warmup = 2 if implicit else 20
+    r = run_implicit(args) if implicit else run_explicit(args)
+    n, kt, g_nodes = r["n"], r["kt"], r["g_nodes"]
 
     kern = {}
-    for k in ("p2g", "g2p"):
+    for k in r["roof_kernels"]:
         ms, launches = kt[k]
         avg_s = ms / 1e3 / max(launches, 1)
         b = algorithmic_bytes(k, n, g_nodes)
         kern[k] = dict(avg_us=avg_s * 1e6, bytes=b, gbs=b / avg_s / 1e9)
-    dom = max(kern, key=lambda k: kern[k]["avg_us"])
+    dom = max(kern, key=lambda k: kern[k]["avg_us"] * kt[k][1])
     roof = dict(bound="hbm", kernel=dom, achieved=round(kern[dom]["gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(kern[dom]["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
                 bytes_per_launch=kern[dom]["bytes"], avg_launch_us=round(kern[dom]["avg_us"], 2))
-    kernels = {k: dict(avg_us=round(v[0] * 1e3 / max(v[1], 1), 2), launches=v[1]) for k, v in kt.items()}
-    for k in ("p2g", "g2p"):
+    kernels = {k: dict(avg_us=round(v[0] * 1e3 / max(v[1], 1), 2), launches=v[1], total_ms=round(v[0], 3))
+               for k, v in kt.items()}
+    for k in kern:
         kernels[k]["gbs"] = round(kern[k]["gbs"], 2)
 
-    base = None
-    if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        base = cpu_baseline(scene, dt, args.cpu_seconds, threads)
-
+    elapsed = r["elapsed"]
     out = dict(
         metric=METRIC,
         value=n * args.steps / elapsed,
@@ -129,14 +246,13 @@ def main():
         higher_is_better=True,
         scaling="weak",
         vs_baseline=None,
-        dtype="f64",
+        dtype=r["dtype"],
         data="synthetic",
-        config=dict(workload="explicit MLS-MPM, fixed-corotated cube 26^3 cells x 8 ppc, dx=1/128, dt=1e-4, "
-                             "sticky ground, FP64 plain path",
-                    n_particles=n, active_nodes=g_nodes, dx=scene["dx"], dt=dt),
+        config=dict(workload=r["workload"], n_particles=n, active_nodes=g_nodes, dx=r["dx"], dt=r["dt"],
+                    **r["extra"]),
         roofline=roof,
         kernels=kernels,
-        cpu_baseline=base,
+        cpu_baseline=r["base"],
     )
     print(json.dumps(out))
 

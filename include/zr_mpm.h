@@ -36,7 +36,10 @@ enum {
   ZR_ERR_STATE = -4
 };
 
-enum { ZR_FP64 = 0 };
+/* ZR_FP32_REFINED: the implicit operator and the Krylov vectors in FP32
+   (FP32 accumulation), inside the FP64 Newton residual that refines each
+   step; the SVD, the stress, inner products and norms stay FP64. */
+enum { ZR_FP64 = 0, ZR_FP32_REFINED = 1 };
 enum { ZR_TRANSFER_MLS = 0, ZR_TRANSFER_APIC = 1 };
 
 /* collision object types, AnalyticCollisionObject::COLLISION_OBJECT_TYPE
@@ -99,7 +102,7 @@ typedef struct zr_implicit_opts {
   double newton_tolerance;
   double cg_tolerance;           /* absolute cap of the CG local tolerance */
   double cg_relative_tolerance;  /* <= 0: Newton-adaptive (reference behaviour) */
-  int32_t precision;             /* ZR_FP64 */
+  int32_t precision;             /* ZR_FP64 or ZR_FP32_REFINED */
   int32_t cg_check_interval;     /* CG iterations between host polls of the
                                     device-side termination flag (0: 8) */
 } zr_implicit_opts;

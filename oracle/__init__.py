@@ -26,7 +26,8 @@ _p = ctypes.c_void_p
 
 class ZroImplicitOpts(ctypes.Structure):
     _fields_ = [("newton_max_iterations", ctypes.c_int32), ("newton_tolerance", _d),
-                ("cg_max_iterations", ctypes.c_int32), ("cg_tolerance", _d), ("cg_relative_tolerance", _d)]
+                ("cg_max_iterations", ctypes.c_int32), ("cg_tolerance", _d), ("cg_relative_tolerance", _d),
+                ("preconditioner", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class ZroImplicitStats(ctypes.Structure):
@@ -35,11 +36,11 @@ class ZroImplicitStats(ctypes.Structure):
 
 
 def implicit_opts(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=20,
-                  cg_tolerance=16 * 2.220446049250313e-16, cg_relative_tolerance=0.0):
+                  cg_tolerance=16 * 2.220446049250313e-16, cg_relative_tolerance=0.0, preconditioner=0):
     """Defaults of the reference: NewtonsMethod(1e-5, 3) (MpmSimulationBase.cpp:51),
     ConjugateGradient(20) with LinearSolver::tolerance 16 eps (LinearSolver.h:26)."""
     return ZroImplicitOpts(int(newton_max_iterations), float(newton_tolerance), int(cg_max_iterations),
-                           float(cg_tolerance), float(cg_relative_tolerance))
+                           float(cg_tolerance), float(cg_relative_tolerance), int(preconditioner), 0)
 
 
 class ZroCollider(ctypes.Structure):
@@ -124,6 +125,7 @@ def lib():
         L.zro_num_node_slots.restype = _i64
         L.zro_num_node_slots.argtypes = [_p]
         L.zro_get_implicit.argtypes = [_p, _p, _p, _p]
+        L.zro_implicit_diagonal.argtypes = [_p, _p]
         _lib = L
     return _lib
 
@@ -321,6 +323,12 @@ class OracleSim:
         r = np.zeros_like(dv)
         lib().zro_implicit_residual(self.h, _ptr(dv), _ptr(r))
         return r.reshape(-1, 3)
+
+    def implicit_diagonal(self):
+        k = self.num_node_slots()
+        d = np.zeros(3 * k)
+        lib().zro_implicit_diagonal(self.h, _ptr(d))
+        return d.reshape(k, 3)
 
     def implicit_energy(self, dv):
         dv = np.ascontiguousarray(dv, dtype=np.float64).reshape(-1)

@@ -94,6 +94,9 @@ typedef struct {
   int32_t cg_max_iterations;       /* ConjugateGradient(20) (BackwardEuler.h:90-91) */
   double cg_tolerance;             /* LinearSolver::tolerance, default 16 eps (LinearSolver.h:26) */
   double cg_relative_tolerance;    /* <= 0: Newton-adaptive min(0.5, sqrt(max(|r|, tol))) */
+  int32_t preconditioner;          /* 0: mass-lumped in/m (MpmSimulationBase.cpp:617-623),
+                                      1: Jacobi on the operator diagonal (KrylovSolvers.h:32-57) */
+  int32_t reserved;
 } zro_implicit_opts;
 
 typedef struct {
@@ -118,6 +121,8 @@ int zro_step_implicit(zro_sim* s, double dt, const zro_implicit_opts* o, zro_imp
 void zro_implicit_multiply(zro_sim* s, const double* x, double* b);
 /* residual at dv: dt m g + dt f(dv) - m dv, projected (BackwardEuler.h:101-124) */
 void zro_implicit_residual(zro_sim* s, const double* dv, double* r);
+/* exact diagonal of the operator at the current Newton state (3 per node) */
+void zro_implicit_diagonal(zro_sim* s, double* diag);
 /* objective energy whose negative gradient is the unprojected residual */
 double zro_implicit_energy(zro_sim* s, const double* dv);
 int64_t zro_num_node_slots(const zro_sim* s);

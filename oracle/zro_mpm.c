@@ -17,6 +17,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
+#include <float.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -932,7 +933,8 @@ void zro_implicit_multiply(zro_sim* s, const double* x, double* b)
 
 /* ConjugateGradient::solve (ConjugateGradient.h:99-147), mass-lumped
    preconditioner out = in / m (MpmSimulationBase.cpp:617-623) */
-static int cg_solve(zro_sim* s, double* x, const double* b, double rel_tol, const zro_implicit_opts* o)
+static int cg_solve(zro_sim* s, double* x, const double* b, double rel_tol, const zro_implicit_opts* o,
+                    const double* inv_diag)
 {
   int64_t nn = s->npages * 32, n3 = 3 * nn;
   double* r = (double*)calloc((size_t)n3, sizeof(double));
@@ -945,7 +947,9 @@ static int cg_solve(zro_sim* s, double* x, const double* b, double rel_tol, cons
 #define PRECOND(dst, src)                                                               \
   for (int64_t k = 0; k < nn; ++k)                                                      \
     for (int d = 0; d < 3; ++d)                                                         \
-      dst[3 * k + d] = node_active(s, k) ? src[3 * k + d] / s->nodes[k].m : 0.0;
+      dst[3 * k + d] = !node_active(s, k) ? 0.0                                         \
+                     : inv_diag ? src[3 * k + d] * inv_diag[3 * k + d]                  \
+                                : src[3 * k + d] / s->nodes[k].m;
   PRECOND(q, r);
   for (int64_t k = 0; k < n3; ++k) p[k] = q[k];
   double zTr = dot_active(s, r, q);
@@ -1018,7 +1022,27 @@ int zro_backward_euler(zro_sim* s, double dt, const zro_implicit_opts* o, zro_im
     if (norm < o->newton_tolerance) { local.converged = 1; break; }
     double rel = o->cg_relative_tolerance > 0 ? o->cg_relative_tolerance
                : fmin(0.5, sqrt(fmax(norm, o->newton_tolerance)));
-    local.cg_iterations += cg_solve(s, step, r, rel, o);
+    double* inv_diag = NULL;
+    if (o->preconditioner == 1) {
+      inv_diag = (double*)calloc((size_t)(3 * nn + 1), sizeof(double));
+      zro_implicit_diagonal(s, inv_diag);
+      for (int64_t k = 0; k < nn; ++k) {
+        /* JacobiPreconditioner: 1 / max(16 eps, |a_ii|); isotropic (largest
+           entry) on slip-collided nodes so z stays in the projected space */
+        double mx = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          double a = fabs(inv_diag[3 * k + d]);
+          inv_diag[3 * k + d] = a;
+          if (a > mx) mx = a;
+        }
+        for (int d = 0; d < 3; ++d) {
+          double a = s->collided[k] ? mx : inv_diag[3 * k + d];
+          inv_diag[3 * k + d] = 1.0 / fmax(16.0 * DBL_EPSILON, a);
+        }
+      }
+    }
+    local.cg_iterations += cg_solve(s, step, r, rel, o, inv_diag);
+    free(inv_diag);
     for (int64_t k = 0; k < 3 * nn; ++k) s->dv[k] += step[k];
     local.newton_iterations++;
   }
@@ -1127,4 +1151,46 @@ void zro_get_implicit(const zro_sim* s, double* dv, double* vn, char* collided)
   if (dv && s->dv) memcpy(dv, s->dv, sizeof(double) * 3 * (size_t)nn);
   if (vn && s->vn) memcpy(vn, s->vn, sizeof(double) * 3 * (size_t)nn);
   if (collided && s->collided) memcpy(collided, s->collided, (size_t)nn);
+}
+
+/* diagonal of multiply(): a_(i,d) = m_i + dt^2 sum_p (V0 dP(e_d (Fn^T grad w_ip)^T) Fn^T grad w_ip)_d,
+   the entries JacobiPreconditioner reads from the assembled matrix
+   (KrylovSolvers.h:32-57, BackwardEuler.h:322-351) */
+void zro_implicit_diagonal(zro_sim* s, double* diag)
+{
+  int64_t nn = s->npages * 32;
+  double dt = s->be_dt;
+  for (int64_t k = 0; k < nn; ++k)
+    for (int d = 0; d < 3; ++d) diag[3 * k + d] = node_active(s, k) ? s->nodes[k].m : 0.0;
+  for (int color = 0; color < 8; ++color) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(s->nthreads)
+    for (int64_t g = 0; g < s->ngroups; ++g) {
+      if ((int)(s->block_offset[g] & 7) != color) continue;
+      for (int64_t idx = s->gfirst[g]; idx <= s->glast[g]; ++idx) {
+        int i = s->order[idx];
+        double Fn[3][3];
+        mat_from_cm(s->Fn + 9 * (int64_t)i, Fn);
+        zro_spline sp;
+        spline_of(s, s->X + 3 * i, &sp);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) {
+              double gw[3];
+              spline_node(s, &sp, a, b, c, gw);
+              int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+              double h[3];
+              for (int q = 0; q < 3; ++q) h[q] = Fn[0][q] * gw[0] + Fn[1][q] * gw[1] + Fn[2][q] * gw[2];
+              for (int d = 0; d < 3; ++d) {
+                double dF[3][3] = { { 0 } }, dFcm[9], dPcm[9], dP[3][3];
+                for (int q = 0; q < 3; ++q) dF[d][q] = h[q];
+                mat_to_cm(dF, dFcm);
+                zro_fcr_first_piola_differential(s->mu[i], s->lam[i], s->F + 9 * (int64_t)i, dFcm, dPcm);
+                mat_from_cm(dPcm, dP);
+                double v = dP[d][0] * h[0] + dP[d][1] * h[1] + dP[d][2] * h[2];
+                diag[3 * k + d] += dt * dt * s->vol[i] * v;
+              }
+            }
+      }
+    }
+  }
 }

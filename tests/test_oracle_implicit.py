@@ -120,3 +120,40 @@ def test_sticky_projection_zeroes_collided_rows():
     g = sim.grid()
     vnew = g["v"] + dv
     assert np.abs(vnew[st["collided"] & active]).max() < 1e-14
+
+
+def test_diagonal_matches_unit_vector_products():
+    """The Jacobi diagonal equals e^T A e for unit vectors (brute force)."""
+    sim, s, active, _ = _setup(seed=8)
+    rng = np.random.default_rng(5)
+    dv = 0.01 * _rand_field(rng, active)
+    sim.implicit_residual(dv)
+    diag = sim.implicit_diagonal()
+    idx = np.nonzero(active)[0]
+    for k in rng.choice(idx, size=12, replace=False):
+        for d in range(3):
+            e = np.zeros((len(active), 3))
+            e[k, d] = 1.0
+            assert abs(sim.implicit_multiply(e)[k, d] - diag[k, d]) <= 1e-10 * abs(diag[k, d])
+
+
+def test_jacobi_and_mass_preconditioners_reach_the_same_step():
+    kw = dict(newton_max_iterations=4, newton_tolerance=1e-10, cg_max_iterations=500, cg_tolerance=1e30,
+              cg_relative_tolerance=1e-10)
+    s = scenes.cube_scene_small(seed=9, ncells=4, E=2e4)
+    n = len(s["mass"])
+    rng = np.random.default_rng(9)
+    F = np.tile(np.eye(3), (n, 1, 1)) + 0.05 * rng.normal(size=(n, 3, 3))
+    mu = s["mu"] * np.exp(rng.normal(size=n))  # strong stiffness contrast
+    its, dvs = [], []
+    for pre in (0, 1):
+        sim = oracle.OracleSim(dx=s["dx"], gravity=(0, -9.8, 0), nthreads=2)
+        sim.set_particles(s["X"], s["V"], s["C"], np.ascontiguousarray(F.transpose(0, 2, 1).reshape(n, 9)),
+                          s["mass"], s["vol"], mu, s["lam"])
+        sim.sort_and_pollute()
+        sim.p2g_apic()
+        st = sim.backward_euler(2e-3, oracle.implicit_opts(preconditioner=pre, **kw))
+        assert st.converged == 1
+        its.append(st.cg_iterations)
+        dvs.append(sim.implicit_state()["dv"])
+    assert np.abs(dvs[0] - dvs[1]).max() < 1e-9 * np.abs(dvs[0]).max()

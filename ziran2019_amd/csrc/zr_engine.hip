@@ -297,6 +297,15 @@ __global__ void k_assign_slots(uint32_t ntouch, const uint32_t* touch_page, cons
   page_slot[q] = (int32_t)s;
 }
 
+// the 8 neighbour page slots of every group, in touch order (o = 4a + 2b + c)
+__global__ void k_group_slots(uint32_t ntouch, const uint32_t* touch_page, const int32_t* page_slot,
+                              int32_t* group_nslot)
+{
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntouch) return;
+  group_nslot[t] = page_slot[touch_page[t]];
+}
+
 // ---------------------------------------------------------------------------
 // P2G, MLS with force (MpmSimulationBase.cpp:488-561) with the fixed-corotated
 // Kirchhoff stress of FBasedMpmForceHelper::updateState fused in
@@ -667,6 +676,7 @@ struct zr_engine {
   uint32_t *group_page = nullptr, *group_count = nullptr, *group_start = nullptr;
   uint32_t *touch_page = nullptr, *touch_slot = nullptr;
   uint32_t* slot_page = nullptr;
+  int32_t* group_nslot = nullptr;  // [8 * groups]
   uint32_t *word_base = nullptr;
 
   uint32_t A = 0, S = 0;  // groups, slots of the current sort
@@ -694,8 +704,14 @@ struct zr_engine {
   int32_t* ci = nullptr;
   double* hsc = nullptr;
   int32_t* hci = nullptr;
-  double *cU = nullptr, *cV = nullptr, *cS = nullptr;
-  int64_t cache_n = 0;
+  // FP32-bulk CG vectors and the operator data in both precisions
+  float *fx = nullptr, *frc = nullptr, *fq = nullptr, *fp = nullptr, *fap = nullptr;
+  size_t imp32_cap = 0;
+  double *c64 = nullptr;  // U9 Q9 S3 K2 frac3 = 26 rows of n
+  float* c32 = nullptr;
+  uint32_t* ccell = nullptr;
+  int64_t c64_n = 0, c32_n = 0, ccell_n = 0;
+  int32_t be_prec = ZR_FP64;
   bool be_ready = false;
   double be_dt = 0.0;
 
@@ -782,7 +798,7 @@ void free_particles(zr_engine* e)
   dfree(e->x); dfree(e->v); dfree(e->C); dfree(e->F); dfree(e->m); dfree(e->vol); dfree(e->mu); dfree(e->lam);
   dfree(e->p_page); dfree(e->p_re); dfree(e->order); dfree(e->keys);
   dfree(e->group_page); dfree(e->group_count); dfree(e->group_start);
-  dfree(e->touch_page); dfree(e->touch_slot); dfree(e->slot_page);
+  dfree(e->touch_page); dfree(e->touch_slot); dfree(e->slot_page); dfree(e->group_nslot);
   e->n = 0;
 }
 
@@ -865,6 +881,7 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   if (rc) return rc;
   k_assign_slots<<<blocks_for(8ull * A, 256), 256, 0, s>>>(8u * A, e->touch_page, e->page_touch, e->touch_slot,
                                                           e->slot_page, e->page_slot);
+  k_group_slots<<<blocks_for(8ull * A, 256), 256, 0, s>>>(8u * A, e->touch_page, e->page_slot, e->group_nslot);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_SORT);
   rc = sync_counts(e);
@@ -946,7 +963,7 @@ int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
 }
 
 // ---------------------------------------------------------------- implicit
-int ensure_implicit(zr_engine* e)
+int ensure_implicit(zr_engine* e, int32_t prec)
 {
   if (e->imp_cap < e->grid_cap) {
     double** arrs[] = { &e->ivn, &e->idv, &e->ir, &e->ix, &e->irc, &e->iq, &e->ip, &e->iap };
@@ -955,6 +972,14 @@ int ensure_implicit(zr_engine* e)
       ZR_HIP(e, hipMalloc(a, 3 * e->grid_cap * sizeof(double)));
     }
     e->imp_cap = e->grid_cap;
+  }
+  if (prec == ZR_FP32_REFINED && e->imp32_cap < e->grid_cap) {
+    float** arrs[] = { &e->fx, &e->frc, &e->fq, &e->fp, &e->fap };
+    for (auto a : arrs) {
+      dfree(*a);
+      ZR_HIP(e, hipMalloc(a, 3 * e->grid_cap * sizeof(float)));
+    }
+    e->imp32_cap = e->grid_cap;
   }
   size_t nb = (e->grid_cap + 255) / 256 + 1;
   if (nb > e->partial_cap) {
@@ -968,17 +993,65 @@ int ensure_implicit(zr_engine* e)
     ZR_HIP(e, hipHostMalloc(&e->hsc, SC_N * sizeof(double)));
     ZR_HIP(e, hipHostMalloc(&e->hci, CI_N * sizeof(int32_t)));
   }
-  if (e->cache_n != e->n) {
-    dfree(e->cU); dfree(e->cV); dfree(e->cS);
-    ZR_HIP(e, hipMalloc(&e->cU, 9 * (size_t)e->n * sizeof(double)));
-    ZR_HIP(e, hipMalloc(&e->cV, 9 * (size_t)e->n * sizeof(double)));
-    ZR_HIP(e, hipMalloc(&e->cS, 3 * (size_t)e->n * sizeof(double)));
-    e->cache_n = e->n;
+  const size_t n = (size_t)(e->n > 0 ? e->n : 1);
+  if (e->ccell_n != e->n) {
+    dfree(e->ccell);
+    ZR_HIP(e, hipMalloc(&e->ccell, n * sizeof(uint32_t)));
+    e->ccell_n = e->n;
+  }
+  if (prec == ZR_FP64 && e->c64_n != e->n) {
+    dfree(e->c64);
+    ZR_HIP(e, hipMalloc(&e->c64, 26 * n * sizeof(double)));
+    e->c64_n = e->n;
+  }
+  if (prec == ZR_FP32_REFINED && e->c32_n != e->n) {
+    dfree(e->c32);
+    ZR_HIP(e, hipMalloc(&e->c32, 26 * n * sizeof(float)));
+    e->c32_n = e->n;
   }
   return ZR_OK;
 }
 
+template <class T>
+HvpCache<T> hvp_cache(zr_engine* e);
+template <>
+HvpCache<double> hvp_cache<double>(zr_engine* e)
+{
+  const int64_t S = e->n;
+  double* b = e->c64;
+  return HvpCache<double>{ b, b + 9 * S, b + 18 * S, b + 21 * S, b + 23 * S, e->ccell, S };
+}
+template <>
+HvpCache<float> hvp_cache<float>(zr_engine* e)
+{
+  const int64_t S = e->n;
+  float* b = e->c32;
+  return HvpCache<float>{ b, b + 9 * S, b + 18 * S, b + 21 * S, b + 23 * S, e->ccell, S };
+}
+
+struct CgVecs64 {
+  using T = double;
+  static double* x(zr_engine* e) { return e->ix; }
+  static double* r(zr_engine* e) { return e->irc; }
+  static double* q(zr_engine* e) { return e->iq; }
+  static double* p(zr_engine* e) { return e->ip; }
+  static double* ap(zr_engine* e) { return e->iap; }
+};
+struct CgVecs32 {
+  using T = float;
+  static float* x(zr_engine* e) { return e->fx; }
+  static float* r(zr_engine* e) { return e->frc; }
+  static float* q(zr_engine* e) { return e->fq; }
+  static float* p(zr_engine* e) { return e->fp; }
+  static float* ap(zr_engine* e) { return e->fap; }
+};
+
 NodeVec3 nv(zr_engine* e, double* p) { return NodeVec3{ p, (int64_t)e->grid_cap }; }
+template <class T>
+NodeVec<T> nvT(zr_engine* e, T* p)
+{
+  return NodeVec<T>{ p, (int64_t)e->grid_cap };
+}
 
 int read_scalars(zr_engine* e)
 {
@@ -989,15 +1062,16 @@ int read_scalars(zr_engine* e)
 }
 
 // Newton state at the current dv + residual + its squared mass norm (device)
+template <class T>
 int be_state_residual(zr_engine* e)
 {
   const double dx = e->cfg.dx, dt = e->be_dt;
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
   ColliderSet cs = collider_set(e);
-  ImpCache cache{ e->cU, e->cV, e->cS };
   prof_mark(e, ZR_PROF_STATE);
-  k_be_state<<<e->A, 256, 0, e->stream>>>(pview(e), cache, e->order, e->group_start, e->group_page, e->page_slot,
-                                          e->ivn, e->idv, (int64_t)e->grid_cap, dx, 1.0 / dx, dt, e->stage);
+  k_be_state<T><<<e->A, 256, 0, e->stream>>>(pview(e), hvp_cache<T>(e), e->order, e->group_start, e->group_page,
+                                             e->page_slot, e->ivn, e->idv, (int64_t)e->grid_cap, dx, 1.0 / dx, dt,
+                                             e->stage);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_STATE);
   k_be_residual<<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->page_group, e->stage, e->gm, nv(e, e->idv),
@@ -1008,47 +1082,50 @@ int be_state_residual(zr_engine* e)
   return ZR_OK;
 }
 
-// A p -> iap (projected when requested); the CG flags gate it when given
-int be_apply(zr_engine* e, const double* pvec, double* out, int project, const int32_t* ci, int red_slot)
+// out = A pvec (projected when requested); the CG flags gate it when given
+template <class T>
+int be_apply(zr_engine* e, const T* pvec, T* out, int project, const int32_t* ci, int red_slot)
 {
   const double dx = e->cfg.dx, dt = e->be_dt;
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
   ColliderSet cs = collider_set(e);
-  ImpCache cache{ e->cU, e->cV, e->cS };
+  T* stage = reinterpret_cast<T*>(e->stage);
   prof_mark(e, ZR_PROF_HVP);
-  k_hvp<<<e->A, 256, 0, e->stream>>>(pview(e), cache, e->order, e->group_start, e->group_page, e->page_slot, pvec,
-                                     (int64_t)e->grid_cap, 1.0 / dx, e->stage, ci);
+  k_hvp<T><<<e->A, 256, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, pvec,
+                                        (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
-  k_cg_ap<<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->page_group, e->stage, e->gm,
-                                     NodeVec3{ const_cast<double*>(pvec), (int64_t)e->grid_cap }, nv(e, out), dx, dt,
-                                     e->time, cs, project, e->partial, ci);
+  k_cg_ap<T><<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->page_group, stage, e->gm,
+                                        NodeVec<T>{ const_cast<T*>(pvec), (int64_t)e->grid_cap }, nvT(e, out), dx, dt,
+                                        e->time, cs, project, e->partial, ci);
   if (red_slot >= 0) k_reduce<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, red_slot, ci);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
 }
 
-// ConjugateGradient::solve (ConjugateGradient.h:99-147) on A ix = ir with the
-// mass-lumped preconditioner; termination is decided on the device and the
-// host polls the flag every check_interval iterations
+// ConjugateGradient::solve (ConjugateGradient.h:99-147) on A x = r with the
+// mass-lumped preconditioner in the vectors' precision, FP64 inner products;
+// termination is decided on the device and the host polls the flag every
+// check_interval iterations
+template <class V>
 int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
 {
+  using T = typename V::T;
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
   hipStream_t s = e->stream;
-  k_cg_init<<<nb, 256, 0, s>>>(nn, e->gm, nv(e, e->ir), nv(e, e->ix), nv(e, e->irc), nv(e, e->iq), nv(e, e->ip),
-                               e->partial);
+  k_cg_init<T><<<nb, 256, 0, s>>>(nn, e->gm, nv(e, e->ir), nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
+                                  nvT(e, V::p(e)), e->partial);
   k_reduce<<<1, 256, 0, s>>>(e->partial, nb, e->sc, SC_ZTR, nullptr);
   k_cg_setup<<<1, 64, 0, s>>>(e->sc, e->ci, rel, o->cg_tolerance);
   ZR_LAUNCH_CHECK(e);
   const int check = o->cg_check_interval > 0 ? o->cg_check_interval : 8;
   for (int k = 0; k < o->cg_max_iterations; ++k) {
-    int rc = be_apply(e, e->ip, e->iap, 1, e->ci, SC_PAP);
+    int rc = be_apply<T>(e, V::p(e), V::ap(e), 1, e->ci, SC_PAP);
     if (rc) return rc;
-    k_cg_update<<<nb, 256, 0, s>>>(nn, e->gm, nv(e, e->ix), nv(e, e->irc), nv(e, e->iq), nv(e, e->ip),
-                                   nv(e, e->iap), e->sc, e->partial, e->ci);
-    k_reduce<<<1, 256, 0, s>>>(e->partial, nb, e->sc, SC_ZTR_NEW, e->ci);
-    k_cg_post<<<1, 64, 0, s>>>(e->sc, e->ci);
-    k_cg_p<<<nb, 256, 0, s>>>(nn, nv(e, e->iq), nv(e, e->ip), e->sc, e->ci);
+    k_cg_update<T><<<nb, 256, 0, s>>>(nn, e->gm, nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
+                                      nvT(e, V::p(e)), nvT(e, V::ap(e)), e->sc, e->partial, e->ci);
+    k_cg_reduce_post<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci);
+    k_cg_p<T><<<nb, 256, 0, s>>>(nn, nvT(e, V::q(e)), nvT(e, V::p(e)), e->sc, e->ci);
     ZR_LAUNCH_CHECK(e);
     if ((k + 1) % check == 0 && k + 1 < o->cg_max_iterations) {
       ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1059,26 +1136,14 @@ int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
   return ZR_OK;
 }
 
-int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* st)
+template <class V>
+int be_newton(zr_engine* e, const zr_implicit_opts* o, zr_implicit_stats& local)
 {
-  if (!e->rasterised || e->mls_rasterised)
-    return fail(e, ZR_ERR_STATE, "backwardEulerStep needs the APIC particlesToGrid of this step");
-  int rc = ensure_implicit(e);
-  if (rc) return rc;
-  zr_implicit_stats local;
-  memset(&local, 0, sizeof(local));
-  e->be_dt = dt;
+  using T = typename V::T;
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
-  hipStream_t s = e->stream;
-  ColliderSet cs = collider_set(e);
-  k_be_init<<<nb, 256, 0, s>>>(nn, e->slot_page, e->gm, e->gv, nv(e, e->ivn), nv(e, e->idv), e->cfg.dx, dt,
-                               e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs);
-  ZR_HIP(e, hipMemsetAsync(e->ci, 0, CI_N * sizeof(int32_t), s));
-  ZR_LAUNCH_CHECK(e);
-  e->be_ready = true;
   // NewtonsMethod::solve (NewtonsMethod.h:42-66)
   for (int it = 0; it < o->newton_max_iterations; ++it) {
-    rc = be_state_residual(e);
+    int rc = be_state_residual<T>(e);
     if (rc) return rc;
     rc = read_scalars(e);
     if (rc) return rc;
@@ -1090,12 +1155,35 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
     }
     const double rel = o->cg_relative_tolerance > 0.0 ? o->cg_relative_tolerance
                                                       : fmin(0.5, sqrt(fmax(norm, o->newton_tolerance)));
-    rc = be_cg(e, rel, o);
+    rc = be_cg<V>(e, rel, o);
     if (rc) return rc;
-    k_axpy3<<<nb, 256, 0, s>>>(nn, nv(e, e->idv), nv(e, e->ix));
+    k_axpy3<T><<<nb, 256, 0, e->stream>>>(nn, nv(e, e->idv), nvT(e, V::x(e)));
     ZR_LAUNCH_CHECK(e);
     local.newton_iterations++;
   }
+  return ZR_OK;
+}
+
+int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* st)
+{
+  if (!e->rasterised || e->mls_rasterised)
+    return fail(e, ZR_ERR_STATE, "backwardEulerStep needs the APIC particlesToGrid of this step");
+  int rc = ensure_implicit(e, o->precision);
+  if (rc) return rc;
+  zr_implicit_stats local;
+  memset(&local, 0, sizeof(local));
+  e->be_dt = dt;
+  e->be_prec = o->precision;
+  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
+  hipStream_t s = e->stream;
+  ColliderSet cs = collider_set(e);
+  k_be_init<<<nb, 256, 0, s>>>(nn, e->slot_page, e->gm, e->gv, nv(e, e->ivn), nv(e, e->idv), e->cfg.dx, dt,
+                               e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs);
+  ZR_HIP(e, hipMemsetAsync(e->ci, 0, CI_N * sizeof(int32_t), s));
+  ZR_LAUNCH_CHECK(e);
+  e->be_ready = true;
+  rc = o->precision == ZR_FP32_REFINED ? be_newton<CgVecs32>(e, o, local) : be_newton<CgVecs64>(e, o, local);
+  if (rc) return rc;
   k_be_finish<<<nb, 256, 0, s>>>(nn, e->gm, e->gv, nv(e, e->idv), e->gvs);
   ZR_LAUNCH_CHECK(e);
   rc = read_scalars(e);
@@ -1162,7 +1250,8 @@ void zr_destroy(zr_engine* e)
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->dcount);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
-  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->cU); dfree(e->cV); dfree(e->cS);
+  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
+  dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);
   if (e->hci) (void)hipHostFree(e->hci);
   for (auto& v : e->prof_ev)
@@ -1240,6 +1329,7 @@ int zr_upload_particles(zr_engine* e, const zr_particles* p)
   ZR_HIP(e, hipMalloc(&e->touch_page, 8 * N * sizeof(uint32_t)));
   ZR_HIP(e, hipMalloc(&e->touch_slot, 8 * N * sizeof(uint32_t)));
   ZR_HIP(e, hipMalloc(&e->slot_page, 8 * N * sizeof(uint32_t)));
+  ZR_HIP(e, hipMalloc(&e->group_nslot, 8 * N * sizeof(int32_t)));
   double* tmp = nullptr;
   ZR_HIP(e, hipMalloc(&tmp, 9 * N * sizeof(double)));
   struct Item {
@@ -1357,7 +1447,8 @@ int zr_particles_to_grid_apic(zr_engine* e)
 
 static bool valid_opts(const zr_implicit_opts* o)
 {
-  return o && o->newton_max_iterations >= 0 && o->cg_max_iterations >= 0 && o->precision == ZR_FP64;
+  return o && o->newton_max_iterations >= 0 && o->cg_max_iterations >= 0 &&
+         (o->precision == ZR_FP64 || o->precision == ZR_FP32_REFINED);
 }
 
 int zr_backward_euler_step(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* st)
@@ -1422,7 +1513,7 @@ int zr_implicit_residual(zr_engine* e, const double* dv, double* r)
   ZR_HIP(e, hipStreamSynchronize(e->stream));
   int rc = upload_nodevec(e, dv, e->idv);
   if (rc) return rc;
-  rc = be_state_residual(e);
+  rc = e->be_prec == ZR_FP32_REFINED ? be_state_residual<float>(e) : be_state_residual<double>(e);
   if (rc) return rc;
   return download_nodevec(e, e->ir, r);
 }
@@ -1434,7 +1525,27 @@ int zr_implicit_multiply(zr_engine* e, const double* x, double* b)
   ZR_HIP(e, hipStreamSynchronize(e->stream));
   int rc = upload_nodevec(e, x, e->ip);
   if (rc) return rc;
-  rc = be_apply(e, e->ip, e->iap, 0, nullptr, -1);
+  if (e->be_prec == ZR_FP32_REFINED) {
+    // through the FP32 operator: cast in, apply, cast out
+    const size_t nn = (size_t)e->S * 32;
+    std::vector<double> h(3 * nn);
+    std::vector<float> hf(3 * nn);
+    for (size_t k = 0; k < 3 * nn; ++k) hf[k] = (float)x[k];
+    std::vector<float> soa(3 * nn);
+    for (size_t k = 0; k < nn; ++k)
+      for (int d = 0; d < 3; ++d) soa[d * nn + k] = hf[3 * k + d];
+    for (int d = 0; d < 3; ++d)
+      ZR_HIP(e, hipMemcpy(e->fp + d * e->grid_cap, soa.data() + d * nn, nn * sizeof(float), hipMemcpyHostToDevice));
+    rc = be_apply<float>(e, e->fp, e->fap, 0, nullptr, -1);
+    if (rc) return rc;
+    ZR_HIP(e, hipStreamSynchronize(e->stream));
+    for (int d = 0; d < 3; ++d)
+      ZR_HIP(e, hipMemcpy(soa.data() + d * nn, e->fap + d * e->grid_cap, nn * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < nn; ++k)
+      for (int d = 0; d < 3; ++d) b[3 * k + d] = (double)soa[d * nn + k];
+    return ZR_OK;
+  }
+  rc = be_apply<double>(e, e->ip, e->iap, 0, nullptr, -1);
   if (rc) return rc;
   return download_nodevec(e, e->iap, b);
 }

@@ -7,9 +7,17 @@
 // per-block partial sums in a fixed shuffle tree, then one block sums the
 // partials in index order.
 //
-// Per-particle cache written by the state kernel and read by the operator:
-// U, V (9 each, column-major) and Sigma (3) of the SVD of the Newton iterate
-// F = (I + dt grad(vn + dv)) Fn.
+// Per-particle operator data, written by the state kernel in SORTED order
+// (position start(g) + j of particle j of group g) so that the operator reads
+// it with unit stride and without the particle index, from the SVD
+// F = U diag(s) V^T of the Newton iterate F = (I + dt grad(vn + dv)) Fn:
+//   U (9), Q = Fn V (9), s (3), k = (2 mu V0, lam V0) (2),
+//   frac = x/dx - base (3) and the base cell relative to the group page.
+// With M = U^T dF V = U^T (G Fn) V = U^T G Q, the operator's particle term is
+//   V0 dP Fn^T = U H(M) V^T Fn^T = U H(M) Q^T,
+// H the fixed-corotated differential in the singular frame (scaled by V0).
+// The operator runs in T = double (plain path) or T = float (FP32 bulk); the
+// state, the Newton residual and every inner product stay FP64.
 
 enum {
   SC_NORM2 = 0,   // Newton residual sum |r|^2 / m
@@ -91,13 +99,13 @@ __device__ __forceinline__ void project_node(const ColliderSet& cs, double t, co
 }
 
 // sum of the <= 8 staged tiles overlapping node (slot page q, element e)
-template <int NF>
-__device__ __forceinline__ void sum_tiles(const double* stage, const int32_t* page_group, uint32_t qx, uint32_t qy,
-                                          uint32_t qz, int ex, int ey, int ez, double acc[NF])
+template <int NF, class T>
+__device__ __forceinline__ void sum_tiles(const T* stage, const int32_t* page_group, uint32_t qx, uint32_t qy,
+                                          uint32_t qz, int ex, int ey, int ez, T acc[NF])
 {
   constexpr int stride = NF * kTileNodes;
 #pragma unroll
-  for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+  for (int f = 0; f < NF; ++f) acc[f] = (T)0;
   for (int a = 0; a < 2; ++a) {
     if (a > (int)qx) break;
     const int tx = ex + 2 * a;
@@ -109,7 +117,7 @@ __device__ __forceinline__ void sum_tiles(const double* stage, const int32_t* pa
         if (tz >= kTileZ || c > (int)qz) break;
         const int32_t src = page_group[page_encode(qx - a, qy - b, qz - c)];
         if (src < 0) continue;
-        const double* base = stage + (size_t)src * stride + (tx * kTileY + ty) * kTileZ + tz;
+        const T* base = stage + (size_t)src * stride + (tx * kTileY + ty) * kTileZ + tz;
 #pragma unroll
         for (int f = 0; f < NF; ++f) acc[f] += base[f * kTileNodes];
       }
@@ -140,9 +148,9 @@ __device__ __forceinline__ void node_pos(const NodeCoord& c, double dx, double x
 
 // v* of the 4x6x6 footprint of group page (px,py,pz) -> LDS, summing up to two
 // node vectors (vn + dv); the 8 neighbour slots are looked up once
+template <class T, class TS>
 __device__ __forceinline__ void load_tile3(uint32_t px, uint32_t py, uint32_t pz, const int32_t* page_slot,
-                                           const double* a, const double* b, int64_t stride, double* tv,
-                                           int32_t* nslot)
+                                           const TS* a, const TS* b, int64_t stride, T* tv, int32_t* nslot)
 {
   const int tid = threadIdx.x;
   if (tid < 8) nslot[tid] = page_slot[page_encode(px + (tid >> 2), py + ((tid >> 1) & 1), pz + (tid & 1))];
@@ -152,21 +160,89 @@ __device__ __forceinline__ void load_tile3(uint32_t px, uint32_t py, uint32_t pz
     const int32_t sl = nslot[((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2)];
     const int64_t node = (int64_t)sl * 32 + elem_of(tx, ty, tz);
     for (int d = 0; d < 3; ++d)
-      tv[d * kTileNodes + t] = sl >= 0 ? a[d * stride + node] + (b ? b[d * stride + node] : 0.0) : 0.0;
+      tv[d * kTileNodes + t] = sl >= 0 ? (T)(a[d * stride + node] + (b ? b[d * stride + node] : (TS)0)) : (T)0;
   }
   __syncthreads();
 }
 
-struct ImpCache {
-  double* U;  // [9][n]
-  double* V;  // [9][n]
-  double* S;  // [3][n]
+// tile load from the group's precomputed neighbour slots (8 entries)
+template <class T, class TS>
+__device__ __forceinline__ void load_tile3_slots(const int32_t* gslot, const TS* a, int64_t stride, T* tv,
+                                                 int32_t* nslot)
+{
+  const int tid = threadIdx.x;
+  if (tid < 8) nslot[tid] = gslot[tid];
+  __syncthreads();
+  for (int t = tid; t < kTileNodes; t += blockDim.x) {
+    const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
+    const int32_t sl = nslot[((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2)];
+    const int64_t node = (int64_t)sl * 32 + elem_of(tx, ty, tz);
+    for (int d = 0; d < 3; ++d) tv[d * kTileNodes + t] = sl >= 0 ? (T)a[d * stride + node] : (T)0;
+  }
+  __syncthreads();
+}
+
+template <class T>
+struct HvpCache {
+  T* U;           // [9][stride] column-major
+  T* Q;           // [9][stride] Fn V, column-major
+  T* S;           // [3][stride] singular values
+  T* K;           // [2][stride] 2 mu V0, lam V0
+  T* frac;        // [3][stride] x/dx - base node, in [0.5, 1.5)
+  uint32_t* cell; // [stride] lx | ly << 1 | lz << 3 within the group page
+  int64_t stride;
 };
 
-// Newton state: F = (I + dt grad(vn + dv)) Fn, SVD cached, and the stress
-// term vPFnT grad w staged per group (MpmForceBase.cpp:305-344,
-// FBasedMpmForceHelper.cpp:68-91, rasterizeForceToTVStack<false>)
-__global__ __launch_bounds__(256) void k_be_state(ParticlesView P, ImpCache cache, const uint32_t* __restrict__ order,
+template <class T>
+struct NodeVec {
+  T* p;
+  int64_t stride;
+  __device__ __forceinline__ T& operator()(int d, int64_t k) const { return p[d * stride + k]; }
+};
+
+// quadratic B-spline weights from d0 = x - base (BSplines.h:55-81)
+template <class T>
+__device__ __forceinline__ void bspline_frac(T d0, T w[3], T dw[3])
+{
+  const T z = (T)1.5 - d0;
+  w[0] = (T)0.5 * z * z;
+  const T d1 = d0 - (T)1;
+  w[1] = (T)0.75 - d1 * d1;
+  const T d2 = (T)1 - d1;
+  const T zz = (T)1.5 - d2;
+  w[2] = (T)0.5 * zz * zz;
+  dw[0] = -z;
+  dw[1] = (T)-2 * d1;
+  dw[2] = zz;
+}
+
+template <class T>
+__device__ __forceinline__ T wgradT(const T w[3][3], const T dw[3][3], T inv_dx, int a, int b, int c, T gw[3])
+{
+  const T wi = w[0][a], wj = w[1][b], wk = w[2][c];
+  const T dwidxi = inv_dx * dw[0][a];
+  const T wij = wi * wj;
+  const T dwijdxi = dwidxi * wj;
+  const T dwijdxj = wi * inv_dx * dw[1][b];
+  gw[0] = dwijdxi * wk;
+  gw[1] = dwijdxj * wk;
+  gw[2] = wij * inv_dx * dw[2][c];
+  return wij * wk;
+}
+
+// keeps the compiler from carrying values computed before this point across
+// it (the weights are recomputed after the per-particle math instead of
+// holding all 27 gradients live)
+__device__ __forceinline__ void opaque(double& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(float& v) { asm volatile("" : "+v"(v)); }
+
+// Newton state: F = (I + dt grad(vn + dv)) Fn (FP64, with the SVD), the
+// stress term vPFnT grad w staged per group (MpmForceBase.cpp:305-344,
+// FBasedMpmForceHelper.cpp:68-91, rasterizeForceToTVStack<false>), and the
+// operator data of every particle in sorted order, in precision T
+template <class T>
+__global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> cache,
+                                                  const uint32_t* __restrict__ order,
                                                   const uint32_t* __restrict__ group_start,
                                                   const uint32_t* __restrict__ group_page,
                                                   const int32_t* __restrict__ page_slot, const double* __restrict__ vn,
@@ -187,18 +263,25 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, ImpCache cach
   const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
   double* wt = tile + (tid >> 6) * kImpStage;
   const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
-  const int64_t S = P.stride;
+  const int64_t S = P.stride, CS = cache.stride;
   for (uint32_t j0 = 0; j0 < n; j0 += 256) {
     const uint32_t j = j0 + perm;
     if (j >= n) continue;
     const uint32_t i = order[start + j];
+    const uint32_t ks = start + j;
     const double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
-    double w[3][3], dw[3][3];
+    double w[3][3], dw[3][3], d0[3];
     int b[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) b[d] = bspline_wdw(xp[d] * inv_dx, w[d], dw[d]);
+    for (int d = 0; d < 3; ++d) {
+      const double xi = xp[d] * inv_dx;
+      b[d] = int_floor(xi - 0.5);
+      d0[d] = xi - (double)b[d];
+      bspline_frac(d0[d], w[d], dw[d]);
+    }
     const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
     if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
+    const int t0 = (lx * kTileY + ly) * kTileZ + lz;
     double G[3][3] = { { 0.0 } };
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -207,8 +290,8 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, ImpCache cach
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           double gw[3];
-          wgrad(w, dw, inv_dx, a, bb, c, gw);
-          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+          wgradT(w, dw, inv_dx, a, bb, c, gw);
+          const int t = t0 + (a * kTileY + bb) * kTileZ + c;
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
             const double vv = tv[r * kTileNodes + t];
@@ -234,17 +317,23 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, ImpCache cach
     M3<double> u, v;
     double s[3];
     svd3(F, u, s, v);
+    const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        cache.U[(c * 3 + r) * S + i] = u.a[r][c];
-        cache.V[(c * 3 + r) * S + i] = v.a[r][c];
+        cache.U[(c * 3 + r) * CS + ks] = (T)u.a[r][c];
+        cache.Q[(c * 3 + r) * CS + ks] = (T)(Fn.a[r][0] * v.a[0][c] + Fn.a[r][1] * v.a[1][c] + Fn.a[r][2] * v.a[2][c]);
       }
 #pragma unroll
-    for (int d = 0; d < 3; ++d) cache.S[d * S + i] = s[d];
+    for (int d = 0; d < 3; ++d) {
+      cache.S[d * CS + ks] = (T)s[d];
+      cache.frac[d * CS + ks] = (T)d0[d];
+    }
+    cache.K[ks] = (T)(2.0 * mu * vol);
+    cache.K[CS + ks] = (T)(lam * vol);
+    cache.cell[ks] = (uint32_t)(lx | (ly << 1) | (lz << 3));
     // P = U diag(2 mu (s - 1) + lam (J - 1) cof(s)) V^T
-    const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i];
     const double J = s[0] * s[1] * s[2];
     const double cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
     double ph[3];
@@ -254,7 +343,8 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, ImpCache cach
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) Pm[r][c] = u.a[r][0] * ph[0] * v.a[c][0] + u.a[r][1] * ph[1] * v.a[c][1] + u.a[r][2] * ph[2] * v.a[c][2];
+      for (int c = 0; c < 3; ++c)
+        Pm[r][c] = u.a[r][0] * ph[0] * v.a[c][0] + u.a[r][1] * ph[1] * v.a[c][1] + u.a[r][2] * ph[2] * v.a[c][2];
     double vp[3][3];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -264,12 +354,19 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, ImpCache cach
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        opaque(w[a][d]);
+        opaque(dw[a][d]);
+      }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
       for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           double gw[3];
-          wgrad(w, dw, inv_dx, a, bb, c, gw);
-          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+          wgradT(w, dw, inv_dx, a, bb, c, gw);
+          const int t = t0 + (a * kTileY + bb) * kTileZ + c;
 #pragma unroll
           for (int r = 0; r < 3; ++r)
             atomicAdd(&wt[r * kTileNodes + t], vp[r][0] * gw[0] + vp[r][1] * gw[1] + vp[r][2] * gw[2]);
@@ -281,143 +378,224 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, ImpCache cach
     out[k] = tile[k] + tile[kImpStage + k] + tile[2 * kImpStage + k] + tile[3 * kImpStage + k];
 }
 
-// Hessian-vector product, particle part: ds = V0 dP(grad p Fn) Fn^T staged
-// as ds grad w (MpmForceBase.cpp:258-302, FBasedMpmForceHelper.cpp:132-155)
-__global__ __launch_bounds__(256) void k_hvp(ParticlesView P, ImpCache cache, const uint32_t* __restrict__ order,
-                                             const uint32_t* __restrict__ group_start,
-                                             const uint32_t* __restrict__ group_page,
-                                             const int32_t* __restrict__ page_slot, const double* __restrict__ pvec,
-                                             int64_t gstride, double inv_dx, double* __restrict__ stage,
+constexpr int kCells = 32;    // 2 x 4 x 4 cells of a page
+constexpr int kAB = 9;        // (a, b) stencil offsets; c is handled inside a task
+
+template <class T>
+struct HvpChunk;
+template <>
+struct HvpChunk<float> {
+  static constexpr int value = 256;
+};
+template <>
+struct HvpChunk<double> {
+  static constexpr int value = 128;
+};
+
+// Hessian-vector product, particle part: ds = V0 dP(grad p Fn) Fn^T rasterised
+// as ds grad w (MpmForceBase.cpp:258-302, FBasedMpmForceHelper.cpp:132-155),
+// in precision T from the sorted operator data.  No atomics, fixed order:
+//   phase 1  thread per particle: separable gather of grad p from the LDS
+//            tile, ds from the singular-frame differential; ds and the 1-D
+//            weights go to LDS
+//   phase 2  task per (cell, a, b): sums ds grad w over the cell's particles
+//            in index order for the three c offsets (separable form
+//            gw = (dw_a w_b w_c, w_a dw_b w_c, w_a w_b dw_c) / dx)
+//   phase 3  thread per tile node: sums its <= 27 partials into the staged tile
+template <class T>
+__global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* __restrict__ group_start,
+                                             const int32_t* __restrict__ group_nslot, const T* __restrict__ pvec,
+                                             int64_t gstride, T inv_dx, T* __restrict__ stage,
                                              const int32_t* __restrict__ ci)
 {
+  constexpr int CH = HvpChunk<T>::value;
   if (ci && ci[CI_DONE]) return;
-  __shared__ double tile[4 * kImpStage];
-  __shared__ double tv[3 * kTileNodes];
+  __shared__ T tv[3 * kTileNodes];
+  __shared__ T pds[27][CH];                      // ds (9, row-major), w (3x3), dw (3x3)
+  __shared__ T contrib[3][kCells * kAB * 3];
+  __shared__ int cbeg[kCells], cend[kCells];
   __shared__ int32_t nslot[8];
   const int tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
-  for (int k = tid; k < 4 * kImpStage; k += 256) tile[k] = 0.0;
+  for (int k = tid; k < 3 * kCells * kAB * 3; k += 256) (&contrib[0][0])[k] = (T)0;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
-  uint32_t px, py, pz;
-  page_decode(group_page[g], px, py, pz);
-  load_tile3(px, py, pz, page_slot, pvec, nullptr, gstride, tv, nslot);
-  const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
-  double* wt = tile + (tid >> 6) * kImpStage;
-  const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
-  const int64_t S = P.stride;
-  for (uint32_t j0 = 0; j0 < n; j0 += 256) {
-    const uint32_t j = j0 + perm;
-    if (j >= n) continue;
-    const uint32_t i = order[start + j];
-    const double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
-    double w[3][3], dw[3][3];
-    int b[3];
+  const int64_t CS = cache.stride;
+  // the first chunk's per-particle loads are issued before the tile barrier
+  // so that their latency overlaps the tile gather
+  T fr0[3] = { (T)1, (T)1, (T)1 };
+  uint32_t cell0 = 0;
+  if ((uint32_t)tid < n && tid < CH) {
+    const uint32_t ks = start + tid;
+    cell0 = cache.cell[ks];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) b[d] = bspline_wdw(xp[d] * inv_dx, w[d], dw[d]);
-    const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
-    if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
-    double G[3][3] = { { 0.0 } };
+    for (int d = 0; d < 3; ++d) fr0[d] = cache.frac[d * CS + ks];
+  }
+  load_tile3_slots(group_nslot + 8 * (size_t)g, pvec, gstride, tv, nslot);
+  for (uint32_t c0 = 0; c0 < n; c0 += CH) {
+    const uint32_t m = n - c0 < (uint32_t)CH ? n - c0 : (uint32_t)CH;
+    if (tid < kCells) {
+      cbeg[tid] = CH;
+      cend[tid] = 0;
+    }
+    __syncthreads();
+    if ((uint32_t)tid < m) {
+      const uint32_t ks = start + c0 + tid;
+      const uint32_t cell = c0 == 0 ? cell0 : cache.cell[ks];
+      const int t0 = ((int)(cell & 1u) * kTileY + (int)((cell >> 1) & 3u)) * kTileZ + (int)((cell >> 3) & 3u);
+      T w[3][3], dw[3][3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          double gw[3];
-          wgrad(w, dw, inv_dx, a, bb, c, gw);
-          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
-#pragma unroll
-          for (int r = 0; r < 3; ++r) {
-            const double vv = tv[r * kTileNodes + t];
-            G[r][0] += vv * gw[0];
-            G[r][1] += vv * gw[1];
-            G[r][2] += vv * gw[2];
-          }
-        }
-    double Fn[3][3], u[3][3], v[3][3], s[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) bspline_frac(c0 == 0 ? fr0[d] : cache.frac[d * CS + ks], w[d], dw[d]);
+      // separable gather: G[r][0] = sum dw_a w_b w_c p, G[r][1] = sum w_a dw_b w_c p,
+      // G[r][2] = sum w_a w_b dw_c p (times 1/dx)
+      T G[3][3];
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        Fn[r][c] = P.F[(c * 3 + r) * S + i];
-        u[r][c] = cache.U[(c * 3 + r) * S + i];
-        v[r][c] = cache.V[(c * 3 + r) * S + i];
+        T U[3], V[3], W[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          U[a] = (T)0;
+          V[a] = (T)0;
+          W[a] = (T)0;
+#pragma unroll
+          for (int bb = 0; bb < 3; ++bb) {
+            const T* row = tv + r * kTileNodes + t0 + (a * kTileY + bb) * kTileZ;
+            const T p0 = row[0], p1 = row[1], p2 = row[2];
+            const T sc = p0 * w[2][0] + p1 * w[2][1] + p2 * w[2][2];
+            const T tc = p0 * dw[2][0] + p1 * dw[2][1] + p2 * dw[2][2];
+            U[a] += sc * w[1][bb];
+            V[a] += sc * dw[1][bb];
+            W[a] += tc * w[1][bb];
+          }
+        }
+        G[r][0] = inv_dx * (dw[0][0] * U[0] + dw[0][1] * U[1] + dw[0][2] * U[2]);
+        G[r][1] = inv_dx * (w[0][0] * V[0] + w[0][1] * V[1] + w[0][2] * V[2]);
+        G[r][2] = inv_dx * (w[0][0] * W[0] + w[0][1] * W[1] + w[0][2] * W[2]);
       }
+      T u[3][3], q[3][3], s[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) s[d] = cache.S[d * S + i];
-    // dF = G Fn ; M = U^T dF V
-    double dF[3][3], T1[3][3], M[3][3];
+      for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 3; ++r) {
+          u[r][c] = cache.U[(c * 3 + r) * CS + ks];
+          q[r][c] = cache.Q[(c * 3 + r) * CS + ks];
+        }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) dF[r][c] = G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c];
+      for (int d = 0; d < 3; ++d) s[d] = cache.S[d * CS + ks];
+      const T k2mu = cache.K[ks], klam = cache.K[CS + ks];
+      // M = U^T (G Q)
+      T T1[3][3], M[3][3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) T1[r][c] = u[0][r] * dF[0][c] + u[1][r] * dF[1][c] + u[2][r] * dF[2][c];
+        for (int c = 0; c < 3; ++c) T1[r][c] = G[r][0] * q[0][c] + G[r][1] * q[1][c] + G[r][2] * q[2][c];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) M[r][c] = T1[r][0] * v[0][c] + T1[r][1] * v[1][c] + T1[r][2] * v[2][c];
-    const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i];
-    const double J = s[0] * s[1] * s[2];
-    const double cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
-    const double cm = cof[0] * M[0][0] + cof[1] * M[1][1] + cof[2] * M[2][2];
-    // skew part of U^T dR V
-    double W01 = 0.0, W02 = 0.0, W12 = 0.0;
-    {
-      double d01 = s[0] + s[1], d02 = s[0] + s[2], d12 = s[1] + s[2];
-      if (fabs(d01) < 1e-12) d01 = d01 < 0.0 ? -1e-12 : 1e-12;
-      if (fabs(d02) < 1e-12) d02 = d02 < 0.0 ? -1e-12 : 1e-12;
-      if (fabs(d12) < 1e-12) d12 = d12 < 0.0 ? -1e-12 : 1e-12;
-      W01 = (M[0][1] - M[1][0]) / d01;
-      W02 = (M[0][2] - M[2][0]) / d02;
-      W12 = (M[1][2] - M[2][1]) / d12;
+        for (int c = 0; c < 3; ++c) M[r][c] = u[0][r] * T1[0][c] + u[1][r] * T1[1][c] + u[2][r] * T1[2][c];
+      const T J = s[0] * s[1] * s[2];
+      const T cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
+      const T cm = cof[0] * M[0][0] + cof[1] * M[1][1] + cof[2] * M[2][2];
+      const T eps = (T)1e-12;
+      T d01 = s[0] + s[1], d02 = s[0] + s[2], d12 = s[1] + s[2];
+      if (fabs(d01) < eps) d01 = d01 < (T)0 ? -eps : eps;
+      if (fabs(d02) < eps) d02 = d02 < (T)0 ? -eps : eps;
+      if (fabs(d12) < eps) d12 = d12 < (T)0 ? -eps : eps;
+      const T W01 = (M[0][1] - M[1][0]) / d01;
+      const T W02 = (M[0][2] - M[2][0]) / d02;
+      const T W12 = (M[1][2] - M[2][1]) / d12;
+      const T lj = klam * (J - (T)1);
+      T H[3][3];
+      H[0][0] = k2mu * M[0][0] + klam * cm * cof[0] + lj * (s[2] * M[1][1] + s[1] * M[2][2]);
+      H[1][1] = k2mu * M[1][1] + klam * cm * cof[1] + lj * (s[2] * M[0][0] + s[0] * M[2][2]);
+      H[2][2] = k2mu * M[2][2] + klam * cm * cof[2] + lj * (s[1] * M[0][0] + s[0] * M[1][1]);
+      H[0][1] = k2mu * (M[0][1] - W01) - lj * s[2] * M[1][0];
+      H[1][0] = k2mu * (M[1][0] + W01) - lj * s[2] * M[0][1];
+      H[0][2] = k2mu * (M[0][2] - W02) - lj * s[1] * M[2][0];
+      H[2][0] = k2mu * (M[2][0] + W02) - lj * s[1] * M[0][2];
+      H[1][2] = k2mu * (M[1][2] - W12) - lj * s[0] * M[2][1];
+      H[2][1] = k2mu * (M[2][1] + W12) - lj * s[0] * M[1][2];
+      // ds = U H Q^T
+      T T2[3][3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) T2[r][c] = u[r][0] * H[0][c] + u[r][1] * H[1][c] + u[r][2] * H[2][c];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pds[r * 3 + c][tid] = T2[r][0] * q[c][0] + T2[r][1] * q[c][1] + T2[r][2] * q[c][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          pds[9 + 3 * d + a][tid] = w[d][a];
+          pds[18 + 3 * d + a][tid] = dw[d][a];
+        }
+      const int e = (int)cell;
+      atomicMin(&cbeg[e], tid);
+      atomicMax(&cend[e], tid + 1);
     }
-    const double lj = lam * (J - 1.0);
-    double H[3][3];
-    H[0][0] = 2.0 * mu * M[0][0] + lam * cm * cof[0] + lj * (s[2] * M[1][1] + s[1] * M[2][2]);
-    H[1][1] = 2.0 * mu * M[1][1] + lam * cm * cof[1] + lj * (s[2] * M[0][0] + s[0] * M[2][2]);
-    H[2][2] = 2.0 * mu * M[2][2] + lam * cm * cof[2] + lj * (s[1] * M[0][0] + s[0] * M[1][1]);
-    H[0][1] = 2.0 * mu * (M[0][1] - W01) - lj * s[2] * M[1][0];
-    H[1][0] = 2.0 * mu * (M[1][0] + W01) - lj * s[2] * M[0][1];
-    H[0][2] = 2.0 * mu * (M[0][2] - W02) - lj * s[1] * M[2][0];
-    H[2][0] = 2.0 * mu * (M[2][0] + W02) - lj * s[1] * M[0][2];
-    H[1][2] = 2.0 * mu * (M[1][2] - W12) - lj * s[0] * M[2][1];
-    H[2][1] = 2.0 * mu * (M[2][1] + W12) - lj * s[0] * M[1][2];
-    // dP = U H V^T ; ds = vol dP Fn^T
-    double T2[3][3], dP[3][3], ds[3][3];
+    __syncthreads();
+    for (int task = tid; task < kCells * kAB; task += 256) {
+      const int e = task / kAB, ab = task - e * kAB;
+      const int a = ab / 3, bb = ab - 3 * a;
+      T acc[3][3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) T2[r][c] = u[r][0] * H[0][c] + u[r][1] * H[1][c] + u[r][2] * H[2][c];
+        for (int r = 0; r < 3; ++r) acc[c][r] = (T)0;
+      const int jb = cbeg[e], je = cend[e];
+      for (int jj = jb; jj < je; ++jj) {
+        const T wa = pds[9 + a][jj], dwa = pds[18 + a][jj];
+        const T wb = pds[12 + bb][jj], dwb = pds[21 + bb][jj];
+        const T x0 = inv_dx * dwa * wb, x1 = inv_dx * wa * dwb, x2 = inv_dx * wa * wb;
+        T A[3], B[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) dP[r][c] = T2[r][0] * v[c][0] + T2[r][1] * v[c][1] + T2[r][2] * v[c][2];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) ds[r][c] = vol * (dP[r][0] * Fn[c][0] + dP[r][1] * Fn[c][1] + dP[r][2] * Fn[c][2]);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int bb = 0; bb < 3; ++bb)
+        for (int r = 0; r < 3; ++r) {
+          A[r] = pds[3 * r][jj] * x0 + pds[3 * r + 1][jj] * x1;
+          B[r] = pds[3 * r + 2][jj] * x2;
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          double gw[3];
-          wgrad(w, dw, inv_dx, a, bb, c, gw);
-          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+          const T wc = pds[15 + c][jj], dwc = pds[24 + c][jj];
 #pragma unroll
-          for (int r = 0; r < 3; ++r)
-            atomicAdd(&wt[r * kTileNodes + t], ds[r][0] * gw[0] + ds[r][1] * gw[1] + ds[r][2] * gw[2]);
+          for (int r = 0; r < 3; ++r) acc[c][r] += A[r] * wc + B[r] * dwc;
         }
+      }
+      if (jb < je) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) contrib[r][task * 3 + c] += acc[c][r];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  double* out = stage + (size_t)g * kImpStage;
-  for (int k = tid; k < kImpStage; k += 256)
-    out[k] = tile[k] + tile[kImpStage + k] + tile[2 * kImpStage + k] + tile[3 * kImpStage + k];
+  T* out = stage + (size_t)g * kImpStage;
+  for (int t = tid; t < kTileNodes; t += 256) {
+    const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
+    T acc0 = (T)0, acc1 = (T)0, acc2 = (T)0;
+    for (int a = 0; a < 3; ++a) {
+      const int cx = tx - a;
+      if (cx < 0 || cx > 1) continue;
+      for (int bb = 0; bb < 3; ++bb) {
+        const int cy = ty - bb;
+        if (cy < 0 || cy > 3) continue;
+        for (int c = 0; c < 3; ++c) {
+          const int cz = tz - c;
+          if (cz < 0 || cz > 3) continue;
+          const int idx = ((cx | (cy << 1) | (cz << 3)) * kAB + a * 3 + bb) * 3 + c;
+          acc0 += contrib[0][idx];
+          acc1 += contrib[1][idx];
+          acc2 += contrib[2][idx];
+        }
+      }
+    }
+    out[t] = acc0;
+    out[kTileNodes + t] = acc1;
+    out[2 * kTileNodes + t] = acc2;
+  }
 }
 
 // buildInitialDvAndVnForNewton (MpmSimulationBase.cpp:1105-1136)
@@ -491,8 +669,10 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
 }
 
 // CG start (ConjugateGradient.h:105-116): x = 0, r = b, z = r / m, p = z
-__global__ __launch_bounds__(256) void k_cg_init(uint32_t nn, const double* __restrict__ gm, NodeVec3 b, NodeVec3 x,
-                                                 NodeVec3 r, NodeVec3 q, NodeVec3 p, double* __restrict__ partial)
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_init(uint32_t nn, const double* __restrict__ gm, NodeVec3 b, NodeVec<T> x,
+                                                 NodeVec<T> r, NodeVec<T> q, NodeVec<T> p,
+                                                 double* __restrict__ partial)
 {
   __shared__ double smem[4];
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
@@ -500,13 +680,13 @@ __global__ __launch_bounds__(256) void k_cg_init(uint32_t nn, const double* __re
   if (k < nn) {
     const double m = gm[k];
     for (int d = 0; d < 3; ++d) {
-      const double rv = b(d, k);
-      const double qv = m != 0.0 ? rv / m : 0.0;
-      x(d, k) = 0.0;
+      const T rv = (T)b(d, k);
+      const T qv = m != 0.0 ? rv / (T)m : (T)0;
+      x(d, k) = (T)0;
       r(d, k) = rv;
       q(d, k) = qv;
       p(d, k) = qv;
-      contrib += rv * qv;
+      contrib += (double)rv * (double)qv;
     }
   }
   const double s = block_sum256(contrib, smem);
@@ -524,12 +704,12 @@ __global__ void k_cg_setup(double* sc, int32_t* ci, double rel, double tol)
 }
 
 // Ap = M p + dt^2 sum ds grad w, projected; partial p^T A p (BackwardEuler.h:322-351)
+template <class T>
 __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __restrict__ slot_page,
-                                               const int32_t* __restrict__ page_group,
-                                               const double* __restrict__ stage, const double* __restrict__ gm,
-                                               NodeVec3 p, NodeVec3 ap, double dx, double dt, double time,
-                                               ColliderSet cs, int project, double* __restrict__ partial,
-                                               const int32_t* ci)
+                                               const int32_t* __restrict__ page_group, const T* __restrict__ stage,
+                                               const double* __restrict__ gm, NodeVec<T> p, NodeVec<T> ap, double dx,
+                                               double dt, double time, ColliderSet cs, int project,
+                                               double* __restrict__ partial, const int32_t* ci)
 {
   if (ci && ci[CI_DONE]) return;
   __shared__ double smem[4];
@@ -537,21 +717,23 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
   double contrib = 0.0;
   if (k < nn) {
     const double m = gm[k];
-    double av[3] = { 0.0, 0.0, 0.0 };
+    T av[3] = { (T)0, (T)0, (T)0 };
     if (m != 0.0) {
       const NodeCoord nc = node_coord(slot_page, k);
-      double f[kImpFields];
+      T f[kImpFields];
       sum_tiles<kImpFields>(stage, page_group, nc.qx, nc.qy, nc.qz, nc.ex, nc.ey, nc.ez, f);
-      for (int d = 0; d < 3; ++d) av[d] = m * p(d, k) + dt * dt * f[d];
+      const T dt2 = (T)(dt * dt);
+      for (int d = 0; d < 3; ++d) av[d] = (T)m * p(d, k) + dt2 * f[d];
       if (project) {
-        double xi[3];
+        double xi[3], a64[3] = { (double)av[0], (double)av[1], (double)av[2] };
         node_pos(nc, dx, xi);
-        project_node(cs, time, xi, av);
+        project_node(cs, time, xi, a64);
+        for (int d = 0; d < 3; ++d) av[d] = (T)a64[d];
       }
     }
     for (int d = 0; d < 3; ++d) {
       ap(d, k) = av[d];
-      contrib += av[d] * p(d, k);
+      contrib += (double)av[d] * (double)p(d, k);
     }
   }
   const double s = block_sum256(contrib, smem);
@@ -559,55 +741,68 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
 }
 
 // x += alpha p, r -= alpha Ap, z = r / m ; partial z^T r
-__global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __restrict__ gm, NodeVec3 x, NodeVec3 r,
-                                                   NodeVec3 q, NodeVec3 p, NodeVec3 ap, const double* __restrict__ sc,
-                                                   double* __restrict__ partial, const int32_t* ci)
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __restrict__ gm, NodeVec<T> x,
+                                                   NodeVec<T> r, NodeVec<T> q, NodeVec<T> p, NodeVec<T> ap,
+                                                   const double* __restrict__ sc, double* __restrict__ partial,
+                                                   const int32_t* ci)
 {
   if (ci[CI_DONE]) return;
   __shared__ double smem[4];
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   double contrib = 0.0;
   if (k < nn) {
-    const double alpha = sc[SC_ZTR] / sc[SC_PAP];
+    const T alpha = (T)(sc[SC_ZTR] / sc[SC_PAP]);
     const double m = gm[k];
     for (int d = 0; d < 3; ++d) {
       x(d, k) += p(d, k) * alpha;
-      const double rv = r(d, k) - ap(d, k) * alpha;
+      const T rv = r(d, k) - ap(d, k) * alpha;
       r(d, k) = rv;
-      const double qv = m != 0.0 ? rv / m : 0.0;
+      const T qv = m != 0.0 ? rv / (T)m : (T)0;
       q(d, k) = qv;
-      contrib += qv * rv;
+      contrib += (double)qv * (double)rv;
     }
   }
   const double s = block_sum256(contrib, smem);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ void k_cg_post(double* sc, int32_t* ci)
+// z^T r reduction fused with the scalar update: beta, termination test
+// (ConjugateGradient.h:134-143)
+__global__ __launch_bounds__(256) void k_cg_reduce_post(const double* __restrict__ partial, uint32_t nb, double* sc,
+                                                        int32_t* ci)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || ci[CI_DONE]) return;
-  const double znew = sc[SC_ZTR_NEW];
-  sc[SC_BETA] = znew / sc[SC_ZTR];
-  sc[SC_ZTR] = znew;
-  ci[CI_ITERS] += 1;
-  if (sqrt(znew) < sc[SC_LOCAL_TOL]) ci[CI_DONE] = 1;
+  __shared__ double smem[4];
+  if (ci[CI_DONE]) return;
+  double acc = 0.0;
+  for (uint32_t b = threadIdx.x; b < nb; b += 256) acc += partial[b];
+  const double znew = block_sum256(acc, smem);
+  if (threadIdx.x == 0) {
+    sc[SC_BETA] = znew / sc[SC_ZTR];
+    sc[SC_ZTR] = znew;
+    ci[CI_ITERS] += 1;
+    if (sqrt(znew) < sc[SC_LOCAL_TOL]) ci[CI_DONE] = 1;
+  }
 }
 
-__global__ __launch_bounds__(256) void k_cg_p(uint32_t nn, NodeVec3 q, NodeVec3 p, const double* __restrict__ sc,
+template <class T>
+__global__ __launch_bounds__(256) void k_cg_p(uint32_t nn, NodeVec<T> q, NodeVec<T> p, const double* __restrict__ sc,
                                               const int32_t* ci)
 {
   if (ci[CI_DONE]) return;
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   if (k >= nn) return;
-  const double beta = sc[SC_BETA];
+  const T beta = (T)sc[SC_BETA];
   for (int d = 0; d < 3; ++d) p(d, k) = q(d, k) + beta * p(d, k);
 }
 
-__global__ __launch_bounds__(256) void k_axpy3(uint32_t nn, NodeVec3 y, NodeVec3 x)
+// Newton update dv += step (the FP64 outer iteration refines the FP32 solve)
+template <class T>
+__global__ __launch_bounds__(256) void k_axpy3(uint32_t nn, NodeVec3 y, NodeVec<T> x)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   if (k >= nn) return;
-  for (int d = 0; d < 3; ++d) y(d, k) += x(d, k);
+  for (int d = 0; d < 3; ++d) y(d, k) += (double)x(d, k);
 }
 
 // constructNewVelocityFromNewtonResult (MpmSimulationBase.cpp:867-877)

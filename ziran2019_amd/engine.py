@@ -311,6 +311,6 @@ class MpmEngine:
             raise ZrError(f"bad dump {what}")
         out = np.zeros(size // np.dtype(_DUMP_DTYPE[what]).itemsize, dtype=_DUMP_DTYPE[what])
         self._check(self.L.zr_debug_dump(self.h, code, out.ctypes.data_as(ctypes.c_void_p), size), "zr_debug_dump")
-        if what in ("grid_v", "grid_f", "grid_vstar"):
+        if what in ("grid_v", "grid_f", "grid_vstar", "dv"):
             out = out.reshape(-1, 3)
         return out

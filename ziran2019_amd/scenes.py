@@ -132,3 +132,66 @@ def cube_scene_small(seed=7, ncells=6, lo_cell=20, dx=1.0 / 64, E=1e4, v0=(0.3, 
     w = np.array([0.0, spin, 0.5 * spin])
     s["V"] = np.asarray(v0)[None, :] + np.cross(w[None, :], r)
     return s
+
+
+def _normal01(counter, seed):
+    """Standard normals by Box-Muller on the splitmix64 stream."""
+    c = np.asarray(counter, dtype=np.uint64)
+    u1 = uniform01(c * np.uint64(2), seed)
+    u2 = uniform01(c * np.uint64(2) + np.uint64(1), seed)
+    u1 = np.maximum(u1, 2.0 ** -53)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def boxes_scene(dx=1.0 / 256, nboxes=64, side_lo=0.025, side_hi=0.075, ppc=8, logE_mu=np.log(5e4),
+                logE_sigma=0.5, nu=0.3, rho=1000.0, seed=20190702, v_scale=0.5, grad_scale=2.0):
+    """Implicit workload: ``nboxes`` non-overlapping seeded boxes with sides
+    U[side_lo, side_hi] (cell-aligned), Young's modulus lognormal(logE_mu,
+    logE_sigma) per box, ``ppc`` particles per cell.  Each box starts with a
+    seeded affine velocity v = v0 + L (x - c) (APIC C = L) so that the first
+    implicit step has a non-trivial elastic solve.  Defaults give about 1.3M
+    particles at dx = 1/256 with 8 ppc."""
+    rng_ctr = 0
+    boxes = []
+    lo_cell = int(np.ceil(0.12 / dx))
+    hi_cell = int(np.floor(0.88 / dx))
+    gap = 3
+    attempt = 0
+    while len(boxes) < nboxes:
+        u = uniform01(np.arange(rng_ctr, rng_ctr + 6, dtype=np.uint64), seed)
+        rng_ctr += 6
+        attempt += 1
+        if attempt > 100000:
+            raise RuntimeError("could not place the boxes")
+        side = np.round((side_lo + (side_hi - side_lo) * u[:3]) / dx).astype(np.int64)
+        side = np.maximum(side, 2)
+        lo = (lo_cell + np.floor(u[3:] * (hi_cell - lo_cell - side))).astype(np.int64)
+        hi = lo + side
+        if any(np.all(lo < b[1] + gap) and np.all(b[0] < hi + gap) for b in boxes):
+            continue
+        boxes.append((lo, hi))
+    Xs, mus, lams, Vs, Cs = [], [], [], [], []
+    for b, (lo, hi) in enumerate(boxes):
+        r = [np.arange(lo[d], hi[d]) for d in range(3)]
+        cells = np.stack(np.meshgrid(*r, indexing="ij"), axis=-1).reshape(-1, 3)
+        X = _particles_in_cells(cells, ppc, dx, seed, 1 + b)
+        z = _normal01(np.arange(1 + 16 * b, 1 + 16 * b + 13, dtype=np.uint64), seed + 1)
+        E = np.exp(logE_mu + logE_sigma * z[0])
+        mu, lam = lame(E, nu)
+        v0 = v_scale * z[1:4]
+        L = grad_scale * z[4:13].reshape(3, 3)
+        c = (lo + hi) * 0.5 * dx
+        n = len(X)
+        Xs.append(X)
+        mus.append(np.full(n, mu))
+        lams.append(np.full(n, lam))
+        Vs.append(v0[None, :] + (X - c[None, :]) @ L.T)
+        Cs.append(np.tile(L.T.reshape(1, 9), (n, 1)))  # column-major of L
+    X = np.concatenate(Xs)
+    perm = morton_order(X, dx)
+    out = _finish(X[perm], dx, rho, ppc, np.concatenate(mus)[perm], np.concatenate(lams)[perm], order=False)
+    out["V"] = np.ascontiguousarray(np.concatenate(Vs)[perm])
+    out["C"] = np.ascontiguousarray(np.concatenate(Cs)[perm])
+    out.update(dx=dx, gravity=(0.0, -9.8, 0.0), boxes=boxes,
+               colliders=[dict(shape=0, type=1, origin=(0.0, 0.1, 0.0), normal=(0.0, 1.0, 0.0))])
+    return out
