@@ -306,6 +306,22 @@ __global__ void k_group_slots(uint32_t ntouch, const uint32_t* touch_page, const
   group_nslot[t] = page_slot[touch_page[t]];
 }
 
+// the <= 8 groups whose staged tiles overlap each slot page (o = 4a + 2b + c
+// for the source page q - (a, b, c); -1 when absent)
+__global__ void k_slot_sources(uint32_t nslots, const uint32_t* slot_page, const int32_t* page_group,
+                               int32_t* slot_src)
+{
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8u * nslots) return;
+  const uint32_t s = t >> 3, o = t & 7u;
+  const uint32_t a = o >> 2, b = (o >> 1) & 1u, c = o & 1u;
+  uint32_t qx, qy, qz;
+  page_decode(slot_page[s], qx, qy, qz);
+  int32_t src = -1;
+  if (a <= qx && b <= qy && c <= qz) src = page_group[page_encode(qx - a, qy - b, qz - c)];
+  slot_src[t] = src;
+}
+
 // ---------------------------------------------------------------------------
 // P2G, MLS with force (MpmSimulationBase.cpp:488-561) with the fixed-corotated
 // Kirchhoff stress of FBasedMpmForceHelper::updateState fused in
@@ -427,7 +443,7 @@ enum { GRID_COMBINE = 0, GRID_COMBINE_UPDATE = 1, GRID_UPDATE = 2 };
 
 template <int NF>
 __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const uint32_t* __restrict__ slot_page,
-                                              const int32_t* __restrict__ page_group,
+                                              const int32_t* __restrict__ slot_src,
                                               const double* __restrict__ stage, GridView G, double dx, double dt,
                                               double gx, double gy, double gz, double time, ColliderSet cs)
 {
@@ -444,22 +460,17 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
 #pragma unroll
     for (int q = 0; q < kP2GFields; ++q) acc[q] = 0.0;
     constexpr int SS = NF * kTileNodes;
-    for (int a = 0; a < 2; ++a) {
-      if (a > (int)qx) break;
-      const int tx = ex + 2 * a;
-      for (int b = 0; b < 2; ++b) {
-        const int ty = ey + 4 * b;
-        if (ty >= kTileY || b > (int)qy) break;
-        for (int c = 0; c < 2; ++c) {
-          const int tz = ez + 4 * c;
-          if (tz >= kTileZ || c > (int)qz) break;
-          const int32_t src = page_group[page_encode(qx - a, qy - b, qz - c)];
-          if (src < 0) continue;
-          const double* base = stage + (size_t)src * SS + (tx * kTileY + ty) * kTileZ + tz;
+    const int32_t* src8 = slot_src + 8 * (size_t)s;
 #pragma unroll
-          for (int q = 0; q < NF; ++q) acc[q] += base[q * kTileNodes];
-        }
-      }
+    for (int o = 0; o < 8; ++o) {
+      const int a = o >> 2, b = (o >> 1) & 1, c = o & 1;
+      const int tx = ex + 2 * a, ty = ey + 4 * b, tz = ez + 4 * c;
+      if (ty >= kTileY || tz >= kTileZ) continue;
+      const int32_t src = src8[o];
+      if (src < 0) continue;
+      const double* base = stage + (size_t)src * SS + (tx * kTileY + ty) * kTileZ + tz;
+#pragma unroll
+      for (int q = 0; q < NF; ++q) acc[q] += base[q * kTileNodes];
     }
     mass = acc[0];
     for (int d = 0; d < 3; ++d) {
@@ -677,6 +688,8 @@ struct zr_engine {
   uint32_t *touch_page = nullptr, *touch_slot = nullptr;
   uint32_t* slot_page = nullptr;
   int32_t* group_nslot = nullptr;  // [8 * groups]
+  int32_t* slot_src = nullptr;     // [8 * slots], capacity grid_cap / 32
+  uint32_t* ticket = nullptr;      // grid-reduction ticket, kept at zero between kernels
   uint32_t *word_base = nullptr;
 
   uint32_t A = 0, S = 0;  // groups, slots of the current sort
@@ -826,6 +839,8 @@ int ensure_grid(zr_engine* e)
     ZR_HIP(e, hipMalloc(&e->gv, 3 * cap * sizeof(double)));
     ZR_HIP(e, hipMalloc(&e->gf, 3 * cap * sizeof(double)));
     ZR_HIP(e, hipMalloc(&e->gvs, 3 * cap * sizeof(double)));
+    dfree(e->slot_src);
+    ZR_HIP(e, hipMalloc(&e->slot_src, (cap / 32 + 1) * 8 * sizeof(int32_t)));
     e->grid_cap = cap;
   }
   if ((size_t)e->A > e->stage_cap) {
@@ -889,6 +904,10 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   e->S = e->hcount[CNT_SLOTS];
   rc = ensure_grid(e);
   if (rc) return rc;
+  if (e->S) {
+    k_slot_sources<<<blocks_for(8ull * e->S, 256), 256, 0, s>>>(e->S, e->slot_page, e->page_group, e->slot_src);
+    ZR_LAUNCH_CHECK(e);
+  }
   e->sorted = true;
   if (st) {
     st->num_groups = A;
@@ -924,11 +943,11 @@ int do_grid(zr_engine* e, int mode, double dt)
   if (e->S) {
     const uint32_t nb = blocks_for((uint64_t)e->S * 32, 256);
     if (e->mls_rasterised)
-      k_grid<kP2GFields><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->page_group, e->stage, G, e->cfg.dx,
+      k_grid<kP2GFields><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->slot_src, e->stage, G, e->cfg.dx,
                                                     dt, e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2],
                                                     e->time, cs);
     else
-      k_grid<4><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->page_group, e->stage, G, e->cfg.dx, dt,
+      k_grid<4><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->slot_src, e->stage, G, e->cfg.dx, dt,
                                            e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs);
   }
   ZR_LAUNCH_CHECK(e);
@@ -1074,31 +1093,31 @@ int be_state_residual(zr_engine* e)
                                              e->stage);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_STATE);
-  k_be_residual<<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->page_group, e->stage, e->gm, nv(e, e->idv),
+  k_be_residual<<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, e->stage, e->gm, nv(e, e->idv),
                                            nv(e, e->ir), dx, dt, e->cfg.gravity[0], e->cfg.gravity[1],
                                            e->cfg.gravity[2], e->time, cs, e->partial);
-  k_reduce<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, SC_NORM2, nullptr);
+  k_reduce_cg<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, e->ci, RED_STORE, SC_NORM2, 0.0, 0.0);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
 }
 
-// out = A pvec (projected when requested); the CG flags gate it when given
+// out = A p, or A (q + beta p) with p updated to that direction when q is
+// given (the CG step); projected when requested; the CG flags gate it
 template <class T>
-int be_apply(zr_engine* e, const T* pvec, T* out, int project, const int32_t* ci, int red_slot)
+int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, const int32_t* ci, bool cg)
 {
   const double dx = e->cfg.dx, dt = e->be_dt;
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
   ColliderSet cs = collider_set(e);
   T* stage = reinterpret_cast<T*>(e->stage);
   prof_mark(e, ZR_PROF_HVP);
-  k_hvp<T><<<e->A, 256, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, pvec,
+  k_hvp<T><<<e->A, 256, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
                                         (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
-  k_cg_ap<T><<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->page_group, stage, e->gm,
-                                        NodeVec<T>{ const_cast<T*>(pvec), (int64_t)e->grid_cap }, nvT(e, out), dx, dt,
-                                        e->time, cs, project, e->partial, ci);
-  if (red_slot >= 0) k_reduce<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, red_slot, ci);
+  k_cg_ap<T><<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
+                                        nvT(e, out), dx, dt, e->time, cs, project, e->partial, e->sc, ci);
+  if (cg) k_reduce_cg<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, e->ci, RED_ALPHA, 0, 0.0, 0.0);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
 }
@@ -1115,17 +1134,15 @@ int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
   hipStream_t s = e->stream;
   k_cg_init<T><<<nb, 256, 0, s>>>(nn, e->gm, nv(e, e->ir), nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
                                   nvT(e, V::p(e)), e->partial);
-  k_reduce<<<1, 256, 0, s>>>(e->partial, nb, e->sc, SC_ZTR, nullptr);
-  k_cg_setup<<<1, 64, 0, s>>>(e->sc, e->ci, rel, o->cg_tolerance);
+  k_reduce_cg<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci, RED_CG_INIT, 0, rel, o->cg_tolerance);
   ZR_LAUNCH_CHECK(e);
   const int check = o->cg_check_interval > 0 ? o->cg_check_interval : 8;
   for (int k = 0; k < o->cg_max_iterations; ++k) {
-    int rc = be_apply<T>(e, V::p(e), V::ap(e), 1, e->ci, SC_PAP);
+    int rc = be_apply<T>(e, V::q(e), V::p(e), V::ap(e), 1, e->ci, true);
     if (rc) return rc;
     k_cg_update<T><<<nb, 256, 0, s>>>(nn, e->gm, nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
                                       nvT(e, V::p(e)), nvT(e, V::ap(e)), e->sc, e->partial, e->ci);
-    k_cg_reduce_post<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci);
-    k_cg_p<T><<<nb, 256, 0, s>>>(nn, nvT(e, V::q(e)), nvT(e, V::p(e)), e->sc, e->ci);
+    k_reduce_cg<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci, RED_BETA, 0, 0.0, 0.0);
     ZR_LAUNCH_CHECK(e);
     if ((k + 1) % check == 0 && k + 1 < o->cg_max_iterations) {
       ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1226,12 +1243,14 @@ int zr_create(const zr_config* cfg, zr_engine** out)
             hipMalloc(&e->page_bits, e->nwords * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->word_base, (e->nwords + 1) * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->dcount, CNT_N * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&e->ticket, sizeof(uint32_t)) == hipSuccess &&
             hipHostMalloc(&e->hcount, CNT_N * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipMemset(e->page_count, 0, e->ndense * sizeof(uint32_t)) == hipSuccess &&
        hipMemset(e->page_group, 0xff, e->ndense * sizeof(int32_t)) == hipSuccess &&
        hipMemset(e->page_touch, 0xff, e->ndense * sizeof(uint32_t)) == hipSuccess &&
        hipMemset(e->page_slot, 0xff, e->ndense * sizeof(int32_t)) == hipSuccess &&
        hipMemset(e->page_bits, 0, e->nwords * sizeof(uint32_t)) == hipSuccess &&
+       hipMemset(e->ticket, 0, sizeof(uint32_t)) == hipSuccess &&
        hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     zr_destroy(e);
@@ -1248,7 +1267,7 @@ void zr_destroy(zr_engine* e)
   free_particles(e);
   dfree(e->page_count); dfree(e->page_group); dfree(e->page_touch); dfree(e->page_slot); dfree(e->page_bits);
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
-  dfree(e->scan_tmp); dfree(e->dcount);
+  dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
@@ -1536,7 +1555,7 @@ int zr_implicit_multiply(zr_engine* e, const double* x, double* b)
       for (int d = 0; d < 3; ++d) soa[d * nn + k] = hf[3 * k + d];
     for (int d = 0; d < 3; ++d)
       ZR_HIP(e, hipMemcpy(e->fp + d * e->grid_cap, soa.data() + d * nn, nn * sizeof(float), hipMemcpyHostToDevice));
-    rc = be_apply<float>(e, e->fp, e->fap, 0, nullptr, -1);
+    rc = be_apply<float>(e, nullptr, e->fp, e->fap, 0, nullptr, false);
     if (rc) return rc;
     ZR_HIP(e, hipStreamSynchronize(e->stream));
     for (int d = 0; d < 3; ++d)
@@ -1545,7 +1564,7 @@ int zr_implicit_multiply(zr_engine* e, const double* x, double* b)
       for (int d = 0; d < 3; ++d) b[3 * k + d] = (double)soa[d * nn + k];
     return ZR_OK;
   }
-  rc = be_apply<double>(e, e->ip, e->iap, 0, nullptr, -1);
+  rc = be_apply<double>(e, nullptr, e->ip, e->iap, 0, nullptr, false);
   if (rc) return rc;
   return download_nodevec(e, e->iap, b);
 }

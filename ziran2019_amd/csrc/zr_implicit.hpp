@@ -26,6 +26,7 @@ enum {
   SC_ZTR_NEW = 3,
   SC_BETA = 4,
   SC_LOCAL_TOL = 5,
+  SC_ALPHA = 6,
   SC_N = 16
 };
 enum { CI_DONE = 0, CI_ITERS = 1, CI_N = 4 };
@@ -66,6 +67,16 @@ __device__ __forceinline__ double block_sum256(double v, double* smem4)
   double s = 0.0;
   if (threadIdx.x == 0) s = ((smem4[0] + smem4[1]) + smem4[2]) + smem4[3];
   return s;
+}
+
+// block partial of a grid-wide sum; the partials are added in index order
+// by a one-block kernel (a device-scope fence per block would force L2
+// write-backs across the XCDs of the device)
+__device__ __forceinline__ void block_partial(double contrib, double* partial)
+{
+  __shared__ double smem[4];
+  const double s = block_sum256(contrib, smem);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // collision projection P = I - K K^T of the node (CollisionObject.h:17-40),
@@ -125,6 +136,28 @@ __device__ __forceinline__ void sum_tiles(const T* stage, const int32_t* page_gr
   }
 }
 
+// sum of the <= 8 staged tiles overlapping a node, from the slot's
+// precomputed source groups (o = 4a + 2b + c, -1 when absent)
+template <int NF, class T>
+__device__ __forceinline__ void sum_tiles_src(const T* stage, const int32_t* src8, int ex, int ey, int ez,
+                                              T acc[NF])
+{
+  constexpr int stride = NF * kTileNodes;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) acc[f] = (T)0;
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    const int a = o >> 2, b = (o >> 1) & 1, c = o & 1;
+    const int tx = ex + 2 * a, ty = ey + 4 * b, tz = ez + 4 * c;
+    if (ty >= kTileY || tz >= kTileZ) continue;
+    const int32_t src = src8[o];
+    if (src < 0) continue;
+    const T* base = stage + (size_t)src * stride + (tx * kTileY + ty) * kTileZ + tz;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] += base[f * kTileNodes];
+  }
+}
+
 struct NodeCoord {
   uint32_t qx, qy, qz;
   int ex, ey, ez;
@@ -178,6 +211,25 @@ __device__ __forceinline__ void load_tile3_slots(const int32_t* gslot, const TS*
     const int32_t sl = nslot[((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2)];
     const int64_t node = (int64_t)sl * 32 + elem_of(tx, ty, tz);
     for (int d = 0; d < 3; ++d) tv[d * kTileNodes + t] = sl >= 0 ? (T)a[d * stride + node] : (T)0;
+  }
+  __syncthreads();
+}
+
+// CG search direction on the fly: p_new = q + beta p_old for the tile nodes
+template <class T>
+__device__ __forceinline__ void load_tile3_dir(const int32_t* gslot, const T* q, const T* p, T beta, int64_t stride,
+                                               T* tv, int32_t* nslot)
+{
+  const int tid = threadIdx.x;
+  if (tid < 8) nslot[tid] = gslot[tid];
+  __syncthreads();
+  for (int t = tid; t < kTileNodes; t += blockDim.x) {
+    const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
+    const int32_t sl = nslot[((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2)];
+    const int64_t node = (int64_t)sl * 32 + elem_of(tx, ty, tz);
+    for (int d = 0; d < 3; ++d)
+      tv[d * kTileNodes + t] = sl >= 0 ? (q ? q[d * stride + node] + beta * p[d * stride + node] : p[d * stride + node])
+                                       : (T)0;
   }
   __syncthreads();
 }
@@ -402,9 +454,12 @@ struct HvpChunk<double> {
 //            in index order for the three c offsets (separable form
 //            gw = (dw_a w_b w_c, w_a dw_b w_c, w_a w_b dw_c) / dx)
 //   phase 3  thread per tile node: sums its <= 27 partials into the staged tile
+// qvec == nullptr: applies the operator to pvec itself; otherwise to the CG
+// direction qvec + beta pvec (beta from sc)
 template <class T>
 __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* __restrict__ group_start,
-                                             const int32_t* __restrict__ group_nslot, const T* __restrict__ pvec,
+                                             const int32_t* __restrict__ group_nslot, const T* __restrict__ qvec,
+                                             const T* __restrict__ pvec, const double* __restrict__ sc,
                                              int64_t gstride, T inv_dx, T* __restrict__ stage,
                                              const int32_t* __restrict__ ci)
 {
@@ -431,7 +486,7 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
 #pragma unroll
     for (int d = 0; d < 3; ++d) fr0[d] = cache.frac[d * CS + ks];
   }
-  load_tile3_slots(group_nslot + 8 * (size_t)g, pvec, gstride, tv, nslot);
+  load_tile3_dir(group_nslot + 8 * (size_t)g, qvec, pvec, qvec ? (T)sc[SC_BETA] : (T)0, gstride, tv, nslot);
   for (uint32_t c0 = 0; c0 < n; c0 += CH) {
     const uint32_t m = n - c0 < (uint32_t)CH ? n - c0 : (uint32_t)CH;
     if (tid < kCells) {
@@ -625,15 +680,14 @@ __global__ __launch_bounds__(256) void k_be_init(uint32_t nn, const uint32_t* __
 }
 
 // computeResidual (BackwardEuler.h:101-124): r = dt m g - dt sum vPFnT grad w - m dv,
-// projected; partial sums of |r|^2 / m (computeNorm, BackwardEuler.h:127-140)
+// projected; sc[SC_NORM2] = sum |r|^2 / m (computeNorm, BackwardEuler.h:127-140)
 __global__ __launch_bounds__(256) void k_be_residual(uint32_t nn, const uint32_t* __restrict__ slot_page,
-                                                     const int32_t* __restrict__ page_group,
+                                                     const int32_t* __restrict__ slot_src,
                                                      const double* __restrict__ stage, const double* __restrict__ gm,
                                                      NodeVec3 dv, NodeVec3 r, double dx, double dt, double gx,
                                                      double gy, double gz, double time, ColliderSet cs,
                                                      double* __restrict__ partial)
 {
-  __shared__ double smem[4];
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   double contrib = 0.0;
   if (k < nn) {
@@ -642,7 +696,7 @@ __global__ __launch_bounds__(256) void k_be_residual(uint32_t nn, const uint32_t
     if (m != 0.0) {
       const NodeCoord nc = node_coord(slot_page, k);
       double f[kImpFields];
-      sum_tiles<kImpFields>(stage, page_group, nc.qx, nc.qy, nc.qz, nc.ex, nc.ey, nc.ez, f);
+      sum_tiles_src<kImpFields>(stage, slot_src + 8 * (size_t)(k >> 5), nc.ex, nc.ey, nc.ez, f);
       const double g[3] = { gx, gy, gz };
       for (int d = 0; d < 3; ++d) rv[d] = dt * g[d] * m - dt * f[d] - m * dv(d, k);
       double xi[3];
@@ -652,8 +706,7 @@ __global__ __launch_bounds__(256) void k_be_residual(uint32_t nn, const uint32_t
     }
     for (int d = 0; d < 3; ++d) r(d, k) = rv[d];
   }
-  const double s = block_sum256(contrib, smem);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  block_partial(contrib, partial);
 }
 
 // sums nb partials in index order into sc[slot]; skipped once CG is done
@@ -668,13 +721,14 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
   if (threadIdx.x == 0) sc[slot] = s;
 }
 
-// CG start (ConjugateGradient.h:105-116): x = 0, r = b, z = r / m, p = z
+// CG start (ConjugateGradient.h:105-119): x = 0, r = b, z = r / m, p = z,
+// z^T r and the local tolerance min(rel |r0|_P, tol); beta = 0 so that the
+// first direction built on the fly is p = z
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_init(uint32_t nn, const double* __restrict__ gm, NodeVec3 b, NodeVec<T> x,
                                                  NodeVec<T> r, NodeVec<T> q, NodeVec<T> p,
                                                  double* __restrict__ partial)
 {
-  __shared__ double smem[4];
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   double contrib = 0.0;
   if (k < nn) {
@@ -689,41 +743,41 @@ __global__ __launch_bounds__(256) void k_cg_init(uint32_t nn, const double* __re
       contrib += (double)rv * (double)qv;
     }
   }
-  const double s = block_sum256(contrib, smem);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  block_partial(contrib, partial);
 }
 
-// local_tolerance = min(rel |r0|_P, tol)  (ConjugateGradient.h:117-119)
-__global__ void k_cg_setup(double* sc, int32_t* ci, double rel, double tol)
-{
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double norm = sqrt(sc[SC_ZTR]);
-  const double lt = rel * norm < tol ? rel * norm : tol;
-  sc[SC_LOCAL_TOL] = lt;
-  ci[CI_DONE] = norm < lt ? 1 : 0;
-}
-
-// Ap = M p + dt^2 sum ds grad w, projected; partial p^T A p (BackwardEuler.h:322-351)
+// direction p = z + beta p, Ap = M p + dt^2 sum ds grad w (projected),
+// alpha = z^T r / p^T A p  (ConjugateGradient.h:127-133, BackwardEuler.h:322-351)
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __restrict__ slot_page,
-                                               const int32_t* __restrict__ page_group, const T* __restrict__ stage,
-                                               const double* __restrict__ gm, NodeVec<T> p, NodeVec<T> ap, double dx,
-                                               double dt, double time, ColliderSet cs, int project,
-                                               double* __restrict__ partial, const int32_t* ci)
+                                               const int32_t* __restrict__ slot_src, const T* __restrict__ stage,
+                                               const double* __restrict__ gm, NodeVec<T> q, NodeVec<T> p,
+                                               NodeVec<T> ap, double dx, double dt, double time, ColliderSet cs,
+                                               int project, double* __restrict__ partial, const double* sc,
+                                               const int32_t* ci)
 {
   if (ci && ci[CI_DONE]) return;
-  __shared__ double smem[4];
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   double contrib = 0.0;
   if (k < nn) {
     const double m = gm[k];
+    T pv[3];
+    if (q.p) {
+      const T beta = (T)sc[SC_BETA];
+      for (int d = 0; d < 3; ++d) {
+        pv[d] = q(d, k) + beta * p(d, k);
+        p(d, k) = pv[d];
+      }
+    } else {
+      for (int d = 0; d < 3; ++d) pv[d] = p(d, k);
+    }
     T av[3] = { (T)0, (T)0, (T)0 };
     if (m != 0.0) {
       const NodeCoord nc = node_coord(slot_page, k);
       T f[kImpFields];
-      sum_tiles<kImpFields>(stage, page_group, nc.qx, nc.qy, nc.qz, nc.ex, nc.ey, nc.ez, f);
+      sum_tiles_src<kImpFields>(stage, slot_src + 8 * (size_t)(k >> 5), nc.ex, nc.ey, nc.ez, f);
       const T dt2 = (T)(dt * dt);
-      for (int d = 0; d < 3; ++d) av[d] = (T)m * p(d, k) + dt2 * f[d];
+      for (int d = 0; d < 3; ++d) av[d] = (T)m * pv[d] + dt2 * f[d];
       if (project) {
         double xi[3], a64[3] = { (double)av[0], (double)av[1], (double)av[2] };
         node_pos(nc, dx, xi);
@@ -733,26 +787,24 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
     }
     for (int d = 0; d < 3; ++d) {
       ap(d, k) = av[d];
-      contrib += (double)av[d] * (double)p(d, k);
+      contrib += (double)av[d] * (double)pv[d];
     }
   }
-  const double s = block_sum256(contrib, smem);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  block_partial(contrib, partial);
 }
 
-// x += alpha p, r -= alpha Ap, z = r / m ; partial z^T r
+// x += alpha p, r -= alpha Ap, z = r / m; beta = z^T r / z^T r_old, the
+// iteration count and the termination test (ConjugateGradient.h:121-143)
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __restrict__ gm, NodeVec<T> x,
                                                    NodeVec<T> r, NodeVec<T> q, NodeVec<T> p, NodeVec<T> ap,
-                                                   const double* __restrict__ sc, double* __restrict__ partial,
-                                                   const int32_t* ci)
+                                                   const double* sc, double* __restrict__ partial, const int32_t* ci)
 {
   if (ci[CI_DONE]) return;
-  __shared__ double smem[4];
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   double contrib = 0.0;
   if (k < nn) {
-    const T alpha = (T)(sc[SC_ZTR] / sc[SC_PAP]);
+    const T alpha = (T)sc[SC_ALPHA];
     const double m = gm[k];
     for (int d = 0; d < 3; ++d) {
       x(d, k) += p(d, k) * alpha;
@@ -763,37 +815,38 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
       contrib += (double)qv * (double)rv;
     }
   }
-  const double s = block_sum256(contrib, smem);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  block_partial(contrib, partial);
 }
 
-// z^T r reduction fused with the scalar update: beta, termination test
-// (ConjugateGradient.h:134-143)
-__global__ __launch_bounds__(256) void k_cg_reduce_post(const double* __restrict__ partial, uint32_t nb, double* sc,
-                                                        int32_t* ci)
+// one-block reductions of the partials with the CG scalar updates
+enum { RED_STORE = 0, RED_CG_INIT = 1, RED_ALPHA = 2, RED_BETA = 3 };
+__global__ __launch_bounds__(256) void k_reduce_cg(const double* __restrict__ partial, uint32_t nb, double* sc,
+                                                   int32_t* ci, int mode, int slot, double rel, double tol)
 {
   __shared__ double smem[4];
-  if (ci[CI_DONE]) return;
+  if (mode != RED_STORE && mode != RED_CG_INIT && ci[CI_DONE]) return;
   double acc = 0.0;
   for (uint32_t b = threadIdx.x; b < nb; b += 256) acc += partial[b];
-  const double znew = block_sum256(acc, smem);
-  if (threadIdx.x == 0) {
-    sc[SC_BETA] = znew / sc[SC_ZTR];
-    sc[SC_ZTR] = znew;
+  const double v = block_sum256(acc, smem);
+  if (threadIdx.x != 0) return;
+  if (mode == RED_STORE) {
+    sc[slot] = v;
+  } else if (mode == RED_CG_INIT) {  // ConjugateGradient.h:115-119
+    const double norm = sqrt(v);
+    const double lt = rel * norm < tol ? rel * norm : tol;
+    sc[SC_ZTR] = v;
+    sc[SC_BETA] = 0.0;
+    sc[SC_LOCAL_TOL] = lt;
+    ci[CI_DONE] = norm < lt ? 1 : 0;
+  } else if (mode == RED_ALPHA) {  // ConjugateGradient.h:129
+    sc[SC_PAP] = v;
+    sc[SC_ALPHA] = sc[SC_ZTR] / v;
+  } else {  // ConjugateGradient.h:136-143
+    sc[SC_BETA] = v / sc[SC_ZTR];
+    sc[SC_ZTR] = v;
     ci[CI_ITERS] += 1;
-    if (sqrt(znew) < sc[SC_LOCAL_TOL]) ci[CI_DONE] = 1;
+    if (sqrt(v) < sc[SC_LOCAL_TOL]) ci[CI_DONE] = 1;
   }
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_cg_p(uint32_t nn, NodeVec<T> q, NodeVec<T> p, const double* __restrict__ sc,
-                                              const int32_t* ci)
-{
-  if (ci[CI_DONE]) return;
-  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-  if (k >= nn) return;
-  const T beta = (T)sc[SC_BETA];
-  for (int d = 0; d < 3; ++d) p(d, k) = q(d, k) + beta * p(d, k);
 }
 
 // Newton update dv += step (the FP64 outer iteration refines the FP32 solve)
