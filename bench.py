@@ -198,6 +198,19 @@ def run_implicit(args):
                 roof_kernels=("hvp", "state"), workload=workload, base=base, extra=extra, dtype="f32-bulk/f64-refined")
 
 
+def copy_calibration(nbytes=256 << 20, launches=5):
+    """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE of
+    its dispatches against ``nbytes`` give the counters' scale factor."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(launches):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(copy_bytes_each_way=nbytes, launches=launches)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,10 +219,15 @@ def main():
     ap.add_argument("--workload", default="implicit-boxes", choices=["implicit-boxes", "explicit-cube"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 256 MiB (5 launches) to scale HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the step would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine has no multi-GPU path; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     implicit = args.workload == "implicit-boxes"
     if args.steps is None:

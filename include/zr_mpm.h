@@ -42,6 +42,13 @@ enum {
 enum { ZR_FP64 = 0, ZR_FP32_REFINED = 1 };
 enum { ZR_TRANSFER_MLS = 0, ZR_TRANSFER_APIC = 1 };
 
+/* elasticity models (Lib/Ziran/Physics/ConstitutiveModel/): fixed-corotated
+   (no reference counterpart; the default), StvkWithHencky, NeoHookeanBorden */
+enum { ZR_MODEL_FIXED_COROTATED = 0, ZR_MODEL_STVK_HENCKY = 1, ZR_MODEL_NEO_HOOKEAN_BORDEN = 2 };
+/* plasticity (Lib/Ziran/Physics/PlasticityApplier.h): VonMisesStvkHencky,
+   DruckerPragerStvkHencky; both assume the StvkWithHencky model */
+enum { ZR_PLASTIC_NONE = 0, ZR_PLASTIC_VON_MISES = 1, ZR_PLASTIC_DRUCKER_PRAGER = 2 };
+
 /* collision object types, AnalyticCollisionObject::COLLISION_OBJECT_TYPE
    (Lib/Ziran/Math/Geometry/CollisionObject.h:47-52) */
 enum { ZR_STICKY = 1, ZR_SLIP = 2, ZR_SEPARATE = 3 };
@@ -83,6 +90,25 @@ typedef struct zr_particles {
   double* mu;      /* n, per-particle constitutive model (fixed-corotated) */
   double* lambda;  /* n   */
 } zr_particles;
+
+/* Per-particle material choice (the constitutive-model and plasticity
+   attributes the reference registers per particle, FBasedMpmForceHelper.h:17-60,
+   PlasticityApplier.h:19-56).  NULL model / plasticity mean all
+   ZR_MODEL_FIXED_COROTATED / ZR_PLASTIC_NONE.  plastic_param holds 4 doubles
+   per particle: von Mises {xi, fail_stress, 0, 0} (PlasticityApplier.cpp:443-462),
+   Drucker-Prager {alpha, beta, cohesion, volume_correction != 0} with alpha
+   from the friction angle as in PlasticityApplier.cpp:249-260.
+   plastic_state is updated by every step: von Mises yield stress (hardened),
+   Drucker-Prager logJp.  degradation is NeoHookeanBorden's g (NULL: 1). */
+typedef struct zr_materials {
+  int64_t n;
+  int32_t* model;          /* n */
+  int32_t* plasticity;     /* n */
+  double* plastic_param;   /* 4n */
+  double* plastic_state;   /* n */
+  double* degradation;     /* n */
+  int32_t* broken;         /* n, download only: von Mises fail_stress reached */
+} zr_materials;
 
 typedef struct zr_step_stats {
   int64_t num_groups;   /* particle pages (particle_group.size()) */
@@ -133,6 +159,11 @@ double zr_get_time(const zr_engine* e);
    (DataManager.h:67-346).  Download fills X, V, C, F (NULL pointers skip). */
 int zr_upload_particles(zr_engine* e, const zr_particles* p);
 int zr_download_particles(zr_engine* e, zr_particles* p);
+/* Per-particle models and plastic state, after zr_upload_particles (which
+   resets every particle to fixed-corotated without plasticity).  Download
+   fills plastic_state, degradation and broken (NULL pointers skip). */
+int zr_set_materials(zr_engine* e, const zr_materials* m);
+int zr_get_materials(zr_engine* e, zr_materials* m);
 
 /* MpmSimulationBase::sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1032-1103) */
 int zr_sort_particles_and_pollute_grid(zr_engine* e, zr_step_stats* st);

@@ -43,6 +43,24 @@ def implicit_opts(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterati
                            float(cg_tolerance), float(cg_relative_tolerance), int(preconditioner), 0)
 
 
+class ZroPfOpts(ctypes.Structure):
+    _fields_ = [("parabolic_M", _d), ("cg_max_iterations", ctypes.c_int32), ("naive", ctypes.c_int32),
+                ("cg_tolerance", _d), ("cg_relative", _d), ("sigma_F", _d)]
+
+
+class ZroPfStats(ctypes.Structure):
+    _fields_ = [("cg_iterations", ctypes.c_int32), ("reserved", ctypes.c_int32), ("residual_norm", _d)]
+
+
+def pf_opts(parabolic_M=0.0, cg_max_iterations=100, naive=False, cg_tolerance=1e-6, cg_relative=1e-12,
+            sigma_F=0.0):
+    """FractureSimulation defaults: ConjugateGradient(100), tolerance 1e-6,
+    relative 1e-12 in double (FractureSimulation.h:70-73,439;
+    fracture/ConjugateGradient.h:64)."""
+    return ZroPfOpts(float(parabolic_M), int(cg_max_iterations), int(bool(naive)), float(cg_tolerance),
+                     float(cg_relative), float(sigma_F))
+
+
 class ZroCollider(ctypes.Structure):
     _fields_ = [
         ("shape", ctypes.c_int32),
@@ -126,6 +144,31 @@ def lib():
         L.zro_num_node_slots.argtypes = [_p]
         L.zro_get_implicit.argtypes = [_p, _p, _p, _p]
         L.zro_implicit_diagonal.argtypes = [_p, _p]
+        for f in ("zro_stvk_hencky_psi",):
+            getattr(L, f).restype = _d
+            getattr(L, f).argtypes = [_d, _d, _p]
+        for f in ("zro_stvk_hencky_first_piola", "zro_stvk_hencky_kirchhoff"):
+            getattr(L, f).argtypes = [_d, _d, _p, _p]
+        L.zro_nhb_psi.restype = _d
+        L.zro_nhb_psi.argtypes = [_d, _d, _d, _p]
+        L.zro_nhb_psi_pos.restype = _d
+        L.zro_nhb_psi_pos.argtypes = [_d, _d, _p]
+        L.zro_nhb_kirchhoff.argtypes = [_d, _d, _d, _p, _p]
+        L.zro_von_mises_project.restype = _i
+        L.zro_von_mises_project.argtypes = [_d, _d, _d, _d, _p, _p, _p]
+        L.zro_drucker_prager_alpha.restype = _d
+        L.zro_drucker_prager_alpha.argtypes = [_d]
+        L.zro_drucker_prager_project.restype = _i
+        L.zro_drucker_prager_project.argtypes = [_d, _d, _d, _d, _d, _i, _p, _p]
+        L.zro_set_materials.argtypes = [_p, _p, _p, _p, _p, _p]
+        L.zro_set_phase_field.argtypes = [_p] + [_p] * 8 + [ctypes.POINTER(ZroPfOpts)]
+        L.zro_get_phase_field.argtypes = [_p, _p, _p, _p]
+        L.zro_phase_field_solve.argtypes = [_p, _d, ctypes.POINTER(ZroPfStats)]
+        L.zro_phase_field_system.argtypes = [_p, _p, _p, _p]
+        L.zro_phase_field_multiply.argtypes = [_p, _p, _p]
+        L.zro_sym3_max_eigenvalue.restype = _d
+        L.zro_sym3_max_eigenvalue.argtypes = [_p]
+        L.zro_get_materials.argtypes = [_p, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -194,6 +237,61 @@ def lame(E, nu):
     return float(mu[0]), float(lam[0])
 
 
+def stvk_hencky_psi(mu, lam, F):
+    return float(lib().zro_stvk_hencky_psi(mu, lam, _ptr(_cm(F))))
+
+
+def stvk_hencky_first_piola(mu, lam, F):
+    P = np.zeros(9)
+    lib().zro_stvk_hencky_first_piola(mu, lam, _ptr(_cm(F)), _ptr(P))
+    return _from_cm(P)
+
+
+def stvk_hencky_kirchhoff(mu, lam, F):
+    t = np.zeros(9)
+    lib().zro_stvk_hencky_kirchhoff(mu, lam, _ptr(_cm(F)), _ptr(t))
+    return _from_cm(t)
+
+
+def nhb_psi(mu, lam, g, F):
+    return float(lib().zro_nhb_psi(mu, lam, g, _ptr(_cm(F))))
+
+
+def nhb_psi_pos(mu, lam, F):
+    return float(lib().zro_nhb_psi_pos(mu, lam, _ptr(_cm(F))))
+
+
+def nhb_kirchhoff(mu, lam, g, F):
+    t = np.zeros(9)
+    lib().zro_nhb_kirchhoff(mu, lam, g, _ptr(_cm(F)), _ptr(t))
+    return _from_cm(t)
+
+
+def von_mises_project(mu, lam, xi, fail_stress, yield_stress, F):
+    """Returns (F_new, yield_stress_new, broken, changed)."""
+    y = np.array([yield_stress], dtype=np.float64)
+    f = _cm(F)
+    b = np.zeros(1, dtype=np.int32)
+    ch = lib().zro_von_mises_project(mu, lam, xi, fail_stress, _ptr(y), _ptr(f), _ptr(b))
+    return _from_cm(f), float(y[0]), bool(b[0]), bool(ch)
+
+
+def drucker_prager_alpha(friction_angle_deg):
+    return float(lib().zro_drucker_prager_alpha(float(friction_angle_deg)))
+
+
+def drucker_prager_project(mu, lam, alpha, beta, cohesion, volume_correction, logJp, F):
+    """Returns (F_new, logJp_new)."""
+    j = np.array([logJp], dtype=np.float64)
+    f = _cm(F)
+    lib().zro_drucker_prager_project(mu, lam, alpha, beta, cohesion, int(bool(volume_correction)), _ptr(j), _ptr(f))
+    return _from_cm(f), float(j[0])
+
+
+def sym3_max_eigenvalue(A):
+    return float(lib().zro_sym3_max_eigenvalue(_ptr(_cm(A))))
+
+
 def fcr_psi(mu, lam, F):
     return float(lib().zro_fcr_psi(mu, lam, _ptr(_cm(F))))
 
@@ -239,6 +337,60 @@ class OracleSim:
         arrs = [_f64(X, 3 * n), _f64(V, 3 * n), _f64(C, 9 * n), _f64(F, 9 * n),
                 _f64(mass, n), _f64(vol, n), _f64(mu, n), _f64(lam, n)]
         lib().zro_set_particles(self.h, n, *[_ptr(a) for a in arrs])
+
+    def set_materials(self, model=None, plastic=None, pparam=None, pstate=None, g=None):
+        n = self.n
+        m = None if model is None else np.ascontiguousarray(np.broadcast_to(model, (n,)), dtype=np.int32)
+        p = None if plastic is None else np.ascontiguousarray(np.broadcast_to(plastic, (n,)), dtype=np.int32)
+        q = None if pparam is None else _f64(np.broadcast_to(np.asarray(pparam, dtype=np.float64).reshape(-1, 4),
+                                                             (n, 4)), 4 * n)
+        st = None if pstate is None else _f64(np.broadcast_to(pstate, (n,)), n)
+        gg = None if g is None else _f64(np.broadcast_to(g, (n,)), n)
+        self._mat_keep = (m, p, q, st, gg)
+        lib().zro_set_materials(self.h, _ptr(m), _ptr(p), _ptr(q), _ptr(st), _ptr(gg))
+
+    def set_phase_field(self, opts, c=None, H=None, l0=None, one_over_sigma_c=None, pf_Fp=None, H_max=None,
+                        residual_phase=None, allow_damage=None):
+        n = self.n
+
+        def f(a):
+            return None if a is None else _f64(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,)), n)
+        arrs = [f(c), f(H), f(l0), f(one_over_sigma_c), f(pf_Fp), f(H_max), f(residual_phase)]
+        al = None if allow_damage is None else np.ascontiguousarray(np.broadcast_to(allow_damage, (n,)),
+                                                                     dtype=np.int32)
+        self._pf_keep = (arrs, al, opts)
+        lib().zro_set_phase_field(self.h, *[_ptr(a) for a in arrs], _ptr(al), ctypes.byref(opts))
+
+    def get_phase_field(self):
+        n = self.n
+        c, H, fp = np.zeros(n), np.zeros(n), np.zeros(n)
+        lib().zro_get_phase_field(self.h, _ptr(c), _ptr(H), _ptr(fp))
+        return dict(c=c, H=H, pf_Fp=fp)
+
+    def phase_field_solve(self, dt):
+        st = ZroPfStats()
+        lib().zro_phase_field_solve(self.h, float(dt), ctypes.byref(st))
+        return st
+
+    def phase_field_system(self):
+        ns = self.num_node_slots()
+        rhs, diag, x = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+        lib().zro_phase_field_system(self.h, _ptr(rhs), _ptr(diag), _ptr(x))
+        return rhs, diag, x
+
+    def phase_field_multiply(self, x):
+        x = _f64(x, self.num_node_slots())
+        b = np.zeros_like(x)
+        lib().zro_phase_field_multiply(self.h, _ptr(x), _ptr(b))
+        return b
+
+    def get_materials(self):
+        n = self.n
+        st = np.zeros(n)
+        b = np.zeros(n, dtype=np.int32)
+        g = np.zeros(n)
+        lib().zro_get_materials(self.h, _ptr(st), _ptr(b), _ptr(g))
+        return dict(pstate=st, broken=b, g=g)
 
     def add_collider(self, shape, type, origin, normal=(0, 1, 0), radius=0.0,
                      velocity=(0, 0, 0), friction=0.0):

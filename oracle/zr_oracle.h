@@ -44,6 +44,31 @@ void zro_fcr_first_piola_differential(double mu, double lam, const double* F,
                                       const double* dF, double* dP);
 void zro_lame(double E, double nu, double* mu, double* lam);
 
+/* ---- StvkWithHencky (StvkWithHencky.cpp:16-143) ---- */
+double zro_stvk_hencky_psi(double mu, double lam, const double* F);
+void zro_stvk_hencky_first_piola(double mu, double lam, const double* F, double* P);
+void zro_stvk_hencky_kirchhoff(double mu, double lam, const double* F, double* tau);
+
+/* ---- NeoHookeanBorden with degradation g (NeoHookeanBorden.cpp:16-86) ---- */
+double zro_nhb_psi(double mu, double lam, double g, const double* F);
+double zro_nhb_psi_pos(double mu, double lam, const double* F);
+void zro_nhb_kirchhoff(double mu, double lam, double g, const double* F, double* tau);
+
+/* ---- return mapping (PlasticityApplier.cpp:249-316, 443-501) ---- */
+int zro_von_mises_project(double mu, double lam, double xi, double fail_stress, double* yield_stress, double* F,
+                          int* broken);
+double zro_drucker_prager_alpha(double friction_angle_deg);
+int zro_drucker_prager_project(double mu, double lam, double alpha, double beta, double cohesion,
+                               int volume_correction, double* logJp, double* F);
+
+/* material codes (same values as ZR_MODEL_* / ZR_PLASTIC_* of the engine) */
+#define ZRO_MODEL_FIXED_COROTATED 0
+#define ZRO_MODEL_STVK_HENCKY 1
+#define ZRO_MODEL_NEO_HOOKEAN_BORDEN 2
+#define ZRO_PLASTIC_NONE 0
+#define ZRO_PLASTIC_VON_MISES 1
+#define ZRO_PLASTIC_DRUCKER_PRAGER 2
+
 /* ---- simulation state ---- */
 typedef struct zro_sim zro_sim;
 
@@ -63,6 +88,13 @@ void zro_destroy(zro_sim* s);
 void zro_set_particles(zro_sim* s, int64_t n, const double* X, const double* V,
                        const double* C, const double* F, const double* mass,
                        const double* vol, const double* mu, const double* lam);
+/* per-particle materials; pparam 4 per particle: von Mises {xi, fail_stress,
+   0, 0}, Drucker-Prager {alpha, beta, cohesion, volume_correction};
+   pstate: von Mises yield stress, Drucker-Prager logJp; g: degradation of
+   NeoHookeanBorden (NULL: 1) */
+void zro_set_materials(zro_sim* s, const int32_t* model, const int32_t* plastic, const double* pparam,
+                       const double* pstate, const double* g);
+void zro_get_materials(const zro_sim* s, double* pstate, int32_t* broken, double* g);
 void zro_get_particles(const zro_sim* s, double* X, double* V, double* C, double* F);
 int zro_add_collider(zro_sim* s, const zro_collider* c);
 void zro_set_time(zro_sim* s, double t);
@@ -115,6 +147,39 @@ int zro_backward_euler(zro_sim* s, double dt, const zro_implicit_opts* o, zro_im
 int zro_g2p_apic(zro_sim* s, double dt);
 /* advanceOneTimeStep, implicit branch (MpmSimulationBase.cpp:282-295) */
 int zro_step_implicit(zro_sim* s, double dt, const zro_implicit_opts* o, zro_implicit_stats* st);
+
+/* ---- CD-MPM phase-field damage (Projects/fracture/FractureSimulation.h) ---- */
+typedef struct {
+  double parabolic_M;        /* FractureSimulation::parabolic_M (<= 0: elliptic form) */
+  int32_t cg_max_iterations; /* ConjugateGradient(100) (FractureSimulation.h:70-73) */
+  int32_t naive;             /* 1: solvePhaseFieldSystemNaive (Rankine), 0: phase field */
+  double cg_tolerance;       /* 1e-6 (FractureSimulation.h:439) */
+  double cg_relative;        /* 1e-12 of the initial residual (fracture/ConjugateGradient.h:64) */
+  double sigma_F;            /* Rankine threshold of the naive form */
+} zro_pf_opts;
+
+typedef struct {
+  int32_t cg_iterations;     /* the value the fracture CG returns */
+  int32_t reserved;
+  double residual_norm;      /* final unpreconditioned residual norm */
+} zro_pf_stats;
+
+/* per-particle PhaseField attributes (PhaseField.h:13-21); NULL arrays keep
+   the addFBasedMpmForceWithPhaseField defaults (MpmParticleHandleBase.cpp:105-114):
+   c 1, H 0, pf_Fp 1, H_max +inf, residual_phase 0.001, allow_damage 1.
+   Particles with a phase field must use the NeoHookeanBorden model. */
+void zro_set_phase_field(zro_sim* s, const double* c, const double* H, const double* l0,
+                         const double* one_over_sigma_c, const double* pf_Fp, const double* H_max,
+                         const double* residual_phase, const int32_t* allow_damage, const zro_pf_opts* o);
+void zro_get_phase_field(const zro_sim* s, double* c, double* H, double* pf_Fp);
+/* solvePhaseFieldSystem / solvePhaseFieldSystemNaive: after zro_p2g_mls of the
+   same step; zro_step_explicit calls it once a phase field is set */
+void zro_phase_field_solve(zro_sim* s, double dt, zro_pf_stats* st);
+/* pieces for tests: per node slot (npages*32) the system of the last solve */
+void zro_phase_field_system(const zro_sim* s, double* rhs, double* diag, double* x);
+void zro_phase_field_multiply(zro_sim* s, const double* x, double* b);
+/* largest eigenvalue of a symmetric 3x3 matrix (column-major) */
+double zro_sym3_max_eigenvalue(const double* A);
 /* matrix-free operator of the last backward-Euler solve at the final Newton
    iterate, on node vectors in page-list x element order (3 per node):
    b = M x + dt^2 sum_p V0 dP(grad x_p Fn) Fn^T grad w_ip   (BackwardEuler.h:322-351) */

@@ -469,3 +469,200 @@ void zro_fcr_first_piola_differential(double mu, double lam, const double* F,
       p[i][j] = 2.0 * mu * (d[i][j] - dr[i][j]) + lam * cd * c[i][j] + lam * (J - 1.0) * dc[i][j];
   to_cm(p, dP);
 }
+
+/* ---- StvkWithHencky (Lib/Ziran/Physics/ConstitutiveModel/StvkWithHencky.cpp) ----
+   psi = mu tr((log S)^2) + lambda/2 (tr log S)^2            (:121-128)
+   P   = U (2 mu S^-1 log S + lambda tr(log S) S^-1) V^T      (:130-143)
+   tau = P F^T (HyperelasticConstitutiveModel.h:75-80) */
+double zro_stvk_hencky_psi(double mu, double lam, const double* F)
+{
+  M3 f, u, v;
+  double sig[3];
+  from_cm(F, f);
+  svd3_rm(f, u, sig, v);
+  double l2 = 0.0, tr = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    double l = log(fabs(sig[i]));
+    l2 += l * l;
+    tr += l;
+  }
+  return mu * l2 + 0.5 * lam * tr * tr;
+}
+
+void zro_stvk_hencky_first_piola(double mu, double lam, const double* F, double* P)
+{
+  M3 f, u, v, p;
+  double sig[3], ls[3], ph[3];
+  from_cm(F, f);
+  svd3_rm(f, u, sig, v);
+  double tr = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    ls[i] = log(fabs(sig[i]));
+    tr += ls[i];
+  }
+  for (int i = 0; i < 3; ++i) {
+    double inv = 1.0 / sig[i];
+    ph[i] = 2.0 * mu * inv * ls[i] + lam * tr * inv;
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) p[i][j] = u[i][0] * ph[0] * v[j][0] + u[i][1] * ph[1] * v[j][1] + u[i][2] * ph[2] * v[j][2];
+  to_cm(p, P);
+}
+
+void zro_stvk_hencky_kirchhoff(double mu, double lam, const double* F, double* tau)
+{
+  double P[9];
+  M3 p, f, t;
+  zro_stvk_hencky_first_piola(mu, lam, F, P);
+  from_cm(P, p);
+  from_cm(F, f);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) t[i][j] = p[i][0] * f[j][0] + p[i][1] * f[j][1] + p[i][2] * f[j][2];
+  to_cm(t, tau);
+}
+
+/* ---- NeoHookeanBorden (Lib/Ziran/Physics/ConstitutiveModel/NeoHookeanBorden.cpp) ----
+   kappa = 0.666666 mu + lambda (:16-21), degradation g on psi+ (:38-52) */
+static double nhb_kappa(double mu, double lam) { return 0.666666 * mu + lam; }
+
+static void nhb_parts(double mu, double lam, const double* F, double* dev, double* vol, double* J)
+{
+  M3 f;
+  from_cm(F, f);
+  double j = det3(f);
+  double a = pow(j, -1.0 / 3.0);
+  double tr = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) tr += (a * f[r][c]) * (a * f[r][c]);
+  *dev = mu / 2.0 * (tr - 3.0);
+  *vol = nhb_kappa(mu, lam) / 2.0 * ((j * j - 1.0) / 2.0 - log(j));
+  *J = j;
+}
+
+double zro_nhb_psi(double mu, double lam, double g, const double* F)
+{
+  double dev, vol, J;
+  nhb_parts(mu, lam, F, &dev, &vol, &J);
+  double pos = J >= 1.0 ? vol + dev : dev;
+  double neg = J >= 1.0 ? 0.0 : vol;
+  return g * pos + neg;
+}
+
+/* get_psi_pos (:54-67) */
+double zro_nhb_psi_pos(double mu, double lam, const double* F)
+{
+  double dev, vol, J;
+  nhb_parts(mu, lam, F, &dev, &vol, &J);
+  return J >= 1.0 ? vol + dev : dev;
+}
+
+/* kirchhoff (:69-86) */
+void zro_nhb_kirchhoff(double mu, double lam, double g, const double* F, double* tau)
+{
+  M3 f, b, t;
+  from_cm(F, f);
+  double J = det3(f);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) b[i][j] = f[i][0] * f[j][0] + f[i][1] * f[j][1] + f[i][2] * f[j][2];
+  double trb = b[0][0] + b[1][1] + b[2][2];
+  double s = mu * pow(J, -2.0 / 3.0);
+  double prime = nhb_kappa(mu, lam) / 2.0 * (J - 1.0 / J);
+  double vol = J * prime;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double dev = s * (b[i][j] - (i == j ? trb / 3.0 : 0.0));
+      double v = i == j ? vol : 0.0;
+      t[i][j] = J >= 1.0 ? g * (dev + v) : g * dev + v;
+    }
+  to_cm(t, tau);
+}
+
+/* ---- return mapping (Lib/Ziran/Physics/PlasticityApplier.cpp) ---- */
+static void rebuild(M3 u, const double* s, M3 v, double* F)
+{
+  M3 f;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) f[i][j] = u[i][0] * s[0] * v[j][0] + u[i][1] * s[1] * v[j][1] + u[i][2] * s[2] * v[j][2];
+  to_cm(f, F);
+}
+
+/* VonMisesStvkHencky::projectStrain (:464-501).  Returns 1 when F changed.
+   *yield_stress is hardened by xi * delta_gamma; *broken is set when the
+   projected first principal Kirchhoff stress reaches fail_stress. */
+int zro_von_mises_project(double mu, double lam, double xi, double fail_stress, double* yield_stress, double* F,
+                          int* broken)
+{
+  M3 f, u, v;
+  double sig[3], eps[3], H[3], e[3];
+  from_cm(F, f);
+  svd3_rm(f, u, sig, v);
+  double tr = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    eps[i] = log(sig[i] > 1e-4 ? sig[i] : 1e-4);
+    tr += eps[i];
+  }
+  double n2 = 0.0;
+  double hat[3];
+  for (int i = 0; i < 3; ++i) {
+    hat[i] = eps[i] - tr / 3.0;
+    n2 += hat[i] * hat[i];
+  }
+  double nrm = sqrt(n2);
+  double dg = nrm - *yield_stress / (2.0 * mu);
+  if (dg <= 0.0) return 0;
+  *yield_stress -= xi * dg;
+  for (int i = 0; i < 3; ++i) H[i] = eps[i] - (dg / nrm) * hat[i];
+  double tau0 = 2.0 * mu * H[0] + lam * (H[0] + H[1] + H[2]);
+  for (int i = 0; i < 3; ++i) e[i] = exp(H[i]);
+  rebuild(u, e, v, F);
+  if (broken && tau0 >= fail_stress) *broken = 1;
+  return 1;
+}
+
+/* DruckerPragerStvkHencky::projectStrain (:275-316); friction angle enters
+   through alpha (:249-260, zro_drucker_prager_alpha). */
+double zro_drucker_prager_alpha(double friction_angle_deg)
+{
+  double s = sin(friction_angle_deg / 180.0 * 3.141592653);
+  return sqrt(2.0 / 3.0) * 2.0 * s / (3.0 - s);
+}
+
+int zro_drucker_prager_project(double mu, double lam, double alpha, double beta, double cohesion,
+                               int volume_correction, double* logJp, double* F)
+{
+  M3 f, u, v;
+  double sig[3], eps[3], hat[3], H[3], e[3];
+  from_cm(F, f);
+  svd3_rm(f, u, sig, v);
+  double sum = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    double a = fabs(sig[i]);
+    eps[i] = log(a > 1e-4 ? a : 1e-4) - cohesion;
+    sum += eps[i];
+  }
+  double tr = sum + *logJp;
+  double n2 = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    hat[i] = eps[i] - tr / 3.0;
+    n2 += hat[i] * hat[i];
+  }
+  if (tr >= 0.0) { /* case II: project to the tip */
+    double c = exp(cohesion);
+    double cs[3] = { c, c, c };
+    rebuild(u, cs, v, F);
+    if (volume_correction) *logJp = beta * sum + *logJp;
+    return 1;
+  }
+  if (mu == 0.0) return 0;
+  *logJp = 0.0;
+  double nrm = sqrt(n2);
+  double dg = nrm + (3.0 * lam + 2.0 * mu) / (2.0 * mu) * tr * alpha;
+  if (dg <= 0.0) { /* case I: inside the yield surface */
+    for (int i = 0; i < 3; ++i) H[i] = eps[i] + cohesion;
+  } else { /* case III: project onto the cone */
+    for (int i = 0; i < 3; ++i) H[i] = eps[i] - (dg / nrm) * hat[i] + cohesion;
+  }
+  for (int i = 0; i < 3; ++i) e[i] = exp(H[i]);
+  rebuild(u, e, v, F);
+  return 1;
+}

@@ -113,6 +113,19 @@ struct zro_sim {
   int nthreads;
   int64_t n;
   double *X, *V, *C, *F, *mass, *vol, *mu, *lam;
+  /* materials: elasticity model and plasticity per particle (NULL: all
+     fixed-corotated, no plasticity), plasticity parameters (4 per particle),
+     plastic state (VM yield stress / DP logJp), NeoHookeanBorden degradation g */
+  int32_t *model, *plastic, *broken;
+  double *pparam, *pstate, *gdeg;
+  /* phase field (PhaseField.h:13-21) per particle and per node slot */
+  int pf_on;
+  zro_pf_opts pf_o;
+  double *pf_c, *pf_H, *pf_l0, *pf_osc, *pf_Fp, *pf_Hmax, *pf_k, *pf_vol;
+  int32_t* pf_allow;
+  double *pf_phase, *pf_mult, *pf_rhs, *pf_diag, *pf_x;
+  int64_t pf_cap;
+  double pf_dt;
   double *vtau, *gradV;
   uint64_t* sorter;
   int32_t* order;
@@ -154,6 +167,14 @@ static void free_particles(zro_sim* s)
   free(s->X); free(s->V); free(s->C); free(s->F); free(s->mass); free(s->vol);
   free(s->mu); free(s->lam); free(s->vtau); free(s->gradV); free(s->sorter); free(s->Fn);
   s->Fn = NULL;
+  free(s->model); free(s->plastic); free(s->broken); free(s->pparam); free(s->pstate); free(s->gdeg);
+  s->model = s->plastic = s->broken = NULL;
+  s->pparam = s->pstate = s->gdeg = NULL;
+  free(s->pf_c); free(s->pf_H); free(s->pf_l0); free(s->pf_osc); free(s->pf_Fp); free(s->pf_Hmax);
+  free(s->pf_k); free(s->pf_vol); free(s->pf_allow);
+  s->pf_c = s->pf_H = s->pf_l0 = s->pf_osc = s->pf_Fp = s->pf_Hmax = s->pf_k = s->pf_vol = NULL;
+  s->pf_allow = NULL;
+  s->pf_on = 0;
   free(s->order); free(s->base_offset); free(s->gfirst); free(s->glast);
   free(s->block_offset);
 }
@@ -165,6 +186,7 @@ void zro_destroy(zro_sim* s)
   free(s->pages); free(s->hkeys); free(s->hvals); free(s->nodes);
   free(s->explicit_collided);
   free(s->vn); free(s->dv); free(s->proj); free(s->collided);
+  free(s->pf_phase); free(s->pf_mult); free(s->pf_rhs); free(s->pf_diag); free(s->pf_x);
   free(s);
 }
 
@@ -336,13 +358,65 @@ static zro_node* node_at(zro_sim* s, int i, int j, int k, int64_t* slot_out)
   return &s->nodes[slot];
 }
 
+/* ---------------- materials ---------------- */
+void zro_set_materials(zro_sim* s, const int32_t* model, const int32_t* plastic, const double* pparam,
+                       const double* pstate, const double* g)
+{
+  int64_t n = s->n;
+  size_t nn = (size_t)(n > 0 ? n : 1);
+  free(s->model); free(s->plastic); free(s->broken); free(s->pparam); free(s->pstate); free(s->gdeg);
+  s->model = (int32_t*)calloc(nn, sizeof(int32_t));
+  s->plastic = (int32_t*)calloc(nn, sizeof(int32_t));
+  s->broken = (int32_t*)calloc(nn, sizeof(int32_t));
+  if (model) memcpy(s->model, model, sizeof(int32_t) * (size_t)n);
+  if (plastic) memcpy(s->plastic, plastic, sizeof(int32_t) * (size_t)n);
+  s->pparam = dup(pparam, 4 * n);
+  s->pstate = dup(pstate, n);
+  s->gdeg = dup(NULL, n);
+  for (int64_t i = 0; i < n; ++i) s->gdeg[i] = g ? g[i] : 1.0;
+}
+
+void zro_get_materials(const zro_sim* s, double* pstate, int32_t* broken, double* g)
+{
+  if (!s->model) return;
+  if (pstate) memcpy(pstate, s->pstate, sizeof(double) * (size_t)s->n);
+  if (broken) memcpy(broken, s->broken, sizeof(int32_t) * (size_t)s->n);
+  if (g) memcpy(g, s->gdeg, sizeof(double) * (size_t)s->n);
+}
+
+/* constitutive_model.kirchhoff for the particle's model */
+static void particle_kirchhoff(const zro_sim* s, int64_t p, double* tau)
+{
+  int m = s->model ? s->model[p] : ZRO_MODEL_FIXED_COROTATED;
+  if (m == ZRO_MODEL_STVK_HENCKY)
+    zro_stvk_hencky_kirchhoff(s->mu[p], s->lam[p], s->F + 9 * p, tau);
+  else if (m == ZRO_MODEL_NEO_HOOKEAN_BORDEN)
+    zro_nhb_kirchhoff(s->mu[p], s->lam[p], s->gdeg[p], s->F + 9 * p, tau);
+  else
+    zro_fcr_kirchhoff(s->mu[p], s->lam[p], s->F + 9 * p, tau);
+}
+
+/* PlasticityApplier::applyPlasticity for one particle (PlasticityApplier.h:39-56) */
+static void apply_plasticity(zro_sim* s, int64_t i)
+{
+  if (!s->plastic) return;
+  const double* q = s->pparam + 4 * i;
+  if (s->plastic[i] == ZRO_PLASTIC_VON_MISES) {
+    int b = 0;
+    zro_von_mises_project(s->mu[i], s->lam[i], q[0], q[1], &s->pstate[i], s->F + 9 * i, &b);
+    if (b) s->broken[i] = 1;
+  } else if (s->plastic[i] == ZRO_PLASTIC_DRUCKER_PRAGER) {
+    zro_drucker_prager_project(s->mu[i], s->lam[i], q[0], q[1], q[2], q[3] != 0.0, &s->pstate[i], s->F + 9 * i);
+  }
+}
+
 /* ---------------- stress: FBasedMpmForceHelper::updateState ---------------- */
 void zro_update_stress(zro_sim* s)
 {
 #pragma omp parallel for num_threads(s->nthreads)
   for (int64_t p = 0; p < s->n; ++p) {
     double tau[9];
-    zro_fcr_kirchhoff(s->mu[p], s->lam[p], s->F + 9 * p, tau);
+    particle_kirchhoff(s, p, tau);
     for (int c = 0; c < 9; ++c) s->vtau[9 * p + c] = s->vol[p] * tau[c];
   }
 }
@@ -609,6 +683,7 @@ int zro_g2p_mls(zro_sim* s, double dt)
         }
       for (int c = 0; c < 3; ++c)
         for (int r = 0; r < 3; ++r) s->F[9 * i + c * 3 + r] = Fn[r][c];
+      apply_plasticity(s, i); /* MpmSimulationBase.cpp:1015 */
     }
   }
   return flags;
@@ -620,6 +695,7 @@ int zro_step_explicit(zro_sim* s, double dt)
   zro_sort_and_pollute(s);
   zro_update_stress(s);
   zro_p2g_mls(s);
+  if (s->pf_on) zro_phase_field_solve(s, dt, NULL); /* FractureSimulation::particlesToGrid (:149-160) */
   zro_grid_update(s, dt);
   int flags = zro_g2p_mls(s, dt);
   s->time += dt;
@@ -1110,6 +1186,7 @@ int zro_g2p_apic(zro_sim* s, double dt)
           Fnew[r][c] = acc;
         }
       mat_to_cm(Fnew, s->F + 9 * (int64_t)i);
+      apply_plasticity(s, i); /* MpmSimulationBase.cpp:1015 */
     }
   }
   return flags;
@@ -1194,3 +1271,287 @@ void zro_implicit_diagonal(zro_sim* s, double* diag)
     }
   }
 }
+
+/* ====================================================================== */
+/* CD-MPM phase-field damage (Projects/fracture/FractureSimulation.h)      */
+/* ====================================================================== */
+
+/* largest eigenvalue of a symmetric 3x3 matrix: the trigonometric closed
+   form of the characteristic cubic (O. K. Smith, CACM 4(4):168, 1961) */
+double zro_sym3_max_eigenvalue(const double* A)
+{
+  double a00 = A[0], a11 = A[4], a22 = A[8];
+  double a01 = 0.5 * (A[3] + A[1]), a02 = 0.5 * (A[6] + A[2]), a12 = 0.5 * (A[7] + A[5]);
+  double p1 = a01 * a01 + a02 * a02 + a12 * a12;
+  double q = (a00 + a11 + a22) / 3.0;
+  double b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+  double p2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0 * p1;
+  if (p2 <= 0.0) return q;
+  double p = sqrt(p2 / 6.0);
+  double ip = 1.0 / p;
+  double c00 = b00 * ip, c11 = b11 * ip, c22 = b22 * ip, c01 = a01 * ip, c02 = a02 * ip, c12 = a12 * ip;
+  double det = c00 * (c11 * c22 - c12 * c12) - c01 * (c01 * c22 - c12 * c02) + c02 * (c01 * c12 - c11 * c02);
+  double r = 0.5 * det;
+  r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
+  double phi = acos(r) / 3.0;
+  return q + 2.0 * p * cos(phi);
+}
+
+static double* pf_dup(const double* a, int64_t n, double dflt)
+{
+  double* r = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; ++i) r[i] = a ? a[i] : dflt;
+  return r;
+}
+
+void zro_set_phase_field(zro_sim* s, const double* c, const double* H, const double* l0,
+                         const double* one_over_sigma_c, const double* pf_Fp, const double* H_max,
+                         const double* residual_phase, const int32_t* allow_damage, const zro_pf_opts* o)
+{
+  int64_t n = s->n;
+  if (!s->model) zro_set_materials(s, NULL, NULL, NULL, NULL, NULL);
+  for (int64_t i = 0; i < n; ++i) s->model[i] = ZRO_MODEL_NEO_HOOKEAN_BORDEN;
+  free(s->pf_c); free(s->pf_H); free(s->pf_l0); free(s->pf_osc); free(s->pf_Fp); free(s->pf_Hmax);
+  free(s->pf_k); free(s->pf_vol); free(s->pf_allow);
+  s->pf_c = pf_dup(c, n, 1.0);
+  s->pf_H = pf_dup(H, n, 0.0);
+  s->pf_l0 = pf_dup(l0, n, 0.0);
+  s->pf_osc = pf_dup(one_over_sigma_c, n, 0.0);
+  s->pf_Fp = pf_dup(pf_Fp, n, 1.0);
+  s->pf_Hmax = pf_dup(H_max, n, HUGE_VAL);
+  s->pf_k = pf_dup(residual_phase, n, 0.001);
+  s->pf_vol = pf_dup(NULL, n, 0.0);
+  s->pf_allow = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; ++i) s->pf_allow[i] = allow_damage ? allow_damage[i] : 1;
+  s->pf_o = *o;
+  s->pf_on = 1;
+}
+
+void zro_get_phase_field(const zro_sim* s, double* c, double* H, double* pf_Fp)
+{
+  if (!s->pf_on) return;
+  if (c) memcpy(c, s->pf_c, sizeof(double) * (size_t)s->n);
+  if (H) memcpy(H, s->pf_H, sizeof(double) * (size_t)s->n);
+  if (pf_Fp) memcpy(pf_Fp, s->pf_Fp, sizeof(double) * (size_t)s->n);
+}
+
+static double det_cm(const double* F)
+{
+  return F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) + F[6] * (F[1] * F[5] - F[4] * F[2]);
+}
+
+/* loops over the particles of the groups in sort order, as the reference's
+   particle_group iteration (the colour phases only order its TBB tasks) */
+#define PF_FOR_PARTICLES(s, i)                                           \
+  for (int64_t g_ = 0; g_ < (s)->ngroups; ++g_)                         \
+    for (int64_t k_ = (s)->gfirst[g_]; k_ <= (s)->glast[g_]; ++k_)       \
+      for (int i = (s)->order[k_], once_ = 1; once_; once_ = 0)
+
+/* the lumped operator (FractureSimulation.h:338-403) on node-slot vectors */
+static void pf_multiply(zro_sim* s, const double* x, double* b)
+{
+  const double M = s->pf_o.parabolic_M, dt = s->pf_dt;
+  int64_t ns = s->npages * 32;
+  memset(b, 0, sizeof(double) * (size_t)ns);
+  PF_FOR_PARTICLES(s, i)
+  {
+    zro_spline sp;
+    spline_of(s, s->X + 3 * i, &sp);
+    double gc[3] = { 0, 0, 0 };
+    for (int a = 0; a < 3; ++a)
+      for (int bb = 0; bb < 3; ++bb)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          spline_node(s, &sp, a, bb, c, gw);
+          int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + bb, sp.base[2] + c);
+          if (s->nodes[k].idx < 0) continue;
+          for (int d = 0; d < 3; ++d) gc[d] += x[k] * gw[d];
+        }
+    double vol = s->pf_vol[i], l0 = s->pf_l0[i];
+    for (int d = 0; d < 3; ++d) gc[d] *= vol * 4.0 * l0 * l0;
+    double lump = M > 0 ? vol * (s->pf_Fp[i] * M + 1.0 / dt) : vol * s->pf_Fp[i];
+    double gscale = M > 0 ? M : 1.0;
+    for (int a = 0; a < 3; ++a)
+      for (int bb = 0; bb < 3; ++bb)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          double w = spline_node(s, &sp, a, bb, c, gw);
+          int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + bb, sp.base[2] + c);
+          if (s->nodes[k].idx < 0) continue;
+          b[k] += lump * w * x[k] + (gc[0] * gw[0] + gc[1] * gw[1] + gc[2] * gw[2]) * gscale;
+        }
+  }
+}
+
+static double pf_dot(const zro_sim* s, const double* a, const double* b)
+{
+  double acc = 0.0;
+  for (int64_t k = 0; k < s->npages * 32; ++k)
+    if (s->nodes[k].idx >= 0) acc += a[k] * b[k];
+  return acc;
+}
+
+/* Projects/fracture/ConjugateGradient.h:50-97 with x = 0 on entry */
+static int pf_cg(zro_sim* s, double* x, const double* rhs, const double* diag, double* res)
+{
+  int64_t ns = s->npages * 32;
+  double* r = (double*)calloc((size_t)ns, sizeof(double));
+  double* mr = (double*)calloc((size_t)ns, sizeof(double));
+  double* sd = (double*)calloc((size_t)ns, sizeof(double));
+  double* q = (double*)calloc((size_t)ns, sizeof(double));
+  memset(x, 0, sizeof(double) * (size_t)ns);
+  memcpy(r, rhs, sizeof(double) * (size_t)ns); /* r = b - A*0 */
+  double rho_old = DBL_MAX;
+  double norm = sqrt(pf_dot(s, r, r));
+  double rel = norm * s->pf_o.cg_relative;
+  int it;
+  for (it = 0;; ++it) {
+    for (int64_t k = 0; k < ns; ++k) mr[k] = s->nodes[k].idx >= 0 ? r[k] / diag[k] : 0.0;
+    double rho = pf_dot(s, mr, r);
+    if (it == 0)
+      memcpy(sd, mr, sizeof(double) * (size_t)ns);
+    else
+      for (int64_t k = 0; k < ns; ++k) sd[k] = rho / rho_old * sd[k] + mr[k];
+    pf_multiply(s, sd, q);
+    double sq = pf_dot(s, sd, q);
+    double alpha = sq != 0.0 ? rho / sq : DBL_MAX;
+    for (int64_t k = 0; k < ns; ++k) {
+      x[k] = alpha * sd[k] + x[k];
+      r[k] = -alpha * q[k] + r[k];
+    }
+    rho_old = rho;
+    norm = sqrt(pf_dot(s, r, r));
+    if (norm < s->pf_o.cg_tolerance || norm < rel || it == s->pf_o.cg_max_iterations) break;
+  }
+  free(r); free(mr); free(sd); free(q);
+  if (res) *res = norm;
+  return it;
+}
+
+static void pf_update_model(zro_sim* s, int i)
+{
+  double c = s->pf_c[i], k = s->pf_k[i];
+  s->gdeg[i] = c * c * (1.0 - k) + k;
+}
+
+void zro_phase_field_solve(zro_sim* s, double dt, zro_pf_stats* st)
+{
+  if (!s->pf_on) return;
+  const zro_pf_opts* o = &s->pf_o;
+  if (st) memset(st, 0, sizeof(*st));
+  if (o->naive) { /* solvePhaseFieldSystemNaive (:509-554) */
+    for (int64_t i = 0; i < s->n; ++i) {
+      const double* F = s->F + 9 * i;
+      double J = det_cm(F), tau[9], cauchy[9];
+      zro_nhb_kirchhoff(s->mu[i], s->lam[i], 1.0, F, tau);
+      for (int c = 0; c < 9; ++c) cauchy[c] = tau[c] / J;
+      double smax = zro_sym3_max_eigenvalue(cauchy);
+      double newc = s->pf_c[i];
+      if (smax > o->sigma_F) newc = fmin(newc, o->sigma_F / smax);
+      s->pf_c[i] = newc;
+      pf_update_model(s, (int)i);
+    }
+    return;
+  }
+  s->pf_dt = dt;
+  int64_t ns = s->npages * 32;
+  if (s->pf_cap < ns) {
+    free(s->pf_phase); free(s->pf_mult); free(s->pf_rhs); free(s->pf_diag); free(s->pf_x);
+    s->pf_phase = (double*)malloc(sizeof(double) * (size_t)ns);
+    s->pf_mult = (double*)malloc(sizeof(double) * (size_t)ns);
+    s->pf_rhs = (double*)malloc(sizeof(double) * (size_t)ns);
+    s->pf_diag = (double*)malloc(sizeof(double) * (size_t)ns);
+    s->pf_x = (double*)malloc(sizeof(double) * (size_t)ns);
+    s->pf_cap = ns;
+  }
+  memset(s->pf_phase, 0, sizeof(double) * (size_t)ns);
+  memset(s->pf_mult, 0, sizeof(double) * (size_t)ns);
+  memset(s->pf_rhs, 0, sizeof(double) * (size_t)ns);
+  memset(s->pf_diag, 0, sizeof(double) * (size_t)ns);
+  const double M = o->parabolic_M;
+  /* phaseP2G (:209-257) */
+  if (M > 0) {
+    PF_FOR_PARTICLES(s, i)
+    {
+      zro_spline sp;
+      spline_of(s, s->X + 3 * i, &sp);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int c = 0; c < 3; ++c) {
+            double gw[3];
+            double w = spline_node(s, &sp, a, b, c, gw);
+            int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+            s->pf_mult[k] += w;
+            s->pf_phase[k] += s->pf_c[i] * w;
+          }
+    }
+    for (int64_t k = 0; k < ns; ++k)
+      if (s->nodes[k].idx >= 0) s->pf_phase[k] /= s->pf_mult[k];
+  }
+  for (int64_t i = 0; i < s->n; ++i) s->pf_vol[i] = s->vol[i] * det_cm(s->F + 9 * i);
+  /* rhs (:303-325) and the lumped preconditioner's diagonal (:405-437) */
+  PF_FOR_PARTICLES(s, i)
+  {
+    zro_spline sp;
+    spline_of(s, s->X + 3 * i, &sp);
+    double vol = s->pf_vol[i];
+    double dg = M > 0 ? vol * (M * s->pf_Fp[i] + 1.0 / dt) : vol * s->pf_Fp[i];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          double w = spline_node(s, &sp, a, b, c, gw);
+          int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+          if (s->nodes[k].idx < 0) continue;
+          s->pf_rhs[k] += (M > 0 ? (M + s->pf_phase[k] / dt) * vol * w : vol * w);
+          s->pf_diag[k] += dg * w;
+        }
+  }
+  double res = 0.0;
+  int its = pf_cg(s, s->pf_x, s->pf_rhs, s->pf_diag, &res);
+  if (st) {
+    st->cg_iterations = its;
+    st->residual_norm = res;
+  }
+  for (int64_t k = 0; k < ns; ++k)
+    if (s->nodes[k].idx >= 0) s->pf_phase[k] = M > 0 ? s->pf_x[k] - s->pf_phase[k] : s->pf_x[k];
+  /* phaseG2P (:457-507) */
+  PF_FOR_PARTICLES(s, i)
+  {
+    if (!s->pf_allow[i]) continue;
+    zro_spline sp;
+    spline_of(s, s->X + 3 * i, &sp);
+    double pf = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          double w = spline_node(s, &sp, a, b, c, gw);
+          int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+          if (s->nodes[k].idx >= 0) pf += w * s->pf_phase[k];
+        }
+    double c0 = s->pf_c[i];
+    double nc = M > 0 ? c0 + pf : pf;
+    double cm = c0 < nc ? c0 : nc;
+    s->pf_c[i] = cm > 0.0 ? cm : 0.0;
+  }
+  for (int64_t i = 0; i < s->n; ++i) {
+    double psi = zro_nhb_psi_pos(s->mu[i], s->lam[i], s->F + 9 * i);
+    if (psi > s->pf_H[i]) { /* PhaseField::Update_Phase_Field_Fp (PhaseField.h:70-76) */
+      s->pf_H[i] = psi < s->pf_Hmax[i] ? psi : s->pf_Hmax[i];
+      s->pf_Fp[i] = 4.0 * s->pf_l0[i] * (1.0 - s->pf_k[i]) * s->pf_H[i] * s->pf_osc[i] + 1.0;
+    }
+    pf_update_model(s, (int)i);
+  }
+}
+
+void zro_phase_field_system(const zro_sim* s, double* rhs, double* diag, double* x)
+{
+  size_t ns = (size_t)(s->npages * 32);
+  if (!s->pf_rhs) return;
+  if (rhs) memcpy(rhs, s->pf_rhs, sizeof(double) * ns);
+  if (diag) memcpy(diag, s->pf_diag, sizeof(double) * ns);
+  if (x) memcpy(x, s->pf_x, sizeof(double) * ns);
+}
+
+void zro_phase_field_multiply(zro_sim* s, const double* x, double* b) { pf_multiply(s, x, b); }

@@ -1,0 +1,95 @@
+"""The C++ host side (ziran2019_amd/host/MpmSimulationGpu) drives the engine
+like a Ziran frame loop: explicit and implicit steps run through it match the
+FP64 oracle on the same seeded particles."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from ziran2019_amd import build as zbuild
+from ziran2019_amd.engine import ZrError
+from _helpers import perturbed, rel_err
+
+pytestmark = pytest.mark.gpu
+
+KEYS = (("X", 3), ("V", 3), ("C", 9), ("F", 9), ("mass", 1), ("vol", 1), ("mu", 1), ("lam", 1))
+
+
+def _write(path, s):
+    n = len(s["mass"])
+    with open(path, "wb") as f:
+        f.write(np.int64(n).tobytes())
+        for k, c in KEYS:
+            f.write(np.ascontiguousarray(np.asarray(s[k], dtype=np.float64).reshape(n * c)).tobytes())
+
+
+def _read(path):
+    raw = np.fromfile(path, dtype=np.uint8)
+    n = int(raw[:8].view(np.int64)[0])
+    vals = raw[8:].view(np.float64)
+    out, o = {}, 0
+    for k, c in KEYS:
+        out[k] = vals[o:o + n * c].reshape(n, c) if c > 1 else vals[o:o + n]
+        o += n * c
+    return out
+
+
+def _run(tmp_path, s, steps, dt, symplectic, ground_y, precision=0):
+    exe = zbuild.HOST_EXE
+    if not os.path.exists(exe):
+        raise ZrError(f"{exe} is missing: build it with `python -m ziran2019_amd.build`")
+    src, dst = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write(src, s)
+    cmd = [exe, src, dst, repr(s["dx"]), str(steps), repr(dt), "1" if symplectic else "0", repr(ground_y),
+           str(precision)]
+    print("host step:", " ".join(cmd[3:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return _read(dst), r.stdout
+
+
+def test_host_explicit_steps_match_oracle(tmp_path):
+    s = perturbed(5, shuffle=False)
+    s["V"][:, 1] -= 1.0
+    y0 = float(s["X"][:, 1].min() - 0.5 * s["dx"])
+    out, log = _run(tmp_path, s, 10, 1e-4, True, y0)
+    assert log.count("step ") == 10
+    ora = oracle.OracleSim(dx=s["dx"], gravity=(0.0, -9.8, 0.0), nthreads=8)
+    ora.set_particles(s["X"], s["V"], s["C"], s["F"], s["mass"], s["vol"], s["mu"], s["lam"])
+    ora.add_collider(shape=0, type=1, origin=(0, y0, 0), normal=(0, 1, 0))
+    for _ in range(10):
+        ora.step(1e-4)
+    X, V, C, F = ora.get_particles()
+    assert rel_err(out["X"], X) < 1e-10
+    assert rel_err(out["V"], V) < 1e-10
+    assert rel_err(out["F"], F) < 1e-10
+
+
+def test_host_implicit_steps_match_oracle(tmp_path):
+    s = perturbed(12, n_cells=5, shuffle=False, F_noise=0.02, C_noise=0.1, E=2e4)
+    s["V"][:, 1] -= 0.5
+    y0 = float(s["X"][:, 1].min() + 0.5 * s["dx"])
+    out, log = _run(tmp_path, s, 3, 2e-3, False, y0)
+    assert log.count("newton") == 3
+    ora = oracle.OracleSim(dx=s["dx"], gravity=(0.0, -9.8, 0.0), nthreads=8)
+    ora.set_particles(s["X"], s["V"], s["C"], s["F"], s["mass"], s["vol"], s["mu"], s["lam"])
+    ora.add_collider(shape=0, type=1, origin=(0, y0, 0), normal=(0, 1, 0))
+    for _ in range(3):
+        ora.step_implicit(2e-3)
+    X, V, C, F = ora.get_particles()
+    assert rel_err(out["X"], X) < 1e-9
+    assert rel_err(out["V"], V) < 1e-9
+    assert rel_err(out["F"], F) < 1e-9
+
+
+def test_host_reports_domain_error(tmp_path):
+    s = perturbed(7, shuffle=False)
+    s["X"][0] = (-0.5, 0.5, 0.5)  # below the grid's lower corner
+    src, dst = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write(src, s)
+    r = subprocess.run([zbuild.HOST_EXE, src, dst, repr(s["dx"]), "1", "1e-4", "1"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1
+    assert "status -3" in r.stderr

@@ -28,6 +28,24 @@ FLAGS = [
 ]
 
 
+HOST = os.path.join(HERE, "host")
+HOST_EXE = os.path.join(LIBDIR, "zr_host_step")
+
+
+def build_host(verbose=False):
+    """The C++ host side (MpmSimulationGpu) and its driver program, linked
+    against libzr_mpm.so with g++."""
+    tmp = HOST_EXE + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(HERE, "..", "include"), "-I", HOST,
+           os.path.join(HOST, "MpmSimulationGpu.cpp"), os.path.join(HOST, "zr_host_step.cpp"),
+           "-L", LIBDIR, "-lzr_mpm", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, HOST_EXE)
+    return HOST_EXE
+
+
 def build(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
     src = os.path.join(CSRC, "zr_engine.hip")
@@ -37,6 +55,7 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -337,7 +337,22 @@ struct ParticlesView {
   const double* mu;
   const double* lam;
   int64_t stride;
+  const int32_t* mat;   // model | plasticity << 4 per particle; null: all fixed-corotated
+  const double* gdeg;   // NeoHookeanBorden degradation g
 };
+
+// constitutive_model.kirchhoff of the particle's model (FBasedMpmForceHelper.cpp:44-64)
+__device__ __forceinline__ void particle_kirchhoff(const ParticlesView& P, uint32_t i, const M3<double>& f,
+                                                   M3<double>& tau)
+{
+  const int model = P.mat ? (P.mat[i] & 15) : ZR_MODEL_FIXED_COROTATED;
+  if (model == ZR_MODEL_STVK_HENCKY)
+    hencky_kirchhoff(P.mu[i], P.lam[i], f, tau);
+  else if (model == ZR_MODEL_NEO_HOOKEAN_BORDEN)
+    nhb_kirchhoff(P.mu[i], P.lam[i], P.gdeg[i], f, tau);
+  else
+    fcr_kirchhoff(P.mu[i], P.lam[i], f, tau);
+}
 
 // FORCE = true: MLS with force (explicit).  FORCE = false: APIC mass and
 // momentum only, particlesToGridHelper<APIC> (MpmSimulationBase.cpp:563-608).
@@ -378,7 +393,7 @@ __global__ __launch_bounds__(256) void k_p2g(ParticlesView P, const uint32_t* __
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
-      fcr_kirchhoff(P.mu[i], P.lam[i], f, tau);
+      particle_kirchhoff(P, i, f, tau);
       const double vol = P.vol[i];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -618,6 +633,41 @@ __global__ __launch_bounds__(256) void k_g2p(ParticlesView P, const uint32_t* __
 }
 
 // ---------------------------------------------------------------------------
+// applyPlasticity after the F update (MpmSimulationBase.cpp:1015-1030): the
+// return map of each particle's plasticity model, in FP64.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_plasticity(uint32_t n, const int32_t* __restrict__ mat, double* F,
+                                                    int64_t S, const double* __restrict__ mu,
+                                                    const double* __restrict__ lam,
+                                                    const double* __restrict__ pparam, double* pstate,
+                                                    int32_t* broken)
+{
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int plastic = mat[i] >> 4;
+  if (plastic == ZR_PLASTIC_NONE) return;
+  M3<double> f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) f.a[r][c] = F[(c * 3 + r) * S + i];
+  double st = pstate[i];
+  if (plastic == ZR_PLASTIC_VON_MISES) {
+    bool br = false;
+    if (!von_mises_project(mu[i], lam[i], pparam[i], pparam[S + i], st, f, br)) return;
+    if (br) broken[i] = 1;
+  } else {
+    drucker_prager_project(mu[i], lam[i], pparam[i], pparam[S + i], pparam[2 * S + i], pparam[3 * S + i] != 0.0,
+                           st, f);
+  }
+  pstate[i] = st;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) F[(c * 3 + r) * S + i] = f.a[r][c];
+}
+
+// ---------------------------------------------------------------------------
 // layout conversion and introspection
 // ---------------------------------------------------------------------------
 __global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, uint32_t n, int ncomp, int64_t stride)
@@ -682,6 +732,12 @@ struct zr_engine {
   int64_t n = 0;
   double *x = nullptr, *v = nullptr, *C = nullptr, *F = nullptr;
   double *m = nullptr, *vol = nullptr, *mu = nullptr, *lam = nullptr;
+  // materials (null until zr_set_materials): model | plasticity << 4,
+  // plasticity parameters [4][n], plastic state, degradation, broken flags
+  int32_t* mat = nullptr;
+  double *pparam = nullptr, *pstate = nullptr, *gdeg = nullptr;
+  int32_t* broken = nullptr;
+  bool has_models = false, has_plastic = false;
   uint32_t *p_page = nullptr, *p_re = nullptr, *order = nullptr;
   uint64_t* keys = nullptr;
   uint32_t *group_page = nullptr, *group_count = nullptr, *group_start = nullptr;
@@ -812,6 +868,8 @@ void free_particles(zr_engine* e)
   dfree(e->p_page); dfree(e->p_re); dfree(e->order); dfree(e->keys);
   dfree(e->group_page); dfree(e->group_count); dfree(e->group_start);
   dfree(e->touch_page); dfree(e->touch_slot); dfree(e->slot_page); dfree(e->group_nslot);
+  dfree(e->mat); dfree(e->pparam); dfree(e->pstate); dfree(e->gdeg); dfree(e->broken);
+  e->has_models = e->has_plastic = false;
   e->n = 0;
 }
 
@@ -826,7 +884,8 @@ ColliderSet collider_set(const zr_engine* e)
 
 ParticlesView pview(zr_engine* e)
 {
-  return ParticlesView{ e->x, e->v, e->C, e->F, e->m, e->vol, e->mu, e->lam, e->n };
+  return ParticlesView{ e->x, e->v, e->C, e->F, e->m, e->vol, e->mu, e->lam, e->n,
+                        e->has_models ? e->mat : nullptr, e->gdeg };
 }
 
 int ensure_grid(zr_engine* e)
@@ -971,6 +1030,11 @@ int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
                                               e->gvs, (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt,
                                               e->cfg.cfl, e->dcount);
   ZR_LAUNCH_CHECK(e);
+  if (e->has_plastic) {
+    k_plasticity<<<blocks_for((uint64_t)e->n, 256), 256, 0, e->stream>>>(
+        (uint32_t)e->n, e->mat, e->F, e->n, e->mu, e->lam, e->pparam, e->pstate, e->broken);
+    ZR_LAUNCH_CHECK(e);
+  }
   prof_mark(e, ZR_PROF_G2P);
   int rc = sync_counts(e);
   if (rc) return rc;
@@ -1402,6 +1466,76 @@ int zr_download_particles(zr_engine* e, zr_particles* p)
   return ZR_OK;
 }
 
+int zr_set_materials(zr_engine* e, const zr_materials* mt)
+{
+  if (!e || !mt) return fail(e, ZR_ERR_ARG, "bad materials view");
+  if (mt->n != e->n) return fail(e, ZR_ERR_ARG, "material count differs from the particle count");
+  const int64_t n = e->n;
+  if (n == 0) return ZR_OK;
+  const size_t N = (size_t)n;
+  std::vector<int32_t> code(N, 0);
+  bool models = false, plastic = false;
+  for (size_t i = 0; i < N; ++i) {
+    const int32_t md = mt->model ? mt->model[i] : ZR_MODEL_FIXED_COROTATED;
+    const int32_t pl = mt->plasticity ? mt->plasticity[i] : ZR_PLASTIC_NONE;
+    if (md < ZR_MODEL_FIXED_COROTATED || md > ZR_MODEL_NEO_HOOKEAN_BORDEN || pl < ZR_PLASTIC_NONE ||
+        pl > ZR_PLASTIC_DRUCKER_PRAGER)
+      return fail(e, ZR_ERR_ARG, "unknown model or plasticity code");
+    models = models || md != ZR_MODEL_FIXED_COROTATED;
+    plastic = plastic || pl != ZR_PLASTIC_NONE;
+    code[i] = md | (pl << 4);
+  }
+  if (plastic && (!mt->plastic_param || !mt->plastic_state))
+    return fail(e, ZR_ERR_ARG, "plasticity needs plastic_param and plastic_state");
+  if (!e->mat) {
+    ZR_HIP(e, hipMalloc(&e->mat, N * sizeof(int32_t)));
+    ZR_HIP(e, hipMalloc(&e->pparam, 4 * N * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->pstate, N * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->gdeg, N * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->broken, N * sizeof(int32_t)));
+  }
+  std::vector<double> pp(4 * N, 0.0), ps(N, 0.0), g(N, 1.0);
+  for (size_t i = 0; i < N; ++i) {
+    for (int k = 0; k < 4; ++k)
+      if (mt->plastic_param) pp[k * N + i] = mt->plastic_param[4 * i + k];  // particle-major -> [4][n]
+    if (mt->plastic_state) ps[i] = mt->plastic_state[i];
+    if (mt->degradation) g[i] = mt->degradation[i];
+  }
+  ZR_HIP(e, hipMemcpyAsync(e->mat, code.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->pparam, pp.data(), 4 * N * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->pstate, ps.data(), N * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->gdeg, g.data(), N * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  ZR_HIP(e, hipMemsetAsync(e->broken, 0, N * sizeof(int32_t), e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  e->has_models = models;
+  e->has_plastic = plastic;
+  return ZR_OK;
+}
+
+int zr_get_materials(zr_engine* e, zr_materials* mt)
+{
+  if (!e || !mt) return fail(e, ZR_ERR_ARG, "bad materials view");
+  if (mt->n != e->n) return fail(e, ZR_ERR_ARG, "material count differs from the particle count");
+  const size_t N = (size_t)e->n;
+  if (N == 0) return ZR_OK;
+  if (!e->mat) {  // never set: defaults
+    for (size_t i = 0; i < N; ++i) {
+      if (mt->plastic_state) mt->plastic_state[i] = 0.0;
+      if (mt->degradation) mt->degradation[i] = 1.0;
+      if (mt->broken) mt->broken[i] = 0;
+    }
+    return ZR_OK;
+  }
+  if (mt->plastic_state)
+    ZR_HIP(e, hipMemcpyAsync(mt->plastic_state, e->pstate, N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  if (mt->degradation)
+    ZR_HIP(e, hipMemcpyAsync(mt->degradation, e->gdeg, N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  if (mt->broken)
+    ZR_HIP(e, hipMemcpyAsync(mt->broken, e->broken, N * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
 int zr_sort_particles_and_pollute_grid(zr_engine* e, zr_step_stats* st)
 {
   if (!e) return ZR_ERR_ARG;
@@ -1470,9 +1604,19 @@ static bool valid_opts(const zr_implicit_opts* o)
          (o->precision == ZR_FP64 || o->precision == ZR_FP32_REFINED);
 }
 
+static bool implicit_materials_ok(zr_engine* e)
+{
+  if (e->has_models || e->has_plastic) {
+    fail(e, ZR_ERR_ARG, "the implicit step supports fixed-corotated particles without plasticity only");
+    return false;
+  }
+  return true;
+}
+
 int zr_backward_euler_step(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats* st)
 {
   if (!e) return ZR_ERR_ARG;
+  if (!implicit_materials_ok(e)) return ZR_ERR_ARG;
   if (!(dt > 0.0) || !valid_opts(o)) return fail(e, ZR_ERR_ARG, "bad dt or implicit options");
   return do_be(e, dt, o, st);
 }
@@ -1488,6 +1632,7 @@ int zr_advance_one_time_step_implicit(zr_engine* e, double dt, const zr_implicit
 {
   if (!e) return ZR_ERR_ARG;
   if (!(dt > 0.0) || !valid_opts(o)) return fail(e, ZR_ERR_ARG, "bad dt or implicit options");
+  if (!implicit_materials_ok(e)) return ZR_ERR_ARG;
   int rc = do_sort(e, st);
   if (rc) return rc;
   rc = do_p2g(e, false);

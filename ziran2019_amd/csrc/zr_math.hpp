@@ -454,4 +454,145 @@ ZR_HD void fcr_kirchhoff(T mu, T lam, const M3<T>& f, M3<T>& tau)
     }
 }
 
+// ---------------------------------------------------------------------------
+// StvkWithHencky (StvkWithHencky.cpp:114-143): with F = U S V^T,
+//   tau = P F^T = U diag(2 mu log|s_i| + lam sum_k log|s_k|) U^T.
+// ---------------------------------------------------------------------------
+template <class T>
+ZR_HD void hencky_kirchhoff(T mu, T lam, const M3<T>& f, M3<T>& tau)
+{
+  M3<T> u, v;
+  T s[3];
+  svd3(f, u, s, v);
+  T l[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) l[i] = log(fabs(s[i]));
+  const T tr = l[0] + l[1] + l[2];
+  T d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) d[i] = (T)2 * mu * l[i] + lam * tr;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      tau.a[i][j] = u.a[i][0] * d[0] * u.a[j][0] + u.a[i][1] * d[1] * u.a[j][1] + u.a[i][2] * d[2] * u.a[j][2];
+}
+
+// ---------------------------------------------------------------------------
+// NeoHookeanBorden with degradation g (NeoHookeanBorden.cpp:16-86):
+//   kappa = 0.666666 mu + lambda, tau_dev = mu J^(-2/3) dev(F F^T),
+//   tau_vol = J kappa/2 (J - 1/J) I; g scales tau_dev (and tau_vol if J >= 1).
+// ---------------------------------------------------------------------------
+template <class T>
+ZR_HD void nhb_kirchhoff(T mu, T lam, T g, const M3<T>& f, M3<T>& tau)
+{
+  const T J = det3(f);
+  M3<T> b;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) b.a[i][j] = f.a[i][0] * f.a[j][0] + f.a[i][1] * f.a[j][1] + f.a[i][2] * f.a[j][2];
+  const T trb = b.a[0][0] + b.a[1][1] + b.a[2][2];
+  const T sdev = mu * pow(J, (T)(-2.0 / 3.0));
+  const T kappa = (T)0.666666 * mu + lam;
+  const T vol = J * (kappa / (T)2 * (J - (T)1 / J));
+  const bool tension = J >= (T)1;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const T dev = sdev * (b.a[i][j] - (i == j ? trb / (T)3 : (T)0));
+      const T v = i == j ? vol : (T)0;
+      tau.a[i][j] = tension ? g * (dev + v) : g * dev + v;
+    }
+}
+
+template <class T>
+ZR_HD void usv_t(const M3<T>& u, const T s[3], const M3<T>& v, M3<T>& f)
+{
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      f.a[i][j] = u.a[i][0] * s[0] * v.a[j][0] + u.a[i][1] * s[1] * v.a[j][1] + u.a[i][2] * s[2] * v.a[j][2];
+}
+
+// VonMisesStvkHencky::projectStrain (PlasticityApplier.cpp:464-501).
+// Returns true when f was projected; yield_stress hardens by xi * dgamma;
+// broken is set when the projected first principal stress reaches fail_stress.
+template <class T>
+ZR_HD bool von_mises_project(T mu, T lam, T xi, T fail_stress, T& yield_stress, M3<T>& f, bool& broken)
+{
+  M3<T> u, v;
+  T s[3], e[3], h[3];
+  svd3(f, u, s, v);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) e[i] = log(s[i] > (T)1e-4 ? s[i] : (T)1e-4);
+  const T tr = e[0] + e[1] + e[2];
+  T n2 = (T)0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    h[i] = e[i] - tr / (T)3;
+    n2 += h[i] * h[i];
+  }
+  const T nrm = sqrt(n2);
+  const T dg = nrm - yield_stress / ((T)2 * mu);
+  if (dg <= (T)0) return false;
+  yield_stress -= xi * dg;
+  T H[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) H[i] = e[i] - (dg / nrm) * h[i];
+  const T tau0 = (T)2 * mu * H[0] + lam * (H[0] + H[1] + H[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) H[i] = exp(H[i]);
+  usv_t(u, H, v, f);
+  if (tau0 >= fail_stress) broken = true;
+  return true;
+}
+
+// DruckerPragerStvkHencky::projectStrain (PlasticityApplier.cpp:275-316).
+template <class T>
+ZR_HD void drucker_prager_project(T mu, T lam, T alpha, T beta, T cohesion, bool volume_correction, T& logJp,
+                                  M3<T>& f)
+{
+  M3<T> u, v;
+  T s[3], e[3], h[3];
+  svd3(f, u, s, v);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const T a = fabs(s[i]);
+    e[i] = log(a > (T)1e-4 ? a : (T)1e-4) - cohesion;
+  }
+  const T sum = e[0] + e[1] + e[2];
+  const T tr = sum + logJp;
+  T n2 = (T)0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    h[i] = e[i] - tr / (T)3;
+    n2 += h[i] * h[i];
+  }
+  T H[3];
+  if (tr >= (T)0) {  // case II: the tip
+    const T c = exp(cohesion);
+    H[0] = H[1] = H[2] = c;
+    usv_t(u, H, v, f);
+    if (volume_correction) logJp = beta * sum + logJp;
+    return;
+  }
+  if (mu == (T)0) return;
+  logJp = (T)0;
+  const T nrm = sqrt(n2);
+  const T dg = nrm + ((T)3 * lam + (T)2 * mu) / ((T)2 * mu) * tr * alpha;
+  if (dg <= (T)0) {  // case I: inside
+#pragma unroll
+    for (int i = 0; i < 3; ++i) H[i] = e[i] + cohesion;
+  } else {  // case III: onto the cone
+#pragma unroll
+    for (int i = 0; i < 3; ++i) H[i] = e[i] - (dg / nrm) * h[i] + cohesion;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) H[i] = exp(H[i]);
+  usv_t(u, H, v, f);
+}
+
 }  // namespace zr

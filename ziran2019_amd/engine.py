@@ -43,7 +43,10 @@ EXPORTS = (
     "zr_advance_one_time_step", "zr_debug_size", "zr_debug_dump", "zr_profile_enable", "zr_profile_read",
     "zr_particles_to_grid_apic", "zr_backward_euler_step", "zr_grid_to_particles_apic",
     "zr_advance_one_time_step_implicit", "zr_implicit_residual", "zr_implicit_multiply",
+    "zr_set_materials", "zr_get_materials",
 )
+ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
+ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER = 0, 1, 2
 PROF = dict(p2g=0, grid=1, g2p=2, sort=3, state=4, hvp=5)
 
 
@@ -99,6 +102,21 @@ class ZrParticles(ctypes.Structure):
                 ("vol", _dp), ("mu", _dp), ("lam", _dp)]
 
 
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class ZrMaterials(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("model", _ip), ("plasticity", _ip), ("plastic_param", _dp),
+                ("plastic_state", _dp), ("degradation", _dp), ("broken", _ip)]
+
+
+def drucker_prager_alpha(friction_angle_deg):
+    """alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)) with the reference's
+    pi = 3.141592653 (PlasticityApplier.cpp:249-260)."""
+    s = np.sin(friction_angle_deg / 180.0 * 3.141592653)
+    return np.sqrt(2.0 / 3.0) * 2.0 * s / (3.0 - s)
+
+
 class ZrStepStats(ctypes.Structure):
     _fields_ = [("num_groups", ctypes.c_int64), ("num_pages", ctypes.c_int64), ("cfl_flags", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -144,6 +162,8 @@ def load_library(path=LIB_PATH):
                                                     ctypes.POINTER(ZrImplicitStats), ctypes.POINTER(ZrStepStats)]
     L.zr_implicit_residual.argtypes = [vp, vp, vp]
     L.zr_implicit_multiply.argtypes = [vp, vp, vp]
+    L.zr_set_materials.argtypes = [vp, ctypes.POINTER(ZrMaterials)]
+    L.zr_get_materials.argtypes = [vp, ctypes.POINTER(ZrMaterials)]
     L.zr_profile_enable.argtypes = [vp, ctypes.c_int32]
     L.zr_profile_read.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_int64)]
@@ -230,6 +250,35 @@ class MpmEngine:
         p = ZrParticles(n, *[_dptr(keep[k]) for k in ("X", "V", "C", "F", "mass", "vol", "mu", "lam")])
         self._check(self.L.zr_upload_particles(self.h, ctypes.byref(p)), "zr_upload_particles")
         self.n = n
+
+    def set_materials(self, model=None, plasticity=None, plastic_param=None, plastic_state=None, degradation=None):
+        """Per-particle models (ZR_MODEL_*), plasticity (ZR_PLASTIC_*), its 4
+        parameters per particle, its state and NeoHookeanBorden's g; scalars
+        broadcast to every particle."""
+        n = self.n
+
+        def i32(a):
+            return None if a is None else np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.int32)
+
+        def f64(a, shape):
+            return None if a is None else np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64),
+                                                                               shape))
+        m, p = i32(model), i32(plasticity)
+        q = f64(None if plastic_param is None else np.asarray(plastic_param, dtype=np.float64).reshape(-1, 4),
+                (n, 4))
+        st, g = f64(plastic_state, (n,)), f64(degradation, (n,))
+        mt = ZrMaterials(n, m.ctypes.data_as(_ip) if m is not None else None,
+                         p.ctypes.data_as(_ip) if p is not None else None, _dptr(q), _dptr(st), _dptr(g), None)
+        self._check(self.L.zr_set_materials(self.h, ctypes.byref(mt)), "set_materials")
+
+    def get_materials(self):
+        n = self.n
+        st = np.zeros(n)
+        g = np.zeros(n)
+        b = np.zeros(n, dtype=np.int32)
+        mt = ZrMaterials(n, None, None, None, _dptr(st), _dptr(g), b.ctypes.data_as(_ip))
+        self._check(self.L.zr_get_materials(self.h, ctypes.byref(mt)), "get_materials")
+        return dict(plastic_state=st, degradation=g, broken=b)
 
     def download(self):
         n = self.n

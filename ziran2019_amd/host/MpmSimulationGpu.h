@@ -1,0 +1,109 @@
+// MpmSimulationGpu.h -- C++ host side of the MI355X MPM engine.
+//
+// A class with the step API of the reference's MpmSimulationBase<double,3>
+// (Lib/MPM/MpmSimulationBase.h:143-232) that owns its particle arrays in the
+// reference's layout (one std::vector per attribute, 3 or 9 doubles per
+// particle, matrices column-major; DataArray.h:43-53) and forwards every step
+// virtual to the C-ABI of libzr_mpm.so.  Errors: a CFL violation throws
+// RestartException (SimulationBase.h:21-29); every other failure throws
+// std::runtime_error with the engine's message where the reference would
+// abort through ZIRAN_ASSERT (Debug.h:19-30).
+#ifndef ZIRAN_AMD_MPM_SIMULATION_GPU_H
+#define ZIRAN_AMD_MPM_SIMULATION_GPU_H
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "zr_mpm.h"
+
+namespace ziran_amd {
+
+class RestartException : public std::runtime_error {
+public:
+  explicit RestartException(const std::string& what) : std::runtime_error(what) {}
+};
+
+// The particle attributes the step reads and writes (Particles m/X/V, the
+// APIC matrix C_name<TM>(), and FBasedMpmForceHelper's F, V0 and model).
+struct ParticleArrays {
+  std::vector<std::array<double, 3>> X, V;
+  std::vector<std::array<double, 9>> C, F;
+  std::vector<double> mass, vol, mu, lambda;
+  size_t count() const { return X.size(); }
+  void resize(size_t n);
+};
+
+struct CollisionObject {
+  int32_t shape = ZR_HALF_SPACE;  // ZR_HALF_SPACE or ZR_SPHERE
+  int32_t type = ZR_STICKY;       // AnalyticCollisionObject::COLLISION_OBJECT_TYPE
+  double friction = 0.0;
+  std::array<double, 3> origin{ { 0.0, 0.0, 0.0 } };
+  std::array<double, 3> normal{ { 0.0, 1.0, 0.0 } };
+  double radius = 0.0;
+  std::array<double, 3> velocity{ { 0.0, 0.0, 0.0 } };
+};
+
+class MpmSimulationGpu {
+public:
+  // MpmSimulationBase members that configure the step (MpmSimulationBase.h:52-118)
+  double dx = 0.0;
+  std::array<double, 3> gravity{ { 0.0, 0.0, 0.0 } };
+  double cfl = 0.6;
+  bool symplectic = true;
+  int newton_max_iterations = 3;           // MpmSimulationBase.cpp:51
+  double newton_tolerance = 1e-5;
+  int cg_max_iterations = 20;              // BackwardEuler.h:90-91
+  double cg_tolerance = 16 * 2.220446049250313e-16;  // LinearSolver.h:26
+  double cg_relative_tolerance = 0.0;      // <= 0: Newton-adaptive
+  int32_t implicit_precision = ZR_FP64;    // or ZR_FP32_REFINED
+  double time = 0.0;                       // SimulationBase::step.time
+
+  ParticleArrays particles;
+
+  MpmSimulationGpu() = default;
+  MpmSimulationGpu(const MpmSimulationGpu&) = delete;
+  MpmSimulationGpu& operator=(const MpmSimulationGpu&) = delete;
+  virtual ~MpmSimulationGpu();
+
+  // addCollisionObject (MpmSimulationBase.cpp:84-88)
+  int addCollisionObject(const CollisionObject& c);
+
+  // initialize / reinitialize: create the engine and upload the particles
+  virtual void initialize();
+  virtual void reinitialize();
+
+  // MpmSimulationBase::advanceOneTimeStep (MpmSimulationBase.cpp:251-296)
+  virtual void advanceOneTimeStep(double dt);
+
+  // the individual step virtuals
+  virtual void sortParticlesAndPolluteGrid();   // MpmSimulationBase.cpp:1032-1103
+  virtual void particlesToGrid();               // :455-608 (MLS for symplectic, APIC for implicit)
+  virtual void gridVelocityExplicitUpdate(double dt);  // :818-877
+  virtual void backwardEulerStep(double dt);    // :610-657
+  virtual void gridToParticles(double dt);      // :879-1016
+
+  // particles on the host after a step (the reference's particle arrays are
+  // always current; here they are downloaded on demand)
+  void downloadParticles();
+
+  const zr_implicit_stats& lastImplicitStats() const { return implicit_stats_; }
+  const zr_step_stats& lastStepStats() const { return step_stats_; }
+
+private:
+  zr_engine* eng_ = nullptr;
+  std::vector<CollisionObject> colliders_;
+  zr_implicit_stats implicit_stats_{};
+  zr_step_stats step_stats_{};
+
+  void check(int rc) const;
+  zr_implicit_opts implicitOpts() const;
+  zr_particles view();
+  void uploadColliders();
+};
+
+}  // namespace ziran_amd
+
+#endif

@@ -1,0 +1,119 @@
+// zr_host_step -- runs MpmSimulationGpu (the C++ host side) on a particle
+// file: the way a Ziran driver (SimulationBase::advanceOneFrame,
+// SimulationBase.h:304-308) calls the step, without Python in the loop.
+//
+//   zr_host_step IN OUT dx steps dt symplectic(1|0) [ground_y] [precision]
+//
+// IN/OUT: int64 n, then X 3n, V 3n, C 9n, F 9n, mass n, vol n, mu n,
+// lambda n doubles (C and F column-major per particle).  ground_y adds a
+// sticky half-space y >= ground_y.  Gravity is (0, -9.8, 0).  Prints one
+// line per step: step, groups, pages, and for implicit steps Newton and CG
+// iteration counts.  Exit status 0 on success, 3 on a CFL restart, 1 on any
+// other error.
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "MpmSimulationGpu.h"
+
+using ziran_amd::MpmSimulationGpu;
+using ziran_amd::ParticleArrays;
+
+namespace {
+
+template <class T>
+bool rd(FILE* f, T* p, size_t count)
+{
+  return std::fread(p, sizeof(T), count, f) == count;
+}
+template <class T>
+bool wr(FILE* f, const T* p, size_t count)
+{
+  return std::fwrite(p, sizeof(T), count, f) == count;
+}
+
+bool read_particles(const char* path, ParticleArrays& p)
+{
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return false;
+  int64_t n = 0;
+  bool ok = rd(f, &n, 1) && n >= 0;
+  if (ok) {
+    p.resize((size_t)n);
+    const size_t m = (size_t)n;
+    ok = (m == 0 || (rd(f, p.X[0].data(), 3 * m) && rd(f, p.V[0].data(), 3 * m) && rd(f, p.C[0].data(), 9 * m) &&
+                     rd(f, p.F[0].data(), 9 * m))) &&
+         rd(f, p.mass.data(), m) && rd(f, p.vol.data(), m) && rd(f, p.mu.data(), m) && rd(f, p.lambda.data(), m);
+  }
+  std::fclose(f);
+  return ok;
+}
+
+bool write_particles(const char* path, const ParticleArrays& p)
+{
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return false;
+  const int64_t n = (int64_t)p.count();
+  const size_t m = p.count();
+  bool ok = wr(f, &n, 1) &&
+            (m == 0 || (wr(f, p.X[0].data(), 3 * m) && wr(f, p.V[0].data(), 3 * m) &&
+                        wr(f, p.C[0].data(), 9 * m) && wr(f, p.F[0].data(), 9 * m))) &&
+            wr(f, p.mass.data(), m) && wr(f, p.vol.data(), m) && wr(f, p.mu.data(), m) &&
+            wr(f, p.lambda.data(), m);
+  return std::fclose(f) == 0 && ok;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+  if (argc < 7) {
+    std::fprintf(stderr, "usage: %s IN OUT dx steps dt symplectic [ground_y] [precision]\n", argv[0]);
+    return 1;
+  }
+  MpmSimulationGpu sim;
+  if (!read_particles(argv[1], sim.particles)) {
+    std::fprintf(stderr, "cannot read %s\n", argv[1]);
+    return 1;
+  }
+  sim.dx = std::atof(argv[3]);
+  const int steps = std::atoi(argv[4]);
+  const double dt = std::atof(argv[5]);
+  sim.symplectic = std::atoi(argv[6]) != 0;
+  sim.gravity = { { 0.0, -9.8, 0.0 } };
+  if (argc > 7) {
+    ziran_amd::CollisionObject ground;
+    ground.shape = ZR_HALF_SPACE;
+    ground.type = ZR_STICKY;
+    ground.origin = { { 0.0, std::atof(argv[7]), 0.0 } };
+    ground.normal = { { 0.0, 1.0, 0.0 } };
+    sim.addCollisionObject(ground);
+  }
+  if (argc > 8) sim.implicit_precision = std::atoi(argv[8]);
+  try {
+    sim.initialize();
+    for (int s = 0; s < steps; ++s) {
+      sim.advanceOneTimeStep(dt);
+      const zr_step_stats& st = sim.lastStepStats();
+      if (sim.symplectic)
+        std::printf("step %d groups %lld pages %lld\n", s, (long long)st.num_groups, (long long)st.num_pages);
+      else
+        std::printf("step %d groups %lld pages %lld newton %d cg %d\n", s, (long long)st.num_groups,
+                    (long long)st.num_pages, sim.lastImplicitStats().newton_iterations,
+                    sim.lastImplicitStats().cg_iterations);
+      std::fflush(stdout);
+    }
+    sim.downloadParticles();
+  } catch (const ziran_amd::RestartException& ex) {
+    std::fprintf(stderr, "restart: %s\n", ex.what());
+    return 3;
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "error: %s\n", ex.what());
+    return 1;
+  }
+  if (!write_particles(argv[2], sim.particles)) {
+    std::fprintf(stderr, "cannot write %s\n", argv[2]);
+    return 1;
+  }
+  return 0;
+}
