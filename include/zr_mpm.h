@@ -110,6 +110,41 @@ typedef struct zr_materials {
   int32_t* broken;         /* n, download only: von Mises fail_stress reached */
 } zr_materials;
 
+/* CD-MPM phase field (Projects/fracture/PhaseField.h:13-21), per particle.
+   NULL arrays take the defaults of addFBasedMpmForceWithPhaseField
+   (MpmParticleHandleBase.cpp:105-114): c 1, H 0, pf_Fp 1, H_max +inf,
+   residual_phase 0.001, allow_damage 1 (l0 and one_over_sigma_c have none). */
+typedef struct zr_phase_field {
+  int64_t n;
+  double* c;                 /* damage, 1 intact .. 0 broken (updated) */
+  double* H;                 /* history of psi+ (updated) */
+  double* l0;
+  double* one_over_sigma_c;
+  double* pf_Fp;             /* (updated) */
+  double* H_max;
+  double* residual_phase;
+  int32_t* allow_damage;
+} zr_phase_field;
+
+/* FractureSimulation options (FractureSimulation.h:50-73,439;
+   fracture/ConjugateGradient.h:64): parabolic_M, CG(100), tolerance 1e-6,
+   relative 1e-12 of the initial residual; naive = solvePhaseFieldSystemNaive
+   with threshold sigma_F.  Only the lumped form (lumping = true) exists. */
+typedef struct zr_phase_field_opts {
+  double parabolic_M;
+  int32_t cg_max_iterations;
+  int32_t naive;
+  double cg_tolerance;
+  double cg_relative;
+  double sigma_F;
+} zr_phase_field_opts;
+
+typedef struct zr_phase_field_stats {
+  int32_t cg_iterations;     /* the fracture CG's return value */
+  int32_t reserved;
+  double residual_norm;
+} zr_phase_field_stats;
+
 typedef struct zr_step_stats {
   int64_t num_groups;   /* particle pages (particle_group.size()) */
   int64_t num_pages;    /* touched SPGrid pages (page_map->Get_Blocks().second) */
@@ -164,6 +199,20 @@ int zr_download_particles(zr_engine* e, zr_particles* p);
    fills plastic_state, degradation and broken (NULL pointers skip). */
 int zr_set_materials(zr_engine* e, const zr_materials* m);
 int zr_get_materials(zr_engine* e, zr_materials* m);
+
+/* FractureSimulation: attaches a phase field to every particle and switches
+   them to NeoHookeanBorden (addFBasedMpmForceWithPhaseField,
+   MpmParticleHandleBase.cpp:95-118).  From then on zr_advance_one_time_step
+   runs the damage solve after P2G (FractureSimulation::particlesToGrid,
+   FractureSimulation.h:149-160). */
+int zr_set_phase_field(zr_engine* e, const zr_phase_field* pf, const zr_phase_field_opts* o);
+/* downloads c, H and pf_Fp (NULL pointers skip) */
+int zr_get_phase_field(zr_engine* e, zr_phase_field* pf);
+/* solvePhaseFieldSystem / solvePhaseFieldSystemNaive (FractureSimulation.h:259-554)
+   on the grid of the last zr_particles_to_grid */
+int zr_phase_field_solve(zr_engine* e, double dt, zr_phase_field_stats* st);
+/* statistics of the last damage solve (also the one inside a step) */
+int zr_phase_field_last_stats(zr_engine* e, zr_phase_field_stats* st);
 
 /* MpmSimulationBase::sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1032-1103) */
 int zr_sort_particles_and_pollute_grid(zr_engine* e, zr_step_stats* st);

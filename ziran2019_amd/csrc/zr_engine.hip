@@ -520,6 +520,7 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
 }
 
 #include "zr_implicit.hpp"
+#include "zr_damage.hpp"
 
 // ---------------------------------------------------------------------------
 // G2P, MLS/APIC (MpmSimulationBase.cpp:906-1016) + evolveStrain
@@ -738,6 +739,18 @@ struct zr_engine {
   double *pparam = nullptr, *pstate = nullptr, *gdeg = nullptr;
   int32_t* broken = nullptr;
   bool has_models = false, has_plastic = false;
+  // phase field: per particle (c, H, l0, 1/sigma_c, pf_Fp, H_max, k, vol, allow)
+  // and per node (phase, rhs, D, x, r, mr, s, q) of the damage solve
+  bool pf_on = false;
+  zr_phase_field_opts pf_opts{};
+  zr_phase_field_stats pf_stats{};
+  double *pf_c = nullptr, *pf_H = nullptr, *pf_l0 = nullptr, *pf_osc = nullptr, *pf_Fp = nullptr,
+         *pf_Hmax = nullptr, *pf_k = nullptr, *pf_vol = nullptr;
+  int32_t* pf_allow = nullptr;
+  double* pf_node = nullptr;  // 8 x grid capacity
+  size_t pf_node_cap = 0;
+  double* pf_partial = nullptr;
+  size_t pf_partial_cap = 0;
   uint32_t *p_page = nullptr, *p_re = nullptr, *order = nullptr;
   uint64_t* keys = nullptr;
   uint32_t *group_page = nullptr, *group_count = nullptr, *group_start = nullptr;
@@ -870,6 +883,9 @@ void free_particles(zr_engine* e)
   dfree(e->touch_page); dfree(e->touch_slot); dfree(e->slot_page); dfree(e->group_nslot);
   dfree(e->mat); dfree(e->pparam); dfree(e->pstate); dfree(e->gdeg); dfree(e->broken);
   e->has_models = e->has_plastic = false;
+  dfree(e->pf_c); dfree(e->pf_H); dfree(e->pf_l0); dfree(e->pf_osc); dfree(e->pf_Fp); dfree(e->pf_Hmax);
+  dfree(e->pf_k); dfree(e->pf_vol); dfree(e->pf_allow);
+  e->pf_on = false;
   e->n = 0;
 }
 
@@ -1045,6 +1061,89 @@ int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
   return (flags & 1u) ? ZR_CFL_RESTART : ZR_OK;
 }
 
+// device scalars and flags of the Krylov solvers, with pinned host mirrors
+int ensure_scalars(zr_engine* e)
+{
+  if (!e->sc) {
+    ZR_HIP(e, hipMalloc(&e->sc, SC_N * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->ci, CI_N * sizeof(int32_t)));
+    ZR_HIP(e, hipHostMalloc(&e->hsc, SC_N * sizeof(double)));
+    ZR_HIP(e, hipHostMalloc(&e->hci, CI_N * sizeof(int32_t)));
+  }
+  return ZR_OK;
+}
+
+// ----------------------------------------------------------- phase field
+PfParticles pf_view(zr_engine* e)
+{
+  return PfParticles{ e->pf_c, e->pf_H, e->pf_l0, e->pf_osc, e->pf_Fp, e->pf_Hmax, e->pf_k, e->pf_allow, e->pf_vol };
+}
+
+int do_pf(zr_engine* e, double dt)
+{
+  e->pf_stats = zr_phase_field_stats{};
+  if (!e->pf_on || e->n == 0) return ZR_OK;
+  const zr_phase_field_opts& o = e->pf_opts;
+  hipStream_t s = e->stream;
+  if (o.naive) {
+    k_pf_naive<<<blocks_for((uint64_t)e->n, 256), 256, 0, s>>>(pview(e), pf_view(e), o.sigma_F, e->gdeg);
+    ZR_LAUNCH_CHECK(e);
+    return ZR_OK;
+  }
+  if (!e->rasterised || !e->mls_rasterised)
+    return fail(e, ZR_ERR_STATE, "phase-field solve before the MLS particlesToGrid of the step");
+  int rc0 = ensure_scalars(e);
+  if (rc0) return rc0;
+  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
+  if (e->pf_node_cap < e->grid_cap) {
+    dfree(e->pf_node);
+    ZR_HIP(e, hipMalloc(&e->pf_node, 8 * e->grid_cap * sizeof(double)));
+    e->pf_node_cap = e->grid_cap;
+  }
+  if (e->pf_partial_cap < 2 * (size_t)nb) {
+    dfree(e->pf_partial);
+    ZR_HIP(e, hipMalloc(&e->pf_partial, 2 * (size_t)nb * sizeof(double)));
+    e->pf_partial_cap = 2 * (size_t)nb;
+  }
+  const size_t C = e->pf_node_cap;
+  double *phase = e->pf_node, *rhs = phase + C, *diag = rhs + C, *x = diag + C, *r = x + C, *mr = r + C,
+         *sv = mr + C, *q = sv + C;
+  const double dx = e->cfg.dx;
+  PfScalars ps{ o.parabolic_M, dt, o.cg_tolerance, o.cg_max_iterations };
+  PfParticles pf = pf_view(e);
+  k_pf_p2g<<<e->A, 256, 0, s>>>(pview(e), pf, ps, e->order, e->group_start, e->group_page, e->stage, 1.0 / dx);
+  k_pf_nodes<<<nb, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, ps, phase, rhs, diag, x, r, mr, e->pf_partial,
+                                nb);
+  k_pf_reduce<<<1, 256, 0, s>>>(e->pf_partial, nb, e->sc, e->ci, PFR_INIT, o.cg_relative, ps);
+  ZR_LAUNCH_CHECK(e);
+  // the fracture CG runs iterations 0..max (it checks it == max after the update)
+  const int total = o.cg_max_iterations + 1;
+  const int check = 8;
+  for (int it = 0; it < total; ++it) {
+    k_pf_apply<<<e->A, 256, 0, s>>>(e->x, e->n, pf, ps, e->order, e->group_start, e->group_page, e->group_nslot,
+                                    e->gm, mr, sv, e->sc, e->ci, e->stage, 1.0 / dx);
+    k_pf_ap<<<nb, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, diag, mr, sv, q, e->sc, e->ci, e->pf_partial);
+    k_pf_reduce<<<1, 256, 0, s>>>(e->pf_partial, nb, e->sc, e->ci, PFR_ALPHA, o.cg_relative, ps);
+    k_pf_update<<<nb, 256, 0, s>>>(nn, e->gm, diag, sv, q, x, r, mr, e->sc, e->ci, e->pf_partial, nb);
+    k_pf_reduce<<<1, 256, 0, s>>>(e->pf_partial, nb, e->sc, e->ci, PFR_CHECK, o.cg_relative, ps);
+    ZR_LAUNCH_CHECK(e);
+    if ((it + 1) % check == 0 || it + 1 == total) {
+      ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      ZR_HIP(e, hipStreamSynchronize(s));
+      if (e->hci[CI_DONE]) break;
+    }
+  }
+  k_pf_g2p<<<e->A, 256, 0, s>>>(pview(e), pf, ps, e->order, e->group_start, e->group_page, e->group_nslot, e->gm, x,
+                                phase, e->gdeg, 1.0 / dx);
+  ZR_LAUNCH_CHECK(e);
+  ZR_HIP(e, hipMemcpyAsync(e->hsc, e->sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, s));
+  ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  ZR_HIP(e, hipStreamSynchronize(s));
+  e->pf_stats.cg_iterations = e->hci[CI_ITERS];
+  e->pf_stats.residual_norm = e->hsc[PF_NORM];
+  return ZR_OK;
+}
+
 // ---------------------------------------------------------------- implicit
 int ensure_implicit(zr_engine* e, int32_t prec)
 {
@@ -1070,12 +1169,8 @@ int ensure_implicit(zr_engine* e, int32_t prec)
     ZR_HIP(e, hipMalloc(&e->partial, nb * sizeof(double)));
     e->partial_cap = nb;
   }
-  if (!e->sc) {
-    ZR_HIP(e, hipMalloc(&e->sc, SC_N * sizeof(double)));
-    ZR_HIP(e, hipMalloc(&e->ci, CI_N * sizeof(int32_t)));
-    ZR_HIP(e, hipHostMalloc(&e->hsc, SC_N * sizeof(double)));
-    ZR_HIP(e, hipHostMalloc(&e->hci, CI_N * sizeof(int32_t)));
-  }
+  int rc = ensure_scalars(e);
+  if (rc) return rc;
   const size_t n = (size_t)(e->n > 0 ? e->n : 1);
   if (e->ccell_n != e->n) {
     dfree(e->ccell);
@@ -1334,6 +1429,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
+  dfree(e->pf_node); dfree(e->pf_partial);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);
   if (e->hci) (void)hipHostFree(e->hci);
@@ -1536,6 +1632,82 @@ int zr_get_materials(zr_engine* e, zr_materials* mt)
   return ZR_OK;
 }
 
+int zr_set_phase_field(zr_engine* e, const zr_phase_field* pf, const zr_phase_field_opts* o)
+{
+  if (!e || !pf || !o) return fail(e, ZR_ERR_ARG, "bad phase-field view");
+  if (pf->n != e->n) return fail(e, ZR_ERR_ARG, "phase-field count differs from the particle count");
+  if (!o->naive && (o->cg_max_iterations < 0 || !(o->cg_tolerance >= 0.0)))
+    return fail(e, ZR_ERR_ARG, "bad phase-field CG options");
+  if (!o->naive && (!pf->l0 || !pf->one_over_sigma_c))
+    return fail(e, ZR_ERR_ARG, "the phase field needs l0 and one_over_sigma_c");
+  const size_t N = (size_t)e->n;
+  if (N == 0) return ZR_OK;
+  // every particle becomes NeoHookeanBorden (keeping its plasticity code)
+  std::vector<int32_t> code(N, ZR_MODEL_NEO_HOOKEAN_BORDEN);
+  if (e->mat) {
+    ZR_HIP(e, hipMemcpy(code.data(), e->mat, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (auto& c : code) c = (c & ~15) | ZR_MODEL_NEO_HOOKEAN_BORDEN;
+  } else {
+    zr_materials m{};
+    m.n = e->n;
+    int rc = zr_set_materials(e, &m);
+    if (rc) return rc;
+  }
+  ZR_HIP(e, hipMemcpy(e->mat, code.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+  e->has_models = true;
+  double** dst[] = { &e->pf_c, &e->pf_H, &e->pf_l0, &e->pf_osc, &e->pf_Fp, &e->pf_Hmax, &e->pf_k, &e->pf_vol };
+  const double* src[] = { pf->c, pf->H, pf->l0, pf->one_over_sigma_c, pf->pf_Fp, pf->H_max, pf->residual_phase,
+                          nullptr };
+  const double dflt[] = { 1.0, 0.0, 0.0, 0.0, 1.0, HUGE_VAL, 0.001, 0.0 };
+  std::vector<double> tmp(N);
+  for (int a = 0; a < 8; ++a) {
+    if (!*dst[a]) ZR_HIP(e, hipMalloc(dst[a], N * sizeof(double)));
+    for (size_t i = 0; i < N; ++i) tmp[i] = src[a] ? src[a][i] : dflt[a];
+    ZR_HIP(e, hipMemcpy(*dst[a], tmp.data(), N * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (!e->pf_allow) ZR_HIP(e, hipMalloc(&e->pf_allow, N * sizeof(int32_t)));
+  std::vector<int32_t> al(N, 1);
+  if (pf->allow_damage)
+    for (size_t i = 0; i < N; ++i) al[i] = pf->allow_damage[i] ? 1 : 0;
+  ZR_HIP(e, hipMemcpy(e->pf_allow, al.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+  e->pf_opts = *o;
+  e->pf_on = true;
+  return ZR_OK;
+}
+
+int zr_get_phase_field(zr_engine* e, zr_phase_field* pf)
+{
+  if (!e || !pf) return fail(e, ZR_ERR_ARG, "bad phase-field view");
+  if (pf->n != e->n) return fail(e, ZR_ERR_ARG, "phase-field count differs from the particle count");
+  if (!e->pf_on) return fail(e, ZR_ERR_STATE, "no phase field was set");
+  const size_t N = (size_t)e->n;
+  if (N == 0) return ZR_OK;
+  if (pf->c) ZR_HIP(e, hipMemcpyAsync(pf->c, e->pf_c, N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  if (pf->H) ZR_HIP(e, hipMemcpyAsync(pf->H, e->pf_H, N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  if (pf->pf_Fp) ZR_HIP(e, hipMemcpyAsync(pf->pf_Fp, e->pf_Fp, N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+int zr_phase_field_solve(zr_engine* e, double dt, zr_phase_field_stats* st)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!(dt > 0.0)) return fail(e, ZR_ERR_ARG, "dt must be positive");
+  if (!e->pf_on) return fail(e, ZR_ERR_STATE, "no phase field was set");
+  int rc = do_pf(e, dt);
+  if (rc) return rc;
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  if (st) *st = e->pf_stats;
+  return ZR_OK;
+}
+
+int zr_phase_field_last_stats(zr_engine* e, zr_phase_field_stats* st)
+{
+  if (!e || !st) return ZR_ERR_ARG;
+  *st = e->pf_stats;
+  return ZR_OK;
+}
+
 int zr_sort_particles_and_pollute_grid(zr_engine* e, zr_step_stats* st)
 {
   if (!e) return ZR_ERR_ARG;
@@ -1578,7 +1750,15 @@ int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st)
   if (rc) return rc;
   rc = do_p2g(e, true);
   if (rc) return rc;
-  rc = do_grid(e, GRID_COMBINE_UPDATE, dt);
+  if (e->pf_on) {  // FractureSimulation::particlesToGrid: the damage solve after P2G
+    rc = do_grid(e, GRID_COMBINE, dt);
+    if (rc) return rc;
+    rc = do_pf(e, dt);
+    if (rc) return rc;
+    rc = do_grid(e, GRID_UPDATE, dt);
+  } else {
+    rc = do_grid(e, GRID_COMBINE_UPDATE, dt);
+  }
   if (rc) return rc;
   rc = do_g2p(e, dt, st, true);
   if (rc < 0) return rc;

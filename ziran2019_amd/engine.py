@@ -43,7 +43,8 @@ EXPORTS = (
     "zr_advance_one_time_step", "zr_debug_size", "zr_debug_dump", "zr_profile_enable", "zr_profile_read",
     "zr_particles_to_grid_apic", "zr_backward_euler_step", "zr_grid_to_particles_apic",
     "zr_advance_one_time_step_implicit", "zr_implicit_residual", "zr_implicit_multiply",
-    "zr_set_materials", "zr_get_materials",
+    "zr_set_materials", "zr_get_materials", "zr_set_phase_field", "zr_get_phase_field", "zr_phase_field_solve",
+    "zr_phase_field_last_stats",
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
 ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER = 0, 1, 2
@@ -117,6 +118,38 @@ def drucker_prager_alpha(friction_angle_deg):
     return np.sqrt(2.0 / 3.0) * 2.0 * s / (3.0 - s)
 
 
+class ZrPhaseField(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("c", _dp), ("H", _dp), ("l0", _dp), ("one_over_sigma_c", _dp),
+                ("pf_Fp", _dp), ("H_max", _dp), ("residual_phase", _dp), ("allow_damage", _ip)]
+
+
+class ZrPhaseFieldOpts(ctypes.Structure):
+    _fields_ = [("parabolic_M", ctypes.c_double), ("cg_max_iterations", ctypes.c_int32),
+                ("naive", ctypes.c_int32), ("cg_tolerance", ctypes.c_double), ("cg_relative", ctypes.c_double),
+                ("sigma_F", ctypes.c_double)]
+
+
+class ZrPhaseFieldStats(ctypes.Structure):
+    _fields_ = [("cg_iterations", ctypes.c_int32), ("reserved", ctypes.c_int32), ("residual_norm", ctypes.c_double)]
+
+
+def phase_field_opts(parabolic_M=0.0, cg_max_iterations=100, naive=False, cg_tolerance=1e-6, cg_relative=1e-12,
+                     sigma_F=0.0):
+    """FractureSimulation defaults (FractureSimulation.h:50-73,439;
+    fracture/ConjugateGradient.h:64)."""
+    return ZrPhaseFieldOpts(float(parabolic_M), int(cg_max_iterations), int(bool(naive)), float(cg_tolerance),
+                            float(cg_relative), float(sigma_F))
+
+
+def nhb_one_over_sigma_c(percentage, mu, lam):
+    """PhaseField::Get_Sigma_C_Based_On_Max_Deformation (PhaseField.h:59-68):
+    1 / psi(F = (1 + percentage) I) of NeoHookeanBorden with g = 1, i.e. the
+    volumetric energy kappa/2 ((J^2 - 1)/2 - log J), kappa = 0.666666 mu + lambda."""
+    J = (1.0 + percentage) ** 3
+    kappa = 0.666666 * mu + lam
+    return 1.0 / (kappa / 2.0 * ((J * J - 1.0) / 2.0 - np.log(J)))
+
+
 class ZrStepStats(ctypes.Structure):
     _fields_ = [("num_groups", ctypes.c_int64), ("num_pages", ctypes.c_int64), ("cfl_flags", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -163,6 +196,10 @@ def load_library(path=LIB_PATH):
     L.zr_implicit_residual.argtypes = [vp, vp, vp]
     L.zr_implicit_multiply.argtypes = [vp, vp, vp]
     L.zr_set_materials.argtypes = [vp, ctypes.POINTER(ZrMaterials)]
+    L.zr_set_phase_field.argtypes = [vp, ctypes.POINTER(ZrPhaseField), ctypes.POINTER(ZrPhaseFieldOpts)]
+    L.zr_get_phase_field.argtypes = [vp, ctypes.POINTER(ZrPhaseField)]
+    L.zr_phase_field_solve.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrPhaseFieldStats)]
+    L.zr_phase_field_last_stats.argtypes = [vp, ctypes.POINTER(ZrPhaseFieldStats)]
     L.zr_get_materials.argtypes = [vp, ctypes.POINTER(ZrMaterials)]
     L.zr_profile_enable.argtypes = [vp, ctypes.c_int32]
     L.zr_profile_read.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
@@ -270,6 +307,35 @@ class MpmEngine:
         mt = ZrMaterials(n, m.ctypes.data_as(_ip) if m is not None else None,
                          p.ctypes.data_as(_ip) if p is not None else None, _dptr(q), _dptr(st), _dptr(g), None)
         self._check(self.L.zr_set_materials(self.h, ctypes.byref(mt)), "set_materials")
+
+    def set_phase_field(self, opts, c=None, H=None, l0=None, one_over_sigma_c=None, pf_Fp=None, H_max=None,
+                        residual_phase=None, allow_damage=None):
+        n = self.n
+
+        def f(a):
+            return None if a is None else np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,)))
+        arrs = [f(c), f(H), f(l0), f(one_over_sigma_c), f(pf_Fp), f(H_max), f(residual_phase)]
+        al = None if allow_damage is None else np.ascontiguousarray(np.broadcast_to(allow_damage, (n,)),
+                                                                     dtype=np.int32)
+        pf = ZrPhaseField(n, *[_dptr(a) for a in arrs], al.ctypes.data_as(_ip) if al is not None else None)
+        self._check(self.L.zr_set_phase_field(self.h, ctypes.byref(pf), ctypes.byref(opts)), "set_phase_field")
+
+    def get_phase_field(self):
+        n = self.n
+        c, H, fp = np.zeros(n), np.zeros(n), np.zeros(n)
+        pf = ZrPhaseField(n, _dptr(c), _dptr(H), None, None, _dptr(fp), None, None, None)
+        self._check(self.L.zr_get_phase_field(self.h, ctypes.byref(pf)), "get_phase_field")
+        return dict(c=c, H=H, pf_Fp=fp)
+
+    def phase_field_solve(self, dt):
+        st = ZrPhaseFieldStats()
+        self._check(self.L.zr_phase_field_solve(self.h, float(dt), ctypes.byref(st)), "phase_field_solve")
+        return st
+
+    def phase_field_last_stats(self):
+        st = ZrPhaseFieldStats()
+        self._check(self.L.zr_phase_field_last_stats(self.h, ctypes.byref(st)), "phase_field_last_stats")
+        return st
 
     def get_materials(self):
         n = self.n

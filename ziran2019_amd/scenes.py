@@ -195,3 +195,48 @@ def boxes_scene(dx=1.0 / 256, nboxes=64, side_lo=0.025, side_hi=0.075, ppc=8, lo
     out.update(dx=dx, gravity=(0.0, -9.8, 0.0), boxes=boxes,
                colliders=[dict(shape=0, type=1, origin=(0.0, 0.1, 0.0), normal=(0.0, 1.0, 0.0))])
     return out
+
+
+def fracture_scene(dx=1.0 / 320, cells=(100, 75, 50), ppc=8, E=500.0, nu=0.4, rho=1000.0, nsites=200,
+                   toughness_sigma=0.3, percentage=0.05, sphere_radius=0.05, sphere_speed=5.0, seed=20190704):
+    """CD-MPM fracture workload: a NeoHookeanBorden block of ``cells`` cells
+    (100 x 75 x 50 = 375,000 cells, 3,000,000 particles at 8 ppc and
+    dx = 1/320) resting on a sticky ground, struck from above by a sticky
+    sphere moving down at ``sphere_speed``.  The block is split into
+    ``nsites`` seeded Voronoi cells; each cell scales the fracture toughness
+    by a lognormal(0, ``toughness_sigma``) factor (one_over_sigma_c divided by
+    it).  Base toughness: PhaseField::Get_Sigma_C_Based_On_Max_Deformation at
+    ``percentage`` (PhaseField.h:59-68); l0 = dx/2, residual phase 1e-3
+    (MpmParticleHandleBase.cpp:105-114)."""
+    cells = np.asarray(cells)
+    ground = 0.1
+    lo = np.array([int(round(0.5 / dx)) - cells[0] // 2, int(np.ceil(ground / dx)), int(round(0.5 / dx)) - cells[2] // 2])
+    r = [np.arange(lo[d], lo[d] + cells[d]) for d in range(3)]
+    cell_idx = np.stack(np.meshgrid(*r, indexing="ij"), axis=-1).reshape(-1, 3)
+    X = _particles_in_cells(cell_idx, ppc, dx, seed, 0)
+    mu, lam = lame(E, nu)
+    perm = morton_order(X, dx)
+    X = X[perm]
+    n = len(X)
+    out = _finish(X, dx, rho, ppc, mu, lam, order=False)
+    # Voronoi sites and their toughness factors
+    box_lo, box_hi = lo * dx, (lo + cells) * dx
+    us = uniform01(np.arange(3 * nsites, dtype=np.uint64) + np.uint64(1 << 44), seed).reshape(nsites, 3)
+    sites = box_lo + us * (box_hi - box_lo)
+    factor = np.exp(toughness_sigma * _normal01(np.arange(nsites, dtype=np.uint64) + np.uint64(1 << 45), seed))
+    owner = np.empty(n, dtype=np.int64)
+    chunk = 1 << 16
+    for s0 in range(0, n, chunk):
+        d2 = ((X[s0:s0 + chunk, None, :] - sites[None, :, :]) ** 2).sum(axis=2)
+        owner[s0:s0 + chunk] = d2.argmin(axis=1)
+    J = (1.0 + percentage) ** 3
+    kappa = 0.666666 * mu + lam
+    base = 1.0 / (kappa / 2.0 * ((J * J - 1.0) / 2.0 - np.log(J)))
+    top = box_hi[1]
+    centre = (0.5 * (box_lo[0] + box_hi[0]), top + sphere_radius + 2 * dx, 0.5 * (box_lo[2] + box_hi[2]))
+    out.update(dx=dx, gravity=(0.0, -9.8, 0.0), owner=owner, sites=sites,
+               one_over_sigma_c=base / factor[owner], l0=0.5 * dx, residual_phase=1e-3,
+               colliders=[dict(shape=0, type=1, origin=(0.0, ground, 0.0), normal=(0.0, 1.0, 0.0)),
+                          dict(shape=1, type=1, origin=centre, radius=sphere_radius,
+                               velocity=(0.0, -sphere_speed, 0.0))])
+    return out
