@@ -166,6 +166,9 @@ def lib():
         L.zro_phase_field_solve.argtypes = [_p, _d, ctypes.POINTER(ZroPfStats)]
         L.zro_phase_field_system.argtypes = [_p, _p, _p, _p]
         L.zro_phase_field_multiply.argtypes = [_p, _p, _p]
+        L.zro_fcr_kirchhoff_adjoint.argtypes = [_d, _d, _p, _p, _p, _p, _p]
+        L.zro_adjoint_step.restype = _i
+        L.zro_adjoint_step.argtypes = [_p, _d] + [_p] * 6
         L.zro_sym3_max_eigenvalue.restype = _d
         L.zro_sym3_max_eigenvalue.argtypes = [_p]
         L.zro_get_materials.argtypes = [_p, _p, _p, _p]
@@ -288,6 +291,16 @@ def drucker_prager_project(mu, lam, alpha, beta, cohesion, volume_correction, lo
     return _from_cm(f), float(j[0])
 
 
+def fcr_kirchhoff_adjoint(mu, lam, F, taubar, Fbar=None):
+    """(Fbar + (dtau/dF)^T taubar, mubar, lambar) of the fixed-corotated tau."""
+    fb = _cm(np.zeros((3, 3)) if Fbar is None else Fbar)
+    mb = np.zeros(1)
+    lb = np.zeros(1)
+    lib().zro_fcr_kirchhoff_adjoint(float(mu), float(lam), _ptr(_cm(F)), _ptr(_cm(taubar)), _ptr(fb), _ptr(mb),
+                                    _ptr(lb))
+    return _from_cm(fb), float(mb[0]), float(lb[0])
+
+
 def sym3_max_eigenvalue(A):
     return float(lib().zro_sym3_max_eigenvalue(_ptr(_cm(A))))
 
@@ -383,6 +396,17 @@ class OracleSim:
         b = np.zeros_like(x)
         lib().zro_phase_field_multiply(self.h, _ptr(x), _ptr(b))
         return b
+
+    def adjoint_step(self, dt, aX, aV, aC, aF, amu, alam):
+        """Reverse of one explicit step from the sim's current (input) state;
+        aX (n,3), aV (n,3), aC (n,9), aF (n,9) are updated in place from the
+        output adjoint to the input adjoint; amu, alam (n,) accumulate."""
+        for a in (aX, aV, aC, aF, amu, alam):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        rc = lib().zro_adjoint_step(self.h, float(dt), _ptr(aX), _ptr(aV), _ptr(aC), _ptr(aF), _ptr(amu),
+                                    _ptr(alam))
+        if rc != 0:
+            raise RuntimeError("adjoint: a collider branch without a derivative was hit")
 
     def get_materials(self):
         n = self.n

@@ -43,6 +43,9 @@ void zro_fcr_kirchhoff(double mu, double lam, const double* F, double* tau);
 void zro_fcr_first_piola_differential(double mu, double lam, const double* F,
                                       const double* dF, double* dP);
 void zro_lame(double E, double nu, double* mu, double* lam);
+/* reverse of tau(F; mu, lam): Fbar += (dtau/dF)^T taubar; returns mubar, lambar */
+void zro_fcr_kirchhoff_adjoint(double mu, double lam, const double* F, const double* taubar, double* Fbar,
+                               double* mubar, double* lambar);
 
 /* ---- StvkWithHencky (StvkWithHencky.cpp:16-143) ---- */
 double zro_stvk_hencky_psi(double mu, double lam, const double* F);
@@ -178,6 +181,13 @@ void zro_phase_field_solve(zro_sim* s, double dt, zro_pf_stats* st);
 /* pieces for tests: per node slot (npages*32) the system of the last solve */
 void zro_phase_field_system(const zro_sim* s, double* rhs, double* diag, double* x);
 void zro_phase_field_multiply(zro_sim* s, const double* x, double* b);
+/* ---- adjoint of the explicit MLS step (no reference counterpart) ---- */
+/* The sim holds the step's input state; a* hold the adjoint of the step's
+   output (particle-major) on entry and of its input on return; amu / alam
+   accumulate.  Returns -1 if a collider branch has no derivative (friction). */
+int zro_adjoint_step(zro_sim* s, double dt, double* aX, double* aV, double* aC, double* aF, double* amu,
+                     double* alam);
+
 /* largest eigenvalue of a symmetric 3x3 matrix (column-major) */
 double zro_sym3_max_eigenvalue(const double* A);
 /* matrix-free operator of the last backward-Euler solve at the final Newton

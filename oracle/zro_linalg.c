@@ -666,3 +666,70 @@ int zro_drucker_prager_project(double mu, double lam, double alpha, double beta,
   rebuild(u, e, v, F);
   return 1;
 }
+
+/* ---- adjoint of the fixed-corotated Kirchhoff stress (no reference counterpart) ---- */
+/* reverse of tau = 2 mu (F - R) F^T + lam (J - 1) J I at F (column-major
+   9-vectors): adds to Fbar, returns mubar / lambar */
+void zro_fcr_kirchhoff_adjoint(double mu, double lam, const double* Fcm, const double* taubar_cm, double* Fbar_cm,
+                               double* mubar, double* lambar)
+{
+  M3 f, u, v, tb, r, fb, A, Ah, Mb, c;
+  double sig[3];
+  from_cm(Fcm, f);
+  from_cm(taubar_cm, tb);
+  from_cm(Fbar_cm, fb);
+  svd3_rm(f, u, sig, v);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r[i][j] = u[i][0] * v[j][0] + u[i][1] * v[j][1] + u[i][2] * v[j][2];
+  double J = f[0][0] * (f[1][1] * f[2][2] - f[2][1] * f[1][2]) - f[0][1] * (f[1][0] * f[2][2] - f[2][0] * f[1][2]) +
+             f[0][2] * (f[1][0] * f[2][1] - f[2][0] * f[1][1]);
+  c[0][0] = f[1][1] * f[2][2] - f[1][2] * f[2][1];
+  c[0][1] = f[1][2] * f[2][0] - f[1][0] * f[2][2];
+  c[0][2] = f[1][0] * f[2][1] - f[1][1] * f[2][0];
+  c[1][0] = f[0][2] * f[2][1] - f[0][1] * f[2][2];
+  c[1][1] = f[0][0] * f[2][2] - f[0][2] * f[2][0];
+  c[1][2] = f[0][1] * f[2][0] - f[0][0] * f[2][1];
+  c[2][0] = f[0][1] * f[1][2] - f[0][2] * f[1][1];
+  c[2][1] = f[0][2] * f[1][0] - f[0][0] * f[1][2];
+  c[2][2] = f[0][0] * f[1][1] - f[0][1] * f[1][0];
+  double trtb = tb[0][0] + tb[1][1] + tb[2][2];
+  /* A = taubar F  (for the dR term) */
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A[i][j] = tb[i][0] * f[0][j] + tb[i][1] * f[1][j] + tb[i][2] * f[2][j];
+  /* Ah = U^T A V */
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) acc += u[k][i] * A[k][l] * v[l][j];
+      Ah[i][j] = acc;
+    }
+  for (int i = 0; i < 3; ++i) Mb[i][i] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = i + 1; j < 3; ++j) {
+      double den = sig[i] + sig[j];
+      if (fabs(den) < 1e-12) den = (den < 0 ? -1e-12 : 1e-12);
+      double x = (Ah[i][j] - Ah[j][i]) / den;
+      Mb[i][j] = x;
+      Mb[j][i] = -x;
+    }
+  double mb = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      /* 2 mu (tb + tb^T) F */
+      double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        t1 += (tb[i][k] + tb[k][i]) * f[k][j];
+        t2 += tb[k][i] * r[k][j];             /* tb^T R */
+        t3 += u[i][k] * (Mb[k][0] * v[j][0] + Mb[k][1] * v[j][1] + Mb[k][2] * v[j][2]);  /* U Mb V^T */
+      }
+      fb[i][j] += 2.0 * mu * (t1 - t2 - t3) + lam * (2.0 * J - 1.0) * trtb * c[i][j];
+      double fr = 0.0;
+      for (int k = 0; k < 3; ++k) fr += (f[i][k] - r[i][k]) * f[j][k];
+      mb += tb[i][j] * 2.0 * fr;
+    }
+  to_cm(fb, Fbar_cm);
+  *mubar = mb;
+  *lambar = trtb * (J - 1.0) * J;
+}
+

@@ -1555,3 +1555,259 @@ void zro_phase_field_system(const zro_sim* s, double* rhs, double* diag, double*
 }
 
 void zro_phase_field_multiply(zro_sim* s, const double* x, double* b) { pf_multiply(s, x, b); }
+
+/* ====================================================================== */
+/* Adjoint of the explicit MLS step (no counterpart in the reference): the  */
+/* reverse of stress -> P2G -> grid update -> G2P + F update for fixed-     */
+/* corotated particles, with the grid topology of the step held fixed.      */
+/* ====================================================================== */
+
+typedef double M3[3][3];
+
+/* Jacobian d v_after / d v_before of the collider handling of one node
+   (multi_object_collision, then trial_collision when nothing collided).
+   Returns 0 when a branch has no supported derivative (friction). */
+static int collide_jacobian(const zro_sim* s, double dt, const double* x, const double* vpre, double J[3][3])
+{
+  double v[3] = { vpre[0], vpre[1], vpre[2] };
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) J[i][j] = i == j ? 1.0 : 0.0;
+  int any = 0;
+  for (int k = 0; k < s->ncol; ++k) {
+    const zro_collider* c = &s->col[k];
+    double b[3], X[3], phi, N[3];
+    collider_b(s, c, s->time, b);
+    for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
+    if (!ls_query_inside(c, X, &phi, N)) continue;
+    any = 1;
+    if (c->type == 1) {
+      memset(J, 0, sizeof(double) * 9);
+      return 1;
+    }
+    double rel[3], dot = 0.0;
+    for (int d = 0; d < 3; ++d) rel[d] = v[d] - c->velocity[d];
+    for (int d = 0; d < 3; ++d) dot += rel[d] * N[d];
+    if (c->type == 2 || dot < 0) {
+      if (c->friction != 0.0 && dot < 0) return 0;
+      double P[3][3], T[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P[i][j] = (i == j ? 1.0 : 0.0) - N[i] * N[j];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T[i][j] = P[i][0] * J[0][j] + P[i][1] * J[1][j] + P[i][2] * J[2][j];
+      memcpy(J, T, sizeof(T));
+      for (int d = 0; d < 3; ++d) rel[d] -= N[d] * dot;
+    }
+    for (int d = 0; d < 3; ++d) v[d] = rel[d] + c->velocity[d];
+  }
+  if (any) return 1;
+  for (int k = 0; k < s->ncol; ++k) { /* trial collision at t + dt */
+    const zro_collider* c = &s->col[k];
+    double b[3], X[3], phi, n[3];
+    collider_b(s, c, s->time + dt, b);
+    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - b[d];
+    if (!ls_query_inside(c, X, &phi, n)) continue;
+    double A[3][3], T[3][3];
+    if (c->shape == 0) {
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - n[i] * n[j];
+    } else {
+      double dist = phi + c->radius;
+      if (dist < 1e-7) return 0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double pr = (i == j ? 1.0 : 0.0) - n[i] * n[j];
+          A[i][j] = (i == j ? 1.0 : 0.0) - n[i] * n[j] - phi / dist * pr;
+        }
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) T[i][j] = A[i][0] * J[0][j] + A[i][1] * J[1][j] + A[i][2] * J[2][j];
+    memcpy(J, T, sizeof(T));
+    for (int d = 0; d < 3; ++d) v[d] -= (phi / dt) * n[d];
+  }
+  return 1;
+}
+
+/* One reverse step.  The sim holds the step's INPUT particle state; the
+   forward sort, stress, P2G and grid update are recomputed here.  On entry
+   aX, aV, aC, aF hold the adjoint of the step's output state (particle-major,
+   matrices column-major); on return the adjoint of its input.  amu, alam
+   accumulate.  Returns 0, or -1 when a collider branch has no derivative. */
+int zro_adjoint_step(zro_sim* s, double dt, double* aX, double* aV, double* aC, double* aF, double* amu,
+                     double* alam)
+{
+  if (!s->n) return 0;
+  zro_sort_and_pollute(s);
+  zro_update_stress(s);
+  zro_p2g_mls(s);
+  /* forward grid quantities before the update: v = p/m, f */
+  int64_t ns = s->npages * 32;
+  double* gv = (double*)calloc((size_t)(3 * ns), sizeof(double));
+  double* gf = (double*)calloc((size_t)(3 * ns), sizeof(double));
+  for (int64_t k = 0; k < ns; ++k) {
+    if (s->nodes[k].idx < 0) continue;
+    for (int d = 0; d < 3; ++d) { gv[3 * k + d] = s->nodes[k].v[d]; gf[3 * k + d] = s->nodes[k].new_v[d]; }
+  }
+  zro_grid_update(s, dt);
+  const double Dinv = s->D_inverse;
+  double* avs = (double*)calloc((size_t)(3 * ns), sizeof(double));  /* adjoint of v* */
+  double* ag = (double*)calloc((size_t)(7 * ns), sizeof(double));   /* adjoint of m, p, f */
+  double* xin = (double*)calloc((size_t)(3 * s->n), sizeof(double)); /* adjoint of x (input) */
+  double* Fin = (double*)calloc((size_t)(9 * s->n), sizeof(double));
+  /* ---- G2P + F update reverse ---- */
+  for (int64_t i = 0; i < s->n; ++i) {
+    const double* Xp = s->X + 3 * i;
+    zro_spline sp;
+    spline_of(s, Xp, &sp);
+    /* forward recompute of v', C' */
+    double pv[3] = { 0, 0, 0 }, B[3][3] = { { 0 } };
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          double w = spline_node(s, &sp, a, b, c, gw);
+          int nc[3] = { sp.base[0] + a, sp.base[1] + b, sp.base[2] + c };
+          zro_node* g = node_at(s, nc[0], nc[1], nc[2], NULL);
+          double d3[3];
+          for (int d = 0; d < 3; ++d) d3[d] = (double)nc[d] * s->dx - Xp[d];
+          for (int r = 0; r < 3; ++r) {
+            pv[r] += w * g->new_v[r];
+            for (int cc = 0; cc < 3; ++cc) B[r][cc] += w * g->new_v[r] * d3[cc];
+          }
+        }
+    M3 Cn, Fo, Fb, Cb, Bb;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) Cn[r][c] = B[r][c] * Dinv;
+    mat_from_cm(s->F + 9 * i, Fo);
+    mat_from_cm(aF + 9 * i, Fb);
+    mat_from_cm(aC + 9 * i, Cb);
+    /* F' = F + dt C' F */
+    M3 Fin_m;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double acc = Fb[r][c];
+        for (int k = 0; k < 3; ++k) acc += dt * Cn[k][r] * Fb[k][c];
+        Fin_m[r][c] = acc;
+        double cb = 0.0;
+        for (int k = 0; k < 3; ++k) cb += Fb[r][k] * Fo[c][k];
+        Cb[r][c] += dt * cb;
+      }
+    mat_to_cm(Fin_m, Fin + 9 * i);
+    double vb[3];
+    for (int d = 0; d < 3; ++d) {
+      xin[3 * i + d] = aX[3 * i + d];
+      vb[d] = aV[3 * i + d] + dt * aX[3 * i + d];
+    }
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) Bb[r][c] = Dinv * Cb[r][c];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          double w = spline_node(s, &sp, a, b, c, gw);
+          int nc[3] = { sp.base[0] + a, sp.base[1] + b, sp.base[2] + c };
+          int64_t slot;
+          zro_node* g = node_at(s, nc[0], nc[1], nc[2], &slot);
+          double d3[3];
+          for (int d = 0; d < 3; ++d) d3[d] = (double)nc[d] * s->dx - Xp[d];
+          const double* vs = g->new_v;
+          double bd[3];
+          for (int r = 0; r < 3; ++r) bd[r] = Bb[r][0] * d3[0] + Bb[r][1] * d3[1] + Bb[r][2] * d3[2];
+          double sc = 0.0;
+          for (int r = 0; r < 3; ++r) {
+            avs[3 * slot + r] += w * (vb[r] + bd[r]);
+            sc += vs[r] * (vb[r] + bd[r]);
+          }
+          for (int d = 0; d < 3; ++d) {
+            double btv = Bb[0][d] * vs[0] + Bb[1][d] * vs[1] + Bb[2][d] * vs[2];
+            xin[3 * i + d] += gw[d] * sc - w * btv;
+          }
+        }
+  }
+  /* ---- grid update reverse ---- */
+  int ok = 1;
+  for (int64_t p = 0; p < s->npages; ++p) {
+    int base[3];
+    zro_linear_to_coord(s->pages[p], base);
+    for (int e = 0; e < 32; ++e) {
+      int64_t k = p * 32 + e;
+      zro_node* g = &s->nodes[k];
+      if (g->idx < 0) continue;
+      double xi[3] = { (double)(base[0] + ((e >> 4) & 1)) * s->dx, (double)(base[1] + ((e >> 2) & 3)) * s->dx,
+                       (double)(base[2] + (e & 3)) * s->dx };
+      double m = g->m, vpre[3], J[3][3], vb[3];
+      for (int d = 0; d < 3; ++d) vpre[d] = gv[3 * k + d] + s->gravity[d] * dt + gf[3 * k + d] * (dt / m);
+      if (!collide_jacobian(s, dt, xi, vpre, J)) ok = 0;
+      for (int d = 0; d < 3; ++d) vb[d] = J[0][d] * avs[3 * k] + J[1][d] * avs[3 * k + 1] + J[2][d] * avs[3 * k + 2];
+      double mb = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        ag[7 * k + 1 + d] = vb[d] / m;
+        ag[7 * k + 4 + d] = dt * vb[d] / m;
+        mb -= (gv[3 * k + d] + gf[3 * k + d] * (dt / m)) * vb[d] / m;
+      }
+      ag[7 * k] = mb;
+    }
+  }
+  /* ---- P2G + stress reverse ---- */
+  for (int64_t i = 0; i < s->n; ++i) {
+    const double* Xp = s->X + 3 * i;
+    zro_spline sp;
+    spline_of(s, Xp, &sp);
+    double mass = s->mass[i];
+    M3 Cm, vt, Cb, vtb;
+    mat_from_cm(s->C + 9 * i, Cm);
+    mat_from_cm(s->vtau + 9 * i, vt);
+    memset(Cb, 0, sizeof(Cb));
+    memset(vtb, 0, sizeof(vtb));
+    double vbar[3] = { 0, 0, 0 }, xb[3] = { 0, 0, 0 };
+    const double* vp = s->V + 3 * i;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          double w = spline_node(s, &sp, a, b, c, gw);
+          int nc[3] = { sp.base[0] + a, sp.base[1] + b, sp.base[2] + c };
+          int64_t slot;
+          node_at(s, nc[0], nc[1], nc[2], &slot);
+          if (s->nodes[slot].idx < 0) continue;
+          const double* A = ag + 7 * slot;
+          double d3[3];
+          for (int d = 0; d < 3; ++d) d3[d] = (double)nc[d] * s->dx - Xp[d];
+          double mom[3], td[3];
+          for (int r = 0; r < 3; ++r) {
+            mom[r] = mass * vp[r] + mass * (Cm[r][0] * d3[0] + Cm[r][1] * d3[1] + Cm[r][2] * d3[2]);
+            td[r] = vt[r][0] * d3[0] + vt[r][1] * d3[1] + vt[r][2] * d3[2];
+          }
+          double sp_ = A[1] * mom[0] + A[2] * mom[1] + A[3] * mom[2];
+          double sf = A[4] * td[0] + A[5] * td[1] + A[6] * td[2];
+          for (int r = 0; r < 3; ++r) {
+            vbar[r] += w * mass * A[1 + r];
+            for (int cc = 0; cc < 3; ++cc) {
+              Cb[r][cc] += w * mass * A[1 + r] * d3[cc];
+              vtb[r][cc] += -w * Dinv * A[4 + r] * d3[cc];
+            }
+          }
+          for (int d = 0; d < 3; ++d) {
+            double ctp = Cm[0][d] * A[1] + Cm[1][d] * A[2] + Cm[2][d] * A[3];
+            double vtf = vt[0][d] * A[4] + vt[1][d] * A[5] + vt[2][d] * A[6];
+            xb[d] += gw[d] * (A[0] * mass + sp_ - Dinv * sf) - w * mass * ctp + w * Dinv * vtf;
+          }
+        }
+    for (int d = 0; d < 3; ++d) {
+      aX[3 * i + d] = xin[3 * i + d] + xb[d];
+      aV[3 * i + d] = vbar[d];
+    }
+    mat_to_cm(Cb, aC + 9 * i);
+    /* stress reverse: vtau = V0 tau(F) */
+    double tbar[9], mub = 0.0, lab = 0.0;
+    M3 tbm;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) tbm[r][c] = s->vol[i] * vtb[r][c];
+    mat_to_cm(tbm, tbar);
+    zro_fcr_kirchhoff_adjoint(s->mu[i], s->lam[i], s->F + 9 * i, tbar, Fin + 9 * i, &mub, &lab);
+    memcpy(aF + 9 * i, Fin + 9 * i, sizeof(double) * 9);
+    amu[i] += mub;
+    alam[i] += lab;
+  }
+  free(gv); free(gf); free(avs); free(ag); free(xin); free(Fin);
+  return ok ? 0 : -1;
+}
