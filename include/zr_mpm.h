@@ -145,6 +145,18 @@ typedef struct zr_phase_field_stats {
   double residual_norm;
 } zr_phase_field_stats;
 
+/* Adjoints of the particle state (same layout as zr_particles) and of the
+   per-particle Lame parameters (accumulated over reverse steps). */
+typedef struct zr_particles_adjoint {
+  int64_t n;
+  double* X;       /* 3n */
+  double* V;       /* 3n */
+  double* C;       /* 9n */
+  double* F;       /* 9n */
+  double* mu;      /* n  */
+  double* lambda;  /* n  */
+} zr_particles_adjoint;
+
 typedef struct zr_step_stats {
   int64_t num_groups;   /* particle pages (particle_group.size()) */
   int64_t num_pages;    /* touched SPGrid pages (page_map->Get_Blocks().second) */
@@ -250,6 +262,22 @@ int zr_advance_one_time_step_implicit(zr_engine* e, double dt, const zr_implicit
    and the unprojected operator b = A x (BackwardEuler.h:322-351). */
 int zr_implicit_residual(zr_engine* e, const double* dv, double* r);
 int zr_implicit_multiply(zr_engine* e, const double* x, double* b);
+
+/* Differentiable form of the explicit MLS step.  No counterpart in the
+   reference; for gradient-based callers (parameter estimation, control) that
+   need d loss / d (state, mu, lambda) of a rollout.  Fixed-corotated
+   particles without plasticity or damage; colliders without friction.
+   zr_adjoint_set: the adjoint of the current state (NULL arrays: zero; mu and
+   lambda set the accumulators).  zr_adjoint_step: with the engine holding
+   the INPUT state of a step of size dt (and its time), replaces the adjoint
+   of that step's output by the adjoint of its input, accumulating mu and
+   lambda; the particle state is left unchanged.  Checkpoints keep X, V, C, F
+   and the time on the device for recompute. */
+int zr_adjoint_set(zr_engine* e, const zr_particles_adjoint* a);
+int zr_adjoint_get(zr_engine* e, zr_particles_adjoint* a);
+int zr_adjoint_step(zr_engine* e, double dt);
+int zr_checkpoint_save(zr_engine* e, int32_t slot);
+int zr_checkpoint_restore(zr_engine* e, int32_t slot);
 
 /* Introspection for parity tests (reference state after the named call):
    sizes in bytes via zr_debug_size, contents via zr_debug_dump. */

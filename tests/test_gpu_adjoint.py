@@ -1,0 +1,135 @@
+"""GPU adjoint of the explicit MLS step (the differentiable form) against
+the FP64 CPU adjoint oracle (1e-10 after one step, 1e-9 over a rollout) and
+against FP64 central differences of the GPU forward rollout (<= 1e-4, the
+required accuracy), through the C-ABI and the torch.autograd binding with
+checkpoint + recompute."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from ziran2019_amd import scenes
+from ziran2019_amd.autodiff import Rollout, lame_from_E, rollout
+from _helpers import make_pair, rel_err
+
+pytestmark = pytest.mark.gpu
+
+DT = 2e-4
+
+
+def _scene(seed=3, ncells=4):
+    s = scenes.cube_scene_small(seed=seed, ncells=ncells, E=1e4)
+    n = len(s["mass"])
+    rng = np.random.default_rng(seed)
+    F = np.tile(np.eye(3), (n, 1, 1)) + 0.03 * rng.normal(size=(n, 3, 3))
+    s["F"] = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(n, 9))
+    s["C"] = 0.5 * rng.normal(size=(n, 9))
+    s["V"][:, 1] -= 1.0
+    y0 = float(s["X"][:, 1].min() + 0.3 * s["dx"])
+    s["colliders"] = [dict(shape=0, type=1, origin=(0.0, y0, 0.0), normal=(0.0, 1.0, 0.0))]
+    s["mu"] = s["mu"] * np.exp(0.3 * rng.normal(size=n))
+    s["lam"] = s["lam"] * np.exp(0.3 * rng.normal(size=n))
+    return s
+
+
+def test_one_step_adjoint_parity():
+    s = _scene()
+    n = len(s["mass"])
+    rng = np.random.default_rng(1)
+    w = dict(X=rng.normal(size=(n, 3)), V=rng.normal(size=(n, 3)), C=rng.normal(size=(n, 9)),
+             F=rng.normal(size=(n, 9)))
+    eng, ora = make_pair(s, colliders=s["colliders"])
+    eng.adjoint_set(**w)
+    eng.adjoint_step(DT)
+    g = eng.adjoint_get()
+    aX, aV, aC, aF = (np.ascontiguousarray(w[k].copy()) for k in ("X", "V", "C", "F"))
+    amu, alam = np.zeros(n), np.zeros(n)
+    ora.adjoint_step(DT, aX, aV, aC, aF, amu, alam)
+    for k, ref in (("X", aX), ("V", aV), ("C", aC), ("F", aF), ("mu", amu), ("lam", alam)):
+        assert rel_err(g[k], ref) < 1e-10, k
+    # the particle state is unchanged by the reverse step
+    out = eng.download()
+    assert np.array_equal(out["X"], s["X"])
+    eng.close()
+
+
+def _loss_and_grad(X, m, target):
+    com = (m[:, None] * X).sum(0) / m.sum()
+    return float(np.sum((com - target) ** 2)), 2 * (com - target)[None, :] * (m / m.sum())[:, None]
+
+
+@pytest.mark.parametrize("every", [4, 16])
+def test_rollout_gradients(every):
+    s = _scene(seed=5)
+    n = len(s["mass"])
+    nu = 0.3
+    rng = np.random.default_rng(5)
+    E = 1e4 * np.exp(0.4 * rng.normal(size=n))
+    V0 = s["V"].copy()
+    steps = 12
+    target = s["X"].mean(0) + np.array([0.01, 0.02, -0.01])
+    prob = Rollout(s, steps, DT, nu=nu, checkpoint_every=every)
+    X = prob.forward(E, V0)
+    loss, gX = _loss_and_grad(X, s["mass"], target)
+    gE, gV = prob.backward(gX)
+    # CPU adjoint oracle over the same rollout
+    mu, lam = lame_from_E(E, nu)
+    states = []
+    st = dict(X=s["X"].copy(), V=V0.copy(), C=s["C"].copy(), F=s["F"].copy())
+    for k in range(steps):
+        states.append(st)
+        sim = oracle.OracleSim(dx=s["dx"], gravity=(0.0, -9.8, 0.0), nthreads=8)
+        sim.set_particles(st["X"], st["V"], st["C"], st["F"], s["mass"], s["vol"], mu, lam)
+        sim.add_collider(**s["colliders"][0])
+        sim.set_time(k * DT)
+        sim.step(DT)
+        Xo, Vo, Co, Fo = sim.get_particles()
+        st = dict(X=Xo, V=Vo, C=Co, F=Fo)
+    assert rel_err(X, st["X"]) < 1e-10
+    aX = np.ascontiguousarray(gX)
+    aV, aC, aF = np.zeros((n, 3)), np.zeros((n, 9)), np.zeros((n, 9))
+    amu, alam = np.zeros(n), np.zeros(n)
+    for k in range(steps - 1, -1, -1):
+        st = states[k]
+        sim = oracle.OracleSim(dx=s["dx"], gravity=(0.0, -9.8, 0.0), nthreads=8)
+        sim.set_particles(st["X"], st["V"], st["C"], st["F"], s["mass"], s["vol"], mu, lam)
+        sim.add_collider(**s["colliders"][0])
+        sim.set_time(k * DT)
+        sim.adjoint_step(DT, aX, aV, aC, aF, amu, alam)
+    gE_o = amu / (2 * (1 + nu)) + alam * nu / ((1 + nu) * (1 - 2 * nu))
+    assert rel_err(gE, gE_o) < 1e-9
+    assert rel_err(gV, aV) < 1e-9
+    # central differences of the GPU forward
+    for grad, which in ((gE, "E"), (gV, "V")):
+        d = rng.normal(size=grad.shape)
+        h = 1e-2 if which == "E" else 1e-6
+        vals = []
+        for sign in (1, -1):
+            Ep = E + sign * h * d if which == "E" else E
+            Vp = V0 + sign * h * d if which == "V" else V0
+            vals.append(_loss_and_grad(prob.forward(Ep, Vp), s["mass"], target)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        ad = np.sum(grad * d)
+        assert abs(fd - ad) <= 1e-4 * abs(fd), (which, fd, ad)
+    prob.close()
+
+
+def test_torch_autograd_binding():
+    s = _scene(seed=7)
+    n = len(s["mass"])
+    rng = np.random.default_rng(7)
+    E = torch.tensor(1e4 * np.exp(0.4 * rng.normal(size=n)), dtype=torch.float64, requires_grad=True)
+    V0 = torch.tensor(s["V"], dtype=torch.float64, requires_grad=True)
+    m = torch.tensor(s["mass"], dtype=torch.float64)
+    target = torch.tensor(s["X"].mean(0) + np.array([0.0, 0.01, 0.0]), dtype=torch.float64)
+    prob = Rollout(s, 10, DT, checkpoint_every=3)
+    X = rollout(E, V0, prob)
+    com = (m[:, None] * X).sum(0) / m.sum()
+    loss = ((com - target) ** 2).sum()
+    loss.backward()
+    gX = (2 * (com - target)[None, :] * (m / m.sum())[:, None]).detach().numpy()
+    prob.forward(E.detach().numpy(), V0.detach().numpy())
+    gE, gV = prob.backward(gX)
+    assert np.allclose(E.grad.numpy(), gE, rtol=1e-12, atol=0)
+    assert np.allclose(V0.grad.numpy(), gV, rtol=1e-12, atol=0)
+    prob.close()

@@ -46,7 +46,7 @@ struct ColliderSet {
   zr_collider c[kMaxColliders];
 };
 
-enum { CNT_GROUPS = 0, CNT_SLOTS = 1, CNT_DOMAIN = 2, CNT_CFL = 3, CNT_TOTAL = 4, CNT_N = 8 };
+enum { CNT_GROUPS = 0, CNT_SLOTS = 1, CNT_DOMAIN = 2, CNT_CFL = 3, CNT_TOTAL = 4, CNT_ADJ_BAD = 5, CNT_N = 8 };
 
 // ---------------------------------------------------------------------------
 // collisions: AnalyticCollisionObject with translation-only motion
@@ -521,6 +521,7 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
 
 #include "zr_implicit.hpp"
 #include "zr_damage.hpp"
+#include "zr_adjoint.hpp"
 
 // ---------------------------------------------------------------------------
 // G2P, MLS/APIC (MpmSimulationBase.cpp:906-1016) + evolveStrain
@@ -751,6 +752,15 @@ struct zr_engine {
   size_t pf_node_cap = 0;
   double* pf_partial = nullptr;
   size_t pf_partial_cap = 0;
+  // differentiable form: particle adjoints [38][n] (x 3, v 3, C 9, F 9, mu,
+  // lambda, G2P scratch x 3 and F 9), node adjoints [7][grid capacity],
+  // device checkpoints of (x, v, C, F) with their times
+  double* adj = nullptr;
+  int64_t adj_n = 0;
+  double* adj_grid = nullptr;
+  size_t adj_grid_cap = 0;
+  std::vector<double*> ckpt;
+  std::vector<double> ckpt_time;
   uint32_t *p_page = nullptr, *p_re = nullptr, *order = nullptr;
   uint64_t* keys = nullptr;
   uint32_t *group_page = nullptr, *group_count = nullptr, *group_start = nullptr;
@@ -886,6 +896,11 @@ void free_particles(zr_engine* e)
   dfree(e->pf_c); dfree(e->pf_H); dfree(e->pf_l0); dfree(e->pf_osc); dfree(e->pf_Fp); dfree(e->pf_Hmax);
   dfree(e->pf_k); dfree(e->pf_vol); dfree(e->pf_allow);
   e->pf_on = false;
+  dfree(e->adj);
+  e->adj_n = 0;
+  for (auto& c : e->ckpt) dfree(c);
+  e->ckpt.clear();
+  e->ckpt_time.clear();
   e->n = 0;
 }
 
@@ -1141,6 +1156,75 @@ int do_pf(zr_engine* e, double dt)
   ZR_HIP(e, hipStreamSynchronize(s));
   e->pf_stats.cg_iterations = e->hci[CI_ITERS];
   e->pf_stats.residual_norm = e->hsc[PF_NORM];
+  return ZR_OK;
+}
+
+// ------------------------------------------------------ differentiable form
+constexpr int kAdjRows = 38;
+
+AdjointView adj_view(zr_engine* e)
+{
+  const int64_t n = e->n;
+  double* a = e->adj;
+  return AdjointView{ a, a + 3 * n, a + 6 * n, a + 15 * n, a + 24 * n, a + 25 * n, a + 26 * n, a + 29 * n };
+}
+
+int ensure_adjoint(zr_engine* e)
+{
+  if (e->adj_n != e->n) {
+    dfree(e->adj);
+    ZR_HIP(e, hipMalloc(&e->adj, kAdjRows * (size_t)e->n * sizeof(double)));
+    ZR_HIP(e, hipMemset(e->adj, 0, kAdjRows * (size_t)e->n * sizeof(double)));
+    e->adj_n = e->n;
+  }
+  return ZR_OK;
+}
+
+bool adjoint_supported(zr_engine* e)
+{
+  if (e->has_models || e->has_plastic || e->pf_on) {
+    fail(e, ZR_ERR_ARG, "the adjoint supports fixed-corotated particles without plasticity or damage only");
+    return false;
+  }
+  return true;
+}
+
+// reverse of one explicit step from the current (input) particle state
+int do_adjoint_step(zr_engine* e, double dt)
+{
+  int rc = do_sort(e, nullptr);
+  if (rc) return rc;
+  rc = do_p2g(e, true);
+  if (rc) return rc;
+  rc = do_grid(e, GRID_COMBINE_UPDATE, dt);
+  if (rc) return rc;
+  if (e->adj_grid_cap < e->grid_cap) {
+    dfree(e->adj_grid);
+    ZR_HIP(e, hipMalloc(&e->adj_grid, kAdjGridFields * e->grid_cap * sizeof(double)));
+    e->adj_grid_cap = e->grid_cap;
+  }
+  const double dx = e->cfg.dx;
+  hipStream_t s = e->stream;
+  AdjointView A = adj_view(e);
+  ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_ADJ_BAD, 0, sizeof(uint32_t), s));
+  k_adj_g2p<<<e->A, 256, 0, s>>>(pview(e), A, e->order, e->group_start, e->group_page, e->group_nslot, e->gvs,
+                                 (int64_t)e->grid_cap, e->stage, dx, 1.0 / dx, 4.0 / (dx * dx), dt);
+  ZR_LAUNCH_CHECK(e);
+  const uint32_t nn = e->S * 32;
+  GridView G{ e->gm, e->gv, e->gf, e->gvs, (int64_t)e->grid_cap };
+  // the grid kernel reads the adjoint stride-wise like the grid fields
+  k_adj_grid<<<blocks_for(nn, 256), 256, 0, s>>>(nn, e->slot_page, e->slot_src, e->stage, G, e->adj_grid, dx, dt,
+                                                 e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time,
+                                                 collider_set(e), e->dcount + CNT_ADJ_BAD);
+  ZR_LAUNCH_CHECK(e);
+  k_adj_p2g<<<e->A, 256, 0, s>>>(pview(e), A, e->order, e->group_start, e->group_page, e->group_nslot, e->adj_grid,
+                                 (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx));
+  ZR_LAUNCH_CHECK(e);
+  rc = sync_counts(e);
+  if (rc) return rc;
+  // the particles were not moved: the sort stays valid, the grid does too
+  if (e->hcount[CNT_ADJ_BAD])
+    return fail(e, ZR_ERR_ARG, "adjoint: a collider branch without a derivative (friction) was hit");
   return ZR_OK;
 }
 
@@ -1429,7 +1513,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
-  dfree(e->pf_node); dfree(e->pf_partial);
+  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);
   if (e->hci) (void)hipHostFree(e->hci);
@@ -1705,6 +1789,124 @@ int zr_phase_field_last_stats(zr_engine* e, zr_phase_field_stats* st)
 {
   if (!e || !st) return ZR_ERR_ARG;
   *st = e->pf_stats;
+  return ZR_OK;
+}
+
+int zr_adjoint_set(zr_engine* e, const zr_particles_adjoint* a)
+{
+  if (!e || !a) return fail(e, ZR_ERR_ARG, "bad adjoint view");
+  if (a->n != e->n) return fail(e, ZR_ERR_ARG, "adjoint count differs from the particle count");
+  if (!adjoint_supported(e)) return ZR_ERR_ARG;
+  if (e->n == 0) return ZR_OK;
+  int rc = ensure_adjoint(e);
+  if (rc) return rc;
+  const size_t N = (size_t)e->n;
+  AdjointView A = adj_view(e);
+  double* tmp = nullptr;
+  ZR_HIP(e, hipMalloc(&tmp, 9 * N * sizeof(double)));
+  struct Item {
+    const double* src;
+    double* dst;
+    int nc;
+  } items[] = { { a->X, A.x, 3 }, { a->V, A.v, 3 }, { a->C, A.C, 9 }, { a->F, A.F, 9 }, { a->mu, A.mu, 1 },
+                { a->lambda, A.lam, 1 } };
+  for (auto& it : items) {
+    hipError_t st;
+    if (it.src) {
+      st = hipMemcpyAsync(tmp, it.src, it.nc * N * sizeof(double), hipMemcpyHostToDevice, e->stream);
+      if (st == hipSuccess) {
+        k_aos_to_soa<<<blocks_for((uint64_t)N * it.nc, 256), 256, 0, e->stream>>>(tmp, it.dst, (uint32_t)N, it.nc,
+                                                                                  e->n);
+        st = hipGetLastError();
+      }
+    } else {
+      st = hipMemsetAsync(it.dst, 0, it.nc * N * sizeof(double), e->stream);
+    }
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) {
+      (void)hipFree(tmp);
+      return fail(e, ZR_ERR_HIP, std::string("adjoint upload: ") + hipGetErrorString(st));
+    }
+  }
+  ZR_HIP(e, hipFree(tmp));
+  return ZR_OK;
+}
+
+int zr_adjoint_get(zr_engine* e, zr_particles_adjoint* a)
+{
+  if (!e || !a) return fail(e, ZR_ERR_ARG, "bad adjoint view");
+  if (a->n != e->n) return fail(e, ZR_ERR_ARG, "adjoint count differs from the particle count");
+  if (e->n == 0) return ZR_OK;
+  if (!e->adj) return fail(e, ZR_ERR_STATE, "no adjoint was set");
+  const size_t N = (size_t)e->n;
+  AdjointView A = adj_view(e);
+  double* tmp = nullptr;
+  ZR_HIP(e, hipMalloc(&tmp, 9 * N * sizeof(double)));
+  struct Item {
+    double* dst;
+    const double* src;
+    int nc;
+  } items[] = { { a->X, A.x, 3 }, { a->V, A.v, 3 }, { a->C, A.C, 9 }, { a->F, A.F, 9 }, { a->mu, A.mu, 1 },
+                { a->lambda, A.lam, 1 } };
+  for (auto& it : items) {
+    if (!it.dst) continue;
+    k_soa_to_aos<<<blocks_for((uint64_t)N * it.nc, 256), 256, 0, e->stream>>>(it.src, tmp, (uint32_t)N, it.nc, e->n);
+    hipError_t st = hipGetLastError();
+    if (st == hipSuccess) st = hipMemcpyAsync(it.dst, tmp, it.nc * N * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) {
+      (void)hipFree(tmp);
+      return fail(e, ZR_ERR_HIP, std::string("adjoint download: ") + hipGetErrorString(st));
+    }
+  }
+  ZR_HIP(e, hipFree(tmp));
+  return ZR_OK;
+}
+
+int zr_adjoint_step(zr_engine* e, double dt)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!(dt > 0.0)) return fail(e, ZR_ERR_ARG, "dt must be positive");
+  if (!adjoint_supported(e)) return ZR_ERR_ARG;
+  if (!e->adj || e->adj_n != e->n) return fail(e, ZR_ERR_STATE, "zr_adjoint_set was not called");
+  int rc = do_adjoint_step(e, dt);
+  if (rc) return rc;
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+int zr_checkpoint_save(zr_engine* e, int32_t slot)
+{
+  if (!e || slot < 0 || slot > 4096) return fail(e, ZR_ERR_ARG, "bad checkpoint slot");
+  const size_t N = (size_t)e->n;
+  if ((size_t)slot >= e->ckpt.size()) {
+    e->ckpt.resize((size_t)slot + 1, nullptr);
+    e->ckpt_time.resize((size_t)slot + 1, 0.0);
+  }
+  if (!e->ckpt[slot]) ZR_HIP(e, hipMalloc(&e->ckpt[slot], 24 * (N ? N : 1) * sizeof(double)));
+  double* c = e->ckpt[slot];
+  ZR_HIP(e, hipMemcpyAsync(c, e->x, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(c + 3 * N, e->v, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(c + 6 * N, e->C, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(c + 15 * N, e->F, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  e->ckpt_time[slot] = e->time;
+  return ZR_OK;
+}
+
+int zr_checkpoint_restore(zr_engine* e, int32_t slot)
+{
+  if (!e || slot < 0 || (size_t)slot >= e->ckpt.size() || !e->ckpt[slot])
+    return fail(e, ZR_ERR_ARG, "no such checkpoint");
+  const size_t N = (size_t)e->n;
+  const double* c = e->ckpt[slot];
+  ZR_HIP(e, hipMemcpyAsync(e->x, c, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->v, c + 3 * N, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->C, c + 6 * N, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->F, c + 15 * N, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  e->time = e->ckpt_time[slot];
+  e->sorted = e->rasterised = e->updated = e->be_ready = false;
   return ZR_OK;
 }
 

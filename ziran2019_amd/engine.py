@@ -44,7 +44,8 @@ EXPORTS = (
     "zr_particles_to_grid_apic", "zr_backward_euler_step", "zr_grid_to_particles_apic",
     "zr_advance_one_time_step_implicit", "zr_implicit_residual", "zr_implicit_multiply",
     "zr_set_materials", "zr_get_materials", "zr_set_phase_field", "zr_get_phase_field", "zr_phase_field_solve",
-    "zr_phase_field_last_stats",
+    "zr_phase_field_last_stats", "zr_adjoint_set", "zr_adjoint_get", "zr_adjoint_step", "zr_checkpoint_save",
+    "zr_checkpoint_restore",
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
 ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER = 0, 1, 2
@@ -150,6 +151,10 @@ def nhb_one_over_sigma_c(percentage, mu, lam):
     return 1.0 / (kappa / 2.0 * ((J * J - 1.0) / 2.0 - np.log(J)))
 
 
+class ZrParticlesAdjoint(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("X", _dp), ("V", _dp), ("C", _dp), ("F", _dp), ("mu", _dp), ("lam", _dp)]
+
+
 class ZrStepStats(ctypes.Structure):
     _fields_ = [("num_groups", ctypes.c_int64), ("num_pages", ctypes.c_int64), ("cfl_flags", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -201,6 +206,11 @@ def load_library(path=LIB_PATH):
     L.zr_phase_field_solve.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrPhaseFieldStats)]
     L.zr_phase_field_last_stats.argtypes = [vp, ctypes.POINTER(ZrPhaseFieldStats)]
     L.zr_get_materials.argtypes = [vp, ctypes.POINTER(ZrMaterials)]
+    L.zr_adjoint_set.argtypes = [vp, ctypes.POINTER(ZrParticlesAdjoint)]
+    L.zr_adjoint_get.argtypes = [vp, ctypes.POINTER(ZrParticlesAdjoint)]
+    L.zr_adjoint_step.argtypes = [vp, ctypes.c_double]
+    L.zr_checkpoint_save.argtypes = [vp, ctypes.c_int32]
+    L.zr_checkpoint_restore.argtypes = [vp, ctypes.c_int32]
     L.zr_profile_enable.argtypes = [vp, ctypes.c_int32]
     L.zr_profile_read.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_int64)]
@@ -336,6 +346,33 @@ class MpmEngine:
         st = ZrPhaseFieldStats()
         self._check(self.L.zr_phase_field_last_stats(self.h, ctypes.byref(st)), "phase_field_last_stats")
         return st
+
+    # ------------------------------------------------- differentiable form
+    def adjoint_set(self, X=None, V=None, C=None, F=None, mu=None, lam=None):
+        n = self.n
+
+        def f(a, c):
+            return None if a is None else np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(n * c))
+        arrs = [f(X, 3), f(V, 3), f(C, 9), f(F, 9), f(mu, 1), f(lam, 1)]
+        a = ZrParticlesAdjoint(n, *[_dptr(x) for x in arrs])
+        self._check(self.L.zr_adjoint_set(self.h, ctypes.byref(a)), "adjoint_set")
+
+    def adjoint_get(self):
+        n = self.n
+        out = dict(X=np.zeros((n, 3)), V=np.zeros((n, 3)), C=np.zeros((n, 9)), F=np.zeros((n, 9)), mu=np.zeros(n),
+                   lam=np.zeros(n))
+        a = ZrParticlesAdjoint(n, *[_dptr(out[k]) for k in ("X", "V", "C", "F", "mu", "lam")])
+        self._check(self.L.zr_adjoint_get(self.h, ctypes.byref(a)), "adjoint_get")
+        return out
+
+    def adjoint_step(self, dt):
+        self._check(self.L.zr_adjoint_step(self.h, float(dt)), "adjoint_step")
+
+    def checkpoint_save(self, slot):
+        self._check(self.L.zr_checkpoint_save(self.h, int(slot)), "checkpoint_save")
+
+    def checkpoint_restore(self, slot):
+        self._check(self.L.zr_checkpoint_restore(self.h, int(slot)), "checkpoint_restore")
 
     def get_materials(self):
         n = self.n
