@@ -61,6 +61,22 @@ def pf_opts(parabolic_M=0.0, cg_max_iterations=100, naive=False, cg_tolerance=1e
                      float(cg_relative), float(sigma_F))
 
 
+class ZroAdmmOpts(ctypes.Structure):
+    _fields_ = [("admm_iterations", ctypes.c_int32), ("minres_max_iterations", ctypes.c_int32),
+                ("rho_scale", _d), ("global_tolerance", _d), ("local_tolerance", _d)]
+
+
+class ZroAdmmStats(ctypes.Structure):
+    _fields_ = [("minres_iterations", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def admm_opts(admm_iterations=100, minres_max_iterations=10000, rho_scale=0.5, global_tolerance=1e-6,
+              local_tolerance=1e-8):
+    """AdmmSimulation defaults (AdmmSimulationFoam.h:55-68)."""
+    return ZroAdmmOpts(int(admm_iterations), int(minres_max_iterations), float(rho_scale), float(global_tolerance),
+                       float(local_tolerance))
+
+
 class ZroCollider(ctypes.Structure):
     _fields_ = [
         ("shape", ctypes.c_int32),
@@ -169,6 +185,10 @@ def lib():
         L.zro_fcr_kirchhoff_adjoint.argtypes = [_d, _d, _p, _p, _p, _p, _p]
         L.zro_adjoint_step.restype = _i
         L.zro_adjoint_step.argtypes = [_p, _d] + [_p] * 6
+        L.zro_step_admm.restype = _i
+        L.zro_step_admm.argtypes = [_p, _d, ctypes.POINTER(ZroAdmmOpts), ctypes.POINTER(ZroAdmmStats)]
+        L.zro_get_admm.argtypes = [_p, _p, _p, _p]
+        L.zro_sym3_eigenvalues.argtypes = [_p, _p]
         L.zro_sym3_max_eigenvalue.restype = _d
         L.zro_sym3_max_eigenvalue.argtypes = [_p]
         L.zro_get_materials.argtypes = [_p, _p, _p, _p]
@@ -301,6 +321,12 @@ def fcr_kirchhoff_adjoint(mu, lam, F, taubar, Fbar=None):
     return _from_cm(fb), float(mb[0]), float(lb[0])
 
 
+def sym3_eigenvalues(A):
+    lam = np.zeros(3)
+    lib().zro_sym3_eigenvalues(_ptr(_cm(A)), _ptr(lam))
+    return lam
+
+
 def sym3_max_eigenvalue(A):
     return float(lib().zro_sym3_max_eigenvalue(_ptr(_cm(A))))
 
@@ -407,6 +433,18 @@ class OracleSim:
                                     _ptr(alam))
         if rc != 0:
             raise RuntimeError("adjoint: a collider branch without a derivative was hit")
+
+    def step_admm(self, dt, opts=None):
+        st = ZroAdmmStats()
+        o = opts if opts is not None else admm_opts()
+        flags = int(lib().zro_step_admm(self.h, float(dt), ctypes.byref(o), ctypes.byref(st)))
+        return flags, st
+
+    def get_admm(self):
+        n = self.n
+        F, y, om = np.zeros(9 * n), np.zeros(9 * n), np.zeros(n)
+        lib().zro_get_admm(self.h, _ptr(F), _ptr(y), _ptr(om))
+        return dict(F=F.reshape(n, 9), y=y.reshape(n, 9), omega=om)
 
     def get_materials(self):
         n = self.n

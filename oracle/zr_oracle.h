@@ -188,6 +188,30 @@ void zro_phase_field_multiply(zro_sim* s, const double* x, double* b);
 int zro_adjoint_step(zro_sim* s, double dt, double* aX, double* aV, double* aC, double* aF, double* amu,
                      double* alam);
 
+/* ---- ADMM step (Projects/admm/AdmmSimulationFoam.h:163-801) ----
+   StvkWithHencky particles with DummyPlasticity (ZRO_PLASTIC_NONE), von
+   Mises or Drucker-Prager; no viscoelastic branch.  Runs the whole
+   AdmmSimulation::advanceOneTimeStep: sort, APIC P2G, admmStep, APIC G2P
+   with the F update and the return mapping. */
+typedef struct {
+  int32_t admm_iterations;        /* admm_max_iterations (reference default 100) */
+  int32_t minres_max_iterations;  /* global_max_iteration 10000 */
+  double rho_scale;               /* 0.5 */
+  double global_tolerance;        /* 1e-6 */
+  double local_tolerance;         /* 1e-8 */
+} zro_admm_opts;
+
+typedef struct {
+  int32_t minres_iterations;      /* total over the ADMM iterations */
+  int32_t reserved;
+} zro_admm_stats;
+
+int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats* st);
+/* per particle ADMM state after the last step: F (9n), y (9n), omega (n) */
+void zro_get_admm(const zro_sim* s, double* F, double* y, double* omega);
+/* all three eigenvalues of a symmetric 3x3 matrix, ascending */
+void zro_sym3_eigenvalues(const double* A, double* lam);
+
 /* largest eigenvalue of a symmetric 3x3 matrix (column-major) */
 double zro_sym3_max_eigenvalue(const double* A);
 /* matrix-free operator of the last backward-Euler solve at the final Newton

@@ -126,6 +126,14 @@ struct zro_sim {
   double *pf_phase, *pf_mult, *pf_rhs, *pf_diag, *pf_x;
   int64_t pf_cap;
   double pf_dt;
+  /* ADMM (AdmmSimulationFoam.h:48-70): per particle F, y, omega, last_dv;
+     per node slot vn, dv, collision projection, preconditioner diagonal */
+  double *admm_F, *admm_y, *admm_omega, *admm_last_dv;
+  int64_t admm_n;
+  int admm_steps;
+  double *admm_vn, *admm_dv, *admm_proj, *admm_diag;
+  char* admm_coll;
+  int64_t admm_cap;
   double *vtau, *gradV;
   uint64_t* sorter;
   int32_t* order;
@@ -187,6 +195,8 @@ void zro_destroy(zro_sim* s)
   free(s->explicit_collided);
   free(s->vn); free(s->dv); free(s->proj); free(s->collided);
   free(s->pf_phase); free(s->pf_mult); free(s->pf_rhs); free(s->pf_diag); free(s->pf_x);
+  free(s->admm_F); free(s->admm_y); free(s->admm_omega); free(s->admm_last_dv);
+  free(s->admm_vn); free(s->admm_dv); free(s->admm_proj); free(s->admm_diag); free(s->admm_coll);
   free(s);
 }
 
@@ -1810,4 +1820,670 @@ int zro_adjoint_step(zro_sim* s, double dt, double* aX, double* aV, double* aC, 
   }
   free(gv); free(gf); free(avs); free(ag); free(xin); free(Fin);
   return ok ? 0 : -1;
+}
+
+/* ====================================================================== */
+/* ADMM step (Projects/admm/AdmmSimulationFoam.h, AdmmFormula.h)           */
+/* ====================================================================== */
+
+/* eigenvalues of a symmetric 3x3 matrix, ascending (the trigonometric closed
+   form of the characteristic cubic, O. K. Smith 1961) */
+void zro_sym3_eigenvalues(const double* A, double* lam)
+{
+  double a00 = A[0], a11 = A[4], a22 = A[8];
+  double a01 = 0.5 * (A[3] + A[1]), a02 = 0.5 * (A[6] + A[2]), a12 = 0.5 * (A[7] + A[5]);
+  double p1 = a01 * a01 + a02 * a02 + a12 * a12;
+  double q = (a00 + a11 + a22) / 3.0;
+  double b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+  double p2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0 * p1;
+  if (p2 <= 0.0) {
+    lam[0] = lam[1] = lam[2] = q;
+    return;
+  }
+  double p = sqrt(p2 / 6.0), ip = 1.0 / p;
+  double c00 = b00 * ip, c11 = b11 * ip, c22 = b22 * ip, c01 = a01 * ip, c02 = a02 * ip, c12 = a12 * ip;
+  double det = c00 * (c11 * c22 - c12 * c12) - c01 * (c01 * c22 - c12 * c02) + c02 * (c01 * c12 - c11 * c02);
+  double r = 0.5 * det;
+  r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
+  double phi = acos(r) / 3.0;
+  double e0 = q + 2.0 * p * cos(phi);
+  double e2 = q + 2.0 * p * cos(phi + 2.0943951023931957);
+  lam[0] = e2;
+  lam[1] = 3.0 * q - e0 - e2;
+  lam[2] = e0;
+}
+
+/* |makePD(A)|_F^2 = sum of the squared nonnegative parts of the eigenvalues
+   (AdmmFormula.h:12-35): the eigenvectors are orthonormal */
+static double pd_norm2_3(const double A[3][3])
+{
+  double a[9], l[3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[j * 3 + i] = A[i][j];
+  zro_sym3_eigenvalues(a, l);
+  double t = 0.0;
+  for (int i = 0; i < 3; ++i) t += l[i] > 0.0 ? l[i] * l[i] : 0.0;
+  return t;
+}
+static double pd_norm2_2(double a, double b, double d)  /* [[a b][b d]] */
+{
+  double m = 0.5 * (a + d), r = sqrt(0.25 * (a - d) * (a - d) + b * b);
+  double l0 = m - r, l1 = m + r, t = 0.0;
+  if (l0 > 0.0) t += l0 * l0;
+  if (l1 > 0.0) t += l1 * l1;
+  return t;
+}
+
+/* StvkWithHencky::firstPiolaDiagonal / firstPiolaDerivativeDiagonal (StvkWithHencky.cpp:150-204) */
+static void hencky_dE(double mu, double lam, const double* s, double* dE)
+{
+  double l[3] = { log(fabs(s[0])), log(fabs(s[1])), log(fabs(s[2])) };
+  double sl = l[0] + l[1] + l[2];
+  for (int i = 0; i < 3; ++i) dE[i] = (2.0 * mu * l[i] + lam * sl) / s[i];
+}
+static void hencky_ddE(double mu, double lam, const double* s, double d[3][3])
+{
+  double l[3] = { log(fabs(s[0])), log(fabs(s[1])), log(fabs(s[2])) };
+  double g = 2.0 * mu + lam;
+  d[0][0] = (g * (1.0 - l[0]) - lam * (l[1] + l[2])) / (s[0] * s[0]);
+  d[1][1] = (g * (1.0 - l[1]) - lam * (l[0] + l[2])) / (s[1] * s[1]);
+  d[2][2] = (g * (1.0 - l[2]) - lam * (l[0] + l[1])) / (s[2] * s[2]);
+  d[0][1] = d[1][0] = lam / (s[0] * s[1]);
+  d[0][2] = d[2][0] = lam / (s[0] * s[2]);
+  d[1][2] = d[2][1] = lam / (s[1] * s[2]);
+}
+/* StvkWithHencky::Bij, 3-D branch (StvkWithHencky.cpp:34-76) with
+   MathTools::diff_interlock_log_over_diff (MathTools.h:197-262) */
+static void hencky_Bij(double mu, double lam, const double* s, int i, int j, double clamp, double B[2][2])
+{
+  double dE[3];
+  hencky_dE(mu, lam, s, dE);
+  double sum = s[i] + s[j];
+  double mag = fabs(sum), sign = sum > 0 ? 1.0 : -1.0;
+  double cl = (mag > clamp ? mag : clamp) * sign;
+  double BP = 0.5 * (dE[i] + dE[j]) / cl;
+  double l[3] = { log(fabs(s[0])), log(fabs(s[1])), log(fabs(s[2])) };
+  double sl = l[0] + l[1] + l[2];
+  double eps = 1e-8;
+  double p = s[i] / s[j] - 1.0;
+  double l1p = fabs(p) < eps ? 1.0 : log1p(p) / p;
+  double dlod = l1p / s[j];
+  double interlock = l[j] - s[j] * dlod;
+  double BM = -0.5 * (lam * sl + 2.0 * mu * interlock) / (s[i] * s[j]);
+  B[0][0] = BP + BM;
+  B[0][1] = -BP + BM;
+  B[1][0] = -BP + BM;
+  B[1][1] = BP + BM;
+}
+
+/* projectSigma of the particle's plasticity (PlasticityApplier.cpp:321-346, 528-547;
+   DummyPlasticity returns sigma) */
+static void admm_project_sigma(const zro_sim* s, int64_t i, const double* sig, double* out)
+{
+  int pl = s->plastic ? s->plastic[i] : ZRO_PLASTIC_NONE;
+  double mu = s->mu[i], lam = s->lam[i];
+  const double* q = s->pparam ? s->pparam + 4 * i : NULL;
+  for (int d = 0; d < 3; ++d) out[d] = sig[d];
+  if (pl == ZRO_PLASTIC_VON_MISES) {
+    double e[3], tr = 0.0, n2 = 0.0, h[3];
+    for (int d = 0; d < 3; ++d) { e[d] = log(sig[d] > 1e-4 ? sig[d] : 1e-4); tr += e[d]; }
+    for (int d = 0; d < 3; ++d) { h[d] = e[d] - tr / 3.0; n2 += h[d] * h[d]; }
+    double nrm = sqrt(n2), dg = nrm - s->pstate[i] / (2.0 * mu);
+    if (dg <= 0.0) return;
+    for (int d = 0; d < 3; ++d) out[d] = exp(e[d] - (dg / nrm) * h[d]);
+  } else if (pl == ZRO_PLASTIC_DRUCKER_PRAGER) {
+    double alpha = q[0], coh = q[2];
+    double e[3], tr = 0.0, n2 = 0.0, h[3];
+    for (int d = 0; d < 3; ++d) {
+      double a = fabs(sig[d]);
+      e[d] = log(a > 1e-4 ? a : 1e-4) - coh;
+      tr += e[d];
+    }
+    for (int d = 0; d < 3; ++d) { h[d] = e[d] - tr / 3.0; n2 += h[d] * h[d]; }
+    if (tr >= 0.0) { for (int d = 0; d < 3; ++d) out[d] = exp(coh); return; }
+    double nrm = sqrt(n2);
+    double dg = nrm + (3.0 * lam + 2.0 * mu) / (2.0 * mu) * tr * alpha;
+    for (int d = 0; d < 3; ++d) out[d] = exp(dg <= 0.0 ? e[d] + coh : e[d] - (dg / nrm) * h[d] + coh);
+  }
+}
+
+/* projectSigmaAndDerivative (PlasticityApplier.cpp:389-423, 580-606; Dummy: identity) */
+static void admm_project_sigma_d(const zro_sim* s, int64_t i, const double* sig, double* zp, double D[3][3])
+{
+  int pl = s->plastic ? s->plastic[i] : ZRO_PLASTIC_NONE;
+  double mu = s->mu[i], lam = s->lam[i];
+  const double* q = s->pparam ? s->pparam + 4 * i : NULL;
+  for (int a = 0; a < 3; ++a) {
+    zp[a] = sig[a];
+    for (int b = 0; b < 3; ++b) D[a][b] = a == b ? 1.0 : 0.0;
+  }
+  if (pl == ZRO_PLASTIC_VON_MISES) {
+    double ty = s->pstate[i];
+    double e[3], tr = 0.0, n2 = 0.0, h[3];
+    for (int d = 0; d < 3; ++d) { e[d] = log(sig[d] > 1e-4 ? sig[d] : 1e-4); tr += e[d]; }
+    for (int d = 0; d < 3; ++d) { h[d] = e[d] - tr / 3.0; n2 += h[d] * h[d]; }
+    double nrm = sqrt(n2), dg = nrm - ty / (2.0 * mu);
+    if (dg <= 0.0) return;
+    double w[3], sv[3], sn = 0.0, sh[3], H[3];
+    for (int d = 0; d < 3; ++d) { w[d] = 1.0 / sig[d]; sv[d] = e[d] - tr / 3.0; sn += sv[d] * sv[d]; }
+    sn = sqrt(sn);
+    for (int d = 0; d < 3; ++d) { sh[d] = sv[d] / sn; H[d] = e[d] - (dg / nrm) * h[d]; zp[d] = exp(H[d]); }
+    double a1 = 1.0 - ty / (2.0 * mu * sn), a2 = ty / (2.0 * mu * sn);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double Iw = r == c ? w[c] : 0.0;
+        double inner = Iw - a1 * (Iw - w[c] / 3.0) - a2 * sh[r] * sh[c] * w[c];
+        D[r][c] = zp[r] * inner;
+      }
+  } else if (pl == ZRO_PLASTIC_DRUCKER_PRAGER) {
+    double alpha = q[0], coh = q[2];
+    double e[3], tr = 0.0, n2 = 0.0, h[3];
+    for (int d = 0; d < 3; ++d) {
+      double a = fabs(sig[d]);
+      e[d] = log(a > 1e-4 ? a : 1e-4) - coh;
+      tr += e[d];
+    }
+    for (int d = 0; d < 3; ++d) { h[d] = e[d] - tr / 3.0; n2 += h[d] * h[d]; }
+    double nrm = sqrt(n2);
+    double dg = nrm + (3.0 * lam + 2.0 * mu) / (2.0 * mu) * tr * alpha;
+    if (dg <= 0.0) {
+      for (int d = 0; d < 3; ++d) zp[d] = exp(e[d] + coh);
+      return;
+    }
+    if (tr > 0.0 || nrm == 0.0) {
+      for (int d = 0; d < 3; ++d) {
+        zp[d] = exp(coh);
+        for (int c = 0; c < 3; ++c) D[d][c] = 0.0;
+      }
+      return;
+    }
+    double w[3], sv[3], sn = 0.0, sh[3];
+    double k = tr;
+    for (int d = 0; d < 3; ++d) { w[d] = 1.0 / sig[d]; sv[d] = e[d] - k / 3.0; sn += sv[d] * sv[d]; }
+    sn = sqrt(sn);
+    for (int d = 0; d < 3; ++d) sh[d] = sv[d] / sn;
+    double pp = alpha * k * (3.0 * lam + 2.0 * mu) / (2.0 * mu * sn);
+    for (int d = 0; d < 3; ++d) zp[d] = exp(e[d] - (dg / nrm) * h[d] + coh);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double t1 = ((1.0 + 2.0 * pp) / 3.0 - pp / k * e[r]) * w[c];
+        double t2 = pp * ((r == c ? 1.0 : 0.0) - sh[r] * sh[c]) * w[c];
+        D[r][c] = zp[r] * (t1 - t2);
+      }
+  }
+}
+
+/* initOmega (AdmmSimulationFoam.h:222-274) for one particle with F = U S V^T */
+static double admm_omega_of(const zro_sim* s, int64_t i, const double* sig)
+{
+  double mu = s->mu[i], lam = s->lam[i];
+  double A[3][3], ps[3], B[2][2];
+  hencky_ddE(mu, lam, sig, A);
+  double total = pd_norm2_3(A);
+  admm_project_sigma(s, i, sig, ps);
+  const int pr[3][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 } };
+  for (int k = 0; k < 3; ++k) {
+    hencky_Bij(mu, lam, ps, pr[k][0], pr[k][1], 1e-6, B);
+    total += pd_norm2_2(B[0][0], B[0][1], B[1][1]);
+  }
+  return sqrt(s->vol[i] * sqrt(total));
+}
+
+/* multiObjectCollisionWithFakeSeparate (CollisionObject.cpp:108-160): resolves v,
+   returns 1 and the projection P = I - K K^T when any collider acts */
+static int admm_collision(const zro_sim* s, double t, const double* x, double* v, double P[3][3])
+{
+  int any = 0, nslip = 0;
+  double K[3][3];
+  for (int k = 0; k < s->ncol; ++k) {
+    const zro_collider* c = &s->col[k];
+    double b[3], X[3], phi, n[3], old[3] = { v[0], v[1], v[2] };
+    collider_b(s, c, t, b);
+    for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
+    if (!ls_query_inside(c, X, &phi, n)) continue;
+    int col = detect_and_resolve(c, b, x, v);
+    if (col && c->type == 3) {
+      double dot = 0.0;
+      for (int d = 0; d < 3; ++d) dot += (old[d] - c->velocity[d]) * n[d];
+      if (dot > 0) {
+        for (int d = 0; d < 3; ++d) v[d] = old[d];
+        col = 0;
+      }
+    }
+    any = any || col;
+    if (!col) continue;
+    if (c->type == 1) {
+      memset(P, 0, sizeof(double) * 9);
+      return 1;
+    }
+    for (int q = 0; q < nslip; ++q) {
+      double dot = K[q][0] * n[0] + K[q][1] * n[1] + K[q][2] * n[2];
+      for (int d = 0; d < 3; ++d) n[d] -= dot * K[q][d];
+    }
+    double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l != 0.0) {
+      for (int d = 0; d < 3; ++d) K[nslip][d] = n[d] / l;
+      if (++nslip == 3) break;
+    }
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      P[i][j] = i == j ? 1.0 : 0.0;
+      for (int q = 0; q < nslip; ++q) P[i][j] -= K[q][i] * K[q][j];
+    }
+  return any;
+}
+
+/* updateCollisionNodes (AdmmSimulationFoam.h:276-294) */
+static void admm_update_collisions(zro_sim* s)
+{
+  for (int64_t p = 0; p < s->npages; ++p) {
+    int base[3];
+    zro_linear_to_coord(s->pages[p], base);
+    for (int e = 0; e < 32; ++e) {
+      int64_t k = p * 32 + e;
+      s->admm_coll[k] = 0;
+      if (s->nodes[k].idx < 0) continue;
+      double xi[3] = { (double)(base[0] + ((e >> 4) & 1)) * s->dx, (double)(base[1] + ((e >> 2) & 3)) * s->dx,
+                       (double)(base[2] + (e & 3)) * s->dx };
+      double vi[3], P[3][3];
+      for (int d = 0; d < 3; ++d) vi[d] = s->admm_dv[3 * k + d] + s->admm_vn[3 * k + d];
+      if (admm_collision(s, s->time, xi, vi, P)) {
+        s->admm_coll[k] = 1;
+        memcpy(s->admm_proj + 9 * k, P, sizeof(P));
+        for (int d = 0; d < 3; ++d) s->admm_dv[3 * k + d] = vi[d] - s->admm_vn[3 * k + d];
+      }
+    }
+  }
+}
+
+static void admm_project(const zro_sim* s, double* r)
+{
+  int64_t ns = s->npages * 32;
+  for (int64_t k = 0; k < ns; ++k) {
+    if (!s->admm_coll[k]) continue;
+    const double* P = s->admm_proj + 9 * k;
+    double t[3];
+    for (int i = 0; i < 3; ++i) t[i] = P[3 * i] * r[3 * k] + P[3 * i + 1] * r[3 * k + 1] + P[3 * i + 2] * r[3 * k + 2];
+    for (int i = 0; i < 3; ++i) r[3 * k + i] = t[i];
+  }
+}
+
+/* sum over the particle's active stencil nodes of vel_i grad w_i^T (3x3) */
+static void admm_grad(zro_sim* s, int64_t i, const double* vel, double G[3][3])
+{
+  zro_spline sp;
+  spline_of(s, s->X + 3 * i, &sp);
+  memset(G, 0, sizeof(double) * 9);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        double gw[3];
+        spline_node(s, &sp, a, b, c, gw);
+        int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+        if (s->nodes[k].idx < 0) continue;
+        for (int r = 0; r < 3; ++r)
+          for (int cc = 0; cc < 3; ++cc) G[r][cc] += vel[3 * k + r] * gw[cc];
+      }
+}
+
+/* out_i += S Fn^T grad w_i over the particle's active nodes */
+static void admm_scatter(zro_sim* s, int64_t i, const double S[3][3], const double Fn[3][3], double* out)
+{
+  zro_spline sp;
+  spline_of(s, s->X + 3 * i, &sp);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        double gw[3], fg[3];
+        spline_node(s, &sp, a, b, c, gw);
+        int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+        if (s->nodes[k].idx < 0) continue;
+        for (int d = 0; d < 3; ++d) fg[d] = Fn[0][d] * gw[0] + Fn[1][d] * gw[1] + Fn[2][d] * gw[2];
+        for (int r = 0; r < 3; ++r) out[3 * k + r] += S[r][0] * fg[0] + S[r][1] * fg[1] + S[r][2] * fg[2];
+      }
+}
+
+/* the global matrix (initMultiplierAndPreconditioner, :407-505) applied
+   matrix-free: b_i = m_i x_i + sum_p c_p (sum_j x_j grad w_j^T Fn) Fn^T grad w_i */
+static void admm_multiply(zro_sim* s, double dt, double rho, const double* x, double* b)
+{
+  int64_t ns = s->npages * 32;
+  memset(b, 0, sizeof(double) * (size_t)(3 * ns));
+  for (int64_t k = 0; k < ns; ++k)
+    if (s->nodes[k].idx >= 0)
+      for (int d = 0; d < 3; ++d) b[3 * k + d] = s->nodes[k].m * x[3 * k + d];
+  PF_FOR_PARTICLES(s, i)
+  {
+    double G[3][3], Fn[3][3], Y[3][3];
+    mat_from_cm(s->F + 9 * (int64_t)i, Fn);
+    admm_grad(s, i, x, G);
+    double cp = rho * s->admm_omega[i] * s->admm_omega[i] * dt * dt;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) Y[r][c] = cp * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
+    admm_scatter(s, i, Y, Fn, b);
+  }
+}
+
+static double admm_dot(const zro_sim* s, const double* a, const double* b)
+{
+  double acc = 0.0;
+  for (int64_t k = 0; k < s->npages * 32; ++k)
+    if (s->nodes[k].idx >= 0) acc += a[3 * k] * b[3 * k] + a[3 * k + 1] * b[3 * k + 1] + a[3 * k + 2] * b[3 * k + 2];
+  return acc;
+}
+
+/* GivensRotation::compute / rowRotation on a 2-vector (Givens.h:56-141) */
+typedef struct { double c, s; } zro_g2;
+static zro_g2 g2_make(void) { zro_g2 g = { 1.0, 0.0 }; return g; }
+static void g2_compute(zro_g2* g, double a, double b)
+{
+  double d = a * a + b * b;
+  g->c = 1.0;
+  g->s = 0.0;
+  double sqrtd = sqrt(d);
+  if (sqrtd) {
+    double t = 1.0 / sqrtd;
+    g->c = a * t;
+    g->s = -b * t;
+  }
+}
+static void g2_rot(const zro_g2* g, double* v)
+{
+  double t1 = v[0], t2 = v[1];
+  v[0] = g->c * t1 - g->s * t2;
+  v[1] = g->s * t1 + g->c * t2;
+}
+
+/* Minres::solve (Minres.h:69-178) with the preconditioner in / diag */
+static int admm_minres(zro_sim* s, double dt, double rho, double* x, const double* b, double tol, double rel,
+                       int max_it)
+{
+  int64_t n3 = 3 * s->npages * 32;
+  size_t bytes = sizeof(double) * (size_t)n3;
+  double *mk = calloc(n3, 8), *mkm1 = calloc(n3, 8), *mkm2 = calloc(n3, 8), *z = calloc(n3, 8);
+  double *qkm1 = calloc(n3, 8), *qk = calloc(n3, 8), *qkp1 = calloc(n3, 8), *tmp;
+  zro_g2 Gk = g2_make(), Gkm1 = g2_make(), Gkm2 = g2_make();
+  double gamma = 0, delta = 0, epsilon = 0, beta_kp1 = 0, alpha_k = 0, beta_k = 0, tk = 0, lt[2];
+  admm_multiply(s, dt, rho, x, qkp1);
+  for (int64_t k = 0; k < n3; ++k) qkp1[k] = b[k] - qkp1[k];
+  admm_project(s, qkp1);
+  for (int64_t k = 0; k < n3; ++k) z[k] = s->nodes[k / 3].idx >= 0 ? qkp1[k] / s->admm_diag[k / 3] : 0.0;
+  double res = sqrt(admm_dot(s, z, qkp1));
+  beta_kp1 = res;
+  double local_tol = rel * res < tol ? rel * res : tol;
+  int it = 0;
+  if (res < local_tol) goto done;
+  if (res > 0)
+    for (int64_t k = 0; k < n3; ++k) { qkp1[k] /= beta_kp1; z[k] /= beta_kp1; }
+  lt[0] = res;
+  lt[1] = 0.0;
+  for (it = 0; it < max_it; ++it) {
+    if (res < local_tol) break;
+    tmp = mkm2; mkm2 = mkm1; mkm1 = mk; mk = tmp;
+    memcpy(mk, z, bytes);
+    beta_k = beta_kp1;
+    tmp = qkm1; qkm1 = qk; qk = qkp1; qkp1 = tmp;
+    admm_multiply(s, dt, rho, mk, qkp1);
+    admm_project(s, qkp1);
+    alpha_k = admm_dot(s, mk, qkp1);
+    for (int64_t k = 0; k < n3; ++k) qkp1[k] -= alpha_k * qk[k];
+    for (int64_t k = 0; k < n3; ++k) qkp1[k] -= beta_k * qkm1[k];
+    for (int64_t k = 0; k < n3; ++k) z[k] = s->nodes[k / 3].idx >= 0 ? qkp1[k] / s->admm_diag[k / 3] : 0.0;
+    double bb = admm_dot(s, z, qkp1);
+    beta_kp1 = sqrt(bb > 0.0 ? bb : 0.0);
+    if (beta_kp1 > 0)
+      for (int64_t k = 0; k < n3; ++k) { qkp1[k] /= beta_kp1; z[k] /= beta_kp1; }
+    /* applyAllPreviousGivensRotationsAndDetermineNewGivens */
+    Gkm2 = Gkm1;
+    Gkm1 = Gk;
+    double ek[2] = { 0.0, beta_k };
+    g2_rot(&Gkm2, ek);
+    epsilon = ek[0];
+    double dk[2] = { ek[1], alpha_k };
+    g2_rot(&Gkm1, dk);
+    delta = dk[0];
+    double tv[2] = { dk[1], beta_kp1 };
+    g2_compute(&Gk, tv[0], tv[1]);
+    g2_rot(&Gk, tv);
+    gamma = tv[0];
+    g2_rot(&Gk, lt);
+    tk = lt[0];
+    double r = lt[1];
+    lt[0] = r;
+    lt[1] = 0.0;
+    res = r < 0 ? -r : r;
+    for (int64_t k = 0; k < n3; ++k) mk[k] = (mk[k] - delta * mkm1[k] - epsilon * mkm2[k]) / gamma;
+    for (int64_t k = 0; k < n3; ++k) x[k] += tk * mk[k];
+  }
+done:
+  free(mk); free(mkm1); free(mkm2); free(z); free(qkm1); free(qk); free(qkp1);
+  return it;
+}
+
+static double* admm_grow(double* p, int64_t n) { free(p); return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+
+int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats* st)
+{
+  const double rho = o->rho_scale;
+  int64_t n = s->n;
+  if (st) memset(st, 0, sizeof(*st));
+  zro_sort_and_pollute(s);
+  zro_p2g_apic(s);
+  int64_t ns = s->npages * 32;
+  if (s->admm_cap < ns) {
+    s->admm_vn = admm_grow(s->admm_vn, 3 * ns);
+    s->admm_dv = admm_grow(s->admm_dv, 3 * ns);
+    s->admm_proj = admm_grow(s->admm_proj, 9 * ns);
+    s->admm_diag = admm_grow(s->admm_diag, ns);
+    free(s->admm_coll);
+    s->admm_coll = (char*)calloc((size_t)ns, 1);
+    s->admm_cap = ns;
+  }
+  int first = s->admm_steps == 0 || s->admm_n != n;
+  if (s->admm_n != n) {
+    s->admm_F = admm_grow(s->admm_F, 9 * n);
+    s->admm_y = admm_grow(s->admm_y, 9 * n);
+    s->admm_omega = admm_grow(s->admm_omega, n);
+    s->admm_last_dv = admm_grow(s->admm_last_dv, 3 * n);
+    s->admm_n = n;
+  }
+  /* initStep (:296-401) */
+  memset(s->admm_vn, 0, sizeof(double) * 3 * ns);
+  memset(s->admm_dv, 0, sizeof(double) * 3 * ns);
+  for (int64_t k = 0; k < ns; ++k)
+    if (s->nodes[k].idx >= 0)
+      for (int d = 0; d < 3; ++d) s->admm_vn[3 * k + d] = s->nodes[k].v[d];
+  if (!first) {
+    double* wsum = (double*)calloc((size_t)ns, sizeof(double));
+    PF_FOR_PARTICLES(s, i)
+    {
+      zro_spline sp;
+      spline_of(s, s->X + 3 * i, &sp);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int c = 0; c < 3; ++c) {
+            double gw[3];
+            double w = spline_node(s, &sp, a, b, c, gw);
+            int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+            if (s->nodes[k].idx < 0) continue;
+            for (int d = 0; d < 3; ++d) s->admm_dv[3 * k + d] += s->admm_last_dv[3 * i + d] * w;
+            wsum[k] += w;
+          }
+    }
+    for (int64_t k = 0; k < ns; ++k)
+      if (s->nodes[k].idx >= 0)
+        for (int d = 0; d < 3; ++d) s->admm_dv[3 * k + d] = s->admm_dv[3 * k + d] * dt / wsum[k];
+    free(wsum);
+  }
+  admm_update_collisions(s);
+  double* nv = (double*)calloc((size_t)(3 * ns), sizeof(double));
+  for (int64_t k = 0; k < 3 * ns; ++k) nv[k] = s->admm_vn[k] + s->admm_dv[k];
+  for (int64_t i = 0; i < n; ++i) {
+    double G[3][3], Fn[3][3], F[3][3];
+    mat_from_cm(s->F + 9 * i, Fn);
+    admm_grad(s, i, nv, G);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) F[r][c] = Fn[r][c] + dt * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
+    mat_to_cm(F, s->admm_F + 9 * i);
+    double U[9], sg[3], V[9];
+    zro_svd3(s->admm_F + 9 * i, U, sg, V);
+    double om = admm_omega_of(s, i, sg);
+    double old = first ? 0.0 : s->admm_omega[i];
+    for (int c = 0; c < 9; ++c) s->admm_y[9 * i + c] = first ? 0.0 : s->admm_y[9 * i + c] * old / om;
+    s->admm_omega[i] = om;
+  }
+  /* preconditioner diagonal */
+  for (int64_t k = 0; k < ns; ++k) s->admm_diag[k] = s->nodes[k].idx >= 0 ? s->nodes[k].m : 0.0;
+  PF_FOR_PARTICLES(s, i)
+  {
+    double Fn[3][3];
+    mat_from_cm(s->F + 9 * (int64_t)i, Fn);
+    double cp = rho * s->admm_omega[i] * s->admm_omega[i] * dt * dt;
+    zro_spline sp;
+    spline_of(s, s->X + 3 * i, &sp);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3], fg[3];
+          spline_node(s, &sp, a, b, c, gw);
+          int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+          if (s->nodes[k].idx < 0) continue;
+          for (int d = 0; d < 3; ++d) fg[d] = Fn[0][d] * gw[0] + Fn[1][d] * gw[1] + Fn[2][d] * gw[2];
+          s->admm_diag[k] += cp * (fg[0] * fg[0] + fg[1] * fg[1] + fg[2] * fg[2]);
+        }
+  }
+  double* rhs = (double*)calloc((size_t)(3 * ns), sizeof(double));
+  for (int tim = 0; tim < o->admm_iterations; ++tim) {
+    admm_update_collisions(s);
+    /* FStep (:507-636) */
+    for (int64_t k = 0; k < 3 * ns; ++k) nv[k] = s->admm_vn[k] + s->admm_dv[k];
+    for (int64_t i = 0; i < n; ++i) {
+      double G[3][3], Fn[3][3], FD[3][3], Fa[3][3];
+      mat_from_cm(s->F + 9 * i, Fn);
+      mat_from_cm(s->admm_F + 9 * i, Fa);
+      admm_grad(s, i, nv, G);
+      double om = s->admm_omega[i];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          FD[r][c] = Fn[r][c] - s->admm_y[9 * i + c * 3 + r] / (om * rho) +
+                     dt * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
+      double fdc[9], U[9], sg[3], V[9];
+      mat_to_cm(FD, fdc);
+      zro_svd3(fdc, U, sg, V);
+      double Um[3][3], Vm[3][3], zi[3];
+      mat_from_cm(U, Um);
+      mat_from_cm(V, Vm);
+      for (int d = 0; d < 3; ++d) {
+        double acc = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) acc += Um[a][d] * Fa[a][b] * Vm[b][d];
+        zi[d] = acc;
+      }
+      double coeff = rho * om * om, vol = s->vol[i];
+      for (int j = 0; j < 5; ++j) {
+        double zp[3], D[3][3], dE[3], ddE[3][3], g[3], P[3][3], Pi[3][3], step[3];
+        admm_project_sigma_d(s, i, zi, zp, D);
+        hencky_dE(s->mu[i], s->lam[i], zp, dE);
+        hencky_ddE(s->mu[i], s->lam[i], zp, ddE);
+        for (int a = 0; a < 3; ++a) {
+          g[a] = vol * dE[a] + coeff * (zi[a] - sg[a]);
+          for (int b = 0; b < 3; ++b)
+            P[a][b] = vol * (ddE[a][0] * D[0][b] + ddE[a][1] * D[1][b] + ddE[a][2] * D[2][b]) + (a == b ? coeff : 0.0);
+        }
+        /* 3x3 inverse by cofactors */
+        double det = P[0][0] * (P[1][1] * P[2][2] - P[2][1] * P[1][2]) - P[0][1] * (P[1][0] * P[2][2] - P[2][0] * P[1][2]) +
+                     P[0][2] * (P[1][0] * P[2][1] - P[2][0] * P[1][1]);
+        Pi[0][0] = (P[1][1] * P[2][2] - P[1][2] * P[2][1]) / det;
+        Pi[0][1] = (P[0][2] * P[2][1] - P[0][1] * P[2][2]) / det;
+        Pi[0][2] = (P[0][1] * P[1][2] - P[0][2] * P[1][1]) / det;
+        Pi[1][0] = (P[1][2] * P[2][0] - P[1][0] * P[2][2]) / det;
+        Pi[1][1] = (P[0][0] * P[2][2] - P[0][2] * P[2][0]) / det;
+        Pi[1][2] = (P[0][2] * P[1][0] - P[0][0] * P[1][2]) / det;
+        Pi[2][0] = (P[1][0] * P[2][1] - P[1][1] * P[2][0]) / det;
+        Pi[2][1] = (P[0][1] * P[2][0] - P[0][0] * P[2][1]) / det;
+        Pi[2][2] = (P[0][0] * P[1][1] - P[0][1] * P[1][0]) / det;
+        double sn = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          step[a] = -(Pi[a][0] * g[0] + Pi[a][1] * g[1] + Pi[a][2] * g[2]);
+          sn += step[a] * step[a];
+        }
+        for (int a = 0; a < 3; ++a) zi[a] += step[a];
+        if (sqrt(sn) < o->local_tolerance) break;
+      }
+      double Fo[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Fo[r][c] = Um[r][0] * zi[0] * Vm[c][0] + Um[r][1] * zi[1] * Vm[c][1] + Um[r][2] * zi[2] * Vm[c][2];
+      mat_to_cm(Fo, s->admm_F + 9 * i);
+    }
+    /* dvStep (:163-220) */
+    memset(rhs, 0, sizeof(double) * 3 * ns);
+    for (int64_t k = 0; k < ns; ++k)
+      if (s->nodes[k].idx >= 0)
+        for (int d = 0; d < 3; ++d) rhs[3 * k + d] = dt * s->nodes[k].m * s->gravity[d];
+    PF_FOR_PARTICLES(s, i)
+    {
+      double G[3][3], Fn[3][3], bF[3][3];
+      mat_from_cm(s->F + 9 * (int64_t)i, Fn);
+      admm_grad(s, i, s->admm_vn, G);
+      double om = s->admm_omega[i];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double fbu = s->admm_F[9 * (int64_t)i + c * 3 + r] - Fn[r][c] + s->admm_y[9 * (int64_t)i + c * 3 + r] / (om * rho) -
+                       dt * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
+          bF[r][c] = dt * rho * om * om * fbu;
+        }
+      admm_scatter(s, i, bF, Fn, rhs);
+    }
+    double rel = 0.5;  /* min(0.5, sqrt(max(residual = 1, 1e-6))) (:785-787) */
+    int its = admm_minres(s, dt, rho, s->admm_dv, rhs, o->global_tolerance, rel, o->minres_max_iterations);
+    if (st) st->minres_iterations += its;
+    /* uStep (:638-671) */
+    for (int64_t k = 0; k < 3 * ns; ++k) nv[k] = s->admm_vn[k] + s->admm_dv[k];
+    for (int64_t i = 0; i < n; ++i) {
+      double G[3][3], Fn[3][3];
+      mat_from_cm(s->F + 9 * i, Fn);
+      admm_grad(s, i, nv, G);
+      double om = s->admm_omega[i];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double t = s->admm_F[9 * i + c * 3 + r] - Fn[r][c] - dt * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
+          s->admm_y[9 * i + c * 3 + r] += rho * om * t;
+        }
+    }
+  }
+  /* constructNewVelocityFromNewtonResult and last_dv (:789-800) */
+  for (int64_t k = 0; k < ns; ++k)
+    for (int d = 0; d < 3; ++d)
+      s->nodes[k].new_v[d] = s->nodes[k].idx >= 0 ? s->admm_vn[3 * k + d] + s->admm_dv[3 * k + d] : 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    zro_spline sp;
+    spline_of(s, s->X + 3 * i, &sp);
+    double acc[3] = { 0, 0, 0 }, ws = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          double w = spline_node(s, &sp, a, b, c, gw);
+          int64_t k = node_slot(s, sp.base[0] + a, sp.base[1] + b, sp.base[2] + c);
+          if (s->nodes[k].idx < 0) continue;
+          for (int d = 0; d < 3; ++d) acc[d] += s->admm_dv[3 * k + d] * w;
+          ws += w;
+        }
+    for (int d = 0; d < 3; ++d) s->admm_last_dv[3 * i + d] = acc[d] / (ws * dt);
+  }
+  free(rhs);
+  free(nv);
+  s->admm_steps++;
+  int flags = zro_g2p_apic(s, dt);
+  s->time += dt;
+  return flags;
+}
+
+void zro_get_admm(const zro_sim* s, double* F, double* y, double* omega)
+{
+  if (!s->admm_F) return;
+  if (F) memcpy(F, s->admm_F, sizeof(double) * 9 * (size_t)s->n);
+  if (y) memcpy(y, s->admm_y, sizeof(double) * 9 * (size_t)s->n);
+  if (omega) memcpy(omega, s->admm_omega, sizeof(double) * (size_t)s->n);
 }

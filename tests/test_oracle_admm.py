@@ -1,0 +1,90 @@
+"""Pins the oracle's ADMM step (Projects/admm/AdmmSimulationFoam.h:163-801)
+on CPU by what the method converges to: with enough ADMM iterations the grid
+velocity solves the backward-Euler momentum balance of the StvkWithHencky
+elastic energy,
+    m_i (v_i - vn_i) - dt m_i g + dt sum_p V0_p P(F_p(v)) Fn_p^T grad w_ip = 0,
+    F_p(v) = (I + dt sum_j v_j grad w_jp^T) Fn_p,
+evaluated here with numpy from independently computed B-spline weights and
+the FD-pinned Hencky first Piola stress; the balance residual falls as the
+iteration count grows.  Also: the symmetric 3x3 eigenvalues used for omega
+agree with numpy.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ziran2019_amd import scenes
+
+
+def _bspline(x, dx):
+    y = x / dx
+    base = int(np.floor(y - 0.5))
+    d = y - base
+    w = np.array([0.5 * (1.5 - d) ** 2, 0.75 - (d - 1.0) ** 2, 0.5 * (d - 0.5) ** 2])
+    dw = np.array([d - 1.5, -2.0 * (d - 1.0), d - 0.5]) / dx
+    return base, w, dw
+
+
+def test_sym3_eigenvalues():
+    rng = np.random.default_rng(2)
+    for _ in range(30):
+        A = rng.normal(size=(3, 3))
+        A = A + A.T
+        assert np.allclose(oracle.sym3_eigenvalues(A), np.linalg.eigvalsh(A), atol=1e-13 * np.abs(A).max())
+    assert np.allclose(oracle.sym3_eigenvalues(np.diag([2.0, 2.0, 2.0])), [2.0, 2.0, 2.0])
+
+
+def _scene(seed=4):
+    s = scenes.cube_scene_small(seed=seed, ncells=3, E=1e4)
+    n = len(s["mass"])
+    rng = np.random.default_rng(seed)
+    F = np.tile(np.eye(3), (n, 1, 1)) + 0.04 * rng.normal(size=(n, 3, 3))
+    s["F"] = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(n, 9))
+    mu, lam = oracle.lame(1e4, 0.3)
+    s["mu"], s["lam"] = np.full(n, mu), np.full(n, lam)
+    return s
+
+
+def _balance_residual(s, iterations, dt=1e-3):
+    n = len(s["mass"])
+    sim = oracle.OracleSim(dx=s["dx"], gravity=(0.0, -9.8, 0.0), nthreads=4)
+    sim.set_particles(s["X"], s["V"], s["C"], s["F"], s["mass"], s["vol"], s["mu"], s["lam"])
+    sim.set_materials(model=1)
+    sim.step_admm(dt, oracle.admm_opts(admm_iterations=iterations, global_tolerance=1e-14))
+    g = sim.grid()
+    info = sim.sort_result()
+    coords = {}
+    for p, base in enumerate(info["pages"]):
+        for e in range(32):
+            coords[tuple(oracle.linear_to_coord(int(base) + (e << 7)))] = p * 32 + e
+    active = g["idx"] >= 0
+    v, vn, m = g["new_v"], g["v"], g["m"]
+    Fn = s["F"].reshape(n, 3, 3).transpose(0, 2, 1)
+    force = np.zeros((len(m), 3))
+    for p in range(n):
+        bw = [_bspline(s["X"][p, d], s["dx"]) for d in range(3)]
+        ks, gs = [], []
+        for a in range(3):
+            for b in range(3):
+                for c in range(3):
+                    k = coords[(bw[0][0] + a, bw[1][0] + b, bw[2][0] + c)]
+                    if active[k]:
+                        ks.append(k)
+                        gs.append(np.array([bw[0][2][a] * bw[1][1][b] * bw[2][1][c],
+                                            bw[0][1][a] * bw[1][2][b] * bw[2][1][c],
+                                            bw[0][1][a] * bw[1][1][b] * bw[2][2][c]]))
+        G = sum(np.outer(v[k], gw) for k, gw in zip(ks, gs))
+        Fp = (np.eye(3) + dt * G) @ Fn[p]
+        P = oracle.stvk_hencky_first_piola(s["mu"][p], s["lam"][p], Fp)
+        S = s["vol"][p] * P @ Fn[p].T
+        for k, gw in zip(ks, gs):
+            force[k] += dt * S @ gw
+    r = m[:, None] * (v - vn) - dt * m[:, None] * np.array([0.0, -9.8, 0.0]) + force
+    return np.abs(r[active]).max() / np.abs(force[active]).max()
+
+
+def test_admm_converges_to_backward_euler_balance():
+    s = _scene()
+    res = [_balance_residual(s, k) for k in (5, 40, 300)]
+    assert res[0] > res[1] > res[2]
+    assert res[2] < 1e-10
