@@ -157,6 +157,22 @@ typedef struct zr_particles_adjoint {
   double* lambda;  /* n  */
 } zr_particles_adjoint;
 
+/* AdmmSimulation options (Projects/admm/AdmmSimulationFoam.h:55-68):
+   admm_max_iterations (reference default 100), Minres(10000), rho_scale 0.5,
+   global tolerance 1e-6, local Newton tolerance 1e-8. */
+typedef struct zr_admm_opts {
+  int32_t admm_iterations;
+  int32_t minres_max_iterations;
+  double rho_scale;
+  double global_tolerance;
+  double local_tolerance;
+} zr_admm_opts;
+
+typedef struct zr_admm_stats {
+  int32_t minres_iterations;   /* total over the ADMM iterations of the step */
+  int32_t reserved;
+} zr_admm_stats;
+
 typedef struct zr_step_stats {
   int64_t num_groups;   /* particle pages (particle_group.size()) */
   int64_t num_pages;    /* touched SPGrid pages (page_map->Get_Blocks().second) */
@@ -262,6 +278,15 @@ int zr_advance_one_time_step_implicit(zr_engine* e, double dt, const zr_implicit
    and the unprojected operator b = A x (BackwardEuler.h:322-351). */
 int zr_implicit_residual(zr_engine* e, const double* dv, double* r);
 int zr_implicit_multiply(zr_engine* e, const double* x, double* b);
+
+/* AdmmSimulation::advanceOneTimeStep (AdmmSimulationFoam.h:803-818): APIC
+   P2G, admmStep (:766-801), APIC G2P with the F update and the return map.
+   Every particle must be StvkWithHencky (zr_set_materials) with no, von Mises
+   or Drucker-Prager plasticity (no plasticity = DummyPlasticity). */
+int zr_advance_one_time_step_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* ast,
+                                  zr_step_stats* st);
+/* the ADMM variables after the last step, host layout (F, y: 9n; omega: n) */
+int zr_get_admm_state(zr_engine* e, double* F, double* y, double* omega);
 
 /* Differentiable form of the explicit MLS step.  No counterpart in the
    reference; for gradient-based callers (parameter estimation, control) that

@@ -1,0 +1,62 @@
+"""GPU parity of the ADMM step (AdmmSimulationFoam.h:163-818) against the FP64
+oracle: StvkWithHencky particles with no / von Mises / Drucker-Prager
+plasticity on a sticky ground, several steps (warm starts, dual rescaling);
+ADMM variables, particle state and MINRES iteration counts."""
+import numpy as np
+import pytest
+
+import oracle
+from ziran2019_amd.engine import (ZR_MODEL_STVK_HENCKY, ZR_PLASTIC_DRUCKER_PRAGER, ZR_PLASTIC_VON_MISES,
+                                  admm_opts, drucker_prager_alpha)
+from _helpers import make_pair, perturbed, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _materials(n, rng, kind):
+    if kind == "elastic":
+        return dict(model=ZR_MODEL_STVK_HENCKY, plasticity=0, plastic_param=(0, 0, 0, 0), plastic_state=0.0)
+    if kind == "vm":
+        return dict(model=ZR_MODEL_STVK_HENCKY, plasticity=ZR_PLASTIC_VON_MISES, plastic_param=(0.0, 1e30, 0, 0),
+                    plastic_state=200.0)
+    if kind == "dp":
+        return dict(model=ZR_MODEL_STVK_HENCKY, plasticity=ZR_PLASTIC_DRUCKER_PRAGER,
+                    plastic_param=(drucker_prager_alpha(30.0), 1.0, 0.0, 1.0), plastic_state=0.0)
+    plastic = rng.integers(0, 3, size=n).astype(np.int32)
+    pp = np.zeros((n, 4))
+    pp[plastic == ZR_PLASTIC_VON_MISES] = (0.0, 1e30, 0.0, 0.0)
+    pp[plastic == ZR_PLASTIC_DRUCKER_PRAGER] = (drucker_prager_alpha(30.0), 1.0, 0.0, 1.0)
+    st = np.where(plastic == ZR_PLASTIC_VON_MISES, 200.0, 0.0)
+    return dict(model=ZR_MODEL_STVK_HENCKY, plasticity=plastic, plastic_param=pp, plastic_state=st)
+
+
+@pytest.mark.parametrize("kind", ["elastic", "vm", "dp", "mixed"])
+def test_admm_steps_parity(kind):
+    s = perturbed(41, n_cells=5, shuffle=False, F_noise=0.03, C_noise=0.1, E=1e5)
+    s["V"][:, 1] -= 0.5
+    n = len(s["mass"])
+    y0 = float(s["X"][:, 1].min() + 0.5 * s["dx"])
+    cols = [dict(shape=0, type=1, origin=(0.0, y0, 0.0), normal=(0.0, 1.0, 0.0))]
+    eng, ora = make_pair(s, colliders=cols)
+    m = _materials(n, np.random.default_rng(41), kind)
+    eng.set_materials(**m)
+    ora.set_materials(model=m["model"], plastic=m["plasticity"], pparam=m["plastic_param"], pstate=m["plastic_state"])
+    dt = 1e-3
+    og = admm_opts(admm_iterations=20)
+    oo = oracle.admm_opts(admm_iterations=20)
+    for step in range(3):
+        sg = eng.advanceOneTimeStepAdmm(dt, og)
+        _, so = ora.step_admm(dt, oo)
+        assert sg.minres_iterations == so.minres_iterations, step
+    a_g, a_o = eng.get_admm_state(), ora.get_admm()
+    assert rel_err(a_g["omega"], a_o["omega"]) < 1e-10
+    assert rel_err(a_g["F"], a_o["F"]) < 1e-9
+    assert rel_err(a_g["y"], a_o["y"]) < 1e-8
+    out = eng.download()
+    X, V, C, F = ora.get_particles()
+    assert rel_err(out["X"], X) < 1e-10
+    assert rel_err(out["V"], V) < 1e-8
+    assert rel_err(out["F"], F) < 1e-9
+    if kind != "elastic":
+        assert rel_err(eng.get_materials()["plastic_state"], ora.get_materials()["pstate"]) < 1e-8 or kind == "dp"
+    eng.close()

@@ -522,6 +522,7 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
 #include "zr_implicit.hpp"
 #include "zr_damage.hpp"
 #include "zr_adjoint.hpp"
+#include "zr_admm.hpp"
 
 // ---------------------------------------------------------------------------
 // G2P, MLS/APIC (MpmSimulationBase.cpp:906-1016) + evolveStrain
@@ -739,7 +740,7 @@ struct zr_engine {
   int32_t* mat = nullptr;
   double *pparam = nullptr, *pstate = nullptr, *gdeg = nullptr;
   int32_t* broken = nullptr;
-  bool has_models = false, has_plastic = false;
+  bool has_models = false, has_plastic = false, all_hencky = false;
   // phase field: per particle (c, H, l0, 1/sigma_c, pf_Fp, H_max, k, vol, allow)
   // and per node (phase, rhs, D, x, r, mr, s, q) of the damage solve
   bool pf_on = false;
@@ -761,6 +762,14 @@ struct zr_engine {
   size_t adj_grid_cap = 0;
   std::vector<double*> ckpt;
   std::vector<double> ckpt_time;
+  // ADMM: per particle F 9, y 9, omega 1, last dv 3 (= 22 rows of n); per
+  // node vn 3, dv 3, proj 9, diag 1, rhs 3 and MINRES q 3x3, z 3, m 3x3
+  double* admm_p = nullptr;
+  int64_t admm_n = 0;
+  int admm_steps = 0;
+  double* admm_node = nullptr;
+  int32_t* admm_coll = nullptr;
+  size_t admm_cap = 0;
   uint32_t *p_page = nullptr, *p_re = nullptr, *order = nullptr;
   uint64_t* keys = nullptr;
   uint32_t *group_page = nullptr, *group_count = nullptr, *group_start = nullptr;
@@ -898,6 +907,9 @@ void free_particles(zr_engine* e)
   e->pf_on = false;
   dfree(e->adj);
   e->adj_n = 0;
+  dfree(e->admm_p);
+  e->admm_n = 0;
+  e->admm_steps = 0;
   for (auto& c : e->ckpt) dfree(c);
   e->ckpt.clear();
   e->ckpt_time.clear();
@@ -1228,6 +1240,127 @@ int do_adjoint_step(zr_engine* e, double dt)
   return ZR_OK;
 }
 
+// ------------------------------------------------------------------- ADMM
+constexpr int kAdmmNodeRows = 3 + 3 + 9 + 1 + 3 + 9 + 3 + 9;  // vn dv proj diag rhs q(3) z m(3)
+
+int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
+{
+  hipStream_t s = e->stream;
+  const int64_t n = e->n;
+  const size_t N = (size_t)n;
+  int rc = ensure_scalars(e);
+  if (rc) return rc;
+  const bool first = e->admm_steps == 0 || e->admm_n != n;
+  if (e->admm_n != n) {
+    dfree(e->admm_p);
+    ZR_HIP(e, hipMalloc(&e->admm_p, 22 * N * sizeof(double)));
+    ZR_HIP(e, hipMemsetAsync(e->admm_p, 0, 22 * N * sizeof(double), s));
+    e->admm_n = n;
+    e->admm_steps = 0;
+  }
+  if (e->admm_cap < e->grid_cap) {
+    dfree(e->admm_node);
+    dfree(e->admm_coll);
+    ZR_HIP(e, hipMalloc(&e->admm_node, kAdmmNodeRows * e->grid_cap * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->admm_coll, e->grid_cap * sizeof(int32_t)));
+    e->admm_cap = e->grid_cap;
+  }
+  size_t nb = (e->grid_cap + 255) / 256 + 1;
+  if (nb > e->partial_cap) {
+    dfree(e->partial);
+    ZR_HIP(e, hipMalloc(&e->partial, nb * sizeof(double)));
+    e->partial_cap = nb;
+  }
+  const int64_t C = (int64_t)e->admm_cap;
+  double* base = e->admm_node;
+  AdmmNodes Nd{ base, base + 3 * C, base + 6 * C, e->admm_coll, base + 15 * C, C };
+  double* rhs = base + 16 * C;
+  double* qb[3] = { base + 19 * C, base + 22 * C, base + 25 * C };
+  double* zb = base + 28 * C;
+  double* mb[3] = { base + 31 * C, base + 34 * C, base + 37 * C };
+  double* pa = e->admm_p;
+  AdmmParticles Ap{ pa, pa + 9 * N, pa + 18 * N, pa + 19 * N };
+  const uint32_t nn = e->S * 32, nbn = blocks_for(nn, 256);
+  const double dx = e->cfg.dx, inv_dx = 1.0 / dx, rho = o->rho_scale;
+  ColliderSet cs = collider_set(e);
+  ParticlesView P = pview(e);
+  const int32_t* mat = e->mat;
+  // initStep
+  if (!first)
+    k_admm_lastdv_p2g<<<e->A, 256, 0, s>>>(P, Ap, e->order, e->group_start, e->group_page, e->stage, inv_dx);
+  k_admm_init_nodes<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, e->gv, Nd, first ? 0 : 1, dt);
+  k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, e->time, cs);
+  k_admm_init_particles<<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
+                                              e->group_page, e->group_nslot, e->gm, e->stage, first ? 1 : 0, dt, rho,
+                                              inv_dx);
+  k_admm_diag<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd.diag);
+  ZR_LAUNCH_CHECK(e);
+  int total_minres = 0;
+  for (int tim = 0; tim < o->admm_iterations; ++tim) {
+    k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, e->time, cs);
+    k_admm_fstep<<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
+                                      e->group_page, e->group_nslot, e->gm, dt, rho, o->local_tolerance, inv_dx);
+    k_admm_scatter<0><<<e->A, 256, 0, s>>>(P, Ap, Nd, nullptr, e->order, e->group_start, e->group_page,
+                                           e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+    k_admm_rhs<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, rhs, C, dt, e->cfg.gravity[0],
+                                   e->cfg.gravity[1], e->cfg.gravity[2]);
+    // MINRES on the global system, warm-started from dv (Minres.h:69-149),
+    // relative tolerance min(0.5, sqrt(max(1, 1e-6))) = 0.5 (:785-787)
+    k_admm_reset_flags<<<1, 64, 0, s>>>(e->ci);
+    k_admm_scatter<1><<<e->A, 256, 0, s>>>(P, Ap, Nd, Nd.dv, e->order, e->group_start, e->group_page,
+                                           e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+    double *q = qb[2], *qk = qb[1], *qkm1 = qb[0], *z = zb, *mk = mb[2], *mkm1 = mb[1], *mkm2 = mb[0];
+    k_minres_init<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, rhs, Nd.dv, q, z, e->partial);
+    k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_INIT, o->global_tolerance, 0.5, 0,
+                                      o->minres_max_iterations);
+    ZR_LAUNCH_CHECK(e);
+    int next_poll = 1;
+    for (int k = 0; k < o->minres_max_iterations; ++k) {
+      k_minres_begin<<<1, 64, 0, s>>>(e->sc, e->ci);
+      // m_{k-2} <- m_{k-1} <- m_k <- z; q_{k-1} <- q_k <- q_{k+1}
+      double* t = mkm2;
+      mkm2 = mkm1;
+      mkm1 = mk;
+      mk = z;
+      z = t;
+      t = qkm1;
+      qkm1 = qk;
+      qk = q;
+      q = t;
+      k_admm_scatter<1><<<e->A, 256, 0, s>>>(P, Ap, Nd, mk, e->order, e->group_start, e->group_page,
+                                             e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+      k_minres_apply<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, mk, q, e->ci, e->partial);
+      k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_ALPHA, 0.0, 0.0, k,
+                                        o->minres_max_iterations);
+      k_minres_lanczos<<<nbn, 256, 0, s>>>(nn, e->gm, Nd, q, qk, qkm1, z, e->sc, e->ci, e->partial);
+      k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_GIVENS, 0.0, 0.0, k,
+                                        o->minres_max_iterations);
+      k_minres_update<<<nbn, 256, 0, s>>>(nn, e->gm, Nd, q, z, mk, mkm1, mkm2, Nd.dv, e->sc, e->ci);
+      ZR_LAUNCH_CHECK(e);
+      if (k + 1 == next_poll || k + 1 == o->minres_max_iterations) {
+        next_poll *= 2;
+        ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        ZR_HIP(e, hipStreamSynchronize(s));
+        if (e->hci[AI_DONE]) break;
+      }
+    }
+    ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    ZR_HIP(e, hipStreamSynchronize(s));
+    total_minres += e->hci[AI_ITERS];
+    k_admm_ustep<<<e->A, 256, 0, s>>>(P, Ap, Nd, e->order, e->group_start, e->group_page, e->group_nslot, e->gm, dt,
+                                      rho, inv_dx);
+    ZR_LAUNCH_CHECK(e);
+  }
+  k_admm_finish_nodes<<<nbn, 256, 0, s>>>(nn, e->gm, Nd, e->gvs);
+  k_admm_lastdv<<<e->A, 256, 0, s>>>(P, Ap, Nd, e->order, e->group_start, e->group_page, e->group_nslot, e->gm, dt,
+                                     inv_dx);
+  ZR_LAUNCH_CHECK(e);
+  e->admm_steps++;
+  e->updated = true;
+  if (st) st->minres_iterations = total_minres;
+  return ZR_OK;
+}
+
 // ---------------------------------------------------------------- implicit
 int ensure_implicit(zr_engine* e, int32_t prec)
 {
@@ -1513,7 +1646,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
-  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid);
+  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);
   if (e->hci) (void)hipHostFree(e->hci);
@@ -1689,6 +1822,8 @@ int zr_set_materials(zr_engine* e, const zr_materials* mt)
   ZR_HIP(e, hipStreamSynchronize(e->stream));
   e->has_models = models;
   e->has_plastic = plastic;
+  e->all_hencky = true;
+  for (size_t i = 0; i < N; ++i) e->all_hencky = e->all_hencky && (code[i] & 15) == ZR_MODEL_STVK_HENCKY;
   return ZR_OK;
 }
 
@@ -1907,6 +2042,57 @@ int zr_checkpoint_restore(zr_engine* e, int32_t slot)
   ZR_HIP(e, hipStreamSynchronize(e->stream));
   e->time = e->ckpt_time[slot];
   e->sorted = e->rasterised = e->updated = e->be_ready = false;
+  return ZR_OK;
+}
+
+int zr_advance_one_time_step_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* ast,
+                                  zr_step_stats* st)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!(dt > 0.0) || !o || o->admm_iterations < 0 || o->minres_max_iterations < 0 || !(o->rho_scale > 0.0))
+    return fail(e, ZR_ERR_ARG, "bad dt or ADMM options");
+  if (e->n > 0 && (!e->mat || !e->all_hencky || e->pf_on))
+    return fail(e, ZR_ERR_ARG, "the ADMM step needs StvkWithHencky particles (zr_set_materials)");
+  if (ast) ast->minres_iterations = 0;
+  int rc = do_sort(e, st);
+  if (rc) return rc;
+  rc = do_p2g(e, false);
+  if (rc) return rc;
+  rc = do_grid(e, GRID_COMBINE, 0.0);
+  if (rc) return rc;
+  rc = do_admm(e, dt, o, ast);
+  if (rc) return rc;
+  rc = do_g2p(e, dt, st, false);
+  if (rc < 0) return rc;
+  e->time += dt;
+  return rc;
+}
+
+int zr_get_admm_state(zr_engine* e, double* F, double* y, double* omega)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!e->admm_p || e->admm_n != e->n) return fail(e, ZR_ERR_STATE, "no ADMM step has run");
+  const size_t N = (size_t)e->n;
+  double* tmp = nullptr;
+  ZR_HIP(e, hipMalloc(&tmp, 9 * N * sizeof(double)));
+  struct Item {
+    double* dst;
+    const double* src;
+    int nc;
+  } items[] = { { F, e->admm_p, 9 }, { y, e->admm_p + 9 * N, 9 }, { omega, e->admm_p + 18 * N, 1 } };
+  for (auto& it : items) {
+    if (!it.dst) continue;
+    k_soa_to_aos<<<blocks_for((uint64_t)N * it.nc, 256), 256, 0, e->stream>>>(it.src, tmp, (uint32_t)N, it.nc, e->n);
+    hipError_t stt = hipGetLastError();
+    if (stt == hipSuccess)
+      stt = hipMemcpyAsync(it.dst, tmp, it.nc * N * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (stt == hipSuccess) stt = hipStreamSynchronize(e->stream);
+    if (stt != hipSuccess) {
+      (void)hipFree(tmp);
+      return fail(e, ZR_ERR_HIP, std::string("admm download: ") + hipGetErrorString(stt));
+    }
+  }
+  ZR_HIP(e, hipFree(tmp));
   return ZR_OK;
 }
 

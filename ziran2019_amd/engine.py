@@ -45,7 +45,7 @@ EXPORTS = (
     "zr_advance_one_time_step_implicit", "zr_implicit_residual", "zr_implicit_multiply",
     "zr_set_materials", "zr_get_materials", "zr_set_phase_field", "zr_get_phase_field", "zr_phase_field_solve",
     "zr_phase_field_last_stats", "zr_adjoint_set", "zr_adjoint_get", "zr_adjoint_step", "zr_checkpoint_save",
-    "zr_checkpoint_restore",
+    "zr_checkpoint_restore", "zr_advance_one_time_step_admm", "zr_get_admm_state",
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
 ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER = 0, 1, 2
@@ -155,6 +155,23 @@ class ZrParticlesAdjoint(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("X", _dp), ("V", _dp), ("C", _dp), ("F", _dp), ("mu", _dp), ("lam", _dp)]
 
 
+class ZrAdmmOpts(ctypes.Structure):
+    _fields_ = [("admm_iterations", ctypes.c_int32), ("minres_max_iterations", ctypes.c_int32),
+                ("rho_scale", ctypes.c_double), ("global_tolerance", ctypes.c_double),
+                ("local_tolerance", ctypes.c_double)]
+
+
+class ZrAdmmStats(ctypes.Structure):
+    _fields_ = [("minres_iterations", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def admm_opts(admm_iterations=100, minres_max_iterations=10000, rho_scale=0.5, global_tolerance=1e-6,
+              local_tolerance=1e-8):
+    """AdmmSimulation defaults (AdmmSimulationFoam.h:55-68)."""
+    return ZrAdmmOpts(int(admm_iterations), int(minres_max_iterations), float(rho_scale), float(global_tolerance),
+                      float(local_tolerance))
+
+
 class ZrStepStats(ctypes.Structure):
     _fields_ = [("num_groups", ctypes.c_int64), ("num_pages", ctypes.c_int64), ("cfl_flags", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -206,6 +223,9 @@ def load_library(path=LIB_PATH):
     L.zr_phase_field_solve.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrPhaseFieldStats)]
     L.zr_phase_field_last_stats.argtypes = [vp, ctypes.POINTER(ZrPhaseFieldStats)]
     L.zr_get_materials.argtypes = [vp, ctypes.POINTER(ZrMaterials)]
+    L.zr_advance_one_time_step_admm.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrAdmmOpts),
+                                                ctypes.POINTER(ZrAdmmStats), ctypes.POINTER(ZrStepStats)]
+    L.zr_get_admm_state.argtypes = [vp, vp, vp, vp]
     L.zr_adjoint_set.argtypes = [vp, ctypes.POINTER(ZrParticlesAdjoint)]
     L.zr_adjoint_get.argtypes = [vp, ctypes.POINTER(ZrParticlesAdjoint)]
     L.zr_adjoint_step.argtypes = [vp, ctypes.c_double]
@@ -346,6 +366,22 @@ class MpmEngine:
         st = ZrPhaseFieldStats()
         self._check(self.L.zr_phase_field_last_stats(self.h, ctypes.byref(st)), "phase_field_last_stats")
         return st
+
+    # --------------------------------------------------------------- ADMM
+    def advanceOneTimeStepAdmm(self, dt, opts=None):
+        st = ZrAdmmStats()
+        o = opts if opts is not None else admm_opts()
+        self._check(self.L.zr_advance_one_time_step_admm(self.h, float(dt), ctypes.byref(o), ctypes.byref(st),
+                                                         ctypes.byref(self.stats)), "advanceOneTimeStep (ADMM)")
+        return st
+
+    def get_admm_state(self):
+        n = self.n
+        F, y, om = np.zeros((n, 9)), np.zeros((n, 9)), np.zeros(n)
+        self._check(self.L.zr_get_admm_state(self.h, F.ctypes.data_as(ctypes.c_void_p),
+                                             y.ctypes.data_as(ctypes.c_void_p),
+                                             om.ctypes.data_as(ctypes.c_void_p)), "get_admm_state")
+        return dict(F=F, y=y, omega=om)
 
     # ------------------------------------------------- differentiable form
     def adjoint_set(self, X=None, V=None, C=None, F=None, mu=None, lam=None):
