@@ -52,6 +52,16 @@ def algorithmic_bytes(kernel, n, g):
     raise ValueError(kernel)
 
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE (x 2.0, from a
+# 256 MiB copy) and WRITE_SIZE (x 1.0) on one MI355X for these workloads
+# (profiles/hbm_traffic_counters.md); None where no counter run exists
+TRAFFIC = {
+    ("implicit-boxes", "hvp"): 190.0e6,
+    ("implicit-boxes", "hvp64"): 330.3e6,
+    ("explicit-cube", "p2g"): 42.8e6,
+    ("explicit-cube", "g2p"): 48.9e6,
+}
+
 IMPLICIT_OPTS = dict(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=500, cg_tolerance=1e30,
                      cg_relative_tolerance=1e-6)
 
@@ -145,7 +155,9 @@ def _implicit_run(scene, opts, steps, warmup, profile):
     for _ in range(warmup):
         eng.advanceOneTimeStepImplicit(2e-3, opts)
     if profile:
-        eng.profile(True)
+        # every 8th launch of each kernel class is timed so that the events
+        # do not perturb the timed step (one launch in 8 gets a marker pair)
+        eng.profile(True, every=8)
     newton = cg = 0
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -168,7 +180,7 @@ def run_implicit(args):
     dt = 2e-3
     n = len(scene["mass"])
     # plain FP64 path, then the FP32-bulk / FP64-refined form from the same state
-    eng64, r64 = _implicit_run(scene, implicit_opts(**IMPLICIT_OPTS), args.steps, args.warmup, profile=False)
+    eng64, r64 = _implicit_run(scene, implicit_opts(**IMPLICIT_OPTS), args.steps, args.warmup, profile=True)
     eng64.close()
     eng, r32 = _implicit_run(scene, implicit_opts(precision=1, **IMPLICIT_OPTS), args.steps, args.warmup,
                              profile=True)
@@ -191,7 +203,9 @@ def run_implicit(args):
     fp64_value = n * args.steps / r64["elapsed"]
     extra = dict(newton_iterations_per_step=r32["newton"], cg_iterations_per_step=r32["cg"],
                  fp64_plain=dict(value=fp64_value, ms_per_step=r64["elapsed"] / args.steps * 1e3,
-                                 newton_iterations_per_step=r64["newton"], cg_iterations_per_step=r64["cg"]),
+                                 newton_iterations_per_step=r64["newton"], cg_iterations_per_step=r64["cg"],
+                                 kernels={k: dict(avg_us=round(v[0] * 1e3 / max(v[1], 1), 2), launches=v[1])
+                                          for k, v in r64["kt"].items()}),
                  speedup_vs_fp64=(n * args.steps / r32["elapsed"]) / fp64_value,
                  fp32_vs_fp64_error=err)
     return dict(n=n, dt=dt, dx=scene["dx"], elapsed=r32["elapsed"], kt=r32["kt"], g_nodes=g_nodes,
@@ -245,7 +259,7 @@ def main():
         kern[k] = dict(avg_us=avg_s * 1e6, bytes=b, gbs=b / avg_s / 1e9)
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * kt[k][1])
     roof = dict(bound="hbm", kernel=dom, achieved=round(kern[dom]["gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(kern[dom]["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(kern[dom]["gbs"] / HBM_PEAK_GBS, 4), traffic=TRAFFIC.get((args.workload, dom)),
                 bytes_per_launch=kern[dom]["bytes"], avg_launch_us=round(kern[dom]["avg_us"], 2))
     kernels = {k: dict(avg_us=round(v[0] * 1e3 / max(v[1], 1), 2), launches=v[1], total_ms=round(v[0], 3))
                for k, v in kt.items()}

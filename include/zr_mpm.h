@@ -327,7 +327,10 @@ int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes);
 /* Kernel timing with HIP events recorded on the engine's stream around each
    launch.  Instrumentation for measurement; it stands in for the reference's
    ZIRAN_TIMER scoped timers (Lib/Ziran/CS/Util/Timer.cpp:34-95), which time
-   host code.  Enabling resets the accumulated times. */
+   host code.  on = 0 turns timing off; on = k > 0 times every k-th launch of
+   each kernel class (k = 1: all), so that sampled timing barely perturbs the
+   step.  Enabling resets the accumulated times; zr_profile_read reports the
+   timed launches only. */
 enum {
   ZR_PROF_P2G = 0,
   ZR_PROF_GRID = 1,

@@ -729,12 +729,13 @@ __global__ __launch_bounds__(256) void k_admm_rhs(uint32_t nn, const int32_t* __
   for (int d = 0; d < 3; ++d) rhs[d * S + k] = f[d];
 }
 
-// MINRES start (Minres.h:75-101): q = P(b - A x), z = q / diag, partial z.q
+// MINRES start (Minres.h:40-101): q = P(b - A x), z = q / diag, partial z.q;
+// m_k, m_{k-1} and q_k start at zero (Minres::reinitialize)
 __global__ __launch_bounds__(256) void k_minres_init(uint32_t nn, const int32_t* __restrict__ slot_src,
                                                      const double* __restrict__ stage, const double* __restrict__ gm,
                                                      AdmmNodes N, const double* __restrict__ b,
                                                      const double* __restrict__ x, double* q, double* z,
-                                                     double* partial)
+                                                     double* zero0, double* zero1, double* zero2, double* partial)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   const int64_t S = N.stride;
@@ -757,6 +758,9 @@ __global__ __launch_bounds__(256) void k_minres_init(uint32_t nn, const int32_t*
     for (int d = 0; d < 3; ++d) {
       q[d * S + k] = r[d];
       z[d * S + k] = zz[d];
+      zero0[d * S + k] = 0.0;
+      zero1[d * S + k] = 0.0;
+      zero2[d * S + k] = 0.0;
     }
   }
   block_partial(acc, partial);
@@ -887,11 +891,12 @@ __global__ __launch_bounds__(256) void k_minres_reduce(const double* __restrict_
     double dk[2] = { ek[1], sc[AD_ALPHA] };
     g2_rot(gkm1_c, gkm1_s, dk);
     double tv[2] = { dk[1], beta_kp1 };
-    double c = 1.0, s = 0.0;
+    double c = 1.0, s = 0.0;  // GivensRotation::compute (Givens.h:56-62)
     const double d = sqrt(tv[0] * tv[0] + tv[1] * tv[1]);
     if (d) {
-      c = tv[0] / d;
-      s = -tv[1] / d;
+      const double t = 1.0 / d;
+      c = tv[0] * t;
+      s = -tv[1] * t;
     }
     g2_rot(c, s, tv);
     double lt[2] = { sc[AD_LT0], sc[AD_LT1] };

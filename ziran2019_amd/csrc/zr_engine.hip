@@ -820,6 +820,10 @@ struct zr_engine {
   bool prof = false;
   std::vector<hipEvent_t> prof_ev[ZR_PROF_COUNT];
   size_t prof_used[ZR_PROF_COUNT] = { 0, 0, 0, 0, 0, 0 };
+  uint64_t prof_seen[ZR_PROF_COUNT] = { 0, 0, 0, 0, 0, 0 };
+  bool prof_phase[ZR_PROF_COUNT] = { false, false, false, false, false, false };
+  bool prof_take[ZR_PROF_COUNT] = { false, false, false, false, false, false };
+  int32_t prof_period = 1;
 };
 
 namespace {
@@ -873,10 +877,16 @@ int run_scan(zr_engine* e, Ld ld, uint32_t n, uint32_t* out, uint32_t* total_dev
   return ZR_OK;
 }
 
-// records one event of a start/stop pair for kernel class k when timing is on
+// records one event of a start/stop pair for kernel class k when timing is
+// on; every prof_period-th launch of a class is timed (calls alternate start
+// and stop)
 int prof_mark(zr_engine* e, int k)
 {
   if (!e->prof) return ZR_OK;
+  const bool start = !e->prof_phase[k];
+  e->prof_phase[k] = start;
+  if (start) e->prof_take[k] = (e->prof_seen[k]++ % (uint64_t)e->prof_period) == 0;
+  if (!e->prof_take[k]) return ZR_OK;
   auto& ev = e->prof_ev[k];
   if (e->prof_used[k] == ev.size()) {
     hipEvent_t h;
@@ -1310,7 +1320,8 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
     k_admm_scatter<1><<<e->A, 256, 0, s>>>(P, Ap, Nd, Nd.dv, e->order, e->group_start, e->group_page,
                                            e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
     double *q = qb[2], *qk = qb[1], *qkm1 = qb[0], *z = zb, *mk = mb[2], *mkm1 = mb[1], *mkm2 = mb[0];
-    k_minres_init<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, rhs, Nd.dv, q, z, e->partial);
+    k_minres_init<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, rhs, Nd.dv, q, z, mk, mkm1, qk,
+                                      e->partial);
     k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_INIT, o->global_tolerance, 0.5, 0,
                                       o->minres_max_iterations);
     ZR_LAUNCH_CHECK(e);
@@ -2390,7 +2401,13 @@ int zr_profile_enable(zr_engine* e, int32_t on)
   if (!e) return ZR_ERR_ARG;
   ZR_HIP(e, hipStreamSynchronize(e->stream));
   e->prof = on != 0;
-  for (auto& u : e->prof_used) u = 0;
+  e->prof_period = on > 0 ? on : 1;
+  for (int k = 0; k < ZR_PROF_COUNT; ++k) {
+    e->prof_used[k] = 0;
+    e->prof_seen[k] = 0;
+    e->prof_phase[k] = false;
+    e->prof_take[k] = false;
+  }
   return ZR_OK;
 }
 

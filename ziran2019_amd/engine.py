@@ -481,8 +481,10 @@ class MpmEngine:
         return b.reshape(-1, 3)
 
     # ---------------------------------------------------------- timing
-    def profile(self, on=True):
-        self._check(self.L.zr_profile_enable(self.h, int(bool(on))), "zr_profile_enable")
+    def profile(self, on=True, every=1):
+        """Time kernel launches with HIP events; every k-th launch of each
+        kernel class when every = k."""
+        self._check(self.L.zr_profile_enable(self.h, int(every) if on else 0), "zr_profile_enable")
 
     def kernel_time(self, which):
         """(total milliseconds, launches) of a kernel class since profiling was enabled."""
