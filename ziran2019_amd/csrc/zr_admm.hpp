@@ -842,6 +842,20 @@ __global__ __launch_bounds__(256) void k_minres_update(uint32_t nn, const double
   }
 }
 
+// the start vectors divided by beta_1 = |r0|_P (Minres.h:95-98)
+__global__ __launch_bounds__(256) void k_minres_scale(uint32_t nn, const double* __restrict__ gm, int64_t S, double* q,
+                                                      double* z, const double* sc, const int32_t* ai)
+{
+  if (ai[AI_DONE]) return;
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  const double b = sc[AD_BETA_KP1];
+  if (k >= nn || gm[k] == 0.0 || !(b > 0.0)) return;
+  for (int d = 0; d < 3; ++d) {
+    q[d * S + k] /= b;
+    z[d * S + k] /= b;
+  }
+}
+
 __device__ __forceinline__ void g2_rot(double c, double s, double v[2])
 {
   const double t1 = v[0], t2 = v[1];

@@ -1324,6 +1324,7 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
                                       e->partial);
     k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_INIT, o->global_tolerance, 0.5, 0,
                                       o->minres_max_iterations);
+    k_minres_scale<<<nbn, 256, 0, s>>>(nn, e->gm, C, q, z, e->sc, e->ci);
     ZR_LAUNCH_CHECK(e);
     int next_poll = 1;
     for (int k = 0; k < o->minres_max_iterations; ++k) {

@@ -471,7 +471,7 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
   __shared__ int cbeg[kCells], cend[kCells];
   __shared__ int32_t nslot[8];
   const int tid = threadIdx.x;
-  const uint32_t g = blockIdx.x;
+  const uint32_t g = xcd_group(blockIdx.x, gridDim.x);
   for (int k = tid; k < 3 * kCells * kAB * 3; k += 256) (&contrib[0][0])[k] = (T)0;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;

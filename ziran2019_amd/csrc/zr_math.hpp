@@ -50,6 +50,17 @@ ZR_HD uint32_t elem_of(int x, int y, int z)
   return (uint32_t)(((x & 1) << 4) | ((y & 3) << 2) | (z & 3));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the device
+// (block b runs on XCD b mod 8).  Groups are numbered in Morton page order,
+// so neighbouring groups share tile nodes; this bijection gives each XCD a
+// contiguous range of groups so that the shared nodes stay in one L2.
+__device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t ngroups)
+{
+  const uint32_t x = b & 7u, l = b >> 3;
+  const uint32_t q = ngroups >> 3, r = ngroups & 7u;
+  return x * q + (x < r ? x : r) + l;
+}
+
 // MathTools.h:21-25
 ZR_HD int int_floor(double x)
 {
