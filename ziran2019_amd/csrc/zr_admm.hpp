@@ -872,8 +872,7 @@ __global__ __launch_bounds__(256) void k_minres_reduce(const double* __restrict_
 {
   __shared__ double smem[4];
   if (mode != MR_INIT && ai[AI_DONE]) return;
-  double acc = 0.0;
-  for (uint32_t b = threadIdx.x; b < nb; b += 256) acc += partial[b];
+  const double acc = thread_partials256(partial, nb);
   const double v = block_sum256(acc, smem);
   if (threadIdx.x != 0) return;
   if (mode == MR_INIT) {

@@ -308,11 +308,8 @@ __global__ __launch_bounds__(256) void k_pf_reduce(const double* __restrict__ pa
 {
   __shared__ double smem[4];
   if (mode != PFR_INIT && ci[CI_DONE]) return;
-  double a0 = 0.0, a1 = 0.0;
-  for (uint32_t b = threadIdx.x; b < nb; b += 256) {
-    a0 += partial[b];
-    if (mode != PFR_ALPHA) a1 += partial[nb + b];
-  }
+  const double a0 = thread_partials256(partial, nb);
+  const double a1 = mode != PFR_ALPHA ? thread_partials256(partial + nb, nb) : 0.0;
   const double s0 = block_sum256(a0, smem);
   __syncthreads();
   const double s1 = block_sum256(a1, smem);

@@ -69,6 +69,23 @@ __device__ __forceinline__ double block_sum256(double v, double* smem4)
   return s;
 }
 
+// this thread's share of an ordered sum of nb block partials (blockDim 256):
+// eight independent strided accumulators so that the loads are all in flight
+// at once, combined in a fixed order
+__device__ __forceinline__ double thread_partials256(const double* __restrict__ partial, uint32_t nb)
+{
+  double a[8] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+  uint32_t b = threadIdx.x;
+  for (; b + 7u * 256u < nb; b += 8u * 256u) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] += partial[b + 256u * j];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (b + 256u * j < nb) a[j] += partial[b + 256u * j];
+  return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 // block partial of a grid-wide sum; the partials are added in index order
 // by a one-block kernel (a device-scope fence per block would force L2
 // write-backs across the XCDs of the device)
@@ -433,27 +450,139 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
 constexpr int kCells = 32;    // 2 x 4 x 4 cells of a page
 constexpr int kAB = 9;        // (a, b) stencil offsets; c is handled inside a task
 
+// Node-major layout of the per-(cell, a, b, c) partial sums of the operator
+// apply: tile node t = (tx, ty, tz) owns n(t) = na(tx) nb(ty) nb(tz)
+// consecutive entries from off(t), one per stencil offset (a, b, c) whose
+// source cell t - (a, b, c) lies in the page, ordered by a, then b, then c.
+// Per axis: na = {1,2,2,1} (2 cells), nb = {1,2,3,3,2,1} (4 cells), with
+// prefix sums {0,1,3,5} and {0,1,3,6,9,11}; 864 entries in all.
+constexpr int hvp_na(int tx) { return (0x1221 >> (4 * tx)) & 15; }
+constexpr int hvp_nb(int ty) { return (0x123321 >> (4 * ty)) & 15; }
+constexpr int hvp_node_off(int tx, int ty, int tz)
+{
+  return ((0x5310 >> (4 * tx)) & 15) * 144 +
+         hvp_na(tx) * (((0xB96310 >> (4 * ty)) & 15) * 12 + hvp_nb(ty) * ((0xB96310 >> (4 * tz)) & 15));
+}
+constexpr int hvp_slot(int tx, int ty, int tz, int a, int b, int c)
+{
+  return hvp_node_off(tx, ty, tz) +
+         ((a - (tx > 1 ? tx - 1 : 0)) * hvp_nb(ty) + (b - (ty > 3 ? ty - 3 : 0))) * hvp_nb(tz) +
+         (c - (tz > 3 ? tz - 3 : 0));
+}
+
+// slots of task (cell e, a, b)'s three c offsets, 10 bits each
+struct HvpTaskSlots {
+  uint32_t v[kCells * kAB];
+};
+constexpr HvpTaskSlots hvp_task_slots()
+{
+  HvpTaskSlots t{};
+  for (int task = 0; task < kCells * kAB; ++task) {
+    const int e = task / kAB, a = (task % kAB) / 3, b = task % 3;
+    const int tx = (e & 1) + a, ty = ((e >> 1) & 3) + b, cz = (e >> 3) & 3;
+    uint32_t v = 0;
+    for (int c = 0; c < 3; ++c) v |= (uint32_t)hvp_slot(tx, ty, cz + c, a, b, c) << (10 * c);
+    t.v[task] = v;
+  }
+  return t;
+}
+// every entry of the layout belongs to exactly one (task, c)
+constexpr bool hvp_task_slots_valid()
+{
+  const HvpTaskSlots t = hvp_task_slots();
+  int hits[kCells * kAB * 3] = {};
+  for (int task = 0; task < kCells * kAB; ++task)
+    for (int c = 0; c < 3; ++c) {
+      const uint32_t k = (t.v[task] >> (10 * c)) & 1023u;
+      if (k >= (uint32_t)(kCells * kAB * 3)) return false;
+      ++hits[k];
+    }
+  for (int k = 0; k < kCells * kAB * 3; ++k)
+    if (hits[k] != 1) return false;
+  return true;
+}
+static_assert(hvp_task_slots_valid(), "node-major layout is a bijection");
+static_assert(hvp_task_slots().v[0] == (0u | (2u << 10) | (5u << 20)), "slots of task (cell 0, a 0, b 0)");
+__constant__ HvpTaskSlots kHvpTaskSlots = hvp_task_slots();
+
+// quotient in the operator's precision: IEEE division in FP64; in FP32 the
+// hardware reciprocal (1 ulp), well inside the FP32 operator's own rounding
+template <class T>
+__device__ __forceinline__ T op_div(T a, T b)
+{
+  return a / b;
+}
+template <>
+__device__ __forceinline__ float op_div<float>(float a, float b)
+{
+  return a * __builtin_amdgcn_rcpf(b);
+}
+
+// particles per chunk of the operator apply, and whether the first chunk's
+// whole operator data is loaded ahead of the tile gather (FP32: the registers
+// are there; FP64 keeps only the cell and the fractions ahead, to stay at
+// three waves per SIMD)
 template <class T>
 struct HvpChunk;
 template <>
 struct HvpChunk<float> {
   static constexpr int value = 256;
+  static constexpr bool prefetch_all = true;
 };
 template <>
 struct HvpChunk<double> {
   static constexpr int value = 128;
+  static constexpr bool prefetch_all = false;
 };
+
+// one particle's operator data; first / last mark the ends of its cell's run
+// of particles in the chunk (particles are sorted by cell within a group)
+template <class T>
+struct HvpParticle {
+  T fr[3], u[3][3], q[3][3], s[3], k2mu, klam;
+  uint32_t cell;
+  bool first, last;
+};
+template <class T>
+__device__ __forceinline__ void hvp_load_cell(const HvpCache<T>& cache, uint32_t ks, uint32_t j, uint32_t m,
+                                              HvpParticle<T>& P)
+{
+  const int64_t CS = cache.stride;
+  P.cell = cache.cell[ks];
+  P.first = j == 0 || cache.cell[ks - 1] != P.cell;
+  P.last = j + 1 == m || cache.cell[ks + 1] != P.cell;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) P.fr[d] = cache.frac[d * CS + ks];
+}
+template <class T>
+__device__ __forceinline__ void hvp_load_frame(const HvpCache<T>& cache, uint32_t ks, HvpParticle<T>& P)
+{
+  const int64_t CS = cache.stride;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      P.u[r][c] = cache.U[(c * 3 + r) * CS + ks];
+      P.q[r][c] = cache.Q[(c * 3 + r) * CS + ks];
+    }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) P.s[d] = cache.S[d * CS + ks];
+  P.k2mu = cache.K[ks];
+  P.klam = cache.K[CS + ks];
+}
 
 // Hessian-vector product, particle part: ds = V0 dP(grad p Fn) Fn^T rasterised
 // as ds grad w (MpmForceBase.cpp:258-302, FBasedMpmForceHelper.cpp:132-155),
 // in precision T from the sorted operator data.  No atomics, fixed order:
 //   phase 1  thread per particle: separable gather of grad p from the LDS
-//            tile, ds from the singular-frame differential; ds and the 1-D
-//            weights go to LDS
+//            tile, ds from the singular-frame differential; ds / dx^2 and
+//            the 1-D weights go to LDS, the cell runs' ends to cbeg / cend
 //   phase 2  task per (cell, a, b): sums ds grad w over the cell's particles
 //            in index order for the three c offsets (separable form
-//            gw = (dw_a w_b w_c, w_a dw_b w_c, w_a w_b dw_c) / dx)
-//   phase 3  thread per tile node: sums its <= 27 partials into the staged tile
+//            gw = (dw_a w_b w_c, w_a dw_b w_c, w_a w_b dw_c) / dx) into its
+//            three entries of the node-major layout
+//   phase 3  thread per tile node: sums its <= 18 partials, a contiguous run of
+//            the node-major layout, into the staged tile
 // qvec == nullptr: applies the operator to pvec itself; otherwise to the CG
 // direction qvec + beta pvec (beta from sc)
 template <class T>
@@ -464,45 +593,64 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
                                              const int32_t* __restrict__ ci)
 {
   constexpr int CH = HvpChunk<T>::value;
+  constexpr int kTasks = kCells * kAB;
+  static_assert(kTasks > 256 && kTasks <= 512, "two task rounds per block");
   if (ci && ci[CI_DONE]) return;
   __shared__ T tv[3 * kTileNodes];
-  __shared__ T pds[27][CH];                      // ds (9, row-major), w (3x3), dw (3x3)
-  __shared__ T contrib[3][kCells * kAB * 3];
+  // ds (9, row-major), w (3x3), dw (3x3); rows padded by one element so that
+  // the rows one particle's tasks read together fall in different banks
+  __shared__ T pds[27][CH + 1];
+  __shared__ T contrib[3][kTasks * 3];
   __shared__ int cbeg[kCells], cend[kCells];
   __shared__ int32_t nslot[8];
   const int tid = threadIdx.x;
   const uint32_t g = xcd_group(blockIdx.x, gridDim.x);
-  for (int k = tid; k < 3 * kCells * kAB * 3; k += 256) (&contrib[0][0])[k] = (T)0;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
-  const int64_t CS = cache.stride;
-  // the first chunk's per-particle loads are issued before the tile barrier
-  // so that their latency overlaps the tile gather
-  T fr0[3] = { (T)1, (T)1, (T)1 };
-  uint32_t cell0 = 0;
-  if ((uint32_t)tid < n && tid < CH) {
-    const uint32_t ks = start + tid;
-    cell0 = cache.cell[ks];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) fr0[d] = cache.frac[d * CS + ks];
+  // the task slots and the first chunk's particle data are loaded ahead of
+  // the tile barrier so that their latency overlaps the tile gather
+  const uint32_t ts0 = kHvpTaskSlots.v[tid];
+  const uint32_t ts1 = tid < kTasks - 256 ? kHvpTaskSlots.v[256 + tid] : 0u;
+  const uint32_t m0 = n < (uint32_t)CH ? n : (uint32_t)CH;
+  HvpParticle<T> P0;
+  if ((uint32_t)tid < m0) {
+    hvp_load_cell(cache, start + tid, tid, m0, P0);
+    if (HvpChunk<T>::prefetch_all) hvp_load_frame(cache, start + tid, P0);
   }
+  if (tid < kCells) {
+    cbeg[tid] = CH;
+    cend[tid] = 0;
+  }
+  if (n == 0)
+    for (int k = tid; k < 3 * kTasks * 3; k += 256) (&contrib[0][0])[k] = (T)0;
   load_tile3_dir(group_nslot + 8 * (size_t)g, qvec, pvec, qvec ? (T)sc[SC_BETA] : (T)0, gstride, tv, nslot);
+  const T inv_dx2 = inv_dx * inv_dx;
   for (uint32_t c0 = 0; c0 < n; c0 += CH) {
     const uint32_t m = n - c0 < (uint32_t)CH ? n - c0 : (uint32_t)CH;
-    if (tid < kCells) {
-      cbeg[tid] = CH;
-      cend[tid] = 0;
+    if (c0 > 0) {
+      if (tid < kCells) {
+        cbeg[tid] = CH;
+        cend[tid] = 0;
+      }
+      __syncthreads();
     }
-    __syncthreads();
     if ((uint32_t)tid < m) {
       const uint32_t ks = start + c0 + tid;
-      const uint32_t cell = c0 == 0 ? cell0 : cache.cell[ks];
+      HvpParticle<T> P;
+      if (c0 == 0) {
+        P = P0;
+        if (!HvpChunk<T>::prefetch_all) hvp_load_frame(cache, ks, P);
+      } else {
+        hvp_load_cell(cache, ks, tid, m, P);
+        hvp_load_frame(cache, ks, P);
+      }
+      const uint32_t cell = P.cell;
       const int t0 = ((int)(cell & 1u) * kTileY + (int)((cell >> 1) & 3u)) * kTileZ + (int)((cell >> 3) & 3u);
       T w[3][3], dw[3][3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) bspline_frac(c0 == 0 ? fr0[d] : cache.frac[d * CS + ks], w[d], dw[d]);
+      for (int d = 0; d < 3; ++d) bspline_frac(P.fr[d], w[d], dw[d]);
       // separable gather: G[r][0] = sum dw_a w_b w_c p, G[r][1] = sum w_a dw_b w_c p,
-      // G[r][2] = sum w_a w_b dw_c p (times 1/dx)
+      // G[r][2] = sum w_a w_b dw_c p (the 1/dx factor is applied to ds)
       T G[3][3];
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
@@ -523,21 +671,14 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
             W[a] += tc * w[1][bb];
           }
         }
-        G[r][0] = inv_dx * (dw[0][0] * U[0] + dw[0][1] * U[1] + dw[0][2] * U[2]);
-        G[r][1] = inv_dx * (w[0][0] * V[0] + w[0][1] * V[1] + w[0][2] * V[2]);
-        G[r][2] = inv_dx * (w[0][0] * W[0] + w[0][1] * W[1] + w[0][2] * W[2]);
+        G[r][0] = dw[0][0] * U[0] + dw[0][1] * U[1] + dw[0][2] * U[2];
+        G[r][1] = w[0][0] * V[0] + w[0][1] * V[1] + w[0][2] * V[2];
+        G[r][2] = w[0][0] * W[0] + w[0][1] * W[1] + w[0][2] * W[2];
       }
-      T u[3][3], q[3][3], s[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          u[r][c] = cache.U[(c * 3 + r) * CS + ks];
-          q[r][c] = cache.Q[(c * 3 + r) * CS + ks];
-        }
-#pragma unroll
-      for (int d = 0; d < 3; ++d) s[d] = cache.S[d * CS + ks];
-      const T k2mu = cache.K[ks], klam = cache.K[CS + ks];
+      const T(&u)[3][3] = P.u;
+      const T(&q)[3][3] = P.q;
+      const T* s = P.s;
+      const T k2mu = P.k2mu, klam = P.klam;
       // M = U^T (G Q)
       T T1[3][3], M[3][3];
 #pragma unroll
@@ -556,9 +697,9 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
       if (fabs(d01) < eps) d01 = d01 < (T)0 ? -eps : eps;
       if (fabs(d02) < eps) d02 = d02 < (T)0 ? -eps : eps;
       if (fabs(d12) < eps) d12 = d12 < (T)0 ? -eps : eps;
-      const T W01 = (M[0][1] - M[1][0]) / d01;
-      const T W02 = (M[0][2] - M[2][0]) / d02;
-      const T W12 = (M[1][2] - M[2][1]) / d12;
+      const T W01 = op_div(M[0][1] - M[1][0], d01);
+      const T W02 = op_div(M[0][2] - M[2][0], d02);
+      const T W12 = op_div(M[1][2] - M[2][1], d12);
       const T lj = klam * (J - (T)1);
       T H[3][3];
       H[0][0] = k2mu * M[0][0] + klam * cm * cof[0] + lj * (s[2] * M[1][1] + s[1] * M[2][2]);
@@ -570,7 +711,7 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
       H[2][0] = k2mu * (M[2][0] + W02) - lj * s[1] * M[0][2];
       H[1][2] = k2mu * (M[1][2] - W12) - lj * s[0] * M[2][1];
       H[2][1] = k2mu * (M[2][1] + W12) - lj * s[0] * M[1][2];
-      // ds = U H Q^T
+      // ds = U H Q^T, stored times 1/dx^2 (the gather's and the scatter's 1/dx)
       T T2[3][3];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
@@ -579,7 +720,8 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pds[r * 3 + c][tid] = T2[r][0] * q[c][0] + T2[r][1] * q[c][1] + T2[r][2] * q[c][2];
+        for (int c = 0; c < 3; ++c)
+          pds[r * 3 + c][tid] = inv_dx2 * (T2[r][0] * q[c][0] + T2[r][1] * q[c][1] + T2[r][2] * q[c][2]);
 #pragma unroll
       for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -587,12 +729,11 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
           pds[9 + 3 * d + a][tid] = w[d][a];
           pds[18 + 3 * d + a][tid] = dw[d][a];
         }
-      const int e = (int)cell;
-      atomicMin(&cbeg[e], tid);
-      atomicMax(&cend[e], tid + 1);
+      if (P.first) cbeg[cell] = tid;
+      if (P.last) cend[cell] = tid + 1;
     }
     __syncthreads();
-    for (int task = tid; task < kCells * kAB; task += 256) {
+    for (int task = tid; task < kTasks; task += 256) {
       const int e = task / kAB, ab = task - e * kAB;
       const int a = ab / 3, bb = ab - 3 * a;
       T acc[3][3];
@@ -604,7 +745,7 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
       for (int jj = jb; jj < je; ++jj) {
         const T wa = pds[9 + a][jj], dwa = pds[18 + a][jj];
         const T wb = pds[12 + bb][jj], dwb = pds[21 + bb][jj];
-        const T x0 = inv_dx * dwa * wb, x1 = inv_dx * wa * dwb, x2 = inv_dx * wa * wb;
+        const T x0 = dwa * wb, x1 = wa * dwb, x2 = wa * wb;
         T A[3], B[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -615,37 +756,40 @@ __global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* 
         for (int c = 0; c < 3; ++c) {
           const T wc = pds[15 + c][jj], dwc = pds[24 + c][jj];
 #pragma unroll
-          for (int r = 0; r < 3; ++r) acc[c][r] += A[r] * wc + B[r] * dwc;
+          for (int r = 0; r < 3; ++r) acc[c][r] = fma(B[r], dwc, fma(A[r], wc, acc[c][r]));
         }
       }
-      if (jb < je) {
+      // the first chunk stores every entry (zeros for an empty cell); later
+      // chunks add to it
+      const uint32_t ts = task < 256 ? ts0 : ts1;
+      if (c0 == 0) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c) {
+          const int k = (int)((ts >> (10 * c)) & 1023u);
 #pragma unroll
-          for (int r = 0; r < 3; ++r) contrib[r][task * 3 + c] += acc[c][r];
+          for (int r = 0; r < 3; ++r) contrib[r][k] = acc[c][r];
+        }
+      } else if (jb < je) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int k = (int)((ts >> (10 * c)) & 1023u);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) contrib[r][k] += acc[c][r];
+        }
       }
     }
     __syncthreads();
   }
+  // phase 3: each tile node sums its run of partials in (a, b, c) order
   T* out = stage + (size_t)g * kImpStage;
   for (int t = tid; t < kTileNodes; t += 256) {
     const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
+    const int k0 = hvp_node_off(tx, ty, tz), nk = hvp_na(tx) * hvp_nb(ty) * hvp_nb(tz);
     T acc0 = (T)0, acc1 = (T)0, acc2 = (T)0;
-    for (int a = 0; a < 3; ++a) {
-      const int cx = tx - a;
-      if (cx < 0 || cx > 1) continue;
-      for (int bb = 0; bb < 3; ++bb) {
-        const int cy = ty - bb;
-        if (cy < 0 || cy > 3) continue;
-        for (int c = 0; c < 3; ++c) {
-          const int cz = tz - c;
-          if (cz < 0 || cz > 3) continue;
-          const int idx = ((cx | (cy << 1) | (cz << 3)) * kAB + a * 3 + bb) * 3 + c;
-          acc0 += contrib[0][idx];
-          acc1 += contrib[1][idx];
-          acc2 += contrib[2][idx];
-        }
-      }
+    for (int k = k0; k < k0 + nk; ++k) {
+      acc0 += contrib[0][k];
+      acc1 += contrib[1][k];
+      acc2 += contrib[2][k];
     }
     out[t] = acc0;
     out[kTileNodes + t] = acc1;
@@ -715,8 +859,7 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
 {
   __shared__ double smem[4];
   if (ci && ci[CI_DONE]) return;
-  double acc = 0.0;
-  for (uint32_t b = threadIdx.x; b < nb; b += 256) acc += partial[b];
+  const double acc = thread_partials256(partial, nb);
   const double s = block_sum256(acc, smem);
   if (threadIdx.x == 0) sc[slot] = s;
 }
@@ -825,8 +968,7 @@ __global__ __launch_bounds__(256) void k_reduce_cg(const double* __restrict__ pa
 {
   __shared__ double smem[4];
   if (mode != RED_STORE && mode != RED_CG_INIT && ci[CI_DONE]) return;
-  double acc = 0.0;
-  for (uint32_t b = threadIdx.x; b < nb; b += 256) acc += partial[b];
+  const double acc = thread_partials256(partial, nb);
   const double v = block_sum256(acc, smem);
   if (threadIdx.x != 0) return;
   if (mode == RED_STORE) {
