@@ -219,7 +219,8 @@ int zr_set_time(zr_engine* e, double t);
 double zr_get_time(const zr_engine* e);
 
 /* Upload / download of the particle attributes: Scene/DataManager state
-   (DataManager.h:67-346).  Download fills X, V, C, F (NULL pointers skip). */
+   (DataManager.h:67-346).  Download fills X, V, C, F (NULL pointers skip).
+   At most 2^29 - 1 particles per engine (ZR_ERR_ARG beyond). */
 int zr_upload_particles(zr_engine* e, const zr_particles* p);
 int zr_download_particles(zr_engine* e, zr_particles* p);
 /* Per-particle models and plastic state, after zr_upload_particles (which

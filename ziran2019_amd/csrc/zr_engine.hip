@@ -33,6 +33,7 @@ using namespace zr;
 
 namespace {
 
+constexpr int64_t kMaxParticles = (int64_t(1) << 29) - 1;
 constexpr int kTileX = 4, kTileY = 6, kTileZ = 6;
 constexpr int kTileNodes = kTileX * kTileY * kTileZ;  // 144
 constexpr int kP2GFields = 7;                          // m, mv[3], f[3]
@@ -721,6 +722,9 @@ inline uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 
 struct zr_engine {
   zr_config cfg{};
   hipStream_t stream = nullptr;
+  // blocks of the persistent operator apply that fit on the device at once
+  // (FP32, FP64): resident blocks per CU times the CU count
+  uint32_t hvp_resident[2] = { 0, 0 };
   std::string err;
   double time = 0.0;
   std::vector<zr_collider> colliders;
@@ -803,6 +807,7 @@ struct zr_engine {
   size_t partial_cap = 0;
   double* sc = nullptr;
   int32_t* ci = nullptr;
+  uint32_t* hvp_ctr = nullptr;  // work counter of the persistent operator apply
   double* hsc = nullptr;
   int32_t* hci = nullptr;
   // FP32-bulk CG vectors and the operator data in both precisions
@@ -1104,6 +1109,8 @@ int ensure_scalars(zr_engine* e)
   if (!e->sc) {
     ZR_HIP(e, hipMalloc(&e->sc, SC_N * sizeof(double)));
     ZR_HIP(e, hipMalloc(&e->ci, CI_N * sizeof(int32_t)));
+    ZR_HIP(e, hipMalloc(&e->hvp_ctr, kHvpCounters * kHvpCounterStride * sizeof(uint32_t)));
+    ZR_HIP(e, hipMemset(e->hvp_ctr, 0, kHvpCounters * kHvpCounterStride * sizeof(uint32_t)));
     ZR_HIP(e, hipHostMalloc(&e->hsc, SC_N * sizeof(double)));
     ZR_HIP(e, hipHostMalloc(&e->hci, CI_N * sizeof(int32_t)));
   }
@@ -1499,12 +1506,15 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, const int32_t*
   ColliderSet cs = collider_set(e);
   T* stage = reinterpret_cast<T*>(e->stage);
   prof_mark(e, ZR_PROF_HVP);
-  k_hvp<T><<<e->A, 256, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
-                                        (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci);
+  const uint32_t resident = e->hvp_resident[sizeof(T) == sizeof(double) ? 1 : 0];
+  const uint32_t nblk = e->A < resident ? e->A : resident;
+  k_hvp<T><<<nblk, 256, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
+                                        (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci, e->A, e->hvp_ctr);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
   k_cg_ap<T><<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
-                                        nvT(e, out), dx, dt, e->time, cs, project, e->partial, e->sc, ci);
+                                        nvT(e, out), dx, dt, e->time, cs, project, e->partial, e->sc, ci,
+                                        e->hvp_ctr);
   if (cg) k_reduce_cg<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, e->ci, RED_ALPHA, 0, 0.0, 0.0);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
@@ -1624,6 +1634,22 @@ int zr_create(const zr_config* cfg, zr_engine** out)
     delete e;
     return ZR_ERR_HIP;
   }
+  {
+    // the persistent operator apply launches one wave of resident blocks
+    // (its blocks never wait on each other, so fewer resident is still correct)
+    int dev = 0, ncu = 0, b32 = 0, b64 = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b32, k_hvp<float>, 256, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b64, k_hvp<double>, 256, 0) != hipSuccess || ncu <= 0 ||
+        b32 <= 0 || b64 <= 0) {
+      (void)hipStreamDestroy(e->stream);
+      delete e;
+      return ZR_ERR_HIP;
+    }
+    e->hvp_resident[0] = (uint32_t)(b32 * ncu);
+    e->hvp_resident[1] = (uint32_t)(b64 * ncu);
+  }
   bool ok = hipMalloc(&e->page_count, e->ndense * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->page_group, e->ndense * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&e->page_touch, e->ndense * sizeof(uint32_t)) == hipSuccess &&
@@ -1657,7 +1683,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
-  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
+  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);
@@ -1699,6 +1725,8 @@ double zr_get_time(const zr_engine* e) { return e ? e->time : 0.0; }
 int zr_upload_particles(zr_engine* e, const zr_particles* p)
 {
   if (!e || !p || p->n < 0) return fail(e, ZR_ERR_ARG, "bad particle view");
+  // per-particle arrays are addressed with 32-bit byte offsets (8-byte values)
+  if (p->n > kMaxParticles) return fail(e, ZR_ERR_ARG, "more than 2^29 - 1 particles");
   if (p->n >= (1ll << 27)) return fail(e, ZR_ERR_ARG, "particle count must be below 2^27");
   if (p->n > 0 && (!p->X || !p->V || !p->C || !p->F || !p->mass || !p->vol || !p->mu || !p->lambda))
     return fail(e, ZR_ERR_ARG, "all particle attributes are required");

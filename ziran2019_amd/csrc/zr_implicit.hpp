@@ -233,22 +233,13 @@ __device__ __forceinline__ void load_tile3_slots(const int32_t* gslot, const TS*
 }
 
 // CG search direction on the fly: p_new = q + beta p_old for the tile nodes
-template <class T>
-__device__ __forceinline__ void load_tile3_dir(const int32_t* gslot, const T* q, const T* p, T beta, int64_t stride,
-                                               T* tv, int32_t* nslot)
+// tile node t (< kTileNodes) of a group's 4x6x6 footprint: which of the 8
+// neighbour page slots holds it (4a + 2b + c) and its element in that page
+__device__ __forceinline__ void tile_node_source(int t, int& src, int& elem)
 {
-  const int tid = threadIdx.x;
-  if (tid < 8) nslot[tid] = gslot[tid];
-  __syncthreads();
-  for (int t = tid; t < kTileNodes; t += blockDim.x) {
-    const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
-    const int32_t sl = nslot[((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2)];
-    const int64_t node = (int64_t)sl * 32 + elem_of(tx, ty, tz);
-    for (int d = 0; d < 3; ++d)
-      tv[d * kTileNodes + t] = sl >= 0 ? (q ? q[d * stride + node] + beta * p[d * stride + node] : p[d * stride + node])
-                                       : (T)0;
-  }
-  __syncthreads();
+  const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
+  src = ((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2);
+  elem = (int)elem_of(tx, ty, tz);
 }
 
 template <class T>
@@ -518,57 +509,120 @@ __device__ __forceinline__ float op_div<float>(float a, float b)
   return a * __builtin_amdgcn_rcpf(b);
 }
 
-// particles per chunk of the operator apply, and whether the first chunk's
-// whole operator data is loaded ahead of the tile gather (FP32: the registers
-// are there; FP64 keeps only the cell and the fractions ahead, to stay at
-// three waves per SIMD)
+// particles per chunk of the operator apply, and the waves per SIMD its LDS
+// allows (the register budget is held to it)
 template <class T>
 struct HvpChunk;
 template <>
 struct HvpChunk<float> {
   static constexpr int value = 256;
-  static constexpr bool prefetch_all = true;
+  static constexpr int waves = 4;
+  static constexpr bool frame_early = true;  // singular frame requested before the gather
 };
 template <>
 struct HvpChunk<double> {
-  static constexpr int value = 128;
-  static constexpr bool prefetch_all = false;
+  static constexpr int value = 256;
+  static constexpr int waves = 2;
+  static constexpr bool frame_early = false;  // after it: FP64 registers
 };
 
-// one particle's operator data; first / last mark the ends of its cell's run
-// of particles in the chunk (particles are sorted by cell within a group)
+// one particle's operator data with the cells of its neighbours in the
+// chunk (~0 past the chunk's ends): particles are sorted by cell within a
+// group, so a particle starts / ends its cell's run where they differ.
+// The loads take a uniform base and a 32-bit byte offset (a field of the
+// operator data spans less than 4 GiB: at most 2^29 - 1 particles).
 template <class T>
 struct HvpParticle {
   T fr[3], u[3][3], q[3][3], s[3], k2mu, klam;
-  uint32_t cell;
-  bool first, last;
+  uint32_t cell, cprev, cnext;
 };
+template <class V>
+__device__ __forceinline__ V ld_off(const V* base, uint32_t byte_off)
+{
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
+}
 template <class T>
-__device__ __forceinline__ void hvp_load_cell(const HvpCache<T>& cache, uint32_t ks, uint32_t j, uint32_t m,
-                                              HvpParticle<T>& P)
+__device__ __forceinline__ void hvp_load(const HvpCache<T>& cache, uint32_t ks, uint32_t j, uint32_t m,
+                                         HvpParticle<T>& P)
 {
   const int64_t CS = cache.stride;
-  P.cell = cache.cell[ks];
-  P.first = j == 0 || cache.cell[ks - 1] != P.cell;
-  P.last = j + 1 == m || cache.cell[ks + 1] != P.cell;
+  // the offsets are formed here, at the issue point, so that the compiler
+  // does not hoist (and hold) the addresses of a prefetch
+  uint32_t o = ks * (uint32_t)sizeof(T), oc = ks * 4u;
+  asm volatile("" : "+v"(o), "+v"(oc));
+  P.cell = ld_off(cache.cell, oc);
+  P.cprev = j > 0 ? ld_off(cache.cell, oc - 4u) : ~0u;
+  P.cnext = j + 1 < m ? ld_off(cache.cell, oc + 4u) : ~0u;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) P.fr[d] = cache.frac[d * CS + ks];
+  for (int d = 0; d < 3; ++d) P.fr[d] = ld_off(cache.frac + d * CS, o);
 }
 template <class T>
 __device__ __forceinline__ void hvp_load_frame(const HvpCache<T>& cache, uint32_t ks, HvpParticle<T>& P)
 {
   const int64_t CS = cache.stride;
+  uint32_t o = ks * (uint32_t)sizeof(T);
+  asm volatile("" : "+v"(o));
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      P.u[r][c] = cache.U[(c * 3 + r) * CS + ks];
-      P.q[r][c] = cache.Q[(c * 3 + r) * CS + ks];
+      P.u[r][c] = ld_off(cache.U + (c * 3 + r) * CS, o);
+      P.q[r][c] = ld_off(cache.Q + (c * 3 + r) * CS, o);
     }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) P.s[d] = cache.S[d * CS + ks];
-  P.k2mu = cache.K[ks];
-  P.klam = cache.K[CS + ks];
+  for (int d = 0; d < 3; ++d) P.s[d] = ld_off(cache.S + d * CS, o);
+  P.k2mu = ld_off(cache.K, o);
+  P.klam = ld_off(cache.K + CS, o);
+}
+
+// work counters of the persistent operator apply: one per XCD, each in its
+// own 256-byte line
+constexpr int kHvpCounterStride = 64;
+constexpr int kHvpCounters = 8;
+
+// the groups of XCD x (block b runs on XCD b mod 8): a contiguous range in
+// proportion to its blocks; its blocks start on the first nbx groups and
+// take the rest from the XCD's counter
+struct XcdGroups {
+  uint32_t lo, hi, nbx, l;
+};
+__device__ __forceinline__ XcdGroups xcd_groups(uint32_t b, uint32_t nblocks, uint32_t ngroups)
+{
+  const uint32_t x = b & 7u, q = nblocks >> 3, r = nblocks & 7u;
+  const uint32_t nbx = q + (x < r ? 1u : 0u);
+  const uint32_t rlo = x * q + (x < r ? x : r);
+  return { (uint32_t)(((uint64_t)ngroups * rlo) / nblocks), (uint32_t)(((uint64_t)ngroups * (rlo + nbx)) / nblocks),
+           nbx, b >> 3 };
+}
+
+// requests of the next group gn: bounds, this thread's tile node value(s)
+// and first-chunk particle (cells and fractions)
+template <class T>
+__device__ __forceinline__ void hvp_issue_next(const HvpCache<T>& cache, const uint32_t* __restrict__ group_start,
+                                               const int32_t* __restrict__ group_nslot, const T* __restrict__ qvec,
+                                               const T* __restrict__ pvec, int64_t gstride, uint32_t gn, bool tnode,
+                                               uint32_t& nstart, uint32_t& nn, int32_t& nsl, HvpParticle<T>& P1,
+                                               T tq[3], T tp[3])
+{
+  constexpr int CH = HvpChunk<T>::value;
+  const int tid = threadIdx.x;
+  nstart = group_start[gn];
+  nn = group_start[gn + 1] - nstart;
+  const uint32_t m1 = nn < (uint32_t)CH ? nn : (uint32_t)CH;
+  if ((uint32_t)tid < m1) hvp_load(cache, nstart + tid, tid, m1, P1);
+  if (tnode) {
+    int tsrc, telem;
+    tile_node_source(tid, tsrc, telem);
+    nsl = group_nslot[8 * (size_t)gn + tsrc];
+    if (nsl >= 0) {
+      const int64_t node = (int64_t)nsl * 32 + telem;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        tp[d] = pvec[d * gstride + node];
+        if (qvec) tq[d] = qvec[d * gstride + node];
+      }
+    }
+  }
 }
 
 // Hessian-vector product, particle part: ds = V0 dP(grad p Fn) Fn^T rasterised
@@ -583,217 +637,295 @@ __device__ __forceinline__ void hvp_load_frame(const HvpCache<T>& cache, uint32_
 //            three entries of the node-major layout
 //   phase 3  thread per tile node: sums its <= 18 partials, a contiguous run of
 //            the node-major layout, into the staged tile
+// Persistent blocks, one wave of them resident, dealt to the XCDs as
+// xcd_groups describes; further groups come from the XCD's work counter in
+// wctr (zero at launch; the kernel that follows resets them), so that the
+// load balances within an XCD.  The counter is read two groups ahead and
+// the next group's bounds, page slots, tile values and first-chunk cells and
+// fractions one group ahead, so that they are in flight while this group's
+// phases 2 and 3 run.  A particle's singular frame
+// is requested at the start of phase 1 (FP32) or after the gather (FP64),
+// within the register budget of the resident waves.
 // qvec == nullptr: applies the operator to pvec itself; otherwise to the CG
 // direction qvec + beta pvec (beta from sc)
 template <class T>
-__global__ __launch_bounds__(256) void k_hvp(HvpCache<T> cache, const uint32_t* __restrict__ group_start,
+__global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cache, const uint32_t* __restrict__ group_start,
                                              const int32_t* __restrict__ group_nslot, const T* __restrict__ qvec,
                                              const T* __restrict__ pvec, const double* __restrict__ sc,
                                              int64_t gstride, T inv_dx, T* __restrict__ stage,
-                                             const int32_t* __restrict__ ci)
+                                             const int32_t* __restrict__ ci, uint32_t ngroups,
+                                             uint32_t* __restrict__ wctr)
 {
   constexpr int CH = HvpChunk<T>::value;
   constexpr int kTasks = kCells * kAB;
   static_assert(kTasks > 256 && kTasks <= 512, "two task rounds per block");
+  static_assert(kTileNodes <= 256, "one tile node per thread");
   if (ci && ci[CI_DONE]) return;
+  const XcdGroups xg = xcd_groups(blockIdx.x, gridDim.x, ngroups);
+  uint32_t g = xg.lo + xg.l;
+  if (g >= xg.hi) return;
+  uint32_t* const xctr = wctr + kHvpCounterStride * (blockIdx.x & 7u);
+  __shared__ uint32_t next_group;
   __shared__ T tv[3 * kTileNodes];
   // ds (9, row-major), w (3x3), dw (3x3); rows padded by one element so that
   // the rows one particle's tasks read together fall in different banks
   __shared__ T pds[27][CH + 1];
   __shared__ T contrib[3][kTasks * 3];
   __shared__ int cbeg[kCells], cend[kCells];
-  __shared__ int32_t nslot[8];
   const int tid = threadIdx.x;
-  const uint32_t g = xcd_group(blockIdx.x, gridDim.x);
-  const uint32_t start = group_start[g];
-  const uint32_t n = group_start[g + 1] - start;
-  // the task slots and the first chunk's particle data are loaded ahead of
-  // the tile barrier so that their latency overlaps the tile gather
-  const uint32_t ts0 = kHvpTaskSlots.v[tid];
-  const uint32_t ts1 = tid < kTasks - 256 ? kHvpTaskSlots.v[256 + tid] : 0u;
-  const uint32_t m0 = n < (uint32_t)CH ? n : (uint32_t)CH;
-  HvpParticle<T> P0;
-  if ((uint32_t)tid < m0) {
-    hvp_load_cell(cache, start + tid, tid, m0, P0);
-    if (HvpChunk<T>::prefetch_all) hvp_load_frame(cache, start + tid, P0);
-  }
-  if (tid < kCells) {
-    cbeg[tid] = CH;
-    cend[tid] = 0;
-  }
-  if (n == 0)
-    for (int k = tid; k < 3 * kTasks * 3; k += 256) (&contrib[0][0])[k] = (T)0;
-  load_tile3_dir(group_nslot + 8 * (size_t)g, qvec, pvec, qvec ? (T)sc[SC_BETA] : (T)0, gstride, tv, nslot);
+  const T beta = qvec ? (T)sc[SC_BETA] : (T)0;
   const T inv_dx2 = inv_dx * inv_dx;
-  for (uint32_t c0 = 0; c0 < n; c0 += CH) {
-    const uint32_t m = n - c0 < (uint32_t)CH ? n - c0 : (uint32_t)CH;
-    if (c0 > 0) {
-      if (tid < kCells) {
-        cbeg[tid] = CH;
-        cend[tid] = 0;
+  const bool tnode = tid < kTileNodes;
+  // thread 0: the group after the next one, from the work counter
+  uint32_t ahead = 0;
+  if (tid == 0) ahead = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
+
+  // prologue: the first group's tile and first-chunk particle data
+  uint32_t start = group_start[g];
+  uint32_t n = group_start[g + 1] - start;
+  // this thread's particle of the current chunk
+  HvpParticle<T> P;
+  {
+    const uint32_t m0 = n < (uint32_t)CH ? n : (uint32_t)CH;
+    if ((uint32_t)tid < m0) {
+      hvp_load(cache, start + tid, tid, m0, P);
+    }
+    if (tnode) {
+      int tsrc, telem;
+      tile_node_source(tid, tsrc, telem);
+      const int32_t sl = group_nslot[8 * (size_t)g + tsrc];
+      const int64_t node = (int64_t)sl * 32 + telem;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        tv[d * kTileNodes + tid] =
+            sl >= 0 ? (qvec ? qvec[d * gstride + node] + beta * pvec[d * gstride + node] : pvec[d * gstride + node])
+                    : (T)0;
+    }
+    if (tid < kCells) {
+      cbeg[tid] = CH;
+      cend[tid] = 0;
+    }
+    __syncthreads();
+  }
+
+  for (;;) {
+    // the next group (read a group ago), visible to the block after this
+    // group's first barrier; and the one after it
+    if (tid == 0) {
+      next_group = ahead;
+      if (ahead < xg.hi) ahead = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
+    }
+    uint32_t gn = 0, nstart = 0, nn = 0;
+    bool more = false;
+    int32_t nsl = -1;
+    HvpParticle<T> P1;
+    T tq[3], tp[3];  // read only where loaded (more && nsl >= 0)
+    bool issued = false;
+    if (n == 0) {  // an empty group stages zeros
+      for (int k = tid; k < 3 * kTasks * 3; k += 256) (&contrib[0][0])[k] = (T)0;
+      __syncthreads();
+    }
+    for (uint32_t c0 = 0; c0 < n; c0 += CH) {
+      const uint32_t m = n - c0 < (uint32_t)CH ? n - c0 : (uint32_t)CH;
+      if (c0 > 0) {
+        if (tid < kCells) {
+          cbeg[tid] = CH;
+          cend[tid] = 0;
+        }
+        __syncthreads();
+      }
+      if ((uint32_t)tid < m) {
+        if (HvpChunk<T>::frame_early) hvp_load_frame(cache, start + c0 + tid, P);
+        const uint32_t cell = P.cell;
+        const int t0 = ((int)(cell & 1u) * kTileY + (int)((cell >> 1) & 3u)) * kTileZ + (int)((cell >> 3) & 3u);
+        T w[3][3], dw[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bspline_frac(P.fr[d], w[d], dw[d]);
+        // separable gather: G[r][0] = sum dw_a w_b w_c p, G[r][1] = sum w_a dw_b w_c p,
+        // G[r][2] = sum w_a w_b dw_c p (the 1/dx factor is applied to ds)
+        T G[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          T U[3], V[3], W[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            U[a] = (T)0;
+            V[a] = (T)0;
+            W[a] = (T)0;
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb) {
+              const T* row = tv + r * kTileNodes + t0 + (a * kTileY + bb) * kTileZ;
+              const T p0 = row[0], p1 = row[1], p2 = row[2];
+              const T sc = p0 * w[2][0] + p1 * w[2][1] + p2 * w[2][2];
+              const T tc = p0 * dw[2][0] + p1 * dw[2][1] + p2 * dw[2][2];
+              U[a] += sc * w[1][bb];
+              V[a] += sc * dw[1][bb];
+              W[a] += tc * w[1][bb];
+            }
+          }
+          G[r][0] = dw[0][0] * U[0] + dw[0][1] * U[1] + dw[0][2] * U[2];
+          G[r][1] = w[0][0] * V[0] + w[0][1] * V[1] + w[0][2] * V[2];
+          G[r][2] = w[0][0] * W[0] + w[0][1] * W[1] + w[0][2] * W[2];
+        }
+        if (!HvpChunk<T>::frame_early) hvp_load_frame(cache, start + c0 + tid, P);
+        const T(&u)[3][3] = P.u;
+        const T(&q)[3][3] = P.q;
+        const T* s = P.s;
+        const T k2mu = P.k2mu, klam = P.klam;
+        // M = U^T (G Q)
+        T T1[3][3], M[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) T1[r][c] = G[r][0] * q[0][c] + G[r][1] * q[1][c] + G[r][2] * q[2][c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) M[r][c] = u[0][r] * T1[0][c] + u[1][r] * T1[1][c] + u[2][r] * T1[2][c];
+        const T J = s[0] * s[1] * s[2];
+        const T cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
+        const T cm = cof[0] * M[0][0] + cof[1] * M[1][1] + cof[2] * M[2][2];
+        const T eps = (T)1e-12;
+        T d01 = s[0] + s[1], d02 = s[0] + s[2], d12 = s[1] + s[2];
+        if (fabs(d01) < eps) d01 = d01 < (T)0 ? -eps : eps;
+        if (fabs(d02) < eps) d02 = d02 < (T)0 ? -eps : eps;
+        if (fabs(d12) < eps) d12 = d12 < (T)0 ? -eps : eps;
+        const T W01 = op_div(M[0][1] - M[1][0], d01);
+        const T W02 = op_div(M[0][2] - M[2][0], d02);
+        const T W12 = op_div(M[1][2] - M[2][1], d12);
+        const T lj = klam * (J - (T)1);
+        T H[3][3];
+        H[0][0] = k2mu * M[0][0] + klam * cm * cof[0] + lj * (s[2] * M[1][1] + s[1] * M[2][2]);
+        H[1][1] = k2mu * M[1][1] + klam * cm * cof[1] + lj * (s[2] * M[0][0] + s[0] * M[2][2]);
+        H[2][2] = k2mu * M[2][2] + klam * cm * cof[2] + lj * (s[1] * M[0][0] + s[0] * M[1][1]);
+        H[0][1] = k2mu * (M[0][1] - W01) - lj * s[2] * M[1][0];
+        H[1][0] = k2mu * (M[1][0] + W01) - lj * s[2] * M[0][1];
+        H[0][2] = k2mu * (M[0][2] - W02) - lj * s[1] * M[2][0];
+        H[2][0] = k2mu * (M[2][0] + W02) - lj * s[1] * M[0][2];
+        H[1][2] = k2mu * (M[1][2] - W12) - lj * s[0] * M[2][1];
+        H[2][1] = k2mu * (M[2][1] + W12) - lj * s[0] * M[1][2];
+        // ds = U H Q^T, stored times 1/dx^2 (the gather's and the scatter's 1/dx)
+        T T2[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) T2[r][c] = u[r][0] * H[0][c] + u[r][1] * H[1][c] + u[r][2] * H[2][c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            pds[r * 3 + c][tid] = inv_dx2 * (T2[r][0] * q[c][0] + T2[r][1] * q[c][1] + T2[r][2] * q[c][2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            pds[9 + 3 * d + a][tid] = w[d][a];
+            pds[18 + 3 * d + a][tid] = dw[d][a];
+          }
+        if (P.cprev != cell) cbeg[cell] = tid;
+        if (P.cnext != cell) cend[cell] = tid + 1;
+      }
+      __syncthreads();
+      if (c0 + CH >= n) {
+        // last chunk: the next group's data are requested here and consumed
+        // after phase 3
+        gn = next_group;
+        more = gn < xg.hi;
+        if (more) hvp_issue_next(cache, group_start, group_nslot, qvec, pvec, gstride, gn, tnode, nstart, nn, nsl,
+                                 P1, tq, tp);
+        issued = true;
+      }
+      for (int task = tid; task < kTasks; task += 256) {
+        const int e = task / kAB, ab = task - e * kAB;
+        const int a = ab / 3, bb = ab - 3 * a;
+        T acc[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) acc[c][r] = (T)0;
+        const int jb = cbeg[e], je = cend[e];
+        for (int jj = jb; jj < je; ++jj) {
+          const T wa = pds[9 + a][jj], dwa = pds[18 + a][jj];
+          const T wb = pds[12 + bb][jj], dwb = pds[21 + bb][jj];
+          const T x0 = dwa * wb, x1 = wa * dwb, x2 = wa * wb;
+          T A[3], B[3];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            A[r] = pds[3 * r][jj] * x0 + pds[3 * r + 1][jj] * x1;
+            B[r] = pds[3 * r + 2][jj] * x2;
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const T wc = pds[15 + c][jj], dwc = pds[24 + c][jj];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) acc[c][r] = fma(B[r], dwc, fma(A[r], wc, acc[c][r]));
+          }
+        }
+        // the first chunk stores every entry (zeros for an empty cell); later
+        // chunks add to it
+        const uint32_t ts = kHvpTaskSlots.v[task];
+        if (c0 == 0) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int k = (int)((ts >> (10 * c)) & 1023u);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) contrib[r][k] = acc[c][r];
+          }
+        } else if (jb < je) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int k = (int)((ts >> (10 * c)) & 1023u);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) contrib[r][k] += acc[c][r];
+          }
+        }
+      }
+      if (c0 + CH < n) {  // the next chunk's particle
+        const uint32_t c1 = c0 + CH;
+        const uint32_t m2 = n - c1 < (uint32_t)CH ? n - c1 : (uint32_t)CH;
+        if ((uint32_t)tid < m2) {
+          hvp_load(cache, start + c1 + tid, tid, m2, P);
+        }
       }
       __syncthreads();
     }
-    if ((uint32_t)tid < m) {
-      const uint32_t ks = start + c0 + tid;
-      HvpParticle<T> P;
-      if (c0 == 0) {
-        P = P0;
-        if (!HvpChunk<T>::prefetch_all) hvp_load_frame(cache, ks, P);
-      } else {
-        hvp_load_cell(cache, ks, tid, m, P);
-        hvp_load_frame(cache, ks, P);
+    if (!issued) {  // an empty group (no chunk ran; its zeroing barrier passed)
+      gn = next_group;
+      more = gn < xg.hi;
+      if (more) hvp_issue_next(cache, group_start, group_nslot, qvec, pvec, gstride, gn, tnode, nstart, nn, nsl,
+                               P1, tq, tp);
+    }
+    // phase 3: each tile node sums its run of partials in (a, b, c) order
+    if (tnode) {
+      const int tx = tid / (kTileY * kTileZ), ty = (tid / kTileZ) % kTileY, tz = tid % kTileZ;
+      const int k0 = hvp_node_off(tx, ty, tz), nk = hvp_na(tx) * hvp_nb(ty) * hvp_nb(tz);
+      T acc0 = (T)0, acc1 = (T)0, acc2 = (T)0;
+      for (int k = k0; k < k0 + nk; ++k) {
+        acc0 += contrib[0][k];
+        acc1 += contrib[1][k];
+        acc2 += contrib[2][k];
       }
-      const uint32_t cell = P.cell;
-      const int t0 = ((int)(cell & 1u) * kTileY + (int)((cell >> 1) & 3u)) * kTileZ + (int)((cell >> 3) & 3u);
-      T w[3][3], dw[3][3];
+      T* out = stage + (size_t)g * kImpStage;
+      out[tid] = acc0;
+      out[kTileNodes + tid] = acc1;
+      out[2 * kTileNodes + tid] = acc2;
+    }
+    if (!more) break;
+    // the next group's tile (tv is free once this group's phase 1 is done)
+    // and empty cell runs
+    if (tnode) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) bspline_frac(P.fr[d], w[d], dw[d]);
-      // separable gather: G[r][0] = sum dw_a w_b w_c p, G[r][1] = sum w_a dw_b w_c p,
-      // G[r][2] = sum w_a w_b dw_c p (the 1/dx factor is applied to ds)
-      T G[3][3];
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        T U[3], V[3], W[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          U[a] = (T)0;
-          V[a] = (T)0;
-          W[a] = (T)0;
-#pragma unroll
-          for (int bb = 0; bb < 3; ++bb) {
-            const T* row = tv + r * kTileNodes + t0 + (a * kTileY + bb) * kTileZ;
-            const T p0 = row[0], p1 = row[1], p2 = row[2];
-            const T sc = p0 * w[2][0] + p1 * w[2][1] + p2 * w[2][2];
-            const T tc = p0 * dw[2][0] + p1 * dw[2][1] + p2 * dw[2][2];
-            U[a] += sc * w[1][bb];
-            V[a] += sc * dw[1][bb];
-            W[a] += tc * w[1][bb];
-          }
-        }
-        G[r][0] = dw[0][0] * U[0] + dw[0][1] * U[1] + dw[0][2] * U[2];
-        G[r][1] = w[0][0] * V[0] + w[0][1] * V[1] + w[0][2] * V[2];
-        G[r][2] = w[0][0] * W[0] + w[0][1] * W[1] + w[0][2] * W[2];
-      }
-      const T(&u)[3][3] = P.u;
-      const T(&q)[3][3] = P.q;
-      const T* s = P.s;
-      const T k2mu = P.k2mu, klam = P.klam;
-      // M = U^T (G Q)
-      T T1[3][3], M[3][3];
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) T1[r][c] = G[r][0] * q[0][c] + G[r][1] * q[1][c] + G[r][2] * q[2][c];
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) M[r][c] = u[0][r] * T1[0][c] + u[1][r] * T1[1][c] + u[2][r] * T1[2][c];
-      const T J = s[0] * s[1] * s[2];
-      const T cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
-      const T cm = cof[0] * M[0][0] + cof[1] * M[1][1] + cof[2] * M[2][2];
-      const T eps = (T)1e-12;
-      T d01 = s[0] + s[1], d02 = s[0] + s[2], d12 = s[1] + s[2];
-      if (fabs(d01) < eps) d01 = d01 < (T)0 ? -eps : eps;
-      if (fabs(d02) < eps) d02 = d02 < (T)0 ? -eps : eps;
-      if (fabs(d12) < eps) d12 = d12 < (T)0 ? -eps : eps;
-      const T W01 = op_div(M[0][1] - M[1][0], d01);
-      const T W02 = op_div(M[0][2] - M[2][0], d02);
-      const T W12 = op_div(M[1][2] - M[2][1], d12);
-      const T lj = klam * (J - (T)1);
-      T H[3][3];
-      H[0][0] = k2mu * M[0][0] + klam * cm * cof[0] + lj * (s[2] * M[1][1] + s[1] * M[2][2]);
-      H[1][1] = k2mu * M[1][1] + klam * cm * cof[1] + lj * (s[2] * M[0][0] + s[0] * M[2][2]);
-      H[2][2] = k2mu * M[2][2] + klam * cm * cof[2] + lj * (s[1] * M[0][0] + s[0] * M[1][1]);
-      H[0][1] = k2mu * (M[0][1] - W01) - lj * s[2] * M[1][0];
-      H[1][0] = k2mu * (M[1][0] + W01) - lj * s[2] * M[0][1];
-      H[0][2] = k2mu * (M[0][2] - W02) - lj * s[1] * M[2][0];
-      H[2][0] = k2mu * (M[2][0] + W02) - lj * s[1] * M[0][2];
-      H[1][2] = k2mu * (M[1][2] - W12) - lj * s[0] * M[2][1];
-      H[2][1] = k2mu * (M[2][1] + W12) - lj * s[0] * M[1][2];
-      // ds = U H Q^T, stored times 1/dx^2 (the gather's and the scatter's 1/dx)
-      T T2[3][3];
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) T2[r][c] = u[r][0] * H[0][c] + u[r][1] * H[1][c] + u[r][2] * H[2][c];
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          pds[r * 3 + c][tid] = inv_dx2 * (T2[r][0] * q[c][0] + T2[r][1] * q[c][1] + T2[r][2] * q[c][2]);
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          pds[9 + 3 * d + a][tid] = w[d][a];
-          pds[18 + 3 * d + a][tid] = dw[d][a];
-        }
-      if (P.first) cbeg[cell] = tid;
-      if (P.last) cend[cell] = tid + 1;
+      for (int d = 0; d < 3; ++d) tv[d * kTileNodes + tid] = nsl >= 0 ? (qvec ? tq[d] + beta * tp[d] : tp[d]) : (T)0;
+    }
+    if (tid < kCells) {
+      cbeg[tid] = CH;
+      cend[tid] = 0;
     }
     __syncthreads();
-    for (int task = tid; task < kTasks; task += 256) {
-      const int e = task / kAB, ab = task - e * kAB;
-      const int a = ab / 3, bb = ab - 3 * a;
-      T acc[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) acc[c][r] = (T)0;
-      const int jb = cbeg[e], je = cend[e];
-      for (int jj = jb; jj < je; ++jj) {
-        const T wa = pds[9 + a][jj], dwa = pds[18 + a][jj];
-        const T wb = pds[12 + bb][jj], dwb = pds[21 + bb][jj];
-        const T x0 = dwa * wb, x1 = wa * dwb, x2 = wa * wb;
-        T A[3], B[3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          A[r] = pds[3 * r][jj] * x0 + pds[3 * r + 1][jj] * x1;
-          B[r] = pds[3 * r + 2][jj] * x2;
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T wc = pds[15 + c][jj], dwc = pds[24 + c][jj];
-#pragma unroll
-          for (int r = 0; r < 3; ++r) acc[c][r] = fma(B[r], dwc, fma(A[r], wc, acc[c][r]));
-        }
-      }
-      // the first chunk stores every entry (zeros for an empty cell); later
-      // chunks add to it
-      const uint32_t ts = task < 256 ? ts0 : ts1;
-      if (c0 == 0) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const int k = (int)((ts >> (10 * c)) & 1023u);
-#pragma unroll
-          for (int r = 0; r < 3; ++r) contrib[r][k] = acc[c][r];
-        }
-      } else if (jb < je) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const int k = (int)((ts >> (10 * c)) & 1023u);
-#pragma unroll
-          for (int r = 0; r < 3; ++r) contrib[r][k] += acc[c][r];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // phase 3: each tile node sums its run of partials in (a, b, c) order
-  T* out = stage + (size_t)g * kImpStage;
-  for (int t = tid; t < kTileNodes; t += 256) {
-    const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
-    const int k0 = hvp_node_off(tx, ty, tz), nk = hvp_na(tx) * hvp_nb(ty) * hvp_nb(tz);
-    T acc0 = (T)0, acc1 = (T)0, acc2 = (T)0;
-    for (int k = k0; k < k0 + nk; ++k) {
-      acc0 += contrib[0][k];
-      acc1 += contrib[1][k];
-      acc2 += contrib[2][k];
-    }
-    out[t] = acc0;
-    out[kTileNodes + t] = acc1;
-    out[2 * kTileNodes + t] = acc2;
+    g = gn;
+    start = nstart;
+    n = nn;
+    P = P1;
   }
 }
 
@@ -897,8 +1029,10 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
                                                const double* __restrict__ gm, NodeVec<T> q, NodeVec<T> p,
                                                NodeVec<T> ap, double dx, double dt, double time, ColliderSet cs,
                                                int project, double* __restrict__ partial, const double* sc,
-                                               const int32_t* ci)
+                                               const int32_t* ci, uint32_t* __restrict__ hvp_ctr)
 {
+  // the operator apply this kernel follows has drained its work counters
+  if (blockIdx.x == 0 && threadIdx.x < kHvpCounters) hvp_ctr[kHvpCounterStride * threadIdx.x] = 0u;
   if (ci && ci[CI_DONE]) return;
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   double contrib = 0.0;
