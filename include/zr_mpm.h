@@ -320,7 +320,11 @@ enum {
   ZR_DUMP_GRID_F = 9,       /* f64[3*32P] rasterised force (new_v after P2G) */
   ZR_DUMP_GRID_VSTAR = 10,  /* f64[3*32P] new_v after the grid update */
   ZR_DUMP_NODE_IDX = 11,    /* i64[32P] GridState::idx (getNumNodes numbering) */
-  ZR_DUMP_DV = 12           /* f64[3*32P] Newton result dv of the last backward-Euler step */
+  ZR_DUMP_DV = 12,          /* f64[3*32P] Newton result dv of the last backward-Euler step */
+  ZR_DUMP_HVP_XCD = 13      /* u32[20] operator-apply group ranges per XCD (FP32, then FP64;
+                               9 bounds each) of the last backward-Euler step, then the block
+                               counts of the two launches; no reference counterpart: a
+                               scheduling detail of this implementation */
 };
 int64_t zr_debug_size(zr_engine* e, int32_t what);
 int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes);

@@ -7,13 +7,15 @@ matrix-free PCG) against the FP64 oracle.
   iterations, 16 eps CG tolerance) whole steps agree to 1e-9, and the solver
   control flow (iteration counts) is identical;
 - with a tight tolerance the GPU Newton converges quadratically like the
-  oracle.
+  oracle;
+- the operator apply's per-XCD work ranges match a numpy restatement.
 """
 import numpy as np
 import pytest
 
 import oracle
-from ziran2019_amd.engine import implicit_opts
+from ziran2019_amd import scenes
+from ziran2019_amd.engine import MpmEngine, implicit_opts
 from _helpers import make_pair, perturbed, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -126,3 +128,57 @@ def test_fp32_refined_step_matches_fp64_oracle():
     assert rel_err(out["V"], V) < 1e-5
     assert rel_err(out["F"] - np.tile(np.eye(3).reshape(1, 9), (len(F), 1)),
                    F - np.tile(np.eye(3).reshape(1, 9), (len(F), 1))) < 1e-4
+
+
+def _xcd_bounds_numpy(sizes, nblk):
+    """Cost-balanced XCD ranges of the persistent operator apply: cost per
+    group 384 + n + 256 per further chunk of 256, split in proportion to the
+    blocks each XCD receives (block b on XCD b mod 8); the group whose cost
+    interval holds a target closes the range before it."""
+    cost = 384 + sizes + 256 * (np.maximum(sizes - 1, 0) // 256)
+    incl = np.cumsum(cost)
+    total = int(incl[-1])
+    q, r = nblk // 8, nblk % 8
+    out = [0]
+    for x in range(1, 8):
+        t = total * (x * q + min(x, r)) // nblk
+        g = int(np.searchsorted(incl, t, side="left")) if t > 0 else -1
+        out.append(g + 1 if t > 0 else 0)
+    out.append(len(sizes))
+    return np.array(out, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_operator_apply_xcd_ranges(precision):
+    """The persistent operator apply's per-XCD group ranges (k_xcd_split)
+    against a numpy restatement, on 1280 groups of 1..400 particles; the
+    step's result is checked against the oracle elsewhere."""
+    rng = np.random.default_rng(21)
+    dx = 1.0 / 128
+    X = []
+    for px in range(16):
+        for py in range(8):
+            for pz in range(10):
+                k = int(rng.integers(1, 400)) if rng.random() < 0.7 else int(rng.integers(1, 20))
+                lo = np.array([8 + 2 * px, 8 + 4 * py, 8 + 4 * pz], dtype=np.float64)
+                X.append((lo + rng.random((k, 3)) * np.array([2.0, 4.0, 4.0]) + 0.5) * dx)
+    X = np.concatenate(X)
+    n = len(X)
+    s = scenes.cube_scene_small(seed=3, ncells=2, E=1e4)
+    s = dict(s, X=X, V=np.zeros((n, 3)), C=np.zeros((n, 9)), F=np.tile(np.eye(3).reshape(1, 9), (n, 1)),
+             mass=np.full(n, s["mass"][0]), vol=np.full(n, s["vol"][0]), mu=np.full(n, s["mu"][0]),
+             lam=np.full(n, s["lam"][0]), dx=dx)
+    eng = MpmEngine(dx=dx, gravity=(0.0, -9.8, 0.0))
+    eng.upload(s)
+    eng.sortParticlesAndPolluteGrid()
+    eng.particlesToGridApic()
+    eng.backwardEulerStep(DT, implicit_opts(newton_max_iterations=1, precision=precision))
+    first, last = eng.dump("group_first"), eng.dump("group_last")
+    sizes = (last - first + 1).astype(np.int64)  # inclusive last index
+    got = eng.dump("hvp_xcd")
+    assert len(sizes) > 1024
+    for p in range(2):
+        nblk = int(got[18 + p])
+        assert 0 < nblk <= len(sizes)
+        assert np.array_equal(got[9 * p:9 * p + 9], _xcd_bounds_numpy(sizes, nblk)), p
+    eng.close()

@@ -807,7 +807,8 @@ struct zr_engine {
   size_t partial_cap = 0;
   double* sc = nullptr;
   int32_t* ci = nullptr;
-  uint32_t* hvp_ctr = nullptr;  // work counter of the persistent operator apply
+  uint32_t* hvp_ctr = nullptr;     // work counters of the persistent operator apply
+  uint32_t* hvp_bounds = nullptr;  // its XCD group ranges, FP32 then FP64 (2 x 9)
   double* hsc = nullptr;
   int32_t* hci = nullptr;
   // FP32-bulk CG vectors and the operator data in both precisions
@@ -1111,6 +1112,7 @@ int ensure_scalars(zr_engine* e)
     ZR_HIP(e, hipMalloc(&e->ci, CI_N * sizeof(int32_t)));
     ZR_HIP(e, hipMalloc(&e->hvp_ctr, kHvpCounters * kHvpCounterStride * sizeof(uint32_t)));
     ZR_HIP(e, hipMemset(e->hvp_ctr, 0, kHvpCounters * kHvpCounterStride * sizeof(uint32_t)));
+    ZR_HIP(e, hipMalloc(&e->hvp_bounds, 18 * sizeof(uint32_t)));
     ZR_HIP(e, hipHostMalloc(&e->hsc, SC_N * sizeof(double)));
     ZR_HIP(e, hipHostMalloc(&e->hci, CI_N * sizeof(int32_t)));
   }
@@ -1509,7 +1511,8 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, const int32_t*
   const uint32_t resident = e->hvp_resident[sizeof(T) == sizeof(double) ? 1 : 0];
   const uint32_t nblk = e->A < resident ? e->A : resident;
   k_hvp<T><<<nblk, 256, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
-                                        (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci, e->A, e->hvp_ctr);
+                                        (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci,
+                                        e->hvp_bounds + (sizeof(T) == sizeof(double) ? 9 : 0), e->hvp_ctr);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
   k_cg_ap<T><<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
@@ -1585,6 +1588,12 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
     return fail(e, ZR_ERR_STATE, "backwardEulerStep needs the APIC particlesToGrid of this step");
   int rc = ensure_implicit(e, o->precision);
   if (rc) return rc;
+  if (e->A > 0) {  // XCD ranges of the operator apply for this step's groups
+    const uint32_t n32 = e->A < e->hvp_resident[0] ? e->A : e->hvp_resident[0];
+    const uint32_t n64 = e->A < e->hvp_resident[1] ? e->A : e->hvp_resident[1];
+    k_xcd_split<<<1, 1024, 0, e->stream>>>(e->group_start, e->A, n32, n64, e->hvp_bounds);
+    ZR_LAUNCH_CHECK(e);
+  }
   zr_implicit_stats local;
   memset(&local, 0, sizeof(local));
   e->be_dt = dt;
@@ -1683,7 +1692,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
-  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
+  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);
@@ -2339,6 +2348,7 @@ int64_t zr_debug_size(zr_engine* e, int32_t what)
     case ZR_DUMP_GRID_VSTAR: return (int64_t)e->S * 32 * 3 * 8;
     case ZR_DUMP_NODE_IDX: return (int64_t)e->S * 32 * 8;
     case ZR_DUMP_DV: return e->be_ready ? (int64_t)e->S * 32 * 3 * 8 : 0;
+    case ZR_DUMP_HVP_XCD: return e->be_ready ? 20 * 4 : 0;
     default: return -1;
   }
 }
@@ -2403,6 +2413,13 @@ int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes)
     double* o = (double*)host;
     for (uint32_t k = 0; k < nn; ++k)
       for (int d = 0; d < 3; ++d) o[3 * (size_t)k + d] = h[(size_t)d * nn + k];
+    return ZR_OK;
+  }
+  if (what == ZR_DUMP_HVP_XCD) {
+    uint32_t* h = static_cast<uint32_t*>(host);
+    ZR_HIP(e, hipMemcpy(h, e->hvp_bounds, 18 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    h[18] = A < e->hvp_resident[0] ? A : e->hvp_resident[0];
+    h[19] = A < e->hvp_resident[1] ? A : e->hvp_resident[1];
     return ZR_OK;
   }
   if (what == ZR_DUMP_NODE_IDX) {

@@ -580,19 +580,82 @@ __device__ __forceinline__ void hvp_load_frame(const HvpCache<T>& cache, uint32_
 constexpr int kHvpCounterStride = 64;
 constexpr int kHvpCounters = 8;
 
-// the groups of XCD x (block b runs on XCD b mod 8): a contiguous range in
-// proportion to its blocks; its blocks start on the first nbx groups and
-// take the rest from the XCD's counter
+// predicted cost of a group in the operator apply, in particle units: a
+// fixed part (tile, node sums, barriers), one per particle and a phase-2
+// pass per further chunk of 256 particles (fitted to per-group timings of
+// both precisions on MI355X)
+constexpr uint32_t kHvpGroupCost = 384, kHvpChunkCost = 256;
+__device__ __forceinline__ uint32_t hvp_group_cost(uint32_t n)
+{
+  return kHvpGroupCost + n + (n > 0 ? kHvpChunkCost * ((n - 1) / 256) : 0u);
+}
+
+// block b of a persistent launch runs on XCD x = b mod 8 as its l-th block;
+// XCD x owns groups [xb[x], xb[x + 1]) and nbx blocks
 struct XcdGroups {
   uint32_t lo, hi, nbx, l;
 };
-__device__ __forceinline__ XcdGroups xcd_groups(uint32_t b, uint32_t nblocks, uint32_t ngroups)
+__device__ __forceinline__ XcdGroups xcd_groups(uint32_t b, uint32_t nblocks, const uint32_t* xb)
 {
   const uint32_t x = b & 7u, q = nblocks >> 3, r = nblocks & 7u;
-  const uint32_t nbx = q + (x < r ? 1u : 0u);
-  const uint32_t rlo = x * q + (x < r ? x : r);
-  return { (uint32_t)(((uint64_t)ngroups * rlo) / nblocks), (uint32_t)(((uint64_t)ngroups * (rlo + nbx)) / nblocks),
-           nbx, b >> 3 };
+  return { xb[x], xb[x + 1], q + (x < r ? 1u : 0u), b >> 3 };
+}
+
+// XCD ranges of the persistent operator apply, of equal predicted cost in
+// proportion to each XCD's blocks, for nblk32 (FP32) and nblk64 (FP64)
+// blocks: bounds[9 p + x] is the first group of XCD x.  One block of 1024.
+__global__ __launch_bounds__(1024) void k_xcd_split(const uint32_t* __restrict__ group_start, uint32_t A,
+                                                    uint32_t nblk32, uint32_t nblk64, uint32_t* __restrict__ bounds)
+{
+  using u64 = unsigned long long;
+  __shared__ u64 wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  u64 t = 0;
+  for (uint32_t g = tid; g < A; g += 1024) t += hvp_group_cost(group_start[g + 1] - group_start[g]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if (lane == 0) wsum[wid] = t;
+  if (tid < 18) bounds[tid] = (tid % 9 == 8) ? A : 0u;
+  __syncthreads();
+  u64 total = 0;
+  for (int w = 0; w < 16; ++w) total += wsum[w];
+  u64 T[2][7];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const uint32_t nb = p ? nblk64 : nblk32, q = nb >> 3, r = nb & 7u;
+#pragma unroll
+    for (int x = 1; x < 8; ++x) T[p][x - 1] = total * (u64)(x * q + ((uint32_t)x < r ? (uint32_t)x : r)) / nb;
+  }
+  __syncthreads();
+  // the group whose cost interval (excl, incl] holds a target closes the range before it
+  u64 carry = 0;
+  for (uint32_t base = 0; base < A; base += 1024) {
+    const uint32_t g = base + tid;
+    const u64 c = g < A ? hvp_group_cost(group_start[g + 1] - group_start[g]) : 0;
+    u64 v = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const u64 y = __shfl_up(v, off, 64);
+      if (lane >= off) v += y;
+    }
+    if (lane == 63) wsum[wid] = v;
+    __syncthreads();
+    u64 wpre = 0, tile = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wid) wpre += wsum[w];
+      tile += wsum[w];
+    }
+    const u64 incl = carry + wpre + v, excl = incl - c;
+    if (g < A) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int x = 1; x < 8; ++x)
+          if (excl < T[p][x - 1] && T[p][x - 1] <= incl) bounds[9 * p + x] = g + 1;
+    }
+    carry += tile;
+    __syncthreads();
+  }
 }
 
 // requests of the next group gn: bounds, this thread's tile node value(s)
@@ -638,7 +701,7 @@ __device__ __forceinline__ void hvp_issue_next(const HvpCache<T>& cache, const u
 //   phase 3  thread per tile node: sums its <= 18 partials, a contiguous run of
 //            the node-major layout, into the staged tile
 // Persistent blocks, one wave of them resident, dealt to the XCDs as
-// xcd_groups describes; further groups come from the XCD's work counter in
+// xcd_groups describes (ranges from k_xcd_split); further groups come from the XCD's work counter in
 // wctr (zero at launch; the kernel that follows resets them), so that the
 // load balances within an XCD.  The counter is read two groups ahead and
 // the next group's bounds, page slots, tile values and first-chunk cells and
@@ -653,7 +716,7 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
                                              const int32_t* __restrict__ group_nslot, const T* __restrict__ qvec,
                                              const T* __restrict__ pvec, const double* __restrict__ sc,
                                              int64_t gstride, T inv_dx, T* __restrict__ stage,
-                                             const int32_t* __restrict__ ci, uint32_t ngroups,
+                                             const int32_t* __restrict__ ci, const uint32_t* __restrict__ xb,
                                              uint32_t* __restrict__ wctr)
 {
   constexpr int CH = HvpChunk<T>::value;
@@ -661,7 +724,7 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
   static_assert(kTasks > 256 && kTasks <= 512, "two task rounds per block");
   static_assert(kTileNodes <= 256, "one tile node per thread");
   if (ci && ci[CI_DONE]) return;
-  const XcdGroups xg = xcd_groups(blockIdx.x, gridDim.x, ngroups);
+  const XcdGroups xg = xcd_groups(blockIdx.x, gridDim.x, xb);
   uint32_t g = xg.lo + xg.l;
   if (g >= xg.hi) return;
   uint32_t* const xctr = wctr + kHvpCounterStride * (blockIdx.x & 7u);

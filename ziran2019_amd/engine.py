@@ -30,10 +30,11 @@ ZR_STICKY, ZR_SLIP, ZR_SEPARATE = 1, 2, 3
 ZR_HALF_SPACE, ZR_SPHERE = 0, 1
 
 DUMP = dict(sorter=0, order=1, base_offset=2, group_first=3, group_last=4, block_offset=5, pages=6,
-            grid_m=7, grid_v=8, grid_f=9, grid_vstar=10, node_idx=11, dv=12)
+            grid_m=7, grid_v=8, grid_f=9, grid_vstar=10, node_idx=11, dv=12, hvp_xcd=13)
 _DUMP_DTYPE = dict(sorter=np.uint64, order=np.int32, base_offset=np.uint64, group_first=np.int64,
                    group_last=np.int64, block_offset=np.uint64, pages=np.uint64, grid_m=np.float64,
-                   grid_v=np.float64, grid_f=np.float64, grid_vstar=np.float64, node_idx=np.int64, dv=np.float64)
+                   grid_v=np.float64, grid_f=np.float64, grid_vstar=np.float64, node_idx=np.int64, dv=np.float64,
+                   hvp_xcd=np.uint32)
 
 # every symbol include/zr_mpm.h declares
 EXPORTS = (
