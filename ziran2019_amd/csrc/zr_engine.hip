@@ -809,6 +809,8 @@ struct zr_engine {
   int32_t* ci = nullptr;
   uint32_t* hvp_ctr = nullptr;     // work counters of the persistent operator apply
   uint32_t* hvp_bounds = nullptr;  // its XCD group ranges, FP32 then FP64 (2 x 9)
+  uint32_t* hvp_order = nullptr;   // their hand-out orders (2 x hvp_order_cap)
+  size_t hvp_order_cap = 0;
   double* hsc = nullptr;
   int32_t* hci = nullptr;
   // FP32-bulk CG vectors and the operator data in both precisions
@@ -1512,7 +1514,8 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, const int32_t*
   const uint32_t nblk = e->A < resident ? e->A : resident;
   k_hvp<T><<<nblk, 256, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
                                         (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci,
-                                        e->hvp_bounds + (sizeof(T) == sizeof(double) ? 9 : 0), e->hvp_ctr);
+                                        e->hvp_bounds + (sizeof(T) == sizeof(double) ? 9 : 0),
+                                        e->hvp_order + (sizeof(T) == sizeof(double) ? (size_t)e->A : 0), e->hvp_ctr);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
   k_cg_ap<T><<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
@@ -1591,7 +1594,13 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
   if (e->A > 0) {  // XCD ranges of the operator apply for this step's groups
     const uint32_t n32 = e->A < e->hvp_resident[0] ? e->A : e->hvp_resident[0];
     const uint32_t n64 = e->A < e->hvp_resident[1] ? e->A : e->hvp_resident[1];
-    k_xcd_split<<<1, 1024, 0, e->stream>>>(e->group_start, e->A, n32, n64, e->hvp_bounds);
+    if (e->hvp_order_cap < e->A) {
+      dfree(e->hvp_order);
+      const size_t cap = (size_t)e->A + e->A / 4 + 64;
+      ZR_HIP(e, hipMalloc(&e->hvp_order, 2 * cap * sizeof(uint32_t)));
+      e->hvp_order_cap = cap;
+    }
+    k_xcd_split<<<1, 1024, 0, e->stream>>>(e->group_start, e->A, n32, n64, e->hvp_bounds, e->hvp_order);
     ZR_LAUNCH_CHECK(e);
   }
   zr_implicit_stats local;
@@ -1692,7 +1701,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
-  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
+  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);

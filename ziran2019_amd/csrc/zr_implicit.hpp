@@ -603,9 +603,14 @@ __device__ __forceinline__ XcdGroups xcd_groups(uint32_t b, uint32_t nblocks, co
 
 // XCD ranges of the persistent operator apply, of equal predicted cost in
 // proportion to each XCD's blocks, for nblk32 (FP32) and nblk64 (FP64)
-// blocks: bounds[9 p + x] is the first group of XCD x.  One block of 1024.
+// blocks: bounds[9 p + x] is the first group of XCD x; and the order in
+// which each range is handed out, largest groups first (by 16-particle
+// bucket; the order within a bucket is arbitrary and does not affect the
+// result, every group writing its own staged tile): perm[p A + k].  One
+// block of 1024.
 __global__ __launch_bounds__(1024) void k_xcd_split(const uint32_t* __restrict__ group_start, uint32_t A,
-                                                    uint32_t nblk32, uint32_t nblk64, uint32_t* __restrict__ bounds)
+                                                    uint32_t nblk32, uint32_t nblk64, uint32_t* __restrict__ bounds,
+                                                    uint32_t* __restrict__ perm)
 {
   using u64 = unsigned long long;
   __shared__ u64 wsum[16];
@@ -656,6 +661,37 @@ __global__ __launch_bounds__(1024) void k_xcd_split(const uint32_t* __restrict__
     carry += tile;
     __syncthreads();
   }
+  // largest-first order within each range: counting sort on (range, 31 - size bucket)
+  __shared__ uint32_t cnt[256];
+  __shared__ uint32_t xb[9];
+  for (int p = 0; p < 2; ++p) {
+    if (tid < 256) cnt[tid] = 0u;
+    if (tid < 9) xb[tid] = bounds[9 * p + tid];
+    __syncthreads();
+    for (uint32_t g = tid; g < A; g += 1024) {
+      uint32_t x = 0;
+      while (x < 7u && g >= xb[x + 1]) ++x;
+      const uint32_t nsz = group_start[g + 1] - group_start[g], bk = nsz / 16u < 31u ? nsz / 16u : 31u;
+      atomicAdd(&cnt[x * 32u + (31u - bk)], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t run = 0;
+      for (int k = 0; k < 256; ++k) {
+        const uint32_t c = cnt[k];
+        cnt[k] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+    for (uint32_t g = tid; g < A; g += 1024) {
+      uint32_t x = 0;
+      while (x < 7u && g >= xb[x + 1]) ++x;
+      const uint32_t nsz = group_start[g + 1] - group_start[g], bk = nsz / 16u < 31u ? nsz / 16u : 31u;
+      perm[(size_t)p * A + atomicAdd(&cnt[x * 32u + (31u - bk)], 1u)] = g;
+    }
+    __syncthreads();
+  }
 }
 
 // requests of the next group gn: bounds, this thread's tile node value(s)
@@ -701,9 +737,10 @@ __device__ __forceinline__ void hvp_issue_next(const HvpCache<T>& cache, const u
 //   phase 3  thread per tile node: sums its <= 18 partials, a contiguous run of
 //            the node-major layout, into the staged tile
 // Persistent blocks, one wave of them resident, dealt to the XCDs as
-// xcd_groups describes (ranges from k_xcd_split); further groups come from the XCD's work counter in
-// wctr (zero at launch; the kernel that follows resets them), so that the
-// load balances within an XCD.  The counter is read two groups ahead and
+// xcd_groups describes (ranges and hand-out order from k_xcd_split); further
+// positions in the XCD's order come from its work counter in wctr (zero at
+// launch; the kernel that follows resets them), so that the load balances
+// within an XCD.  The counter is read three groups ahead, the order two and
 // the next group's bounds, page slots, tile values and first-chunk cells and
 // fractions one group ahead, so that they are in flight while this group's
 // phases 2 and 3 run.  A particle's singular frame
@@ -717,7 +754,7 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
                                              const T* __restrict__ pvec, const double* __restrict__ sc,
                                              int64_t gstride, T inv_dx, T* __restrict__ stage,
                                              const int32_t* __restrict__ ci, const uint32_t* __restrict__ xb,
-                                             uint32_t* __restrict__ wctr)
+                                             const uint32_t* __restrict__ order, uint32_t* __restrict__ wctr)
 {
   constexpr int CH = HvpChunk<T>::value;
   constexpr int kTasks = kCells * kAB;
@@ -725,8 +762,8 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
   static_assert(kTileNodes <= 256, "one tile node per thread");
   if (ci && ci[CI_DONE]) return;
   const XcdGroups xg = xcd_groups(blockIdx.x, gridDim.x, xb);
-  uint32_t g = xg.lo + xg.l;
-  if (g >= xg.hi) return;
+  if (xg.lo + xg.l >= xg.hi) return;
+  uint32_t g = order[xg.lo + xg.l];
   uint32_t* const xctr = wctr + kHvpCounterStride * (blockIdx.x & 7u);
   __shared__ uint32_t next_group;
   __shared__ T tv[3 * kTileNodes];
@@ -739,9 +776,15 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
   const T beta = qvec ? (T)sc[SC_BETA] : (T)0;
   const T inv_dx2 = inv_dx * inv_dx;
   const bool tnode = tid < kTileNodes;
-  // thread 0: the group after the next one, from the work counter
-  uint32_t ahead = 0;
-  if (tid == 0) ahead = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
+  // thread 0's pipeline over the XCD's hand-out order: the next group's
+  // number (read a group ago) and the position after it (counted a group
+  // ago); ~0u past the range
+  uint32_t ahead_g = ~0u, ahead_k = ~0u;
+  if (tid == 0) {
+    const uint32_t k1 = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
+    ahead_g = k1 < xg.hi ? order[k1] : ~0u;
+    if (k1 < xg.hi) ahead_k = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
+  }
 
   // prologue: the first group's tile and first-chunk particle data
   uint32_t start = group_start[g];
@@ -775,8 +818,9 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
     // the next group (read a group ago), visible to the block after this
     // group's first barrier; and the one after it
     if (tid == 0) {
-      next_group = ahead;
-      if (ahead < xg.hi) ahead = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
+      next_group = ahead_g;
+      ahead_g = ahead_k < xg.hi ? order[ahead_k] : ~0u;
+      if (ahead_k < xg.hi) ahead_k = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
     }
     uint32_t gn = 0, nstart = 0, nn = 0;
     bool more = false;
@@ -893,7 +937,7 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
         // last chunk: the next group's data are requested here and consumed
         // after phase 3
         gn = next_group;
-        more = gn < xg.hi;
+        more = gn != ~0u;
         if (more) hvp_issue_next(cache, group_start, group_nslot, qvec, pvec, gstride, gn, tnode, nstart, nn, nsl,
                                  P1, tq, tp);
         issued = true;
@@ -954,7 +998,7 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
     }
     if (!issued) {  // an empty group (no chunk ran; its zeroing barrier passed)
       gn = next_group;
-      more = gn < xg.hi;
+      more = gn != ~0u;
       if (more) hvp_issue_next(cache, group_start, group_nslot, qvec, pvec, gstride, gn, tnode, nstart, nn, nsl,
                                P1, tq, tp);
     }
