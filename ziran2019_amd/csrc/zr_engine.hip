@@ -1512,7 +1512,7 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, const int32_t*
   prof_mark(e, ZR_PROF_HVP);
   const uint32_t resident = e->hvp_resident[sizeof(T) == sizeof(double) ? 1 : 0];
   const uint32_t nblk = e->A < resident ? e->A : resident;
-  k_hvp<T><<<nblk, 256, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
+  k_hvp<T><<<nblk, HvpChunk<T>::threads, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
                                         (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci,
                                         e->hvp_bounds + (sizeof(T) == sizeof(double) ? 9 : 0),
                                         e->hvp_order + (sizeof(T) == sizeof(double) ? (size_t)e->A : 0), e->hvp_ctr);
@@ -1658,8 +1658,10 @@ int zr_create(const zr_config* cfg, zr_engine** out)
     int dev = 0, ncu = 0, b32 = 0, b64 = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b32, k_hvp<float>, 256, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b64, k_hvp<double>, 256, 0) != hipSuccess || ncu <= 0 ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b32, k_hvp<float>, HvpChunk<float>::threads, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b64, k_hvp<double>, HvpChunk<double>::threads, 0) !=
+            hipSuccess ||
+        ncu <= 0 ||
         b32 <= 0 || b64 <= 0) {
       (void)hipStreamDestroy(e->stream);
       delete e;

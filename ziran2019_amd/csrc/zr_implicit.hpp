@@ -516,12 +516,14 @@ struct HvpChunk;
 template <>
 struct HvpChunk<float> {
   static constexpr int value = 256;
+  static constexpr int threads = 256;
   static constexpr int waves = 4;
   static constexpr bool frame_early = true;  // singular frame requested before the gather
 };
 template <>
 struct HvpChunk<double> {
   static constexpr int value = 256;
+  static constexpr int threads = 256;
   static constexpr int waves = 2;
   static constexpr bool frame_early = false;  // after it: FP64 registers
 };
@@ -749,7 +751,7 @@ __device__ __forceinline__ void hvp_issue_next(const HvpCache<T>& cache, const u
 // qvec == nullptr: applies the operator to pvec itself; otherwise to the CG
 // direction qvec + beta pvec (beta from sc)
 template <class T>
-__global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cache, const uint32_t* __restrict__ group_start,
+__global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cache, const uint32_t* __restrict__ group_start,
                                              const int32_t* __restrict__ group_nslot, const T* __restrict__ qvec,
                                              const T* __restrict__ pvec, const double* __restrict__ sc,
                                              int64_t gstride, T inv_dx, T* __restrict__ stage,
@@ -758,8 +760,9 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
 {
   constexpr int CH = HvpChunk<T>::value;
   constexpr int kTasks = kCells * kAB;
-  static_assert(kTasks > 256 && kTasks <= 512, "two task rounds per block");
-  static_assert(kTileNodes <= 256, "one tile node per thread");
+  constexpr int NT = HvpChunk<T>::threads;
+  static_assert(kTasks <= 2 * NT && CH <= NT, "at most two task rounds, one particle per thread");
+  static_assert(kTileNodes <= NT, "one tile node per thread");
   if (ci && ci[CI_DONE]) return;
   const XcdGroups xg = xcd_groups(blockIdx.x, gridDim.x, xb);
   if (xg.lo + xg.l >= xg.hi) return;
@@ -829,7 +832,7 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
     T tq[3], tp[3];  // read only where loaded (more && nsl >= 0)
     bool issued = false;
     if (n == 0) {  // an empty group stages zeros
-      for (int k = tid; k < 3 * kTasks * 3; k += 256) (&contrib[0][0])[k] = (T)0;
+      for (int k = tid; k < 3 * kTasks * 3; k += NT) (&contrib[0][0])[k] = (T)0;
       __syncthreads();
     }
     for (uint32_t c0 = 0; c0 < n; c0 += CH) {
@@ -942,7 +945,7 @@ __global__ __launch_bounds__(256, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cac
                                  P1, tq, tp);
         issued = true;
       }
-      for (int task = tid; task < kTasks; task += 256) {
+      for (int task = tid; task < kTasks; task += NT) {
         const int e = task / kAB, ab = task - e * kAB;
         const int a = ab / 3, bb = ab - 3 * a;
         T acc[3][3];
