@@ -36,6 +36,12 @@ namespace {
 constexpr int64_t kMaxParticles = (int64_t(1) << 29) - 1;
 constexpr int kTileX = 4, kTileY = 6, kTileZ = 6;
 constexpr int kTileNodes = kTileX * kTileY * kTileZ;  // 144
+constexpr int kTransferThreads = 256;                  // block of the P2G transfers
+// the APIC G2P (FP64 gradients of the weights, 164 registers) keeps more
+// groups in flight per CU with blocks of two waves (measured: 194 -> 145 us
+// on ~1M particles); the MLS G2P is fastest at four
+template <bool MLS>
+constexpr int g2p_threads() { return MLS ? 256 : 128; }
 constexpr int kP2GFields = 7;                          // m, mv[3], f[3]
 constexpr int kStageStride = kP2GFields * kTileNodes;  // doubles per group
 constexpr int kMaxColliders = 8;
@@ -358,17 +364,18 @@ __device__ __forceinline__ void particle_kirchhoff(const ParticlesView& P, uint3
 // FORCE = true: MLS with force (explicit).  FORCE = false: APIC mass and
 // momentum only, particlesToGridHelper<APIC> (MpmSimulationBase.cpp:563-608).
 template <bool FORCE>
-__global__ __launch_bounds__(256) void k_p2g(ParticlesView P, const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const uint32_t* __restrict__ order,
                                              const uint32_t* __restrict__ group_start,
                                              const uint32_t* __restrict__ group_page, double* __restrict__ stage,
                                              double dx, double inv_dx, double Dinv)
 {
   constexpr int NF = FORCE ? kP2GFields : 4;
   constexpr int SS = NF * kTileNodes;
-  __shared__ double tile[4 * SS];
+  constexpr int NT = kTransferThreads, NW = NT / 64;
+  __shared__ double tile[NW * SS];
   const int tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
-  for (int k = tid; k < 4 * SS; k += 256) tile[k] = 0.0;
+  for (int k = tid; k < NW * SS; k += NT) tile[k] = 0.0;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
   uint32_t px, py, pz;
@@ -376,11 +383,11 @@ __global__ __launch_bounds__(256) void k_p2g(ParticlesView P, const uint32_t* __
   const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
   __syncthreads();
   double* wt = tile + (tid >> 6) * SS;
-  // lanes of a wave take particles 8 apart so that they mostly sit in
+  // lanes of a wave take particles NT/32 apart so that they mostly sit in
   // different cells and their LDS adds rarely collide
-  const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
+  const uint32_t perm = (uint32_t)((tid & 31) * (NT / 32) + (tid >> 5));
   const int64_t S = P.stride;
-  for (uint32_t j0 = 0; j0 < n; j0 += 256) {
+  for (uint32_t j0 = 0; j0 < n; j0 += NT) {
     const uint32_t j = j0 + perm;
     if (j >= n) continue;
     const uint32_t i = order[start + j];
@@ -440,7 +447,12 @@ __global__ __launch_bounds__(256) void k_p2g(ParticlesView P, const uint32_t* __
   }
   __syncthreads();
   double* out = stage + (size_t)g * SS;
-  for (int k = tid; k < SS; k += 256) out[k] = tile[k] + tile[SS + k] + tile[2 * SS + k] + tile[3 * SS + k];
+  for (int k = tid; k < SS; k += NT) {
+    double acc = tile[k];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) acc += tile[w * SS + k];
+    out[k] = acc;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -532,7 +544,7 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
 // MLS: grad v = C.  APIC (implicit path): grad v = sum v* grad w^T
 // (MpmSimulationBase.cpp:941-956 with !USE_MLS_MPM).
 template <bool MLS>
-__global__ __launch_bounds__(256) void k_g2p(ParticlesView P, const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(g2p_threads<MLS>()) void k_g2p(ParticlesView P, const uint32_t* __restrict__ order,
                                                  const uint32_t* __restrict__ group_start,
                                                  const uint32_t* __restrict__ group_page,
                                                  const int32_t* __restrict__ page_slot, const double* __restrict__ gvs,
@@ -549,7 +561,8 @@ __global__ __launch_bounds__(256) void k_g2p(ParticlesView P, const uint32_t* __
   page_decode(group_page[g], px, py, pz);
   if (tid < 8) nslot[tid] = page_slot[page_encode(px + (tid >> 2), py + ((tid >> 1) & 1), pz + (tid & 1))];
   __syncthreads();
-  for (int t = tid; t < kTileNodes; t += 256) {
+  constexpr int NT = g2p_threads<MLS>();
+  for (int t = tid; t < kTileNodes; t += NT) {
     const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
     const int32_t sl = nslot[((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2)];
     const int64_t node = (int64_t)sl * 32 + elem_of(tx, ty, tz);
@@ -560,7 +573,7 @@ __global__ __launch_bounds__(256) void k_g2p(ParticlesView P, const uint32_t* __
   const int64_t S = P.stride;
   const double dx2 = dx * dx;
   uint32_t flags = 0;
-  for (uint32_t j = tid; j < n; j += 256) {
+  for (uint32_t j = tid; j < n; j += NT) {
     const uint32_t i = order[start + j];
     double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
     double w[3][3], dw[3][3];
@@ -1042,10 +1055,10 @@ int do_p2g(zr_engine* e, bool mls)
   const double dx = e->cfg.dx;
   prof_mark(e, ZR_PROF_P2G);
   if (mls)
-    k_p2g<true><<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
+    k_p2g<true><<<e->A, kTransferThreads, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
                                              1.0 / dx, 4.0 / (dx * dx));
   else
-    k_p2g<false><<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
+    k_p2g<false><<<e->A, kTransferThreads, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
                                               1.0 / dx, 4.0 / (dx * dx));
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_P2G);
@@ -1083,11 +1096,11 @@ int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
   ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_CFL, 0, sizeof(uint32_t), e->stream));
   prof_mark(e, ZR_PROF_G2P);
   if (mls)
-    k_g2p<true><<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot, e->gvs,
+    k_g2p<true><<<e->A, g2p_threads<true>(), 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot, e->gvs,
                                              (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt, e->cfg.cfl,
                                              e->dcount);
   else
-    k_g2p<false><<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot,
+    k_g2p<false><<<e->A, g2p_threads<false>(), 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot,
                                               e->gvs, (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt,
                                               e->cfg.cfl, e->dcount);
   ZR_LAUNCH_CHECK(e);
