@@ -34,6 +34,8 @@ using namespace zr;
 namespace {
 
 constexpr int64_t kMaxParticles = (int64_t(1) << 29) - 1;
+// blocks of the grid-stride node kernels of the CG loop (one partial each)
+constexpr uint32_t kCgBlocks = 1024;
 constexpr int kTileX = 4, kTileY = 6, kTileZ = 6;
 constexpr int kTileNodes = kTileX * kTileY * kTileZ;  // 144
 constexpr int kTransferThreads = 256;                  // block of the P2G transfers
@@ -818,6 +820,7 @@ struct zr_engine {
   size_t imp_cap = 0;
   double* partial = nullptr;
   size_t partial_cap = 0;
+  double* partial2 = nullptr;  // second set of block partials (z^T r of the CG update)
   double* sc = nullptr;
   int32_t* ci = nullptr;
   uint32_t* hvp_ctr = nullptr;     // work counters of the persistent operator apply
@@ -1422,6 +1425,7 @@ int ensure_implicit(zr_engine* e, int32_t prec)
     ZR_HIP(e, hipMalloc(&e->partial, nb * sizeof(double)));
     e->partial_cap = nb;
   }
+  if (!e->partial2) ZR_HIP(e, hipMalloc(&e->partial2, kCgBlocks * sizeof(double)));
   int rc = ensure_scalars(e);
   if (rc) return rc;
   const size_t n = (size_t)(e->n > 0 ? e->n : 1);
@@ -1515,11 +1519,13 @@ int be_state_residual(zr_engine* e)
 
 // out = A p, or A (q + beta p) with p updated to that direction when q is
 // given (the CG step); projected when requested; the CG flags gate it
+// cg_iter >= 1: the CG iteration whose direction is qvec + beta pvec, beta
+// from the z^T r partials of the previous update (e->partial2); 0 otherwise
 template <class T>
-int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, const int32_t* ci, bool cg)
+int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, int32_t* ci, int cg_iter)
 {
   const double dx = e->cfg.dx, dt = e->be_dt;
-  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
+  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256), ncg = nb < kCgBlocks ? nb : kCgBlocks;
   ColliderSet cs = collider_set(e);
   T* stage = reinterpret_cast<T*>(e->stage);
   prof_mark(e, ZR_PROF_HVP);
@@ -1528,13 +1534,13 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, const int32_t*
   k_hvp<T><<<nblk, HvpChunk<T>::threads, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
                                         (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci,
                                         e->hvp_bounds + (sizeof(T) == sizeof(double) ? 9 : 0),
-                                        e->hvp_order + (sizeof(T) == sizeof(double) ? (size_t)e->A : 0), e->hvp_ctr);
+                                        e->hvp_order + (sizeof(T) == sizeof(double) ? (size_t)e->A : 0), e->hvp_ctr,
+                                        e->partial2, ncg, cg_iter, e->sc);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
-  k_cg_ap<T><<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
-                                        nvT(e, out), dx, dt, e->time, cs, project, e->partial, e->sc, ci,
-                                        e->hvp_ctr);
-  if (cg) k_reduce_cg<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, e->ci, RED_ALPHA, 0, 0.0, 0.0);
+  k_cg_ap<T><<<ncg, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
+                                         nvT(e, out), dx, dt, e->time, cs, project, e->partial, e->sc, ci,
+                                         e->hvp_ctr);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
 }
@@ -1554,18 +1560,32 @@ int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
   k_reduce_cg<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci, RED_CG_INIT, 0, rel, o->cg_tolerance);
   ZR_LAUNCH_CHECK(e);
   const int check = o->cg_check_interval > 0 ? o->cg_check_interval : 8;
-  for (int k = 0; k < o->cg_max_iterations; ++k) {
-    int rc = be_apply<T>(e, V::q(e), V::p(e), V::ap(e), 1, e->ci, true);
+  const uint32_t ncg = nb < kCgBlocks ? nb : kCgBlocks;
+  // per iteration: the operator apply (which first closes the previous
+  // update: beta, count, termination), A p with p^T A p partials, and the
+  // update (alpha from those partials) with z^T r partials
+  int k = 0;
+  for (; k < o->cg_max_iterations; ++k) {
+    int rc = be_apply<T>(e, V::q(e), V::p(e), V::ap(e), 1, e->ci, k);
     if (rc) return rc;
-    k_cg_update<T><<<nb, 256, 0, s>>>(nn, e->gm, nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
-                                      nvT(e, V::p(e)), nvT(e, V::ap(e)), e->sc, e->partial, e->ci);
-    k_reduce_cg<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci, RED_BETA, 0, 0.0, 0.0);
+    k_cg_update<T><<<ncg, 256, 0, s>>>(nn, e->gm, nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
+                                       nvT(e, V::p(e)), nvT(e, V::ap(e)), e->sc, e->partial, ncg, e->partial2,
+                                       e->ci, k);
     ZR_LAUNCH_CHECK(e);
     if ((k + 1) % check == 0 && k + 1 < o->cg_max_iterations) {
       ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
       ZR_HIP(e, hipStreamSynchronize(s));
-      if (e->hci[CI_DONE]) break;
+      if (e->hci[CI_DONE]) {
+        ++k;
+        break;
+      }
     }
+  }
+  // closes the last update when the loop ran out of iterations (a no-op
+  // once the solve has terminated)
+  if (k > 0) {
+    k_reduce_cg<<<1, 256, 0, s>>>(e->partial2, ncg, e->sc, e->ci, RED_FINISH, k - 1, 0.0, 0.0);
+    ZR_LAUNCH_CHECK(e);
   }
   return ZR_OK;
 }
@@ -1716,7 +1736,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
-  dfree(e->partial); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
+  dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);
@@ -2341,7 +2361,7 @@ int zr_implicit_multiply(zr_engine* e, const double* x, double* b)
       for (int d = 0; d < 3; ++d) soa[d * nn + k] = hf[3 * k + d];
     for (int d = 0; d < 3; ++d)
       ZR_HIP(e, hipMemcpy(e->fp + d * e->grid_cap, soa.data() + d * nn, nn * sizeof(float), hipMemcpyHostToDevice));
-    rc = be_apply<float>(e, nullptr, e->fp, e->fap, 0, nullptr, false);
+    rc = be_apply<float>(e, nullptr, e->fp, e->fap, 0, nullptr, 0);
     if (rc) return rc;
     ZR_HIP(e, hipStreamSynchronize(e->stream));
     for (int d = 0; d < 3; ++d)
@@ -2350,7 +2370,7 @@ int zr_implicit_multiply(zr_engine* e, const double* x, double* b)
       for (int d = 0; d < 3; ++d) b[3 * k + d] = (double)soa[d * nn + k];
     return ZR_OK;
   }
-  rc = be_apply<double>(e, nullptr, e->ip, e->iap, 0, nullptr, false);
+  rc = be_apply<double>(e, nullptr, e->ip, e->iap, 0, nullptr, 0);
   if (rc) return rc;
   return download_nodevec(e, e->iap, b);
 }

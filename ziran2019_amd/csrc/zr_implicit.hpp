@@ -21,14 +21,17 @@
 
 enum {
   SC_NORM2 = 0,   // Newton residual sum |r|^2 / m
-  SC_ZTR = 1,     // CG z^T r
+  SC_ZTR = 1,     // CG z^T r of even iterations
   SC_PAP = 2,     // CG p^T A p
-  SC_ZTR_NEW = 3,
+  SC_ZTR_NEW = 3, // CG z^T r of odd iterations
   SC_BETA = 4,
   SC_LOCAL_TOL = 5,
   SC_ALPHA = 6,
   SC_N = 16
 };
+// z_k^T r_k lives in alternating slots so that the blocks of one kernel can
+// read the previous value while block 0 stores the new one
+__device__ __forceinline__ int ztr_slot(int k) { return (k & 1) ? SC_ZTR_NEW : SC_ZTR; }
 enum { CI_DONE = 0, CI_ITERS = 1, CI_N = 4 };
 
 constexpr int kImpFields = 3;
@@ -84,6 +87,15 @@ __device__ __forceinline__ double thread_partials256(const double* __restrict__ 
   for (int j = 0; j < 8; ++j)
     if (b + 256u * j < nb) a[j] += partial[b + 256u * j];
   return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
+// the ordered sum of nb block partials (blockDim 256), on every thread
+__device__ __forceinline__ double block_total256(const double* __restrict__ partial, uint32_t nb, double* smem5)
+{
+  const double s = block_sum256(thread_partials256(partial, nb), smem5);
+  if (threadIdx.x == 0) smem5[4] = s;
+  __syncthreads();
+  return smem5[4];
 }
 
 // block partial of a grid-wide sum; the partials are added in index order
@@ -753,10 +765,12 @@ __device__ __forceinline__ void hvp_issue_next(const HvpCache<T>& cache, const u
 template <class T>
 __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hvp(HvpCache<T> cache, const uint32_t* __restrict__ group_start,
                                              const int32_t* __restrict__ group_nslot, const T* __restrict__ qvec,
-                                             const T* __restrict__ pvec, const double* __restrict__ sc,
+                                             const T* __restrict__ pvec, const double* sc,
                                              int64_t gstride, T inv_dx, T* __restrict__ stage,
-                                             const int32_t* __restrict__ ci, const uint32_t* __restrict__ xb,
-                                             const uint32_t* __restrict__ order, uint32_t* __restrict__ wctr)
+                                             int32_t* __restrict__ ci, const uint32_t* __restrict__ xb,
+                                             const uint32_t* __restrict__ order, uint32_t* __restrict__ wctr,
+                                             const double* __restrict__ upd_partial, uint32_t n_upd, int cg_iter,
+                                             double* __restrict__ scw)
 {
   constexpr int CH = HvpChunk<T>::value;
   constexpr int kTasks = kCells * kAB;
@@ -776,7 +790,24 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
   __shared__ T contrib[3][kTasks * 3];
   __shared__ int cbeg[kCells], cend[kCells];
   const int tid = threadIdx.x;
-  const T beta = qvec ? (T)sc[SC_BETA] : (T)0;
+  T beta = (T)0;
+  if (qvec && cg_iter > 0) {
+    // ConjugateGradient.h:136-143 for the update of iteration cg_iter - 1:
+    // z^T r from its block partials, beta, the count and the termination
+    // test, alike in every block; block 0 records them
+    __shared__ double red[5];
+    const double ztr = block_total256(upd_partial, n_upd, red);
+    const double b64 = ztr / sc[ztr_slot(cg_iter - 1)];
+    const bool done = sqrt(ztr) < sc[SC_LOCAL_TOL];
+    if (blockIdx.x == 0 && tid == 0) {
+      scw[ztr_slot(cg_iter)] = ztr;
+      scw[SC_BETA] = b64;
+      ci[CI_ITERS] += 1;
+      if (done) ci[CI_DONE] = 1;
+    }
+    if (done) return;
+    beta = (T)b64;
+  }
   const T inv_dx2 = inv_dx * inv_dx;
   const bool tnode = tid < kTileNodes;
   // thread 0's pipeline over the XCD's hand-out order: the next group's
@@ -1144,13 +1175,12 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
   // the operator apply this kernel follows has drained its work counters
   if (blockIdx.x == 0 && threadIdx.x < kHvpCounters) hvp_ctr[kHvpCounterStride * threadIdx.x] = 0u;
   if (ci && ci[CI_DONE]) return;
-  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   double contrib = 0.0;
-  if (k < nn) {
+  const T beta = q.p ? (T)sc[SC_BETA] : (T)0;
+  for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < nn; k += gridDim.x * 256u) {
     const double m = gm[k];
     T pv[3];
     if (q.p) {
-      const T beta = (T)sc[SC_BETA];
       for (int d = 0; d < 3; ++d) {
         pv[d] = q(d, k) + beta * p(d, k);
         p(d, k) = pv[d];
@@ -1180,18 +1210,28 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
   block_partial(contrib, partial);
 }
 
-// x += alpha p, r -= alpha Ap, z = r / m; beta = z^T r / z^T r_old, the
-// iteration count and the termination test (ConjugateGradient.h:121-143)
+// x += alpha p, r -= alpha Ap, z = r / m (ConjugateGradient.h:127-135) with
+// alpha = z^T r / p^T A p, p^T A p summed from its block partials in every
+// block; this kernel's z^T r partials go to upd_partial (grid-stride, one
+// partial per block)
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __restrict__ gm, NodeVec<T> x,
                                                    NodeVec<T> r, NodeVec<T> q, NodeVec<T> p, NodeVec<T> ap,
-                                                   const double* sc, double* __restrict__ partial, const int32_t* ci)
+                                                   double* __restrict__ sc, const double* __restrict__ pap_partial,
+                                                   uint32_t n_pap, double* __restrict__ upd_partial,
+                                                   const int32_t* ci, int cg_iter)
 {
   if (ci[CI_DONE]) return;
-  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  __shared__ double red[5];
+  const double pap = block_total256(pap_partial, n_pap, red);
+  const double a64 = sc[ztr_slot(cg_iter)] / pap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[SC_PAP] = pap;
+    sc[SC_ALPHA] = a64;
+  }
+  const T alpha = (T)a64;
   double contrib = 0.0;
-  if (k < nn) {
-    const T alpha = (T)sc[SC_ALPHA];
+  for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < nn; k += gridDim.x * 256u) {
     const double m = gm[k];
     for (int d = 0; d < 3; ++d) {
       x(d, k) += p(d, k) * alpha;
@@ -1202,11 +1242,11 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
       contrib += (double)qv * (double)rv;
     }
   }
-  block_partial(contrib, partial);
+  block_partial(contrib, upd_partial);
 }
 
 // one-block reductions of the partials with the CG scalar updates
-enum { RED_STORE = 0, RED_CG_INIT = 1, RED_ALPHA = 2, RED_BETA = 3 };
+enum { RED_STORE = 0, RED_CG_INIT = 1, RED_FINISH = 2 };
 __global__ __launch_bounds__(256) void k_reduce_cg(const double* __restrict__ partial, uint32_t nb, double* sc,
                                                    int32_t* ci, int mode, int slot, double rel, double tol)
 {
@@ -1224,12 +1264,8 @@ __global__ __launch_bounds__(256) void k_reduce_cg(const double* __restrict__ pa
     sc[SC_BETA] = 0.0;
     sc[SC_LOCAL_TOL] = lt;
     ci[CI_DONE] = norm < lt ? 1 : 0;
-  } else if (mode == RED_ALPHA) {  // ConjugateGradient.h:129
-    sc[SC_PAP] = v;
-    sc[SC_ALPHA] = sc[SC_ZTR] / v;
-  } else {  // ConjugateGradient.h:136-143
-    sc[SC_BETA] = v / sc[SC_ZTR];
-    sc[SC_ZTR] = v;
+  } else {  // ConjugateGradient.h:136-143 for the last update of a solve (slot = its iteration)
+    sc[ztr_slot(slot + 1)] = v;
     ci[CI_ITERS] += 1;
     if (sqrt(v) < sc[SC_LOCAL_TOL]) ci[CI_DONE] = 1;
   }
