@@ -4,7 +4,7 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload implicit-boxes|e
 
 One step = MpmSimulationBase::advanceOneTimeStep on the whole particle set,
 inputs resident in HBM:
-  implicit-boxes (default; the metric's workload): 64 seeded boxes, ~0.97M
+  implicit-boxes (default; the metric's workload): 64 seeded boxes, ~0.98M
     particles at dx = 1/256, dt = 2e-3, backward Euler with Newton (tol 1e-5,
     <= 3 iterations) and mass-preconditioned CG (relative tolerance 1e-6)
   explicit-cube: 140,608 particles, dx = 1/128, dt = 1e-4, explicit MLS
