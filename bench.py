@@ -32,11 +32,17 @@ def algorithmic_bytes(kernel, n, g):
 
     n particles, g active grid nodes (m != 0), FP64 (8 B) values, 4 B index:
       P2G : reads x3 v3 C9 F9 m1 V0 1 mu1 lambda1 (28 values) + index,
-            writes m, mv3, f3 per node (7 values)
-      G2P : reads x3 F9 + index and v*3 per node, writes x3 v3 C9 F9
+            writes m, mv3, f3 per node (7 values); the APIC P2G of the
+            implicit path reads 16 values and writes 4 per node
+      G2P : (MLS and APIC alike)
+            reads x3 F9 + index and v*3 per node, writes x3 v3 C9 F9
     """
     if kernel == "p2g":
         return n * (28 * F64 + 4) + g * 7 * F64
+    if kernel == "p2g_apic":
+        # APIC P2G of the implicit path (no stress): reads x3 v3 C9 m1 (16 values) + index,
+        # writes m, mv3 per node
+        return n * (16 * F64 + 4) + g * 4 * F64
     if kernel == "g2p":
         return n * (36 * F64 + 4) + g * 3 * F64
     if kernel == "hvp":
@@ -144,7 +150,7 @@ def run_explicit(args):
                 "FP64 plain path")
     base = None if args.no_cpu_baseline else cpu_baseline(scene, dt, args.cpu_seconds, min(16, os.cpu_count() or 1))
     return dict(n=n, dt=dt, dx=scene["dx"], elapsed=t1 - t0, kt=kt, g_nodes=g_nodes, roof_kernels=("p2g", "g2p"),
-                workload=workload, base=base, extra={}, dtype="f64")
+                bw_kernels=dict(p2g="p2g", g2p="g2p"), workload=workload, base=base, extra={}, dtype="f64")
 
 
 def _implicit_run(scene, opts, steps, warmup, profile):
@@ -209,7 +215,9 @@ def run_implicit(args):
                  speedup_vs_fp64=(n * args.steps / r32["elapsed"]) / fp64_value,
                  fp32_vs_fp64_error=err)
     return dict(n=n, dt=dt, dx=scene["dx"], elapsed=r32["elapsed"], kt=r32["kt"], g_nodes=g_nodes,
-                roof_kernels=("hvp", "state"), workload=workload, base=base, extra=extra, dtype="f32-bulk/f64-refined")
+                roof_kernels=("hvp", "state"),
+                bw_kernels=dict(hvp="hvp", state="state", p2g="p2g_apic", g2p="g2p"),
+                workload=workload, base=base, extra=extra, dtype="f32-bulk/f64-refined")
 
 
 def copy_calibration(nbytes=256 << 20, launches=5):
@@ -252,12 +260,12 @@ def main():
     n, kt, g_nodes = r["n"], r["kt"], r["g_nodes"]
 
     kern = {}
-    for k in r["roof_kernels"]:
+    for k, kind in r["bw_kernels"].items():
         ms, launches = kt[k]
         avg_s = ms / 1e3 / max(launches, 1)
-        b = algorithmic_bytes(k, n, g_nodes)
+        b = algorithmic_bytes(kind, n, g_nodes)
         kern[k] = dict(avg_us=avg_s * 1e6, bytes=b, gbs=b / avg_s / 1e9)
-    dom = max(kern, key=lambda k: kern[k]["avg_us"] * kt[k][1])
+    dom = max(r["roof_kernels"], key=lambda k: kern[k]["avg_us"] * kt[k][1])
     roof = dict(bound="hbm", kernel=dom, achieved=round(kern[dom]["gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(kern[dom]["gbs"] / HBM_PEAK_GBS, 4), traffic=TRAFFIC.get((args.workload, dom)),
                 bytes_per_launch=kern[dom]["bytes"], avg_launch_us=round(kern[dom]["avg_us"], 2))
@@ -265,6 +273,7 @@ def main():
                for k, v in kt.items()}
     for k in kern:
         kernels[k]["gbs"] = round(kern[k]["gbs"], 2)
+        kernels[k]["hbm_frac"] = round(kern[k]["gbs"] / HBM_PEAK_GBS, 4)
 
     elapsed = r["elapsed"]
     out = dict(
