@@ -200,3 +200,61 @@ def test_full_size_cube_properties():
     v = eng.dump("grid_v")
     mom = (m[:, None] * v).sum(0)
     assert np.allclose(mom, (s["mass"][:, None] * out["V"]).sum(0), rtol=1e-10, atol=1e-15)
+
+
+def test_single_particle_steps():
+    """One particle: sort, transfers and ten steps against the oracle."""
+    s = perturbed(9, shuffle=False)
+    one = {k: (np.ascontiguousarray(np.asarray(v)[:1]) if isinstance(v, np.ndarray) and len(v) == len(s["mass"])
+               else v) for k, v in s.items()}
+    eng, ora = make_pair(one)
+    for _ in range(10):
+        eng.advanceOneTimeStep(1e-4)
+        ora.step(1e-4)
+    out = eng.download()
+    X, V, C, F = ora.get_particles()
+    assert rel_err(out["X"], X) < 1e-13
+    assert rel_err(out["V"], V) < 1e-12
+    assert rel_err(out["C"], C) < 1e-10
+    assert rel_err(out["F"], F) < 1e-13
+
+
+def test_particles_at_domain_bounds():
+    """Base nodes at 0 and at the engine's upper bound 2^L - 8: sort keys and
+    groups bit-exact, a step against the oracle."""
+    from ziran2019_amd.engine import grid_log2_for
+    dx = 1.0 / 64
+    lim = (1 << grid_log2_for(dx)) - 8  # the engine's default grid for dx
+    rng = np.random.default_rng(10)
+    lo = (0.5 + rng.random((40, 3)) * 0.999) * dx            # base node 0
+    hi = (lim + 0.5 + rng.random((40, 3)) * 0.999) * dx      # base node lim
+    X = np.concatenate([lo, hi])
+    n = len(X)
+    s = dict(X=X, V=0.1 * rng.normal(size=(n, 3)), C=0.1 * rng.normal(size=(n, 9)),
+             F=np.tile(np.eye(3).reshape(1, 9), (n, 1)), mass=np.full(n, 1e-3), vol=np.full(n, 1e-6),
+             mu=np.full(n, 3e3), lam=np.full(n, 5e3), dx=dx)
+    eng, ora = make_pair(s, gravity=(0.0, 0.0, 0.0))
+    eng.sortParticlesAndPolluteGrid()
+    ora.sort_and_pollute()
+    info = ora.sort_result()
+    assert np.array_equal(eng.dump("sorter"), info["sorter"])
+    assert np.array_equal(eng.dump("order"), info["order"])
+    eng2, ora2 = make_pair(s, gravity=(0.0, 0.0, 0.0))
+    eng2.advanceOneTimeStep(1e-4)
+    ora2.step(1e-4)
+    out = eng2.download()
+    X1, V1, C1, F1 = ora2.get_particles()
+    assert rel_err(out["X"], X1) < 1e-13
+    assert rel_err(out["V"], V1) < 1e-11
+    eng.close()
+    eng2.close()
+
+
+def test_empty_upload_is_a_state_error():
+    """No particles: the step reports a state error instead of launching."""
+    eng = MpmEngine(dx=1.0 / 64)
+    eng.upload(dict(X=np.zeros((0, 3)), V=np.zeros((0, 3)), C=np.zeros((0, 9)), F=np.zeros((0, 9)),
+                    mass=np.zeros(0), vol=np.zeros(0), mu=np.zeros(0), lam=np.zeros(0), dx=1.0 / 64))
+    with pytest.raises(ZrError):
+        eng.advanceOneTimeStep(1e-4)
+    eng.close()

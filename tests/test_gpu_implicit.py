@@ -182,3 +182,67 @@ def test_operator_apply_xcd_ranges(precision):
         assert 0 < nblk <= len(sizes)
         assert np.array_equal(got[9 * p:9 * p + 9], _xcd_bounds_numpy(sizes, nblk)), p
     eng.close()
+
+
+def _dense_page_scene(seed=16, extra=700):
+    """The perturbed cube plus ``extra`` particles in one SPGrid page (base
+    nodes in a 2x4x4 block), so that its group holds several chunks of the
+    operator apply (more than 256 particles)."""
+    s = perturbed(seed, n_cells=5, F_noise=0.04, C_noise=0.2, E=2e4)
+    rng = np.random.default_rng(seed)
+    dx = s["dx"]
+    base = np.floor(s["X"] / dx - 0.5).astype(np.int64)
+    page = base // np.array([2, 4, 4])
+    pages, counts = np.unique(page, axis=0, return_counts=True)
+    p = pages[np.argmax(counts)]
+    lo = p * np.array([2, 4, 4]) + 0.5
+    Xe = (lo[None, :] + rng.random((extra, 3)) * np.array([2.0, 4.0, 4.0]) * (1 - 1e-9)) * dx
+    src = rng.integers(0, len(s["mass"]), extra)
+    out = dict(s)
+    out["X"] = np.ascontiguousarray(np.concatenate([s["X"], Xe]))
+    for k in ("V", "C", "F", "mass", "vol", "mu", "lam"):
+        out[k] = np.ascontiguousarray(np.concatenate([np.asarray(s[k]), np.asarray(s[k])[src]]))
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_operator_dense_page_parity(precision):
+    """Groups of several chunks (the operator apply's multi-chunk path,
+    > 256 particles in one page) against the oracle's operator."""
+    s = _dense_page_scene()
+    eng, ora = make_pair(s)
+    eng.sortParticlesAndPolluteGrid()
+    first, last = eng.dump("group_first"), eng.dump("group_last")
+    assert (last - first + 1).max() > 512
+    eng.particlesToGridApic()
+    eng.backwardEulerStep(DT, implicit_opts(newton_max_iterations=1, precision=precision))
+    ora.sort_and_pollute()
+    ora.p2g_apic()
+    ora.backward_euler(DT, oracle.implicit_opts(newton_max_iterations=1))
+    active = ora.grid()["idx"] >= 0
+    rng = np.random.default_rng(5)
+    # both sides linearised at the same dv (their Newton iterates may differ
+    # by a CG iteration on this scene)
+    dv = 0.01 * rng.normal(size=(len(active), 3))
+    dv[~active] = 0.0
+    assert rel_err(eng.implicit_residual(dv), ora.implicit_residual(dv)) < 1e-11
+    x = rng.normal(size=(len(active), 3))
+    x[~active] = 0.0
+    tol = 1e-11 if precision == 0 else 2e-5
+    assert rel_err(eng.implicit_multiply(x), ora.implicit_multiply(x)) < tol
+
+
+def test_fp32_refined_step_dense_page():
+    """Two FP32-refined steps with a multi-chunk group against the FP64 oracle."""
+    s = _dense_page_scene(seed=17)
+    kw = dict(newton_max_iterations=5, newton_tolerance=1e-9, cg_max_iterations=400, cg_tolerance=1e30,
+              cg_relative_tolerance=1e-6)
+    eng, ora = make_pair(s)
+    for _ in range(2):
+        eng.advanceOneTimeStepImplicit(DT, implicit_opts(precision=1, **kw))
+        ora.step_implicit(DT, oracle.implicit_opts(**kw))
+    out = eng.download()
+    X, V, C, F = ora.get_particles()
+    disp = np.abs(X - s["X"]).max()
+    assert np.abs(out["X"] - X).max() / disp < 1e-5
+    assert rel_err(out["V"], V) < 1e-5
