@@ -826,6 +826,8 @@ struct zr_engine {
   uint32_t* hvp_ctr = nullptr;     // work counters of the persistent operator apply
   uint32_t* hvp_bounds = nullptr;  // its XCD group ranges, FP32 then FP64 (2 x 9)
   uint32_t* hvp_order = nullptr;   // their hand-out orders (2 x hvp_order_cap)
+  uint8_t* cell_order = nullptr;   // each group's cells by particle count (32 x hvp_order_cap)
+  bool cell_order_set = false;     // written for this sort's groups
   size_t hvp_order_cap = 0;
   double* hsc = nullptr;
   int32_t* hci = nullptr;
@@ -1013,6 +1015,7 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   rc = sync_counts(e);
   if (rc) return rc;
   e->A = e->hcount[CNT_GROUPS];
+  e->cell_order_set = false;
   e->S = 0;
   if (e->hcount[CNT_DOMAIN]) {
     // tables hold clamped entries; keep A so the next sort clears them
@@ -1506,8 +1509,9 @@ int be_state_residual(zr_engine* e)
   prof_mark(e, ZR_PROF_STATE);
   k_be_state<T><<<e->A, 256, 0, e->stream>>>(pview(e), hvp_cache<T>(e), e->order, e->group_start, e->group_page,
                                              e->page_slot, e->ivn, e->idv, (int64_t)e->grid_cap, dx, 1.0 / dx, dt,
-                                             e->stage);
+                                             e->stage, e->cell_order_set ? nullptr : e->cell_order);
   ZR_LAUNCH_CHECK(e);
+  e->cell_order_set = true;  // the cells do not move within a step
   prof_mark(e, ZR_PROF_STATE);
   k_be_residual<<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, e->stage, e->gm, nv(e, e->idv),
                                            nv(e, e->ir), dx, dt, e->cfg.gravity[0], e->cfg.gravity[1],
@@ -1535,7 +1539,7 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, int32_t* ci, i
                                         (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci,
                                         e->hvp_bounds + (sizeof(T) == sizeof(double) ? 9 : 0),
                                         e->hvp_order + (sizeof(T) == sizeof(double) ? (size_t)e->A : 0), e->hvp_ctr,
-                                        e->partial2, ncg, cg_iter, e->sc);
+                                        e->partial2, ncg, cg_iter, e->sc, e->cell_order);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
   k_cg_ap<T><<<ncg, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
@@ -1629,8 +1633,11 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
     const uint32_t n64 = e->A < e->hvp_resident[1] ? e->A : e->hvp_resident[1];
     if (e->hvp_order_cap < e->A) {
       dfree(e->hvp_order);
+      dfree(e->cell_order);
+      e->hvp_order_cap = 0;
       const size_t cap = (size_t)e->A + e->A / 4 + 64;
       ZR_HIP(e, hipMalloc(&e->hvp_order, 2 * cap * sizeof(uint32_t)));
+      ZR_HIP(e, hipMalloc(&e->cell_order, cap * kCells));
       e->hvp_order_cap = cap;
     }
     k_xcd_split<<<1, 1024, 0, e->stream>>>(e->group_start, e->A, n32, n64, e->hvp_bounds, e->hvp_order);
@@ -1736,7 +1743,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
-  dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
+  dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);

@@ -308,10 +308,15 @@ __device__ __forceinline__ T wgradT(const T w[3][3], const T dw[3][3], T inv_dx,
 __device__ __forceinline__ void opaque(double& v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void opaque(float& v) { asm volatile("" : "+v"(v)); }
 
+constexpr int kCells = 32;  // 2 x 4 x 4 cells of a page
+
 // Newton state: F = (I + dt grad(vn + dv)) Fn (FP64, with the SVD), the
 // stress term vPFnT grad w staged per group (MpmForceBase.cpp:305-344,
 // FBasedMpmForceHelper.cpp:68-91, rasterizeForceToTVStack<false>), and the
-// operator data of every particle in sorted order, in precision T
+// operator data of every particle in sorted order, in precision T; unless
+// cell_order is null, with the cells of each group's first 256 particles by
+// particle count, largest first (ties by cell), in cell_order[32 g ..] for
+// the operator apply's task deal
 template <class T>
 __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> cache,
                                                   const uint32_t* __restrict__ order,
@@ -319,14 +324,17 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
                                                   const uint32_t* __restrict__ group_page,
                                                   const int32_t* __restrict__ page_slot, const double* __restrict__ vn,
                                                   const double* __restrict__ dv, int64_t gstride, double dx,
-                                                  double inv_dx, double dt, double* __restrict__ stage)
+                                                  double inv_dx, double dt, double* __restrict__ stage,
+                                                  uint8_t* __restrict__ cell_order)
 {
   __shared__ double tile[4 * kImpStage];
   __shared__ double tv[3 * kTileNodes];
   __shared__ int32_t nslot[8];
+  __shared__ int ccount[kCells];
   const int tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
   for (int k = tid; k < 4 * kImpStage; k += 256) tile[k] = 0.0;
+  if (tid < kCells) ccount[tid] = 0;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
   uint32_t px, py, pz;
@@ -353,6 +361,7 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
     }
     const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
     if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
+    if (cell_order && j0 == 0) atomicAdd(&ccount[lx | (ly << 1) | (lz << 3)], 1);
     const int t0 = (lx * kTileY + ly) * kTileZ + lz;
     double G[3][3] = { { 0.0 } };
 #pragma unroll
@@ -448,9 +457,18 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
   double* out = stage + (size_t)g * kImpStage;
   for (int k = tid; k < kImpStage; k += 256)
     out[k] = tile[k] + tile[kImpStage + k] + tile[2 * kImpStage + k] + tile[3 * kImpStage + k];
+  if (cell_order && tid < kCells) {
+    const int cnt = ccount[tid];
+    int rank = 0;
+#pragma unroll
+    for (int e2 = 0; e2 < kCells; ++e2) {
+      const int c2 = ccount[e2];
+      rank += (c2 > cnt || (c2 == cnt && e2 < tid)) ? 1 : 0;
+    }
+    cell_order[(size_t)g * kCells + rank] = (uint8_t)tid;
+  }
 }
 
-constexpr int kCells = 32;    // 2 x 4 x 4 cells of a page
 constexpr int kAB = 9;        // (a, b) stencil offsets; c is handled inside a task
 
 // Node-major layout of the per-(cell, a, b, c) partial sums of the operator
@@ -708,17 +726,18 @@ __global__ __launch_bounds__(1024) void k_xcd_split(const uint32_t* __restrict__
   }
 }
 
-// requests of the next group gn: bounds, this thread's tile node value(s)
-// and first-chunk particle (cells and fractions)
+// requests of the next group gn: bounds, this thread's tile node value(s),
+// first-chunk particle (cells and fractions) and cell order entry
 template <class T>
 __device__ __forceinline__ void hvp_issue_next(const HvpCache<T>& cache, const uint32_t* __restrict__ group_start,
                                                const int32_t* __restrict__ group_nslot, const T* __restrict__ qvec,
                                                const T* __restrict__ pvec, int64_t gstride, uint32_t gn, bool tnode,
-                                               uint32_t& nstart, uint32_t& nn, int32_t& nsl, HvpParticle<T>& P1,
-                                               T tq[3], T tp[3])
+                                               const uint8_t* __restrict__ cell_order, uint32_t& nstart, uint32_t& nn,
+                                               int32_t& nsl, int& nord, HvpParticle<T>& P1, T tq[3], T tp[3])
 {
   constexpr int CH = HvpChunk<T>::value;
   const int tid = threadIdx.x;
+  if (tid < kCells) nord = cell_order[(size_t)gn * kCells + tid];
   nstart = group_start[gn];
   nn = group_start[gn + 1] - nstart;
   const uint32_t m1 = nn < (uint32_t)CH ? nn : (uint32_t)CH;
@@ -747,7 +766,9 @@ __device__ __forceinline__ void hvp_issue_next(const HvpCache<T>& cache, const u
 //   phase 2  task per (cell, a, b): sums ds grad w over the cell's particles
 //            in index order for the three c offsets (separable form
 //            gw = (dw_a w_b w_c, w_a dw_b w_c, w_a w_b dw_c) / dx) into its
-//            three entries of the node-major layout
+//            three entries of the node-major layout; the tasks go to the
+//            lanes by the cell order of k_be_state (cells with most
+//            particles first), so that a wave's lanes loop about as often
 //   phase 3  thread per tile node: sums its <= 18 partials, a contiguous run of
 //            the node-major layout, into the staged tile
 // Persistent blocks, one wave of them resident, dealt to the XCDs as
@@ -770,7 +791,7 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
                                              int32_t* __restrict__ ci, const uint32_t* __restrict__ xb,
                                              const uint32_t* __restrict__ order, uint32_t* __restrict__ wctr,
                                              const double* __restrict__ upd_partial, uint32_t n_upd, int cg_iter,
-                                             double* __restrict__ scw)
+                                             double* __restrict__ scw, const uint8_t* __restrict__ cell_order)
 {
   constexpr int CH = HvpChunk<T>::value;
   constexpr int kTasks = kCells * kAB;
@@ -789,6 +810,7 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
   __shared__ T pds[27][CH + 1];
   __shared__ T contrib[3][kTasks * 3];
   __shared__ int cbeg[kCells], cend[kCells];
+  __shared__ int cell_rank[kCells];  // the group's cells in task-deal order
   const int tid = threadIdx.x;
   T beta = (T)0;
   if (qvec && cg_iter > 0) {
@@ -844,6 +866,7 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
     if (tid < kCells) {
       cbeg[tid] = CH;
       cend[tid] = 0;
+      cell_rank[tid] = cell_order[(size_t)g * kCells + tid];
     }
     __syncthreads();
   }
@@ -859,6 +882,7 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
     uint32_t gn = 0, nstart = 0, nn = 0;
     bool more = false;
     int32_t nsl = -1;
+    int nord = 0;
     HvpParticle<T> P1;
     T tq[3], tp[3];  // read only where loaded (more && nsl >= 0)
     bool issued = false;
@@ -972,12 +996,13 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
         // after phase 3
         gn = next_group;
         more = gn != ~0u;
-        if (more) hvp_issue_next(cache, group_start, group_nslot, qvec, pvec, gstride, gn, tnode, nstart, nn, nsl,
-                                 P1, tq, tp);
+        if (more) hvp_issue_next(cache, group_start, group_nslot, qvec, pvec, gstride, gn, tnode, cell_order, nstart,
+                                 nn, nsl, nord, P1, tq, tp);
         issued = true;
       }
-      for (int task = tid; task < kTasks; task += NT) {
-        const int e = task / kAB, ab = task - e * kAB;
+      for (int k = tid; k < kTasks; k += NT) {
+        const int rk = k / kAB, ab = k - rk * kAB;
+        const int e = cell_rank[rk], task = e * kAB + ab;
         const int a = ab / 3, bb = ab - 3 * a;
         T acc[3][3];
 #pragma unroll
@@ -1033,8 +1058,8 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
     if (!issued) {  // an empty group (no chunk ran; its zeroing barrier passed)
       gn = next_group;
       more = gn != ~0u;
-      if (more) hvp_issue_next(cache, group_start, group_nslot, qvec, pvec, gstride, gn, tnode, nstart, nn, nsl,
-                               P1, tq, tp);
+      if (more) hvp_issue_next(cache, group_start, group_nslot, qvec, pvec, gstride, gn, tnode, cell_order, nstart,
+                               nn, nsl, nord, P1, tq, tp);
     }
     // phase 3: each tile node sums its run of partials in (a, b, c) order
     if (tnode) {
@@ -1061,6 +1086,7 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
     if (tid < kCells) {
       cbeg[tid] = CH;
       cend[tid] = 0;
+      cell_rank[tid] = nord;
     }
     __syncthreads();
     g = gn;
