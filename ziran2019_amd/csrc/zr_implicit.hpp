@@ -187,6 +187,39 @@ __device__ __forceinline__ void sum_tiles_src(const T* stage, const int32_t* src
   }
 }
 
+// the same with the page's 8 source groups already in registers
+template <int NF, class T>
+__device__ __forceinline__ void sum_tiles_src8(const T* stage, const int32_t src8[8], int ex, int ey, int ez,
+                                               T acc[NF])
+{
+  constexpr int stride = NF * kTileNodes;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) acc[f] = (T)0;
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    const int a = o >> 2, b = (o >> 1) & 1, c = o & 1;
+    const int tx = ex + 2 * a, ty = ey + 4 * b, tz = ez + 4 * c;
+    if (ty >= kTileY || tz >= kTileZ) continue;
+    if (src8[o] < 0) continue;
+    const T* base = stage + (size_t)src8[o] * stride + (tx * kTileY + ty) * kTileZ + tz;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] += base[f * kTileNodes];
+  }
+}
+__device__ __forceinline__ void load_src8(const int32_t* __restrict__ slot_src, uint32_t k, int32_t src8[8])
+{
+  const int4* v = reinterpret_cast<const int4*>(slot_src + 8 * (size_t)(k >> 5));
+  const int4 lo = v[0], hi = v[1];
+  src8[0] = lo.x;
+  src8[1] = lo.y;
+  src8[2] = lo.z;
+  src8[3] = lo.w;
+  src8[4] = hi.x;
+  src8[5] = hi.y;
+  src8[6] = hi.z;
+  src8[7] = hi.w;
+}
+
 struct NodeCoord {
   uint32_t qx, qy, qz;
   int ex, ey, ez;
@@ -798,7 +831,6 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
   constexpr int NT = HvpChunk<T>::threads;
   static_assert(kTasks <= 2 * NT && CH <= NT, "at most two task rounds, one particle per thread");
   static_assert(kTileNodes <= NT, "one tile node per thread");
-  if (ci && ci[CI_DONE]) return;
   const XcdGroups xg = xcd_groups(blockIdx.x, gridDim.x, xb);
   if (xg.lo + xg.l >= xg.hi) return;
   uint32_t g = order[xg.lo + xg.l];
@@ -812,6 +844,49 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
   __shared__ int cbeg[kCells], cend[kCells];
   __shared__ int cell_rank[kCells];  // the group's cells in task-deal order
   const int tid = threadIdx.x;
+  const T inv_dx2 = inv_dx * inv_dx;
+  const bool tnode = tid < kTileNodes;
+  // thread 0's pipeline over the XCD's hand-out order: the next group's
+  // number (read a group ago) and the position after it (counted a group
+  // ago); ~0u past the range
+  uint32_t ahead_g = ~0u, ahead_k = ~0u;
+  if (tid == 0) {
+    const uint32_t k1 = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
+    ahead_g = k1 < xg.hi ? order[k1] : ~0u;
+    if (k1 < xg.hi) ahead_k = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
+  }
+
+  // prologue: the first group's tile and first-chunk particle data,
+  // requested before beta is summed (the work counters this touches on a
+  // finished solve are reset by the kernel that follows)
+  uint32_t start = group_start[g];
+  uint32_t n = group_start[g + 1] - start;
+  // this thread's particle of the current chunk
+  HvpParticle<T> P;
+  int32_t sl0 = -1;
+  T tq0[3], tp0[3];  // read only where loaded (sl0 >= 0)
+  {
+    const uint32_t m0 = n < (uint32_t)CH ? n : (uint32_t)CH;
+    if ((uint32_t)tid < m0) {
+      hvp_load(cache, start + tid, tid, m0, P);
+    }
+    if (tnode) {
+      int tsrc, telem;
+      tile_node_source(tid, tsrc, telem);
+      sl0 = group_nslot[8 * (size_t)g + tsrc];
+      if (sl0 >= 0) {
+        const int64_t node = (int64_t)sl0 * 32 + telem;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          tp0[d] = pvec[d * gstride + node];
+          if (qvec) tq0[d] = qvec[d * gstride + node];
+        }
+      }
+    }
+  }
+  // a finished solve's launches stop here (its flag is read beside the
+  // requests above rather than before them)
+  if (ci && ci[CI_DONE]) return;
   T beta = (T)0;
   if (qvec && cg_iter > 0) {
     // ConjugateGradient.h:136-143 for the update of iteration cg_iter - 1:
@@ -830,46 +905,16 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
     if (done) return;
     beta = (T)b64;
   }
-  const T inv_dx2 = inv_dx * inv_dx;
-  const bool tnode = tid < kTileNodes;
-  // thread 0's pipeline over the XCD's hand-out order: the next group's
-  // number (read a group ago) and the position after it (counted a group
-  // ago); ~0u past the range
-  uint32_t ahead_g = ~0u, ahead_k = ~0u;
-  if (tid == 0) {
-    const uint32_t k1 = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
-    ahead_g = k1 < xg.hi ? order[k1] : ~0u;
-    if (k1 < xg.hi) ahead_k = xg.lo + xg.nbx + atomicAdd(xctr, 1u);
-  }
-
-  // prologue: the first group's tile and first-chunk particle data
-  uint32_t start = group_start[g];
-  uint32_t n = group_start[g + 1] - start;
-  // this thread's particle of the current chunk
-  HvpParticle<T> P;
-  {
-    const uint32_t m0 = n < (uint32_t)CH ? n : (uint32_t)CH;
-    if ((uint32_t)tid < m0) {
-      hvp_load(cache, start + tid, tid, m0, P);
-    }
-    if (tnode) {
-      int tsrc, telem;
-      tile_node_source(tid, tsrc, telem);
-      const int32_t sl = group_nslot[8 * (size_t)g + tsrc];
-      const int64_t node = (int64_t)sl * 32 + telem;
+  if (tnode) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        tv[d * kTileNodes + tid] =
-            sl >= 0 ? (qvec ? qvec[d * gstride + node] + beta * pvec[d * gstride + node] : pvec[d * gstride + node])
-                    : (T)0;
-    }
-    if (tid < kCells) {
-      cbeg[tid] = CH;
-      cend[tid] = 0;
-      cell_rank[tid] = cell_order[(size_t)g * kCells + tid];
-    }
-    __syncthreads();
+    for (int d = 0; d < 3; ++d) tv[d * kTileNodes + tid] = sl0 >= 0 ? (qvec ? tq0[d] + beta * tp0[d] : tp0[d]) : (T)0;
   }
+  if (tid < kCells) {
+    cbeg[tid] = CH;
+    cend[tid] = 0;
+    cell_rank[tid] = cell_order[(size_t)g * kCells + tid];
+  }
+  __syncthreads();
 
   for (;;) {
     // the next group (read a group ago), visible to the block after this
@@ -1200,30 +1245,47 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
 {
   // the operator apply this kernel follows has drained its work counters
   if (blockIdx.x == 0 && threadIdx.x < kHvpCounters) hvp_ctr[kHvpCounterStride * threadIdx.x] = 0u;
+  // the first node's operands and its page's source groups are requested
+  // before the flag and beta are known, so that the staged sums are one
+  // dependent request away from the launch
+  const uint32_t k0 = blockIdx.x * 256u + threadIdx.x, kstep = gridDim.x * 256u;
+  double m = 0.0;
+  int32_t src8[8];
+  T qv[3], pv0[3];
+  auto fetch = [&](uint32_t k) {
+    m = gm[k];
+    load_src8(slot_src, k, src8);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pv0[d] = p(d, k);
+      if (q.p) qv[d] = q(d, k);
+    }
+  };
+  if (k0 < nn) fetch(k0);
   if (ci && ci[CI_DONE]) return;
   double contrib = 0.0;
   const T beta = q.p ? (T)sc[SC_BETA] : (T)0;
-  for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < nn; k += gridDim.x * 256u) {
-    const double m = gm[k];
+  for (uint32_t k = k0; k < nn; k += kstep) {
+    if (k != k0) fetch(k);
     T pv[3];
     if (q.p) {
       for (int d = 0; d < 3; ++d) {
-        pv[d] = q(d, k) + beta * p(d, k);
+        pv[d] = qv[d] + beta * pv0[d];
         p(d, k) = pv[d];
       }
     } else {
-      for (int d = 0; d < 3; ++d) pv[d] = p(d, k);
+      for (int d = 0; d < 3; ++d) pv[d] = pv0[d];
     }
     T av[3] = { (T)0, (T)0, (T)0 };
     if (m != 0.0) {
-      const NodeCoord nc = node_coord(slot_page, k);
+      const uint32_t e = k & 31u;
       T f[kImpFields];
-      sum_tiles_src<kImpFields>(stage, slot_src + 8 * (size_t)(k >> 5), nc.ex, nc.ey, nc.ez, f);
+      sum_tiles_src8<kImpFields>(stage, src8, (int)(e >> 4), (int)((e >> 2) & 3u), (int)(e & 3u), f);
       const T dt2 = (T)(dt * dt);
       for (int d = 0; d < 3; ++d) av[d] = (T)m * pv[d] + dt2 * f[d];
       if (project) {
         double xi[3], a64[3] = { (double)av[0], (double)av[1], (double)av[2] };
-        node_pos(nc, dx, xi);
+        node_pos(node_coord(slot_page, k), dx, xi);
         project_node(cs, time, xi, a64);
         for (int d = 0; d < 3; ++d) av[d] = (T)a64[d];
       }
@@ -1247,6 +1309,22 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
                                                    uint32_t n_pap, double* __restrict__ upd_partial,
                                                    const int32_t* ci, int cg_iter)
 {
+  // this thread's first node's operands are requested before alpha is
+  // known (they do not depend on it), so that the two waits overlap
+  const uint32_t k0 = blockIdx.x * 256u + threadIdx.x, kstep = gridDim.x * 256u;
+  double m = 0.0;
+  T xv[3], pv[3], rv[3], av[3];
+  auto fetch = [&](uint32_t k) {
+    m = gm[k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xv[d] = x(d, k);
+      pv[d] = p(d, k);
+      rv[d] = r(d, k);
+      av[d] = ap(d, k);
+    }
+  };
+  if (k0 < nn) fetch(k0);
   if (ci[CI_DONE]) return;
   __shared__ double red[5];
   const double pap = block_total256(pap_partial, n_pap, red);
@@ -1257,15 +1335,16 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
   }
   const T alpha = (T)a64;
   double contrib = 0.0;
-  for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < nn; k += gridDim.x * 256u) {
-    const double m = gm[k];
+  for (uint32_t k = k0; k < nn; k += kstep) {
+    if (k != k0) fetch(k);
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-      x(d, k) += p(d, k) * alpha;
-      const T rv = r(d, k) - ap(d, k) * alpha;
-      r(d, k) = rv;
-      const T qv = m != 0.0 ? rv / (T)m : (T)0;
+      x(d, k) = xv[d] + pv[d] * alpha;
+      const T rn = rv[d] - av[d] * alpha;
+      r(d, k) = rn;
+      const T qv = m != 0.0 ? rn / (T)m : (T)0;
       q(d, k) = qv;
-      contrib += (double)qv * (double)rv;
+      contrib += (double)qv * (double)rn;
     }
   }
   block_partial(contrib, upd_partial);
