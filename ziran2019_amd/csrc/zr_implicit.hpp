@@ -1248,6 +1248,9 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
   // the first node's operands and its page's source groups are requested
   // before the flag and beta are known, so that the staged sums are one
   // dependent request away from the launch
+  // A node without mass holds zeros in every CG vector (the start sets
+  // them so and nothing adds to them), so its direction and A p are not
+  // touched in a solve; the plain product (no q) still writes its zero A p.
   const uint32_t k0 = blockIdx.x * 256u + threadIdx.x, kstep = gridDim.x * 256u;
   double m = 0.0;
   int32_t src8[8];
@@ -1255,10 +1258,12 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
   auto fetch = [&](uint32_t k) {
     m = gm[k];
     load_src8(slot_src, k, src8);
+    if (m != 0.0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      pv0[d] = p(d, k);
-      if (q.p) qv[d] = q(d, k);
+      for (int d = 0; d < 3; ++d) {
+        pv0[d] = p(d, k);
+        if (q.p) qv[d] = q(d, k);
+      }
     }
   };
   if (k0 < nn) fetch(k0);
@@ -1267,6 +1272,11 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
   const T beta = q.p ? (T)sc[SC_BETA] : (T)0;
   for (uint32_t k = k0; k < nn; k += kstep) {
     if (k != k0) fetch(k);
+    if (m == 0.0) {
+      if (!q.p)
+        for (int d = 0; d < 3; ++d) ap(d, k) = (T)0;
+      continue;
+    }
     T pv[3];
     if (q.p) {
       for (int d = 0; d < 3; ++d) {
@@ -1277,7 +1287,7 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
       for (int d = 0; d < 3; ++d) pv[d] = pv0[d];
     }
     T av[3] = { (T)0, (T)0, (T)0 };
-    if (m != 0.0) {
+    {
       const uint32_t e = k & 31u;
       T f[kImpFields];
       sum_tiles_src8<kImpFields>(stage, src8, (int)(e >> 4), (int)((e >> 2) & 3u), (int)(e & 3u), f);
@@ -1314,14 +1324,17 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
   const uint32_t k0 = blockIdx.x * 256u + threadIdx.x, kstep = gridDim.x * 256u;
   double m = 0.0;
   T xv[3], pv[3], rv[3], av[3];
+  // a node without mass keeps its zeros (see k_cg_ap)
   auto fetch = [&](uint32_t k) {
     m = gm[k];
+    if (m != 0.0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      xv[d] = x(d, k);
-      pv[d] = p(d, k);
-      rv[d] = r(d, k);
-      av[d] = ap(d, k);
+      for (int d = 0; d < 3; ++d) {
+        xv[d] = x(d, k);
+        pv[d] = p(d, k);
+        rv[d] = r(d, k);
+        av[d] = ap(d, k);
+      }
     }
   };
   if (k0 < nn) fetch(k0);
@@ -1337,6 +1350,7 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
   double contrib = 0.0;
   for (uint32_t k = k0; k < nn; k += kstep) {
     if (k != k0) fetch(k);
+    if (m == 0.0) continue;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       x(d, k) = xv[d] + pv[d] * alpha;
