@@ -62,8 +62,8 @@ def algorithmic_bytes(kernel, n, g):
 # 256 MiB copy) and WRITE_SIZE (x 1.0) on one MI355X for these workloads
 # (profiles/hbm_traffic_counters.md); None where no counter run exists
 TRAFFIC = {
-    ("implicit-boxes", "hvp"): 136.2e6,
-    ("implicit-boxes", "hvp64"): 269.0e6,
+    ("implicit-boxes", "hvp"): 137.3e6,
+    ("implicit-boxes", "hvp64"): 269.9e6,
     ("explicit-cube", "p2g"): 42.8e6,
     ("explicit-cube", "g2p"): 48.9e6,
 }
