@@ -233,6 +233,28 @@ def copy_calibration(nbytes=256 << 20, launches=5):
     print(json.dumps(dict(copy_bytes_each_way=nbytes, launches=launches)))
 
 
+def calculate_dt_timing(calls):
+    """calculateDt (evalMaxParticleSpeed: |v| and the position bounds, one
+    reduction over the particles) on the implicit-boxes particles, ``calls``
+    synchronous calls; kernel durations come from a rocprofv3 trace of this
+    command, the wall time per call includes the host round trip."""
+    from ziran2019_amd import scenes
+    from ziran2019_amd.engine import MpmEngine
+    scene = scenes.boxes_scene()
+    n = len(scene["mass"])
+    eng = MpmEngine(dx=scene["dx"], gravity=scene["gravity"])
+    eng.upload(scene)
+    eng.set_colliders(scene["colliders"])
+    dt = eng.calculateDt(2e-3)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        dt = eng.calculateDt(2e-3)
+    t1 = time.perf_counter()
+    eng.close()
+    print(json.dumps(dict(workload="calculateDt on the implicit-boxes particles", n_particles=n, calls=calls,
+                          dt=dt, wall_us_per_call=(t1 - t0) / calls * 1e6, algorithmic_bytes=n * 6 * F64)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,9 +265,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (5 launches) to scale HBM counters")
+    ap.add_argument("--calculate-dt", type=int, default=0, metavar="CALLS",
+                    help="only time CALLS calls of calculateDt on the implicit-boxes particles")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.calculate_dt > 0:
+        calculate_dt_timing(args.calculate_dt)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine has no multi-GPU path; "

@@ -258,6 +258,18 @@ int zr_grid_to_particles(zr_engine* e, double dt, zr_step_stats* st);
    (MpmSimulationBase.cpp:256-281); advances the engine time by dt */
 int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st);
 
+/* MpmSimulationBase::evalMaxParticleSpeed (MpmSimulationBase.cpp:1143-1170)
+   on the engine's particles: out[0] = max |v| (Eigen norm), out[1..3] = max
+   corner, out[4..6] = min corner of the positions.  With no particles:
+   0, -FLT_MAX (x3), FLT_MAX (x3), as the reference's reduction start. */
+int zr_eval_max_particle_speed(zr_engine* e, double out[7]);
+/* MpmSimulationBase::calculateDt (MpmSimulationBase.cpp:766-790): cfl dx /
+   the larger of the particles' max speed and the colliders' speed (constant
+   translations: |velocity|, AnalyticCollisionObject::evalMaxSpeed,
+   CollisionObject.cpp:158-194), or max_dt (the reference's step.max_dt)
+   when both are 0.  The caller applies TimeStepping::nextDt. */
+int zr_calculate_dt(zr_engine* e, double max_dt, double* dt);
+
 /* particlesToGrid of the implicit path: APIC mass and momentum only
    (particlesToGridHelper<APIC>, MpmSimulationBase.cpp:563-608) + getNumNodes +
    normalisation (MpmSimulationBase.cpp:474-485) */

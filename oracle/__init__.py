@@ -192,6 +192,11 @@ def lib():
         L.zro_sym3_max_eigenvalue.restype = _d
         L.zro_sym3_max_eigenvalue.argtypes = [_p]
         L.zro_get_materials.argtypes = [_p, _p, _p, _p]
+        L.zro_eval_max_particle_speed.argtypes = [_p, _p]
+        L.zro_calculate_dt.restype = _d
+        L.zro_calculate_dt.argtypes = [_p, _d]
+        L.zro_next_dt.restype = _d
+        L.zro_next_dt.argtypes = [_d] * 6
         _lib = L
     return _lib
 
@@ -353,6 +358,12 @@ def fcr_first_piola_differential(mu, lam, F, dF):
     return _from_cm(dP)
 
 
+def next_dt(dt, min_dt, max_dt, max_dt_original, frame_dt, time_since_last_frame):
+    """TimeStepping::nextDt (TimeStepping.h:45-61)."""
+    return float(lib().zro_next_dt(float(dt), float(min_dt), float(max_dt), float(max_dt_original),
+                                   float(frame_dt), float(time_since_last_frame)))
+
+
 # ---------------------------------------------------------------- simulation
 class OracleSim:
     """FP64 CPU MPM simulation (explicit MLS step), restating
@@ -503,6 +514,16 @@ class OracleSim:
 
     def step(self, dt):
         return int(lib().zro_step_explicit(self.h, float(dt)))
+
+    # ---- adaptive time step (MpmSimulationBase.cpp:766-790,1143-1170)
+    def eval_max_particle_speed(self):
+        """(max |v|, max corner, min corner) as evalMaxParticleSpeed."""
+        out = np.zeros(7)
+        lib().zro_eval_max_particle_speed(self.h, _ptr(out))
+        return out[0], out[1:4].copy(), out[4:7].copy()
+
+    def calculate_dt(self, max_dt):
+        return float(lib().zro_calculate_dt(self.h, float(max_dt)))
 
     # ---- implicit backward Euler
     def p2g_apic(self):

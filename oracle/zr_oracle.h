@@ -122,6 +122,20 @@ int zro_g2p_mls(zro_sim* s, double dt);
 /* the whole symplectic step (MpmSimulationBase.cpp:256-281) */
 int zro_step_explicit(zro_sim* s, double dt);
 
+/* ---- adaptive time step ---- */
+/* evalMaxParticleSpeed (MpmSimulationBase.cpp:1143-1170): out[0] = max |v|
+   (the APIC speed, out[0] of the reference's pair), out[1..3] = max corner,
+   out[4..6] = min corner of the positions; the reduction starts from
+   (0, float lowest, -float lowest) as the reference's does */
+void zro_eval_max_particle_speed(const zro_sim* s, double out[7]);
+/* calculateDt (MpmSimulationBase.cpp:766-790) with quadratic splines and
+   colliders translating at constant velocity (evalMaxSpeed = |dbdt|,
+   CollisionObject.cpp:158-194): cfl dx / max speed, or max_dt when it is 0 */
+double zro_calculate_dt(const zro_sim* s, double max_dt);
+/* TimeStepping::nextDt (TimeStepping.h:45-61) on its state */
+double zro_next_dt(double dt, double min_dt, double max_dt, double max_dt_original, double frame_dt,
+                   double time_since_last_frame);
+
 /* ---- implicit backward Euler (APIC transfers) ---- */
 typedef struct {
   int32_t newton_max_iterations;   /* NewtonsMethod max_iterations (MpmSimulationBase.cpp:51: 3) */

@@ -494,6 +494,61 @@ int64_t zro_p2g_mls(zro_sim* s)
   return s->num_nodes;
 }
 
+/* ---------------- adaptive time step ---------------- */
+/* evalMaxParticleSpeed (MpmSimulationBase.cpp:1143-1170): a map-reduce of
+   (|v|, x, -x) under the component-wise max, from (0, lowest, lowest) with
+   lowest = numeric_limits<float>::lowest() */
+void zro_eval_max_particle_speed(const zro_sim* s, double out[7])
+{
+  const double lowest = -(double)FLT_MAX;
+  out[0] = 0.0;
+  for (int k = 1; k < 7; ++k) out[k] = lowest;
+  for (int64_t i = 0; i < s->n; ++i) {
+    const double* v = s->V + 3 * i;
+    const double* x = s->X + 3 * i;
+    const double sq = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    const double sp = sqrt(sq);
+    if (sp > out[0]) out[0] = sp;
+    for (int d = 0; d < 3; ++d) {
+      if (x[d] > out[1 + d]) out[1 + d] = x[d];
+      if (-x[d] > out[4 + d]) out[4 + d] = -x[d];
+    }
+  }
+  /* the min corner is the negated max of -x */
+  for (int d = 0; d < 3; ++d) out[4 + d] = -out[4 + d];
+}
+
+/* calculateDt (MpmSimulationBase.cpp:766-790): the corners widened by
+   (degree + 2) dx only bound the colliders' speed queries, which for a
+   constant translation is |dbdt| anywhere (CollisionObject.cpp:158-194) */
+double zro_calculate_dt(const zro_sim* s, double max_dt)
+{
+  double r[7];
+  zro_eval_max_particle_speed(s, r);
+  double speed = r[0];
+  for (int k = 0; k < s->ncol; ++k) {
+    const double* u = s->col[k].velocity;
+    const double cs = sqrt((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+    if (cs > speed) speed = cs;
+  }
+  double dt = max_dt;
+  if (speed != 0.0) dt = s->cfl * s->dx / speed;
+  return dt;
+}
+
+/* TimeStepping::nextDt (TimeStepping.h:45-61) */
+double zro_next_dt(double dt, double min_dt, double max_dt, double max_dt_original, double frame_dt,
+                   double time_since_last_frame)
+{
+  if (max_dt_original < max_dt) max_dt = max_dt_original;
+  dt = (dt < min_dt) ? min_dt : (dt > max_dt) ? max_dt : dt;
+  if (time_since_last_frame + dt >= frame_dt)
+    dt = frame_dt - time_since_last_frame;
+  else if (time_since_last_frame + 2 * dt > frame_dt)
+    dt = (frame_dt - time_since_last_frame) / 2;
+  return dt;
+}
+
 /* ---------------- colliders ---------------- */
 static void collider_b(const zro_sim* s, const zro_collider* c, double t, double* b)
 {

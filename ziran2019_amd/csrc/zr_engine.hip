@@ -22,6 +22,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -652,6 +653,71 @@ __global__ __launch_bounds__(g2p_threads<MLS>()) void k_g2p(ParticlesView P, con
 }
 
 // ---------------------------------------------------------------------------
+// evalMaxParticleSpeed (MpmSimulationBase.cpp:1143-1170): the component-wise
+// maximum of (|v|, x, -x) over the particles, from (0, lowest, lowest) with
+// lowest = numeric_limits<float>::lowest().  |v| is Eigen's norm(),
+// sqrt((v0 v0 + v1 v1) + v2 v2), rounded after every operation; a maximum
+// is exact in any order, so the result does not depend on the launch shape.
+// ---------------------------------------------------------------------------
+constexpr int kSpeedFields = 7;
+constexpr uint32_t kSpeedBlocks = 512;
+__device__ __forceinline__ double speed_of(double v0, double v1, double v2)
+{
+#pragma clang fp contract(off)
+  return sqrt((v0 * v0 + v1 * v1) + v2 * v2);
+}
+__device__ __forceinline__ void speed_max(double& a, double b) { a = b > a ? b : a; }
+__device__ __forceinline__ void speed_block_max(double m[kSpeedFields], double (*smem)[4])
+{
+#pragma unroll
+  for (int j = 0; j < kSpeedFields; ++j)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) speed_max(m[j], __shfl_xor(m[j], off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < kSpeedFields; ++j) smem[j][wid] = m[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSpeedFields; ++j)
+    for (int w = 0; w < 4; ++w) speed_max(m[j], smem[j][w]);
+}
+__global__ __launch_bounds__(256) void k_max_speed(const double* __restrict__ x, const double* __restrict__ v,
+                                                   int64_t S, uint32_t n, double* __restrict__ partial)
+{
+  __shared__ double smem[kSpeedFields][4];
+  double m[kSpeedFields];
+  m[0] = 0.0;
+  for (int j = 1; j < kSpeedFields; ++j) m[j] = -(double)__FLT_MAX__;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    speed_max(m[0], speed_of(v[i], v[S + i], v[2 * S + i]));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double xd = x[d * S + i];
+      speed_max(m[1 + d], xd);
+      speed_max(m[4 + d], -xd);
+    }
+  }
+  speed_block_max(m, smem);
+  if (threadIdx.x < kSpeedFields) partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = m[threadIdx.x];
+}
+// one block: the block maxima of k_max_speed -> out[7] with the min corner
+// negated back
+__global__ __launch_bounds__(256) void k_max_speed_final(const double* __restrict__ partial, uint32_t nb,
+                                                         double* __restrict__ out)
+{
+  __shared__ double smem[kSpeedFields][4];
+  double m[kSpeedFields];
+  m[0] = 0.0;
+  for (int j = 1; j < kSpeedFields; ++j) m[j] = -(double)__FLT_MAX__;
+  for (uint32_t b = threadIdx.x; b < nb; b += 256u)
+#pragma unroll
+    for (int j = 0; j < kSpeedFields; ++j) speed_max(m[j], partial[(size_t)j * nb + b]);
+  speed_block_max(m, smem);
+  if (threadIdx.x < kSpeedFields) out[threadIdx.x] = threadIdx.x >= 4 ? -m[threadIdx.x] : m[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
 // applyPlasticity after the F update (MpmSimulationBase.cpp:1015-1030): the
 // return map of each particle's plasticity model, in FP64.
 // ---------------------------------------------------------------------------
@@ -810,6 +876,8 @@ struct zr_engine {
   uint32_t* scan_tmp = nullptr;
   size_t scan_cap = 0;
   uint32_t* dcount = nullptr;
+  // evalMaxParticleSpeed: 7 block maxima per block, then the 7 results
+  double* speed = nullptr;
   uint32_t* hcount = nullptr;
 
   // implicit backward Euler: node vectors [3][grid_cap], reduction partials,
@@ -1741,7 +1809,7 @@ void zr_destroy(zr_engine* e)
   free_particles(e);
   dfree(e->page_count); dfree(e->page_group); dfree(e->page_touch); dfree(e->page_slot); dfree(e->page_bits);
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
-  dfree(e->scan_tmp); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
+  dfree(e->scan_tmp); dfree(e->speed); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
@@ -1781,6 +1849,42 @@ int zr_set_time(zr_engine* e, double t)
   return ZR_OK;
 }
 double zr_get_time(const zr_engine* e) { return e ? e->time : 0.0; }
+
+int zr_eval_max_particle_speed(zr_engine* e, double* out)
+{
+  if (!e || !out) return ZR_ERR_ARG;
+  if (!e->speed) {
+    ZR_HIP(e, hipMalloc(&e->speed, (kSpeedFields * kSpeedBlocks + kSpeedFields) * sizeof(double)));
+  }
+  const uint32_t n = (uint32_t)e->n;
+  uint32_t nb = blocks_for((uint64_t)n, 256);
+  if (nb > kSpeedBlocks) nb = kSpeedBlocks;
+  if (nb == 0) nb = 1;
+  double* res = e->speed + kSpeedFields * kSpeedBlocks;
+  k_max_speed<<<nb, 256, 0, e->stream>>>(e->x, e->v, e->n, n, e->speed);
+  k_max_speed_final<<<1, 256, 0, e->stream>>>(e->speed, nb, res);
+  ZR_LAUNCH_CHECK(e);
+  ZR_HIP(e, hipMemcpyAsync(out, res, kSpeedFields * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+int zr_calculate_dt(zr_engine* e, double max_dt, double* dt)
+{
+  if (!e || !dt || !(max_dt > 0.0)) return ZR_ERR_ARG;
+  double r[kSpeedFields];
+  const int rc = zr_eval_max_particle_speed(e, r);
+  if (rc) return rc;
+  double speed = r[0];
+  for (const zr_collider& c : e->colliders) {
+#pragma clang fp contract(off)
+    const double* u = c.velocity;
+    const double cs = std::sqrt((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+    if (cs > speed) speed = cs;
+  }
+  *dt = speed != 0.0 ? e->cfg.cfl * e->cfg.dx / speed : max_dt;
+  return ZR_OK;
+}
 
 int zr_upload_particles(zr_engine* e, const zr_particles* p)
 {

@@ -47,6 +47,7 @@ EXPORTS = (
     "zr_set_materials", "zr_get_materials", "zr_set_phase_field", "zr_get_phase_field", "zr_phase_field_solve",
     "zr_phase_field_last_stats", "zr_adjoint_set", "zr_adjoint_get", "zr_adjoint_step", "zr_checkpoint_save",
     "zr_checkpoint_restore", "zr_advance_one_time_step_admm", "zr_get_admm_state",
+    "zr_eval_max_particle_speed", "zr_calculate_dt",
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
 ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER = 0, 1, 2
@@ -56,6 +57,48 @@ PROF = dict(p2g=0, grid=1, g2p=2, sort=3, state=4, hvp=5)
 class RestartException(RuntimeError):
     """A particle travelled more than one grid cell in the step
     (MpmSimulationBase.cpp:983-1002)."""
+
+
+class TimeStepping:
+    """The frame clock of SimulationBase::step (TimeStepping.h:8-103):
+    substep sizes between min_dt and max_dt that land exactly on the frame
+    times."""
+
+    SUBSTEP, FRAME = 1, 0
+
+    def __init__(self, frame_dt=1.0 / 24, min_dt=1e-6, max_dt=None):
+        self.time = 0.0
+        self.time_since_last_frame = 0.0
+        self.frame_dt = frame_dt
+        self.min_dt = min_dt
+        self.max_dt = frame_dt if max_dt is None else max_dt
+        self.max_dt_original = self.max_dt
+
+    def nextDt(self, dt):
+        """TimeStepping::nextDt (TimeStepping.h:45-61)."""
+        if not dt > 0:
+            raise ValueError("nextDt needs a positive dt")
+        self.max_dt = min(self.max_dt, self.max_dt_original)
+        dt = self.min_dt if dt < self.min_dt else (self.max_dt if dt > self.max_dt else dt)
+        if self.time_since_last_frame + dt >= self.frame_dt:
+            dt = self.frame_dt - self.time_since_last_frame
+        elif self.time_since_last_frame + 2 * dt > self.frame_dt:
+            dt = (self.frame_dt - self.time_since_last_frame) / 2
+        return dt
+
+    def advance(self, dt):
+        """TimeStepping::advance (TimeStepping.h:63-80)."""
+        self.time += dt
+        self.time_since_last_frame += dt
+        if self.time_since_last_frame >= self.frame_dt:
+            self.time_since_last_frame = 0.0
+            return self.FRAME
+        return self.SUBSTEP
+
+    def reset(self, frame):
+        """TimeStepping::reset (TimeStepping.h:82-90)."""
+        self.time = frame * self.frame_dt
+        self.time_since_last_frame = 0.0
 
 
 class ZrError(RuntimeError):
@@ -207,6 +250,8 @@ def load_library(path=LIB_PATH):
     L.zr_grid_velocity_explicit_update.argtypes = [vp, ctypes.c_double]
     L.zr_grid_to_particles.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
     L.zr_advance_one_time_step.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
+    L.zr_eval_max_particle_speed.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.zr_calculate_dt.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     L.zr_debug_size.argtypes = [vp, ctypes.c_int32]
     L.zr_debug_size.restype = ctypes.c_int64
     L.zr_debug_dump.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_size_t]
@@ -444,6 +489,34 @@ class MpmEngine:
     def advanceOneTimeStep(self, dt):
         self._check(self.L.zr_advance_one_time_step(self.h, float(dt), ctypes.byref(self.stats)),
                     "advanceOneTimeStep")
+
+    # ---------------------------------------------------------- adaptive dt
+    def evalMaxParticleSpeed(self):
+        """(max |v|, max corner, min corner), MpmSimulationBase::evalMaxParticleSpeed."""
+        out = (ctypes.c_double * 7)()
+        self._check(self.L.zr_eval_max_particle_speed(self.h, out), "evalMaxParticleSpeed")
+        return out[0], np.array(out[1:4]), np.array(out[4:7])
+
+    def calculateDt(self, max_dt):
+        """MpmSimulationBase::calculateDt: cfl dx / max speed, or max_dt."""
+        dt = ctypes.c_double()
+        self._check(self.L.zr_calculate_dt(self.h, float(max_dt), ctypes.byref(dt)), "calculateDt")
+        return dt.value
+
+    def advanceOneFrame(self, step, advance=None):
+        """SimulationBase::advanceOneFrame (SimulationBase.h:288-321): substeps
+        of step.nextDt(calculateDt()) until the frame time is reached; advance
+        is the substep (advanceOneTimeStep by default).  A CFL violation raises
+        RestartException with the substep's state on the device.  Returns the
+        substep sizes."""
+        advance = advance if advance is not None else self.advanceOneTimeStep
+        dts = []
+        while True:
+            dt = step.nextDt(self.calculateDt(step.max_dt))
+            advance(dt)
+            dts.append(dt)
+            if step.advance(dt) == TimeStepping.FRAME:
+                return dts
 
     # ---------------------------------------------------------- implicit
     def particlesToGridApic(self):

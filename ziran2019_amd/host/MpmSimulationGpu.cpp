@@ -1,6 +1,7 @@
 // MpmSimulationGpu.cpp -- forwards MpmSimulationBase's step API to libzr_mpm.so.
 #include "MpmSimulationGpu.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace ziran_amd {
@@ -134,6 +135,56 @@ void MpmSimulationGpu::advanceOneTimeStep(double dt)
     check(zr_advance_one_time_step_implicit(eng_, dt, &o, &implicit_stats_, &step_stats_));
   }
   time += dt;
+}
+
+double MpmSimulationGpu::calculateDt()
+{
+  if (!eng_) throw std::runtime_error("initialize() was not called");
+  double dt = 0.0;
+  check(zr_calculate_dt(eng_, step.max_dt, &dt));
+  return dt;
+}
+
+void MpmSimulationGpu::advanceOneFrame()
+{
+  time = step.time;
+  frame_substeps_ = 0;
+  TimeStepping::Event ev;
+  do {
+    const double dt = step.nextDt(calculateDt());
+    advanceOneTimeStep(dt);
+    ev = step.advance(dt);
+    ++frame_substeps_;
+  } while (ev == TimeStepping::Substep);
+}
+
+double TimeStepping::nextDt(double dt)
+{
+  if (!(dt > 0)) throw std::runtime_error("nextDt needs a positive dt");
+  max_dt = std::min(max_dt, max_dt_original);
+  dt = (dt < min_dt) ? min_dt : (dt > max_dt) ? max_dt : dt;
+  if (time_since_last_frame + dt >= frame_dt)
+    dt = frame_dt - time_since_last_frame;
+  else if (time_since_last_frame + 2 * dt > frame_dt)
+    dt = (frame_dt - time_since_last_frame) / 2;
+  return dt;
+}
+
+TimeStepping::Event TimeStepping::advance(double dt)
+{
+  time += dt;
+  time_since_last_frame += dt;
+  if (time_since_last_frame >= frame_dt) {
+    time_since_last_frame = 0;
+    return Frame;
+  }
+  return Substep;
+}
+
+void TimeStepping::reset(int frame)
+{
+  time = frame * frame_dt;
+  time_since_last_frame = 0;
 }
 
 void MpmSimulationGpu::sortParticlesAndPolluteGrid()

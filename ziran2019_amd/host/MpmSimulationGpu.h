@@ -46,6 +46,22 @@ struct CollisionObject {
   std::array<double, 3> velocity{ { 0.0, 0.0, 0.0 } };
 };
 
+// SimulationBase::step (TimeStepping.h:8-103): substep sizes between min_dt
+// and max_dt that land exactly on the frame times
+struct TimeStepping {
+  enum Event { Frame, Substep };
+  double time = 0.0;
+  double time_since_last_frame = 0.0;
+  double frame_dt = 1.0 / 24;
+  double min_dt = 1e-6;
+  double max_dt = 1.0 / 24;
+  double max_dt_original = 1.0 / 24;
+
+  double nextDt(double dt);   // TimeStepping.h:45-61
+  Event advance(double dt);   // TimeStepping.h:63-80
+  void reset(int frame);      // TimeStepping.h:82-90
+};
+
 class MpmSimulationGpu {
 public:
   // MpmSimulationBase members that configure the step (MpmSimulationBase.h:52-118)
@@ -60,6 +76,7 @@ public:
   double cg_relative_tolerance = 0.0;      // <= 0: Newton-adaptive
   int32_t implicit_precision = ZR_FP64;    // or ZR_FP32_REFINED
   double time = 0.0;                       // SimulationBase::step.time
+  TimeStepping step;                       // SimulationBase::step
 
   ParticleArrays particles;
 
@@ -78,6 +95,14 @@ public:
   // MpmSimulationBase::advanceOneTimeStep (MpmSimulationBase.cpp:251-296)
   virtual void advanceOneTimeStep(double dt);
 
+  // MpmSimulationBase::calculateDt (MpmSimulationBase.cpp:766-790): cfl dx
+  // over the largest particle or collider speed, step.max_dt when still
+  virtual double calculateDt();
+  // SimulationBase::advanceOneFrame (SimulationBase.h:288-321): substeps of
+  // step.nextDt(calculateDt()) up to the next frame time; a CFL violation
+  // throws RestartException (the state of the failed substep is on the device)
+  virtual void advanceOneFrame();
+
   // the individual step virtuals
   virtual void sortParticlesAndPolluteGrid();   // MpmSimulationBase.cpp:1032-1103
   virtual void particlesToGrid();               // :455-608 (MLS for symplectic, APIC for implicit)
@@ -91,12 +116,14 @@ public:
 
   const zr_implicit_stats& lastImplicitStats() const { return implicit_stats_; }
   const zr_step_stats& lastStepStats() const { return step_stats_; }
+  int lastFrameSubsteps() const { return frame_substeps_; }
 
 private:
   zr_engine* eng_ = nullptr;
   std::vector<CollisionObject> colliders_;
   zr_implicit_stats implicit_stats_{};
   zr_step_stats step_stats_{};
+  int frame_substeps_ = 0;
 
   void check(int rc) const;
   zr_implicit_opts implicitOpts() const;

@@ -2,14 +2,16 @@
 // file: the way a Ziran driver (SimulationBase::advanceOneFrame,
 // SimulationBase.h:304-308) calls the step, without Python in the loop.
 //
-//   zr_host_step IN OUT dx steps dt symplectic(1|0) [ground_y] [precision]
+//   zr_host_step IN OUT dx steps dt symplectic(1|0) [ground_y] [precision] [cfl]
 //
 // IN/OUT: int64 n, then X 3n, V 3n, C 9n, F 9n, mass n, vol n, mu n,
 // lambda n doubles (C and F column-major per particle).  ground_y adds a
 // sticky half-space y >= ground_y.  Gravity is (0, -9.8, 0).  Prints one
 // line per step: step, groups, pages, and for implicit steps Newton and CG
-// iteration counts.  Exit status 0 on success, 3 on a CFL restart, 1 on any
-// other error.
+// iteration counts.  A negative dt runs `steps` frames of length -dt
+// instead (SimulationBase::advanceOneFrame with the CFL time step,
+// max_dt = -dt; cfl 0.6 unless given) and prints one line per frame: frame,
+// substeps, time.  Exit status 0 on success, 3 on a CFL restart, 1 on any other error.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -68,7 +70,7 @@ bool write_particles(const char* path, const ParticleArrays& p)
 int main(int argc, char** argv)
 {
   if (argc < 7) {
-    std::fprintf(stderr, "usage: %s IN OUT dx steps dt symplectic [ground_y] [precision]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s IN OUT dx steps dt symplectic [ground_y] [precision] [cfl]\n", argv[0]);
     return 1;
   }
   MpmSimulationGpu sim;
@@ -90,9 +92,18 @@ int main(int argc, char** argv)
     sim.addCollisionObject(ground);
   }
   if (argc > 8) sim.implicit_precision = std::atoi(argv[8]);
+  if (argc > 9) sim.cfl = std::atof(argv[9]);
   try {
     sim.initialize();
-    for (int s = 0; s < steps; ++s) {
+    if (dt < 0.0) {
+      sim.step.frame_dt = sim.step.max_dt = sim.step.max_dt_original = -dt;
+      for (int f = 0; f < steps; ++f) {
+        sim.advanceOneFrame();
+        std::printf("frame %d substeps %d time %.17g\n", f, sim.lastFrameSubsteps(), sim.step.time);
+        std::fflush(stdout);
+      }
+    }
+    for (int s = 0; dt > 0.0 && s < steps; ++s) {
       sim.advanceOneTimeStep(dt);
       const zr_step_stats& st = sim.lastStepStats();
       if (sim.symplectic)
