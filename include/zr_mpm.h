@@ -46,8 +46,9 @@ enum { ZR_TRANSFER_MLS = 0, ZR_TRANSFER_APIC = 1 };
    (no reference counterpart; the default), StvkWithHencky, NeoHookeanBorden */
 enum { ZR_MODEL_FIXED_COROTATED = 0, ZR_MODEL_STVK_HENCKY = 1, ZR_MODEL_NEO_HOOKEAN_BORDEN = 2 };
 /* plasticity (Lib/Ziran/Physics/PlasticityApplier.h): VonMisesStvkHencky,
-   DruckerPragerStvkHencky; both assume the StvkWithHencky model */
-enum { ZR_PLASTIC_NONE = 0, ZR_PLASTIC_VON_MISES = 1, ZR_PLASTIC_DRUCKER_PRAGER = 2 };
+   DruckerPragerStvkHencky (both assume the StvkWithHencky model) and
+   NonAssociativeCamClay (on the NeoHookeanBorden model, whose kappa it uses) */
+enum { ZR_PLASTIC_NONE = 0, ZR_PLASTIC_VON_MISES = 1, ZR_PLASTIC_DRUCKER_PRAGER = 2, ZR_PLASTIC_CAM_CLAY = 3 };
 
 /* collision object types, AnalyticCollisionObject::COLLISION_OBJECT_TYPE
    (Lib/Ziran/Math/Geometry/CollisionObject.h:47-52) */
@@ -97,9 +98,11 @@ typedef struct zr_particles {
    ZR_MODEL_FIXED_COROTATED / ZR_PLASTIC_NONE.  plastic_param holds 4 doubles
    per particle: von Mises {xi, fail_stress, 0, 0} (PlasticityApplier.cpp:443-462),
    Drucker-Prager {alpha, beta, cohesion, volume_correction != 0} with alpha
-   from the friction angle as in PlasticityApplier.cpp:249-260.
+   from the friction angle as in PlasticityApplier.cpp:249-260, Cam-Clay
+   {M, beta, xi, 0} with M from the friction angle as in
+   PlasticityApplier.cpp:10-19 (Cam-Clay needs ZR_MODEL_NEO_HOOKEAN_BORDEN).
    plastic_state is updated by every step: von Mises yield stress (hardened),
-   Drucker-Prager logJp.  degradation is NeoHookeanBorden's g (NULL: 1). */
+   Drucker-Prager and Cam-Clay logJp.  degradation is NeoHookeanBorden's g (NULL: 1). */
 typedef struct zr_materials {
   int64_t n;
   int32_t* model;          /* n */

@@ -176,6 +176,10 @@ def lib():
         L.zro_drucker_prager_alpha.argtypes = [_d]
         L.zro_drucker_prager_project.restype = _i
         L.zro_drucker_prager_project.argtypes = [_d, _d, _d, _d, _d, _i, _p, _p]
+        L.zro_cam_clay_M.restype = _d
+        L.zro_cam_clay_M.argtypes = [_d, _i]
+        L.zro_cam_clay_project.restype = _i
+        L.zro_cam_clay_project.argtypes = [_d, _d, _d, _d, _d, _p, _p]
         L.zro_set_materials.argtypes = [_p, _p, _p, _p, _p, _p]
         L.zro_set_phase_field.argtypes = [_p] + [_p] * 8 + [ctypes.POINTER(ZroPfOpts)]
         L.zro_get_phase_field.argtypes = [_p, _p, _p, _p]
@@ -314,6 +318,18 @@ def drucker_prager_project(mu, lam, alpha, beta, cohesion, volume_correction, lo
     f = _cm(F)
     lib().zro_drucker_prager_project(mu, lam, alpha, beta, cohesion, int(bool(volume_correction)), _ptr(j), _ptr(f))
     return _from_cm(f), float(j[0])
+
+
+def cam_clay_M(friction_angle_deg, dim=3):
+    return float(lib().zro_cam_clay_M(float(friction_angle_deg), int(dim)))
+
+
+def cam_clay_project(mu, lam, M, beta, xi, logJp, F):
+    """Returns (F_new, logJp_new, moved)."""
+    j = np.array([logJp], dtype=np.float64)
+    f = _cm(F)
+    moved = lib().zro_cam_clay_project(mu, lam, M, beta, xi, _ptr(j), _ptr(f))
+    return _from_cm(f), float(j[0]), bool(moved)
 
 
 def fcr_kirchhoff_adjoint(mu, lam, F, taubar, Fbar=None):

@@ -63,6 +63,10 @@ int zro_von_mises_project(double mu, double lam, double xi, double fail_stress, 
 double zro_drucker_prager_alpha(double friction_angle_deg);
 int zro_drucker_prager_project(double mu, double lam, double alpha, double beta, double cohesion,
                                int volume_correction, double* logJp, double* F);
+/* NonAssociativeCamClay (PlasticityApplier.cpp:10-19, 36-135) on a
+   NeoHookeanBorden model; M from the friction angle */
+double zro_cam_clay_M(double friction_angle_deg, int dim);
+int zro_cam_clay_project(double mu, double lam, double M, double beta, double xi, double* logJp, double* F);
 
 /* material codes (same values as ZR_MODEL_* / ZR_PLASTIC_* of the engine) */
 #define ZRO_MODEL_FIXED_COROTATED 0
@@ -71,6 +75,7 @@ int zro_drucker_prager_project(double mu, double lam, double alpha, double beta,
 #define ZRO_PLASTIC_NONE 0
 #define ZRO_PLASTIC_VON_MISES 1
 #define ZRO_PLASTIC_DRUCKER_PRAGER 2
+#define ZRO_PLASTIC_CAM_CLAY 3
 
 /* ---- simulation state ---- */
 typedef struct zro_sim zro_sim;

@@ -667,6 +667,83 @@ int zro_drucker_prager_project(double mu, double lam, double alpha, double beta,
   return 1;
 }
 
+/* ---- NonAssociativeCamClay (PlasticityApplier.cpp:10-19, 36-135) ---- */
+/* M from the friction angle in degrees (the constructor, :10-19) */
+double zro_cam_clay_M(double friction_angle_deg, int dim)
+{
+  double sin_phi = sin(friction_angle_deg / 180.0 * 3.141592653);
+  double mohr_columb_friction = sqrt(2.0 / 3.0) * 2.0 * sin_phi / (3.0 - sin_phi);
+  return mohr_columb_friction * (double)dim / sqrt(2.0 / (6.0 - dim));
+}
+
+/* projectStrain on a NeoHookeanBorden model (kappa = 0.666666 mu + lambda,
+   NeoHookeanBorden.cpp:20): the live branch is Compare_With_Physbam on the
+   singular values with p0 = kappa (1e-5 + sinh(xi max(-logJp, 0))), then
+   F = U diag(Se) V^T.  Returns 1 when the state was outside the surface or
+   at a tip. */
+int zro_cam_clay_project(double mu, double lam, double M, double beta, double xi, double* logJp, double* F)
+{
+  M3 f, u, v;
+  double Se[3];
+  from_cm(F, f);
+  svd3_rm(f, u, Se, v);
+  const double kappa = 0.666666 * mu + lam;
+  const double mlj = -*logJp;
+  const double p0 = kappa * (0.00001 + sinh(xi * (mlj > 0.0 ? mlj : 0.0)));
+  const double a = (1.0 + 2.0 * beta) * (6.0 - 3.0) / 2.0;
+  const double b = beta * p0, c = p0, M2 = M * M;
+  double Je = 1.0;
+  for (int i = 0; i < 3; ++i) Je *= Se[i];
+  double Se2[3], sum = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    Se2[i] = Se[i] * Se[i];
+    sum += Se2[i];
+  }
+  sum /= 3.0;
+  double s_hat[3];
+  const double scal = mu * pow(Je, -2.0 / 3.0);
+  for (int i = 0; i < 3; ++i) s_hat[i] = scal * (Se2[i] - sum);
+  const double Uprime = kappa / 2.0 * (Je - 1.0 / Je);
+  const double p_trial = -Uprime * Je;
+  int moved = 0;
+  if (p_trial > c) { /* tips */
+    double Je_new = sqrt(-2.0 * c / kappa + 1.0);
+    for (int i = 0; i < 3; ++i) Se[i] = pow(Je_new, 1.0 / 3.0);
+    *logJp += log(Je / Je_new);
+    moved = 1;
+  } else if (p_trial < -b) {
+    double Je_new = sqrt(2.0 * b / kappa + 1.0);
+    for (int i = 0; i < 3; ++i) Se[i] = pow(Je_new, 1.0 / 3.0);
+    *logJp += log(Je / Je_new);
+    moved = 1;
+  } else {
+    double k = sqrt(-M2 * (p_trial + b) * (p_trial - c) / a);
+    double s_norm = sqrt((s_hat[0] * s_hat[0] + s_hat[1] * s_hat[1]) + s_hat[2] * s_hat[2]);
+    double y = a * (s_norm * s_norm) + M2 * (p_trial + b) * (p_trial - c);
+    if (y >= 1e-4) { /* outside: onto the surface, with the "fake" hardening */
+      double pc = (1.0 - beta) * p0 / 2.0;
+      if (p0 > 1e-4 && p_trial < p0 - 1e-4 && p_trial > -beta * p0 + 1e-4) {
+        double aa = M2 * ((p_trial - pc) * (p_trial - pc)) / (a * (s_norm * s_norm));
+        double dd = 1.0 + aa;
+        double ff = aa * beta * p0 - aa * p0 - 2.0 * pc;
+        double gg = pc * pc - aa * beta * (p0 * p0);
+        double zz = sqrt(fabs(ff * ff - 4.0 * dd * gg));
+        double p1 = (-ff + zz) / (2.0 * dd);
+        double p2 = (-ff - zz) / (2.0 * dd);
+        double p_fake = (p_trial - pc) * (p1 - pc) > 0.0 ? p1 : p2;
+        double Je_new_fake = sqrt(fabs(-2.0 * p_fake / kappa + 1.0));
+        if (Je_new_fake > 1e-4) *logJp += log(Je / Je_new_fake);
+      }
+      double ssum = (Se2[0] + Se2[1]) + Se2[2];
+      double scale = k / mu * pow(Je, 2.0 / 3.0);
+      for (int i = 0; i < 3; ++i) Se[i] = sqrt(scale * s_hat[i] / s_norm + 1.0 / 3.0 * ssum);
+      moved = 1;
+    }
+  }
+  rebuild(u, Se, v, F);
+  return moved;
+}
+
 /* ---- adjoint of the fixed-corotated Kirchhoff stress (no reference counterpart) ---- */
 /* reverse of tau = 2 mu (F - R) F^T + lam (J - 1) J I at F (column-major
    9-vectors): adds to Fbar, returns mubar / lambar */

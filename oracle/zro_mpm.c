@@ -417,6 +417,8 @@ static void apply_plasticity(zro_sim* s, int64_t i)
     if (b) s->broken[i] = 1;
   } else if (s->plastic[i] == ZRO_PLASTIC_DRUCKER_PRAGER) {
     zro_drucker_prager_project(s->mu[i], s->lam[i], q[0], q[1], q[2], q[3] != 0.0, &s->pstate[i], s->F + 9 * i);
+  } else if (s->plastic[i] == ZRO_PLASTIC_CAM_CLAY) {
+    zro_cam_clay_project(s->mu[i], s->lam[i], q[0], q[1], q[2], &s->pstate[i], s->F + 9 * i);
   }
 }
 

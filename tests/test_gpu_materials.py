@@ -1,12 +1,13 @@
 """GPU parity of the StvkWithHencky / NeoHookeanBorden models and the von
-Mises / Drucker-Prager return maps (explicit MLS steps) against the FP64
+Mises / Drucker-Prager / Cam-Clay return maps (explicit MLS steps) against the FP64
 oracle on the same seeded particles.  Tolerances: 1e-11 relative after one
 step (summation order only), 1e-9 after several."""
 import numpy as np
 import pytest
 
-from ziran2019_amd.engine import (ZR_MODEL_NEO_HOOKEAN_BORDEN, ZR_MODEL_STVK_HENCKY, ZR_PLASTIC_DRUCKER_PRAGER,
-                                  ZR_PLASTIC_VON_MISES, ZrError, drucker_prager_alpha, implicit_opts)
+from ziran2019_amd.engine import (ZR_MODEL_NEO_HOOKEAN_BORDEN, ZR_MODEL_STVK_HENCKY, ZR_PLASTIC_CAM_CLAY,
+                                  ZR_PLASTIC_DRUCKER_PRAGER, ZR_PLASTIC_VON_MISES, ZrError, cam_clay_M,
+                                  drucker_prager_alpha, implicit_opts)
 from _helpers import make_pair, perturbed, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -82,6 +83,30 @@ def test_drucker_prager_sand(cohesion, beta):
     _check(out, ref, 1e-9)
     scale = max(np.abs(om["pstate"]).max(), 1e-12)
     assert np.abs(gm["plastic_state"] - om["pstate"]).max() < 1e-9 * scale
+
+
+@pytest.mark.parametrize("logJp,beta", [(np.log(0.98), 0.5), (0.0, 2.0)])
+def test_cam_clay_snow(logJp, beta):
+    # NeoHookeanBorden + NonAssociativeCamClay as in the reference's snow
+    # demos (friction angle 45, xi 3): compressed against the ground
+    s = perturbed(27, shuffle=False, E=1e4, F_noise=0.05)
+    s["V"][:, 1] -= 1.0
+    mats = dict(model=ZR_MODEL_NEO_HOOKEAN_BORDEN, plasticity=ZR_PLASTIC_CAM_CLAY,
+                plastic_param=(cam_clay_M(45.0), beta, 3.0, 0.0), plastic_state=logJp)
+    out, ref, gm, om = _run(s, mats, 8)
+    _check(out, ref, 1e-9)
+    scale = max(np.abs(om["pstate"]).max(), 1e-12)
+    assert np.abs(gm["plastic_state"] - om["pstate"]).max() < 1e-9 * scale
+    assert np.any(gm["plastic_state"] != logJp)  # the return map moved states
+
+
+def test_cam_clay_needs_neo_hookean_borden():
+    s = perturbed(28, shuffle=False)
+    eng, _ = make_pair(s)
+    with pytest.raises(ZrError):
+        eng.set_materials(model=ZR_MODEL_STVK_HENCKY, plasticity=ZR_PLASTIC_CAM_CLAY,
+                          plastic_param=(cam_clay_M(45.0), 0.5, 3.0, 0.0), plastic_state=0.0)
+    eng.close()
 
 
 def test_neo_hookean_borden_with_degradation():

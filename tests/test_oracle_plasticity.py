@@ -200,3 +200,88 @@ def test_oracle_step_applies_plasticity():
     for p in range(0, n, 7):
         Fp = Fo[p].reshape(3, 3).T
         assert _dev_log_norm(Fp)[0] <= ty / (2 * mu) + 1e-12
+
+
+# ---- NonAssociativeCamClay (PlasticityApplier.cpp:10-19, 36-135) on NeoHookeanBorden
+def _cc_setup():
+    mu, lam = 3e3, 4e3
+    kappa = 0.666666 * mu + lam  # NeoHookeanBorden.cpp:20
+    return mu, lam, kappa
+
+
+def test_cam_clay_M_closed_form():
+    # M = sqrt(2/3) 2 s / (3 - s) * 3 / sqrt(2/3) = 6 s / (3 - s) in 3-D
+    for phi in (15.0, 30.0, 45.0, 60.0):
+        s = np.sin(phi / 180.0 * 3.141592653)
+        assert oracle.cam_clay_M(phi) == pytest.approx(6.0 * s / (3.0 - s), rel=1e-15)
+        assert engine_cam_clay_M(phi) == pytest.approx(oracle.cam_clay_M(phi), rel=1e-15)
+
+
+def engine_cam_clay_M(phi):
+    from ziran2019_amd.engine import cam_clay_M
+    return cam_clay_M(phi)
+
+
+@pytest.mark.parametrize("J", [0.9, 1.2])
+def test_cam_clay_tips(J):
+    # logJp = 0: p0 = kappa 1e-5, so compression (p > p0) and tension
+    # (p < -beta p0) both land on a tip: a volumetric Se with the tip's Je
+    mu, lam, kappa = _cc_setup()
+    beta, xi = 0.5, 3.0
+    R = np.linalg.qr(np.random.default_rng(1).normal(size=(3, 3)))[0]
+    R *= np.sign(np.linalg.det(R))
+    F = R @ np.diag([J ** (1 / 3)] * 3)
+    Fn, lj, moved = oracle.cam_clay_project(mu, lam, oracle.cam_clay_M(45.0), beta, xi, 0.0, F)
+    p0 = kappa * 1e-5
+    Je_new = np.sqrt(-2 * p0 / kappa + 1) if J < 1 else np.sqrt(2 * beta * p0 / kappa + 1)
+    assert moved
+    sv = np.linalg.svd(Fn, compute_uv=False)
+    assert np.allclose(sv, Je_new ** (1 / 3), rtol=1e-13)
+    assert lj == pytest.approx(np.log(J / Je_new), rel=1e-12)
+
+
+def test_cam_clay_inside_keeps_state():
+    mu, lam, kappa = _cc_setup()
+    F = np.diag([1.01, 0.995, 1.0])
+    Fn, lj, moved = oracle.cam_clay_project(mu, lam, oracle.cam_clay_M(45.0), 0.5, 3.0, -0.5, F)
+    assert not moved
+    assert lj == -0.5
+    assert np.allclose(Fn, F, rtol=0, atol=1e-14)
+
+
+def test_cam_clay_projection_onto_surface():
+    mu, lam, kappa = _cc_setup()
+    M, beta, xi, lj0 = oracle.cam_clay_M(45.0), 0.5, 3.0, -0.05
+    rng = np.random.default_rng(3)
+    U = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    V = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    U *= np.sign(np.linalg.det(U))
+    V *= np.sign(np.linalg.det(V))
+    sig = np.array([1.3, 0.85, 0.93])
+    F = U @ np.diag(sig) @ V.T
+    Fn, lj, moved = oracle.cam_clay_project(mu, lam, M, beta, xi, lj0, F)
+    assert moved
+    # trial quantities from the closed forms
+    p0 = kappa * (1e-5 + np.sinh(xi * max(-lj0, 0.0)))
+    Je = sig.prod()
+    s2 = sig ** 2
+    s_hat = mu * Je ** (-2 / 3) * (s2 - s2.mean())
+    p = -kappa / 2 * (Je - 1 / Je) * Je
+    a = (1 + 2 * beta) * 3 / 2
+    assert -beta * p0 < p < p0
+    assert a * s_hat @ s_hat + M * M * (p + beta * p0) * (p - p0) > 1e-4  # outside
+    # the return keeps the mean of Se^2 and puts the deviatoric part at the
+    # surface's radius k along the trial direction (with the trial Je)
+    k = np.sqrt(-M * M * (p + beta * p0) * (p - p0) / a)
+    sn = np.sort(np.linalg.svd(Fn, compute_uv=False) ** 2)
+    want = np.sort(k / mu * Je ** (2 / 3) * s_hat / np.linalg.norm(s_hat) + s2.mean())
+    assert np.allclose(sn, want, rtol=1e-12)
+    # hardening: the logJp step is the tip of the line from the surface's
+    # centre (pc, 0) through the trial point (p, q) where it meets the surface
+    pc = (1 - beta) * p0 / 2
+    q = np.sqrt(1.5) * np.linalg.norm(s_hat)
+    Je_fake = Je / np.exp(lj - lj0)
+    p_fake = kappa / 2 * (1 - Je_fake ** 2)
+    q_fake = q * (p_fake - pc) / (p - pc)
+    y_fake = (1 + 2 * beta) * q_fake ** 2 + M * M * (p_fake + beta * p0) * (p_fake - p0)
+    assert abs(y_fake) < 1e-9 * (M * M * p0 * p0)

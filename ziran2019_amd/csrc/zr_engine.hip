@@ -741,9 +741,11 @@ __global__ __launch_bounds__(256) void k_plasticity(uint32_t n, const int32_t* _
     bool br = false;
     if (!von_mises_project(mu[i], lam[i], pparam[i], pparam[S + i], st, f, br)) return;
     if (br) broken[i] = 1;
-  } else {
+  } else if (plastic == ZR_PLASTIC_DRUCKER_PRAGER) {
     drucker_prager_project(mu[i], lam[i], pparam[i], pparam[S + i], pparam[2 * S + i], pparam[3 * S + i] != 0.0,
                            st, f);
+  } else {
+    cam_clay_project(mu[i], lam[i], pparam[i], pparam[S + i], pparam[2 * S + i], st, f);
   }
   pstate[i] = st;
 #pragma unroll
@@ -1996,8 +1998,10 @@ int zr_set_materials(zr_engine* e, const zr_materials* mt)
     const int32_t md = mt->model ? mt->model[i] : ZR_MODEL_FIXED_COROTATED;
     const int32_t pl = mt->plasticity ? mt->plasticity[i] : ZR_PLASTIC_NONE;
     if (md < ZR_MODEL_FIXED_COROTATED || md > ZR_MODEL_NEO_HOOKEAN_BORDEN || pl < ZR_PLASTIC_NONE ||
-        pl > ZR_PLASTIC_DRUCKER_PRAGER)
+        pl > ZR_PLASTIC_CAM_CLAY)
       return fail(e, ZR_ERR_ARG, "unknown model or plasticity code");
+    if (pl == ZR_PLASTIC_CAM_CLAY && md != ZR_MODEL_NEO_HOOKEAN_BORDEN)
+      return fail(e, ZR_ERR_ARG, "Cam-Clay plasticity needs the NeoHookeanBorden model");
     models = models || md != ZR_MODEL_FIXED_COROTATED;
     plastic = plastic || pl != ZR_PLASTIC_NONE;
     code[i] = md | (pl << 4);

@@ -606,4 +606,76 @@ ZR_HD void drucker_prager_project(T mu, T lam, T alpha, T beta, T cohesion, bool
   usv_t(u, H, v, f);
 }
 
+// NonAssociativeCamClay::projectStrain (PlasticityApplier.cpp:113-135) on a
+// NeoHookeanBorden model (kappa = 0.666666 mu + lambda,
+// NeoHookeanBorden.cpp:20): the live branch is Compare_With_Physbam
+// (PlasticityApplier.cpp:36-109) on the singular values, with
+// p0 = kappa (1e-5 + sinh(xi max(-logJp, 0))): the tips of the yield surface
+// (p beyond p0 or below -beta p0) set Se to a volumetric state, a trial state
+// outside the surface (y >= 1e-4) keeps its mean of Se^2 and scales its
+// deviatoric part onto the surface, with logJp hardened through the
+// intersection of the line to the surface's centre (the reference's "fake"
+// hardening).  M comes from the friction angle (PlasticityApplier.cpp:10-19).
+template <class T>
+ZR_HD void cam_clay_project(T mu, T lam, T M, T beta, T xi, T& logJp, M3<T>& f)
+{
+  M3<T> u, v;
+  T Se[3];
+  svd3(f, u, Se, v);
+  const T kappa = (T)0.666666 * mu + lam;
+  const T p0 = kappa * ((T)0.00001 + sinh(xi * (-logJp > (T)0 ? -logJp : (T)0)));
+  const T a = ((T)1 + (T)2 * beta) * ((T)6 - (T)3) / (T)2;
+  const T b = beta * p0;
+  const T c = p0;
+  const T M2 = M * M;
+  T Je = (T)1;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) Je *= Se[i];
+  T Se2[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) Se2[i] = Se[i] * Se[i];
+  T mean = (T)0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) mean += Se2[i];
+  mean /= (T)3;
+  const T sc = mu * pow(Je, -(T)2 / (T)3);
+  T sh[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) sh[i] = sc * (Se2[i] - mean);
+  const T Uprime = kappa / (T)2 * (Je - (T)1 / Je);
+  const T p_trial = -Uprime * Je;
+  if (p_trial > c || p_trial < -b) {  // the tips
+    const T Je_new = p_trial > c ? sqrt(-(T)2 * c / kappa + (T)1) : sqrt((T)2 * b / kappa + (T)1);
+    const T se = pow(Je_new, (T)1 / (T)3);
+    Se[0] = Se[1] = Se[2] = se;
+    logJp += log(Je / Je_new);
+  } else {
+    const T k = sqrt(-M2 * (p_trial + b) * (p_trial - c) / a);
+    const T sn2 = (sh[0] * sh[0] + sh[1] * sh[1]) + sh[2] * sh[2];
+    const T sn = sqrt(sn2);
+    const T y = a * (sn * sn) + M2 * (p_trial + b) * (p_trial - c);
+    // inside the yield surface Se stays; F is rebuilt from the SVD either way
+    if (y >= (T)1e-4) {
+      const T pc = ((T)1 - beta) * p0 / (T)2;
+      if (p0 > (T)1e-4 && p_trial < p0 - (T)1e-4 && p_trial > -beta * p0 + (T)1e-4) {
+        const T aa = M2 * ((p_trial - pc) * (p_trial - pc)) / (a * (sn * sn));
+        const T dd = (T)1 + aa;
+        const T ff = aa * beta * p0 - aa * p0 - (T)2 * pc;
+        const T gg = pc * pc - aa * beta * (p0 * p0);
+        const T zz = sqrt(fabs(ff * ff - (T)4 * dd * gg));
+        const T p1 = (-ff + zz) / ((T)2 * dd);
+        const T p2 = (-ff - zz) / ((T)2 * dd);
+        const T p_fake = (p_trial - pc) * (p1 - pc) > (T)0 ? p1 : p2;
+        const T Je_new_fake = sqrt(fabs(-(T)2 * p_fake / kappa + (T)1));
+        if (Je_new_fake > (T)1e-4) logJp += log(Je / Je_new_fake);
+      }
+      const T ssum = (Se2[0] + Se2[1]) + Se2[2];
+      const T scale = k / mu * pow(Je, (T)2 / (T)3);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) Se[i] = sqrt(scale * sh[i] / sn + (T)1 / (T)3 * ssum);
+    }
+  }
+  usv_t(u, Se, v, f);
+}
+
 }  // namespace zr

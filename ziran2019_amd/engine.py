@@ -50,7 +50,7 @@ EXPORTS = (
     "zr_eval_max_particle_speed", "zr_calculate_dt",
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
-ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER = 0, 1, 2
+ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER, ZR_PLASTIC_CAM_CLAY = 0, 1, 2, 3
 PROF = dict(p2g=0, grid=1, g2p=2, sort=3, state=4, hvp=5)
 
 
@@ -154,6 +154,13 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 class ZrMaterials(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("model", _ip), ("plasticity", _ip), ("plastic_param", _dp),
                 ("plastic_state", _dp), ("degradation", _dp), ("broken", _ip)]
+
+
+def cam_clay_M(friction_angle_deg, dim=3):
+    """NonAssociativeCamClay's M from the friction angle with the reference's
+    pi = 3.141592653 (PlasticityApplier.cpp:10-19)."""
+    s = np.sin(friction_angle_deg / 180.0 * 3.141592653)
+    return np.sqrt(2.0 / 3.0) * 2.0 * s / (3.0 - s) * dim / np.sqrt(2.0 / (6.0 - dim))
 
 
 def drucker_prager_alpha(friction_angle_deg):
