@@ -43,7 +43,10 @@ enum { ZR_FP64 = 0, ZR_FP32_REFINED = 1 };
 enum { ZR_TRANSFER_MLS = 0, ZR_TRANSFER_APIC = 1 };
 
 /* elasticity models (Lib/Ziran/Physics/ConstitutiveModel/): fixed-corotated
-   (no reference counterpart; the default), StvkWithHencky, NeoHookeanBorden */
+   (no reference counterpart; the default), StvkWithHencky, NeoHookeanBorden.
+   StvkWithHenckyIsotropic has StvkWithHencky's energy and stress
+   (StvkWithHenckyIsotropic.h: psi = mu tr(log S)^2 + lambda/2 (tr log S)^2)
+   and uses ZR_MODEL_STVK_HENCKY. */
 enum { ZR_MODEL_FIXED_COROTATED = 0, ZR_MODEL_STVK_HENCKY = 1, ZR_MODEL_NEO_HOOKEAN_BORDEN = 2 };
 /* plasticity (Lib/Ziran/Physics/PlasticityApplier.h): VonMisesStvkHencky,
    DruckerPragerStvkHencky (both assume the StvkWithHencky model) and
@@ -231,6 +234,19 @@ int zr_download_particles(zr_engine* e, zr_particles* p);
    fills plastic_state, degradation and broken (NULL pointers skip). */
 int zr_set_materials(zr_engine* e, const zr_materials* m);
 int zr_get_materials(zr_engine* e, zr_materials* m);
+/* FElasticNonequilibratedBasedMpmForceHelper
+   (Lib/MPM/Force/FElasticNonequilibratedBasedMpmForceHelper.h:27-87): a
+   second elastic strain per particle (F_neq, 9 doubles column-major; NULL:
+   identity) with a StvkWithHencky model of its own (mu, lambda per
+   particle), whose V0 tau adds to the particle's stress and which follows
+   the flow and relaxes with viscosities viscosity_d (deviatoric) and
+   viscosity_v (volumetric), setParameters (:31-35).  Explicit MLS step only
+   (the implicit, ADMM and adjoint steps reject it).  mu = lambda = NULL
+   switches it off; an upload of particles clears it. */
+int zr_set_viscoelastic(zr_engine* e, const double* F_neq, const double* mu, const double* lambda,
+                        double viscosity_d, double viscosity_v);
+/* the current F_neq (9 doubles per particle, column-major) */
+int zr_get_viscoelastic(zr_engine* e, double* F_neq);
 
 /* FractureSimulation: attaches a phase field to every particle and switches
    them to NeoHookeanBorden (addFBasedMpmForceWithPhaseField,

@@ -176,6 +176,9 @@ def lib():
         L.zro_drucker_prager_alpha.argtypes = [_d]
         L.zro_drucker_prager_project.restype = _i
         L.zro_drucker_prager_project.argtypes = [_d, _d, _d, _d, _d, _i, _p, _p]
+        L.zro_set_viscoelastic.argtypes = [_p, _p, _p, _p, _d, _d]
+        L.zro_get_viscoelastic.argtypes = [_p, _p]
+        L.zro_visco_relax.argtypes = [_d, _d, _d, _d, _d, _p, _p]
         L.zro_cam_clay_M.restype = _d
         L.zro_cam_clay_M.argtypes = [_d, _i]
         L.zro_cam_clay_project.restype = _i
@@ -318,6 +321,14 @@ def drucker_prager_project(mu, lam, alpha, beta, cohesion, volume_correction, lo
     f = _cm(F)
     lib().zro_drucker_prager_project(mu, lam, alpha, beta, cohesion, int(bool(volume_correction)), _ptr(j), _ptr(f))
     return _from_cm(f), float(j[0])
+
+
+def visco_relax(mu, lam, viscosity_d, viscosity_v, dt, gradV, Fneq):
+    """evolveStrain of FElasticNonequilibratedBasedMpmForceHelper on one Fneq."""
+    f = _cm(Fneq)
+    lib().zro_visco_relax(float(mu), float(lam), float(viscosity_d), float(viscosity_v), float(dt), _ptr(_cm(gradV)),
+                          _ptr(f))
+    return _from_cm(f)
 
 
 def cam_clay_M(friction_angle_deg, dim=3):
@@ -530,6 +541,20 @@ class OracleSim:
 
     def step(self, dt):
         return int(lib().zro_step_explicit(self.h, float(dt)))
+
+    # ---- viscoelastic branch (FElasticNonequilibratedBasedMpmForceHelper.h)
+    def set_viscoelastic(self, Fneq, mu, lam, viscosity_d, viscosity_v):
+        n = self.n
+        self._fneq = None if Fneq is None else np.ascontiguousarray(np.asarray(Fneq, dtype=np.float64).reshape(9 * n))
+        self._mun = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (n,)))
+        self._lamn = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, dtype=np.float64), (n,)))
+        lib().zro_set_viscoelastic(self.h, _ptr(self._fneq), _ptr(self._mun), _ptr(self._lamn), float(viscosity_d),
+                                   float(viscosity_v))
+
+    def get_viscoelastic(self):
+        out = np.zeros(9 * self.n)
+        lib().zro_get_viscoelastic(self.h, _ptr(out))
+        return out.reshape(self.n, 9)
 
     # ---- adaptive time step (MpmSimulationBase.cpp:766-790,1143-1170)
     def eval_max_particle_speed(self):

@@ -104,6 +104,14 @@ void zro_set_materials(zro_sim* s, const int32_t* model, const int32_t* plastic,
                        const double* pstate, const double* g);
 void zro_get_materials(const zro_sim* s, double* pstate, int32_t* broken, double* g);
 void zro_get_particles(const zro_sim* s, double* X, double* V, double* C, double* F);
+/* FElasticNonequilibratedBasedMpmForceHelper (:27-87): Fe_Nonequilibrated
+   (9 per particle, column-major; NULL: identity) with a StvkWithHencky model
+   (mu, lam per particle) and viscosities; mu == NULL switches it off */
+void zro_set_viscoelastic(zro_sim* s, const double* Fneq, const double* mu, const double* lam, double visc_d,
+                          double visc_v);
+void zro_get_viscoelastic(const zro_sim* s, double* Fneq);
+void zro_visco_relax(double mu, double lam, double visc_d, double visc_v, double dt, const double* gradV_cm,
+                     double* Fneq_cm);
 int zro_add_collider(zro_sim* s, const zro_collider* c);
 void zro_set_time(zro_sim* s, double t);
 

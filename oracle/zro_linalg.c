@@ -667,6 +667,46 @@ int zro_drucker_prager_project(double mu, double lam, double alpha, double beta,
   return 1;
 }
 
+/* ---- FElasticNonequilibratedBasedMpmForceHelper::evolveStrain
+   (FElasticNonequilibratedBasedMpmForceHelper.h:62-87) ---- */
+/* Fneq = (I + dt gradV) Fneq (the form of the engine's F update), then the
+   Hencky strain relaxation with alpha = 2 mu / eta_d and
+   beta = 2 (2 mu + 3 lambda) / (9 eta_v) - 2 mu / (3 eta_d):
+   eps = (eps_trial - dt beta / (1 + dt (alpha + 3 beta)) t 1) / (1 + dt alpha),
+   where t is Eigen's trace() of the 3x1 vector eps_trial, i.e. its first
+   entry (the reference calls trace() on a vector). */
+void zro_visco_relax(double mu, double lam, double visc_d, double visc_v, double dt, const double* gradV_cm,
+                     double* Fneq_cm)
+{
+  double Fo[3][3], G[3][3];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) {
+      Fo[r][c] = Fneq_cm[c * 3 + r];
+      G[r][c] = gradV_cm[c * 3 + r];
+    }
+  M3 f, u, v;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double acc = Fo[r][c];
+      for (int k = 0; k < 3; ++k) acc += dt * G[r][k] * Fo[k][c];
+      f[r][c] = acc;
+    }
+  double sig[3], eps[3], e[3];
+  svd3_rm(f, u, sig, v);
+  const double alpha = 2.0 * mu / visc_d;
+  const double beta = 2.0 * (2.0 * mu + lam * 3.0) / (9.0 * visc_v) - 2.0 * mu / (visc_d * 3.0);
+  for (int i = 0; i < 3; ++i) eps[i] = log(fabs(sig[i]));
+  const double t = eps[0];
+  const double k = dt * beta / (1.0 + dt * (alpha + 3.0 * beta)) * t;
+  for (int i = 0; i < 3; ++i) {
+    const double ei = 1.0 / (1.0 + dt * alpha) * (eps[i] - k);
+    e[i] = exp(ei);
+  }
+  double out[9];
+  rebuild(u, e, v, out);
+  memcpy(Fneq_cm, out, sizeof(out));
+}
+
 /* ---- NonAssociativeCamClay (PlasticityApplier.cpp:10-19, 36-135) ---- */
 /* M from the friction angle in degrees (the constructor, :10-19) */
 double zro_cam_clay_M(double friction_angle_deg, int dim)

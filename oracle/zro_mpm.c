@@ -156,6 +156,10 @@ struct zro_sim {
   double be_dt;
   int ncol;
   zro_collider col[ZRO_MAX_COLLIDERS];
+  /* FElasticNonequilibratedBasedMpmForceHelper: Fe_Nonequilibrated and its
+     StvkWithHencky parameters per particle, viscosities */
+  double *fneq, *mu_neq, *lam_neq;
+  double visc_d, visc_v;
 };
 
 zro_sim* zro_create(double dx, const double* gravity, double cfl, int nthreads)
@@ -178,6 +182,8 @@ static void free_particles(zro_sim* s)
   free(s->model); free(s->plastic); free(s->broken); free(s->pparam); free(s->pstate); free(s->gdeg);
   s->model = s->plastic = s->broken = NULL;
   s->pparam = s->pstate = s->gdeg = NULL;
+  free(s->fneq); free(s->mu_neq); free(s->lam_neq);
+  s->fneq = s->mu_neq = s->lam_neq = NULL;
   free(s->pf_c); free(s->pf_H); free(s->pf_l0); free(s->pf_osc); free(s->pf_Fp); free(s->pf_Hmax);
   free(s->pf_k); free(s->pf_vol); free(s->pf_allow);
   s->pf_c = s->pf_H = s->pf_l0 = s->pf_osc = s->pf_Fp = s->pf_Hmax = s->pf_k = s->pf_vol = NULL;
@@ -386,6 +392,28 @@ void zro_set_materials(zro_sim* s, const int32_t* model, const int32_t* plastic,
   for (int64_t i = 0; i < n; ++i) s->gdeg[i] = g ? g[i] : 1.0;
 }
 
+void zro_set_viscoelastic(zro_sim* s, const double* Fneq, const double* mu, const double* lam, double visc_d,
+                          double visc_v)
+{
+  free(s->fneq); free(s->mu_neq); free(s->lam_neq);
+  s->fneq = s->mu_neq = s->lam_neq = NULL;
+  if (!mu || !lam) return;
+  const int64_t n = s->n;
+  s->fneq = dup(Fneq, 9 * n);
+  if (!Fneq)
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 9; ++k) s->fneq[9 * i + k] = (k % 4 == 0) ? 1.0 : 0.0;
+  s->mu_neq = dup(mu, n);
+  s->lam_neq = dup(lam, n);
+  s->visc_d = visc_d;
+  s->visc_v = visc_v;
+}
+
+void zro_get_viscoelastic(const zro_sim* s, double* Fneq)
+{
+  if (s->fneq && Fneq) memcpy(Fneq, s->fneq, sizeof(double) * 9 * (size_t)s->n);
+}
+
 void zro_get_materials(const zro_sim* s, double* pstate, int32_t* broken, double* g)
 {
   if (!s->model) return;
@@ -430,6 +458,10 @@ void zro_update_stress(zro_sim* s)
     double tau[9];
     particle_kirchhoff(s, p, tau);
     for (int c = 0; c < 9; ++c) s->vtau[9 * p + c] = s->vol[p] * tau[c];
+    if (s->fneq) { /* FElasticNonequilibratedBasedMpmForceHelper::updateState (:36-60) */
+      zro_stvk_hencky_kirchhoff(s->mu_neq[p], s->lam_neq[p], s->fneq + 9 * p, tau);
+      for (int c = 0; c < 9; ++c) s->vtau[9 * p + c] += s->vol[p] * tau[c];
+    }
   }
 }
 
@@ -750,6 +782,12 @@ int zro_g2p_mls(zro_sim* s, double dt)
         }
       for (int c = 0; c < 3; ++c)
         for (int r = 0; r < 3; ++r) s->F[9 * i + c * 3 + r] = Fn[r][c];
+      if (s->fneq) { /* the viscoelastic helper's evolveStrain with gradV = C */
+        double g9[9];
+        for (int c = 0; c < 3; ++c)
+          for (int r = 0; r < 3; ++r) g9[c * 3 + r] = Cn[r][c];
+        zro_visco_relax(s->mu_neq[i], s->lam_neq[i], s->visc_d, s->visc_v, dt, g9, s->fneq + 9 * i);
+      }
       apply_plasticity(s, i); /* MpmSimulationBase.cpp:1015 */
     }
   }

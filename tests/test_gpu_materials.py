@@ -141,3 +141,31 @@ def test_implicit_rejects_other_models():
     with pytest.raises(ZrError):
         eng.advanceOneTimeStepImplicit(1e-3, implicit_opts())
     eng.close()
+
+
+@pytest.mark.parametrize("viscosities", [(5.0, 8.0), (0.2, 0.5)])
+def test_viscoelastic_branch(viscosities):
+    # FElasticNonequilibratedBasedMpmForceHelper on top of the particles' own
+    # model: a pre-strained branch that relaxes while the body falls
+    s = perturbed(29, shuffle=False, E=1e4, F_noise=0.03)
+    s["V"][:, 1] -= 0.5
+    n = len(s["mass"])
+    rng = np.random.default_rng(29)
+    Fn = np.tile(np.eye(3), (n, 1, 1)) + 0.05 * rng.normal(size=(n, 3, 3))
+    Fneq = np.ascontiguousarray(Fn.transpose(0, 2, 1).reshape(n, 9))
+    mu_n = np.full(n, 2e3) * np.exp(0.2 * rng.normal(size=n))
+    lam_n = np.full(n, 3e3)
+    eng, ora = make_pair(s, colliders=_ground(s))
+    eng.set_viscoelastic(Fneq, mu_n, lam_n, *viscosities)
+    ora.set_viscoelastic(Fneq, mu_n, lam_n, *viscosities)
+    for _ in range(8):
+        eng.advanceOneTimeStep(1e-4)
+        ora.step(1e-4)
+    out = eng.download()
+    _check(out, ora.get_particles(), 1e-9)
+    assert rel_err(eng.get_viscoelastic(), ora.get_viscoelastic()) < 1e-9
+    with pytest.raises(ZrError):  # the implicit step has no viscoelastic branch
+        eng.advanceOneTimeStepImplicit(1e-3, implicit_opts())
+    eng.set_viscoelastic(None, None, None, 1.0, 1.0)  # off again
+    eng.advanceOneTimeStep(1e-4)
+    eng.close()

@@ -349,6 +349,7 @@ struct ParticlesView {
   int64_t stride;
   const int32_t* mat;   // model | plasticity << 4 per particle; null: all fixed-corotated
   const double* gdeg;   // NeoHookeanBorden degradation g
+  const double* vtau_add;  // the viscoelastic branch's V0 tau, added to the stress (null: none)
 };
 
 // constitutive_model.kirchhoff of the particle's model (FBasedMpmForceHelper.cpp:44-64)
@@ -410,6 +411,11 @@ __global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int r = 0; r < 3; ++r) st[r][c] = vol * tau.a[r][c];
+      if (P.vtau_add)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) st[r][c] += P.vtau_add[(c * 3 + r) * S + i];
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -755,6 +761,82 @@ __global__ __launch_bounds__(256) void k_plasticity(uint32_t n, const int32_t* _
 }
 
 // ---------------------------------------------------------------------------
+// The viscoelastic branch, FElasticNonequilibratedBasedMpmForceHelper
+// (FElasticNonequilibratedBasedMpmForceHelper.h:27-87): a second elastic
+// strain Fe_Nonequilibrated per particle with a StvkWithHencky model of its
+// own, whose V0 tau adds to the stress (updateState, :36-60) and which moves
+// with the flow and relaxes viscously in the Hencky strain (evolveStrain,
+// :62-87).  Explicit MLS step only (gradV = C).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_visco_stress(uint32_t n, const double* __restrict__ fneq,
+                                                      const double* __restrict__ mu, const double* __restrict__ lam,
+                                                      const double* __restrict__ vol, int64_t S,
+                                                      double* __restrict__ vtau)
+{
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  M3<double> f, tau;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) f.a[r][c] = fneq[(c * 3 + r) * S + i];
+  hencky_kirchhoff(mu[i], lam[i], f, tau);
+  const double v0 = vol[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) vtau[(c * 3 + r) * S + i] = v0 * tau.a[r][c];
+}
+
+// Fneq <- (I + dt C) Fneq in the form of the F update, then the Hencky
+// strain relaxation with alpha = 2 mu / eta_d and
+// beta = 2 (2 mu + 3 lambda) / (9 eta_v) - 2 mu / (3 eta_d):
+// eps = (eps_trial - dt beta / (1 + dt (alpha + 3 beta)) t 1) / (1 + dt alpha)
+// where t is the reference's eps_trial.trace() of a 3x1 vector: its first
+// entry (the log of the largest singular value).
+__global__ __launch_bounds__(256) void k_visco_strain(uint32_t n, const double* __restrict__ C, double* fneq,
+                                                      const double* __restrict__ mu, const double* __restrict__ lam,
+                                                      int64_t S, double dt, double visc_d, double visc_v)
+{
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double Fo[3][3], G[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      Fo[r][c] = fneq[(c * 3 + r) * S + i];
+      G[r][c] = C[(c * 3 + r) * S + i];
+    }
+  M3<double> f, u, v;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double acc = Fo[r][c];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc += dt * G[r][k] * Fo[k][c];
+      f.a[r][c] = acc;
+    }
+  double sig[3], e[3];
+  svd3(f, u, sig, v);
+  const double m = mu[i], l = lam[i];
+  const double alpha = 2.0 * m / visc_d;
+  const double beta = 2.0 * (2.0 * m + l * 3.0) / (9.0 * visc_v) - 2.0 * m / (visc_d * 3.0);
+  double eps[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) eps[d] = log(fabs(sig[d]));
+  const double k = dt * beta / (1.0 + dt * (alpha + 3.0 * beta)) * eps[0];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) e[d] = exp(1.0 / (1.0 + dt * alpha) * (eps[d] - k));
+  usv_t(u, e, v, f);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) fneq[(c * 3 + r) * S + i] = f.a[r][c];
+}
+
+// ---------------------------------------------------------------------------
 // layout conversion and introspection
 // ---------------------------------------------------------------------------
 __global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, uint32_t n, int ncomp, int64_t stride)
@@ -828,6 +910,11 @@ struct zr_engine {
   double *pparam = nullptr, *pstate = nullptr, *gdeg = nullptr;
   int32_t* broken = nullptr;
   bool has_models = false, has_plastic = false, all_hencky = false;
+  // viscoelastic branch: Fe_Nonequilibrated [9][n], its StvkWithHencky mu /
+  // lambda, V0 tau of the branch [9][n], the two viscosities
+  double *fneq = nullptr, *mu_neq = nullptr, *lam_neq = nullptr, *vtau_neq = nullptr;
+  double visc_d = 0.0, visc_v = 0.0;
+  bool visco = false;
   // phase field: per particle (c, H, l0, 1/sigma_c, pf_Fp, H_max, k, vol, allow)
   // and per node (phase, rhs, D, x, r, mr, s, q) of the damage solve
   bool pf_on = false;
@@ -1008,6 +1095,8 @@ void free_particles(zr_engine* e)
   dfree(e->touch_page); dfree(e->touch_slot); dfree(e->slot_page); dfree(e->group_nslot);
   dfree(e->mat); dfree(e->pparam); dfree(e->pstate); dfree(e->gdeg); dfree(e->broken);
   e->has_models = e->has_plastic = false;
+  dfree(e->fneq); dfree(e->mu_neq); dfree(e->lam_neq); dfree(e->vtau_neq);
+  e->visco = false;
   dfree(e->pf_c); dfree(e->pf_H); dfree(e->pf_l0); dfree(e->pf_osc); dfree(e->pf_Fp); dfree(e->pf_Hmax);
   dfree(e->pf_k); dfree(e->pf_vol); dfree(e->pf_allow);
   e->pf_on = false;
@@ -1034,7 +1123,7 @@ ColliderSet collider_set(const zr_engine* e)
 ParticlesView pview(zr_engine* e)
 {
   return ParticlesView{ e->x, e->v, e->C, e->F, e->m, e->vol, e->mu, e->lam, e->n,
-                        e->has_models ? e->mat : nullptr, e->gdeg };
+                        e->has_models ? e->mat : nullptr, e->gdeg, e->visco ? e->vtau_neq : nullptr };
 }
 
 int ensure_grid(zr_engine* e)
@@ -1130,6 +1219,11 @@ int do_p2g(zr_engine* e, bool mls)
   if (!e->sorted) return fail(e, ZR_ERR_STATE, "particlesToGrid before sortParticlesAndPolluteGrid");
   const double dx = e->cfg.dx;
   prof_mark(e, ZR_PROF_P2G);
+  if (mls && e->visco) {
+    k_visco_stress<<<blocks_for((uint64_t)e->n, 256), 256, 0, e->stream>>>((uint32_t)e->n, e->fneq, e->mu_neq,
+                                                                            e->lam_neq, e->vol, e->n, e->vtau_neq);
+    ZR_LAUNCH_CHECK(e);
+  }
   if (mls)
     k_p2g<true><<<e->A, kTransferThreads, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
                                              1.0 / dx, 4.0 / (dx * dx));
@@ -1180,6 +1274,11 @@ int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
                                               e->gvs, (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt,
                                               e->cfg.cfl, e->dcount);
   ZR_LAUNCH_CHECK(e);
+  if (mls && e->visco) {  // the viscoelastic helper's evolveStrain with gradV = C
+    k_visco_strain<<<blocks_for((uint64_t)e->n, 256), 256, 0, e->stream>>>(
+        (uint32_t)e->n, e->C, e->fneq, e->mu_neq, e->lam_neq, e->n, dt, e->visc_d, e->visc_v);
+    ZR_LAUNCH_CHECK(e);
+  }
   if (e->has_plastic) {
     k_plasticity<<<blocks_for((uint64_t)e->n, 256), 256, 0, e->stream>>>(
         (uint32_t)e->n, e->mat, e->F, e->n, e->mu, e->lam, e->pparam, e->pstate, e->broken);
@@ -1304,8 +1403,9 @@ int ensure_adjoint(zr_engine* e)
 
 bool adjoint_supported(zr_engine* e)
 {
-  if (e->has_models || e->has_plastic || e->pf_on) {
-    fail(e, ZR_ERR_ARG, "the adjoint supports fixed-corotated particles without plasticity or damage only");
+  if (e->has_models || e->has_plastic || e->pf_on || e->visco) {
+    fail(e, ZR_ERR_ARG,
+         "the adjoint supports fixed-corotated particles without plasticity, damage or a viscoelastic branch only");
     return false;
   }
   return true;
@@ -2035,6 +2135,50 @@ int zr_set_materials(zr_engine* e, const zr_materials* mt)
   return ZR_OK;
 }
 
+int zr_set_viscoelastic(zr_engine* e, const double* F_neq, const double* mu, const double* lambda,
+                        double viscosity_d, double viscosity_v)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (!mu && !lambda) {  // switch the branch off
+    e->visco = false;
+    return ZR_OK;
+  }
+  if (!mu || !lambda) return fail(e, ZR_ERR_ARG, "the viscoelastic branch needs mu and lambda");
+  if (!(viscosity_d > 0.0) || !(viscosity_v > 0.0)) return fail(e, ZR_ERR_ARG, "viscosities must be positive");
+  const size_t N = (size_t)e->n;
+  if (N == 0) return fail(e, ZR_ERR_STATE, "upload the particles before the viscoelastic branch");
+  if (!e->fneq) {
+    ZR_HIP(e, hipMalloc(&e->fneq, 9 * N * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->vtau_neq, 9 * N * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->mu_neq, N * sizeof(double)));
+    ZR_HIP(e, hipMalloc(&e->lam_neq, N * sizeof(double)));
+  }
+  std::vector<double> f(9 * N);
+  for (size_t i = 0; i < N; ++i)
+    for (int k = 0; k < 9; ++k) f[k * N + i] = F_neq ? F_neq[9 * i + k] : (k % 4 == 0 ? 1.0 : 0.0);
+  ZR_HIP(e, hipMemcpyAsync(e->fneq, f.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->mu_neq, mu, N * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  ZR_HIP(e, hipMemcpyAsync(e->lam_neq, lambda, N * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  e->visc_d = viscosity_d;
+  e->visc_v = viscosity_v;
+  e->visco = true;
+  return ZR_OK;
+}
+
+int zr_get_viscoelastic(zr_engine* e, double* F_neq)
+{
+  if (!e || !F_neq) return ZR_ERR_ARG;
+  if (!e->visco) return fail(e, ZR_ERR_STATE, "no viscoelastic branch");
+  const size_t N = (size_t)e->n;
+  std::vector<double> f(9 * N);
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  ZR_HIP(e, hipMemcpy(f.data(), e->fneq, 9 * N * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < N; ++i)
+    for (int k = 0; k < 9; ++k) F_neq[9 * i + k] = f[k * N + i];
+  return ZR_OK;
+}
+
 int zr_get_materials(zr_engine* e, zr_materials* mt)
 {
   if (!e || !mt) return fail(e, ZR_ERR_ARG, "bad materials view");
@@ -2261,6 +2405,7 @@ int zr_advance_one_time_step_admm(zr_engine* e, double dt, const zr_admm_opts* o
     return fail(e, ZR_ERR_ARG, "bad dt or ADMM options");
   if (e->n > 0 && (!e->mat || !e->all_hencky || e->pf_on))
     return fail(e, ZR_ERR_ARG, "the ADMM step needs StvkWithHencky particles (zr_set_materials)");
+  if (e->visco) return fail(e, ZR_ERR_ARG, "the ADMM step has no viscoelastic branch");
   if (ast) ast->minres_iterations = 0;
   int rc = do_sort(e, st);
   if (rc) return rc;
@@ -2362,9 +2507,12 @@ int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st)
   return rc;
 }
 
+static bool implicit_materials_ok(zr_engine* e);
+
 int zr_particles_to_grid_apic(zr_engine* e)
 {
   if (!e) return ZR_ERR_ARG;
+  if (!implicit_materials_ok(e)) return ZR_ERR_ARG;
   int rc = do_p2g(e, false);
   if (rc) return rc;
   rc = do_grid(e, GRID_COMBINE, 0.0);
@@ -2382,8 +2530,9 @@ static bool valid_opts(const zr_implicit_opts* o)
 
 static bool implicit_materials_ok(zr_engine* e)
 {
-  if (e->has_models || e->has_plastic) {
-    fail(e, ZR_ERR_ARG, "the implicit step supports fixed-corotated particles without plasticity only");
+  if (e->has_models || e->has_plastic || e->visco) {
+    fail(e, ZR_ERR_ARG,
+         "the implicit step supports fixed-corotated particles without plasticity or a viscoelastic branch only");
     return false;
   }
   return true;
@@ -2400,6 +2549,7 @@ int zr_backward_euler_step(zr_engine* e, double dt, const zr_implicit_opts* o, z
 int zr_grid_to_particles_apic(zr_engine* e, double dt, zr_step_stats* st)
 {
   if (!e) return ZR_ERR_ARG;
+  if (!implicit_materials_ok(e)) return ZR_ERR_ARG;
   return do_g2p(e, dt, st, false);
 }
 

@@ -47,7 +47,7 @@ EXPORTS = (
     "zr_set_materials", "zr_get_materials", "zr_set_phase_field", "zr_get_phase_field", "zr_phase_field_solve",
     "zr_phase_field_last_stats", "zr_adjoint_set", "zr_adjoint_get", "zr_adjoint_step", "zr_checkpoint_save",
     "zr_checkpoint_restore", "zr_advance_one_time_step_admm", "zr_get_admm_state",
-    "zr_eval_max_particle_speed", "zr_calculate_dt",
+    "zr_eval_max_particle_speed", "zr_calculate_dt", "zr_set_viscoelastic", "zr_get_viscoelastic",
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
 ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER, ZR_PLASTIC_CAM_CLAY = 0, 1, 2, 3
@@ -258,6 +258,8 @@ def load_library(path=LIB_PATH):
     L.zr_grid_to_particles.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
     L.zr_advance_one_time_step.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
     L.zr_eval_max_particle_speed.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.zr_set_viscoelastic.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double]
+    L.zr_get_viscoelastic.argtypes = [vp, vp]
     L.zr_calculate_dt.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     L.zr_debug_size.argtypes = [vp, ctypes.c_int32]
     L.zr_debug_size.restype = ctypes.c_int64
@@ -496,6 +498,27 @@ class MpmEngine:
     def advanceOneTimeStep(self, dt):
         self._check(self.L.zr_advance_one_time_step(self.h, float(dt), ctypes.byref(self.stats)),
                     "advanceOneTimeStep")
+
+    # ---------------------------------------------------------- viscoelastic
+    def set_viscoelastic(self, F_neq, mu, lam, viscosity_d, viscosity_v):
+        """FElasticNonequilibratedBasedMpmForceHelper: F_neq (n x 9 column-major,
+        None: identity), its StvkWithHencky mu / lambda (scalars or per
+        particle) and the two viscosities; mu = lam = None switches it off."""
+        n = self.n
+        if mu is None and lam is None:
+            self._check(self.L.zr_set_viscoelastic(self.h, None, None, None, 1.0, 1.0), "zr_set_viscoelastic")
+            return
+        self._visco = [None if F_neq is None else np.ascontiguousarray(np.asarray(F_neq, dtype=np.float64).reshape(9 * n)),
+                       np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (n,))),
+                       np.ascontiguousarray(np.broadcast_to(np.asarray(lam, dtype=np.float64), (n,)))]
+        f, m, l = (None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in self._visco)
+        self._check(self.L.zr_set_viscoelastic(self.h, f, m, l, float(viscosity_d), float(viscosity_v)),
+                    "zr_set_viscoelastic")
+
+    def get_viscoelastic(self):
+        out = np.zeros(9 * self.n)
+        self._check(self.L.zr_get_viscoelastic(self.h, out.ctypes.data_as(ctypes.c_void_p)), "zr_get_viscoelastic")
+        return out.reshape(self.n, 9)
 
     # ---------------------------------------------------------- adaptive dt
     def evalMaxParticleSpeed(self):
