@@ -1,6 +1,7 @@
 """Benchmark of the MPM time step on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload implicit-boxes|explicit-cube]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload implicit-boxes|explicit-cube|admm-slabs|
+                                                                fracture-sphere|adjoint-cube]
 
 One step = MpmSimulationBase::advanceOneTimeStep on the whole particle set,
 inputs resident in HBM:
@@ -233,6 +234,96 @@ def copy_calibration(nbytes=256 << 20, launches=5):
     print(json.dumps(dict(copy_bytes_each_way=nbytes, launches=launches)))
 
 
+def run_row_workload(args):
+    """The other rows at their configuration sizes (one JSON line each, not
+    the metric): admm-slabs (ADMM step, 2.05M particles in a von Mises and a
+    Drucker-Prager slab, 20 ADMM iterations, dt 1e-3), fracture-sphere (the
+    explicit step with the CD-MPM phase-field solve, 3M NeoHookeanBorden
+    particles struck by a sphere at 5 m/s, dt 5e-4) and adjoint-cube (a
+    64-step explicit rollout of 262,144 particles at dt 2e-4 with per-particle
+    E, forward with a checkpoint every 8 steps plus the adjoint back to E and
+    V0; one "step" here is one rollout)."""
+    import numpy as np
+    from ziran2019_amd import scenes
+    from ziran2019_amd.engine import ZR_MODEL_NEO_HOOKEAN_BORDEN, MpmEngine, admm_opts, phase_field_opts
+    wl = args.workload
+    extra = {}
+    eng = None
+    if wl == "adjoint-cube":
+        from ziran2019_amd.autodiff import Rollout
+        s = scenes.adjoint_scene()
+        dt, nsteps = 2e-4, 64
+        prob = Rollout(s, nsteps, dt, checkpoint_every=8)
+        rng = np.random.default_rng(5)
+        V0 = 0.1 * rng.normal(size=(len(s["mass"]), 3))
+        target = s["X"].mean(axis=0) + np.array([0.05, 0.0, 0.0])
+
+        def step():
+            X = prob.forward(s["E"], V0)
+            com = X.mean(axis=0)
+            gX = np.tile(2.0 * (com - target) / len(X), (len(X), 1))
+            gE, gV = prob.backward(gX)
+            extra["loss"] = float(((com - target) ** 2).sum())
+            extra["grad_E_norm"] = float(np.linalg.norm(gE))
+        workload = ("differentiable explicit MLS-MPM rollout, 262144 particles at dx=1/128, 64 steps dt=2e-4, "
+                    "E lognormal(ln 1e4, 0.4), checkpoints every 8 steps, adjoint to E and V0; value counts the "
+                    "64 forward steps of each rollout")
+        n, per = len(s["mass"]), nsteps
+    else:
+        if wl == "admm-slabs":
+            s = scenes.slabs_scene()
+            dt = 1e-3
+        else:
+            s = scenes.fracture_scene()
+            dt = 5e-4
+        n = len(s["mass"])
+        eng = MpmEngine(dx=s["dx"], gravity=s["gravity"])
+        eng.upload(s)
+        eng.set_colliders(s["colliders"])
+        if wl == "admm-slabs":
+            eng.set_materials(**s["materials"])
+            o = admm_opts(admm_iterations=20)
+            minres = []
+
+            def step():
+                minres.append(eng.advanceOneTimeStepAdmm(dt, o).minres_iterations)
+            workload = ("ADMM step (StvkWithHencky, von Mises slab with yield stress 200 Pa beside a Drucker-Prager "
+                        "slab at 30 deg), 2048000 particles at 8 ppc, dx=1/256, dt=1e-3, 20 ADMM iterations")
+            extra["minres_iterations"] = minres
+        else:
+            eng.set_materials(model=ZR_MODEL_NEO_HOOKEAN_BORDEN)
+            eng.set_phase_field(phase_field_opts(), l0=s["l0"], one_over_sigma_c=s["one_over_sigma_c"],
+                                residual_phase=s["residual_phase"])
+
+            def step():
+                eng.advanceOneTimeStep(dt)
+            workload = ("explicit MLS step with the CD-MPM phase-field solve (CG), 3000000 NeoHookeanBorden "
+                        "particles in a Voronoi-toughened block, sticky sphere at 5 m/s, dx=1/320, dt=5e-4")
+        per = 1
+    for _ in range(args.warmup):
+        step()
+    if eng is not None:
+        eng.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    t1 = time.perf_counter()
+    kernels = {}
+    if eng is not None:
+        for k in ("p2g", "grid", "g2p", "sort"):
+            ms, launches = eng.kernel_time(k)
+            kernels[k] = dict(avg_us=round(ms * 1e3 / max(launches, 1), 2), launches=launches)
+        eng.close()
+    else:
+        prob.close()
+    el = t1 - t0
+    if "minres_iterations" in extra:
+        extra["minres_iterations"] = extra["minres_iterations"][-args.steps:]
+    print(json.dumps(dict(workload=wl, config=workload, n_particles=n, steps=args.steps, warmup=args.warmup,
+                          ms_per_step=el / args.steps * 1e3, value=n * per * args.steps / el,
+                          unit="particle-steps/s", kernels=kernels, **extra)))
+
+
 def calculate_dt_timing(calls):
     """calculateDt (evalMaxParticleSpeed: |v| and the position bounds, one
     reduction over the particles) on the implicit-boxes particles, ``calls``
@@ -260,7 +351,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="implicit-boxes", choices=["implicit-boxes", "explicit-cube"])
+    ap.add_argument("--workload", default="implicit-boxes",
+                    choices=["implicit-boxes", "explicit-cube", "admm-slabs", "fracture-sphere", "adjoint-cube"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -278,6 +370,11 @@ def main():
         print("bench.py measures one GPU only: the engine has no multi-GPU path; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload in ("admm-slabs", "fracture-sphere", "adjoint-cube"):
+        args.steps = 3 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        run_row_workload(args)
+        return
     implicit = args.workload == "implicit-boxes"
     if args.steps is None:
         args.steps = 10 if implicit else 1000

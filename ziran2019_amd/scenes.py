@@ -122,6 +122,66 @@ def cube_scene(dx=1.0 / 128, lo_cell=51, ncells=26, ppc=8, E=1e4, nu=0.3, rho=10
     return out
 
 
+def slabs_scene(dx=1.0 / 256, cells=(64, 40, 50), gap=8, ppc=8, E=1e4, nu=0.3, rho=1000.0, yield_stress=200.0,
+                friction_angle=30.0, seed=20190703):
+    """Elastoplastic ADMM workload: two slabs of ``cells`` cells side by side
+    in x (64 x 40 x 50 = 128,000 cells each, 2,048,000 particles at 8 ppc
+    and dx = 1/256), ``gap`` cells apart, on a sticky ground, StvkWithHencky;
+    the first slab von Mises (yield stress ``yield_stress`` Pa, no hardening),
+    the second Drucker-Prager sand (``friction_angle`` degrees, beta 1, no
+    cohesion, volume correction).  Materials come as the ``materials`` entry,
+    the keyword arguments of MpmEngine.set_materials."""
+    cells = np.asarray(cells)
+    ground = 0.1
+    lo0 = np.array([int(round(0.5 / dx)) - cells[0] - gap // 2, int(np.ceil(ground / dx)),
+                    int(round(0.5 / dx)) - cells[2] // 2])
+    los = [lo0, lo0 + np.array([cells[0] + gap, 0, 0])]
+    Xs = []
+    for k, lo in enumerate(los):
+        r = [np.arange(lo[d], lo[d] + cells[d]) for d in range(3)]
+        cidx = np.stack(np.meshgrid(*r, indexing="ij"), axis=-1).reshape(-1, 3)
+        Xs.append(_particles_in_cells(cidx, ppc, dx, seed, k))
+    X = np.concatenate(Xs)
+    slab = np.concatenate([np.zeros(len(Xs[0]), np.int32), np.ones(len(Xs[1]), np.int32)])
+    perm = morton_order(X, dx)
+    X, slab = X[perm], slab[perm]
+    mu, lam = lame(E, nu)
+    out = _finish(X, dx, rho, ppc, mu, lam, order=False)
+    n = len(X)
+    s_phi = np.sin(friction_angle / 180.0 * 3.141592653)
+    alpha = np.sqrt(2.0 / 3.0) * 2.0 * s_phi / (3.0 - s_phi)  # PlasticityApplier.cpp:249-260
+    pp = np.zeros((n, 4))
+    pp[slab == 0] = (0.0, 1e30, 0.0, 0.0)
+    pp[slab == 1] = (alpha, 1.0, 0.0, 1.0)
+    out.update(dx=dx, gravity=(0.0, -9.8, 0.0), slab=slab,
+               materials=dict(model=1, plasticity=np.where(slab == 0, 1, 2).astype(np.int32), plastic_param=pp,
+                              plastic_state=np.where(slab == 0, yield_stress, 0.0)),
+               colliders=[dict(shape=0, type=1, origin=(0.0, ground, 0.0), normal=(0.0, 1.0, 0.0))])
+    return out
+
+
+def adjoint_scene(dx=1.0 / 128, ncells=32, lo_cell=48, ppc=8, logE_mu=np.log(1e4), logE_sigma=0.4, nu=0.3,
+                  rho=1000.0, seed=20190705):
+    """Differentiable explicit workload: a cube of ``ncells``^3 cells (32^3 x 8
+    = 262,144 particles at dx = 1/128) standing on a sticky ground (half a
+    cell into its bottom layer, so that the ground's reaction and with it the
+    centre of mass depend on the moduli) with per-particle Young's modulus
+    lognormal(``logE_mu``, ``logE_sigma``) from the seed; the ``E`` entry holds
+    the moduli (mu / lam are set from them with ``nu``)."""
+    r = np.arange(lo_cell, lo_cell + ncells)
+    cells = np.stack(np.meshgrid(r, r, r, indexing="ij"), axis=-1).reshape(-1, 3)
+    X = _particles_in_cells(cells, ppc, dx, seed, 0)
+    perm = morton_order(X, dx)
+    X = X[perm]
+    n = len(X)
+    E = np.exp(logE_mu + logE_sigma * _normal01(np.arange(n, dtype=np.uint64) + np.uint64(1 << 46), seed))
+    mu, lam = lame(E, nu)
+    out = _finish(X, dx, rho, ppc, mu, lam, order=False)
+    out.update(dx=dx, gravity=(0.0, -9.8, 0.0), E=E,
+               colliders=[dict(shape=0, type=1, origin=(0.0, (lo_cell + 0.5) * dx, 0.0), normal=(0.0, 1.0, 0.0))])
+    return out
+
+
 def cube_scene_small(seed=7, ncells=6, lo_cell=20, dx=1.0 / 64, E=1e4, v0=(0.3, -0.2, 0.1), spin=2.0):
     """A small cube with a nonzero initial velocity field (translation plus a
     rigid spin) used by parity tests."""
