@@ -71,9 +71,13 @@ TRAFFIC = {
 
 IMPLICIT_OPTS = dict(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=500, cg_tolerance=1e30,
                      cg_relative_tolerance=1e-6)
+# the reference's default lumped mass (the diagonal of the operator's mass
+# part) or JacobiPreconditioner on the whole operator diagonal; the metric
+# is quoted with the first, the second is measured beside it
+PRECONDITIONERS = dict(jacobi=1, mass=0)
 
 
-def cpu_baseline_implicit(scene, dt, threads, nboxes=8):
+def cpu_baseline_implicit(scene, dt, threads, opts, nboxes=8):
     """One implicit oracle step on the particles of the first ``nboxes`` boxes."""
     import numpy as np
 
@@ -92,7 +96,7 @@ def cpu_baseline_implicit(scene, dt, threads, nboxes=8):
     for c in scene["colliders"]:
         sim.add_collider(**c)
     t0 = time.perf_counter()
-    flags, st = sim.step_implicit(dt, oracle.implicit_opts(**IMPLICIT_OPTS))
+    flags, st = sim.step_implicit(dt, oracle.implicit_opts(**opts))
     el = time.perf_counter() - t0
     n = int(keep.sum())
     return dict(value=n / el, unit="particle-steps/s", cores=threads, kind="port",
@@ -187,10 +191,10 @@ def run_implicit(args):
     dt = 2e-3
     n = len(scene["mass"])
     # plain FP64 path, then the FP32-bulk / FP64-refined form from the same state
-    eng64, r64 = _implicit_run(scene, implicit_opts(**IMPLICIT_OPTS), args.steps, args.warmup, profile=True)
+    opts = dict(IMPLICIT_OPTS, preconditioner=PRECONDITIONERS[args.preconditioner])
+    eng64, r64 = _implicit_run(scene, implicit_opts(**opts), args.steps, args.warmup, profile=True)
     eng64.close()
-    eng, r32 = _implicit_run(scene, implicit_opts(precision=1, **IMPLICIT_OPTS), args.steps, args.warmup,
-                             profile=True)
+    eng, r32 = _implicit_run(scene, implicit_opts(precision=1, **opts), args.steps, args.warmup, profile=True)
     eng.sortParticlesAndPolluteGrid()
     eng.particlesToGridApic()
     g_nodes = int(np.count_nonzero(eng.dump("grid_m")))
@@ -204,9 +208,10 @@ def run_implicit(args):
                                   np.abs(r64["final"]["V"]).max()),
                steps=args.steps + args.warmup)
     workload = (f"implicit backward Euler, 64 seeded fixed-corotated boxes (E lognormal(ln 5e4, 0.5)), {n} particles "
-                "at 8 ppc, dx=1/256, dt=2e-3, Newton tol 1e-5 (<=3 its) + mass-preconditioned CG rel tol 1e-6; "
+                f"at 8 ppc, dx=1/256, dt=2e-3, Newton tol 1e-5 (<=3 its) + {args.preconditioner}-preconditioned CG "
+                "rel tol 1e-6; "
                 "FP32 operator and Krylov vectors inside the FP64 Newton residual")
-    base = None if args.no_cpu_baseline else cpu_baseline_implicit(scene, dt, min(16, os.cpu_count() or 1))
+    base = None if args.no_cpu_baseline else cpu_baseline_implicit(scene, dt, min(16, os.cpu_count() or 1), opts)
     fp64_value = n * args.steps / r64["elapsed"]
     extra = dict(newton_iterations_per_step=r32["newton"], cg_iterations_per_step=r32["cg"],
                  fp64_plain=dict(value=fp64_value, ms_per_step=r64["elapsed"] / args.steps * 1e3,
@@ -215,6 +220,18 @@ def run_implicit(args):
                                           for k, v in r64["kt"].items()}),
                  speedup_vs_fp64=(n * args.steps / r32["elapsed"]) / fp64_value,
                  fp32_vs_fp64_error=err)
+    if args.preconditioner == "mass":
+        # the same steps with JacobiPreconditioner on the operator diagonal
+        jopts = dict(IMPLICIT_OPTS, preconditioner=PRECONDITIONERS["jacobi"])
+        e, j64 = _implicit_run(scene, implicit_opts(**jopts), args.steps, args.warmup, profile=False)
+        e.close()
+        e, j32 = _implicit_run(scene, implicit_opts(precision=1, **jopts), args.steps, args.warmup, profile=False)
+        e.close()
+        extra["jacobi_operator_diagonal"] = dict(
+            value=n * args.steps / j32["elapsed"], ms_per_step=j32["elapsed"] / args.steps * 1e3,
+            cg_iterations_per_step=j32["cg"], newton_iterations_per_step=j32["newton"],
+            fp64_value=n * args.steps / j64["elapsed"], fp64_cg_iterations_per_step=j64["cg"],
+            speedup_vs_fp64=j64["elapsed"] / j32["elapsed"])
     return dict(n=n, dt=dt, dx=scene["dx"], elapsed=r32["elapsed"], kt=r32["kt"], g_nodes=g_nodes,
                 roof_kernels=("hvp", "state"),
                 bw_kernels=dict(hvp="hvp", state="state", p2g="p2g_apic", g2p="g2p"),
@@ -354,6 +371,8 @@ def main():
     ap.add_argument("--workload", default="implicit-boxes",
                     choices=["implicit-boxes", "explicit-cube", "admm-slabs", "fracture-sphere", "adjoint-cube"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--preconditioner", choices=sorted(PRECONDITIONERS), default="mass",
+                    help="CG preconditioner of the implicit-boxes workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (5 launches) to scale HBM counters")

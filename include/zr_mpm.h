@@ -41,6 +41,13 @@ enum {
    step; the SVD, the stress, inner products and norms stay FP64. */
 enum { ZR_FP64 = 0, ZR_FP32_REFINED = 1 };
 enum { ZR_TRANSFER_MLS = 0, ZR_TRANSFER_APIC = 1 };
+/* CG preconditioner of the implicit step: the lumped mass, z = r / m
+   (BackwardEuler::precondition, MpmSimulationBase.cpp:617-623), or
+   JacobiPreconditioner (KrylovSolvers.h:32-57) on the operator's diagonal
+   m + dt^2 sum_p (V0 dP(e_d h^T) h)_d, h = Fn^T grad w, recomputed every
+   Newton iteration; a node a collider touches gets its largest entry on
+   every axis. */
+enum { ZR_PRECOND_MASS = 0, ZR_PRECOND_JACOBI = 1 };
 
 /* elasticity models (Lib/Ziran/Physics/ConstitutiveModel/): fixed-corotated
    (no reference counterpart; the default), StvkWithHencky, NeoHookeanBorden.
@@ -200,6 +207,8 @@ typedef struct zr_implicit_opts {
   int32_t precision;             /* ZR_FP64 or ZR_FP32_REFINED */
   int32_t cg_check_interval;     /* CG iterations between host polls of the
                                     device-side termination flag (0: 8) */
+  int32_t preconditioner;        /* ZR_PRECOND_MASS or ZR_PRECOND_JACOBI */
+  int32_t reserved;
 } zr_implicit_opts;
 
 typedef struct zr_implicit_stats {
@@ -352,10 +361,14 @@ enum {
   ZR_DUMP_GRID_VSTAR = 10,  /* f64[3*32P] new_v after the grid update */
   ZR_DUMP_NODE_IDX = 11,    /* i64[32P] GridState::idx (getNumNodes numbering) */
   ZR_DUMP_DV = 12,          /* f64[3*32P] Newton result dv of the last backward-Euler step */
-  ZR_DUMP_HVP_XCD = 13      /* u32[20] operator-apply group ranges per XCD (FP32, then FP64;
+  ZR_DUMP_HVP_XCD = 13,     /* u32[20] operator-apply group ranges per XCD (FP32, then FP64;
                                9 bounds each) of the last backward-Euler step, then the block
                                counts of the two launches; no reference counterpart: a
                                scheduling detail of this implementation */
+  ZR_DUMP_JACOBI = 14       /* f64[3*32P] JacobiPreconditioner's inverse diagonal (ZR_PRECOND_JACOBI)
+                               at the Newton state of the last backward-Euler solve, computed by
+                               the call in that solve's precision (after zr_implicit_residual:
+                               at its dv) */
 };
 int64_t zr_debug_size(zr_engine* e, int32_t what);
 int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes);

@@ -985,6 +985,15 @@ struct zr_engine {
   uint32_t* hvp_order = nullptr;   // their hand-out orders (2 x hvp_order_cap)
   uint8_t* cell_order = nullptr;   // each group's cells by particle count (32 x hvp_order_cap)
   bool cell_order_set = false;     // written for this sort's groups
+  // Jacobi preconditioner: staged diagonal sums per group (kImpStage each),
+  // the inverse diagonal [3][grid_cap] in the CG vectors' precision (double
+  // storage), the nodes a collider touched at the step start
+  double* jstage = nullptr;
+  size_t jstage_cap = 0;
+  double* pinv = nullptr;
+  uint8_t* collided = nullptr;
+  size_t jac_cap = 0, collided_cap = 0;
+  bool jacobi = false;             // this step's CG uses it
   size_t hvp_order_cap = 0;
   double* hsc = nullptr;
   int32_t* hci = nullptr;
@@ -1669,17 +1678,25 @@ int read_scalars(zr_engine* e)
   return ZR_OK;
 }
 
-// Newton state at the current dv + residual + its squared mass norm (device)
+// Newton state at the current dv + residual + its squared mass norm (device);
+// with jac, also the operator diagonal's particle sums for the Jacobi
+// preconditioner (staged in e->jstage)
 template <class T>
-int be_state_residual(zr_engine* e)
+int be_state_residual(zr_engine* e, bool jac = false)
 {
   const double dx = e->cfg.dx, dt = e->be_dt;
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
   ColliderSet cs = collider_set(e);
+  uint8_t* co = e->cell_order_set ? nullptr : e->cell_order;
   prof_mark(e, ZR_PROF_STATE);
-  k_be_state<T><<<e->A, 256, 0, e->stream>>>(pview(e), hvp_cache<T>(e), e->order, e->group_start, e->group_page,
-                                             e->page_slot, e->ivn, e->idv, (int64_t)e->grid_cap, dx, 1.0 / dx, dt,
-                                             e->stage, e->cell_order_set ? nullptr : e->cell_order);
+  if (jac)
+    k_be_state<T, true><<<e->A, 256, 0, e->stream>>>(pview(e), hvp_cache<T>(e), e->order, e->group_start,
+                                                     e->group_page, e->page_slot, e->ivn, e->idv,
+                                                     (int64_t)e->grid_cap, dx, 1.0 / dx, dt, e->stage, co, e->jstage);
+  else
+    k_be_state<T, false><<<e->A, 256, 0, e->stream>>>(pview(e), hvp_cache<T>(e), e->order, e->group_start,
+                                                      e->group_page, e->page_slot, e->ivn, e->idv,
+                                                      (int64_t)e->grid_cap, dx, 1.0 / dx, dt, e->stage, co, nullptr);
   ZR_LAUNCH_CHECK(e);
   e->cell_order_set = true;  // the cells do not move within a step
   prof_mark(e, ZR_PROF_STATE);
@@ -1719,8 +1736,39 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, int32_t* ci, i
   return ZR_OK;
 }
 
+// buffers of the Jacobi preconditioner for the current grid and groups
+int ensure_jacobi(zr_engine* e)
+{
+  if (e->jac_cap < e->grid_cap) {
+    dfree(e->pinv);
+    e->jac_cap = 0;
+    ZR_HIP(e, hipMalloc(&e->pinv, 3 * e->grid_cap * sizeof(double)));
+    e->jac_cap = e->grid_cap;
+  }
+  if (e->jstage_cap < (size_t)e->A || !e->jstage) {
+    dfree(e->jstage);
+    e->jstage_cap = 0;
+    const size_t cap = (size_t)e->A + e->A / 4 + 64;
+    ZR_HIP(e, hipMalloc(&e->jstage, cap * kImpStage * sizeof(double)));
+    e->jstage_cap = cap;
+  }
+  return ZR_OK;
+}
+
+// the inverse diagonal from the sums that the last k_be_state<T, true> staged
+template <class T>
+int jacobi_inverse(zr_engine* e)
+{
+  const uint32_t nn = e->S * 32;
+  k_jacobi<T><<<blocks_for(nn, 256), 256, 0, e->stream>>>(nn, e->slot_src, e->jstage, e->gm, e->collided, e->be_dt,
+                                                          nvT(e, reinterpret_cast<T*>(e->pinv)));
+  ZR_LAUNCH_CHECK(e);
+  return ZR_OK;
+}
+
 // ConjugateGradient::solve (ConjugateGradient.h:99-147) on A x = r with the
-// mass-lumped preconditioner in the vectors' precision, FP64 inner products;
+// mass-lumped or the Jacobi preconditioner in the vectors' precision, FP64
+// inner products;
 // termination is decided on the device and the host polls the flag every
 // check_interval iterations
 template <class V>
@@ -1729,8 +1777,9 @@ int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
   using T = typename V::T;
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
   hipStream_t s = e->stream;
+  const NodeVec<T> pinv = nvT(e, e->jacobi ? reinterpret_cast<T*>(e->pinv) : (T*)nullptr);
   k_cg_init<T><<<nb, 256, 0, s>>>(nn, e->gm, nv(e, e->ir), nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
-                                  nvT(e, V::p(e)), e->partial);
+                                  nvT(e, V::p(e)), pinv, e->partial);
   k_reduce_cg<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci, RED_CG_INIT, 0, rel, o->cg_tolerance);
   ZR_LAUNCH_CHECK(e);
   const int check = o->cg_check_interval > 0 ? o->cg_check_interval : 8;
@@ -1743,7 +1792,7 @@ int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
     int rc = be_apply<T>(e, V::q(e), V::p(e), V::ap(e), 1, e->ci, k);
     if (rc) return rc;
     k_cg_update<T><<<ncg, 256, 0, s>>>(nn, e->gm, nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
-                                       nvT(e, V::p(e)), nvT(e, V::ap(e)), e->sc, e->partial, ncg, e->partial2,
+                                       nvT(e, V::p(e)), nvT(e, V::ap(e)), pinv, e->sc, e->partial, ncg, e->partial2,
                                        e->ci, k);
     ZR_LAUNCH_CHECK(e);
     if ((k + 1) % check == 0 && k + 1 < o->cg_max_iterations) {
@@ -1771,7 +1820,7 @@ int be_newton(zr_engine* e, const zr_implicit_opts* o, zr_implicit_stats& local)
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
   // NewtonsMethod::solve (NewtonsMethod.h:42-66)
   for (int it = 0; it < o->newton_max_iterations; ++it) {
-    int rc = be_state_residual<T>(e);
+    int rc = be_state_residual<T>(e, e->jacobi);
     if (rc) return rc;
     rc = read_scalars(e);
     if (rc) return rc;
@@ -1783,6 +1832,10 @@ int be_newton(zr_engine* e, const zr_implicit_opts* o, zr_implicit_stats& local)
     }
     const double rel = o->cg_relative_tolerance > 0.0 ? o->cg_relative_tolerance
                                                       : fmin(0.5, sqrt(fmax(norm, o->newton_tolerance)));
+    if (e->jacobi) {
+      rc = jacobi_inverse<T>(e);
+      if (rc) return rc;
+    }
     rc = be_cg<V>(e, rel, o);
     if (rc) return rc;
     k_axpy3<T><<<nb, 256, 0, e->stream>>>(nn, nv(e, e->idv), nvT(e, V::x(e)));
@@ -1817,11 +1870,23 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
   memset(&local, 0, sizeof(local));
   e->be_dt = dt;
   e->be_prec = o->precision;
+  e->jacobi = o->preconditioner == ZR_PRECOND_JACOBI;
+  if (e->jacobi) {
+    rc = ensure_jacobi(e);
+    if (rc) return rc;
+  }
+  if (e->collided_cap < e->grid_cap) {  // nodes a collider touches, for the Jacobi preconditioner
+    dfree(e->collided);
+    e->collided_cap = 0;
+    ZR_HIP(e, hipMalloc(&e->collided, e->grid_cap));
+    e->collided_cap = e->grid_cap;
+  }
   const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256);
   hipStream_t s = e->stream;
   ColliderSet cs = collider_set(e);
   k_be_init<<<nb, 256, 0, s>>>(nn, e->slot_page, e->gm, e->gv, nv(e, e->ivn), nv(e, e->idv), e->cfg.dx, dt,
-                               e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs);
+                               e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs,
+                               e->collided);
   ZR_HIP(e, hipMemsetAsync(e->ci, 0, CI_N * sizeof(int32_t), s));
   ZR_LAUNCH_CHECK(e);
   e->be_ready = true;
@@ -1913,7 +1978,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->speed); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
-  dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
+  dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->jstage); dfree(e->pinv); dfree(e->collided); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   if (e->hsc) (void)hipHostFree(e->hsc);
@@ -2525,7 +2590,8 @@ int zr_particles_to_grid_apic(zr_engine* e)
 static bool valid_opts(const zr_implicit_opts* o)
 {
   return o && o->newton_max_iterations >= 0 && o->cg_max_iterations >= 0 &&
-         (o->precision == ZR_FP64 || o->precision == ZR_FP32_REFINED);
+         (o->precision == ZR_FP64 || o->precision == ZR_FP32_REFINED) &&
+         (o->preconditioner == ZR_PRECOND_MASS || o->preconditioner == ZR_PRECOND_JACOBI);
 }
 
 static bool implicit_materials_ok(zr_engine* e)
@@ -2658,6 +2724,7 @@ int64_t zr_debug_size(zr_engine* e, int32_t what)
     case ZR_DUMP_NODE_IDX: return (int64_t)e->S * 32 * 8;
     case ZR_DUMP_DV: return e->be_ready ? (int64_t)e->S * 32 * 3 * 8 : 0;
     case ZR_DUMP_HVP_XCD: return e->be_ready ? 20 * 4 : 0;
+    case ZR_DUMP_JACOBI: return e->be_ready ? (int64_t)e->S * 32 * 3 * 8 : 0;
     default: return -1;
   }
 }
@@ -2722,6 +2789,33 @@ int zr_debug_dump(zr_engine* e, int32_t what, void* host, size_t bytes)
     double* o = (double*)host;
     for (uint32_t k = 0; k < nn; ++k)
       for (int d = 0; d < 3; ++d) o[3 * (size_t)k + d] = h[(size_t)d * nn + k];
+    return ZR_OK;
+  }
+  if (what == ZR_DUMP_JACOBI) {
+    int rc = ensure_jacobi(e);
+    if (rc) return rc;
+    const bool f32 = e->be_prec == ZR_FP32_REFINED;
+    rc = f32 ? be_state_residual<float>(e, true) : be_state_residual<double>(e, true);
+    if (rc) return rc;
+    rc = f32 ? jacobi_inverse<float>(e) : jacobi_inverse<double>(e);
+    if (rc) return rc;
+    ZR_HIP(e, hipStreamSynchronize(s));
+    std::vector<double> h(3 * (size_t)nn);
+    if (f32) {
+      std::vector<float> hf(nn);
+      for (int d = 0; d < 3; ++d) {
+        ZR_HIP(e, hipMemcpy(hf.data(), reinterpret_cast<const float*>(e->pinv) + (size_t)d * e->grid_cap,
+                            nn * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nn; ++k) h[(size_t)d * nn + k] = hf[k];
+      }
+    } else {
+      for (int d = 0; d < 3; ++d)
+        ZR_HIP(e, hipMemcpy(h.data() + (size_t)d * nn, e->pinv + (size_t)d * e->grid_cap, nn * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    double* out = static_cast<double*>(host);
+    for (size_t k = 0; k < nn; ++k)
+      for (int d = 0; d < 3; ++d) out[3 * k + d] = h[(size_t)d * nn + k];
     return ZR_OK;
   }
   if (what == ZR_DUMP_HVP_XCD) {

@@ -341,6 +341,49 @@ __device__ __forceinline__ T wgradT(const T w[3][3], const T dw[3][3], T inv_dx,
 __device__ __forceinline__ void opaque(double& v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void opaque(float& v) { asm volatile("" : "+v"(v)); }
 
+// quotient in the operator's precision: IEEE division in FP64; in FP32 the
+// hardware reciprocal (1 ulp), well inside the FP32 operator's own rounding
+template <class T>
+__device__ __forceinline__ T op_div(T a, T b)
+{
+  return a / b;
+}
+template <>
+__device__ __forceinline__ float op_div<float>(float a, float b)
+{
+  return a * __builtin_amdgcn_rcpf(b);
+}
+
+// fixed-corotated differential in the singular frame: for F = U diag(s) V^T
+// and M = U^T dF V, H = U^T (V0 dP) V with k2mu = 2 mu V0, klam = lam V0
+// (the rotation's part through W = (M - M^T) / (s_i + s_j), |s_i + s_j|
+// kept >= 1e-12)
+template <class T>
+__device__ __forceinline__ void fcr_frame_differential(const T s[3], T k2mu, T klam, const T M[3][3], T H[3][3])
+{
+  const T J = s[0] * s[1] * s[2];
+  const T cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
+  const T cm = cof[0] * M[0][0] + cof[1] * M[1][1] + cof[2] * M[2][2];
+  const T eps = (T)1e-12;
+  T d01 = s[0] + s[1], d02 = s[0] + s[2], d12 = s[1] + s[2];
+  if (fabs(d01) < eps) d01 = d01 < (T)0 ? -eps : eps;
+  if (fabs(d02) < eps) d02 = d02 < (T)0 ? -eps : eps;
+  if (fabs(d12) < eps) d12 = d12 < (T)0 ? -eps : eps;
+  const T W01 = op_div(M[0][1] - M[1][0], d01);
+  const T W02 = op_div(M[0][2] - M[2][0], d02);
+  const T W12 = op_div(M[1][2] - M[2][1], d12);
+  const T lj = klam * (J - (T)1);
+  H[0][0] = k2mu * M[0][0] + klam * cm * cof[0] + lj * (s[2] * M[1][1] + s[1] * M[2][2]);
+  H[1][1] = k2mu * M[1][1] + klam * cm * cof[1] + lj * (s[2] * M[0][0] + s[0] * M[2][2]);
+  H[2][2] = k2mu * M[2][2] + klam * cm * cof[2] + lj * (s[1] * M[0][0] + s[0] * M[1][1]);
+  H[0][1] = k2mu * (M[0][1] - W01) - lj * s[2] * M[1][0];
+  H[1][0] = k2mu * (M[1][0] + W01) - lj * s[2] * M[0][1];
+  H[0][2] = k2mu * (M[0][2] - W02) - lj * s[1] * M[2][0];
+  H[2][0] = k2mu * (M[2][0] + W02) - lj * s[1] * M[0][2];
+  H[1][2] = k2mu * (M[1][2] - W12) - lj * s[0] * M[2][1];
+  H[2][1] = k2mu * (M[2][1] + W12) - lj * s[0] * M[1][2];
+}
+
 constexpr int kCells = 32;  // 2 x 4 x 4 cells of a page
 
 // Newton state: F = (I + dt grad(vn + dv)) Fn (FP64, with the SVD), the
@@ -349,8 +392,12 @@ constexpr int kCells = 32;  // 2 x 4 x 4 cells of a page
 // operator data of every particle in sorted order, in precision T; unless
 // cell_order is null, with the cells of each group's first 256 particles by
 // particle count, largest first (ties by cell), in cell_order[32 g ..] for
-// the operator apply's task deal
-template <class T>
+// the operator apply's task deal.  JAC: also the operator's diagonal
+// without the mass, sum_p (V0 dP(e_d h^T) h)_d with h = Fn^T grad w_ip, staged
+// per group in dstage (the node kernel adds m and dt^2), for the Jacobi
+// preconditioner; in the singular frame, with u = U^T e_d and g = Q^T grad w,
+// it is u^T H(u g^T) g = sum_c g_c u^T H(u e_c^T) g (H linear in M).
+template <class T, bool JAC>
 __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> cache,
                                                   const uint32_t* __restrict__ order,
                                                   const uint32_t* __restrict__ group_start,
@@ -358,15 +405,18 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
                                                   const int32_t* __restrict__ page_slot, const double* __restrict__ vn,
                                                   const double* __restrict__ dv, int64_t gstride, double dx,
                                                   double inv_dx, double dt, double* __restrict__ stage,
-                                                  uint8_t* __restrict__ cell_order)
+                                                  uint8_t* __restrict__ cell_order, double* __restrict__ dstage)
 {
   __shared__ double tile[4 * kImpStage];
+  __shared__ double dtile[JAC ? 4 * kImpStage : 1];
   __shared__ double tv[3 * kTileNodes];
   __shared__ int32_t nslot[8];
   __shared__ int ccount[kCells];
   const int tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
   for (int k = tid; k < 4 * kImpStage; k += 256) tile[k] = 0.0;
+  if (JAC)
+    for (int k = tid; k < 4 * kImpStage; k += 256) dtile[k] = 0.0;
   if (tid < kCells) ccount[tid] = 0;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
@@ -375,6 +425,7 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
   load_tile3(px, py, pz, page_slot, vn, dv, gstride, tv, nslot);
   const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
   double* wt = tile + (tid >> 6) * kImpStage;
+  double* wd = dtile + (JAC ? (tid >> 6) * kImpStage : 0);
   const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
   const int64_t S = P.stride, CS = cache.stride;
   for (uint32_t j0 = 0; j0 < n; j0 += 256) {
@@ -465,6 +516,30 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         vp[r][c] = vol * (Pm[r][0] * Fn.a[c][0] + Pm[r][1] * Fn.a[c][1] + Pm[r][2] * Fn.a[c][2]);
+    // JAC: Kd[d][c][r] = (u^T H(u e_c^T))_r, u = row d of U; Qm = Fn V
+    double Kd[JAC ? 3 : 1][3][3], Qm[3][3];
+    if (JAC) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          Qm[r][c] = Fn.a[r][0] * v.a[0][c] + Fn.a[r][1] * v.a[1][c] + Fn.a[r][2] * v.a[2][c];
+      const double k2mu = 2.0 * mu * vol, klam = lam * vol;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double M[3][3], H[3][3];
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) M[r][q] = q == c ? u.a[d][r] : 0.0;
+          fcr_frame_differential(s, k2mu, klam, M, H);
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            Kd[JAC ? d : 0][c][q] = u.a[d][0] * H[0][q] + u.a[d][1] * H[1][q] + u.a[d][2] * H[2][q];
+        }
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -484,12 +559,31 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
 #pragma unroll
           for (int r = 0; r < 3; ++r)
             atomicAdd(&wt[r * kTileNodes + t], vp[r][0] * gw[0] + vp[r][1] * gw[1] + vp[r][2] * gw[2]);
+          if (JAC) {
+            double gq[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) gq[q] = Qm[0][q] * gw[0] + Qm[1][q] * gw[1] + Qm[2][q] * gw[2];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              double acc = 0.0;
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                acc += gq[c] * (Kd[JAC ? d : 0][c][0] * gq[0] + Kd[JAC ? d : 0][c][1] * gq[1] +
+                                Kd[JAC ? d : 0][c][2] * gq[2]);
+              atomicAdd(&wd[d * kTileNodes + t], acc);
+            }
+          }
         }
   }
   __syncthreads();
   double* out = stage + (size_t)g * kImpStage;
   for (int k = tid; k < kImpStage; k += 256)
     out[k] = tile[k] + tile[kImpStage + k] + tile[2 * kImpStage + k] + tile[3 * kImpStage + k];
+  if (JAC) {
+    double* dout = dstage + (size_t)g * kImpStage;
+    for (int k = tid; k < kImpStage; k += 256)
+      dout[k] = dtile[k] + dtile[kImpStage + k] + dtile[2 * kImpStage + k] + dtile[3 * kImpStage + k];
+  }
   if (cell_order && tid < kCells) {
     const int cnt = ccount[tid];
     int rank = 0;
@@ -558,19 +652,6 @@ constexpr bool hvp_task_slots_valid()
 static_assert(hvp_task_slots_valid(), "node-major layout is a bijection");
 static_assert(hvp_task_slots().v[0] == (0u | (2u << 10) | (5u << 20)), "slots of task (cell 0, a 0, b 0)");
 __constant__ HvpTaskSlots kHvpTaskSlots = hvp_task_slots();
-
-// quotient in the operator's precision: IEEE division in FP64; in FP32 the
-// hardware reciprocal (1 ulp), well inside the FP32 operator's own rounding
-template <class T>
-__device__ __forceinline__ T op_div(T a, T b)
-{
-  return a / b;
-}
-template <>
-__device__ __forceinline__ float op_div<float>(float a, float b)
-{
-  return a * __builtin_amdgcn_rcpf(b);
-}
 
 // particles per chunk of the operator apply, and the waves per SIMD its LDS
 // allows (the register budget is held to it)
@@ -992,28 +1073,8 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
         for (int r = 0; r < 3; ++r)
 #pragma unroll
           for (int c = 0; c < 3; ++c) M[r][c] = u[0][r] * T1[0][c] + u[1][r] * T1[1][c] + u[2][r] * T1[2][c];
-        const T J = s[0] * s[1] * s[2];
-        const T cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
-        const T cm = cof[0] * M[0][0] + cof[1] * M[1][1] + cof[2] * M[2][2];
-        const T eps = (T)1e-12;
-        T d01 = s[0] + s[1], d02 = s[0] + s[2], d12 = s[1] + s[2];
-        if (fabs(d01) < eps) d01 = d01 < (T)0 ? -eps : eps;
-        if (fabs(d02) < eps) d02 = d02 < (T)0 ? -eps : eps;
-        if (fabs(d12) < eps) d12 = d12 < (T)0 ? -eps : eps;
-        const T W01 = op_div(M[0][1] - M[1][0], d01);
-        const T W02 = op_div(M[0][2] - M[2][0], d02);
-        const T W12 = op_div(M[1][2] - M[2][1], d12);
-        const T lj = klam * (J - (T)1);
         T H[3][3];
-        H[0][0] = k2mu * M[0][0] + klam * cm * cof[0] + lj * (s[2] * M[1][1] + s[1] * M[2][2]);
-        H[1][1] = k2mu * M[1][1] + klam * cm * cof[1] + lj * (s[2] * M[0][0] + s[0] * M[2][2]);
-        H[2][2] = k2mu * M[2][2] + klam * cm * cof[2] + lj * (s[1] * M[0][0] + s[0] * M[1][1]);
-        H[0][1] = k2mu * (M[0][1] - W01) - lj * s[2] * M[1][0];
-        H[1][0] = k2mu * (M[1][0] + W01) - lj * s[2] * M[0][1];
-        H[0][2] = k2mu * (M[0][2] - W02) - lj * s[1] * M[2][0];
-        H[2][0] = k2mu * (M[2][0] + W02) - lj * s[1] * M[0][2];
-        H[1][2] = k2mu * (M[1][2] - W12) - lj * s[0] * M[2][1];
-        H[2][1] = k2mu * (M[2][1] + W12) - lj * s[0] * M[1][2];
+        fcr_frame_differential(s, k2mu, klam, M, H);
         // ds = U H Q^T, stored times 1/dx^2 (the gather's and the scatter's 1/dx)
         T T2[3][3];
 #pragma unroll
@@ -1145,13 +1206,14 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
 __global__ __launch_bounds__(256) void k_be_init(uint32_t nn, const uint32_t* __restrict__ slot_page,
                                                  const double* __restrict__ gm, const double* __restrict__ gv,
                                                  NodeVec3 vn, NodeVec3 dv, double dx, double dt, double gx, double gy,
-                                                 double gz, double time, ColliderSet cs)
+                                                 double gz, double time, ColliderSet cs, uint8_t* __restrict__ collided)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   if (k >= nn) return;
   const double m = gm[k];
   if (m == 0.0) {
     for (int d = 0; d < 3; ++d) vn(d, k) = dv(d, k) = 0.0;
+    if (collided) collided[k] = 0;
     return;
   }
   const int64_t S = vn.stride;
@@ -1160,6 +1222,7 @@ __global__ __launch_bounds__(256) void k_be_init(uint32_t nn, const uint32_t* __
   double xi[3];
   node_pos(node_coord(slot_page, k), dx, xi);
   const bool hit = multi_object_collision(cs, time, xi, vi);
+  if (collided) collided[k] = hit ? 1 : 0;
   const double g[3] = { gx, gy, gz };
   for (int d = 0; d < 3; ++d) {
     dv(d, k) = hit ? vi[d] - v[d] : g[d] * dt;
@@ -1208,12 +1271,42 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
   if (threadIdx.x == 0) sc[slot] = s;
 }
 
+// JacobiPreconditioner (KrylovSolvers.h:32-57) on the operator's diagonal
+// a_(k,d) = m_k + dt^2 (staged particle sums): pinv = 1 / max(16 eps, |a|),
+// with the node's largest |a| on every axis of a node a collider touched at
+// the step start, so that z = pinv r stays in the projected space; nodes
+// without mass hold 0
+template <class T>
+__global__ __launch_bounds__(256) void k_jacobi(uint32_t nn, const int32_t* __restrict__ slot_src,
+                                                const double* __restrict__ dstage, const double* __restrict__ gm,
+                                                const uint8_t* __restrict__ collided, double dt, NodeVec<T> pinv)
+{
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  if (k >= nn) return;
+  const double m = gm[k];
+  if (m == 0.0) {
+    for (int d = 0; d < 3; ++d) pinv(d, k) = (T)0;
+    return;
+  }
+  const uint32_t e = k & 31u;
+  double f[3];
+  sum_tiles_src<3>(dstage, slot_src + 8 * (size_t)(k >> 5), (int)(e >> 4), (int)((e >> 2) & 3u), (int)(e & 3u), f);
+  double a[3], mx = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    a[d] = fabs(m + dt * dt * f[d]);
+    mx = a[d] > mx ? a[d] : mx;
+  }
+  const bool iso = collided[k] != 0;
+  for (int d = 0; d < 3; ++d) pinv(d, k) = (T)(1.0 / fmax(16.0 * 2.220446049250313e-16, iso ? mx : a[d]));
+}
+
 // CG start (ConjugateGradient.h:105-119): x = 0, r = b, z = r / m, p = z,
 // z^T r and the local tolerance min(rel |r0|_P, tol); beta = 0 so that the
-// first direction built on the fly is p = z
+// first direction built on the fly is p = z.  z = pinv r (Jacobi) when pinv
+// is given, else r / m.
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_init(uint32_t nn, const double* __restrict__ gm, NodeVec3 b, NodeVec<T> x,
-                                                 NodeVec<T> r, NodeVec<T> q, NodeVec<T> p,
+                                                 NodeVec<T> r, NodeVec<T> q, NodeVec<T> p, NodeVec<T> pinv,
                                                  double* __restrict__ partial)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
@@ -1222,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_cg_init(uint32_t nn, const double* __re
     const double m = gm[k];
     for (int d = 0; d < 3; ++d) {
       const T rv = (T)b(d, k);
-      const T qv = m != 0.0 ? rv / (T)m : (T)0;
+      const T qv = m != 0.0 ? (pinv.p ? rv * pinv(d, k) : rv / (T)m) : (T)0;
       x(d, k) = (T)0;
       r(d, k) = rv;
       q(d, k) = qv;
@@ -1308,14 +1401,15 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
   block_partial(contrib, partial);
 }
 
-// x += alpha p, r -= alpha Ap, z = r / m (ConjugateGradient.h:127-135) with
+// x += alpha p, r -= alpha Ap, z = r / m or pinv r (ConjugateGradient.h:127-135) with
 // alpha = z^T r / p^T A p, p^T A p summed from its block partials in every
 // block; this kernel's z^T r partials go to upd_partial (grid-stride, one
 // partial per block)
 template <class T>
 __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __restrict__ gm, NodeVec<T> x,
                                                    NodeVec<T> r, NodeVec<T> q, NodeVec<T> p, NodeVec<T> ap,
-                                                   double* __restrict__ sc, const double* __restrict__ pap_partial,
+                                                   NodeVec<T> pinv, double* __restrict__ sc,
+                                                   const double* __restrict__ pap_partial,
                                                    uint32_t n_pap, double* __restrict__ upd_partial,
                                                    const int32_t* ci, int cg_iter)
 {
@@ -1323,7 +1417,7 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
   // known (they do not depend on it), so that the two waits overlap
   const uint32_t k0 = blockIdx.x * 256u + threadIdx.x, kstep = gridDim.x * 256u;
   double m = 0.0;
-  T xv[3], pv[3], rv[3], av[3];
+  T xv[3], pv[3], rv[3], av[3], jv[3];
   // a node without mass keeps its zeros (see k_cg_ap)
   auto fetch = [&](uint32_t k) {
     m = gm[k];
@@ -1334,6 +1428,7 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
         pv[d] = p(d, k);
         rv[d] = r(d, k);
         av[d] = ap(d, k);
+        jv[d] = pinv.p ? pinv(d, k) : (T)0;
       }
     }
   };
@@ -1356,7 +1451,7 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
       x(d, k) = xv[d] + pv[d] * alpha;
       const T rn = rv[d] - av[d] * alpha;
       r(d, k) = rn;
-      const T qv = m != 0.0 ? rn / (T)m : (T)0;
+      const T qv = pinv.p ? rn * jv[d] : rn / (T)m;
       q(d, k) = qv;
       contrib += (double)qv * (double)rn;
     }

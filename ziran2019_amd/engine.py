@@ -30,11 +30,11 @@ ZR_STICKY, ZR_SLIP, ZR_SEPARATE = 1, 2, 3
 ZR_HALF_SPACE, ZR_SPHERE = 0, 1
 
 DUMP = dict(sorter=0, order=1, base_offset=2, group_first=3, group_last=4, block_offset=5, pages=6,
-            grid_m=7, grid_v=8, grid_f=9, grid_vstar=10, node_idx=11, dv=12, hvp_xcd=13)
+            grid_m=7, grid_v=8, grid_f=9, grid_vstar=10, node_idx=11, dv=12, hvp_xcd=13, jacobi=14)
 _DUMP_DTYPE = dict(sorter=np.uint64, order=np.int32, base_offset=np.uint64, group_first=np.int64,
                    group_last=np.int64, block_offset=np.uint64, pages=np.uint64, grid_m=np.float64,
                    grid_v=np.float64, grid_f=np.float64, grid_vstar=np.float64, node_idx=np.int64, dv=np.float64,
-                   hvp_xcd=np.uint32)
+                   hvp_xcd=np.uint32, jacobi=np.float64)
 
 # every symbol include/zr_mpm.h declares
 EXPORTS = (
@@ -51,6 +51,7 @@ EXPORTS = (
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
 ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER, ZR_PLASTIC_CAM_CLAY = 0, 1, 2, 3
+PRECOND_MASS, PRECOND_JACOBI = 0, 1
 PROF = dict(p2g=0, grid=1, g2p=2, sort=3, state=4, hvp=5)
 
 
@@ -124,7 +125,8 @@ class ZrImplicitOpts(ctypes.Structure):
     _fields_ = [("newton_max_iterations", ctypes.c_int32), ("cg_max_iterations", ctypes.c_int32),
                 ("newton_tolerance", ctypes.c_double), ("cg_tolerance", ctypes.c_double),
                 ("cg_relative_tolerance", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("cg_check_interval", ctypes.c_int32)]
+                ("cg_check_interval", ctypes.c_int32), ("preconditioner", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class ZrImplicitStats(ctypes.Structure):
@@ -135,12 +137,16 @@ class ZrImplicitStats(ctypes.Structure):
 
 def implicit_opts(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=20,
                   cg_tolerance=16 * 2.220446049250313e-16, cg_relative_tolerance=0.0, precision=0,
-                  cg_check_interval=8):
+                  cg_check_interval=8, preconditioner=0):
     """Reference defaults: NewtonsMethod(1e-5, 3) (MpmSimulationBase.cpp:51),
     ConjugateGradient(20) (BackwardEuler.h:90-91), LinearSolver::tolerance
-    16 eps (LinearSolver.h:26), Newton-adaptive relative tolerance."""
+    16 eps (LinearSolver.h:26), Newton-adaptive relative tolerance, the
+    mass-lumped preconditioner (MpmSimulationBase.cpp:617-623);
+    preconditioner=1 is JacobiPreconditioner (KrylovSolvers.h:32-57) on the
+    operator's diagonal."""
     return ZrImplicitOpts(int(newton_max_iterations), int(cg_max_iterations), float(newton_tolerance),
-                          float(cg_tolerance), float(cg_relative_tolerance), int(precision), int(cg_check_interval))
+                          float(cg_tolerance), float(cg_relative_tolerance), int(precision), int(cg_check_interval),
+                          int(preconditioner), 0)
 
 
 class ZrParticles(ctypes.Structure):
@@ -605,6 +611,6 @@ class MpmEngine:
             raise ZrError(f"bad dump {what}")
         out = np.zeros(size // np.dtype(_DUMP_DTYPE[what]).itemsize, dtype=_DUMP_DTYPE[what])
         self._check(self.L.zr_debug_dump(self.h, code, out.ctypes.data_as(ctypes.c_void_p), size), "zr_debug_dump")
-        if what in ("grid_v", "grid_f", "grid_vstar", "dv"):
+        if what in ("grid_v", "grid_f", "grid_vstar", "dv", "jacobi"):
             out = out.reshape(-1, 3)
         return out

@@ -41,6 +41,7 @@ zr_implicit_opts MpmSimulationGpu::implicitOpts() const
   o.cg_relative_tolerance = cg_relative_tolerance;
   o.precision = implicit_precision;
   o.cg_check_interval = 8;
+  o.preconditioner = preconditioner;
   return o;
 }
 

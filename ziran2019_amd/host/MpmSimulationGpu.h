@@ -75,6 +75,7 @@ public:
   double cg_tolerance = 16 * 2.220446049250313e-16;  // LinearSolver.h:26
   double cg_relative_tolerance = 0.0;      // <= 0: Newton-adaptive
   int32_t implicit_precision = ZR_FP64;    // or ZR_FP32_REFINED
+  int32_t preconditioner = ZR_PRECOND_MASS;  // or ZR_PRECOND_JACOBI (KrylovSolvers.h:32-57)
   double time = 0.0;                       // SimulationBase::step.time
   TimeStepping step;                       // SimulationBase::step
 
