@@ -78,14 +78,27 @@ typedef struct zr_config {
 /* An analytic collision object with a constant-velocity translation:
    AnalyticCollisionObject + HalfSpace / Sphere level sets
    (CollisionObject.cpp:48-105,340-405; AnalyticLevelSet.cpp:259-287,435-450). */
+/* An analytic collision object moving as x = R s X + b(t) with
+   b(t) = translation + velocity t (AnalyticCollisionObject,
+   CollisionObject.h:81-172): the level set is queried at
+   X = R^T (x - b) / s, its normal is R N, and the object's velocity at x is
+   omega x (x - b) + (ds/dt / s)(x - b) + velocity (CollisionObject.cpp:340-405).
+   R, omega, s and ds/dt are the state the reference's updateState callback
+   sets; a caller that changes them over time sets the colliders again.
+   Zero-initialised extra fields give the static frame (R = I, s = 1). */
 typedef struct zr_collider {
   int32_t shape;        /* ZR_HALF_SPACE or ZR_SPHERE */
   int32_t type;         /* ZR_STICKY, ZR_SLIP or ZR_SEPARATE */
   double friction;
-  double origin[3];     /* half-space point / sphere centre at time 0 */
-  double normal[3];     /* half-space outward normal */
+  double origin[3];     /* half-space point / sphere centre (material space) */
+  double normal[3];     /* half-space outward normal (material space) */
   double radius;        /* sphere radius */
   double velocity[3];   /* dbdt */
+  double translation[3];/* b at time 0 */
+  double rotation[4];   /* R as a quaternion <w,x,y,z> (setRotation); all 0: identity */
+  double omega[3];      /* angular velocity (setAngularVelocity) */
+  double scale;         /* s (setScaling); 0: 1 */
+  double dscale_dt;     /* ds/dt */
 } zr_collider;
 
 /* Host views of the particle attributes (Particles m/X/V plus the F-based

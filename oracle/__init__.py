@@ -86,6 +86,11 @@ class ZroCollider(ctypes.Structure):
         ("normal", _d * 3),
         ("radius", _d),
         ("velocity", _d * 3),
+        ("translation", _d * 3),
+        ("rotation", _d * 4),
+        ("omega", _d * 3),
+        ("scale", _d),
+        ("dscale_dt", _d),
     ]
 
 
@@ -493,9 +498,11 @@ class OracleSim:
         return dict(pstate=st, broken=b, g=g)
 
     def add_collider(self, shape, type, origin, normal=(0, 1, 0), radius=0.0,
-                     velocity=(0, 0, 0), friction=0.0):
+                     velocity=(0, 0, 0), friction=0.0, translation=(0, 0, 0), rotation=(0, 0, 0, 0),
+                     omega=(0, 0, 0), scale=0.0, dscale_dt=0.0):
         c = ZroCollider(int(shape), int(type), float(friction), (_d * 3)(*origin),
-                        (_d * 3)(*normal), float(radius), (_d * 3)(*velocity))
+                        (_d * 3)(*normal), float(radius), (_d * 3)(*velocity), (_d * 3)(*translation),
+                        (_d * 4)(*rotation), (_d * 3)(*omega), float(scale), float(dscale_dt))
         assert lib().zro_add_collider(self.h, ctypes.byref(c)) == 0
 
     def set_time(self, t):

@@ -89,6 +89,11 @@ typedef struct {
   double normal[3];     /* half-space outward normal (normalised here) */
   double radius;
   double velocity[3];   /* constant translation velocity (dbdt) */
+  double translation[3];/* b at t = 0: b(t) = translation + velocity t */
+  double rotation[4];   /* R as a quaternion <w,x,y,z>; all 0: identity */
+  double omega[3];      /* angular velocity */
+  double scale;         /* s; 0: 1 */
+  double dscale_dt;     /* ds/dt */
 } zro_collider;
 
 zro_sim* zro_create(double dx, const double* gravity, double cfl, int nthreads);

@@ -552,6 +552,11 @@ void zro_eval_max_particle_speed(const zro_sim* s, double out[7])
   for (int d = 0; d < 3; ++d) out[4 + d] = -out[4 + d];
 }
 
+static void collider_b(const zro_sim* s, const zro_collider* c, double t, double* b);
+static void collider_rotation(const zro_collider* c, double R[9]);
+static double collider_scale(const zro_collider* c);
+static void collider_velocity(const zro_collider* c, const double* xb, double* vo);
+
 /* calculateDt (MpmSimulationBase.cpp:766-790): the corners widened by
    (degree + 2) dx only bound the colliders' speed queries, which for a
    constant translation is |dbdt| anywhere (CollisionObject.cpp:158-194) */
@@ -560,9 +565,52 @@ double zro_calculate_dt(const zro_sim* s, double max_dt)
   double r[7];
   zro_eval_max_particle_speed(s, r);
   double speed = r[0];
+  /* AnalyticCollisionObject::evalMaxSpeed (CollisionObject.cpp:158-194) over
+     the particle box widened by (degree + 2) dx (MpmSimulationBase.cpp:774-781):
+     with spin or scaling, the object's velocity at the corners of either box
+     that the other box's "any axis" test keeps; a sphere's box is its centre
+     +- radius, a half-space has none (-1) */
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) { hi[d] = r[1 + d] + 4.0 * s->dx; lo[d] = r[4 + d] - 4.0 * s->dx; }
   for (int k = 0; k < s->ncol; ++k) {
-    const double* u = s->col[k].velocity;
-    const double cs = sqrt((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+    const zro_collider* c = &s->col[k];
+    const double* u = c->velocity;
+    double cs = 0.0;
+    if (c->dscale_dt != 0.0 || c->omega[0] != 0.0 || c->omega[1] != 0.0 || c->omega[2] != 0.0) {
+      if (c->shape != 1) return -1.0;
+      double R[9], b[3], mn[3], mx[3], pts[16][3];
+      int np = 0;
+      collider_rotation(c, R);
+      double sc = collider_scale(c), inv_s = 1.0 / sc;
+      collider_b(s, c, s->time, b);
+      for (int d = 0; d < 3; ++d) { mn[d] = c->origin[d] - c->radius; mx[d] = c->origin[d] + c->radius; }
+      for (int i = 0; i < 8; ++i) {
+        double x[3], X[3];
+        for (int d = 0; d < 3; ++d) x[d] = (i & (1 << d)) ? lo[d] : hi[d];
+        for (int d = 0; d < 3; ++d)
+          X[d] = (R[d] * (x[0] - b[0]) + R[3 + d] * (x[1] - b[1]) + R[6 + d] * (x[2] - b[2])) * inv_s;
+        int a1 = mn[0] < X[0] || mn[1] < X[1] || mn[2] < X[2];
+        int a2 = X[0] < mx[0] || X[1] < mx[1] || X[2] < mx[2];
+        if (a1 && a2) { memcpy(pts[np], x, sizeof(x)); ++np; }
+      }
+      for (int i = 0; i < 8; ++i) {
+        double X[3], x[3];
+        for (int d = 0; d < 3; ++d) X[d] = (i & (1 << d)) ? mn[d] : mx[d];
+        for (int d = 0; d < 3; ++d) x[d] = sc * (R[3 * d] * X[0] + R[3 * d + 1] * X[1] + R[3 * d + 2] * X[2]) + b[d];
+        int a1 = lo[0] < x[0] || lo[1] < x[1] || lo[2] < x[2];
+        int a2 = x[0] < hi[0] || x[1] < hi[1] || x[2] < hi[2];
+        if (a1 && a2) { memcpy(pts[np], x, sizeof(x)); ++np; }
+      }
+      for (int i = 0; i < np; ++i) {
+        double xb[3], v[3];
+        for (int d = 0; d < 3; ++d) xb[d] = pts[i][d] - b[d];
+        collider_velocity(c, xb, v);
+        double sp = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        if (sp > cs) cs = sp;
+      }
+    } else {
+      cs = sqrt((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+    }
     if (cs > speed) speed = cs;
   }
   double dt = max_dt;
@@ -584,11 +632,30 @@ double zro_next_dt(double dt, double min_dt, double max_dt, double max_dt_origin
 }
 
 /* ---------------- colliders ---------------- */
+/* the object's frame x = R s X + b(t), b(t) = translation + velocity t
+   (CollisionObject.h:81-172) */
 static void collider_b(const zro_sim* s, const zro_collider* c, double t, double* b)
 {
   (void)s;
-  for (int d = 0; d < 3; ++d) b[d] = c->velocity[d] * t;
+  for (int d = 0; d < 3; ++d) b[d] = c->translation[d] + c->velocity[d] * t;
 }
+
+/* R (row-major) from the quaternion <w,x,y,z>, Eigen's toRotationMatrix
+   (Rotation.h:53-57): R = I + 2w[q]x + 2[q]x^2 written entrywise; all zero:
+   the identity */
+static void collider_rotation(const zro_collider* c, double R[9])
+{
+  double w = c->rotation[0], x = c->rotation[1], y = c->rotation[2], z = c->rotation[3];
+  if (w == 0.0 && x == 0.0 && y == 0.0 && z == 0.0) {
+    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    return;
+  }
+  double tx = 2 * x, ty = 2 * y, tz = 2 * z;
+  R[0] = 1 - (ty * y + tz * z); R[1] = ty * x - tz * w;       R[2] = tz * x + ty * w;
+  R[3] = ty * x + tz * w;       R[4] = 1 - (tx * x + tz * z); R[5] = tz * y - tx * w;
+  R[6] = tz * x - ty * w;       R[7] = tz * y + tx * w;       R[8] = 1 - (tx * x + ty * y);
+}
+static double collider_scale(const zro_collider* c) { return c->scale == 0.0 ? 1.0 : c->scale; }
 
 /* level-set queryInside in material space (AnalyticLevelSet.cpp:111-119, 435-450) */
 static int ls_query_inside(const zro_collider* c, const double* X, double* phi, double* n)
@@ -614,14 +681,38 @@ static int ls_query_inside(const zro_collider* c, const double* X, double* phi, 
   }
 }
 
-/* AnalyticCollisionObject::detectAndResolveCollision (CollisionObject.cpp:340-405),
-   translation-only transforms (R = I, s = 1, omega = 0) */
+/* AnalyticCollisionObject::queryInside in world space (CollisionObject.cpp:
+   279-296): xb = x - b, X = R^T xb / s; inside: n = R N, phi scaled by s */
+static int collider_query(const zro_collider* c, const double* b, const double* x, double* phi, double* n,
+                          double* xb)
+{
+  double R[9], X[3], N[3];
+  collider_rotation(c, R);
+  double inv_s = 1.0 / collider_scale(c);
+  for (int d = 0; d < 3; ++d) xb[d] = x[d] - b[d];
+  for (int d = 0; d < 3; ++d) X[d] = (R[d] * xb[0] + R[3 + d] * xb[1] + R[6 + d] * xb[2]) * inv_s;
+  if (!ls_query_inside(c, X, phi, N)) return 0;
+  for (int d = 0; d < 3; ++d) n[d] = R[3 * d] * N[0] + R[3 * d + 1] * N[1] + R[3 * d + 2] * N[2];
+  *phi *= collider_scale(c);
+  return 1;
+}
+
+/* the object's velocity at x: omega x (x - b) + (ds/dt / s)(x - b) + db/dt */
+static void collider_velocity(const zro_collider* c, const double* xb, double* vo)
+{
+  double g = c->dscale_dt * (1.0 / collider_scale(c));
+  const double* w = c->omega;
+  vo[0] = (w[1] * xb[2] - w[2] * xb[1]) + g * xb[0] + c->velocity[0];
+  vo[1] = (w[2] * xb[0] - w[0] * xb[2]) + g * xb[1] + c->velocity[1];
+  vo[2] = (w[0] * xb[1] - w[1] * xb[0]) + g * xb[2] + c->velocity[2];
+}
+
+/* AnalyticCollisionObject::detectAndResolveCollision (CollisionObject.cpp:340-405) */
 static int detect_and_resolve(const zro_collider* c, const double* b, const double* x, double* v)
 {
-  double X[3], phi, N[3];
-  for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
-  if (!ls_query_inside(c, X, &phi, N)) return 0;
-  const double* vo = c->velocity;
+  double xb[3], phi, N[3], vo[3];
+  if (!collider_query(c, b, x, &phi, N, xb)) return 0;
+  collider_velocity(c, xb, vo);
   for (int d = 0; d < 3; ++d) v[d] -= vo[d];
   if (c->type == 1) {
     v[0] = v[1] = v[2] = 0.0;
@@ -656,9 +747,8 @@ static int multi_object_collision(const zro_sim* s, double t, const double* x, d
     any = hit || any;
     if (hit) {
       if (s->col[k].type == 1) break;
-      double X[3], phi;
-      for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
-      ls_query_inside(&s->col[k], X, &phi, n);
+      double xb[3], phi;
+      collider_query(&s->col[k], b, x, &phi, n, xb);
       for (int c2 = 0; c2 < nslip; ++c2) {
         double dot = basis[c2][0] * n[0] + basis[c2][1] * n[1] + basis[c2][2] * n[2];
         for (int d = 0; d < 3; ++d) n[d] -= dot * basis[c2][d];
@@ -677,10 +767,10 @@ static int multi_object_collision(const zro_sim* s, double t, const double* x, d
 static void trial_collision(const zro_sim* s, double t, const double* x, double dt, double* v)
 {
   for (int k = 0; k < s->ncol; ++k) {
-    double b[3], X[3], phi, n[3];
+    double b[3], xt[3], xb[3], phi, n[3];
     collider_b(s, &s->col[k], t, b);
-    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - b[d];
-    if (ls_query_inside(&s->col[k], X, &phi, n))
+    for (int d = 0; d < 3; ++d) xt[d] = x[d] + dt * v[d];
+    if (collider_query(&s->col[k], b, xt, &phi, n, xb))
       for (int d = 0; d < 3; ++d) v[d] -= (phi / dt) * n[d];
   }
 }
@@ -906,9 +996,8 @@ static int collision_projection(const zro_sim* s, double t, const double* x, dou
     any = hit || any;
     if (!hit) continue;
     if (s->col[k].type == 1) { sticky = 1; break; }
-    double X[3], phi, n[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
-    ls_query_inside(&s->col[k], X, &phi, n);
+    double xb[3], phi, n[3];
+    collider_query(&s->col[k], b, x, &phi, n, xb);
     for (int c2 = 0; c2 < nslip; ++c2) {
       double dot = K[c2][0] * n[0] + K[c2][1] * n[1] + K[c2][2] * n[2];
       for (int d = 0; d < 3; ++d) n[d] -= dot * K[c2][d];
@@ -2132,14 +2221,14 @@ static int admm_collision(const zro_sim* s, double t, const double* x, double* v
   double K[3][3];
   for (int k = 0; k < s->ncol; ++k) {
     const zro_collider* c = &s->col[k];
-    double b[3], X[3], phi, n[3], old[3] = { v[0], v[1], v[2] };
+    double b[3], xb[3], phi, n[3], vo[3], old[3] = { v[0], v[1], v[2] };
     collider_b(s, c, t, b);
-    for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
-    if (!ls_query_inside(c, X, &phi, n)) continue;
+    if (!collider_query(c, b, x, &phi, n, xb)) continue;
+    collider_velocity(c, xb, vo);
     int col = detect_and_resolve(c, b, x, v);
     if (col && c->type == 3) {
       double dot = 0.0;
-      for (int d = 0; d < 3; ++d) dot += (old[d] - c->velocity[d]) * n[d];
+      for (int d = 0; d < 3; ++d) dot += (old[d] - vo[d]) * n[d];
       if (dot > 0) {
         for (int d = 0; d < 3; ++d) v[d] = old[d];
         col = 0;

@@ -41,9 +41,9 @@ __device__ bool collide_jacobian(const ColliderSet& cs, double t, double dt, con
     for (int j = 0; j < 3; ++j) J[i][j] = i == j ? 1.0 : 0.0;
   bool any = false;
   for (int k = 0; k < cs.n; ++k) {
-    const zr_collider& c = cs.c[k];
+    const DevCollider& c = cs.c[k];
     double X[3], phi, N[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] - c.velocity[d] * t;
+    for (int d = 0; d < 3; ++d) X[d] = x[d] - (c.b0[d] + c.velocity[d] * t);
     if (!ls_query_inside(c, X, phi, N)) continue;
     any = true;
     if (c.type == ZR_STICKY) {
@@ -68,9 +68,9 @@ __device__ bool collide_jacobian(const ColliderSet& cs, double t, double dt, con
   }
   if (any) return true;
   for (int k = 0; k < cs.n; ++k) {
-    const zr_collider& c = cs.c[k];
+    const DevCollider& c = cs.c[k];
     double X[3], phi, n[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - c.velocity[d] * (t + dt);
+    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - (c.b0[d] + c.velocity[d] * (t + dt));
     if (!ls_query_inside(c, X, phi, n)) continue;
     double A[3][3], T[3][3];
     if (c.shape == ZR_HALF_SPACE) {

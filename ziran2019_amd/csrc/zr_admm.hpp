@@ -275,15 +275,14 @@ __device__ bool admm_collision(const ColliderSet& cs, double t, const double x[3
   int nslip = 0;
   double K[3][3];
   for (int k = 0; k < cs.n; ++k) {
-    const zr_collider& c = cs.c[k];
-    double X[3], phi, n[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] - c.velocity[d] * t;
-    if (!ls_query_inside(c, X, phi, n)) continue;
+    const DevCollider& c = cs.c[k];
+    double xb[3], phi, n[3], vo[3];
+    if (!collider_query(c, t, x, phi, n, xb)) continue;
+    collider_velocity(c, xb, vo);
     const double old[3] = { v[0], v[1], v[2] };
     bool col = detect_and_resolve(c, t, x, v);
     if (col && c.type == ZR_SEPARATE) {
-      const double dot = (old[0] - c.velocity[0]) * n[0] + (old[1] - c.velocity[1]) * n[1] +
-                         (old[2] - c.velocity[2]) * n[2];
+      const double dot = (old[0] - vo[0]) * n[0] + (old[1] - vo[1]) * n[1] + (old[2] - vo[2]) * n[2];
       if (dot > 0.0) {
         v[0] = old[0];
         v[1] = old[1];

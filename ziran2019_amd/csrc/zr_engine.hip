@@ -25,6 +25,7 @@
 #include <cmath>
 #include <string>
 #include <vector>
+#include <array>
 
 #include "../../include/zr_mpm.h"
 #include "zr_math.hpp"
@@ -51,19 +52,26 @@ constexpr int kMaxColliders = 8;
 constexpr int kSortLds = 2048;  // particles per page sorted in LDS
 constexpr uint32_t kNone = 0xffffffffu;
 
+// a collider as the kernels read it: the frame's rotation as a matrix
+// (row-major), 1/s, b(0)
+struct DevCollider {
+  int32_t shape, type;
+  double friction, origin[3], normal[3], radius, velocity[3], b0[3], R[9], omega[3], s, inv_s, dsdt;
+};
 struct ColliderSet {
   int32_t n;
-  zr_collider c[kMaxColliders];
+  DevCollider c[kMaxColliders];
 };
 
 enum { CNT_GROUPS = 0, CNT_SLOTS = 1, CNT_DOMAIN = 2, CNT_CFL = 3, CNT_TOTAL = 4, CNT_ADJ_BAD = 5, CNT_N = 8 };
 
 // ---------------------------------------------------------------------------
-// collisions: AnalyticCollisionObject with translation-only motion
-// (CollisionObject.cpp:48-105,232-246,340-405; AnalyticLevelSet.cpp:111-119,
-// 259-287,435-450)
+// collisions: AnalyticCollisionObject moving as x = R s X + b(t)
+// (CollisionObject.cpp:48-105,232-246,279-296,340-405; AnalyticLevelSet.cpp:
+// 111-119,259-287,435-450).  With R = I, s = 1 and omega = ds/dt = 0 every
+// expression below reduces exactly to the translation-only one.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool ls_query_inside(const zr_collider& c, const double X[3], double& phi, double n[3])
+__device__ __forceinline__ bool ls_query_inside(const DevCollider& c, const double X[3], double& phi, double n[3])
 {
   if (c.shape == ZR_HALF_SPACE) {
     double sd = c.normal[0] * (X[0] - c.origin[0]) + c.normal[1] * (X[1] - c.origin[1]) +
@@ -96,12 +104,35 @@ __device__ __forceinline__ bool ls_query_inside(const zr_collider& c, const doub
   return false;
 }
 
-__device__ __forceinline__ bool detect_and_resolve(const zr_collider& c, double t, const double x[3], double v[3])
+// queryInside in world space (CollisionObject.cpp:279-296): xb = x - b(t),
+// X = R^T xb / s; inside: n = R N and phi scaled by s
+__device__ __forceinline__ bool collider_query(const DevCollider& c, double t, const double x[3], double& phi,
+                                               double n[3], double xb[3])
 {
-  double X[3], phi, N[3];
-  for (int d = 0; d < 3; ++d) X[d] = x[d] - c.velocity[d] * t;
+  double X[3], N[3];
+  for (int d = 0; d < 3; ++d) xb[d] = x[d] - (c.b0[d] + c.velocity[d] * t);
+  for (int d = 0; d < 3; ++d) X[d] = (c.R[d] * xb[0] + c.R[3 + d] * xb[1] + c.R[6 + d] * xb[2]) * c.inv_s;
   if (!ls_query_inside(c, X, phi, N)) return false;
-  for (int d = 0; d < 3; ++d) v[d] -= c.velocity[d];
+  for (int d = 0; d < 3; ++d) n[d] = c.R[3 * d] * N[0] + c.R[3 * d + 1] * N[1] + c.R[3 * d + 2] * N[2];
+  phi *= c.s;
+  return true;
+}
+
+// the object's velocity at x: omega x (x - b) + (ds/dt / s)(x - b) + db/dt
+__device__ __forceinline__ void collider_velocity(const DevCollider& c, const double xb[3], double vo[3])
+{
+  const double g = c.dsdt * c.inv_s;
+  vo[0] = (c.omega[1] * xb[2] - c.omega[2] * xb[1]) + g * xb[0] + c.velocity[0];
+  vo[1] = (c.omega[2] * xb[0] - c.omega[0] * xb[2]) + g * xb[1] + c.velocity[1];
+  vo[2] = (c.omega[0] * xb[1] - c.omega[1] * xb[0]) + g * xb[2] + c.velocity[2];
+}
+
+__device__ __forceinline__ bool detect_and_resolve(const DevCollider& c, double t, const double x[3], double v[3])
+{
+  double xb[3], phi, N[3], vo[3];
+  if (!collider_query(c, t, x, phi, N, xb)) return false;
+  collider_velocity(c, xb, vo);
+  for (int d = 0; d < 3; ++d) v[d] -= vo[d];
   if (c.type == ZR_STICKY) {
     v[0] = v[1] = v[2] = 0.0;
   } else {
@@ -118,7 +149,7 @@ __device__ __forceinline__ bool detect_and_resolve(const zr_collider& c, double 
       }
     }
   }
-  for (int d = 0; d < 3; ++d) v[d] += c.velocity[d];
+  for (int d = 0; d < 3; ++d) v[d] += vo[d];
   return true;
 }
 
@@ -136,9 +167,9 @@ __device__ __forceinline__ bool multi_object_collision(const ColliderSet& cs, do
 __device__ __forceinline__ void trial_collision(const ColliderSet& cs, double t, const double x[3], double dt, double v[3])
 {
   for (int k = 0; k < cs.n; ++k) {
-    double X[3], phi, n[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - cs.c[k].velocity[d] * t;
-    if (ls_query_inside(cs.c[k], X, phi, n))
+    double xt[3], xb[3], phi, n[3];
+    for (int d = 0; d < 3; ++d) xt[d] = x[d] + dt * v[d];
+    if (collider_query(cs.c[k], t, xt, phi, n, xb))
       for (int d = 0; d < 3; ++d) v[d] -= (phi / dt) * n[d];
   }
 }
@@ -1120,12 +1151,65 @@ void free_particles(zr_engine* e)
   e->n = 0;
 }
 
+// R from the quaternion <w,x,y,z> as Eigen::Quaternion::toRotationMatrix
+// forms it (Rotation.h:53-57), row-major; all zero: the identity
+void collider_rotation(const zr_collider& k, double R[9])
+{
+  const double w = k.rotation[0], x = k.rotation[1], y = k.rotation[2], z = k.rotation[3];
+  if (w == 0.0 && x == 0.0 && y == 0.0 && z == 0.0) {
+    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    return;
+  }
+  const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
+  const double twx = tx * w, twy = ty * w, twz = tz * w, txx = tx * x, txy = ty * x, txz = tz * x, tyy = ty * y,
+               tyz = tz * y, tzz = tz * z;
+  R[0] = 1 - (tyy + tzz);
+  R[1] = txy - twz;
+  R[2] = txz + twy;
+  R[3] = txy + twz;
+  R[4] = 1 - (txx + tzz);
+  R[5] = tyz - twx;
+  R[6] = txz - twy;
+  R[7] = tyz + twx;
+  R[8] = 1 - (txx + tyy);
+}
+double collider_scale(const zr_collider& k) { return k.scale == 0.0 ? 1.0 : k.scale; }
+
+// a collider whose frame neither rotates nor scales (a translation only)
+bool collider_translates_only(const zr_collider& k)
+{
+  double R[9];
+  collider_rotation(k, R);
+  for (int i = 0; i < 9; ++i)
+    if (R[i] != ((i % 4 == 0) ? 1.0 : 0.0)) return false;
+  return collider_scale(k) == 1.0 && k.dscale_dt == 0.0 && k.omega[0] == 0.0 && k.omega[1] == 0.0 &&
+         k.omega[2] == 0.0;
+}
+
 ColliderSet collider_set(const zr_engine* e)
 {
   ColliderSet cs;
   memset(&cs, 0, sizeof(cs));
   cs.n = (int32_t)e->colliders.size();
-  for (int k = 0; k < cs.n; ++k) cs.c[k] = e->colliders[k];
+  for (int k = 0; k < cs.n; ++k) {
+    const zr_collider& a = e->colliders[k];
+    DevCollider& c = cs.c[k];
+    c.shape = a.shape;
+    c.type = a.type;
+    c.friction = a.friction;
+    c.radius = a.radius;
+    for (int d = 0; d < 3; ++d) {
+      c.origin[d] = a.origin[d];
+      c.normal[d] = a.normal[d];
+      c.velocity[d] = a.velocity[d];
+      c.b0[d] = a.translation[d];
+      c.omega[d] = a.omega[d];
+    }
+    collider_rotation(a, c.R);
+    c.s = collider_scale(a);
+    c.inv_s = 1.0 / c.s;
+    c.dsdt = a.dscale_dt;
+  }
   return cs;
 }
 
@@ -1417,6 +1501,11 @@ bool adjoint_supported(zr_engine* e)
          "the adjoint supports fixed-corotated particles without plasticity, damage or a viscoelastic branch only");
     return false;
   }
+  for (const zr_collider& k : e->colliders)
+    if (!collider_translates_only(k)) {
+      fail(e, ZR_ERR_ARG, "the adjoint supports translating colliders only (no rotation, spin or scaling)");
+      return false;
+    }
   return true;
 }
 
@@ -2005,6 +2094,7 @@ int zr_set_colliders(zr_engine* e, const zr_collider* c, int32_t n)
       return fail(e, ZR_ERR_ARG, "unknown collider shape");
     }
     if (k.type < ZR_STICKY || k.type > ZR_SEPARATE) return fail(e, ZR_ERR_ARG, "unknown collider type");
+    if (!(collider_scale(k) > 0.0)) return fail(e, ZR_ERR_ARG, "collider scale must be positive");
   }
   return ZR_OK;
 }
@@ -2043,10 +2133,57 @@ int zr_calculate_dt(zr_engine* e, double max_dt, double* dt)
   const int rc = zr_eval_max_particle_speed(e, r);
   if (rc) return rc;
   double speed = r[0];
+  // AnalyticCollisionObject::evalMaxSpeed (CollisionObject.cpp:158-194) over
+  // the particle box widened by (degree + 2) dx (MpmSimulationBase.cpp:774-781)
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    hi[d] = r[1 + d] + 4.0 * e->cfg.dx;
+    lo[d] = r[4 + d] - 4.0 * e->cfg.dx;
+  }
   for (const zr_collider& c : e->colliders) {
 #pragma clang fp contract(off)
     const double* u = c.velocity;
-    const double cs = std::sqrt((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+    double cs = 0.0;
+    if (c.dscale_dt != 0.0 || c.omega[0] != 0.0 || c.omega[1] != 0.0 || c.omega[2] != 0.0) {
+      if (c.shape != ZR_SPHERE) return fail(e, ZR_ERR_ARG, "a spinning or scaling half-space has no bounds");
+      double R[9], b[3], mn[3], mx[3];
+      collider_rotation(c, R);
+      const double s = collider_scale(c), inv_s = 1.0 / s;
+      for (int d = 0; d < 3; ++d) {
+        b[d] = c.translation[d] + u[d] * e->time;
+        mn[d] = c.origin[d] - c.radius;
+        mx[d] = c.origin[d] + c.radius;
+      }
+      std::vector<std::array<double, 3>> corners;
+      for (int i = 0; i < 8; ++i) {  // particle-box corners inside the object's box (either test per axis)
+        double x[3], X[3];
+        for (int d = 0; d < 3; ++d) x[d] = (i & (1 << d)) ? lo[d] : hi[d];
+        for (int d = 0; d < 3; ++d)
+          X[d] = (R[d] * (x[0] - b[0]) + R[3 + d] * (x[1] - b[1]) + R[6 + d] * (x[2] - b[2])) * inv_s;
+        const bool a1 = (mn[0] < X[0]) || (mn[1] < X[1]) || (mn[2] < X[2]);
+        const bool a2 = (X[0] < mx[0]) || (X[1] < mx[1]) || (X[2] < mx[2]);
+        if (a1 && a2) corners.push_back({ x[0], x[1], x[2] });
+      }
+      for (int i = 0; i < 8; ++i) {  // object-box corners inside the particle box
+        double X[3], x[3];
+        for (int d = 0; d < 3; ++d) X[d] = (i & (1 << d)) ? mn[d] : mx[d];
+        for (int d = 0; d < 3; ++d) x[d] = s * (R[3 * d] * X[0] + R[3 * d + 1] * X[1] + R[3 * d + 2] * X[2]) + b[d];
+        const bool a1 = (lo[0] < x[0]) || (lo[1] < x[1]) || (lo[2] < x[2]);
+        const bool a2 = (x[0] < hi[0]) || (x[1] < hi[1]) || (x[2] < hi[2]);
+        if (a1 && a2) corners.push_back({ x[0], x[1], x[2] });
+      }
+      const double g = c.dscale_dt * inv_s;
+      for (const auto& x : corners) {
+        const double xb[3] = { x[0] - b[0], x[1] - b[1], x[2] - b[2] };
+        const double v[3] = { (c.omega[1] * xb[2] - c.omega[2] * xb[1]) + g * xb[0] + u[0],
+                              (c.omega[2] * xb[0] - c.omega[0] * xb[2]) + g * xb[1] + u[1],
+                              (c.omega[0] * xb[1] - c.omega[1] * xb[0]) + g * xb[2] + u[2] };
+        const double sp = std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        if (sp > cs) cs = sp;
+      }
+    } else {
+      cs = std::sqrt((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+    }
     if (cs > speed) speed = cs;
   }
   *dt = speed != 0.0 ? e->cfg.cfl * e->cfg.dx / speed : max_dt;

@@ -115,9 +115,8 @@ __device__ __forceinline__ void project_node(const ColliderSet& cs, double t, co
   double K[3][3];
   int nslip = 0;
   for (int k = 0; k < cs.n; ++k) {
-    double X[3], phi, n[3];
-    for (int d = 0; d < 3; ++d) X[d] = xi[d] - cs.c[k].velocity[d] * t;
-    if (!ls_query_inside(cs.c[k], X, phi, n)) continue;
+    double xb[3], phi, n[3];
+    if (!collider_query(cs.c[k], t, xi, phi, n, xb)) continue;
     if (cs.c[k].type == ZR_STICKY) {
       r[0] = r[1] = r[2] = 0.0;
       return;

@@ -115,7 +115,9 @@ class ZrConfig(ctypes.Structure):
 class ZrCollider(ctypes.Structure):
     _fields_ = [("shape", ctypes.c_int32), ("type", ctypes.c_int32), ("friction", ctypes.c_double),
                 ("origin", ctypes.c_double * 3), ("normal", ctypes.c_double * 3), ("radius", ctypes.c_double),
-                ("velocity", ctypes.c_double * 3)]
+                ("velocity", ctypes.c_double * 3), ("translation", ctypes.c_double * 3),
+                ("rotation", ctypes.c_double * 4), ("omega", ctypes.c_double * 3), ("scale", ctypes.c_double),
+                ("dscale_dt", ctypes.c_double)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -360,6 +362,11 @@ class MpmEngine:
             arr[k].normal = (ctypes.c_double * 3)(*c.get("normal", (0, 1, 0)))
             arr[k].radius = float(c.get("radius", 0.0))
             arr[k].velocity = (ctypes.c_double * 3)(*c.get("velocity", (0, 0, 0)))
+            arr[k].translation = (ctypes.c_double * 3)(*c.get("translation", (0, 0, 0)))
+            arr[k].rotation = (ctypes.c_double * 4)(*c.get("rotation", (0, 0, 0, 0)))
+            arr[k].omega = (ctypes.c_double * 3)(*c.get("omega", (0, 0, 0)))
+            arr[k].scale = float(c.get("scale", 0.0))
+            arr[k].dscale_dt = float(c.get("dscale_dt", 0.0))
         self._check(self.L.zr_set_colliders(self.h, arr, len(colliders)), "zr_set_colliders")
 
     def set_time(self, t):

@@ -85,7 +85,12 @@ void MpmSimulationGpu::uploadColliders()
       z[i].origin[d] = c.origin[d];
       z[i].normal[d] = c.normal[d];
       z[i].velocity[d] = c.velocity[d];
+      z[i].translation[d] = c.translation[d];
+      z[i].omega[d] = c.omega[d];
     }
+    for (int d = 0; d < 4; ++d) z[i].rotation[d] = c.rotation[d];
+    z[i].scale = c.scale;
+    z[i].dscale_dt = c.dscale_dt;
   }
   check(zr_set_colliders(eng_, z.data(), (int32_t)z.size()));
 }

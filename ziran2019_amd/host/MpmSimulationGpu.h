@@ -43,7 +43,11 @@ struct CollisionObject {
   std::array<double, 3> origin{ { 0.0, 0.0, 0.0 } };
   std::array<double, 3> normal{ { 0.0, 1.0, 0.0 } };
   double radius = 0.0;
-  std::array<double, 3> velocity{ { 0.0, 0.0, 0.0 } };
+  std::array<double, 3> velocity{ { 0.0, 0.0, 0.0 } };     // setTranslation's dbdt
+  std::array<double, 3> translation{ { 0.0, 0.0, 0.0 } };  // setTranslation's b (at time 0)
+  std::array<double, 4> rotation{ { 1.0, 0.0, 0.0, 0.0 } };  // setRotation <w,x,y,z>
+  std::array<double, 3> omega{ { 0.0, 0.0, 0.0 } };        // setAngularVelocity
+  double scale = 1.0, dscale_dt = 0.0;                      // setScaling
 };
 
 // SimulationBase::step (TimeStepping.h:8-103): substep sizes between min_dt
