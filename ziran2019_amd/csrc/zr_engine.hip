@@ -1155,6 +1155,7 @@ void free_particles(zr_engine* e)
 // forms it (Rotation.h:53-57), row-major; all zero: the identity
 void collider_rotation(const zr_collider& k, double R[9])
 {
+#pragma clang fp contract(off)
   const double w = k.rotation[0], x = k.rotation[1], y = k.rotation[2], z = k.rotation[3];
   if (w == 0.0 && x == 0.0 && y == 0.0 && z == 0.0) {
     for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
