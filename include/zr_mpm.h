@@ -75,17 +75,23 @@ typedef struct zr_config {
   int32_t reserved;
 } zr_config;
 
-/* An analytic collision object with a constant-velocity translation:
-   AnalyticCollisionObject + HalfSpace / Sphere level sets
-   (CollisionObject.cpp:48-105,340-405; AnalyticLevelSet.cpp:259-287,435-450). */
-/* An analytic collision object moving as x = R s X + b(t) with
-   b(t) = translation + velocity t (AnalyticCollisionObject,
-   CollisionObject.h:81-172): the level set is queried at
-   X = R^T (x - b) / s, its normal is R N, and the object's velocity at x is
-   omega x (x - b) + (ds/dt / s)(x - b) + velocity (CollisionObject.cpp:340-405).
-   R, omega, s and ds/dt are the state the reference's updateState callback
-   sets; a caller that changes them over time sets the colliders again.
-   Zero-initialised extra fields give the static frame (R = I, s = 1). */
+/* An analytic collision object (AnalyticCollisionObject + HalfSpace / Sphere
+   level sets, CollisionObject.cpp:48-105,340-405; AnalyticLevelSet.cpp:
+   259-287,435-450) moving as x = R(t) s(t) X + b(t) (CollisionObject.h:81-172)
+   at constant rates from its state at time 0, which is the state the
+   reference's updateState(t) callbacks set for such motion
+   (e.g. AdmmInit3D.h:298-313):
+     b(t) = translation + velocity t,
+     R(t) = exp(t [omega]x) R0   (R0 from the quaternion `rotation`),
+     s(t) = scale + dscale_dt t.
+   The level set is queried at X = R^T (x - b) / s, its normal is R N, and the
+   object's velocity at x is omega x (x - b) + (ds/dt / s)(x - b) + velocity
+   (CollisionObject.cpp:340-405).  A step from t to t + dt uses the state at
+   t for the grid update and the collision projections and the state at
+   t + dt for the trial collision (updateState(t + dt) before trialCollision,
+   MpmSimulationBase.cpp:267-278).  Other motions are set again (with
+   zr_set_colliders) between steps.  Zero-initialised extra fields give the
+   static frame (R = I, s = 1). */
 typedef struct zr_collider {
   int32_t shape;        /* ZR_HALF_SPACE or ZR_SPHERE */
   int32_t type;         /* ZR_STICKY, ZR_SLIP or ZR_SEPARATE */

@@ -123,6 +123,11 @@ def lib():
         for f in ("zro_fcr_first_piola", "zro_fcr_kirchhoff"):
             getattr(L, f).argtypes = [_d, _d, _p, _p]
         L.zro_fcr_first_piola_differential.argtypes = [_d, _d, _p, _p, _p]
+        L.zro_collider_frame.argtypes = [ctypes.POINTER(ZroCollider), _d, _p, _p, _p]
+        L.zro_project_sigma.argtypes = [_d, _d, _i, _p, _d, _p, _p]
+        L.zro_project_sigma_and_derivative.argtypes = [_d, _d, _i, _p, _d, _p, _p, _p]
+        L.zro_fake_separate.restype = _i
+        L.zro_fake_separate.argtypes = [ctypes.POINTER(ZroCollider), _i, _d, _p, _p, _p]
         L.zro_lame.argtypes = [_d, _d, _p, _p]
         L.zro_create.restype = _p
         L.zro_create.argtypes = [_d, _p, _d, _i]
@@ -397,6 +402,56 @@ def next_dt(dt, min_dt, max_dt, max_dt_original, frame_dt, time_since_last_frame
 
 
 # ---------------------------------------------------------------- simulation
+def make_collider(shape, type, origin, normal=(0, 1, 0), radius=0.0, velocity=(0, 0, 0), friction=0.0,
+                  translation=(0, 0, 0), rotation=(0, 0, 0, 0), omega=(0, 0, 0), scale=0.0, dscale_dt=0.0):
+    """The oracle's collider record (zro_add_collider normalises a
+    half-space normal, as HalfSpace does, AnalyticLevelSet.cpp:262-266;
+    zro_fake_separate and zro_collider_frame take it as given)."""
+    return ZroCollider(int(shape), int(type), float(friction), (_d * 3)(*origin),
+                       (_d * 3)(*normal), float(radius), (_d * 3)(*velocity), (_d * 3)(*translation),
+                       (_d * 4)(*rotation), (_d * 3)(*omega), float(scale), float(dscale_dt))
+
+
+def project_sigma(mu, lam, plastic, pparam, pstate, sig):
+    """projectSigma of the ADMM local step (PlasticityApplier.cpp:319-346,
+    526-547)."""
+    q = np.ascontiguousarray(pparam, dtype=np.float64)
+    sv = np.ascontiguousarray(sig, dtype=np.float64)
+    out = np.zeros(3)
+    lib().zro_project_sigma(float(mu), float(lam), int(plastic), _ptr(q), float(pstate), _ptr(sv), _ptr(out))
+    return out
+
+
+def project_sigma_and_derivative(mu, lam, plastic, pparam, pstate, sig):
+    """projectSigmaAndDerivative (PlasticityApplier.cpp:389-423, 580-606):
+    (projected sigma, d/d sigma as a 3x3 matrix)."""
+    q = np.ascontiguousarray(pparam, dtype=np.float64)
+    sv = np.ascontiguousarray(sig, dtype=np.float64)
+    zp, D = np.zeros(3), np.zeros(9)
+    lib().zro_project_sigma_and_derivative(float(mu), float(lam), int(plastic), _ptr(q), float(pstate), _ptr(sv),
+                                           _ptr(zp), _ptr(D))
+    return zp, D.reshape(3, 3)
+
+
+def collider_frame(t, **kw):
+    """(b, R, s) of a collider at time t: the state updateState(t) sets for
+    constant rates (translation velocity, angular velocity, ds/dt)."""
+    c = make_collider(**kw)
+    b = np.zeros(3); R = np.zeros(9); s = np.zeros(1)
+    lib().zro_collider_frame(ctypes.byref(c), float(t), _ptr(b), _ptr(R), _ptr(s))
+    return b, R.reshape(3, 3), float(s[0])
+
+
+def fake_separate(colliders, x, v, t=0.0):
+    """multiObjectCollisionWithFakeSeparate on one node: (any, v, P)."""
+    arr = (ZroCollider * max(len(colliders), 1))(*[make_collider(**c) for c in colliders])
+    xv = np.ascontiguousarray(x, dtype=np.float64)
+    vv = np.array(v, dtype=np.float64)
+    P = np.zeros(9)
+    hit = lib().zro_fake_separate(arr, len(colliders), float(t), _ptr(xv), _ptr(vv), _ptr(P))
+    return bool(hit), vv, P.reshape(3, 3)
+
+
 class OracleSim:
     """FP64 CPU MPM simulation (explicit MLS step), restating
     MpmSimulationBase<double,3> (Lib/MPM/MpmSimulationBase.cpp)."""
@@ -497,12 +552,8 @@ class OracleSim:
         lib().zro_get_materials(self.h, _ptr(st), _ptr(b), _ptr(g))
         return dict(pstate=st, broken=b, g=g)
 
-    def add_collider(self, shape, type, origin, normal=(0, 1, 0), radius=0.0,
-                     velocity=(0, 0, 0), friction=0.0, translation=(0, 0, 0), rotation=(0, 0, 0, 0),
-                     omega=(0, 0, 0), scale=0.0, dscale_dt=0.0):
-        c = ZroCollider(int(shape), int(type), float(friction), (_d * 3)(*origin),
-                        (_d * 3)(*normal), float(radius), (_d * 3)(*velocity), (_d * 3)(*translation),
-                        (_d * 4)(*rotation), (_d * 3)(*omega), float(scale), float(dscale_dt))
+    def add_collider(self, **kw):
+        c = make_collider(**kw)
         assert lib().zro_add_collider(self.h, ctypes.byref(c)) == 0
 
     def set_time(self, t):

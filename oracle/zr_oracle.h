@@ -118,6 +118,14 @@ void zro_get_viscoelastic(const zro_sim* s, double* Fneq);
 void zro_visco_relax(double mu, double lam, double visc_d, double visc_v, double dt, const double* gradV_cm,
                      double* Fneq_cm);
 int zro_add_collider(zro_sim* s, const zro_collider* c);
+/* updateState(t) of an object moving at constant rates from its time-0
+   state: b(t) = translation + velocity t, R(t) = exp(t [omega]x) R0 (row-major),
+   s(t) = s0 + ds/dt t */
+void zro_collider_frame(const zro_collider* c, double t, double* b, double* R, double* s);
+/* multiObjectCollisionWithFakeSeparate (CollisionObject.cpp:108-160) on one
+   node at x with velocity v (resolved in place); P = I - K K^T (9 doubles,
+   row-major); returns 1 when any object acts */
+int zro_fake_separate(const zro_collider* cols, int ncol, double t, const double* x, double* v, double* P);
 void zro_set_time(zro_sim* s, double t);
 
 /* sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1032-1103) */
@@ -239,6 +247,15 @@ typedef struct {
 } zro_admm_stats;
 
 int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats* st);
+/* The ADMM local step's plastic projection of singular values, projectSigma
+   (PlasticityApplier.cpp:319-346 Drucker-Prager, :526-547 von Mises; other
+   plasticity: identity), and projectSigmaAndDerivative (:389-423, :580-606):
+   pparam as zr_materials.plastic_param (4 doubles), pstate the von Mises
+   yield stress; D is d out / d sig, row-major. */
+void zro_project_sigma(double mu, double lam, int plastic, const double* pparam, double pstate, const double* sig,
+                       double* out);
+void zro_project_sigma_and_derivative(double mu, double lam, int plastic, const double* pparam, double pstate,
+                                      const double* sig, double* zp, double* D);
 /* per particle ADMM state after the last step: F (9n), y (9n), omega (n) */
 void zro_get_admm(const zro_sim* s, double* F, double* y, double* omega);
 /* all three eigenvalues of a symmetric 3x3 matrix, ascending */

@@ -552,10 +552,12 @@ void zro_eval_max_particle_speed(const zro_sim* s, double out[7])
   for (int d = 0; d < 3; ++d) out[4 + d] = -out[4 + d];
 }
 
-static void collider_b(const zro_sim* s, const zro_collider* c, double t, double* b);
-static void collider_rotation(const zro_collider* c, double R[9]);
-static double collider_scale(const zro_collider* c);
-static void collider_velocity(const zro_collider* c, const double* xb, double* vo);
+/* the object's frame at a time t: x = R(t) s(t) X + b(t) */
+typedef struct {
+  double b[3], R[9], s, inv_s;
+} zro_frame;
+static void collider_frame(const zro_collider* c, double t, zro_frame* f);
+static void collider_velocity(const zro_collider* c, const zro_frame* f, const double* xb, double* vo);
 
 /* calculateDt (MpmSimulationBase.cpp:766-790): the corners widened by
    (degree + 2) dx only bound the colliders' speed queries, which for a
@@ -578,11 +580,12 @@ double zro_calculate_dt(const zro_sim* s, double max_dt)
     double cs = 0.0;
     if (c->dscale_dt != 0.0 || c->omega[0] != 0.0 || c->omega[1] != 0.0 || c->omega[2] != 0.0) {
       if (c->shape != 1) return -1.0;
-      double R[9], b[3], mn[3], mx[3], pts[16][3];
+      double mn[3], mx[3], pts[16][3];
       int np = 0;
-      collider_rotation(c, R);
-      double sc = collider_scale(c), inv_s = 1.0 / sc;
-      collider_b(s, c, s->time, b);
+      zro_frame fr;
+      collider_frame(c, s->time, &fr);
+      const double *R = fr.R, *b = fr.b;
+      double sc = fr.s, inv_s = fr.inv_s;
       for (int d = 0; d < 3; ++d) { mn[d] = c->origin[d] - c->radius; mx[d] = c->origin[d] + c->radius; }
       for (int i = 0; i < 8; ++i) {
         double x[3], X[3];
@@ -604,7 +607,7 @@ double zro_calculate_dt(const zro_sim* s, double max_dt)
       for (int i = 0; i < np; ++i) {
         double xb[3], v[3];
         for (int d = 0; d < 3; ++d) xb[d] = pts[i][d] - b[d];
-        collider_velocity(c, xb, v);
+        collider_velocity(c, &fr, xb, v);
         double sp = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
         if (sp > cs) cs = sp;
       }
@@ -632,30 +635,54 @@ double zro_next_dt(double dt, double min_dt, double max_dt, double max_dt_origin
 }
 
 /* ---------------- colliders ---------------- */
-/* the object's frame x = R s X + b(t), b(t) = translation + velocity t
-   (CollisionObject.h:81-172) */
-static void collider_b(const zro_sim* s, const zro_collider* c, double t, double* b)
-{
-  (void)s;
-  for (int d = 0; d < 3; ++d) b[d] = c->translation[d] + c->velocity[d] * t;
-}
-
 /* R (row-major) from the quaternion <w,x,y,z>, Eigen's toRotationMatrix
-   (Rotation.h:53-57): R = I + 2w[q]x + 2[q]x^2 written entrywise; all zero:
-   the identity */
-static void collider_rotation(const zro_collider* c, double R[9])
+   (Rotation.h:53-57): R = I + 2w[q]x + 2[q]x^2 written entrywise */
+static void quat_to_matrix(const double q[4], double R[9])
 {
-  double w = c->rotation[0], x = c->rotation[1], y = c->rotation[2], z = c->rotation[3];
-  if (w == 0.0 && x == 0.0 && y == 0.0 && z == 0.0) {
-    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
-    return;
-  }
+  double w = q[0], x = q[1], y = q[2], z = q[3];
   double tx = 2 * x, ty = 2 * y, tz = 2 * z;
   R[0] = 1 - (ty * y + tz * z); R[1] = ty * x - tz * w;       R[2] = tz * x + ty * w;
   R[3] = ty * x + tz * w;       R[4] = 1 - (tx * x + tz * z); R[5] = tz * y - tx * w;
   R[6] = tz * x - ty * w;       R[7] = tz * y + tx * w;       R[8] = 1 - (tx * x + ty * y);
 }
-static double collider_scale(const zro_collider* c) { return c->scale == 0.0 ? 1.0 : c->scale; }
+
+/* The state updateState(t) sets for an object moving at constant rates from
+   its time-0 state (CollisionObject.h:81-172; the reference's scene callbacks
+   write the same form, e.g. AdmmInit3D.h:298-313: rotation
+   (cos(theta t/2), 0, sin(theta t/2), 0) with omega (0, theta, 0)):
+   b(t) = translation + velocity t, s(t) = s0 + ds/dt t, and
+   R(t) = exp(t [omega]x) R0, as the quaternion product
+   (cos(|omega| t/2), sin(|omega| t/2) omega/|omega|) q0.  A zero quaternion
+   means the identity, scale 0 means 1. */
+static void collider_frame(const zro_collider* c, double t, zro_frame* f)
+{
+  double q0[4] = { c->rotation[0], c->rotation[1], c->rotation[2], c->rotation[3] };
+  if (q0[0] == 0.0 && q0[1] == 0.0 && q0[2] == 0.0 && q0[3] == 0.0) q0[0] = 1.0;
+  const double* w = c->omega;
+  double wn = sqrt((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
+  double q[4] = { q0[0], q0[1], q0[2], q0[3] };
+  if (wn != 0.0 && t != 0.0) {
+    double h = 0.5 * wn * t, sh = sin(h) / wn;
+    double a[4] = { cos(h), sh * w[0], sh * w[1], sh * w[2] };
+    q[0] = a[0] * q0[0] - a[1] * q0[1] - a[2] * q0[2] - a[3] * q0[3];
+    q[1] = a[0] * q0[1] + a[1] * q0[0] + a[2] * q0[3] - a[3] * q0[2];
+    q[2] = a[0] * q0[2] - a[1] * q0[3] + a[2] * q0[0] + a[3] * q0[1];
+    q[3] = a[0] * q0[3] + a[1] * q0[2] - a[2] * q0[1] + a[3] * q0[0];
+  }
+  quat_to_matrix(q, f->R);
+  for (int d = 0; d < 3; ++d) f->b[d] = c->translation[d] + c->velocity[d] * t;
+  f->s = (c->scale == 0.0 ? 1.0 : c->scale) + c->dscale_dt * t;
+  f->inv_s = 1.0 / f->s;
+}
+
+void zro_collider_frame(const zro_collider* c, double t, double* b, double* R, double* s)
+{
+  zro_frame f;
+  collider_frame(c, t, &f);
+  memcpy(b, f.b, sizeof(f.b));
+  memcpy(R, f.R, sizeof(f.R));
+  *s = f.s;
+}
 
 /* level-set queryInside in material space (AnalyticLevelSet.cpp:111-119, 435-450) */
 static int ls_query_inside(const zro_collider* c, const double* X, double* phi, double* n)
@@ -683,24 +710,23 @@ static int ls_query_inside(const zro_collider* c, const double* X, double* phi, 
 
 /* AnalyticCollisionObject::queryInside in world space (CollisionObject.cpp:
    279-296): xb = x - b, X = R^T xb / s; inside: n = R N, phi scaled by s */
-static int collider_query(const zro_collider* c, const double* b, const double* x, double* phi, double* n,
+static int collider_query(const zro_collider* c, const zro_frame* f, const double* x, double* phi, double* n,
                           double* xb)
 {
-  double R[9], X[3], N[3];
-  collider_rotation(c, R);
-  double inv_s = 1.0 / collider_scale(c);
-  for (int d = 0; d < 3; ++d) xb[d] = x[d] - b[d];
-  for (int d = 0; d < 3; ++d) X[d] = (R[d] * xb[0] + R[3 + d] * xb[1] + R[6 + d] * xb[2]) * inv_s;
+  double X[3], N[3];
+  const double* R = f->R;
+  for (int d = 0; d < 3; ++d) xb[d] = x[d] - f->b[d];
+  for (int d = 0; d < 3; ++d) X[d] = (R[d] * xb[0] + R[3 + d] * xb[1] + R[6 + d] * xb[2]) * f->inv_s;
   if (!ls_query_inside(c, X, phi, N)) return 0;
   for (int d = 0; d < 3; ++d) n[d] = R[3 * d] * N[0] + R[3 * d + 1] * N[1] + R[3 * d + 2] * N[2];
-  *phi *= collider_scale(c);
+  *phi *= f->s;
   return 1;
 }
 
 /* the object's velocity at x: omega x (x - b) + (ds/dt / s)(x - b) + db/dt */
-static void collider_velocity(const zro_collider* c, const double* xb, double* vo)
+static void collider_velocity(const zro_collider* c, const zro_frame* f, const double* xb, double* vo)
 {
-  double g = c->dscale_dt * (1.0 / collider_scale(c));
+  double g = c->dscale_dt * f->inv_s;
   const double* w = c->omega;
   vo[0] = (w[1] * xb[2] - w[2] * xb[1]) + g * xb[0] + c->velocity[0];
   vo[1] = (w[2] * xb[0] - w[0] * xb[2]) + g * xb[1] + c->velocity[1];
@@ -708,11 +734,11 @@ static void collider_velocity(const zro_collider* c, const double* xb, double* v
 }
 
 /* AnalyticCollisionObject::detectAndResolveCollision (CollisionObject.cpp:340-405) */
-static int detect_and_resolve(const zro_collider* c, const double* b, const double* x, double* v)
+static int detect_and_resolve(const zro_collider* c, const zro_frame* f, const double* x, double* v)
 {
   double xb[3], phi, N[3], vo[3];
-  if (!collider_query(c, b, x, &phi, N, xb)) return 0;
-  collider_velocity(c, xb, vo);
+  if (!collider_query(c, f, x, &phi, N, xb)) return 0;
+  collider_velocity(c, f, xb, vo);
   for (int d = 0; d < 3; ++d) v[d] -= vo[d];
   if (c->type == 1) {
     v[0] = v[1] = v[2] = 0.0;
@@ -741,14 +767,15 @@ static int multi_object_collision(const zro_sim* s, double t, const double* x, d
   int any = 0, nslip = 0;
   double basis[3][3];
   for (int k = 0; k < s->ncol; ++k) {
-    double b[3], n[3] = { 0, 0, 0 };
-    collider_b(s, &s->col[k], t, b);
-    int hit = detect_and_resolve(&s->col[k], b, x, v);
+    double n[3] = { 0, 0, 0 };
+    zro_frame fr;
+    collider_frame(&s->col[k], t, &fr);
+    int hit = detect_and_resolve(&s->col[k], &fr, x, v);
     any = hit || any;
     if (hit) {
       if (s->col[k].type == 1) break;
       double xb[3], phi;
-      collider_query(&s->col[k], b, x, &phi, n, xb);
+      collider_query(&s->col[k], &fr, x, &phi, n, xb);
       for (int c2 = 0; c2 < nslip; ++c2) {
         double dot = basis[c2][0] * n[0] + basis[c2][1] * n[1] + basis[c2][2] * n[2];
         for (int d = 0; d < 3; ++d) n[d] -= dot * basis[c2][d];
@@ -767,10 +794,11 @@ static int multi_object_collision(const zro_sim* s, double t, const double* x, d
 static void trial_collision(const zro_sim* s, double t, const double* x, double dt, double* v)
 {
   for (int k = 0; k < s->ncol; ++k) {
-    double b[3], xt[3], xb[3], phi, n[3];
-    collider_b(s, &s->col[k], t, b);
+    double xt[3], xb[3], phi, n[3];
+    zro_frame fr;
+    collider_frame(&s->col[k], t, &fr);
     for (int d = 0; d < 3; ++d) xt[d] = x[d] + dt * v[d];
-    if (collider_query(&s->col[k], b, xt, &phi, n, xb))
+    if (collider_query(&s->col[k], &fr, xt, &phi, n, xb))
       for (int d = 0; d < 3; ++d) v[d] -= (phi / dt) * n[d];
   }
 }
@@ -990,14 +1018,14 @@ static int collision_projection(const zro_sim* s, double t, const double* x, dou
   int any = 0, nslip = 0, sticky = 0;
   double K[3][3] = { { 0 } };
   for (int k = 0; k < s->ncol; ++k) {
-    double b[3];
-    collider_b(s, &s->col[k], t, b);
-    int hit = detect_and_resolve(&s->col[k], b, x, v);
+    zro_frame fr;
+    collider_frame(&s->col[k], t, &fr);
+    int hit = detect_and_resolve(&s->col[k], &fr, x, v);
     any = hit || any;
     if (!hit) continue;
     if (s->col[k].type == 1) { sticky = 1; break; }
     double xb[3], phi, n[3];
-    collider_query(&s->col[k], b, x, &phi, n, xb);
+    collider_query(&s->col[k], &fr, x, &phi, n, xb);
     for (int c2 = 0; c2 < nslip; ++c2) {
       double dot = K[c2][0] * n[0] + K[c2][1] * n[1] + K[c2][2] * n[2];
       for (int d = 0; d < 3; ++d) n[d] -= dot * K[c2][d];
@@ -1769,9 +1797,10 @@ static int collide_jacobian(const zro_sim* s, double dt, const double* x, const 
   int any = 0;
   for (int k = 0; k < s->ncol; ++k) {
     const zro_collider* c = &s->col[k];
-    double b[3], X[3], phi, N[3];
-    collider_b(s, c, s->time, b);
-    for (int d = 0; d < 3; ++d) X[d] = x[d] - b[d];
+    double X[3], phi, N[3];
+    zro_frame fr;
+    collider_frame(c, s->time, &fr);
+    for (int d = 0; d < 3; ++d) X[d] = x[d] - fr.b[d];
     if (!ls_query_inside(c, X, &phi, N)) continue;
     any = 1;
     if (c->type == 1) {
@@ -1796,9 +1825,10 @@ static int collide_jacobian(const zro_sim* s, double dt, const double* x, const 
   if (any) return 1;
   for (int k = 0; k < s->ncol; ++k) { /* trial collision at t + dt */
     const zro_collider* c = &s->col[k];
-    double b[3], X[3], phi, n[3];
-    collider_b(s, c, s->time + dt, b);
-    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - b[d];
+    double X[3], phi, n[3];
+    zro_frame fr;
+    collider_frame(c, s->time + dt, &fr);
+    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - fr.b[d];
     if (!ls_query_inside(c, X, &phi, n)) continue;
     double A[3][3], T[3][3];
     if (c->shape == 0) {
@@ -2102,17 +2132,14 @@ static void hencky_Bij(double mu, double lam, const double* s, int i, int j, dou
 
 /* projectSigma of the particle's plasticity (PlasticityApplier.cpp:321-346, 528-547;
    DummyPlasticity returns sigma) */
-static void admm_project_sigma(const zro_sim* s, int64_t i, const double* sig, double* out)
+static void project_sigma(double mu, double lam, int pl, const double* q, double ty, const double* sig, double* out)
 {
-  int pl = s->plastic ? s->plastic[i] : ZRO_PLASTIC_NONE;
-  double mu = s->mu[i], lam = s->lam[i];
-  const double* q = s->pparam ? s->pparam + 4 * i : NULL;
   for (int d = 0; d < 3; ++d) out[d] = sig[d];
   if (pl == ZRO_PLASTIC_VON_MISES) {
     double e[3], tr = 0.0, n2 = 0.0, h[3];
     for (int d = 0; d < 3; ++d) { e[d] = log(sig[d] > 1e-4 ? sig[d] : 1e-4); tr += e[d]; }
     for (int d = 0; d < 3; ++d) { h[d] = e[d] - tr / 3.0; n2 += h[d] * h[d]; }
-    double nrm = sqrt(n2), dg = nrm - s->pstate[i] / (2.0 * mu);
+    double nrm = sqrt(n2), dg = nrm - ty / (2.0 * mu);
     if (dg <= 0.0) return;
     for (int d = 0; d < 3; ++d) out[d] = exp(e[d] - (dg / nrm) * h[d]);
   } else if (pl == ZRO_PLASTIC_DRUCKER_PRAGER) {
@@ -2131,18 +2158,21 @@ static void admm_project_sigma(const zro_sim* s, int64_t i, const double* sig, d
   }
 }
 
-/* projectSigmaAndDerivative (PlasticityApplier.cpp:389-423, 580-606; Dummy: identity) */
-static void admm_project_sigma_d(const zro_sim* s, int64_t i, const double* sig, double* zp, double D[3][3])
+static void admm_project_sigma(const zro_sim* s, int64_t i, const double* sig, double* out)
 {
-  int pl = s->plastic ? s->plastic[i] : ZRO_PLASTIC_NONE;
-  double mu = s->mu[i], lam = s->lam[i];
-  const double* q = s->pparam ? s->pparam + 4 * i : NULL;
+  project_sigma(s->mu[i], s->lam[i], s->plastic ? s->plastic[i] : ZRO_PLASTIC_NONE,
+                s->pparam ? s->pparam + 4 * i : NULL, s->pstate ? s->pstate[i] : 0.0, sig, out);
+}
+
+/* projectSigmaAndDerivative (PlasticityApplier.cpp:389-423, 580-606; Dummy: identity) */
+static void project_sigma_d(double mu, double lam, int pl, const double* q, double ty, const double* sig, double* zp,
+                            double D[3][3])
+{
   for (int a = 0; a < 3; ++a) {
     zp[a] = sig[a];
     for (int b = 0; b < 3; ++b) D[a][b] = a == b ? 1.0 : 0.0;
   }
   if (pl == ZRO_PLASTIC_VON_MISES) {
-    double ty = s->pstate[i];
     double e[3], tr = 0.0, n2 = 0.0, h[3];
     for (int d = 0; d < 3; ++d) { e[d] = log(sig[d] > 1e-4 ? sig[d] : 1e-4); tr += e[d]; }
     for (int d = 0; d < 3; ++d) { h[d] = e[d] - tr / 3.0; n2 += h[d] * h[d]; }
@@ -2197,6 +2227,26 @@ static void admm_project_sigma_d(const zro_sim* s, int64_t i, const double* sig,
   }
 }
 
+static void admm_project_sigma_d(const zro_sim* s, int64_t i, const double* sig, double* zp, double D[3][3])
+{
+  project_sigma_d(s->mu[i], s->lam[i], s->plastic ? s->plastic[i] : ZRO_PLASTIC_NONE,
+                  s->pparam ? s->pparam + 4 * i : NULL, s->pstate ? s->pstate[i] : 0.0, sig, zp, D);
+}
+
+void zro_project_sigma(double mu, double lam, int plastic, const double* pparam, double pstate, const double* sig,
+                       double* out)
+{
+  project_sigma(mu, lam, plastic, pparam, pstate, sig, out);
+}
+
+void zro_project_sigma_and_derivative(double mu, double lam, int plastic, const double* pparam, double pstate,
+                                      const double* sig, double* zp, double* D)
+{
+  double M[3][3];
+  project_sigma_d(mu, lam, plastic, pparam, pstate, sig, zp, M);
+  memcpy(D, M, sizeof(M));
+}
+
 /* initOmega (AdmmSimulationFoam.h:222-274) for one particle with F = U S V^T */
 static double admm_omega_of(const zro_sim* s, int64_t i, const double* sig)
 {
@@ -2214,22 +2264,39 @@ static double admm_omega_of(const zro_sim* s, int64_t i, const double* sig)
 }
 
 /* multiObjectCollisionWithFakeSeparate (CollisionObject.cpp:108-160): resolves v,
-   returns 1 and the projection P = I - K K^T when any collider acts */
-static int admm_collision(const zro_sim* s, double t, const double* x, double* v, double P[3][3])
+   returns 1 and the projection P = I - K K^T when any collider acts.
+
+   A separating object releases the node when its relative normal velocity
+   (v - v_obj).n is positive.  Inside the ADMM loop that test is repeated on
+   nodes the previous iteration projected: their relative normal velocity is
+   then exactly zero (the projection removed it and the global step, whose
+   preconditioner is a scalar per node, AdmmSimulationFoam.h:476-503, only
+   moves dv within the projected space), so in floating point the reference
+   decides it by rounding noise, and a 1e-15 perturbation of the input changes
+   the step's result by percent.  The test here is against a rounding bound,
+   (v - v_obj).n > kSeparateRoundoff sum_d (|v_d| + |v_obj,d|), which gives the
+   exact-arithmetic outcome: a node with zero relative normal velocity stays
+   in contact. */
+#define ZRO_SEPARATE_ROUNDOFF 1e-12
+int zro_fake_separate(const zro_collider* cols, int ncol, double t, const double* x, double* v, double* Pout)
 {
   int any = 0, nslip = 0;
-  double K[3][3];
-  for (int k = 0; k < s->ncol; ++k) {
-    const zro_collider* c = &s->col[k];
-    double b[3], xb[3], phi, n[3], vo[3], old[3] = { v[0], v[1], v[2] };
-    collider_b(s, c, t, b);
-    if (!collider_query(c, b, x, &phi, n, xb)) continue;
-    collider_velocity(c, xb, vo);
-    int col = detect_and_resolve(c, b, x, v);
+  double K[3][3], P[3][3];
+  for (int k = 0; k < ncol; ++k) {
+    const zro_collider* c = &cols[k];
+    double xb[3], phi, n[3], vo[3], old[3] = { v[0], v[1], v[2] };
+    zro_frame fr;
+    collider_frame(c, t, &fr);
+    if (!collider_query(c, &fr, x, &phi, n, xb)) continue;
+    collider_velocity(c, &fr, xb, vo);
+    int col = detect_and_resolve(c, &fr, x, v);
     if (col && c->type == 3) {
-      double dot = 0.0;
-      for (int d = 0; d < 3; ++d) dot += (old[d] - vo[d]) * n[d];
-      if (dot > 0) {
+      double dot = 0.0, scale = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        dot += (old[d] - vo[d]) * n[d];
+        scale += fabs(old[d]) + fabs(vo[d]);
+      }
+      if (dot > ZRO_SEPARATE_ROUNDOFF * scale) {
         for (int d = 0; d < 3; ++d) v[d] = old[d];
         col = 0;
       }
@@ -2237,7 +2304,7 @@ static int admm_collision(const zro_sim* s, double t, const double* x, double* v
     any = any || col;
     if (!col) continue;
     if (c->type == 1) {
-      memset(P, 0, sizeof(double) * 9);
+      memset(Pout, 0, sizeof(double) * 9);
       return 1;
     }
     for (int q = 0; q < nslip; ++q) {
@@ -2255,7 +2322,13 @@ static int admm_collision(const zro_sim* s, double t, const double* x, double* v
       P[i][j] = i == j ? 1.0 : 0.0;
       for (int q = 0; q < nslip; ++q) P[i][j] -= K[q][i] * K[q][j];
     }
+  memcpy(Pout, P, sizeof(P));
   return any;
+}
+
+static int admm_collision(const zro_sim* s, double t, const double* x, double* v, double P[3][3])
+{
+  return zro_fake_separate(s->col, s->ncol, t, x, v, &P[0][0]);
 }
 
 /* updateCollisionNodes (AdmmSimulationFoam.h:276-294) */

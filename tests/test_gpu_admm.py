@@ -58,5 +58,8 @@ def test_admm_steps_parity(kind):
     assert rel_err(out["V"], V) < 1e-8
     assert rel_err(out["F"], F) < 1e-9
     if kind != "elastic":
-        assert rel_err(eng.get_materials()["plastic_state"], ora.get_materials()["pstate"]) < 1e-8 or kind == "dp"
+        pg, po = eng.get_materials()["plastic_state"], ora.get_materials()["pstate"]
+        worst = int(np.argmax(np.abs(pg - po)))
+        print(kind, "plastic state: worst particle", worst, pg[worst], po[worst], flush=True)
+        assert rel_err(pg, po) < 1e-8
     eng.close()

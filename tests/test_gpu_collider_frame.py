@@ -77,18 +77,13 @@ def test_adjoint_refuses_a_spinning_collider():
     prob.close()
 
 
-_ADMM_SEPARATE = pytest.mark.skip(
-    reason="open: the ADMM step with a separating collider (identity frame too) takes other MINRES counts "
-           "than the oracle (136 vs 106 at the first step of this scene); sticky colliders match")
-
-
-@pytest.mark.parametrize("separating", [None, pytest.param("plain", marks=_ADMM_SEPARATE),
-                                        pytest.param("framed", marks=_ADMM_SEPARATE)])
+@pytest.mark.parametrize("separating", [None, "plain", "framed"])
 def test_admm_stretch_with_a_spinning_sphere(separating):
     """The ADMM stretch scene's colliders (AdmmInit3D.h:77-117): a sticky
     floor, a static sticky sphere at one end of the bar and one spinning
-    about the bar's axis at the other (R = I, b fixed, omega along x);
-    optionally a separating sphere, plain or in a rotated frame."""
+    about the bar's axis at the other (omega along x); optionally a
+    separating sphere, plain or in a rotated frame, whose release test is
+    taken against a rounding bound in both the engine and the oracle."""
     from ziran2019_amd.engine import ZR_MODEL_STVK_HENCKY, admm_opts
     s = perturbed(45, n_cells=5, shuffle=False, F_noise=0.02, C_noise=0.1, E=1e5)
     X = s["X"]

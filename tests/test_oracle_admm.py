@@ -45,11 +45,18 @@ def _scene(seed=4):
     return s
 
 
-def _balance_residual(s, iterations, dt=1e-3):
+def _balance_residual(s, iterations, dt=1e-3, plastic=None):
+    """Balance residual after an ADMM step, relative to the largest nodal
+    force.  With plasticity (kind, pparam, pstate) the stress is the Hencky
+    stress of the return-mapped F(v): the local step's gradient is
+    vol dE(projectSigma(z)) (AdmmSimulationFoam.h:534-547)."""
     n = len(s["mass"])
     sim = oracle.OracleSim(dx=s["dx"], gravity=(0.0, -9.8, 0.0), nthreads=4)
     sim.set_particles(s["X"], s["V"], s["C"], s["F"], s["mass"], s["vol"], s["mu"], s["lam"])
-    sim.set_materials(model=1)
+    if plastic is None:
+        sim.set_materials(model=1)
+    else:
+        sim.set_materials(model=1, plastic=plastic[0], pparam=plastic[1], pstate=plastic[2])
     sim.step_admm(dt, oracle.admm_opts(admm_iterations=iterations, global_tolerance=1e-14))
     g = sim.grid()
     info = sim.sort_result()
@@ -75,6 +82,10 @@ def _balance_residual(s, iterations, dt=1e-3):
                                             bw[0][1][a] * bw[1][1][b] * bw[2][2][c]]))
         G = sum(np.outer(v[k], gw) for k, gw in zip(ks, gs))
         Fp = (np.eye(3) + dt * G) @ Fn[p]
+        if plastic is not None:
+            U, sig, Vt = np.linalg.svd(Fp)
+            Fp = U @ np.diag(oracle.project_sigma(s["mu"][p], s["lam"][p], plastic[0], plastic[1], plastic[2],
+                                                  sig)) @ Vt
         P = oracle.stvk_hencky_first_piola(s["mu"][p], s["lam"][p], Fp)
         S = s["vol"][p] * P @ Fn[p].T
         for k, gw in zip(ks, gs):
@@ -88,3 +99,72 @@ def test_admm_converges_to_backward_euler_balance():
     res = [_balance_residual(s, k) for k in (5, 40, 300)]
     assert res[0] > res[1] > res[2]
     assert res[2] < 1e-10
+
+
+MU, LAM = oracle.lame(1e4, 0.3)
+VM = (1, (0.0, 1e30, 0.0, 0.0), 200.0)
+DP = (2, (oracle.drucker_prager_alpha(30.0), 1.0, 0.0, 1.0), 0.0)
+# singular values in each branch of projectSigma: von Mises inside / on the
+# yield surface; Drucker-Prager inside / on the cone / at the tip
+REGIMES = [("vm-inside", VM[0], VM[1], 1e4, (1.01, 0.99, 1.0)),
+           ("vm-yield", VM[0], VM[1], VM[2], (1.3, 0.8, 1.05)),
+           ("dp-inside", DP[0], DP[1], 0.0, (0.95, 0.92, 0.9)),
+           ("dp-cone", DP[0], DP[1], 0.0, (1.2, 0.9, 0.8)),
+           ("dp-tip", DP[0], DP[1], 0.0, (1.3, 1.1, 1.05))]
+
+
+@pytest.mark.parametrize("name,kind,pparam,pstate,sig", REGIMES)
+def test_project_sigma_is_the_return_map(name, kind, pparam, pstate, sig):
+    """projectSigma equals the singular values of projectStrain applied to
+    diag(sigma) (the return maps, pinned by their yield surfaces in
+    test_oracle_plasticity.py), for logJp = 0 and no hardening."""
+    z = oracle.project_sigma(MU, LAM, kind, pparam, pstate, sig)
+    F = np.diag(sig)
+    if kind == 1:
+        Fr = oracle.von_mises_project(MU, LAM, 0.0, 1e30, pstate, F)[0]
+    else:
+        Fr = oracle.drucker_prager_project(MU, LAM, pparam[0], 1.0, 0.0, True, 0.0, F)[0]
+    want = np.linalg.svd(Fr, compute_uv=False)
+    assert np.abs(np.sort(z)[::-1] - want).max() < 1e-14
+    if name.endswith("inside"):
+        assert np.abs(z - np.asarray(sig)).max() < 1e-15
+    if name == "dp-tip":
+        assert np.array_equal(z, np.ones(3))
+
+
+@pytest.mark.parametrize("name,kind,pparam,pstate,sig", REGIMES)
+def test_project_sigma_derivative_is_its_jacobian(name, kind, pparam, pstate, sig):
+    """projectSigmaAndDerivative (PlasticityApplier.cpp:389-423, 580-606):
+    the projected values are projectSigma's and the matrix is its Jacobian by
+    FP64 central differences (a dropped term of the cone derivative, a sign
+    or a transposed factor fails this)."""
+    sig = np.asarray(sig, dtype=np.float64)
+    zp, D = oracle.project_sigma_and_derivative(MU, LAM, kind, pparam, pstate, sig)
+    assert np.abs(zp - oracle.project_sigma(MU, LAM, kind, pparam, pstate, sig)).max() < 1e-15
+    h = 1e-6
+    J = np.zeros((3, 3))
+    for c in range(3):
+        e = np.zeros(3)
+        e[c] = h
+        J[:, c] = (oracle.project_sigma(MU, LAM, kind, pparam, pstate, sig + e) -
+                   oracle.project_sigma(MU, LAM, kind, pparam, pstate, sig - e)) / (2 * h)
+    assert np.abs(D - J).max() < 1e-8 * max(1.0, np.abs(J).max())
+    if name.endswith("yield") or name.endswith("cone"):
+        assert np.abs(D - np.eye(3)).max() > 1e-2  # the projection acts
+
+
+@pytest.mark.parametrize("plastic", [VM, DP], ids=["von-mises", "drucker-prager"])
+def test_plastic_admm_converges_to_projected_stress_balance(plastic):
+    """With von Mises or Drucker-Prager particles the converged ADMM step
+    balances momentum with the stress of the return-mapped F(v), and the
+    return map acts on the scene's particles."""
+    s = _scene(seed=5)
+    n = len(s["mass"])
+    Fn = s["F"].reshape(n, 3, 3).transpose(0, 2, 1)
+    moved = sum(np.abs(oracle.project_sigma(MU, LAM, plastic[0], plastic[1], plastic[2],
+                                            np.linalg.svd(Fn[p], compute_uv=False)) -
+                       np.linalg.svd(Fn[p], compute_uv=False)).max() > 1e-6 for p in range(n))
+    assert moved > n // 10
+    res = [_balance_residual(s, k, plastic=plastic) for k in (5, 40, 300)]
+    assert res[0] > res[1] > res[2]
+    assert res[2] < 1e-9

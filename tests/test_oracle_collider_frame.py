@@ -121,3 +121,139 @@ def test_eval_max_speed_of_a_spinning_sphere():
     corners = np.array([[lo[d] if (i >> d) & 1 else hi[d] for d in range(3)] for i in range(8)])
     speed = np.linalg.norm(np.cross(omega, corners - c), axis=1).max()
     assert dt == pytest.approx(CFL * s["dx"] / speed, rel=1e-14)
+
+
+def _skew(w):
+    return np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+
+
+def test_collider_frame_follows_constant_rates():
+    """updateState(t) for constant rates: b(t) = b0 + v t, s(t) = s0 + ds/dt t
+    and R(t) = exp(t [omega]x) R0 (the rotation the reference's scene
+    callbacks write as (cos(theta t/2), 0, sin(theta t/2), 0) for omega =
+    (0, theta, 0), AdmmInit3D.h:298-313)."""
+    from scipy.linalg import expm
+    q0 = _quat((0.3, -1.0, 0.4), 0.9)
+    omega = np.array([0.7, -1.3, 2.1])
+    kw = dict(shape=1, type=1, origin=(0.0, 0.0, 0.0), radius=0.1, translation=(0.1, 0.2, 0.3),
+              velocity=(1.0, -2.0, 0.5), rotation=q0, omega=omega, scale=1.4, dscale_dt=-0.2)
+    for t in (0.0, 1e-3, 0.37, 2.5):
+        b, R, s = oracle.collider_frame(t, **kw)
+        assert np.allclose(b, np.array([0.1, 0.2, 0.3]) + t * np.array([1.0, -2.0, 0.5]), rtol=0, atol=1e-15)
+        assert s == pytest.approx(1.4 - 0.2 * t, rel=1e-15)
+        want = expm(t * _skew(omega)) @ _rot(q0)
+        assert np.abs(R - want).max() < 1e-14
+    # the reference scene's form, and no rotation without omega
+    theta = 24 * 5 / 180 * np.pi
+    for t in (0.1, 1.3):
+        _, R, _ = oracle.collider_frame(t, shape=1, type=1, origin=(0, 0, 0), omega=(0.0, theta, 0.0))
+        assert np.abs(R - _rot((np.cos(theta * t / 2), 0.0, np.sin(theta * t / 2), 0.0))).max() < 1e-15
+    _, R, s = oracle.collider_frame(3.0, shape=1, type=1, origin=(0, 0, 0), rotation=q0)
+    assert np.abs(R - _rot(q0)).max() < 1e-16
+    assert s == 1.0
+
+
+def test_growing_spinning_sphere_over_several_steps():
+    """A sticky sphere that translates, spins and grows: in step k (from t_k
+    to t_k + dt) every node inside takes the object's velocity at t_k,
+    omega x (x - b(t_k)) + (ds/dt / s(t_k))(x - b(t_k)) + db/dt, and the
+    B-spline G2P reproduces that affine field at the particles' positions at
+    t_k.  With s held at its time-0 value the dilation rate would be
+    ds/dt / s0, 10% off after the steps below."""
+    s = perturbed(34, n_cells=4, F_noise=0.0, C_noise=0.0)
+    X0 = s["X"].copy()
+    c = X0.mean(axis=0)
+    radius = np.sqrt(3) * np.abs(X0 - c).max() + 6 * s["dx"]
+    omega, dsdt, vb, s0 = np.array([2.0, -1.0, 3.0]), 40.0, np.array([0.05, -0.02, 0.01]), 1.0
+    sphere = dict(shape=1, type=1, origin=(0.0, 0.0, 0.0), radius=radius, translation=c, velocity=vb,
+                  omega=omega, scale=s0, dscale_dt=dsdt)
+    o = _sim(s, [sphere], gravity=(0.0, 0.0, 0.0))
+    K = 5
+    for _ in range(K - 1):
+        o.step(DT)
+    Xk = o.get_particles()[0].copy()
+    o.step(DT)
+    V = o.get_particles()[1]
+    tk = (K - 1) * DT
+    bk, sk = c + vb * tk, s0 + dsdt * tk
+    want = np.cross(omega, Xk - bk) + (dsdt / sk) * (Xk - bk) + vb
+    assert np.abs(V - want).max() < 1e-11 * np.abs(want).max()
+    stale = np.cross(omega, Xk - bk) + (dsdt / s0) * (Xk - bk) + vb
+    assert np.abs(stale - want).max() > 1e-3 * np.abs(want).max()
+
+
+def _sep(**kw):
+    d = dict(shape=1, type=3, origin=(0.0, 0.0, 0.0), radius=1.0)
+    d.update(kw)
+    return d
+
+
+def test_fake_separate_single_node_closed_forms():
+    """multiObjectCollisionWithFakeSeparate (CollisionObject.cpp:108-160) on
+    one node inside a separating unit sphere, outward normal n = x/|x|:
+    approaching (v.n < 0) projects the normal part away and keeps the node in
+    the projected set (P = I - n n^T); separating (v.n > 0) leaves v and the
+    node alone; v.n = 0 stays in contact, also when the zero carries
+    rounding noise; friction shortens the tangential part by mu |v.n|;
+    everything is relative to the object's own velocity."""
+    x = np.array([0.3, -0.4, 0.5])
+    n = x / np.linalg.norm(x)
+    t1 = np.cross(n, [1.0, 0.0, 0.0]); t1 /= np.linalg.norm(t1)
+    vt = 0.7 * t1
+    I = np.eye(3)
+    hit, v, P = oracle.fake_separate([_sep()], x, vt - 2.0 * n)
+    assert hit and np.abs(v - vt).max() < 1e-15 and np.abs(P - (I - np.outer(n, n))).max() < 1e-15
+    hit, v, P = oracle.fake_separate([_sep()], x, vt + 2.0 * n)
+    assert not hit and np.array_equal(v, vt + 2.0 * n) and np.array_equal(P, I)
+    for noise in (0.0, 3e-17, -3e-17):
+        hit, v, P = oracle.fake_separate([_sep()], x, vt + noise * n)
+        assert hit and np.abs(P - (I - np.outer(n, n))).max() < 1e-15
+    # Coulomb friction: |v_t| -> |v_t| - mu |v.n|, or zero when that is negative
+    hit, v, _ = oracle.fake_separate([_sep(friction=0.2)], x, vt - 2.0 * n)
+    assert hit and np.abs(v - vt * (1 - 0.2 * 2.0 / 0.7)).max() < 1e-15
+    hit, v, _ = oracle.fake_separate([_sep(friction=0.5)], x, vt - 2.0 * n)
+    assert hit and np.abs(v).max() < 1e-15
+    # a moving object: the test and the projection act on v - v_obj
+    vo = np.array([0.4, 1.0, -0.3])
+    hit, v, _ = oracle.fake_separate([_sep(velocity=vo)], x, vo + vt - 2.0 * n)
+    assert hit and np.abs(v - (vo + vt)).max() < 1e-15
+    hit, v, _ = oracle.fake_separate([_sep(velocity=vo)], x, vo + vt + 2.0 * n)
+    assert not hit
+    # a sticky object wins outright; two contacts give an orthonormal basis
+    hit, v, P = oracle.fake_separate([_sep(), dict(shape=1, type=1, origin=(0, 0, 0), radius=1.0)], x,
+                                     vt - 2.0 * n)
+    assert hit and np.array_equal(P, np.zeros((3, 3))) and np.abs(v).max() == 0.0
+    m = np.array([0.0, 1.0, 0.0])
+    floor = dict(shape=0, type=3, origin=(0.0, 0.0, 0.0), normal=tuple(m))
+    hit, v, P = oracle.fake_separate([_sep(), floor], x, vt - 2.0 * n - 1.0 * m)
+    k2 = m - (m @ n) * n
+    k2 /= np.linalg.norm(k2)
+    assert hit and np.abs(P - (I - np.outer(n, n) - np.outer(k2, k2))).max() < 1e-14
+
+
+def test_admm_step_with_a_separating_sphere_is_well_conditioned():
+    """The ADMM step re-tests the separation of nodes the previous iteration
+    projected, whose relative normal velocity is zero in exact arithmetic.
+    With that test taken against a rounding bound, a 1e-15 change of the
+    input velocities changes the step's result at the 1e-14 level (taken
+    against zero, as the reference writes it, by 1-13%), and the separating
+    sphere still acts."""
+    def run(scale, separating=True):
+        s = perturbed(45, n_cells=5, shuffle=False, F_noise=0.02, C_noise=0.1, E=1e5)
+        s["V"] = s["V"] * scale
+        lo, hi, c = s["X"].min(axis=0), s["X"].max(axis=0), s["X"].mean(axis=0)
+        r = 0.3 * (hi[1] - lo[1])
+        cols = [dict(shape=0, type=1, origin=(0.0, lo[1] - 0.5 * s["dx"], 0.0), normal=(0.0, 1.0, 0.0)),
+                dict(shape=1, type=1, origin=(0.0, 0.0, 0.0), radius=r, translation=(lo[0], c[1], c[2]))]
+        if separating:
+            cols.append(_frames(s)[0][1])
+        o = _sim(s, cols)
+        o.set_materials(model=1, plastic=0, pparam=(0, 0, 0, 0), pstate=0.0)
+        counts = [o.step_admm(1e-3, oracle.admm_opts(admm_iterations=20))[1].minres_iterations for _ in range(2)]
+        return counts, o.get_particles()[1]
+    ca, Va = run(1.0)
+    cb, Vb = run(1.0 + 1e-15)
+    _, Vn = run(1.0, separating=False)
+    assert ca == cb
+    assert np.abs(Va - Vb).max() < 1e-12 * np.abs(Va).max()
+    assert np.abs(Va - Vn).max() > 1e-6 * np.abs(Va).max()

@@ -147,3 +147,29 @@ def test_fcr_invariances(oracle_lib):
         tau = oracle.fcr_kirchhoff(mu, lam, F)
         assert np.allclose(tau, P @ F.T, rtol=1e-12, atol=1e-10 * np.abs(tau).max())
         assert np.allclose(tau, tau.T, atol=1e-9 * np.abs(tau).max())
+
+
+def test_fcr_hand_values(oracle_lib):
+    """Fixed-corotated psi = mu |F - R|^2 + lambda/2 (J - 1)^2 and
+    P = 2 mu (F - R) + lambda (J - 1) J F^-T (Stomakhin et al. 2012) at
+    diagonal F, worked by hand: R = I for positive entries; for an inverted F
+    the rotation comes from the sign-adjusted SVD (only the smallest singular
+    value negative), so diag(1.2, 0.9, -1.1) has sigma = (1.2, 1.1, -0.9)."""
+    mu, lam = 2.0, 3.0
+    F = np.diag([1.2, 0.9, 1.1])
+    # mu (0.2^2 + 0.1^2 + 0.1^2) + 1.5 (1.188 - 1)^2 = 0.12 + 0.053016
+    assert oracle.fcr_psi(mu, lam, F) == pytest.approx(0.173016, rel=1e-14)
+    P = oracle.fcr_first_piola(mu, lam, F)
+    # 2 mu (F_ii - 1) + lambda (J - 1) J / F_ii with J = 1.188
+    want = np.diag([0.8 + 3 * 0.188 * 1.188 / 1.2, -0.4 + 3 * 0.188 * 1.188 / 0.9,
+                    0.4 + 3 * 0.188 * 1.188 / 1.1])
+    assert np.abs(P - want).max() < 1e-14
+    # inverted: mu (0.2^2 + 0.1^2 + 1.9^2) + 1.5 (-1.188 - 1)^2 = 7.32 + 7.181016
+    Fi = np.diag([1.2, 0.9, -1.1])
+    assert oracle.fcr_psi(mu, lam, Fi) == pytest.approx(14.501016, rel=1e-14)
+    # the same energy for any rotation applied on either side
+    rng = np.random.default_rng(10)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] *= -1
+    assert oracle.fcr_psi(mu, lam, q @ Fi @ q.T) == pytest.approx(14.501016, rel=1e-12)

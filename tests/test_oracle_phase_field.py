@@ -193,3 +193,62 @@ def test_naive_rankine_cut():
         expect = min(1.0, sigma_F / smax) if smax > sigma_F else 1.0
         assert c[p] == pytest.approx(expect, rel=1e-12)
     assert c.min() < 1.0
+
+
+def _lattice_bar(nx, nyz, dx, lo=8):
+    """8 particles per cell on a regular lattice (cell centre +- dx/4) over
+    cells [lo, lo + nx) x [lo, lo + nyz)^2; unit density, identity F."""
+    o = (np.arange(2) + 0.5) * 0.5
+    cells = np.stack(np.meshgrid(np.arange(nx), np.arange(nyz), np.arange(nyz), indexing="ij"), -1).reshape(-1, 3)
+    sub = np.stack(np.meshgrid(o, o, o, indexing="ij"), -1).reshape(-1, 3)
+    X = ((cells[:, None, :] + lo + sub[None, :, :]) * dx).reshape(-1, 3)
+    n = len(X)
+    vol = np.full(n, dx ** 3 / 8)
+    mu, lam = scenes.lame(500.0, 0.4)
+    return dict(X=X, V=np.zeros((n, 3)), C=np.zeros((n, 9)), F=np.tile(np.eye(3).reshape(9), (n, 1)),
+                mass=vol.copy(), vol=vol, mu=np.full(n, mu), lam=np.full(n, lam), dx=dx)
+
+
+@pytest.mark.parametrize("M", [0.0, 8.0])
+def test_uniform_block_has_the_closed_form_solution(M):
+    """With uniform c0 and pf_Fp = f the lumped system (FractureSimulation.h:
+    312-437) is solved by a constant, because the gradient term annihilates
+    constants: x = (M + c0/dt) / (M f + 1/dt) for M > 0, x = 1/f for M = 0;
+    phaseG2P then gives c = min(c0, x) (the increment form for M > 0)."""
+    s = _lattice_bar(6, 6, 1.0 / 64)
+    dt, c0, f = 1e-3, 0.9, 3.0
+    opts = oracle.pf_opts(parabolic_M=M, cg_max_iterations=500, cg_tolerance=1e-300, cg_relative=1e-15)
+    sim = _sim(s, opts, c=c0, l0=2 * s["dx"], pf_Fp=f, one_over_sigma_c=1e-3, H=1e300)
+    sim.sort_and_pollute()
+    sim.update_stress()
+    sim.p2g()
+    sim.phase_field_solve(dt)
+    c = sim.get_phase_field()["c"]
+    x = (M + c0 / dt) / (M * f + 1.0 / dt) if M > 0 else 1.0 / f
+    assert np.abs(c - min(c0, x)).max() < 1e-10
+    assert x < c0  # the solve lowers c
+
+
+def test_damage_front_decays_over_2_l0():
+    """Next to a fully damaged region (pf_Fp = 1e4) an intact one (pf_Fp = 1)
+    approaches c = 1 as 1 - c ~ exp(-x / (2 l0)): the 1D form of
+    f c - 4 l0^2 c'' = 1 (CD-MPM's phase-field equation, the coefficient
+    vol 4 l0^2 of FractureSimulation.h:360).  A coefficient of 2 l0^2 or
+    8 l0^2 changes the length by a factor sqrt(2)."""
+    dx = 1.0 / 128
+    s = _lattice_bar(72, 4, dx)
+    l0 = 3 * dx
+    xmid = (8 + 24) * dx
+    f = np.where(s["X"][:, 0] < xmid, 1e4, 1.0)
+    opts = oracle.pf_opts(parabolic_M=0.0, cg_max_iterations=2000, cg_tolerance=1e-300, cg_relative=1e-14)
+    sim = _sim(s, opts, l0=l0, pf_Fp=f, one_over_sigma_c=1e-3, H=1e300)
+    sim.sort_and_pollute()
+    sim.update_stress()
+    sim.p2g()
+    sim.phase_field_solve(1e-3)
+    c = sim.get_phase_field()["c"]
+    x = s["X"][:, 0] - xmid
+    sel = (x > 3 * dx) & (x < 20 * dx)
+    slope = np.polyfit(x[sel], np.log(1.0 - c[sel]), 1)[0]
+    assert -1.0 / slope == pytest.approx(2 * l0, rel=0.08)
+    assert c[s["X"][:, 0] < xmid - 3 * dx].max() < 1e-3

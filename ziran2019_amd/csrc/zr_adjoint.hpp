@@ -30,10 +30,11 @@ struct AdjointView {
   double* Fg;    // [9][n] F-adjoint of the F update (scratch)
 };
 
-// d v_after / d v_before of the node's collider handling: multi_object_collision,
-// then trial_collision at t + dt when nothing collided.  Returns false for a
-// branch without a derivative (Coulomb friction, a node at a sphere centre).
-__device__ bool collide_jacobian(const ColliderSet& cs, double t, double dt, const double x[3], const double vpre[3],
+// d v_after / d v_before of the node's collider handling: multi_object_collision
+// in the frame at t, then trial_collision in the frame at t + dt when nothing
+// collided (translating colliders: R = I, s = 1).  Returns false for a branch
+// without a derivative (Coulomb friction, a node at a sphere centre).
+__device__ bool collide_jacobian(const ColliderSet& cs, double dt, const double x[3], const double vpre[3],
                                  double J[3][3])
 {
   double v[3] = { vpre[0], vpre[1], vpre[2] };
@@ -43,7 +44,7 @@ __device__ bool collide_jacobian(const ColliderSet& cs, double t, double dt, con
   for (int k = 0; k < cs.n; ++k) {
     const DevCollider& c = cs.c[k];
     double X[3], phi, N[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] - (c.b0[d] + c.velocity[d] * t);
+    for (int d = 0; d < 3; ++d) X[d] = x[d] - c.f[0].b[d];
     if (!ls_query_inside(c, X, phi, N)) continue;
     any = true;
     if (c.type == ZR_STICKY) {
@@ -70,7 +71,7 @@ __device__ bool collide_jacobian(const ColliderSet& cs, double t, double dt, con
   for (int k = 0; k < cs.n; ++k) {
     const DevCollider& c = cs.c[k];
     double X[3], phi, n[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - (c.b0[d] + c.velocity[d] * (t + dt));
+    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - c.f[1].b[d];
     if (!ls_query_inside(c, X, phi, n)) continue;
     double A[3][3], T[3][3];
     if (c.shape == ZR_HALF_SPACE) {
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(256) void k_adj_g2p(ParticlesView P, AdjointView A,
 __global__ __launch_bounds__(256) void k_adj_grid(uint32_t nn, const uint32_t* __restrict__ slot_page,
                                                   const int32_t* __restrict__ slot_src,
                                                   const double* __restrict__ stage, GridView G, double* ag,
-                                                  double dx, double dt, double gx, double gy, double gz, double time,
+                                                  double dx, double dt, double gx, double gy, double gz,
                                                   ColliderSet cs, uint32_t* bad)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(256) void k_adj_grid(uint32_t nn, const uint32_t* _
     const double vpre[3] = { v[0] + gx * dt + f[0] * dt_over_mi, v[1] + gy * dt + f[1] * dt_over_mi,
                              v[2] + gz * dt + f[2] * dt_over_mi };
     double J[3][3];
-    if (!collide_jacobian(cs, time, dt, xi, vpre, J)) atomicAdd(bad, 1u);
+    if (!collide_jacobian(cs, dt, xi, vpre, J)) atomicAdd(bad, 1u);
     double mb = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {

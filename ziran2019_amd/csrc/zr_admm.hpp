@@ -268,8 +268,22 @@ __device__ __forceinline__ double admm_omega(const PlasticParams& q, double mu, 
   return sqrt(vol * sqrt(total));
 }
 
-// multiObjectCollisionWithFakeSeparate (CollisionObject.cpp:108-160)
-__device__ bool admm_collision(const ColliderSet& cs, double t, const double x[3], double v[3], double P[3][3])
+// multiObjectCollisionWithFakeSeparate (CollisionObject.cpp:108-160) in the
+// frame at the step's start.
+//
+// A separating object releases the node when its relative normal velocity
+// (v - v_obj).n is positive.  The ADMM loop repeats that test on nodes the
+// previous iteration projected, whose relative normal velocity is then
+// exactly zero: the projection removed it, and the global step only moves dv
+// inside the projected space because its preconditioner is one scalar per
+// node (AdmmSimulationFoam.h:476-503).  Tested against zero in floating point,
+// those nodes are released or kept by rounding noise, and a 1e-15 change of
+// the input changes the step's result by percent.  The test here is against
+// a rounding bound, (v - v_obj).n > kSeparateRoundoff sum_d (|v_d| + |v_obj,d|),
+// which gives the exact-arithmetic outcome: zero relative normal velocity
+// stays in contact.
+constexpr double kSeparateRoundoff = 1e-12;
+__device__ bool admm_collision(const ColliderSet& cs, const double x[3], double v[3], double P[3][3])
 {
   bool any = false;
   int nslip = 0;
@@ -277,13 +291,17 @@ __device__ bool admm_collision(const ColliderSet& cs, double t, const double x[3
   for (int k = 0; k < cs.n; ++k) {
     const DevCollider& c = cs.c[k];
     double xb[3], phi, n[3], vo[3];
-    if (!collider_query(c, t, x, phi, n, xb)) continue;
-    collider_velocity(c, xb, vo);
+    if (!collider_query(c, 0, x, phi, n, xb)) continue;
+    collider_velocity(c, 0, xb, vo);
     const double old[3] = { v[0], v[1], v[2] };
-    bool col = detect_and_resolve(c, t, x, v);
+    bool col = detect_and_resolve(c, 0, x, v);
     if (col && c.type == ZR_SEPARATE) {
-      const double dot = (old[0] - vo[0]) * n[0] + (old[1] - vo[1]) * n[1] + (old[2] - vo[2]) * n[2];
-      if (dot > 0.0) {
+      double dot = 0.0, scale = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        dot += (old[d] - vo[d]) * n[d];
+        scale += fabs(old[d]) + fabs(vo[d]);
+      }
+      if (dot > kSeparateRoundoff * scale) {
         v[0] = old[0];
         v[1] = old[1];
         v[2] = old[2];
@@ -346,7 +364,7 @@ __global__ __launch_bounds__(256) void k_admm_init_nodes(uint32_t nn, const int3
 // updateCollisionNodes (:276-294)
 __global__ __launch_bounds__(256) void k_admm_collide(uint32_t nn, const uint32_t* __restrict__ slot_page,
                                                       const double* __restrict__ gm, AdmmNodes N, double dx,
-                                                      double time, ColliderSet cs)
+                                                      ColliderSet cs)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   if (k >= nn) return;
@@ -356,7 +374,7 @@ __global__ __launch_bounds__(256) void k_admm_collide(uint32_t nn, const uint32_
     double xi[3], vi[3], P[3][3];
     node_pos(node_coord(slot_page, k), dx, xi);
     for (int d = 0; d < 3; ++d) vi[d] = N.dv[d * S + k] + N.vn[d * S + k];
-    if (admm_collision(cs, time, xi, vi, P)) {
+    if (admm_collision(cs, xi, vi, P)) {
       hit = 1;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) N.proj[(3 * i + j) * S + k] = P[i][j];

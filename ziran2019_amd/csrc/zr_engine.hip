@@ -52,11 +52,17 @@ constexpr int kMaxColliders = 8;
 constexpr int kSortLds = 2048;  // particles per page sorted in LDS
 constexpr uint32_t kNone = 0xffffffffu;
 
-// a collider as the kernels read it: the frame's rotation as a matrix
-// (row-major), 1/s, b(0)
+// a collider as the kernels read it: its rates and its frame x = R s X + b
+// at two times, f[0] at the step's start t and f[1] at t + dt (the state
+// updateState(t + dt) sets before the trial collision and G2P,
+// MpmSimulationBase.cpp:273-278), each with R as a row-major matrix and 1/s
+struct ColliderFrame {
+  double b[3], R[9], s, inv_s;
+};
 struct DevCollider {
   int32_t shape, type;
-  double friction, origin[3], normal[3], radius, velocity[3], b0[3], R[9], omega[3], s, inv_s, dsdt;
+  double friction, origin[3], normal[3], radius, velocity[3], omega[3], dsdt;
+  ColliderFrame f[2];
 };
 struct ColliderSet {
   int32_t n;
@@ -104,34 +110,35 @@ __device__ __forceinline__ bool ls_query_inside(const DevCollider& c, const doub
   return false;
 }
 
-// queryInside in world space (CollisionObject.cpp:279-296): xb = x - b(t),
-// X = R^T xb / s; inside: n = R N and phi scaled by s
-__device__ __forceinline__ bool collider_query(const DevCollider& c, double t, const double x[3], double& phi,
+// queryInside in world space (CollisionObject.cpp:279-296) in frame fr:
+// xb = x - b, X = R^T xb / s; inside: n = R N and phi scaled by s
+__device__ __forceinline__ bool collider_query(const DevCollider& c, int fr, const double x[3], double& phi,
                                                double n[3], double xb[3])
 {
+  const ColliderFrame& f = c.f[fr];
   double X[3], N[3];
-  for (int d = 0; d < 3; ++d) xb[d] = x[d] - (c.b0[d] + c.velocity[d] * t);
-  for (int d = 0; d < 3; ++d) X[d] = (c.R[d] * xb[0] + c.R[3 + d] * xb[1] + c.R[6 + d] * xb[2]) * c.inv_s;
+  for (int d = 0; d < 3; ++d) xb[d] = x[d] - f.b[d];
+  for (int d = 0; d < 3; ++d) X[d] = (f.R[d] * xb[0] + f.R[3 + d] * xb[1] + f.R[6 + d] * xb[2]) * f.inv_s;
   if (!ls_query_inside(c, X, phi, N)) return false;
-  for (int d = 0; d < 3; ++d) n[d] = c.R[3 * d] * N[0] + c.R[3 * d + 1] * N[1] + c.R[3 * d + 2] * N[2];
-  phi *= c.s;
+  for (int d = 0; d < 3; ++d) n[d] = f.R[3 * d] * N[0] + f.R[3 * d + 1] * N[1] + f.R[3 * d + 2] * N[2];
+  phi *= f.s;
   return true;
 }
 
 // the object's velocity at x: omega x (x - b) + (ds/dt / s)(x - b) + db/dt
-__device__ __forceinline__ void collider_velocity(const DevCollider& c, const double xb[3], double vo[3])
+__device__ __forceinline__ void collider_velocity(const DevCollider& c, int fr, const double xb[3], double vo[3])
 {
-  const double g = c.dsdt * c.inv_s;
+  const double g = c.dsdt * c.f[fr].inv_s;
   vo[0] = (c.omega[1] * xb[2] - c.omega[2] * xb[1]) + g * xb[0] + c.velocity[0];
   vo[1] = (c.omega[2] * xb[0] - c.omega[0] * xb[2]) + g * xb[1] + c.velocity[1];
   vo[2] = (c.omega[0] * xb[1] - c.omega[1] * xb[0]) + g * xb[2] + c.velocity[2];
 }
 
-__device__ __forceinline__ bool detect_and_resolve(const DevCollider& c, double t, const double x[3], double v[3])
+__device__ __forceinline__ bool detect_and_resolve(const DevCollider& c, int fr, const double x[3], double v[3])
 {
   double xb[3], phi, N[3], vo[3];
-  if (!collider_query(c, t, x, phi, N, xb)) return false;
-  collider_velocity(c, xb, vo);
+  if (!collider_query(c, fr, x, phi, N, xb)) return false;
+  collider_velocity(c, fr, xb, vo);
   for (int d = 0; d < 3; ++d) v[d] -= vo[d];
   if (c.type == ZR_STICKY) {
     v[0] = v[1] = v[2] = 0.0;
@@ -153,23 +160,23 @@ __device__ __forceinline__ bool detect_and_resolve(const DevCollider& c, double 
   return true;
 }
 
-__device__ __forceinline__ bool multi_object_collision(const ColliderSet& cs, double t, const double x[3], double v[3])
+__device__ __forceinline__ bool multi_object_collision(const ColliderSet& cs, int fr, const double x[3], double v[3])
 {
   bool any = false;
   for (int k = 0; k < cs.n; ++k) {
-    bool hit = detect_and_resolve(cs.c[k], t, x, v);
+    bool hit = detect_and_resolve(cs.c[k], fr, x, v);
     any = any || hit;
     if (hit && cs.c[k].type == ZR_STICKY) break;
   }
   return any;
 }
 
-__device__ __forceinline__ void trial_collision(const ColliderSet& cs, double t, const double x[3], double dt, double v[3])
+__device__ __forceinline__ void trial_collision(const ColliderSet& cs, int fr, const double x[3], double dt, double v[3])
 {
   for (int k = 0; k < cs.n; ++k) {
     double xt[3], xb[3], phi, n[3];
     for (int d = 0; d < 3; ++d) xt[d] = x[d] + dt * v[d];
-    if (collider_query(cs.c[k], t, xt, phi, n, xb))
+    if (collider_query(cs.c[k], fr, xt, phi, n, xb))
       for (int d = 0; d < 3; ++d) v[d] -= (phi / dt) * n[d];
   }
 }
@@ -513,7 +520,7 @@ template <int NF>
 __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const uint32_t* __restrict__ slot_page,
                                               const int32_t* __restrict__ slot_src,
                                               const double* __restrict__ stage, GridView G, double dx, double dt,
-                                              double gx, double gy, double gz, double time, ColliderSet cs)
+                                              double gx, double gy, double gz, ColliderSet cs)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   if (k >= nslots * 32u) return;
@@ -566,8 +573,8 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
     vi[0] = v[0] + gx * dt + f[0] * dt_over_mi;
     vi[1] = v[1] + gy * dt + f[1] * dt_over_mi;
     vi[2] = v[2] + gz * dt + f[2] * dt_over_mi;
-    bool collided = multi_object_collision(cs, time, xi, vi);
-    if (!collided) trial_collision(cs, time + dt, xi, dt, vi);
+    bool collided = multi_object_collision(cs, 0, xi, vi);
+    if (!collided) trial_collision(cs, 1, xi, dt, vi);
   }
   for (int d = 0; d < 3; ++d) G.vs[d * S + k] = vi[d];
 }
@@ -1152,15 +1159,11 @@ void free_particles(zr_engine* e)
 }
 
 // R from the quaternion <w,x,y,z> as Eigen::Quaternion::toRotationMatrix
-// forms it (Rotation.h:53-57), row-major; all zero: the identity
-void collider_rotation(const zr_collider& k, double R[9])
+// forms it (Rotation.h:53-57), row-major
+void quat_to_matrix(const double q[4], double R[9])
 {
 #pragma clang fp contract(off)
-  const double w = k.rotation[0], x = k.rotation[1], y = k.rotation[2], z = k.rotation[3];
-  if (w == 0.0 && x == 0.0 && y == 0.0 && z == 0.0) {
-    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
-    return;
-  }
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
   const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
   const double twx = tx * w, twy = ty * w, twz = tz * w, txx = tx * x, txy = ty * x, txz = tz * x, tyy = ty * y,
                tyz = tz * y, tzz = tz * z;
@@ -1174,20 +1177,48 @@ void collider_rotation(const zr_collider& k, double R[9])
   R[7] = tyz + twx;
   R[8] = 1 - (txx + tyy);
 }
-double collider_scale(const zr_collider& k) { return k.scale == 0.0 ? 1.0 : k.scale; }
+// The state updateState(t) sets for an object moving at constant rates from
+// its time-0 state (CollisionObject.h:81-172; the reference's scene callbacks
+// write this form, e.g. AdmmInit3D.h:298-313: rotation
+// (cos(theta t/2), 0, sin(theta t/2), 0) with omega (0, theta, 0)):
+// b(t) = translation + velocity t, s(t) = s0 + ds/dt t and
+// R(t) = exp(t [omega]x) R0, as the quaternion product
+// (cos(|omega| t/2), sin(|omega| t/2) omega/|omega|) q0.  A zero quaternion
+// means the identity, scale 0 means 1.
+void collider_frame(const zr_collider& c, double t, ColliderFrame& f)
+{
+#pragma clang fp contract(off)
+  double q0[4] = { c.rotation[0], c.rotation[1], c.rotation[2], c.rotation[3] };
+  if (q0[0] == 0.0 && q0[1] == 0.0 && q0[2] == 0.0 && q0[3] == 0.0) q0[0] = 1.0;
+  const double* w = c.omega;
+  const double wn = std::sqrt((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
+  double q[4] = { q0[0], q0[1], q0[2], q0[3] };
+  if (wn != 0.0 && t != 0.0) {
+    const double h = 0.5 * wn * t, sh = std::sin(h) / wn;
+    const double a[4] = { std::cos(h), sh * w[0], sh * w[1], sh * w[2] };
+    q[0] = a[0] * q0[0] - a[1] * q0[1] - a[2] * q0[2] - a[3] * q0[3];
+    q[1] = a[0] * q0[1] + a[1] * q0[0] + a[2] * q0[3] - a[3] * q0[2];
+    q[2] = a[0] * q0[2] - a[1] * q0[3] + a[2] * q0[0] + a[3] * q0[1];
+    q[3] = a[0] * q0[3] + a[1] * q0[2] - a[2] * q0[1] + a[3] * q0[0];
+  }
+  quat_to_matrix(q, f.R);
+  for (int d = 0; d < 3; ++d) f.b[d] = c.translation[d] + c.velocity[d] * t;
+  f.s = (c.scale == 0.0 ? 1.0 : c.scale) + c.dscale_dt * t;
+  f.inv_s = 1.0 / f.s;
+}
 
 // a collider whose frame neither rotates nor scales (a translation only)
 bool collider_translates_only(const zr_collider& k)
 {
-  double R[9];
-  collider_rotation(k, R);
+  ColliderFrame f;
+  collider_frame(k, 0.0, f);
   for (int i = 0; i < 9; ++i)
-    if (R[i] != ((i % 4 == 0) ? 1.0 : 0.0)) return false;
-  return collider_scale(k) == 1.0 && k.dscale_dt == 0.0 && k.omega[0] == 0.0 && k.omega[1] == 0.0 &&
-         k.omega[2] == 0.0;
+    if (f.R[i] != ((i % 4 == 0) ? 1.0 : 0.0)) return false;
+  return f.s == 1.0 && k.dscale_dt == 0.0 && k.omega[0] == 0.0 && k.omega[1] == 0.0 && k.omega[2] == 0.0;
 }
 
-ColliderSet collider_set(const zr_engine* e)
+// the colliders with their frames at the engine time t and at t + dt
+ColliderSet collider_set(const zr_engine* e, double dt = 0.0)
 {
   ColliderSet cs;
   memset(&cs, 0, sizeof(cs));
@@ -1203,13 +1234,11 @@ ColliderSet collider_set(const zr_engine* e)
       c.origin[d] = a.origin[d];
       c.normal[d] = a.normal[d];
       c.velocity[d] = a.velocity[d];
-      c.b0[d] = a.translation[d];
       c.omega[d] = a.omega[d];
     }
-    collider_rotation(a, c.R);
-    c.s = collider_scale(a);
-    c.inv_s = 1.0 / c.s;
     c.dsdt = a.dscale_dt;
+    collider_frame(a, e->time, c.f[0]);
+    collider_frame(a, e->time + dt, c.f[1]);
   }
   return cs;
 }
@@ -1335,17 +1364,16 @@ int do_p2g(zr_engine* e, bool mls)
 int do_grid(zr_engine* e, int mode, double dt)
 {
   GridView G{ e->gm, e->gv, e->gf, e->gvs, (int64_t)e->grid_cap };
-  ColliderSet cs = collider_set(e);
+  ColliderSet cs = collider_set(e, dt);
   prof_mark(e, ZR_PROF_GRID);
   if (e->S) {
     const uint32_t nb = blocks_for((uint64_t)e->S * 32, 256);
     if (e->mls_rasterised)
       k_grid<kP2GFields><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->slot_src, e->stage, G, e->cfg.dx,
-                                                    dt, e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2],
-                                                    e->time, cs);
+                                                    dt, e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], cs);
     else
       k_grid<4><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->slot_src, e->stage, G, e->cfg.dx, dt,
-                                           e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs);
+                                           e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], cs);
   }
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_GRID);
@@ -1535,8 +1563,8 @@ int do_adjoint_step(zr_engine* e, double dt)
   GridView G{ e->gm, e->gv, e->gf, e->gvs, (int64_t)e->grid_cap };
   // the grid kernel reads the adjoint stride-wise like the grid fields
   k_adj_grid<<<blocks_for(nn, 256), 256, 0, s>>>(nn, e->slot_page, e->slot_src, e->stage, G, e->adj_grid, dx, dt,
-                                                 e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time,
-                                                 collider_set(e), e->dcount + CNT_ADJ_BAD);
+                                                 e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2],
+                                                 collider_set(e, dt), e->dcount + CNT_ADJ_BAD);
   ZR_LAUNCH_CHECK(e);
   k_adj_p2g<<<e->A, 256, 0, s>>>(pview(e), A, e->order, e->group_start, e->group_page, e->group_nslot, e->adj_grid,
                                  (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx));
@@ -1598,7 +1626,7 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
   if (!first)
     k_admm_lastdv_p2g<<<e->A, 256, 0, s>>>(P, Ap, e->order, e->group_start, e->group_page, e->stage, inv_dx);
   k_admm_init_nodes<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, e->gv, Nd, first ? 0 : 1, dt);
-  k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, e->time, cs);
+  k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, cs);
   k_admm_init_particles<<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
                                               e->group_page, e->group_nslot, e->gm, e->stage, first ? 1 : 0, dt, rho,
                                               inv_dx);
@@ -1606,7 +1634,7 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
   ZR_LAUNCH_CHECK(e);
   int total_minres = 0;
   for (int tim = 0; tim < o->admm_iterations; ++tim) {
-    k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, e->time, cs);
+    k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, cs);
     k_admm_fstep<<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
                                       e->group_page, e->group_nslot, e->gm, dt, rho, o->local_tolerance, inv_dx);
     k_admm_scatter<0><<<e->A, 256, 0, s>>>(P, Ap, Nd, nullptr, e->order, e->group_start, e->group_page,
@@ -1792,7 +1820,7 @@ int be_state_residual(zr_engine* e, bool jac = false)
   prof_mark(e, ZR_PROF_STATE);
   k_be_residual<<<nb, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, e->stage, e->gm, nv(e, e->idv),
                                            nv(e, e->ir), dx, dt, e->cfg.gravity[0], e->cfg.gravity[1],
-                                           e->cfg.gravity[2], e->time, cs, e->partial);
+                                           e->cfg.gravity[2], cs, e->partial);
   k_reduce_cg<<<1, 256, 0, e->stream>>>(e->partial, nb, e->sc, e->ci, RED_STORE, SC_NORM2, 0.0, 0.0);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
@@ -1820,7 +1848,7 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, int32_t* ci, i
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
   k_cg_ap<T><<<ncg, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
-                                         nvT(e, out), dx, dt, e->time, cs, project, e->partial, e->sc, ci,
+                                         nvT(e, out), dx, dt, cs, project, e->partial, e->sc, ci,
                                          e->hvp_ctr);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
@@ -1975,7 +2003,7 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
   hipStream_t s = e->stream;
   ColliderSet cs = collider_set(e);
   k_be_init<<<nb, 256, 0, s>>>(nn, e->slot_page, e->gm, e->gv, nv(e, e->ivn), nv(e, e->idv), e->cfg.dx, dt,
-                               e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], e->time, cs,
+                               e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], cs,
                                e->collided);
   ZR_HIP(e, hipMemsetAsync(e->ci, 0, CI_N * sizeof(int32_t), s));
   ZR_LAUNCH_CHECK(e);
@@ -2095,7 +2123,7 @@ int zr_set_colliders(zr_engine* e, const zr_collider* c, int32_t n)
       return fail(e, ZR_ERR_ARG, "unknown collider shape");
     }
     if (k.type < ZR_STICKY || k.type > ZR_SEPARATE) return fail(e, ZR_ERR_ARG, "unknown collider type");
-    if (!(collider_scale(k) > 0.0)) return fail(e, ZR_ERR_ARG, "collider scale must be positive");
+    if (!(k.scale == 0.0 || k.scale > 0.0)) return fail(e, ZR_ERR_ARG, "collider scale must be positive");
   }
   return ZR_OK;
 }
@@ -2147,11 +2175,12 @@ int zr_calculate_dt(zr_engine* e, double max_dt, double* dt)
     double cs = 0.0;
     if (c.dscale_dt != 0.0 || c.omega[0] != 0.0 || c.omega[1] != 0.0 || c.omega[2] != 0.0) {
       if (c.shape != ZR_SPHERE) return fail(e, ZR_ERR_ARG, "a spinning or scaling half-space has no bounds");
-      double R[9], b[3], mn[3], mx[3];
-      collider_rotation(c, R);
-      const double s = collider_scale(c), inv_s = 1.0 / s;
+      double mn[3], mx[3];
+      ColliderFrame fr;
+      collider_frame(c, e->time, fr);
+      const double *R = fr.R, *b = fr.b;
+      const double s = fr.s, inv_s = fr.inv_s;
       for (int d = 0; d < 3; ++d) {
-        b[d] = c.translation[d] + u[d] * e->time;
         mn[d] = c.origin[d] - c.radius;
         mx[d] = c.origin[d] + c.radius;
       }

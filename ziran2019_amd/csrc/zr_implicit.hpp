@@ -110,13 +110,13 @@ __device__ __forceinline__ void block_partial(double contrib, double* partial)
 
 // collision projection P = I - K K^T of the node (CollisionObject.h:17-40),
 // with the normal basis of multiObjectCollision (CollisionObject.cpp:66-105)
-__device__ __forceinline__ void project_node(const ColliderSet& cs, double t, const double xi[3], double r[3])
+__device__ __forceinline__ void project_node(const ColliderSet& cs, const double xi[3], double r[3])
 {
   double K[3][3];
   int nslip = 0;
   for (int k = 0; k < cs.n; ++k) {
     double xb[3], phi, n[3];
-    if (!collider_query(cs.c[k], t, xi, phi, n, xb)) continue;
+    if (!collider_query(cs.c[k], 0, xi, phi, n, xb)) continue;
     if (cs.c[k].type == ZR_STICKY) {
       r[0] = r[1] = r[2] = 0.0;
       return;
@@ -1205,7 +1205,7 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
 __global__ __launch_bounds__(256) void k_be_init(uint32_t nn, const uint32_t* __restrict__ slot_page,
                                                  const double* __restrict__ gm, const double* __restrict__ gv,
                                                  NodeVec3 vn, NodeVec3 dv, double dx, double dt, double gx, double gy,
-                                                 double gz, double time, ColliderSet cs, uint8_t* __restrict__ collided)
+                                                 double gz, ColliderSet cs, uint8_t* __restrict__ collided)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   if (k >= nn) return;
@@ -1220,7 +1220,7 @@ __global__ __launch_bounds__(256) void k_be_init(uint32_t nn, const uint32_t* __
   double vi[3] = { v[0], v[1], v[2] };
   double xi[3];
   node_pos(node_coord(slot_page, k), dx, xi);
-  const bool hit = multi_object_collision(cs, time, xi, vi);
+  const bool hit = multi_object_collision(cs, 0, xi, vi);
   if (collided) collided[k] = hit ? 1 : 0;
   const double g[3] = { gx, gy, gz };
   for (int d = 0; d < 3; ++d) {
@@ -1235,7 +1235,7 @@ __global__ __launch_bounds__(256) void k_be_residual(uint32_t nn, const uint32_t
                                                      const int32_t* __restrict__ slot_src,
                                                      const double* __restrict__ stage, const double* __restrict__ gm,
                                                      NodeVec3 dv, NodeVec3 r, double dx, double dt, double gx,
-                                                     double gy, double gz, double time, ColliderSet cs,
+                                                     double gy, double gz, ColliderSet cs,
                                                      double* __restrict__ partial)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
@@ -1251,7 +1251,7 @@ __global__ __launch_bounds__(256) void k_be_residual(uint32_t nn, const uint32_t
       for (int d = 0; d < 3; ++d) rv[d] = dt * g[d] * m - dt * f[d] - m * dv(d, k);
       double xi[3];
       node_pos(nc, dx, xi);
-      project_node(cs, time, xi, rv);
+      project_node(cs, xi, rv);
       contrib = (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]) / m;
     }
     for (int d = 0; d < 3; ++d) r(d, k) = rv[d];
@@ -1331,7 +1331,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __restrict__ slot_page,
                                                const int32_t* __restrict__ slot_src, const T* __restrict__ stage,
                                                const double* __restrict__ gm, NodeVec<T> q, NodeVec<T> p,
-                                               NodeVec<T> ap, double dx, double dt, double time, ColliderSet cs,
+                                               NodeVec<T> ap, double dx, double dt, ColliderSet cs,
                                                int project, double* __restrict__ partial, const double* sc,
                                                const int32_t* ci, uint32_t* __restrict__ hvp_ctr)
 {
@@ -1388,7 +1388,7 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
       if (project) {
         double xi[3], a64[3] = { (double)av[0], (double)av[1], (double)av[2] };
         node_pos(node_coord(slot_page, k), dx, xi);
-        project_node(cs, time, xi, a64);
+        project_node(cs, xi, a64);
         for (int d = 0; d < 3; ++d) av[d] = (T)a64[d];
       }
     }
