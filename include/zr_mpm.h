@@ -48,6 +48,12 @@ enum { ZR_TRANSFER_MLS = 0, ZR_TRANSFER_APIC = 1 };
    Newton iteration; a node a collider touches gets its largest entry on
    every axis. */
 enum { ZR_PRECOND_MASS = 0, ZR_PRECOND_JACOBI = 1 };
+/* linear solver of the Newton step: ConjugateGradient (switchLinearSolver(CG),
+   ConjugateGradient.h:99-147) or Minres (Minres.h:69-178), the reference's
+   default for the backward-Euler objective (BackwardEuler.h:55-62: Minres(20),
+   tolerance 1), which also solves with the indefinite Hessians of the
+   elasticity models */
+enum { ZR_SOLVER_CG = 0, ZR_SOLVER_MINRES = 1 };
 
 /* elasticity models (Lib/Ziran/Physics/ConstitutiveModel/): fixed-corotated
    (no reference counterpart; the default), StvkWithHencky, NeoHookeanBorden.
@@ -214,9 +220,11 @@ typedef struct zr_step_stats {
 
 /* Options of the implicit backward-Euler step.  Reference defaults:
    NewtonsMethod(tol 1e-5, 3 iterations) (MpmSimulationBase.cpp:51),
-   ConjugateGradient(20) (BackwardEuler.h:90-91) with LinearSolver::tolerance
-   16 eps (LinearSolver.h:26) and the Newton-adaptive relative tolerance
-   min(0.5, sqrt(max(|r|, tol))) (NewtonsMethod.h:61). */
+   Minres(20) with tolerance 1 (BackwardEuler.h:55-62), or after
+   switchLinearSolver(CG) ConjugateGradient(20) (BackwardEuler.h:90-91) with
+   LinearSolver::tolerance 16 eps (LinearSolver.h:26); the Newton-adaptive
+   relative tolerance min(0.5, sqrt(max(|r|, tol))) (NewtonsMethod.h:61).
+   The cg_* fields set whichever solver runs. */
 typedef struct zr_implicit_opts {
   int32_t newton_max_iterations;
   int32_t cg_max_iterations;
@@ -227,7 +235,7 @@ typedef struct zr_implicit_opts {
   int32_t cg_check_interval;     /* CG iterations between host polls of the
                                     device-side termination flag (0: 8) */
   int32_t preconditioner;        /* ZR_PRECOND_MASS or ZR_PRECOND_JACOBI */
-  int32_t reserved;
+  int32_t solver;                /* ZR_SOLVER_CG or ZR_SOLVER_MINRES */
 } zr_implicit_opts;
 
 typedef struct zr_implicit_stats {

@@ -27,7 +27,7 @@ _p = ctypes.c_void_p
 class ZroImplicitOpts(ctypes.Structure):
     _fields_ = [("newton_max_iterations", ctypes.c_int32), ("newton_tolerance", _d),
                 ("cg_max_iterations", ctypes.c_int32), ("cg_tolerance", _d), ("cg_relative_tolerance", _d),
-                ("preconditioner", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("preconditioner", ctypes.c_int32), ("solver", ctypes.c_int32)]
 
 
 class ZroImplicitStats(ctypes.Structure):
@@ -36,11 +36,13 @@ class ZroImplicitStats(ctypes.Structure):
 
 
 def implicit_opts(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=20,
-                  cg_tolerance=16 * 2.220446049250313e-16, cg_relative_tolerance=0.0, preconditioner=0):
+                  cg_tolerance=16 * 2.220446049250313e-16, cg_relative_tolerance=0.0, preconditioner=0, solver=0):
     """Defaults of the reference: NewtonsMethod(1e-5, 3) (MpmSimulationBase.cpp:51),
-    ConjugateGradient(20) with LinearSolver::tolerance 16 eps (LinearSolver.h:26)."""
+    ConjugateGradient(20) with LinearSolver::tolerance 16 eps (LinearSolver.h:26).
+    solver=1 is Minres (the reference's default linear solver, BackwardEuler.h:55-62,
+    with tolerance 1: pass cg_tolerance=1.0 for the reference's setting)."""
     return ZroImplicitOpts(int(newton_max_iterations), float(newton_tolerance), int(cg_max_iterations),
-                           float(cg_tolerance), float(cg_relative_tolerance), int(preconditioner), 0)
+                           float(cg_tolerance), float(cg_relative_tolerance), int(preconditioner), int(solver))
 
 
 class ZroPfOpts(ctypes.Structure):

@@ -171,7 +171,8 @@ typedef struct {
   double cg_relative_tolerance;    /* <= 0: Newton-adaptive min(0.5, sqrt(max(|r|, tol))) */
   int32_t preconditioner;          /* 0: mass-lumped in/m (MpmSimulationBase.cpp:617-623),
                                       1: Jacobi on the operator diagonal (KrylovSolvers.h:32-57) */
-  int32_t reserved;
+  int32_t solver;                  /* 0: ConjugateGradient (switchLinearSolver(CG)),
+                                      1: Minres, the reference's default (BackwardEuler.h:55-62) */
 } zro_implicit_opts;
 
 typedef struct {

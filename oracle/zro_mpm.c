@@ -1229,6 +1229,105 @@ void zro_implicit_multiply(zro_sim* s, const double* x, double* b)
   free(ds);
 }
 
+/* GivensRotation::compute / rowRotation on a 2-vector (Givens.h:56-141) */
+typedef struct { double c, s; } zro_g2;
+static zro_g2 g2_make(void) { zro_g2 g = { 1.0, 0.0 }; return g; }
+static void g2_compute(zro_g2* g, double a, double b)
+{
+  double d = a * a + b * b;
+  g->c = 1.0;
+  g->s = 0.0;
+  double sqrtd = sqrt(d);
+  if (sqrtd) {
+    double t = 1.0 / sqrtd;
+    g->c = a * t;
+    g->s = -b * t;
+  }
+}
+static void g2_rot(const zro_g2* g, double* v)
+{
+  double t1 = v[0], t2 = v[1];
+  v[0] = g->c * t1 - g->s * t2;
+  v[1] = g->s * t1 + g->c * t2;
+}
+
+/* Minres::solve (Minres.h:69-178) on the backward-Euler operator for the
+   Newton step: x starts at zero (computeStep, BackwardEuler.h:202), the
+   residual is projected (BackwardEuler::project) and preconditioned with
+   in / m or the Jacobi inverse diagonal; LinearSolver::dotProduct
+   (LinearSolver.h:34-37).  Returns the iteration count as Minres::solve does. */
+static int minres_solve(zro_sim* s, double* x, const double* b, double rel_tol, const zro_implicit_opts* o,
+                        const double* inv_diag)
+{
+  int64_t nn = s->npages * 32, n3 = 3 * nn;
+  size_t bytes = sizeof(double) * (size_t)n3;
+  double *mk = calloc(n3, 8), *mkm1 = calloc(n3, 8), *mkm2 = calloc(n3, 8), *z = calloc(n3, 8);
+  double *qkm1 = calloc(n3, 8), *qk = calloc(n3, 8), *qkp1 = calloc(n3, 8), *tmp;
+  zro_g2 Gk = g2_make(), Gkm1 = g2_make(), Gkm2 = g2_make();
+  double gamma = 0, delta = 0, epsilon = 0, beta_kp1 = 0, alpha_k = 0, beta_k = 0, tk = 0, lt[2];
+#define PRECOND(dst, src)                                                               \
+  for (int64_t k = 0; k < nn; ++k)                                                      \
+    for (int d = 0; d < 3; ++d)                                                         \
+      dst[3 * k + d] = !node_active(s, k) ? 0.0                                         \
+                     : inv_diag ? src[3 * k + d] * inv_diag[3 * k + d]                  \
+                                : src[3 * k + d] / s->nodes[k].m;
+  for (int64_t k = 0; k < n3; ++k) x[k] = 0.0;
+  for (int64_t k = 0; k < n3; ++k) qkp1[k] = b[k];
+  project_vec(s, qkp1);
+  PRECOND(z, qkp1);
+  double res = sqrt(dot_active(s, z, qkp1));
+  beta_kp1 = res;
+  double local_tol = rel_tol * res < o->cg_tolerance ? rel_tol * res : o->cg_tolerance;
+  int it = 0;
+  if (res < local_tol) goto done;
+  if (res > 0)
+    for (int64_t k = 0; k < n3; ++k) { qkp1[k] /= beta_kp1; z[k] /= beta_kp1; }
+  lt[0] = res;
+  lt[1] = 0.0;
+  for (it = 0; it < o->cg_max_iterations; ++it) {
+    if (res < local_tol) break;
+    tmp = mkm2; mkm2 = mkm1; mkm1 = mk; mk = tmp;
+    memcpy(mk, z, bytes);
+    beta_k = beta_kp1;
+    tmp = qkm1; qkm1 = qk; qk = qkp1; qkp1 = tmp;
+    zro_implicit_multiply(s, mk, qkp1);
+    project_vec(s, qkp1);
+    alpha_k = dot_active(s, mk, qkp1);
+    for (int64_t k = 0; k < n3; ++k) qkp1[k] -= alpha_k * qk[k];
+    for (int64_t k = 0; k < n3; ++k) qkp1[k] -= beta_k * qkm1[k];
+    PRECOND(z, qkp1);
+    double bb = dot_active(s, z, qkp1);
+    beta_kp1 = sqrt(bb > 0.0 ? bb : 0.0);
+    if (beta_kp1 > 0)
+      for (int64_t k = 0; k < n3; ++k) { qkp1[k] /= beta_kp1; z[k] /= beta_kp1; }
+    /* applyAllPreviousGivensRotationsAndDetermineNewGivens (Minres.h:151-178) */
+    Gkm2 = Gkm1;
+    Gkm1 = Gk;
+    double ek[2] = { 0.0, beta_k };
+    g2_rot(&Gkm2, ek);
+    epsilon = ek[0];
+    double dk[2] = { ek[1], alpha_k };
+    g2_rot(&Gkm1, dk);
+    delta = dk[0];
+    double tv[2] = { dk[1], beta_kp1 };
+    g2_compute(&Gk, tv[0], tv[1]);
+    g2_rot(&Gk, tv);
+    gamma = tv[0];
+    g2_rot(&Gk, lt);
+    tk = lt[0];
+    double r = lt[1];
+    lt[0] = r;
+    lt[1] = 0.0;
+    res = r < 0 ? -r : r;
+    for (int64_t k = 0; k < n3; ++k) mk[k] = (mk[k] - delta * mkm1[k] - epsilon * mkm2[k]) / gamma;
+    for (int64_t k = 0; k < n3; ++k) x[k] += tk * mk[k];
+  }
+done:
+#undef PRECOND
+  free(mk); free(mkm1); free(mkm2); free(z); free(qkm1); free(qk); free(qkp1);
+  return it;
+}
+
 /* ConjugateGradient::solve (ConjugateGradient.h:99-147), mass-lumped
    preconditioner out = in / m (MpmSimulationBase.cpp:617-623) */
 static int cg_solve(zro_sim* s, double* x, const double* b, double rel_tol, const zro_implicit_opts* o,
@@ -1339,7 +1438,8 @@ int zro_backward_euler(zro_sim* s, double dt, const zro_implicit_opts* o, zro_im
         }
       }
     }
-    local.cg_iterations += cg_solve(s, step, r, rel, o, inv_diag);
+    local.cg_iterations += o->solver == 1 ? minres_solve(s, step, r, rel, o, inv_diag)
+                                          : cg_solve(s, step, r, rel, o, inv_diag);
     free(inv_diag);
     for (int64_t k = 0; k < 3 * nn; ++k) s->dv[k] += step[k];
     local.newton_iterations++;
@@ -2428,28 +2528,6 @@ static double admm_dot(const zro_sim* s, const double* a, const double* b)
   for (int64_t k = 0; k < s->npages * 32; ++k)
     if (s->nodes[k].idx >= 0) acc += a[3 * k] * b[3 * k] + a[3 * k + 1] * b[3 * k + 1] + a[3 * k + 2] * b[3 * k + 2];
   return acc;
-}
-
-/* GivensRotation::compute / rowRotation on a 2-vector (Givens.h:56-141) */
-typedef struct { double c, s; } zro_g2;
-static zro_g2 g2_make(void) { zro_g2 g = { 1.0, 0.0 }; return g; }
-static void g2_compute(zro_g2* g, double a, double b)
-{
-  double d = a * a + b * b;
-  g->c = 1.0;
-  g->s = 0.0;
-  double sqrtd = sqrt(d);
-  if (sqrtd) {
-    double t = 1.0 / sqrtd;
-    g->c = a * t;
-    g->s = -b * t;
-  }
-}
-static void g2_rot(const zro_g2* g, double* v)
-{
-  double t1 = v[0], t2 = v[1];
-  v[0] = g->c * t1 - g->s * t2;
-  v[1] = g->s * t1 + g->c * t2;
 }
 
 /* Minres::solve (Minres.h:69-178) with the preconditioner in / diag */

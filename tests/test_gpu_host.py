@@ -93,3 +93,53 @@ def test_host_reports_domain_error(tmp_path):
                        text=True, timeout=120)
     assert r.returncode == 1
     assert "status -3" in r.stderr
+
+
+def test_simulate_with_automatic_restart_in_both_hosts(tmp_path):
+    """SimulationBase::simulate with MpmSimulationBase's automatic restart
+    (SimulationBase.h:325-355, MpmSimulationBase.cpp:983-1011) in the C++
+    host and in the Python frame loop: at cfl 1.5 the CFL time step moves
+    particles 1.5 cells, the step flags it and the frame is re-read from its
+    restart file with max_dt halved until it passes.  Both hosts take the
+    same restarts, write byte-identical restart files (the DataManager
+    framing) and end in the same bits."""
+    from ziran2019_amd.engine import MpmEngine, TimeStepping
+    from ziran2019_amd.simulation import Simulation
+    s = perturbed(21, n_cells=4, shuffle=False)
+    y0 = float(s["X"][:, 1].min() - 2.5 * s["dx"])
+    frames, frame_dt, cfl = 3, 0.1, 1.5
+    exe = zbuild.HOST_EXE
+    src, dst = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    cdir, pdir = tmp_path / "cpp", tmp_path / "py"
+    cdir.mkdir()
+    pdir.mkdir()
+    _write(src, s)
+    cmd = [exe, "simulate", src, dst, repr(s["dx"]), str(frames), repr(frame_dt), repr(frame_dt), repr(cfl),
+           str(cdir), repr(y0)]
+    print("host simulate:", " ".join(cmd[4:]), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    cpp_events = []
+    for line in r.stdout.splitlines():
+        w = line.split()
+        if w[0] == "frame":
+            cpp_events.append(("frame", int(w[1]), int(w[3]), float(w[5])))
+        elif w[0] == "restart":
+            cpp_events.append(("restart", int(w[3]), int(w[6])))
+    print("C++ host:", cpp_events, flush=True)
+    assert any(e[0] == "restart" for e in cpp_events)
+    eng = MpmEngine(dx=s["dx"], gravity=(0.0, -9.8, 0.0), cfl=cfl)
+    eng.upload(s)
+    eng.set_colliders([dict(shape=0, type=1, origin=(0.0, y0, 0.0), normal=(0.0, 1.0, 0.0))])
+    sim = Simulation(eng, TimeStepping(frame_dt=frame_dt, max_dt=frame_dt), output_dir=str(pdir), end_frame=frames)
+    sim.simulate()
+    print("Python:", sim.events, flush=True)
+    assert sim.events == cpp_events
+    out = eng.download()
+    host = _read(dst)
+    for k in ("X", "V", "C", "F"):
+        assert np.array_equal(out[k], host[k]), k
+    for f in range(frames + 1):
+        a, b = (cdir / f"restart_{f}.dat").read_bytes(), (pdir / f"restart_{f}.dat").read_bytes()
+        assert a == b, f
+    eng.close()

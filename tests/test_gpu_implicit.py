@@ -246,3 +246,53 @@ def test_fp32_refined_step_dense_page():
     disp = np.abs(X - s["X"]).max()
     assert np.abs(out["X"] - X).max() / disp < 1e-5
     assert rel_err(out["V"], V) < 1e-5
+
+
+@pytest.mark.parametrize("pre", [0, 1], ids=["mass", "jacobi"])
+def test_minres_steps_parity_reference_defaults(pre):
+    """The reference's default implicit solve: Newton(1e-5, 3) with Minres(20)
+    at tolerance 1 (BackwardEuler.h:55-62), both preconditioners, FP64:
+    the oracle's Newton and MINRES iteration counts and states; the
+    FP32-bulk form stays within 1e-5 of the FP64 oracle."""
+    from ziran2019_amd.engine import minres_opts
+    s = perturbed(17, n_cells=5, shuffle=False, F_noise=0.03, C_noise=0.1, E=2e4)
+    s["V"][:, 1] -= 0.5
+    cols = _floor(s, depth=0.5)
+    eng, ora = make_pair(s, colliders=cols)
+    e32, _ = make_pair(s, colliders=cols)
+    oo = oracle.implicit_opts(cg_max_iterations=20, cg_tolerance=1.0, solver=1, preconditioner=pre)
+    for step in range(3):
+        st_g = eng.advanceOneTimeStepImplicit(DT, minres_opts(preconditioner=pre))
+        st_32 = e32.advanceOneTimeStepImplicit(DT, minres_opts(preconditioner=pre, precision=1))
+        flags, st_o = ora.step_implicit(DT, oo)
+        print("step", step, "newton", st_g.newton_iterations, st_o.newton_iterations, "minres", st_g.cg_iterations,
+              st_o.cg_iterations, "fp32", st_32.cg_iterations, flush=True)
+        assert st_g.newton_iterations == st_o.newton_iterations
+        assert st_g.cg_iterations == st_o.cg_iterations
+        for k in range(st_o.num_residuals):
+            assert abs(st_g.residual_norm[k] - st_o.residual_norm[k]) <= 1e-8 * st_o.residual_norm[0]
+    out, o32 = eng.download(), e32.download()
+    X, V, C, F = ora.get_particles()
+    assert rel_err(out["X"], X) < 1e-11
+    assert rel_err(out["V"], V) < 1e-8
+    assert rel_err(out["F"], F) < 1e-10
+    assert np.abs(o32["X"] - X).max() < 1e-5 * np.abs(X - s["X"]).max()
+    eng.close()
+    e32.close()
+
+
+def test_minres_tight_newton_converges_like_oracle():
+    s = perturbed(18, n_cells=5, shuffle=False, F_noise=0.05, C_noise=0.2, E=2e4)
+    kw = dict(newton_max_iterations=6, newton_tolerance=1e-11, cg_max_iterations=400, cg_tolerance=1e30,
+              cg_relative_tolerance=1e-10, solver=1)
+    eng, ora = make_pair(s)
+    st_g = eng.advanceOneTimeStepImplicit(DT, implicit_opts(**kw))
+    flags, st_o = ora.step_implicit(DT, oracle.implicit_opts(**kw))
+    assert st_g.converged == 1 and st_o.converged == 1
+    assert st_g.newton_iterations == st_o.newton_iterations
+    assert st_g.cg_iterations == st_o.cg_iterations
+    out = eng.download()
+    X, V, C, F = ora.get_particles()
+    assert rel_err(out["X"], X) < 1e-11
+    assert rel_err(out["V"], V) < 1e-8
+    eng.close()

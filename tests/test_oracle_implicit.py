@@ -157,3 +157,59 @@ def test_jacobi_and_mass_preconditioners_reach_the_same_step():
         its.append(st.cg_iterations)
         dvs.append(sim.implicit_state()["dv"])
     assert np.abs(dvs[0] - dvs[1]).max() < 1e-9 * np.abs(dvs[0]).max()
+
+
+def _implicit_sim(seed, ncells=4, E=2e4):
+    s = scenes.cube_scene_small(seed=seed, ncells=ncells, E=E)
+    n = len(s["mass"])
+    rng = np.random.default_rng(seed)
+    F = np.tile(np.eye(3), (n, 1, 1)) + 0.05 * rng.normal(size=(n, 3, 3))
+    sim = oracle.OracleSim(dx=s["dx"], gravity=(0, -9.8, 0), nthreads=2)
+    sim.set_particles(s["X"], s["V"], s["C"], np.ascontiguousarray(F.transpose(0, 2, 1).reshape(n, 9)),
+                      s["mass"], s["vol"], s["mu"], s["lam"])
+    sim.sort_and_pollute()
+    sim.p2g_apic()
+    return sim
+
+
+@pytest.mark.parametrize("pre", [0, 1], ids=["mass", "jacobi"])
+def test_minres_newton_reaches_the_cg_step(pre):
+    """Minres (the reference's default linear solver, BackwardEuler.h:55-62)
+    and ConjugateGradient drive Newton to the same step."""
+    kw = dict(newton_max_iterations=6, newton_tolerance=1e-11, cg_max_iterations=600, cg_tolerance=1e30,
+              cg_relative_tolerance=1e-11, preconditioner=pre)
+    dvs = []
+    for solver in (0, 1):
+        sim = _implicit_sim(11)
+        st = sim.backward_euler(2e-3, oracle.implicit_opts(solver=solver, **kw))
+        assert st.converged == 1
+        dvs.append(sim.implicit_state()["dv"])
+    assert np.abs(dvs[0] - dvs[1]).max() < 1e-9 * np.abs(dvs[0]).max()
+
+
+def test_minres_minimises_the_preconditioned_residual():
+    """After k iterations from x = 0, Minres's step minimises |r0 - A x| in
+    the norm of the mass preconditioner over the Krylov space, so that norm
+    never grows with k and is never above CG's after as many iterations."""
+    base = _implicit_sim(12)
+    base.backward_euler(2e-3, oracle.implicit_opts(newton_max_iterations=0))
+    dv0 = base.implicit_state()["dv"]
+    r0 = base.implicit_residual(dv0)
+    m = base.grid()["m"]
+    active = base.grid()["idx"] >= 0
+
+    def res_after(solver, k):
+        sim = _implicit_sim(12)
+        sim.backward_euler(2e-3, oracle.implicit_opts(newton_max_iterations=1, cg_max_iterations=k,
+                                                      cg_tolerance=0.0, cg_relative_tolerance=1e-300,
+                                                      solver=solver))
+        x = sim.implicit_state()["dv"] - dv0
+        rr = (r0 - base.implicit_multiply(x))[active]
+        return np.sqrt(np.sum(rr * rr / m[active, None]))
+    mr = [res_after(1, k) for k in (1, 2, 4, 8, 16, 32)]
+    cg = [res_after(0, k) for k in (1, 2, 4, 8, 16, 32)]
+    floor = 1e-12 * mr[0]  # both reach rounding level by 32 iterations
+    assert all(b <= a * (1 + 1e-12) + floor for a, b in zip(mr, mr[1:]))
+    assert all(a <= b * (1 + 1e-9) + floor for a, b in zip(mr, cg))
+    assert all(a < b for a, b in zip(mr[:4], cg[:4]))
+    assert mr[-1] < 1e-12 * mr[0]

@@ -13,22 +13,6 @@
 // passes use the group's LDS tile of a node vector; scatters use per-wave
 // LDS tiles staged per group and summed per node in a fixed order.
 
-constexpr int kAdmmScalarN = 16;
-enum {
-  AD_RES = 0,      // MINRES residual (preconditioned norm)
-  AD_LTOL = 1,     // local tolerance
-  AD_ALPHA = 2,
-  AD_BETA_K = 3,
-  AD_BETA_KP1 = 4,
-  AD_GAMMA = 5,
-  AD_DELTA = 6,
-  AD_EPS = 7,
-  AD_TK = 8,
-  AD_LT0 = 9,      // last two Givens-transformed rhs components
-  AD_LT1 = 10,
-  AD_GK_C = 11, AD_GK_S = 12, AD_GKM1_C = 13, AD_GKM1_S = 14,
-};
-enum { AI_DONE = 0, AI_ITERS = 1, AI_N = 4 };
 
 struct AdmmParticles {
   double* F;       // [9][n] ADMM F
@@ -873,90 +857,6 @@ __global__ __launch_bounds__(256) void k_minres_scale(uint32_t nn, const double*
   }
 }
 
-__device__ __forceinline__ void g2_rot(double c, double s, double v[2])
-{
-  const double t1 = v[0], t2 = v[1];
-  v[0] = c * t1 - s * t2;
-  v[1] = s * t1 + c * t2;
-}
-
-enum { MR_INIT = 0, MR_ALPHA = 1, MR_GIVENS = 2 };
-
-// the MINRES scalar logic (Minres.h:75-178) in one thread after an ordered sum
-__global__ __launch_bounds__(256) void k_minres_reduce(const double* __restrict__ partial, uint32_t nb, double* sc,
-                                                       int32_t* ai, int mode, double tol, double rel, int it,
-                                                       int max_it)
-{
-  __shared__ double smem[4];
-  if (mode != MR_INIT && ai[AI_DONE]) return;
-  const double acc = thread_partials256(partial, nb);
-  const double v = block_sum256(acc, smem);
-  if (threadIdx.x != 0) return;
-  if (mode == MR_INIT) {
-    const double res = sqrt(v);
-    sc[AD_RES] = res;
-    sc[AD_BETA_KP1] = res;
-    const double lt = rel * res < tol ? rel * res : tol;
-    sc[AD_LTOL] = lt;
-    sc[AD_LT0] = res;
-    sc[AD_LT1] = 0.0;
-    sc[AD_GK_C] = 1.0;
-    sc[AD_GK_S] = 0.0;
-    sc[AD_GKM1_C] = 1.0;
-    sc[AD_GKM1_S] = 0.0;
-    ai[AI_ITERS] = 0;
-    ai[AI_DONE] = res < lt || max_it <= 0 ? 1 : 0;
-    ai[AI_DONE + 2] = ai[AI_DONE];
-  } else if (mode == MR_ALPHA) {
-    sc[AD_ALPHA] = v;
-  } else {
-    const double bb = v > 0.0 ? v : 0.0;
-    const double beta_kp1 = sqrt(bb);
-    sc[AD_BETA_KP1] = beta_kp1;
-    // applyAllPreviousGivensRotationsAndDetermineNewGivens
-    const double gkm2_c = sc[AD_GKM1_C], gkm2_s = sc[AD_GKM1_S];
-    const double gkm1_c = sc[AD_GK_C], gkm1_s = sc[AD_GK_S];
-    double ek[2] = { 0.0, sc[AD_BETA_K] };
-    g2_rot(gkm2_c, gkm2_s, ek);
-    double dk[2] = { ek[1], sc[AD_ALPHA] };
-    g2_rot(gkm1_c, gkm1_s, dk);
-    double tv[2] = { dk[1], beta_kp1 };
-    double c = 1.0, s = 0.0;  // GivensRotation::compute (Givens.h:56-62)
-    const double d = sqrt(tv[0] * tv[0] + tv[1] * tv[1]);
-    if (d) {
-      const double t = 1.0 / d;
-      c = tv[0] * t;
-      s = -tv[1] * t;
-    }
-    g2_rot(c, s, tv);
-    double lt[2] = { sc[AD_LT0], sc[AD_LT1] };
-    g2_rot(c, s, lt);
-    sc[AD_EPS] = ek[0];
-    sc[AD_DELTA] = dk[0];
-    sc[AD_GAMMA] = tv[0];
-    sc[AD_TK] = lt[0];
-    sc[AD_LT0] = lt[1];
-    sc[AD_LT1] = 0.0;
-    sc[AD_GKM1_C] = gkm1_c;
-    sc[AD_GKM1_S] = gkm1_s;
-    sc[AD_GK_C] = c;
-    sc[AD_GK_S] = s;
-    const double res = lt[1] < 0.0 ? -lt[1] : lt[1];
-    sc[AD_RES] = res;
-    ai[AI_ITERS] = it + 1;
-    if (res < sc[AD_LTOL] || it + 1 >= max_it) ai[AI_DONE] = 1;
-  }
-}
-
-// the start of each MINRES iteration: beta_k = beta_{k+1}; the update of this
-// iteration still runs when the convergence flag is raised by its Givens step
-__global__ void k_minres_begin(double* sc, int32_t* ai)
-{
-  if (threadIdx.x != 0) return;
-  ai[AI_DONE + 2] = ai[AI_DONE];
-  if (ai[AI_DONE]) return;
-  sc[AD_BETA_K] = sc[AD_BETA_KP1];
-}
 
 // uStep (:638-671): y += rho omega (F - Fn - dt grad(vn + dv) Fn)
 __global__ __launch_bounds__(256) void k_admm_ustep(ParticlesView P, AdmmParticles A, AdmmNodes N,

@@ -580,6 +580,7 @@ __global__ __launch_bounds__(256) void k_grid(int mode, uint32_t nslots, const u
 }
 
 #include "zr_implicit.hpp"
+#include "zr_minres.hpp"
 #include "zr_damage.hpp"
 #include "zr_adjoint.hpp"
 #include "zr_admm.hpp"
@@ -726,18 +727,21 @@ __device__ __forceinline__ void speed_block_max(double m[kSpeedFields], double (
   for (int j = 0; j < kSpeedFields; ++j)
     for (int w = 0; w < 4; ++w) speed_max(m[j], smem[j][w]);
 }
+// component d of particle i at i * ps + d * cs: structure of arrays (ps = 1,
+// cs = stride) or the reference's particle-major layout (ps = 3, cs = 1)
 __global__ __launch_bounds__(256) void k_max_speed(const double* __restrict__ x, const double* __restrict__ v,
-                                                   int64_t S, uint32_t n, double* __restrict__ partial)
+                                                   int64_t ps, int64_t cs, uint32_t n, double* __restrict__ partial)
 {
   __shared__ double smem[kSpeedFields][4];
   double m[kSpeedFields];
   m[0] = 0.0;
   for (int j = 1; j < kSpeedFields; ++j) m[j] = -(double)__FLT_MAX__;
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    speed_max(m[0], speed_of(v[i], v[S + i], v[2 * S + i]));
+    const int64_t b = (int64_t)i * ps;
+    speed_max(m[0], speed_of(v[b], v[b + cs], v[b + 2 * cs]));
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double xd = x[d * S + i];
+      const double xd = x[b + d * cs];
       speed_max(m[1 + d], xd);
       speed_max(m[4 + d], -xd);
     }
@@ -1038,6 +1042,10 @@ struct zr_engine {
   // FP32-bulk CG vectors and the operator data in both precisions
   float *fx = nullptr, *frc = nullptr, *fq = nullptr, *fp = nullptr, *fap = nullptr;
   size_t imp32_cap = 0;
+  // the three Krylov vectors MINRES needs beyond CG's, per precision
+  double* mr64 = nullptr;
+  float* mr32 = nullptr;
+  size_t mr64_cap = 0, mr32_cap = 0;
   double *c64 = nullptr;  // U9 Q9 S3 K2 frac3 = 26 rows of n
   float* c32 = nullptr;
   uint32_t* ccell = nullptr;
@@ -1766,6 +1774,8 @@ HvpCache<float> hvp_cache<float>(zr_engine* e)
 
 struct CgVecs64 {
   using T = double;
+  static double*& extra(zr_engine* e) { return e->mr64; }
+  static size_t& extra_cap(zr_engine* e) { return e->mr64_cap; }
   static double* x(zr_engine* e) { return e->ix; }
   static double* r(zr_engine* e) { return e->irc; }
   static double* q(zr_engine* e) { return e->iq; }
@@ -1774,6 +1784,8 @@ struct CgVecs64 {
 };
 struct CgVecs32 {
   using T = float;
+  static float*& extra(zr_engine* e) { return e->mr32; }
+  static size_t& extra_cap(zr_engine* e) { return e->mr32_cap; }
   static float* x(zr_engine* e) { return e->fx; }
   static float* r(zr_engine* e) { return e->frc; }
   static float* q(zr_engine* e) { return e->fq; }
@@ -1931,6 +1943,69 @@ int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
   return ZR_OK;
 }
 
+// Minres::solve (Minres.h:69-178) on A x = r for the Newton step, the
+// reference's default linear solver (BackwardEuler.h:55-62): vectors in the
+// operator's precision, FP64 inner products and scalars on the device (the
+// Givens QR in one thread), the host polling the termination flag every
+// check_interval iterations.  The seven Krylov vectors (q_{k+1}, q_k, q_{k-1},
+// z, m_k, m_{k-1}, m_{k-2}) rotate by pointer; `iters` returns the count.
+template <class V>
+int be_minres(zr_engine* e, double rel, const zr_implicit_opts* o, int& iters)
+{
+  using T = typename V::T;
+  const uint32_t nn = e->S * 32, nb = blocks_for(nn, 256), ncg = nb < kCgBlocks ? nb : kCgBlocks;
+  hipStream_t s = e->stream;
+  if (V::extra_cap(e) < e->grid_cap) {
+    dfree(V::extra(e));
+    V::extra_cap(e) = 0;
+    ZR_HIP(e, hipMalloc(&V::extra(e), 9 * e->grid_cap * sizeof(T)));
+    V::extra_cap(e) = e->grid_cap;
+  }
+  T* ex = V::extra(e);
+  const size_t vs = 3 * e->grid_cap;
+  T *qkp1 = V::r(e), *qk = V::q(e), *qkm1 = V::p(e), *z = V::ap(e);
+  T *mk = ex, *mkm1 = ex + vs, *mkm2 = ex + 2 * vs;
+  const NodeVec<T> pinv = nvT(e, e->jacobi ? reinterpret_cast<T*>(e->pinv) : (T*)nullptr);
+  const int max_it = o->cg_max_iterations;
+  k_imr_init<T><<<nb, 256, 0, s>>>(nn, e->gm, nv(e, e->ir), nvT(e, V::x(e)), nvT(e, qkp1), nvT(e, z), nvT(e, qk),
+                                   nvT(e, mk), nvT(e, mkm1), pinv, e->partial);
+  k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci, MR_INIT, o->cg_tolerance, rel, 0, max_it);
+  k_imr_scale<T><<<nb, 256, 0, s>>>(nn, e->gm, nvT(e, qkp1), nvT(e, z), e->sc, e->ci);
+  ZR_LAUNCH_CHECK(e);
+  const int check = o->cg_check_interval > 0 ? o->cg_check_interval : 8;
+  for (int k = 0; k < max_it; ++k) {
+    k_minres_begin<<<1, 64, 0, s>>>(e->sc, e->ci);
+    T* t = mkm2;  // m_{k-2} <- m_{k-1} <- m_k <- z; z takes the freed vector
+    mkm2 = mkm1;
+    mkm1 = mk;
+    mk = z;
+    z = t;
+    t = qkm1;  // q_{k-1} <- q_k <- q_{k+1}; q_{k+1} takes the freed vector
+    qkm1 = qk;
+    qk = qkp1;
+    qkp1 = t;
+    // q_{k+1} = P A m_k with the partials of alpha = m_k . q_{k+1}
+    int rc = be_apply<T>(e, nullptr, mk, qkp1, 1, e->ci, 0);
+    if (rc) return rc;
+    k_minres_reduce<<<1, 256, 0, s>>>(e->partial, ncg, e->sc, e->ci, MR_ALPHA, 0.0, 0.0, k, max_it);
+    k_imr_lanczos<T><<<nb, 256, 0, s>>>(nn, e->gm, nvT(e, qkp1), nvT(e, qk), nvT(e, qkm1), nvT(e, z), pinv, e->sc,
+                                        e->ci, e->partial);
+    k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nb, e->sc, e->ci, MR_GIVENS, 0.0, 0.0, k, max_it);
+    k_imr_update<T><<<nb, 256, 0, s>>>(nn, e->gm, nvT(e, qkp1), nvT(e, z), nvT(e, mk), nvT(e, mkm1),
+                                       nvT(e, mkm2), nvT(e, V::x(e)), e->sc, e->ci);
+    ZR_LAUNCH_CHECK(e);
+    if ((k + 1) % check == 0 && k + 1 < max_it) {
+      ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      ZR_HIP(e, hipStreamSynchronize(s));
+      if (e->hci[AI_DONE]) break;
+    }
+  }
+  ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  ZR_HIP(e, hipStreamSynchronize(s));
+  iters = e->hci[AI_ITERS];
+  return ZR_OK;
+}
+
 template <class V>
 int be_newton(zr_engine* e, const zr_implicit_opts* o, zr_implicit_stats& local)
 {
@@ -1954,7 +2029,13 @@ int be_newton(zr_engine* e, const zr_implicit_opts* o, zr_implicit_stats& local)
       rc = jacobi_inverse<T>(e);
       if (rc) return rc;
     }
-    rc = be_cg<V>(e, rel, o);
+    if (o->solver == ZR_SOLVER_MINRES) {
+      int its = 0;
+      rc = be_minres<V>(e, rel, o, its);
+      local.cg_iterations += its;
+    } else {
+      rc = be_cg<V>(e, rel, o);
+    }
     if (rc) return rc;
     k_axpy3<T><<<nb, 256, 0, e->stream>>>(nn, nv(e, e->idv), nvT(e, V::x(e)));
     ZR_LAUNCH_CHECK(e);
@@ -2014,7 +2095,7 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
   ZR_LAUNCH_CHECK(e);
   rc = read_scalars(e);
   if (rc) return rc;
-  local.cg_iterations = e->hci[CI_ITERS];
+  if (o->solver != ZR_SOLVER_MINRES) local.cg_iterations = e->hci[CI_ITERS];
   e->updated = true;
   if (st) *st = local;
   return ZR_OK;
@@ -2099,6 +2180,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->jstage); dfree(e->pinv); dfree(e->collided); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
   dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
+  dfree(e->mr64); dfree(e->mr32);
   if (e->hsc) (void)hipHostFree(e->hsc);
   if (e->hci) (void)hipHostFree(e->hci);
   for (auto& v : e->prof_ev)
@@ -2147,7 +2229,7 @@ int zr_eval_max_particle_speed(zr_engine* e, double* out)
   if (nb > kSpeedBlocks) nb = kSpeedBlocks;
   if (nb == 0) nb = 1;
   double* res = e->speed + kSpeedFields * kSpeedBlocks;
-  k_max_speed<<<nb, 256, 0, e->stream>>>(e->x, e->v, e->n, n, e->speed);
+  k_max_speed<<<nb, 256, 0, e->stream>>>(e->x, e->v, 1, e->n, n, e->speed);
   k_max_speed_final<<<1, 256, 0, e->stream>>>(e->speed, nb, res);
   ZR_LAUNCH_CHECK(e);
   ZR_HIP(e, hipMemcpyAsync(out, res, kSpeedFields * sizeof(double), hipMemcpyDeviceToHost, e->stream));
@@ -2758,7 +2840,8 @@ static bool valid_opts(const zr_implicit_opts* o)
 {
   return o && o->newton_max_iterations >= 0 && o->cg_max_iterations >= 0 &&
          (o->precision == ZR_FP64 || o->precision == ZR_FP32_REFINED) &&
-         (o->preconditioner == ZR_PRECOND_MASS || o->preconditioner == ZR_PRECOND_JACOBI);
+         (o->preconditioner == ZR_PRECOND_MASS || o->preconditioner == ZR_PRECOND_JACOBI) &&
+         (o->solver == ZR_SOLVER_CG || o->solver == ZR_SOLVER_MINRES);
 }
 
 static bool implicit_materials_ok(zr_engine* e)

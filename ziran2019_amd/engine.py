@@ -57,7 +57,12 @@ PROF = dict(p2g=0, grid=1, g2p=2, sort=3, state=4, hvp=5)
 
 class RestartException(RuntimeError):
     """A particle travelled more than one grid cell in the step
-    (MpmSimulationBase.cpp:983-1002)."""
+    (MpmSimulationBase.cpp:983-1002); ``frame`` is the frame whose saved
+    state a restart reads (SimulationBase.h:21-29), -1 when none is named."""
+
+    def __init__(self, what, frame=-1):
+        super().__init__(what)
+        self.frame = frame
 
 
 class TimeStepping:
@@ -128,7 +133,7 @@ class ZrImplicitOpts(ctypes.Structure):
                 ("newton_tolerance", ctypes.c_double), ("cg_tolerance", ctypes.c_double),
                 ("cg_relative_tolerance", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("cg_check_interval", ctypes.c_int32), ("preconditioner", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("solver", ctypes.c_int32)]
 
 
 class ZrImplicitStats(ctypes.Structure):
@@ -139,16 +144,26 @@ class ZrImplicitStats(ctypes.Structure):
 
 def implicit_opts(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=20,
                   cg_tolerance=16 * 2.220446049250313e-16, cg_relative_tolerance=0.0, precision=0,
-                  cg_check_interval=8, preconditioner=0):
+                  cg_check_interval=8, preconditioner=0, solver=0):
     """Reference defaults: NewtonsMethod(1e-5, 3) (MpmSimulationBase.cpp:51),
     ConjugateGradient(20) (BackwardEuler.h:90-91), LinearSolver::tolerance
     16 eps (LinearSolver.h:26), Newton-adaptive relative tolerance, the
     mass-lumped preconditioner (MpmSimulationBase.cpp:617-623);
     preconditioner=1 is JacobiPreconditioner (KrylovSolvers.h:32-57) on the
-    operator's diagonal."""
+    operator's diagonal; solver=1 is Minres (Minres.h:69-178), the
+    reference's default linear solver (BackwardEuler.h:55-62, with tolerance
+    1: cg_tolerance=1.0), as ``minres_opts`` sets it up."""
     return ZrImplicitOpts(int(newton_max_iterations), int(cg_max_iterations), float(newton_tolerance),
                           float(cg_tolerance), float(cg_relative_tolerance), int(precision), int(cg_check_interval),
-                          int(preconditioner), 0)
+                          int(preconditioner), int(solver))
+
+
+def minres_opts(**kw):
+    """The reference's default implicit solve: Newton(1e-5, 3) with Minres(20)
+    at tolerance 1 (BackwardEuler.h:55-62, MpmSimulationBase.cpp:51)."""
+    d = dict(cg_max_iterations=20, cg_tolerance=1.0, solver=1)
+    d.update(kw)
+    return implicit_opts(**d)
 
 
 class ZrParticles(ctypes.Structure):
@@ -385,6 +400,8 @@ class MpmEngine:
         p = ZrParticles(n, *[_dptr(keep[k]) for k in ("X", "V", "C", "F", "mass", "vol", "mu", "lam")])
         self._check(self.L.zr_upload_particles(self.h, ctypes.byref(p)), "zr_upload_particles")
         self.n = n
+        # the per-particle constants the steps do not change, for the restart state
+        self.constants = {k: keep[k].copy() for k in ("mass", "vol", "mu", "lam")}
 
     def set_materials(self, model=None, plasticity=None, plastic_param=None, plastic_state=None, degradation=None):
         """Per-particle models (ZR_MODEL_*), plasticity (ZR_PLASTIC_*), its 4
@@ -405,6 +422,9 @@ class MpmEngine:
         mt = ZrMaterials(n, m.ctypes.data_as(_ip) if m is not None else None,
                          p.ctypes.data_as(_ip) if p is not None else None, _dptr(q), _dptr(st), _dptr(g), None)
         self._check(self.L.zr_set_materials(self.h, ctypes.byref(mt)), "set_materials")
+        self._mat = dict(model=np.zeros(n, np.int32) if m is None else m.copy(),
+                         plasticity=np.zeros(n, np.int32) if p is None else p.copy(),
+                         plastic_param=np.zeros((n, 4)) if q is None else q.copy())
 
     def set_phase_field(self, opts, c=None, H=None, l0=None, one_over_sigma_c=None, pf_Fp=None, H_max=None,
                         residual_phase=None, allow_damage=None):
@@ -417,6 +437,12 @@ class MpmEngine:
                                                                      dtype=np.int32)
         pf = ZrPhaseField(n, *[_dptr(a) for a in arrs], al.ctypes.data_as(_ip) if al is not None else None)
         self._check(self.L.zr_set_phase_field(self.h, ctypes.byref(pf), ctypes.byref(opts)), "set_phase_field")
+        # the fields the steps do not change (defaults of addFBasedMpmForceWithPhaseField)
+        dflt = dict(l0=0.0, one_over_sigma_c=0.0, H_max=np.inf, residual_phase=1e-3)
+        self._pf = dict(opts=opts, **{k: (np.full(n, dflt[k]) if a is None else a.copy())
+                                      for k, a in zip(("l0", "one_over_sigma_c", "H_max", "residual_phase"),
+                                                      arrs[2:4] + arrs[5:7])},
+                        allow_damage=np.ones(n, np.int32) if al is None else al.copy())
 
     def get_phase_field(self):
         n = self.n
@@ -494,6 +520,64 @@ class MpmEngine:
         self._check(self.L.zr_download_particles(self.h, ctypes.byref(p)), "zr_download_particles")
         return dict(X=X.reshape(n, 3), V=V.reshape(n, 3), C=C.reshape(n, 9), F=F.reshape(n, 9))
 
+    # ------------------------------------------------------- restart state
+    def write_state(self, f):
+        """MpmSimulationBase::writeState (MpmSimulationBase.cpp:731-745) of
+        every particle attribute the engine holds, downloaded from the
+        device, in DataManager::writeData's framing (ziran2019_amd/state.py);
+        f is a binary stream or a path."""
+        from . import state
+        if isinstance(f, (str, os.PathLike)):
+            with open(f, "wb") as fh:
+                return self.write_state(fh)
+        n, c, out = self.n, self.constants, self.download()
+        arrays = {"m": c["mass"], "X": out["X"], "V": out["V"], "C": out["C"], "F": out["F"],
+                  "element measure": c["vol"], "mu": c["mu"], "lambda": c["lam"]}
+        if getattr(self, "_mat", None) is not None:
+            g = self.get_materials()
+            arrays.update({"material": self._mat["model"] | (self._mat["plasticity"] << 4),
+                           "plastic parameters": self._mat["plastic_param"],
+                           "plastic state": g["plastic_state"], "degradation": g["degradation"]})
+        if getattr(self, "_visco_set", None) is not None:
+            arrays["Fe_Nonequilibrated"] = self.get_viscoelastic()
+        if getattr(self, "_pf", None) is not None:
+            pf = dict(self.get_phase_field(), vol=c["vol"], **{k: v for k, v in self._pf.items() if k != "opts"})
+            arrays["phase field"] = state.pack_phase_field(pf)
+        state.write(f, n, arrays)
+
+    def read_state(self, f):
+        """MpmSimulationBase::readState: the particles of a state written by
+        write_state (or by the C++ host) uploaded to the device, with the
+        materials, the viscoelastic strain and the phase field when the file
+        holds them.  As DataManager::readData needs its arrays created first,
+        a file with the viscoelastic strain needs set_viscoelastic (for the
+        viscosities) and the phase field keeps this engine's solver options."""
+        from . import state
+        if isinstance(f, (str, os.PathLike)):
+            with open(f, "rb") as fh:
+                return self.read_state(fh)
+        n, a = state.read(f)
+        scene = dict(X=a["X"], V=a["V"], C=a["C"], F=a["F"], mass=a["m"], vol=a["element measure"], mu=a["mu"],
+                     lam=a["lambda"])
+        pf_opts = self._pf["opts"] if getattr(self, "_pf", None) is not None else phase_field_opts()
+        visco = getattr(self, "_visco_set", None)
+        self.upload(scene)
+        self._mat = self._pf = self._visco_set = None
+        if "material" in a:
+            self.set_materials(model=a["material"] & 15, plasticity=a["material"] >> 4,
+                               plastic_param=a["plastic parameters"], plastic_state=a["plastic state"],
+                               degradation=a["degradation"])
+        if "Fe_Nonequilibrated" in a:
+            if visco is None:
+                raise ZrError("read_state: the file holds a viscoelastic strain; call set_viscoelastic first")
+            self.set_viscoelastic(a["Fe_Nonequilibrated"], visco[0], visco[1], visco[2], visco[3])
+        if "phase field" in a:
+            pf = state.unpack_phase_field(a["phase field"])
+            self.set_phase_field(pf_opts, c=pf["c"], H=pf["H"], l0=pf["l0"], one_over_sigma_c=pf["one_over_sigma_c"],
+                                 pf_Fp=pf["pf_Fp"], H_max=pf["H_max"], residual_phase=pf["residual_phase"],
+                                 allow_damage=pf["allow_damage"])
+        return n
+
     # ---------------------------------------------------------- step API
     def sortParticlesAndPolluteGrid(self):
         self._check(self.L.zr_sort_particles_and_pollute_grid(self.h, ctypes.byref(self.stats)),
@@ -527,6 +611,7 @@ class MpmEngine:
         f, m, l = (None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in self._visco)
         self._check(self.L.zr_set_viscoelastic(self.h, f, m, l, float(viscosity_d), float(viscosity_v)),
                     "zr_set_viscoelastic")
+        self._visco_set = (self._visco[1].copy(), self._visco[2].copy(), float(viscosity_d), float(viscosity_v))
 
     def get_viscoelastic(self):
         out = np.zeros(9 * self.n)

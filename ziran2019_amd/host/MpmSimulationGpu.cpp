@@ -3,6 +3,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <fstream>
+#include <istream>
+#include <ostream>
 
 namespace ziran_amd {
 
@@ -133,14 +137,41 @@ void MpmSimulationGpu::advanceOneTimeStep(double dt)
   if (!eng_) throw std::runtime_error("initialize() was not called");
   check(zr_set_time(eng_, time));
   step_stats_ = zr_step_stats{};
+  int rc;
   if (symplectic) {
-    check(zr_advance_one_time_step(eng_, dt, &step_stats_));
+    rc = zr_advance_one_time_step(eng_, dt, &step_stats_);
   } else {
     zr_implicit_opts o = implicitOpts();
     implicit_stats_ = zr_implicit_stats{};
-    check(zr_advance_one_time_step_implicit(eng_, dt, &o, &implicit_stats_, &step_stats_));
+    rc = zr_advance_one_time_step_implicit(eng_, dt, &o, &implicit_stats_, &step_stats_);
   }
+  if (rc != ZR_CFL_RESTART) check(rc);
   time += dt;
+  cflPolicy(step_stats_.cfl_flags);
+}
+
+void MpmSimulationGpu::cflPolicy(int32_t flags)
+{
+  const bool faster_than_grid_cell = (flags & 1) != 0, faster_than_half_grid_cell = (flags & 2) != 0;
+  if (faster_than_grid_cell && autorestart) {
+    int frame_to_restart = frame - 1;
+    if (step.max_dt <= step.min_dt) {  // unable to shrink dt further
+      frame_to_restart--;
+      if (frame_to_restart < start_frame)
+        throw std::runtime_error("unstable initial conditions: shrink min_dt or change other parameters");
+    } else {
+      const double new_max_dt = std::max(step.max_dt / 2, step.min_dt);
+      restart_callbacks.emplace_back([new_max_dt, this](int) { step.max_dt = new_max_dt; });
+    }
+    latest_frame_restarted_ = std::max(frame_to_restart, latest_frame_restarted_);
+    throw RestartException("a particle moved more than one cell in the step", frame_to_restart);
+  }
+  if (!faster_than_half_grid_cell && frame > latest_frame_restarted_ + 1) {
+    // the reference restores max_dt towards the original by a factor 1
+    // (MpmSimulationBase.cpp:1004-1011): only a max_dt above the original moves
+    const double new_max_dt = std::min(step.max_dt_original, step.max_dt * (double)1);
+    if (new_max_dt != step.max_dt) step.max_dt = new_max_dt;
+  }
 }
 
 double MpmSimulationGpu::calculateDt()
@@ -215,13 +246,195 @@ void MpmSimulationGpu::backwardEulerStep(double dt)
 
 void MpmSimulationGpu::gridToParticles(double dt)
 {
-  check(symplectic ? zr_grid_to_particles(eng_, dt, &step_stats_) : zr_grid_to_particles_apic(eng_, dt, &step_stats_));
+  const int rc =
+      symplectic ? zr_grid_to_particles(eng_, dt, &step_stats_) : zr_grid_to_particles_apic(eng_, dt, &step_stats_);
+  if (rc != ZR_CFL_RESTART) check(rc);
+  cflPolicy(step_stats_.cfl_flags);
 }
 
 void MpmSimulationGpu::downloadParticles()
 {
   zr_particles z = view();
   check(zr_download_particles(eng_, &z));
+}
+
+
+// ---------------------------------------------------------------------------
+// restart state: Scene::writeState -> DataManager::writeData framing
+// (Scene.h:189-217, DataManager.h:263-293, DataArray.h:100-117,
+// BinaryIO.h:82-101): writeEntry writes trivially copyable values as raw
+// bytes, a string as its uint64 length and characters, a std::vector as its
+// uint64 size, uint64 element size and elements.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <class T>
+void put(std::ostream& out, const T& v)
+{
+  out.write(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+template <class T>
+T get(std::istream& in)
+{
+  T v{};
+  in.read(reinterpret_cast<char*>(&v), sizeof(T));
+  if (!in) throw std::runtime_error("restart state: unexpected end of data");
+  return v;
+}
+
+// one DataArray<T> with the single range [0, count) (DisjointRanges default
+// grain 2^7, DisjointRanges.h:51)
+void put_array(std::ostream& out, const std::string& name, const void* data, size_t elem_bytes, int32_t count)
+{
+  put<uint64_t>(out, name.size());
+  out.write(name.data(), (std::streamsize)name.size());
+  put<int32_t>(out, 7);
+  put<uint64_t>(out, count > 0 ? 1 : 0);
+  put<uint64_t>(out, 2 * sizeof(int32_t));
+  if (count > 0) {
+    put<int32_t>(out, 0);
+    put<int32_t>(out, count);
+  }
+  put<uint64_t>(out, (uint64_t)count);
+  put<uint64_t>(out, (uint64_t)elem_bytes);
+  out.write(reinterpret_cast<const char*>(data), (std::streamsize)(elem_bytes * (size_t)count));
+}
+
+struct ArrayRef {
+  const char* name;
+  void* data;
+  size_t elem_bytes;
+};
+
+}  // namespace
+
+void MpmSimulationGpu::writeState(std::ostream& out)
+{
+  if (eng_) downloadParticles();
+  ParticleArrays& p = particles;
+  const int32_t n = (int32_t)p.count();
+  const ArrayRef arrays[] = { { "m", p.mass.data(), sizeof(double) },
+                              { "X", p.X.data(), 3 * sizeof(double) },
+                              { "V", p.V.data(), 3 * sizeof(double) },
+                              { "C", p.C.data(), 9 * sizeof(double) },
+                              { "F", p.F.data(), 9 * sizeof(double) },
+                              { "element measure", p.vol.data(), sizeof(double) },
+                              { "mu", p.mu.data(), sizeof(double) },
+                              { "lambda", p.lambda.data(), sizeof(double) } };
+  put<int32_t>(out, n);
+  put<uint64_t>(out, sizeof(arrays) / sizeof(arrays[0]));
+  for (const ArrayRef& a : arrays) put_array(out, a.name, a.data, a.elem_bytes, n);
+  // Scene::writeState's mesh index vectors (no meshes): Vector<int,3> and Vector<int,2>
+  put<uint64_t>(out, 0);
+  put<uint64_t>(out, 3 * sizeof(int32_t));
+  put<uint64_t>(out, 0);
+  put<uint64_t>(out, 2 * sizeof(int32_t));
+  if (!out) throw std::runtime_error("restart state: write failed");
+}
+
+void MpmSimulationGpu::readState(std::istream& in)
+{
+  const int32_t n = get<int32_t>(in);
+  if (n < 0) throw std::runtime_error("restart state: negative particle count");
+  ParticleArrays& p = particles;
+  p.resize((size_t)n);
+  const ArrayRef arrays[] = { { "m", p.mass.data(), sizeof(double) },
+                              { "X", p.X.data(), 3 * sizeof(double) },
+                              { "V", p.V.data(), 3 * sizeof(double) },
+                              { "C", p.C.data(), 9 * sizeof(double) },
+                              { "F", p.F.data(), 9 * sizeof(double) },
+                              { "element measure", p.vol.data(), sizeof(double) },
+                              { "mu", p.mu.data(), sizeof(double) },
+                              { "lambda", p.lambda.data(), sizeof(double) } };
+  const uint64_t count = get<uint64_t>(in);
+  std::vector<bool> seen(sizeof(arrays) / sizeof(arrays[0]), false);
+  for (uint64_t a = 0; a < count; ++a) {
+    // DataManager::readData finds each array by its name, in any order
+    const uint64_t len = get<uint64_t>(in);
+    std::string name(len, '\0');
+    in.read(&name[0], (std::streamsize)len);
+    (void)get<int32_t>(in);  // grain
+    const uint64_t nranges = get<uint64_t>(in), rbytes = get<uint64_t>(in);
+    if (rbytes != 2 * sizeof(int32_t)) throw std::runtime_error("restart state: bad range size in " + name);
+    int64_t length = 0;
+    for (uint64_t r = 0; r < nranges; ++r) {
+      const int32_t lo = get<int32_t>(in), hi = get<int32_t>(in);
+      length += hi - lo;
+    }
+    const uint64_t size = get<uint64_t>(in), bytes = get<uint64_t>(in);
+    if ((int64_t)size != length) throw std::runtime_error("restart state: ranges and values differ in " + name);
+    size_t k = 0;
+    while (k < seen.size() && name != arrays[k].name) ++k;
+    if (k == seen.size()) {  // an attribute this host does not hold
+      in.ignore((std::streamsize)(size * bytes));
+      continue;
+    }
+    if (bytes != arrays[k].elem_bytes || (int64_t)size != n)
+      throw std::runtime_error("restart state: array " + name + " does not match the particles");
+    in.read(reinterpret_cast<char*>(arrays[k].data), (std::streamsize)(size * bytes));
+    seen[k] = true;
+  }
+  for (size_t k = 0; k < seen.size(); ++k)
+    if (!seen[k]) throw std::runtime_error(std::string("restart state: no array ") + arrays[k].name);
+  (void)get<uint64_t>(in);  // Scene::readState's mesh index vectors
+  const uint64_t b3 = get<uint64_t>(in);
+  (void)get<uint64_t>(in);
+  const uint64_t b2 = get<uint64_t>(in);
+  if (b3 != 3 * sizeof(int32_t) || b2 != 2 * sizeof(int32_t)) throw std::runtime_error("restart state: bad trailer");
+  if (eng_) reinitialize();
+}
+
+std::string MpmSimulationGpu::outputFileName() const { return "restart_" + std::to_string(frame) + ".dat"; }
+
+void MpmSimulationGpu::write(const std::string& filename)
+{
+  if (!write_files) return;
+  std::ofstream f(output_dir + "/" + filename, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open " + output_dir + "/" + filename);
+  writeState(f);
+  std::ofstream last(output_dir + "/last_written.txt");
+  last << frame;
+}
+
+void MpmSimulationGpu::read(const std::string& filename)
+{
+  std::ifstream f(output_dir + "/" + filename, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open " + output_dir + "/" + filename);
+  readState(f);
+}
+
+void MpmSimulationGpu::restart(int new_restart_frame)
+{
+  restart_frame = new_restart_frame;
+  frame = restart_frame;
+  step.reset(frame);
+  time = step.time;
+  restarting = true;
+}
+
+void MpmSimulationGpu::simulate()
+{
+  if (!restarting) {
+    restart_frame = start_frame;
+    frame = start_frame;
+    write(outputFileName());
+  }
+  for (frame = restart_frame + 1; frame <= end_frame; frame++) {
+    if (restarting) {
+      frame--;  // the state of the last frame
+      read(outputFileName());
+      for (auto& f : restart_callbacks) f(frame);
+      restart_callbacks.clear();
+      restarting = false;
+      frame++;
+    }
+    try {
+      advanceOneFrame();
+      write(outputFileName());
+    } catch (RestartException& e) {
+      restart(e.frame);
+    }
+  }
 }
 
 }  // namespace ziran_amd

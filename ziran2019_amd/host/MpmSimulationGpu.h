@@ -13,6 +13,8 @@
 
 #include <array>
 #include <cstdint>
+#include <functional>
+#include <iosfwd>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -21,9 +23,11 @@
 
 namespace ziran_amd {
 
+// SimulationBase.h:21-29: the frame whose saved state a restart reads
 class RestartException : public std::runtime_error {
 public:
-  explicit RestartException(const std::string& what) : std::runtime_error(what) {}
+  explicit RestartException(const std::string& what, int frame = -1) : std::runtime_error(what), frame(frame) {}
+  int frame;
 };
 
 // The particle attributes the step reads and writes (Particles m/X/V, the
@@ -82,6 +86,14 @@ public:
   int32_t preconditioner = ZR_PRECOND_MASS;  // or ZR_PRECOND_JACOBI (KrylovSolvers.h:32-57)
   double time = 0.0;                       // SimulationBase::step.time
   TimeStepping step;                       // SimulationBase::step
+  // SimulationBase's frame loop (SimulationBase.h:35-78,325-355) and the
+  // CFL restart policy of MpmSimulationBase (MpmSimulationBase.cpp:983-1011)
+  bool autorestart = true;
+  int frame = 0, start_frame = 0, end_frame = 0, restart_frame = 0;
+  bool restarting = false;
+  bool write_files = true;
+  std::string output_dir = ".";
+  std::vector<std::function<void(int)>> restart_callbacks;
 
   ParticleArrays particles;
 
@@ -119,6 +131,29 @@ public:
   // always current; here they are downloaded on demand)
   void downloadParticles();
 
+  // SimulationBase::writeState / readState (SimulationBase.h:130-144) in the
+  // framing of Scene::writeState -> DataManager::writeData
+  // (Scene.h:189-217, DataManager.h:263-293, DataArray.h:100-117): the
+  // particle count, the number of arrays, then per array its name, ranges
+  // and values ("m", "X", "V", "C", "F", "element measure" as the reference
+  // names them; "mu" and "lambda" hold the per-particle model), then the two
+  // empty mesh index vectors.  writeState downloads from the device first;
+  // readState uploads what it read, so the device holds that state again.
+  virtual void writeState(std::ostream& out);
+  virtual void readState(std::istream& in);
+  // SimulationBase::write / read (SimulationBase.h:149-187): the state in
+  // output_dir/filename (and last_written.txt with the frame)
+  virtual void write(const std::string& filename);
+  virtual void read(const std::string& filename);
+  // SimulationBase::outputFileName (SimulationBase.h:192-200): restart_<frame>.dat
+  std::string outputFileName() const;
+  // SimulationBase::restart and simulate (SimulationBase.h:275-355): frames
+  // start_frame + 1 .. end_frame, the state written after each; a
+  // RestartException re-reads the state of the frame it names and runs the
+  // restart callbacks (the halved max_dt) before going on
+  virtual void restart(int new_restart_frame);
+  virtual void simulate();
+
   const zr_implicit_stats& lastImplicitStats() const { return implicit_stats_; }
   const zr_step_stats& lastStepStats() const { return step_stats_; }
   int lastFrameSubsteps() const { return frame_substeps_; }
@@ -130,7 +165,13 @@ private:
   zr_step_stats step_stats_{};
   int frame_substeps_ = 0;
 
+  int latest_frame_restarted_ = 0;
+
   void check(int rc) const;
+  // the restart policy of gridToParticles (MpmSimulationBase.cpp:983-1011)
+  // on the step's CFL flags (bit 0: a particle moved more than a cell,
+  // bit 1: more than half the CFL distance)
+  void cflPolicy(int32_t flags);
   zr_implicit_opts implicitOpts() const;
   zr_particles view();
   void uploadColliders();

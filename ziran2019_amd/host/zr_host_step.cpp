@@ -12,6 +12,15 @@
 // instead (SimulationBase::advanceOneFrame with the CFL time step,
 // max_dt = -dt; cfl 0.6 unless given) and prints one line per frame: frame,
 // substeps, time.  Exit status 0 on success, 3 on a CFL restart, 1 on any other error.
+//
+//   zr_host_step simulate IN OUT dx frames frame_dt max_dt cfl DIR [ground_y]
+//
+// runs SimulationBase::simulate (frames 1..frames of length frame_dt, the
+// CFL time step capped at max_dt, the restart state written to
+// DIR/restart_<frame>.dat after each frame) with MpmSimulationBase's
+// automatic restart: a particle moving more than a cell re-reads the last
+// frame's state with max_dt halved.  Prints one line per frame run (frame,
+// substeps, max_dt) and one per restart; OUT holds the final particles.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -67,8 +76,71 @@ bool write_particles(const char* path, const ParticleArrays& p)
 
 }  // namespace
 
+int simulate_main(int argc, char** argv)
+{
+  if (argc < 10) {
+    std::fprintf(stderr, "usage: %s simulate IN OUT dx frames frame_dt max_dt cfl DIR [ground_y]\n", argv[0]);
+    return 1;
+  }
+  MpmSimulationGpu sim;
+  if (!read_particles(argv[2], sim.particles)) {
+    std::fprintf(stderr, "cannot read %s\n", argv[2]);
+    return 1;
+  }
+  sim.dx = std::atof(argv[4]);
+  sim.end_frame = std::atoi(argv[5]);
+  sim.step.frame_dt = std::atof(argv[6]);
+  sim.step.max_dt = sim.step.max_dt_original = std::atof(argv[7]);
+  sim.cfl = std::atof(argv[8]);
+  sim.output_dir = argv[9];
+  sim.gravity = { { 0.0, -9.8, 0.0 } };
+  if (argc > 10) {
+    ziran_amd::CollisionObject ground;
+    ground.shape = ZR_HALF_SPACE;
+    ground.type = ZR_STICKY;
+    ground.origin = { { 0.0, std::atof(argv[10]), 0.0 } };
+    ground.normal = { { 0.0, 1.0, 0.0 } };
+    sim.addCollisionObject(ground);
+  }
+  try {
+    sim.initialize();
+    // SimulationBase::simulate with a report per frame and per restart
+    sim.restart_frame = sim.frame = sim.start_frame;
+    sim.write(sim.outputFileName());
+    for (sim.frame = sim.restart_frame + 1; sim.frame <= sim.end_frame; sim.frame++) {
+      if (sim.restarting) {
+        sim.frame--;
+        sim.read(sim.outputFileName());
+        for (auto& f : sim.restart_callbacks) f(sim.frame);
+        sim.restart_callbacks.clear();
+        sim.restarting = false;
+        sim.frame++;
+      }
+      try {
+        sim.advanceOneFrame();
+        sim.write(sim.outputFileName());
+        std::printf("frame %d substeps %d max_dt %.17g\n", sim.frame, sim.lastFrameSubsteps(), sim.step.max_dt);
+      } catch (const ziran_amd::RestartException& e) {
+        std::printf("restart from frame %d at frame %d\n", e.frame, sim.frame);
+        sim.restart(e.frame);
+      }
+      std::fflush(stdout);
+    }
+    sim.downloadParticles();
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "error: %s\n", ex.what());
+    return 1;
+  }
+  if (!write_particles(argv[3], sim.particles)) {
+    std::fprintf(stderr, "cannot write %s\n", argv[3]);
+    return 1;
+  }
+  return 0;
+}
+
 int main(int argc, char** argv)
 {
+  if (argc > 1 && std::string(argv[1]) == "simulate") return simulate_main(argc, argv);
   if (argc < 7) {
     std::fprintf(stderr, "usage: %s IN OUT dx steps dt symplectic [ground_y] [precision] [cfl]\n", argv[0]);
     return 1;
