@@ -251,7 +251,9 @@ typedef struct zr_engine zr_engine;
 /* MpmSimulationBase constructor + setDx (MpmSimulationBase.cpp:33-72) */
 int zr_create(const zr_config* cfg, zr_engine** out);
 void zr_destroy(zr_engine* e);
-/* message of the last failed call on this engine */
+/* message of the last failed call on this engine; with NULL, of the last
+   failed call without an engine (zr_eval_max_particle_speed_noalloc) on the
+   calling thread */
 const char* zr_last_error(const zr_engine* e);
 
 /* addCollisionObject (MpmSimulationBase.cpp:84-88); replaces the whole set */
@@ -324,6 +326,20 @@ int zr_eval_max_particle_speed(zr_engine* e, double out[7]);
    CollisionObject.cpp:158-194), or max_dt (the reference's step.max_dt)
    when both are 0.  The caller applies TimeStepping::nextDt. */
 int zr_calculate_dt(zr_engine* e, double max_dt, double* dt);
+
+/* zr_eval_max_particle_speed on caller-owned DEVICE arrays, allocating
+   nothing: no counterpart in the reference; for callers that cannot allocate
+   device memory (code under HIP graph capture, serving loops, harnesses that
+   forbid hipMalloc).  x, v: n particles in the reference layout (3 doubles
+   per particle, particle-major); out: 7 doubles on the device, with the
+   meaning of zr_eval_max_particle_speed's out; stream: a hipStream_t (NULL:
+   the null stream).  Enqueues two kernels and returns; out is ready when the
+   stream reaches them.  The block maxima live in a fixed device array of
+   512 x 7 doubles (28 KiB) per device, held until the process ends and
+   shared by every call on that device: calls that overlap on one device
+   race.  n <= 2^32 - 1 (ZR_ERR_ARG beyond); errors are described by
+   zr_last_error(NULL). */
+int zr_eval_max_particle_speed_noalloc(const double* x, const double* v, int64_t n, double* out, void* stream);
 
 /* particlesToGrid of the implicit path: APIC mass and momentum only
    (particlesToGridHelper<APIC>, MpmSimulationBase.cpp:563-608) + getNumNodes +

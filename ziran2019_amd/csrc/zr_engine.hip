@@ -26,6 +26,7 @@
 #include <string>
 #include <vector>
 #include <array>
+#include <mutex>
 
 #include "../../include/zr_mpm.h"
 #include "zr_math.hpp"
@@ -2106,6 +2107,28 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
 // ===========================================================================
 // C-ABI
 // ===========================================================================
+// evalMaxParticleSpeed without allocation (zr_eval_max_particle_speed_noalloc):
+// the block maxima live in a file-scope device array; each device holds its
+// own copy, whose address is taken once per device (hipGetSymbolAddress, with
+// that device current) and passed to the kernels.  Every block writes its
+// partials before the final pass reads them, so nothing needs clearing;
+// calls that overlap on one device share the array and race.
+namespace {
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) double scratch[kSpeedFields * kSpeedBlocks];
+std::mutex g_scratch_mu;
+double* g_scratch[kNoAllocDevices] = {};
+// message of the last failed call without an engine on this thread (a
+// string literal: nothing is allocated)
+thread_local const char* g_noalloc_err = "";
+
+int noalloc_fail(int code, const char* msg)
+{
+  g_noalloc_err = msg;
+  return code;
+}
+}  // namespace
+
 extern "C" {
 
 int zr_create(const zr_config* cfg, zr_engine** out)
@@ -2190,7 +2213,7 @@ void zr_destroy(zr_engine* e)
   delete e;
 }
 
-const char* zr_last_error(const zr_engine* e) { return e ? e->err.c_str() : "null engine"; }
+const char* zr_last_error(const zr_engine* e) { return e ? e->err.c_str() : g_noalloc_err; }
 
 int zr_set_colliders(zr_engine* e, const zr_collider* c, int32_t n)
 {
@@ -2299,6 +2322,35 @@ int zr_calculate_dt(zr_engine* e, double max_dt, double* dt)
     if (cs > speed) speed = cs;
   }
   *dt = speed != 0.0 ? e->cfg.cfl * e->cfg.dx / speed : max_dt;
+  return ZR_OK;
+}
+
+// evalMaxParticleSpeed on caller-owned device arrays, allocating nothing
+int zr_eval_max_particle_speed_noalloc(const double* x, const double* v, int64_t n, double* out, void* stream)
+{
+  if (!out || n < 0 || (n > 0 && (!x || !v))) return noalloc_fail(ZR_ERR_ARG, "null pointer or negative count");
+  if (n > (int64_t)UINT32_MAX) return noalloc_fail(ZR_ERR_ARG, "more than 2^32 - 1 particles");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return noalloc_fail(ZR_ERR_HIP, "hipGetDevice failed");
+  if (dev < 0 || dev >= kNoAllocDevices) return noalloc_fail(ZR_ERR_ARG, "device index beyond 63");
+  double* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+        return noalloc_fail(ZR_ERR_HIP, "hipGetSymbolAddress of the scratch array failed");
+      g_scratch[dev] = static_cast<double*>(p);
+    }
+    ptr = g_scratch[dev];
+  }
+  uint32_t nb = blocks_for((uint64_t)n, 256);
+  if (nb > kSpeedBlocks) nb = kSpeedBlocks;
+  if (nb == 0) nb = 1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  k_max_speed<<<nb, 256, 0, s>>>(x, v, 3, 1, (uint32_t)n, ptr);
+  k_max_speed_final<<<1, 256, 0, s>>>(ptr, nb, out);
+  if (hipGetLastError() != hipSuccess) return noalloc_fail(ZR_ERR_HIP, "kernel launch failed");
   return ZR_OK;
 }
 

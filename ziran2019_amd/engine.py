@@ -48,6 +48,7 @@ EXPORTS = (
     "zr_phase_field_last_stats", "zr_adjoint_set", "zr_adjoint_get", "zr_adjoint_step", "zr_checkpoint_save",
     "zr_checkpoint_restore", "zr_advance_one_time_step_admm", "zr_get_admm_state",
     "zr_eval_max_particle_speed", "zr_calculate_dt", "zr_set_viscoelastic", "zr_get_viscoelastic",
+    "zr_eval_max_particle_speed_noalloc",
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
 ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER, ZR_PLASTIC_CAM_CLAY = 0, 1, 2, 3
@@ -281,6 +282,7 @@ def load_library(path=LIB_PATH):
     L.zr_grid_to_particles.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
     L.zr_advance_one_time_step.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ZrStepStats)]
     L.zr_eval_max_particle_speed.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.zr_eval_max_particle_speed_noalloc.argtypes = [vp, vp, ctypes.c_int64, vp, vp]
     L.zr_set_viscoelastic.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double]
     L.zr_get_viscoelastic.argtypes = [vp, vp]
     L.zr_calculate_dt.argtypes = [vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
