@@ -737,7 +737,8 @@ __global__ __launch_bounds__(256) void k_max_speed(const double* __restrict__ x,
   double m[kSpeedFields];
   m[0] = 0.0;
   for (int j = 1; j < kSpeedFields; ++j) m[j] = -(double)__FLT_MAX__;
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+  // 64-bit index: with n up to 2^32 - 1 a 32-bit i + stride would wrap
+  for (uint64_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
     const int64_t b = (int64_t)i * ps;
     speed_max(m[0], speed_of(v[b], v[b + cs], v[b + 2 * cs]));
 #pragma unroll
