@@ -29,32 +29,32 @@ F32 = 4
 
 
 def algorithmic_bytes(kernel, n, g):
-    """Required HBM bytes per launch.
+    """Required HBM bytes per launch, each stream counted once (n particles,
+    g active grid nodes with m != 0, T the value size, 4 B sorted index):
 
-    n particles, g active grid nodes (m != 0), FP64 (8 B) values, 4 B index:
-      P2G : reads x3 v3 C9 F9 m1 V0 1 mu1 lambda1 (28 values) + index,
-            writes m, mv3, f3 per node (7 values); the APIC P2G of the
-            implicit path reads 16 values and writes 4 per node
-      G2P : (MLS and APIC alike)
-            reads x3 F9 + index and v*3 per node, writes x3 v3 C9 F9
+      P2G (MLS, stress fused from F): n (26T + 4) + g 7T
+            reads x3 v3 C9 F9 m1 V0 1, writes m, mv3, f3 per node
+      P2G (APIC, implicit path, no stress): n (16T + 4) + g 4T
+            reads x3 v3 C9 m1, writes m, mv3 per node
+      G2P (+F update): n (36T + 4) + g 3T
+            reads x3 F9 and v*3 per node, writes x3 v3 C9 F9
+      operator apply (fixed-corotated): n (34T + 4) + g 7T
+            per particle x3 Fn9 U9 Sigma3 V9 V0 1, per node x3 in, m, b3 out
+      Newton state: n (15 FP64 + 4 + 26T + 4) + g 6 FP64
+            reads x3 Fn9 mu lambda V0 and (vn + dv) per node, writes the
+            operator data (26 values + cell)
     """
     if kernel == "p2g":
-        return n * (28 * F64 + 4) + g * 7 * F64
+        return n * (26 * F64 + 4) + g * 7 * F64
     if kernel == "p2g_apic":
-        # APIC P2G of the implicit path (no stress): reads x3 v3 C9 m1 (16 values) + index,
-        # writes m, mv3 per node
         return n * (16 * F64 + 4) + g * 4 * F64
     if kernel == "g2p":
         return n * (36 * F64 + 4) + g * 3 * F64
     if kernel == "hvp":
-        # FP32 operator data in sorted order: U9 Q9 Sigma3 k2 frac3 (26 values) + cell 4 B,
-        # p3 per node (FP32)
-        return n * (26 * F32 + 4) + g * 3 * F32
+        return n * (34 * F32 + 4) + g * 7 * F32
     if kernel == "hvp64":
-        return n * (26 * F64 + 4) + g * 3 * F64
+        return n * (34 * F64 + 4) + g * 7 * F64
     if kernel == "state":
-        # reads x3 Fn9 mu1 lambda1 V0 1 (15 values) + index and (vn + dv) 6 per node,
-        # writes the FP32 operator data (26 values + cell)
         return n * (15 * F64 + 4 + 26 * F32 + 4) + g * 6 * F64
     raise ValueError(kernel)
 
@@ -75,56 +75,124 @@ IMPLICIT_OPTS = dict(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iter
 # part) or JacobiPreconditioner on the whole operator diagonal; the metric
 # is quoted with the first, the second is measured beside it
 PRECONDITIONERS = dict(jacobi=1, mass=0)
+# what holds each kernel below its HBM roofline, from its rocprofv3 SQ counters
+LIMITER = dict(hvp="latency and VALU/LDS issue (waves wait ~66% of cycles; HBM traffic 0.95x the algorithmic bytes)")
+# steps after the timed region whose kernel launches are timed with HIP events
+SAMPLE_STEPS = 3
 
 
-def cpu_baseline_implicit(scene, dt, threads, opts, nboxes=8):
-    """One implicit oracle step on the particles of the first ``nboxes`` boxes."""
+def host_cpu():
+    """(CPU model from lscpu, online cores, cores this process may use): the
+    threads a baseline uses are the scheduler's share, capped by
+    OMP_NUM_THREADS where the host sets it."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    online = os.cpu_count() or 1
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else online
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return model, online, share
+
+
+def _box_subset(scene, idx):
+    """The particles of boxes ``idx`` of the implicit-boxes scene."""
     import numpy as np
+    cells = np.floor(scene["X"] / scene["dx"]).astype(np.int64)
+    keep = np.zeros(len(cells), dtype=bool)
+    for i in idx:
+        lo, hi = scene["boxes"][i]
+        keep |= np.all((cells >= lo) & (cells < hi), axis=1)
+    n = len(cells)
+    return {k: (np.ascontiguousarray(v[keep]) if isinstance(v, np.ndarray) and len(v) == n else v)
+            for k, v in scene.items()}
+
+
+def _median_implicit_steps(sub, colliders, dt, threads, opts, timed):
+    """One untimed implicit oracle step, then ``timed`` steps: (median
+    seconds per step, Newton and CG iterations of each timed step)."""
+    import statistics
 
     import oracle
-    from ziran2019_amd import scenes
-    dx = scene["dx"]
-    b = scenes.base_nodes(scene["X"], dx)
-    keep = np.zeros(len(b), dtype=bool)
-    for lo, hi in scene["boxes"][:nboxes]:
-        c = np.floor(scene["X"] / dx).astype(np.int64)
-        keep |= np.all((c >= lo) & (c < hi), axis=1)
-    sub = {k: (np.ascontiguousarray(np.asarray(v)[keep]) if isinstance(v, np.ndarray) and len(v) == len(b) else v)
-           for k, v in scene.items()}
-    sim = oracle.OracleSim(dx=dx, gravity=scene["gravity"], nthreads=threads)
+    sim = oracle.OracleSim(dx=sub["dx"], gravity=sub["gravity"], nthreads=threads)
     sim.set_particles(sub["X"], sub["V"], sub["C"], sub["F"], sub["mass"], sub["vol"], sub["mu"], sub["lam"])
-    for c in scene["colliders"]:
+    for c in colliders:
         sim.add_collider(**c)
-    t0 = time.perf_counter()
-    flags, st = sim.step_implicit(dt, oracle.implicit_opts(**opts))
-    el = time.perf_counter() - t0
-    n = int(keep.sum())
-    return dict(value=n / el, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"FP64 C oracle (oracle/zro_mpm.c, OpenMP) implicit step on the {n} particles of the first "
-                       f"{nboxes} of the 64 boxes, same solver settings ({st.newton_iterations} Newton / "
-                       f"{st.cg_iterations} CG iterations), {el:.1f} s")
+    sim.step_implicit(dt, oracle.implicit_opts(**opts))  # first step excluded
+    times, its = [], []
+    for _ in range(timed):
+        t0 = time.perf_counter()
+        _, st = sim.step_implicit(dt, oracle.implicit_opts(**opts))
+        times.append(time.perf_counter() - t0)
+        its.append((st.newton_iterations, st.cg_iterations))
+    return statistics.median(times), its
 
 
-def cpu_baseline(scene, dt, seconds, threads):
+def cpu_baseline_implicit(scene, dt, opts, multi_boxes=(0, 1, 2, 3), single_boxes=(3,), timed=5):
+    """The FP64 C oracle (the reference's loop structure, OpenMP) on the
+    host: the median of ``timed`` implicit steps after one untimed step, on
+    all the cores of this process's share (the particles of the first four
+    boxes) and on one core (the particles of box 3, 3584 particles); the
+    same solver settings as the GPU run.  The per-step work varies with the
+    Newton and CG counts, which are reported beside the times."""
+    model, online, share = host_cpu()
+    multi = _box_subset(scene, multi_boxes)
+    single = _box_subset(scene, single_boxes)
+    tm, its_m = _median_implicit_steps(multi, scene["colliders"], dt, share, opts, timed)
+    ts, its_s = _median_implicit_steps(single, scene["colliders"], dt, 1, opts, timed)
+    nm, ns = len(multi["mass"]), len(single["mass"])
+    return dict(value=nm / tm, unit="particle-steps/s", cores=share, kind="port",
+                sample=f"FP64 C oracle (oracle/zro_mpm.c, OpenMP, {share} threads) implicit steps on the {nm} "
+                       f"particles of boxes {list(multi_boxes)} of the 64, median of {timed} steps after one "
+                       f"untimed step ({tm:.2f} s per step; Newton/CG per step {its_m}), same solver settings",
+                one_core=dict(value=ns / ts, cores=1,
+                              sample=f"the {ns} particles of box {list(single_boxes)}, median of {timed} steps "
+                                     f"after one untimed step ({ts:.2f} s per step; Newton/CG per step {its_s})"),
+                cpu_model=model, online_cores=online)
+
+
+def cpu_baseline(scene, dt, seconds):
+    """The FP64 C oracle on the same explicit workload: one untimed step,
+    then 5 runs of equal step counts sized so that all runs take about
+    ``seconds``; the median run's rate, on the process's core share and on
+    one core."""
+    import statistics
+
     import oracle
-    sim = oracle.OracleSim(dx=scene["dx"], gravity=scene["gravity"], nthreads=threads)
-    sim.set_particles(scene["X"], scene["V"], scene["C"], scene["F"], scene["mass"], scene["vol"], scene["mu"],
-                      scene["lam"])
-    for c in scene["colliders"]:
-        sim.add_collider(**c)
-    sim.step(dt)  # first step excluded
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        sim.step(dt)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or steps >= 200:
-            break
+    model, online, share = host_cpu()
     n = len(scene["mass"])
-    return dict(value=n * steps / el, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"FP64 C oracle (oracle/zro_mpm.c, OpenMP) on the same {n}-particle workload, "
-                       f"{steps} steps after one untimed step, {el:.1f} s")
+    out = {}
+    for threads in (share, 1):
+        sim = oracle.OracleSim(dx=scene["dx"], gravity=scene["gravity"], nthreads=threads)
+        sim.set_particles(scene["X"], scene["V"], scene["C"], scene["F"], scene["mass"], scene["vol"], scene["mu"],
+                          scene["lam"])
+        for c in scene["colliders"]:
+            sim.add_collider(**c)
+        t0 = time.perf_counter()
+        sim.step(dt)  # first step excluded
+        first = time.perf_counter() - t0
+        k = max(1, int(seconds / 2 / 5 / max(first, 1e-3)))
+        runs = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            for _ in range(k):
+                sim.step(dt)
+            runs.append(time.perf_counter() - t0)
+        out[threads] = (n * k / statistics.median(runs), k)
+    (v_all, k_all), (v_one, k_one) = out[share], out[1]
+    return dict(value=v_all, unit="particle-steps/s", cores=share, kind="port",
+                sample=f"FP64 C oracle (oracle/zro_mpm.c, OpenMP) on the same {n}-particle workload, median of 5 "
+                       f"runs of {k_all} steps after one untimed step",
+                one_core=dict(value=v_one, cores=1, sample=f"median of 5 runs of {k_one} steps, one thread"),
+                cpu_model=model, online_cores=online)
 
 
 def run_explicit(args):
@@ -140,11 +208,14 @@ def run_explicit(args):
     eng.set_colliders(scene["colliders"])
     for _ in range(args.warmup):
         eng.advanceOneTimeStep(dt)
-    eng.profile(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         eng.advanceOneTimeStep(dt)
     t1 = time.perf_counter()
+    # per-kernel durations from a separate pass with event markers
+    eng.profile(True)
+    for _ in range(50):
+        eng.advanceOneTimeStep(dt)
     kt = {k: eng.kernel_time(k) for k in ("p2g", "grid", "g2p", "sort")}
     eng.profile(False)
     eng.sortParticlesAndPolluteGrid()
@@ -153,22 +224,22 @@ def run_explicit(args):
     eng.close()
     workload = ("explicit MLS-MPM, fixed-corotated cube 26^3 cells x 8 ppc, dx=1/128, dt=1e-4, sticky ground, "
                 "FP64 plain path")
-    base = None if args.no_cpu_baseline else cpu_baseline(scene, dt, args.cpu_seconds, min(16, os.cpu_count() or 1))
+    base = None if args.no_cpu_baseline else cpu_baseline(scene, dt, args.cpu_seconds)
     return dict(n=n, dt=dt, dx=scene["dx"], elapsed=t1 - t0, kt=kt, g_nodes=g_nodes, roof_kernels=("p2g", "g2p"),
                 bw_kernels=dict(p2g="p2g", g2p="g2p"), workload=workload, base=base, extra={}, dtype="f64")
 
 
-def _implicit_run(scene, opts, steps, warmup, profile):
+def _implicit_run(scene, opts, steps, warmup, sample_steps):
+    """warmup steps, the timed steps (no event markers), then
+    ``sample_steps`` more steps with a HIP event pair around every kernel
+    launch for the per-kernel durations (a separate pass, so that the
+    markers stay out of the timed region)."""
     from ziran2019_amd.engine import MpmEngine
     eng = MpmEngine(dx=scene["dx"], gravity=scene["gravity"])
     eng.upload(scene)
     eng.set_colliders(scene["colliders"])
     for _ in range(warmup):
         eng.advanceOneTimeStepImplicit(2e-3, opts)
-    if profile:
-        # every 8th launch of each kernel class is timed so that the events
-        # do not perturb the timed step (one launch in 8 gets a marker pair)
-        eng.profile(True, every=8)
     newton = cg = 0
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -176,8 +247,13 @@ def _implicit_run(scene, opts, steps, warmup, profile):
         newton += st.newton_iterations
         cg += st.cg_iterations
     t1 = time.perf_counter()
-    kt = {k: eng.kernel_time(k) for k in ("p2g", "grid", "g2p", "sort", "state", "hvp")} if profile else None
-    eng.profile(False)
+    kt = None
+    if sample_steps:
+        eng.profile(True, every=1)
+        for _ in range(sample_steps):
+            eng.advanceOneTimeStepImplicit(2e-3, opts)
+        kt = {k: eng.kernel_time(k) for k in ("p2g", "grid", "g2p", "sort", "state", "hvp")}
+        eng.profile(False)
     final = eng.download()
     return eng, dict(elapsed=t1 - t0, kt=kt, newton=newton / steps, cg=cg / steps, final=final)
 
@@ -192,9 +268,9 @@ def run_implicit(args):
     n = len(scene["mass"])
     # plain FP64 path, then the FP32-bulk / FP64-refined form from the same state
     opts = dict(IMPLICIT_OPTS, preconditioner=PRECONDITIONERS[args.preconditioner])
-    eng64, r64 = _implicit_run(scene, implicit_opts(**opts), args.steps, args.warmup, profile=True)
+    eng64, r64 = _implicit_run(scene, implicit_opts(**opts), args.steps, args.warmup, SAMPLE_STEPS)
     eng64.close()
-    eng, r32 = _implicit_run(scene, implicit_opts(precision=1, **opts), args.steps, args.warmup, profile=True)
+    eng, r32 = _implicit_run(scene, implicit_opts(precision=1, **opts), args.steps, args.warmup, SAMPLE_STEPS)
     eng.sortParticlesAndPolluteGrid()
     eng.particlesToGridApic()
     g_nodes = int(np.count_nonzero(eng.dump("grid_m")))
@@ -211,7 +287,7 @@ def run_implicit(args):
                 f"at 8 ppc, dx=1/256, dt=2e-3, Newton tol 1e-5 (<=3 its) + {args.preconditioner}-preconditioned CG "
                 "rel tol 1e-6; "
                 "FP32 operator and Krylov vectors inside the FP64 Newton residual")
-    base = None if args.no_cpu_baseline else cpu_baseline_implicit(scene, dt, min(16, os.cpu_count() or 1), opts)
+    base = None if args.no_cpu_baseline else cpu_baseline_implicit(scene, dt, opts)
     fp64_value = n * args.steps / r64["elapsed"]
     extra = dict(newton_iterations_per_step=r32["newton"], cg_iterations_per_step=r32["cg"],
                  fp64_plain=dict(value=fp64_value, ms_per_step=r64["elapsed"] / args.steps * 1e3,
@@ -223,9 +299,9 @@ def run_implicit(args):
     if args.preconditioner == "mass":
         # the same steps with JacobiPreconditioner on the operator diagonal
         jopts = dict(IMPLICIT_OPTS, preconditioner=PRECONDITIONERS["jacobi"])
-        e, j64 = _implicit_run(scene, implicit_opts(**jopts), args.steps, args.warmup, profile=False)
+        e, j64 = _implicit_run(scene, implicit_opts(**jopts), args.steps, args.warmup, 0)
         e.close()
-        e, j32 = _implicit_run(scene, implicit_opts(precision=1, **jopts), args.steps, args.warmup, profile=False)
+        e, j32 = _implicit_run(scene, implicit_opts(precision=1, **jopts), args.steps, args.warmup, 0)
         e.close()
         extra["jacobi_operator_diagonal"] = dict(
             value=n * args.steps / j32["elapsed"], ms_per_step=j32["elapsed"] / args.steps * 1e3,
@@ -409,7 +485,7 @@ def main():
         b = algorithmic_bytes(kind, n, g_nodes)
         kern[k] = dict(avg_us=avg_s * 1e6, bytes=b, gbs=b / avg_s / 1e9)
     dom = max(r["roof_kernels"], key=lambda k: kern[k]["avg_us"] * kt[k][1])
-    roof = dict(bound="hbm", kernel=dom, achieved=round(kern[dom]["gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
+    roof = dict(bound="hbm", limiter=LIMITER.get(dom), kernel=dom, achieved=round(kern[dom]["gbs"], 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(kern[dom]["gbs"] / HBM_PEAK_GBS, 4), traffic=TRAFFIC.get((args.workload, dom)),
                 bytes_per_launch=kern[dom]["bytes"], avg_launch_us=round(kern[dom]["avg_us"], 2))
     kernels = {k: dict(avg_us=round(v[0] * 1e3 / max(v[1], 1), 2), launches=v[1], total_ms=round(v[0], 3))
