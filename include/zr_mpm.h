@@ -75,7 +75,9 @@ typedef struct zr_config {
   double dx;            /* MpmSimulationBase::dx */
   double gravity[3];    /* MpmSimulationBase::gravity */
   double cfl;           /* MpmSimulationBase::cfl (used by the CFL flags) */
-  int32_t grid_log2;    /* node coordinates live in [0, 2^grid_log2); 6..10 */
+  int32_t grid_log2;    /* node coordinates live in [0, 2^grid_log2); 6..12 (12: the
+                           reference's 4096^3 SPGrid extent; the dense page tables
+                           take 4 x 2^(3 grid_log2 - 3) bytes, 32 GiB at 12) */
   int32_t precision;    /* ZR_FP64 */
   int32_t transfer;     /* ZR_TRANSFER_MLS (mls_mpm = true) */
   int32_t reserved;
@@ -166,8 +168,20 @@ typedef struct zr_phase_field {
 
 /* FractureSimulation options (FractureSimulation.h:50-73,439;
    fracture/ConjugateGradient.h:64): parabolic_M, CG(100), tolerance 1e-6,
-   relative 1e-12 of the initial residual; naive = solvePhaseFieldSystemNaive
-   with threshold sigma_F.  Only the lumped form (lumping = true) exists. */
+   relative 1e-12 of the initial residual.  naive chooses the damage rule:
+   ZR_DAMAGE_PHASE_FIELD (solvePhaseFieldSystem), ZR_DAMAGE_RANKINE
+   (useNaiveDamage: solvePhaseFieldSystemNaive with threshold sigma_F), or
+   ZR_DAMAGE_PHASE_FIELD_RANKINE: the phase-field solve and then the Rankine
+   cut c <- min(c, sigma_F / sigma_max) on its result (the reference runs the
+   two as alternatives, FractureSimulation.h:149-160; this rule applies both,
+   keeps c monotone and takes the smaller of the two damages' c).
+   delete_threshold is deleteParticles' delete_particle_threshold
+   (FractureSimulation.h:162-207, run at the start of every explicit step):
+   particles with c below it are parked at (4, 4, 4) at rest with F = I and
+   allow_damage off; 0 (the reference's default) deletes nothing, and the
+   grid must reach (4, 4, 4) (grid_log2 with 2^grid_log2 - 8 > 4 / dx).
+   Only the lumped form (lumping = true) exists. */
+enum { ZR_DAMAGE_PHASE_FIELD = 0, ZR_DAMAGE_RANKINE = 1, ZR_DAMAGE_PHASE_FIELD_RANKINE = 2 };
 typedef struct zr_phase_field_opts {
   double parabolic_M;
   int32_t cg_max_iterations;
@@ -175,6 +189,7 @@ typedef struct zr_phase_field_opts {
   double cg_tolerance;
   double cg_relative;
   double sigma_F;
+  double delete_threshold;
 } zr_phase_field_opts;
 
 typedef struct zr_phase_field_stats {

@@ -47,7 +47,7 @@ def implicit_opts(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterati
 
 class ZroPfOpts(ctypes.Structure):
     _fields_ = [("parabolic_M", _d), ("cg_max_iterations", ctypes.c_int32), ("naive", ctypes.c_int32),
-                ("cg_tolerance", _d), ("cg_relative", _d), ("sigma_F", _d)]
+                ("cg_tolerance", _d), ("cg_relative", _d), ("sigma_F", _d), ("delete_threshold", _d)]
 
 
 class ZroPfStats(ctypes.Structure):
@@ -55,12 +55,14 @@ class ZroPfStats(ctypes.Structure):
 
 
 def pf_opts(parabolic_M=0.0, cg_max_iterations=100, naive=False, cg_tolerance=1e-6, cg_relative=1e-12,
-            sigma_F=0.0):
+            sigma_F=0.0, delete_threshold=0.0):
     """FractureSimulation defaults: ConjugateGradient(100), tolerance 1e-6,
     relative 1e-12 in double (FractureSimulation.h:70-73,439;
-    fracture/ConjugateGradient.h:64)."""
-    return ZroPfOpts(float(parabolic_M), int(cg_max_iterations), int(bool(naive)), float(cg_tolerance),
-                     float(cg_relative), float(sigma_F))
+    fracture/ConjugateGradient.h:64).  naive: 0 phase field, 1 (True) the
+    Rankine naive damage, 2 the phase field then the Rankine cut;
+    delete_threshold: deleteParticles' threshold."""
+    return ZroPfOpts(float(parabolic_M), int(cg_max_iterations), int(naive), float(cg_tolerance),
+                     float(cg_relative), float(sigma_F), float(delete_threshold))
 
 
 class ZroAdmmOpts(ctypes.Structure):

@@ -196,10 +196,12 @@ int zro_step_implicit(zro_sim* s, double dt, const zro_implicit_opts* o, zro_imp
 typedef struct {
   double parabolic_M;        /* FractureSimulation::parabolic_M (<= 0: elliptic form) */
   int32_t cg_max_iterations; /* ConjugateGradient(100) (FractureSimulation.h:70-73) */
-  int32_t naive;             /* 1: solvePhaseFieldSystemNaive (Rankine), 0: phase field */
+  int32_t naive;             /* 0: phase field, 1: solvePhaseFieldSystemNaive (Rankine),
+                                2: the phase field, then the Rankine cut of its result */
   double cg_tolerance;       /* 1e-6 (FractureSimulation.h:439) */
   double cg_relative;        /* 1e-12 of the initial residual (fracture/ConjugateGradient.h:64) */
   double sigma_F;            /* Rankine threshold of the naive form */
+  double delete_threshold;   /* deleteParticles' delete_particle_threshold (:162-207) */
 } zro_pf_opts;
 
 typedef struct {
@@ -219,6 +221,8 @@ void zro_get_phase_field(const zro_sim* s, double* c, double* H, double* pf_Fp);
 /* solvePhaseFieldSystem / solvePhaseFieldSystemNaive: after zro_p2g_mls of the
    same step; zro_step_explicit calls it once a phase field is set */
 void zro_phase_field_solve(zro_sim* s, double dt, zro_pf_stats* st);
+/* FractureSimulation::deleteParticles (:162-207); zro_step_explicit runs it first */
+void zro_delete_particles(zro_sim* s);
 /* pieces for tests: per node slot (npages*32) the system of the last solve */
 void zro_phase_field_system(const zro_sim* s, double* rhs, double* diag, double* x);
 void zro_phase_field_multiply(zro_sim* s, const double* x, double* b);

@@ -915,6 +915,7 @@ int zro_g2p_mls(zro_sim* s, double dt)
 /* advanceOneTimeStep, symplectic branch (MpmSimulationBase.cpp:256-281) */
 int zro_step_explicit(zro_sim* s, double dt)
 {
+  zro_delete_particles(s);
   zro_sort_and_pollute(s);
   zro_update_stress(s);
   zro_p2g_mls(s);
@@ -1756,23 +1757,47 @@ static void pf_update_model(zro_sim* s, int i)
   s->gdeg[i] = c * c * (1.0 - k) + k;
 }
 
+/* solvePhaseFieldSystemNaive (:509-554): c cut to sigma_F / sigma_max, the
+   largest principal Cauchy stress of the undegraded model */
+static void pf_rankine_cut(zro_sim* s)
+{
+  const zro_pf_opts* o = &s->pf_o;
+  for (int64_t i = 0; i < s->n; ++i) {
+    const double* F = s->F + 9 * i;
+    double J = det_cm(F), tau[9], cauchy[9];
+    zro_nhb_kirchhoff(s->mu[i], s->lam[i], 1.0, F, tau);
+    for (int c = 0; c < 9; ++c) cauchy[c] = tau[c] / J;
+    double smax = zro_sym3_max_eigenvalue(cauchy);
+    double newc = s->pf_c[i];
+    if (smax > o->sigma_F) newc = fmin(newc, o->sigma_F / smax);
+    s->pf_c[i] = newc;
+    pf_update_model(s, (int)i);
+  }
+}
+
+/* FractureSimulation::deleteParticles (:162-207), which reinitialize() runs
+   at the start of every step (MpmSimulationBase.cpp:262) */
+void zro_delete_particles(zro_sim* s)
+{
+  if (!s->pf_on) return;
+  for (int64_t i = 0; i < s->n; ++i) {
+    if (!(s->pf_c[i] < s->pf_o.delete_threshold)) continue;
+    for (int d = 0; d < 3; ++d) {
+      s->X[3 * i + d] = 4.0;
+      s->V[3 * i + d] = 0.0;
+    }
+    for (int k = 0; k < 9; ++k) s->F[9 * i + k] = (k % 4 == 0) ? 1.0 : 0.0;
+    s->pf_allow[i] = 0;
+  }
+}
+
 void zro_phase_field_solve(zro_sim* s, double dt, zro_pf_stats* st)
 {
   if (!s->pf_on) return;
   const zro_pf_opts* o = &s->pf_o;
   if (st) memset(st, 0, sizeof(*st));
-  if (o->naive) { /* solvePhaseFieldSystemNaive (:509-554) */
-    for (int64_t i = 0; i < s->n; ++i) {
-      const double* F = s->F + 9 * i;
-      double J = det_cm(F), tau[9], cauchy[9];
-      zro_nhb_kirchhoff(s->mu[i], s->lam[i], 1.0, F, tau);
-      for (int c = 0; c < 9; ++c) cauchy[c] = tau[c] / J;
-      double smax = zro_sym3_max_eigenvalue(cauchy);
-      double newc = s->pf_c[i];
-      if (smax > o->sigma_F) newc = fmin(newc, o->sigma_F / smax);
-      s->pf_c[i] = newc;
-      pf_update_model(s, (int)i);
-    }
+  if (o->naive == 1) {
+    pf_rankine_cut(s);
     return;
   }
   s->pf_dt = dt;
@@ -1865,6 +1890,7 @@ void zro_phase_field_solve(zro_sim* s, double dt, zro_pf_stats* st)
     }
     pf_update_model(s, (int)i);
   }
+  if (o->naive == 2) pf_rankine_cut(s);
 }
 
 void zro_phase_field_system(const zro_sim* s, double* rhs, double* diag, double* x)
@@ -2363,6 +2389,45 @@ static double admm_omega_of(const zro_sim* s, int64_t i, const double* sig)
   return sqrt(s->vol[i] * sqrt(total));
 }
 
+/* initOmega's second half (AdmmSimulationFoam.h:254-270) for the viscoelastic
+   branch's F2 = U S V^T: the StvkWithHencky model of that branch, no
+   projection */
+static double admm_visco_omega_of(const zro_sim* s, int64_t i, const double* sig)
+{
+  double mu = s->mu_neq[i], lam = s->lam_neq[i];
+  double A[3][3], B[2][2];
+  hencky_ddE(mu, lam, sig, A);
+  double total = pd_norm2_3(A);
+  const int pr[3][2] = { { 0, 1 }, { 0, 2 }, { 1, 2 } };
+  for (int k = 0; k < 3; ++k) {
+    hencky_Bij(mu, lam, sig, pr[k][0], pr[k][1], 1e-6, B);
+    total += pd_norm2_2(B[0][0], B[0][1], B[1][1]);
+  }
+  return sqrt(s->vol[i] * sqrt(total));
+}
+
+/* The viscoelastic branch's return map inside FStepSolve (AdmmSimulationFoam.h:
+   583-603): the relaxed singular values zp = exp(eps) with
+   eps = (log|z| - c tr(log|z|) 1) / (1 + dt alpha), c = dt beta / (1 + dt (alpha
+   + 3 beta)), where tr of the 3-vector is Eigen's trace() of a 3x1 matrix, its
+   first entry; and the derivative the reference forms,
+   D = diag(zp) (diag(1/z) - c 1 (1/z)^T) / (1 + dt alpha) (the full sum of the
+   log's derivatives, as written there). */
+void zro_admm_visco_map(double mu, double lam, double visc_d, double visc_v, double dt, const double* z, double* zp,
+                        double D[3][3])
+{
+  const double alpha = 2.0 * mu / visc_d;
+  const double beta = 2.0 * (2.0 * mu + lam * 3.0) / (9.0 * visc_v) - 2.0 * mu / (visc_d * 3.0);
+  const double c = dt * beta / (1.0 + dt * (alpha + 3.0 * beta));
+  const double h = 1.0 / (1.0 + dt * alpha);
+  double e[3];
+  for (int d = 0; d < 3; ++d) e[d] = log(fabs(z[d]));
+  const double tr = e[0];
+  for (int d = 0; d < 3; ++d) zp[d] = exp(h * (e[d] - c * tr));
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) D[a][b] = zp[a] * h * ((a == b ? 1.0 / z[b] : 0.0) - c / z[b]);
+}
+
 /* multiObjectCollisionWithFakeSeparate (CollisionObject.cpp:108-160): resolves v,
    returns 1 and the projection P = I - K K^T when any collider acts.
 
@@ -2519,6 +2584,14 @@ static void admm_multiply(zro_sim* s, double dt, double rho, const double* x, do
     for (int r = 0; r < 3; ++r)
       for (int c = 0; c < 3; ++c) Y[r][c] = cp * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
     admm_scatter(s, i, Y, Fn, b);
+    if (s->fneq) { /* the viscoelastic branch's term with FFn (:449-471) */
+      double Fv[3][3];
+      mat_from_cm(s->fneq + 9 * (int64_t)i, Fv);
+      double om2 = s->admm_omega[s->n + i], c2 = rho * om2 * om2 * dt * dt;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Y[r][c] = c2 * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+      admm_scatter(s, i, Y, Fv, b);
+    }
   }
 }
 

@@ -7,8 +7,8 @@ from ziran2019_amd import scenes
 from ziran2019_amd.engine import MpmEngine
 
 
-def make_pair(s, gravity=(0.0, -9.8, 0.0), colliders=(), nthreads=8):
-    eng = MpmEngine(dx=s["dx"], gravity=gravity)
+def make_pair(s, gravity=(0.0, -9.8, 0.0), colliders=(), nthreads=8, grid_log2=None):
+    eng = MpmEngine(dx=s["dx"], gravity=gravity, grid_log2=grid_log2)
     eng.upload(s)
     eng.set_colliders(list(colliders))
     ora = oracle.OracleSim(dx=s["dx"], gravity=gravity, nthreads=nthreads)

@@ -16,8 +16,8 @@ from ziran2019_amd.engine import MpmEngine, RestartException, ZrError
 pytestmark = pytest.mark.gpu
 
 
-def make_pair(s, gravity=(0.0, -9.8, 0.0), colliders=(), nthreads=8):
-    eng = MpmEngine(dx=s["dx"], gravity=gravity)
+def make_pair(s, gravity=(0.0, -9.8, 0.0), colliders=(), nthreads=8, grid_log2=None):
+    eng = MpmEngine(dx=s["dx"], gravity=gravity, grid_log2=grid_log2)
     eng.upload(s)
     eng.set_colliders(list(colliders))
     ora = oracle.OracleSim(dx=s["dx"], gravity=gravity, nthreads=nthreads)
@@ -219,12 +219,16 @@ def test_single_particle_steps():
     assert rel_err(out["F"], F) < 1e-13
 
 
-def test_particles_at_domain_bounds():
-    """Base nodes at 0 and at the engine's upper bound 2^L - 8: sort keys and
-    groups bit-exact, a step against the oracle."""
+@pytest.mark.parametrize("L", [None, 11, 12])
+def test_particles_at_domain_bounds(L):
+    """Base nodes at 0 and at the engine's upper bound 2^L - 8 (the default
+    grid for dx, and the larger extents up to the reference's 4096^3, whose
+    page codes use 11 bits of x): sort keys and groups bit-exact, a step
+    against the oracle."""
     from ziran2019_amd.engine import grid_log2_for
     dx = 1.0 / 64
-    lim = (1 << grid_log2_for(dx)) - 8  # the engine's default grid for dx
+    L = grid_log2_for(dx) if L is None else L
+    lim = (1 << L) - 8
     rng = np.random.default_rng(10)
     lo = (0.5 + rng.random((40, 3)) * 0.999) * dx            # base node 0
     hi = (lim + 0.5 + rng.random((40, 3)) * 0.999) * dx      # base node lim
@@ -233,20 +237,22 @@ def test_particles_at_domain_bounds():
     s = dict(X=X, V=0.1 * rng.normal(size=(n, 3)), C=0.1 * rng.normal(size=(n, 9)),
              F=np.tile(np.eye(3).reshape(1, 9), (n, 1)), mass=np.full(n, 1e-3), vol=np.full(n, 1e-6),
              mu=np.full(n, 3e3), lam=np.full(n, 5e3), dx=dx)
-    eng, ora = make_pair(s, gravity=(0.0, 0.0, 0.0))
+    print("domain bounds: grid_log2", L, "upper base node", lim, flush=True)
+    eng, ora = make_pair(s, gravity=(0.0, 0.0, 0.0), grid_log2=L)
     eng.sortParticlesAndPolluteGrid()
     ora.sort_and_pollute()
     info = ora.sort_result()
     assert np.array_equal(eng.dump("sorter"), info["sorter"])
     assert np.array_equal(eng.dump("order"), info["order"])
-    eng2, ora2 = make_pair(s, gravity=(0.0, 0.0, 0.0))
+    assert np.array_equal(eng.dump("pages"), info["pages"])
+    eng.close()
+    eng2, ora2 = make_pair(s, gravity=(0.0, 0.0, 0.0), grid_log2=L)
     eng2.advanceOneTimeStep(1e-4)
     ora2.step(1e-4)
     out = eng2.download()
     X1, V1, C1, F1 = ora2.get_particles()
     assert rel_err(out["X"], X1) < 1e-13
     assert rel_err(out["V"], V1) < 1e-11
-    eng.close()
     eng2.close()
 
 

@@ -111,6 +111,45 @@ def test_fp32_operator_close_to_fp64_oracle():
     assert rel_err(eng.implicit_multiply(x), ora.implicit_multiply(x)) < 2e-5
 
 
+def test_fp32_operator_near_inversion():
+    """Particles close to inversion (singular values 1, 0.6 and -0.6 + delta,
+    delta in [1e-6, 1e-4]: the rotation terms of the operator divide by
+    s_1 + s_2 = delta).  The FP32 operator keeps s_1 + s_2 summed in FP64, so
+    its product still agrees with the FP64 oracle to FP32 accuracy; the sum
+    of the rounded singular values would be off by up to 4e-8 / delta."""
+    s = perturbed(16, n_cells=5, F_noise=0.0, C_noise=0.0, E=2e4)
+    n = len(s["mass"])
+    rng = np.random.default_rng(16)
+
+    def rotation():
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        return q * np.sign(np.linalg.det(q))
+    F = np.empty((n, 3, 3))
+    near = rng.random(n) < 0.3
+    for p in range(n):
+        if near[p]:
+            delta = 10.0 ** rng.uniform(-6, -4)
+            F[p] = rotation() @ np.diag([1.0, 0.6, -0.6 + delta]) @ rotation().T
+        else:
+            F[p] = np.eye(3) + 0.03 * rng.normal(size=(3, 3))
+    s["F"] = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(n, 9))
+    s["V"][:] = 0.0
+    s["C"][:] = 0.0
+    print("near-inversion operator:", int(near.sum()), "of", n, "particles", flush=True)
+    eng, ora = make_pair(s, gravity=(0.0, 0.0, 0.0))
+    eng.sortParticlesAndPolluteGrid()
+    eng.particlesToGridApic()
+    eng.backwardEulerStep(DT, implicit_opts(newton_max_iterations=1, precision=1))
+    ora.sort_and_pollute()
+    ora.p2g_apic()
+    ora.backward_euler(DT, oracle.implicit_opts(newton_max_iterations=1))
+    active = ora.grid()["idx"] >= 0
+    x = rng.normal(size=(len(active), 3))
+    x[~active] = 0.0
+    got, want = eng.implicit_multiply(x), ora.implicit_multiply(x)
+    assert rel_err(got, want) < 2e-5
+
+
 def test_fp32_refined_step_matches_fp64_oracle():
     """FP32 operator inside the FP64 Newton: after 5 steps the particle state
     is within 1e-5 of the FP64 oracle (relative to the displacement)."""

@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from ziran2019_amd import scenes
-from ziran2019_amd.engine import nhb_one_over_sigma_c, phase_field_opts
+from ziran2019_amd.engine import ZR_DAMAGE_PHASE_FIELD_RANKINE, nhb_one_over_sigma_c, phase_field_opts
 from _helpers import make_pair, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -25,16 +25,18 @@ def _scene(seed, ncells=6, stretch=0.12, E=500.0, nu=0.4):
     return s, osc
 
 
-def _pair(s, osc, M, naive=False, sigma_F=0.0, colliders=(), pf_Fp=None, tol=0.0):
+def _pair(s, osc, M, naive=False, sigma_F=0.0, colliders=(), pf_Fp=None, tol=0.0, c=None, delete_threshold=0.0,
+          grid_log2=None):
     """tol = 0 leaves CG to its relative tolerance (1e-12 |r0|), so that the
     solve runs many iterations; the reference's absolute 1e-6 stops it after
     one or two at these particle volumes."""
-    eng, ora = make_pair(s, colliders=colliders)
+    eng, ora = make_pair(s, colliders=colliders, grid_log2=grid_log2)
     l0 = 0.5 * s["dx"]
     allow = (np.arange(len(s["mass"])) % 7 != 0).astype(np.int32)
-    kw = dict(l0=l0, one_over_sigma_c=osc, allow_damage=allow, pf_Fp=pf_Fp)
-    eng.set_phase_field(phase_field_opts(parabolic_M=M, naive=naive, sigma_F=sigma_F, cg_tolerance=tol), **kw)
-    ora.set_phase_field(oracle.pf_opts(parabolic_M=M, naive=naive, sigma_F=sigma_F, cg_tolerance=tol), **kw)
+    kw = dict(l0=l0, one_over_sigma_c=osc, allow_damage=allow, pf_Fp=pf_Fp, c=c)
+    o = dict(parabolic_M=M, naive=naive, sigma_F=sigma_F, cg_tolerance=tol, delete_threshold=delete_threshold)
+    eng.set_phase_field(phase_field_opts(**o), **kw)
+    ora.set_phase_field(oracle.pf_opts(**o), **kw)
     return eng, ora
 
 
@@ -98,4 +100,50 @@ def test_naive_rankine_parity():
     assert pg["c"].min() < 1.0
     out = eng.download()
     assert rel_err(out["X"], ora.get_particles()[0]) < 1e-10
+    eng.close()
+
+
+def test_phase_field_then_rankine_steps_parity():
+    """ZR_DAMAGE_PHASE_FIELD_RANKINE (the phase-field solve, then the Rankine
+    cut of its c) inside explicit steps, against the oracle."""
+    s, osc = _scene(34, stretch=0.15)
+    eng, ora = _pair(s, osc, 15.0, naive=ZR_DAMAGE_PHASE_FIELD_RANKINE, sigma_F=40.0)
+    dt = 1e-4
+    for _ in range(4):
+        eng.advanceOneTimeStep(dt)
+        ora.step(dt)
+    out = eng.download()
+    X, V, C, F = ora.get_particles()
+    assert rel_err(out["X"], X) < 1e-9
+    assert rel_err(out["V"], V) < 1e-9
+    assert rel_err(out["F"], F) < 1e-9
+    pg, po = eng.get_phase_field(), ora.get_phase_field()
+    assert rel_err(pg["c"], po["c"]) < 1e-9
+    assert pg["c"].min() < 1.0
+    assert rel_err(eng.get_materials()["degradation"], ora.get_materials()["g"]) < 1e-9
+    eng.close()
+
+
+def test_delete_particles_parity():
+    """deleteParticles at the start of every explicit step: particles whose c
+    is below the threshold are parked at (4, 4, 4) (a grid that reaches it:
+    2^9 - 8 nodes at dx = 1/64), against the oracle over several steps."""
+    s, osc = _scene(35, stretch=0.15)
+    n = len(s["mass"])
+    c0 = np.ones(n)
+    c0[::11] = 1e-3
+    eng, ora = _pair(s, osc, 15.0, c=c0, delete_threshold=0.01, grid_log2=9)
+    dt = 1e-4
+    for _ in range(3):
+        eng.advanceOneTimeStep(dt)
+        ora.step(dt)
+    out = eng.download()
+    X, V, C, F = ora.get_particles()
+    parked = np.abs(X - 4.0).max(axis=1) < 1e-6
+    assert parked.sum() == len(c0[::11])
+    assert rel_err(out["X"], X) < 1e-12
+    assert rel_err(out["V"], V) < 1e-9
+    assert rel_err(out["F"], F) < 1e-9
+    pg, po = eng.get_phase_field(), ora.get_phase_field()
+    assert rel_err(pg["c"], po["c"]) < 1e-9
     eng.close()

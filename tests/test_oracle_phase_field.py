@@ -252,3 +252,68 @@ def test_damage_front_decays_over_2_l0():
     slope = np.polyfit(x[sel], np.log(1.0 - c[sel]), 1)[0]
     assert -1.0 / slope == pytest.approx(2 * l0, rel=0.08)
     assert c[s["X"][:, 0] < xmid - 3 * dx].max() < 1e-3
+
+
+def test_phase_field_then_rankine_rule():
+    """ZR_DAMAGE_PHASE_FIELD_RANKINE: the phase-field solve, then the Rankine
+    cut of its result: c = min(c_pf, sigma_F / sigma_max) where sigma_max
+    (numpy's eigensolver on the undegraded Cauchy stress) exceeds sigma_F;
+    g follows the final c."""
+    s = _scene(7, stretch=0.1)
+    n = len(s["mass"])
+    F = s["F"].reshape(n, 3, 3).transpose(0, 2, 1)
+    smax = np.empty(n)
+    for p in range(n):
+        cauchy = oracle.nhb_kirchhoff(s["mu"][p], s["lam"][p], 1.0, F[p]) / np.linalg.det(F[p])
+        smax[p] = np.linalg.eigvalsh(0.5 * (cauchy + cauchy.T)).max()
+    sigma_F = float(np.median(smax))  # about half the particles are cut
+    pf = dict(l0=0.5 * s["dx"], one_over_sigma_c=np.full(n, 3.0))
+    out = {}
+    for rule in (0, 2):
+        sim = _sim(s, oracle.pf_opts(naive=rule, sigma_F=sigma_F), **pf)
+        sim.sort_and_pollute()
+        sim.update_stress()
+        sim.p2g()
+        sim.phase_field_solve(1e-4)
+        out[rule] = sim.get_phase_field()["c"]
+    cut = 0
+    for p in range(n):
+        expect = min(out[0][p], sigma_F / smax[p]) if smax[p] > sigma_F else out[0][p]
+        cut += expect < out[0][p]
+        assert out[2][p] == pytest.approx(expect, rel=1e-12)
+    assert 0 < cut < n
+    assert out[0].min() < 1.0  # the phase field itself damaged some particles
+
+
+def test_delete_particles_parks_at_rest():
+    """deleteParticles (FractureSimulation.h:162-207) at the start of the
+    step: a particle whose c is below the threshold is parked at (4, 4, 4)
+    with v = 0 and F = I, its C kept.  Alone there, one MLS step moves it as
+    a free particle: the grid velocity is C (x_i - x_p) + g dt, whose
+    weighted first moment vanishes and whose weighted second moment gives C
+    back (sum_i w_i (x_i - x_p)(x_i - x_p)^T = dx^2 / 4 I), so v = g dt,
+    x = (4, 4, 4) + g dt^2, C unchanged and F = I + dt C."""
+    s = _scene(8, ncells=3)
+    n = len(s["mass"])
+    rng = np.random.default_rng(8)
+    s["C"] = np.ascontiguousarray(0.3 * rng.normal(size=(n, 9)))
+    c0 = np.ones(n)
+    gone = [3, 17]
+    c0[gone] = 1e-3
+    sim = _sim(s, oracle.pf_opts(delete_threshold=0.01), c=c0, l0=0.5 * s["dx"], one_over_sigma_c=np.full(n, 3.0))
+    dt = 1e-4
+    sim.step(dt)
+    X, V, C, F = sim.get_particles()
+    g = np.array([0.0, -9.8, 0.0])
+    Cg = s["C"][gone].reshape(-1, 3, 3).transpose(0, 2, 1)
+    # the parked particles share one position, so their grid sees the
+    # mass-weighted mean of their C
+    m = s["mass"][gone]
+    Cm = (m[:, None, None] * Cg).sum(axis=0) / m.sum()
+    for k, p in enumerate(gone):
+        assert np.allclose(X[p], 4.0 + g * dt * dt, rtol=0, atol=1e-14)
+        assert np.allclose(V[p], g * dt, rtol=0, atol=1e-13)
+        assert np.allclose(C[p].reshape(3, 3).T, Cm, rtol=0, atol=1e-12)
+        assert np.allclose(F[p].reshape(3, 3).T, np.eye(3) + dt * Cm, rtol=0, atol=1e-14)
+    keep = np.setdiff1d(np.arange(n), gone)
+    assert np.abs(X[keep] - s["X"][keep]).max() < 1e-3

@@ -447,6 +447,27 @@ __device__ __forceinline__ double sym3_max_eig(const M3<double>& A)
   return q + 2.0 * p * cos(acos(r) / 3.0);
 }
 
+// FractureSimulation::deleteParticles (:162-207), run by reinitialize() at
+// the start of every step (MpmSimulationBase.cpp:262): particles whose c is
+// below the threshold are parked at (4, 4, 4) at rest with F = I and take no
+// further damage
+__global__ __launch_bounds__(256) void k_pf_delete(ParticlesView P, const double* __restrict__ c,
+                                                   int32_t* __restrict__ allow, double threshold)
+{
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= (uint32_t)P.stride) return;
+  if (!(c[i] < threshold)) return;
+  const int64_t S = P.stride;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    P.x[d * S + i] = 4.0;
+    P.v[d * S + i] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) P.F[k * S + i] = (k % 4 == 0) ? 1.0 : 0.0;
+  allow[i] = 0;
+}
+
 // solvePhaseFieldSystemNaive (:509-554): Rankine cut of c by the largest
 // principal Cauchy stress of the undegraded model; the FP64 threshold test
 __global__ __launch_bounds__(256) void k_pf_naive(ParticlesView P, PfParticles pf, double sigma_F, double* gdeg)

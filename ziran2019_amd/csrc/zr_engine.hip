@@ -186,8 +186,8 @@ __device__ __forceinline__ void trial_collision(const ColliderSet& cs, int fr, c
 // binning and sort (MpmSimulationBase.cpp:1032-1103)
 // ---------------------------------------------------------------------------
 __global__ void k_reset_tables(const uint32_t* group_page, uint32_t ngroups, const uint32_t* slot_page, uint32_t nslots,
-                               uint32_t* page_count, int32_t* page_group, uint32_t* page_bits, uint32_t* page_touch,
-                               int32_t* page_slot)
+                               uint32_t* page_count, int32_t* page_group, uint32_t* page_bits, uint32_t* page_sum,
+                               uint32_t* page_touch, int32_t* page_slot)
 {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < ngroups) {
@@ -195,6 +195,7 @@ __global__ void k_reset_tables(const uint32_t* group_page, uint32_t ngroups, con
     page_count[p] = 0u;
     page_group[p] = -1;
     page_bits[p >> 5] = 0u;
+    page_sum[p >> 10] = 0u;
   }
   if (t < nslots) {
     uint32_t p = slot_page[t];
@@ -203,8 +204,12 @@ __global__ void k_reset_tables(const uint32_t* group_page, uint32_t ngroups, con
   }
 }
 
+// page_bits flags the touched pages, page_sum the page_bits words that are
+// not zero (bit w & 31 of word w >> 5), so that listing the pages reads the
+// summary (a 4096^3 extent has 2^26 page_bits words and 2^21 summary words)
 __global__ void k_bin(const double* __restrict__ x, int64_t stride, uint32_t n, double inv_dx, int lim,
-                      uint32_t* page_count, uint32_t* page_bits, uint32_t* p_page, uint32_t* p_re, uint32_t* counters)
+                      uint32_t* page_count, uint32_t* page_bits, uint32_t* page_sum, uint32_t* p_page, uint32_t* p_re,
+                      uint32_t* counters)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
@@ -235,7 +240,10 @@ __global__ void k_bin(const double* __restrict__ x, int64_t stride, uint32_t n, 
     uint32_t base = 0;
     if (lane == leader) {
       base = atomicAdd(&page_count[lpc], (uint32_t)__popcll(m));
-      if (base == 0u) atomicOr(&page_bits[lpc >> 5], 1u << (lpc & 31u));
+      if (base == 0u) {
+        const uint32_t old = atomicOr(&page_bits[lpc >> 5], 1u << (lpc & 31u));
+        if (old == 0u) atomicOr(&page_sum[lpc >> 10], 1u << ((lpc >> 5) & 31u));
+      }
     }
     base = __shfl(base, leader, 64);
     if ((m >> lane) & 1ull) r = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
@@ -246,22 +254,38 @@ __global__ void k_bin(const double* __restrict__ x, int64_t stride, uint32_t n, 
   p_re[i] = (r << 5) | e;
 }
 
-__global__ void k_list_groups(const uint32_t* page_bits, uint32_t nwords, const uint32_t* word_base,
-                              const uint32_t* page_count, uint32_t* group_page, int32_t* page_group,
-                              uint32_t* group_count)
+// The touched pages in ascending page code (the reference's sorted key
+// order): sum_base[s] is the first group of summary word s; 32 lanes per
+// summary word, grid-stride, each lane one page_bits word, the lanes' page
+// counts prefix-summed across the 32 lanes.
+__global__ __launch_bounds__(256) void k_list_groups(const uint32_t* page_sum, uint32_t nsum,
+                                                     const uint32_t* page_bits, const uint32_t* sum_base,
+                                                     const uint32_t* page_count, uint32_t* group_page,
+                                                     int32_t* page_group, uint32_t* group_count)
 {
-  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nwords) return;
-  uint32_t bits = page_bits[w];
-  uint32_t g = word_base[w];
-  while (bits) {
-    uint32_t b = (uint32_t)__builtin_ctz(bits);
-    bits &= bits - 1u;
-    uint32_t pc = w * 32u + b;
-    group_page[g] = pc;
-    page_group[pc] = (int32_t)g;
-    group_count[g] = page_count[pc];
-    ++g;
+  const uint32_t lane = threadIdx.x & 31u;
+  for (uint32_t s = blockIdx.x * 8u + (threadIdx.x >> 5); s < nsum; s += gridDim.x * 8u) {
+    const uint32_t sb = page_sum[s];
+    if (!sb) continue;  // alike in the 32 lanes of s
+    const uint32_t w = s * 32u + lane;
+    uint32_t bits = ((sb >> lane) & 1u) ? page_bits[w] : 0u;
+    const uint32_t c = (uint32_t)__popc(bits);
+    uint32_t incl = c;
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+      const uint32_t t = __shfl_up(incl, off, 32);
+      if (lane >= (uint32_t)off) incl += t;
+    }
+    uint32_t g = sum_base[s] + incl - c;
+    while (bits) {
+      const uint32_t b = (uint32_t)__builtin_ctz(bits);
+      bits &= bits - 1u;
+      const uint32_t pc = w * 32u + b;
+      group_page[g] = pc;
+      page_group[pc] = (int32_t)g;
+      group_count[g] = page_count[pc];
+      ++g;
+    }
   }
 }
 
@@ -938,10 +962,11 @@ struct zr_engine {
   double time = 0.0;
   std::vector<zr_collider> colliders;
 
-  uint32_t L = 0, lim = 0, ndense = 0, nwords = 0;
+  uint32_t L = 0, lim = 0, ndense = 0, nwords = 0, nsum = 0;
   uint32_t* page_count = nullptr;
   int32_t* page_group = nullptr;
   uint32_t* page_bits = nullptr;
+  uint32_t* page_sum = nullptr;
   uint32_t* page_touch = nullptr;
   int32_t* page_slot = nullptr;
 
@@ -996,7 +1021,7 @@ struct zr_engine {
   int32_t* group_nslot = nullptr;  // [8 * groups]
   int32_t* slot_src = nullptr;     // [8 * slots], capacity grid_cap / 32
   uint32_t* ticket = nullptr;      // grid-reduction ticket, kept at zero between kernels
-  uint32_t *word_base = nullptr;
+  uint32_t *sum_base = nullptr;
 
   uint32_t A = 0, S = 0;  // groups, slots of the current sort
   bool sorted = false, rasterised = false, updated = false;
@@ -1291,17 +1316,22 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   uint32_t rst = e->A > e->S ? e->A : e->S;
   if (rst)
     k_reset_tables<<<blocks_for(rst, 256), 256, 0, s>>>(e->group_page, e->A, e->slot_page, e->S, e->page_count,
-                                                       e->page_group, e->page_bits, e->page_touch, e->page_slot);
+                                                       e->page_group, e->page_bits, e->page_sum, e->page_touch,
+                                                       e->page_slot);
   e->sorted = e->rasterised = e->updated = false;
   prof_mark(e, ZR_PROF_SORT);
   ZR_HIP(e, hipMemsetAsync(e->dcount, 0, CNT_N * sizeof(uint32_t), s));
   k_bin<<<blocks_for(n, 256), 256, 0, s>>>(e->x, e->n, n, 1.0 / e->cfg.dx, (int)e->lim, e->page_count, e->page_bits,
-                                           e->p_page, e->p_re, e->dcount);
+                                           e->page_sum, e->p_page, e->p_re, e->dcount);
   ZR_LAUNCH_CHECK(e);
-  int rc = run_scan(e, LdPopc{ e->page_bits }, e->nwords, e->word_base, e->dcount + CNT_GROUPS);
+  int rc = run_scan(e, LdPopcSum{ e->page_sum, e->page_bits }, e->nsum, e->sum_base, e->dcount + CNT_GROUPS);
   if (rc) return rc;
-  k_list_groups<<<blocks_for(e->nwords, 256), 256, 0, s>>>(e->page_bits, e->nwords, e->word_base, e->page_count,
-                                                           e->group_page, e->page_group, e->group_count);
+  {
+    uint32_t nb = (e->nsum + 7u) / 8u;
+    if (nb > 2048u) nb = 2048u;
+    k_list_groups<<<nb, 256, 0, s>>>(e->page_sum, e->nsum, e->page_bits, e->sum_base, e->page_count, e->group_page,
+                                     e->page_group, e->group_count);
+  }
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_SORT);
   rc = sync_counts(e);
@@ -1453,7 +1483,7 @@ int do_pf(zr_engine* e, double dt)
   if (!e->pf_on || e->n == 0) return ZR_OK;
   const zr_phase_field_opts& o = e->pf_opts;
   hipStream_t s = e->stream;
-  if (o.naive) {
+  if (o.naive == ZR_DAMAGE_RANKINE) {
     k_pf_naive<<<blocks_for((uint64_t)e->n, 256), 256, 0, s>>>(pview(e), pf_view(e), o.sigma_F, e->gdeg);
     ZR_LAUNCH_CHECK(e);
     return ZR_OK;
@@ -1503,6 +1533,8 @@ int do_pf(zr_engine* e, double dt)
   }
   k_pf_g2p<<<e->A, 256, 0, s>>>(pview(e), pf, ps, e->order, e->group_start, e->group_page, e->group_nslot, e->gm, x,
                                 phase, e->gdeg, 1.0 / dx);
+  if (o.naive == ZR_DAMAGE_PHASE_FIELD_RANKINE)
+    k_pf_naive<<<blocks_for((uint64_t)e->n, 256), 256, 0, s>>>(pview(e), pf, o.sigma_F, e->gdeg);
   ZR_LAUNCH_CHECK(e);
   ZR_HIP(e, hipMemcpyAsync(e->hsc, e->sc, SC_N * sizeof(double), hipMemcpyDeviceToHost, s));
   ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -2136,7 +2168,7 @@ int zr_create(const zr_config* cfg, zr_engine** out)
 {
   if (!cfg || !out) return ZR_ERR_ARG;
   *out = nullptr;
-  if (!(cfg->dx > 0.0) || cfg->grid_log2 < 6 || cfg->grid_log2 > 10 || cfg->precision != ZR_FP64 ||
+  if (!(cfg->dx > 0.0) || cfg->grid_log2 < 6 || cfg->grid_log2 > 12 || cfg->precision != ZR_FP64 ||
       cfg->transfer != ZR_TRANSFER_MLS)
     return ZR_ERR_ARG;
   zr_engine* e = new zr_engine();
@@ -2145,6 +2177,7 @@ int zr_create(const zr_config* cfg, zr_engine** out)
   e->lim = (1u << e->L) - 8u;
   e->ndense = 1u << (3 * e->L - 5);
   e->nwords = (e->ndense + 31u) / 32u;
+  e->nsum = (e->nwords + 31u) / 32u;
   hipError_t st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (st != hipSuccess) {
     delete e;
@@ -2173,7 +2206,8 @@ int zr_create(const zr_config* cfg, zr_engine** out)
             hipMalloc(&e->page_touch, e->ndense * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->page_slot, e->ndense * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&e->page_bits, e->nwords * sizeof(uint32_t)) == hipSuccess &&
-            hipMalloc(&e->word_base, (e->nwords + 1) * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&e->page_sum, e->nsum * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&e->sum_base, (e->nsum + 1) * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->dcount, CNT_N * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->ticket, sizeof(uint32_t)) == hipSuccess &&
             hipHostMalloc(&e->hcount, CNT_N * sizeof(uint32_t)) == hipSuccess;
@@ -2182,6 +2216,7 @@ int zr_create(const zr_config* cfg, zr_engine** out)
        hipMemset(e->page_touch, 0xff, e->ndense * sizeof(uint32_t)) == hipSuccess &&
        hipMemset(e->page_slot, 0xff, e->ndense * sizeof(int32_t)) == hipSuccess &&
        hipMemset(e->page_bits, 0, e->nwords * sizeof(uint32_t)) == hipSuccess &&
+       hipMemset(e->page_sum, 0, e->nsum * sizeof(uint32_t)) == hipSuccess &&
        hipMemset(e->ticket, 0, sizeof(uint32_t)) == hipSuccess &&
        hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
@@ -2198,7 +2233,7 @@ void zr_destroy(zr_engine* e)
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   free_particles(e);
   dfree(e->page_count); dfree(e->page_group); dfree(e->page_touch); dfree(e->page_slot); dfree(e->page_bits);
-  dfree(e->word_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
+  dfree(e->page_sum); dfree(e->sum_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->speed); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->jstage); dfree(e->pinv); dfree(e->collided); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
@@ -2369,7 +2404,7 @@ int zr_upload_particles(zr_engine* e, const zr_particles* p)
     uint32_t rst = e->A > e->S ? e->A : e->S;
     k_reset_tables<<<blocks_for(rst, 256), 256, 0, e->stream>>>(e->group_page, e->A, e->slot_page, e->S,
                                                                 e->page_count, e->page_group, e->page_bits,
-                                                                e->page_touch, e->page_slot);
+                                                                e->page_sum, e->page_touch, e->page_slot);
     ZR_LAUNCH_CHECK(e);
     ZR_HIP(e, hipStreamSynchronize(e->stream));
     e->A = e->S = 0;
@@ -2574,9 +2609,11 @@ int zr_set_phase_field(zr_engine* e, const zr_phase_field* pf, const zr_phase_fi
 {
   if (!e || !pf || !o) return fail(e, ZR_ERR_ARG, "bad phase-field view");
   if (pf->n != e->n) return fail(e, ZR_ERR_ARG, "phase-field count differs from the particle count");
-  if (!o->naive && (o->cg_max_iterations < 0 || !(o->cg_tolerance >= 0.0)))
+  if (o->naive != ZR_DAMAGE_PHASE_FIELD && o->naive != ZR_DAMAGE_RANKINE && o->naive != ZR_DAMAGE_PHASE_FIELD_RANKINE)
+    return fail(e, ZR_ERR_ARG, "unknown damage rule");
+  if (o->naive != ZR_DAMAGE_RANKINE && (o->cg_max_iterations < 0 || !(o->cg_tolerance >= 0.0)))
     return fail(e, ZR_ERR_ARG, "bad phase-field CG options");
-  if (!o->naive && (!pf->l0 || !pf->one_over_sigma_c))
+  if (o->naive != ZR_DAMAGE_RANKINE && (!pf->l0 || !pf->one_over_sigma_c))
     return fail(e, ZR_ERR_ARG, "the phase field needs l0 and one_over_sigma_c");
   const size_t N = (size_t)e->n;
   if (N == 0) return ZR_OK;
@@ -2854,6 +2891,11 @@ int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st)
 {
   if (!e) return ZR_ERR_ARG;
   if (!(dt > 0.0)) return fail(e, ZR_ERR_ARG, "dt must be positive");
+  if (e->pf_on && e->n > 0) {  // reinitialize(): FractureSimulation::deleteParticles
+    k_pf_delete<<<blocks_for((uint64_t)e->n, 256), 256, 0, e->stream>>>(pview(e), e->pf_c, e->pf_allow,
+                                                                        e->pf_opts.delete_threshold);
+    ZR_LAUNCH_CHECK(e);
+  }
   int rc = do_sort(e, st);
   if (rc) return rc;
   rc = do_p2g(e, true);

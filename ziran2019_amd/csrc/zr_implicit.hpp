@@ -356,15 +356,15 @@ __device__ __forceinline__ float op_div<float>(float a, float b)
 // fixed-corotated differential in the singular frame: for F = U diag(s) V^T
 // and M = U^T dF V, H = U^T (V0 dP) V with k2mu = 2 mu V0, klam = lam V0
 // (the rotation's part through W = (M - M^T) / (s_i + s_j), |s_i + s_j|
-// kept >= 1e-12)
+// kept >= 1e-12), from the pair sums d_ij = s_i + s_j
 template <class T>
-__device__ __forceinline__ void fcr_frame_differential(const T s[3], T k2mu, T klam, const T M[3][3], T H[3][3])
+__device__ __forceinline__ void fcr_frame_differential_sums(const T s[3], T d01, T d02, T d12, T k2mu, T klam,
+                                                            const T M[3][3], T H[3][3])
 {
   const T J = s[0] * s[1] * s[2];
   const T cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
   const T cm = cof[0] * M[0][0] + cof[1] * M[1][1] + cof[2] * M[2][2];
   const T eps = (T)1e-12;
-  T d01 = s[0] + s[1], d02 = s[0] + s[2], d12 = s[1] + s[2];
   if (fabs(d01) < eps) d01 = d01 < (T)0 ? -eps : eps;
   if (fabs(d02) < eps) d02 = d02 < (T)0 ? -eps : eps;
   if (fabs(d12) < eps) d12 = d12 < (T)0 ? -eps : eps;
@@ -381,6 +381,25 @@ __device__ __forceinline__ void fcr_frame_differential(const T s[3], T k2mu, T k
   H[2][0] = k2mu * (M[2][0] + W02) - lj * s[1] * M[0][2];
   H[1][2] = k2mu * (M[1][2] - W12) - lj * s[0] * M[2][1];
   H[2][1] = k2mu * (M[2][1] + W12) - lj * s[0] * M[1][2];
+}
+template <class T>
+__device__ __forceinline__ void fcr_frame_differential(const T s[3], T k2mu, T klam, const T M[3][3], T H[3][3])
+{
+  fcr_frame_differential_sums(s, s[0] + s[1], s[0] + s[2], s[1] + s[2], k2mu, klam, M, H);
+}
+// The FP32 operator data hold t = s_1 + s_2, summed in FP64, in place of
+// s_2.  Only s_2 can be negative (the SVD's sign convention), so s_1 + s_2
+// and s_0 + s_2 are the sums that cancel as an element inverts, and the
+// rotation terms divide by them; from t both come out to FP32 relative
+// accuracy (s_0 - s_1 is exact when s_1 <= s_0 <= 2 s_1 and far from
+// cancelling otherwise), where the sum of two rounded singular values
+// would carry an absolute error of their size.
+__device__ __forceinline__ void fcr_frame_differential_t(const float st[3], float k2mu, float klam,
+                                                         const float M[3][3], float H[3][3])
+{
+  const float t = st[2];
+  const float s[3] = { st[0], st[1], t - st[1] };
+  fcr_frame_differential_sums(s, s[0] + s[1], (s[0] - s[1]) + t, t, k2mu, klam, M, H);
 }
 
 constexpr int kCells = 32;  // 2 x 4 x 4 cells of a page
@@ -491,7 +510,7 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
       }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      cache.S[d * CS + ks] = (T)s[d];
+      cache.S[d * CS + ks] = (T)(d == 2 && sizeof(T) == sizeof(float) ? s[1] + s[2] : s[d]);
       cache.frac[d * CS + ks] = (T)d0[d];
     }
     cache.K[ks] = (T)(2.0 * mu * vol);
@@ -1073,7 +1092,10 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
 #pragma unroll
           for (int c = 0; c < 3; ++c) M[r][c] = u[0][r] * T1[0][c] + u[1][r] * T1[1][c] + u[2][r] * T1[2][c];
         T H[3][3];
-        fcr_frame_differential(s, k2mu, klam, M, H);
+        if constexpr (sizeof(T) == sizeof(float))
+          fcr_frame_differential_t(s, k2mu, klam, M, H);
+        else
+          fcr_frame_differential(s, k2mu, klam, M, H);
         // ds = U H Q^T, stored times 1/dx^2 (the gather's and the scatter's 1/dx)
         T T2[3][3];
 #pragma unroll

@@ -17,23 +17,27 @@ namespace zr {
 // (low x bit, two low y bits, two low z bits) and the page code interleaves
 // page coordinates (px = x>>1, pz = z>>2, py = y>>2) as bits x,z,y,x,z,y,...
 // ---------------------------------------------------------------------------
+// Page coordinates have up to 11 bits (px of a 4096-node extent: 2048 pages);
+// the interleave is done in 64 bits and the code fits 31 bits.
 ZR_HD uint32_t spread3(uint32_t v)
 {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
+  uint64_t x = v & 0x7ffu;
+  x = (x | (x << 32)) & 0x001f00000000ffffull;
+  x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return (uint32_t)x;
 }
 ZR_HD uint32_t compact3(uint32_t v)
 {
-  v &= 0x09249249u;
-  v = (v ^ (v >> 2)) & 0x030C30C3u;
-  v = (v ^ (v >> 4)) & 0x0300F00Fu;
-  v = (v ^ (v >> 8)) & 0xFF0000FFu;
-  v = (v ^ (v >> 16)) & 0x000003FFu;
-  return v;
+  uint64_t x = v & 0x49249249u;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+  x = (x ^ (x >> 8)) & 0x001f0000ff0000ffull;
+  x = (x ^ (x >> 16)) & 0x001f00000000ffffull;
+  x = (x ^ (x >> 32)) & 0x00000000001fffffull;
+  return (uint32_t)x;
 }
 ZR_HD uint32_t page_encode(uint32_t px, uint32_t py, uint32_t pz)
 {

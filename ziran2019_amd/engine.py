@@ -202,19 +202,27 @@ class ZrPhaseField(ctypes.Structure):
 class ZrPhaseFieldOpts(ctypes.Structure):
     _fields_ = [("parabolic_M", ctypes.c_double), ("cg_max_iterations", ctypes.c_int32),
                 ("naive", ctypes.c_int32), ("cg_tolerance", ctypes.c_double), ("cg_relative", ctypes.c_double),
-                ("sigma_F", ctypes.c_double)]
+                ("sigma_F", ctypes.c_double), ("delete_threshold", ctypes.c_double)]
 
 
 class ZrPhaseFieldStats(ctypes.Structure):
     _fields_ = [("cg_iterations", ctypes.c_int32), ("reserved", ctypes.c_int32), ("residual_norm", ctypes.c_double)]
 
 
+# damage rules of zr_phase_field_opts.naive
+ZR_DAMAGE_PHASE_FIELD, ZR_DAMAGE_RANKINE, ZR_DAMAGE_PHASE_FIELD_RANKINE = 0, 1, 2
+
+
 def phase_field_opts(parabolic_M=0.0, cg_max_iterations=100, naive=False, cg_tolerance=1e-6, cg_relative=1e-12,
-                     sigma_F=0.0):
+                     sigma_F=0.0, delete_threshold=0.0):
     """FractureSimulation defaults (FractureSimulation.h:50-73,439;
-    fracture/ConjugateGradient.h:64)."""
-    return ZrPhaseFieldOpts(float(parabolic_M), int(cg_max_iterations), int(bool(naive)), float(cg_tolerance),
-                            float(cg_relative), float(sigma_F))
+    fracture/ConjugateGradient.h:64).  naive: False / ZR_DAMAGE_PHASE_FIELD,
+    True / ZR_DAMAGE_RANKINE (useNaiveDamage) or
+    ZR_DAMAGE_PHASE_FIELD_RANKINE (the phase-field solve, then the Rankine
+    cut of its c); delete_threshold: deleteParticles' threshold (0 deletes
+    nothing)."""
+    return ZrPhaseFieldOpts(float(parabolic_M), int(cg_max_iterations), int(naive), float(cg_tolerance),
+                            float(cg_relative), float(sigma_F), float(delete_threshold))
 
 
 def nhb_one_over_sigma_c(percentage, mu, lam):
