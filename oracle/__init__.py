@@ -193,6 +193,7 @@ def lib():
         L.zro_set_viscoelastic.argtypes = [_p, _p, _p, _p, _d, _d]
         L.zro_get_viscoelastic.argtypes = [_p, _p]
         L.zro_visco_relax.argtypes = [_d, _d, _d, _d, _d, _p, _p]
+        L.zro_admm_visco_map.argtypes = [_d, _d, _d, _d, _d, _p, _p, _p]
         L.zro_cam_clay_M.restype = _d
         L.zro_cam_clay_M.argtypes = [_d, _i]
         L.zro_cam_clay_project.restype = _i
@@ -343,6 +344,16 @@ def visco_relax(mu, lam, viscosity_d, viscosity_v, dt, gradV, Fneq):
     lib().zro_visco_relax(float(mu), float(lam), float(viscosity_d), float(viscosity_v), float(dt), _ptr(_cm(gradV)),
                           _ptr(f))
     return _from_cm(f)
+
+
+def admm_visco_map(mu, lam, viscosity_d, viscosity_v, dt, z):
+    """The ADMM local step's viscoelastic return map: (relaxed singular values,
+    the derivative the reference forms, row-major)."""
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    zp, D = np.zeros(3), np.zeros((3, 3))
+    lib().zro_admm_visco_map(float(mu), float(lam), float(viscosity_d), float(viscosity_v), float(dt), _ptr(z),
+                             _ptr(zp), _ptr(D))
+    return zp, D
 
 
 def cam_clay_M(friction_angle_deg, dim=3):

@@ -252,6 +252,11 @@ typedef struct {
 } zro_admm_stats;
 
 int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats* st);
+/* The viscoelastic branch's relaxation map in the ADMM local step
+   (AdmmSimulationFoam.h:583-603): relaxed singular values zp of z and the
+   derivative D the reference forms (row-major). */
+void zro_admm_visco_map(double mu, double lam, double visc_d, double visc_v, double dt, const double* z, double* zp,
+                        double D[3][3]);
 /* The ADMM local step's plastic projection of singular values, projectSigma
    (PlasticityApplier.cpp:319-346 Drucker-Prager, :526-547 von Mises; other
    plasticity: identity), and projectSigmaAndDerivative (:389-423, :580-606):

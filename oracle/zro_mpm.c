@@ -130,6 +130,7 @@ struct zro_sim {
      per node slot vn, dv, collision projection, preconditioner diagonal */
   double *admm_F, *admm_y, *admm_omega, *admm_last_dv;
   int64_t admm_n;
+  int admm_visco;
   int admm_steps;
   double *admm_vn, *admm_dv, *admm_proj, *admm_diag;
   char* admm_coll;
@@ -2669,6 +2670,76 @@ done:
   return it;
 }
 
+/* 3x3 inverse by cofactors */
+static void inv3(const double P[3][3], double Pi[3][3])
+{
+  double det = P[0][0] * (P[1][1] * P[2][2] - P[2][1] * P[1][2]) - P[0][1] * (P[1][0] * P[2][2] - P[2][0] * P[1][2]) +
+               P[0][2] * (P[1][0] * P[2][1] - P[2][0] * P[1][1]);
+  Pi[0][0] = (P[1][1] * P[2][2] - P[1][2] * P[2][1]) / det;
+  Pi[0][1] = (P[0][2] * P[2][1] - P[0][1] * P[2][2]) / det;
+  Pi[0][2] = (P[0][1] * P[1][2] - P[0][2] * P[1][1]) / det;
+  Pi[1][0] = (P[1][2] * P[2][0] - P[1][0] * P[2][2]) / det;
+  Pi[1][1] = (P[0][0] * P[2][2] - P[0][2] * P[2][0]) / det;
+  Pi[1][2] = (P[0][2] * P[1][0] - P[0][0] * P[1][2]) / det;
+  Pi[2][0] = (P[1][0] * P[2][1] - P[1][1] * P[2][0]) / det;
+  Pi[2][1] = (P[0][1] * P[2][0] - P[0][0] * P[2][1]) / det;
+  Pi[2][2] = (P[0][0] * P[1][1] - P[0][1] * P[1][0]) / det;
+}
+
+/* FStepSolve's viscoelastic branch (AdmmSimulationFoam.h:566-611): F2 = U
+   diag(z) V^T with z the Newton solution (at most 20 iterations) of
+   vol dE(zp(z)) + rho omega2^2 (z - sigma) = 0, zp the relaxation map of
+   zro_admm_visco_map with the branch's StvkWithHencky model, sigma the
+   singular values of FFn - y2 / (rho omega2) + dt G FFn; G is the particle's
+   grad(vn + dv). */
+static void admm_visco_fstep(zro_sim* s, int64_t i, const double G[3][3], double dt, double rho, double local_tol)
+{
+  const int64_t j = s->n + i;
+  double Fv[3][3], FD[3][3], Fa[3][3];
+  mat_from_cm(s->fneq + 9 * i, Fv);
+  mat_from_cm(s->admm_F + 9 * j, Fa);
+  double om = s->admm_omega[j];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      FD[r][c] = Fv[r][c] - s->admm_y[9 * j + c * 3 + r] / (om * rho) +
+                 dt * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+  double fdc[9], U[9], sg[3], V[9], Um[3][3], Vm[3][3], zi[3];
+  mat_to_cm(FD, fdc);
+  zro_svd3(fdc, U, sg, V);
+  mat_from_cm(U, Um);
+  mat_from_cm(V, Vm);
+  for (int d = 0; d < 3; ++d) {
+    double acc = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) acc += Um[a][d] * Fa[a][b] * Vm[b][d];
+    zi[d] = acc;
+  }
+  const double mu = s->mu_neq[i], lam = s->lam_neq[i], coeff = rho * om * om, vol = s->vol[i];
+  for (int it = 0; it < 20; ++it) {
+    double zp[3], D[3][3], dE[3], ddE[3][3], g[3], P[3][3], Pi[3][3], step[3];
+    zro_admm_visco_map(mu, lam, s->visc_d, s->visc_v, dt, zi, zp, D);
+    hencky_dE(mu, lam, zp, dE);
+    hencky_ddE(mu, lam, zp, ddE);
+    for (int a = 0; a < 3; ++a) {
+      g[a] = vol * dE[a] + coeff * (zi[a] - sg[a]);
+      for (int b = 0; b < 3; ++b)
+        P[a][b] = vol * (ddE[a][0] * D[0][b] + ddE[a][1] * D[1][b] + ddE[a][2] * D[2][b]) + (a == b ? coeff : 0.0);
+    }
+    inv3(P, Pi);
+    double sn = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      step[a] = -(Pi[a][0] * g[0] + Pi[a][1] * g[1] + Pi[a][2] * g[2]);
+      sn += step[a] * step[a];
+    }
+    for (int a = 0; a < 3; ++a) zi[a] += step[a];
+    if (sqrt(sn) < local_tol) break;
+  }
+  double Fo[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Fo[r][c] = Um[r][0] * zi[0] * Vm[c][0] + Um[r][1] * zi[1] * Vm[c][1] + Um[r][2] * zi[2] * Vm[c][2];
+  mat_to_cm(Fo, s->admm_F + 9 * j);
+}
+
 static double* admm_grow(double* p, int64_t n) { free(p); return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
 
 int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats* st)
@@ -2688,13 +2759,23 @@ int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats*
     s->admm_coll = (char*)calloc((size_t)ns, 1);
     s->admm_cap = ns;
   }
-  int first = s->admm_steps == 0 || s->admm_n != n;
-  if (s->admm_n != n) {
-    s->admm_F = admm_grow(s->admm_F, 9 * n);
-    s->admm_y = admm_grow(s->admm_y, 9 * n);
-    s->admm_omega = admm_grow(s->admm_omega, n);
+  /* the viscoelastic branch (enable_visco) keeps a second F, y and omega per
+     particle at i + n, as the reference's arrays of 2 * particles.count; its
+     elastic branch is StvkWithHenckyIsotropic without plasticity
+     (use_elasticity_plasticity = false) */
+  const int visco = s->fneq != NULL;
+  if (visco && s->plastic)
+    for (int64_t i = 0; i < n; ++i)
+      if (s->plastic[i] != ZRO_PLASTIC_NONE) return -2;
+  const int64_t nb = visco ? 2 * n : n;
+  int first = s->admm_steps == 0 || s->admm_n != n || s->admm_visco != visco;
+  if (s->admm_n != n || s->admm_visco != visco) {
+    s->admm_F = admm_grow(s->admm_F, 9 * nb);
+    s->admm_y = admm_grow(s->admm_y, 9 * nb);
+    s->admm_omega = admm_grow(s->admm_omega, nb);
     s->admm_last_dv = admm_grow(s->admm_last_dv, 3 * n);
     s->admm_n = n;
+    s->admm_visco = visco;
   }
   /* initStep (:296-401) */
   memset(s->admm_vn, 0, sizeof(double) * 3 * ns);
@@ -2740,6 +2821,19 @@ int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats*
     double old = first ? 0.0 : s->admm_omega[i];
     for (int c = 0; c < 9; ++c) s->admm_y[9 * i + c] = first ? 0.0 : s->admm_y[9 * i + c] * old / om;
     s->admm_omega[i] = om;
+    if (visco) { /* F2 = (I + dt grad(vn + dv)) FFn, its omega and y (:372-400) */
+      const int64_t j = n + i;
+      double Fv[3][3], F2[3][3];
+      mat_from_cm(s->fneq + 9 * i, Fv);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) F2[r][c] = Fv[r][c] + dt * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+      mat_to_cm(F2, s->admm_F + 9 * j);
+      zro_svd3(s->admm_F + 9 * j, U, sg, V);
+      double om2 = admm_visco_omega_of(s, i, sg);
+      double old2 = first ? 0.0 : s->admm_omega[j];
+      for (int c = 0; c < 9; ++c) s->admm_y[9 * j + c] = first ? 0.0 : s->admm_y[9 * j + c] * old2 / om2;
+      s->admm_omega[j] = om2;
+    }
   }
   /* preconditioner diagonal */
   for (int64_t k = 0; k < ns; ++k) s->admm_diag[k] = s->nodes[k].idx >= 0 ? s->nodes[k].m : 0.0;
@@ -2759,6 +2853,12 @@ int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats*
           if (s->nodes[k].idx < 0) continue;
           for (int d = 0; d < 3; ++d) fg[d] = Fn[0][d] * gw[0] + Fn[1][d] * gw[1] + Fn[2][d] * gw[2];
           s->admm_diag[k] += cp * (fg[0] * fg[0] + fg[1] * fg[1] + fg[2] * fg[2]);
+          if (visco) { /* :488-497 */
+            double Fv[3][3], om2 = s->admm_omega[n + i];
+            mat_from_cm(s->fneq + 9 * (int64_t)i, Fv);
+            for (int d = 0; d < 3; ++d) fg[d] = Fv[0][d] * gw[0] + Fv[1][d] * gw[1] + Fv[2][d] * gw[2];
+            s->admm_diag[k] += rho * om2 * om2 * dt * dt * (fg[0] * fg[0] + fg[1] * fg[1] + fg[2] * fg[2]);
+          }
         }
   }
   double* rhs = (double*)calloc((size_t)(3 * ns), sizeof(double));
@@ -2823,6 +2923,7 @@ int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats*
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) Fo[r][c] = Um[r][0] * zi[0] * Vm[c][0] + Um[r][1] * zi[1] * Vm[c][1] + Um[r][2] * zi[2] * Vm[c][2];
       mat_to_cm(Fo, s->admm_F + 9 * i);
+      if (visco) admm_visco_fstep(s, i, G, dt, rho, o->local_tolerance);
     }
     /* dvStep (:163-220) */
     memset(rhs, 0, sizeof(double) * 3 * ns);
@@ -2842,6 +2943,18 @@ int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats*
           bF[r][c] = dt * rho * om * om * fbu;
         }
       admm_scatter(s, i, bF, Fn, rhs);
+      if (visco) { /* :183-201 */
+        const int64_t j = n + i;
+        double Fv[3][3], om2 = s->admm_omega[j];
+        mat_from_cm(s->fneq + 9 * (int64_t)i, Fv);
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            double fbu = s->admm_F[9 * j + c * 3 + r] - Fv[r][c] + s->admm_y[9 * j + c * 3 + r] / (om2 * rho) -
+                         dt * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+            bF[r][c] = dt * rho * om2 * om2 * fbu;
+          }
+        admm_scatter(s, i, bF, Fv, rhs);
+      }
     }
     double rel = 0.5;  /* min(0.5, sqrt(max(residual = 1, 1e-6))) (:785-787) */
     int its = admm_minres(s, dt, rho, s->admm_dv, rhs, o->global_tolerance, rel, o->minres_max_iterations);
@@ -2858,6 +2971,16 @@ int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats*
           double t = s->admm_F[9 * i + c * 3 + r] - Fn[r][c] - dt * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
           s->admm_y[9 * i + c * 3 + r] += rho * om * t;
         }
+      if (visco) { /* :661-668 */
+        const int64_t j = n + i;
+        double Fv[3][3], om2 = s->admm_omega[j];
+        mat_from_cm(s->fneq + 9 * i, Fv);
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            double t = s->admm_F[9 * j + c * 3 + r] - Fv[r][c] - dt * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+            s->admm_y[9 * j + c * 3 + r] += rho * om2 * t;
+          }
+      }
     }
   }
   /* constructNewVelocityFromNewtonResult and last_dv (:789-800) */
@@ -2884,6 +3007,9 @@ int zro_step_admm(zro_sim* s, double dt, const zro_admm_opts* o, zro_admm_stats*
   free(nv);
   s->admm_steps++;
   int flags = zro_g2p_apic(s, dt);
+  if (visco) /* gridToParticles' evolveStrain of the viscoelastic helper, with gradV */
+    for (int64_t i = 0; i < n; ++i)
+      zro_visco_relax(s->mu_neq[i], s->lam_neq[i], s->visc_d, s->visc_v, dt, s->gradV + 9 * i, s->fneq + 9 * i);
   s->time += dt;
   return flags;
 }

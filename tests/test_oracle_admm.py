@@ -168,3 +168,131 @@ def test_plastic_admm_converges_to_projected_stress_balance(plastic):
     res = [_balance_residual(s, k, plastic=plastic) for k in (5, 40, 300)]
     assert res[0] > res[1] > res[2]
     assert res[2] < 1e-9
+
+
+VISCO = dict(viscosity_d=0.05, viscosity_v=0.2)
+
+
+@pytest.mark.parametrize("z", [(1.2, 1.0, 0.9), (1.05, 0.97, -0.8), (0.7, 0.6, 0.5)])
+def test_admm_visco_map_is_the_helper_relaxation(z):
+    """The ADMM local step's viscoelastic map (AdmmSimulationFoam.h:583-603)
+    relaxes the singular values as the viscoelastic helper's evolveStrain
+    does (FElasticNonequilibratedBasedMpmForceHelper.h:62-87, pinned in
+    test_oracle_visco.py): on Fneq = diag(z) with no velocity gradient the
+    helper returns diag(zp)."""
+    mu, lam, dt = 300.0, 500.0, 1e-3
+    zp, _ = oracle.admm_visco_map(mu, lam, VISCO["viscosity_d"], VISCO["viscosity_v"], dt, np.array(z))
+    out = oracle.visco_relax(mu, lam, VISCO["viscosity_d"], VISCO["viscosity_v"], dt, np.zeros((3, 3)),
+                             np.diag(np.abs(z)))
+    assert np.allclose(np.diag(out), zp, rtol=1e-14, atol=0)
+    assert np.allclose(out - np.diag(np.diag(out)), 0.0, atol=1e-15)
+
+
+def test_admm_visco_map_derivative():
+    """The map's derivative as the reference forms it: the Jacobian of the
+    map with the full sum of log|z| in place of its first entry, each row
+    scaled by zp / zp_full (central differences of that map here)."""
+    mu, lam, dt = 300.0, 500.0, 1e-3
+    vd, vv = VISCO["viscosity_d"], VISCO["viscosity_v"]
+    alpha = 2 * mu / vd
+    beta = 2 * (2 * mu + 3 * lam) / (9 * vv) - 2 * mu / (3 * vd)
+    c = dt * beta / (1 + dt * (alpha + 3 * beta))
+    h = 1 / (1 + dt * alpha)
+
+    def full(z):
+        e = np.log(np.abs(z))
+        return np.exp(h * (e - c * e.sum()))
+    z = np.array([1.15, 0.95, 0.85])
+    zp, D = oracle.admm_visco_map(mu, lam, vd, vv, dt, z)
+    J = np.zeros((3, 3))
+    for j in range(3):
+        e = np.zeros(3)
+        e[j] = 1e-6
+        J[:, j] = (full(z + e) - full(z - e)) / 2e-6
+    assert np.allclose(D, (zp / full(z))[:, None] * J, rtol=1e-8, atol=0)
+
+
+def _visco_balance_residual(s, iterations, dt=1e-3):
+    """Balance residual of an ADMM step with the viscoelastic branch: the
+    converged grid velocity solves
+        m_i (v_i - vn_i) - dt m_i g + dt sum_p V0 (P1 Fn^T + P2 FFn^T) grad w_ip = 0
+    with P1 the Hencky stress of F1 = (I + dt G) Fn and P2 the branch's
+    Hencky stress of U diag(zp) V^T, F2 = (I + dt G) FFn = U diag(z) V^T and
+    zp the relaxation map of z."""
+    n = len(s["mass"])
+    sim = oracle.OracleSim(dx=s["dx"], gravity=(0.0, -9.8, 0.0), nthreads=4)
+    sim.set_particles(s["X"], s["V"], s["C"], s["F"], s["mass"], s["vol"], s["mu"], s["lam"])
+    sim.set_materials(model=1)
+    sim.set_viscoelastic(s["Fneq"], s["mu2"], s["lam2"], VISCO["viscosity_d"], VISCO["viscosity_v"])
+    flags, _ = sim.step_admm(dt, oracle.admm_opts(admm_iterations=iterations, global_tolerance=1e-14))
+    assert flags >= 0
+    g = sim.grid()
+    info = sim.sort_result()
+    coords = {}
+    for p, base in enumerate(info["pages"]):
+        for e in range(32):
+            coords[tuple(oracle.linear_to_coord(int(base) + (e << 7)))] = p * 32 + e
+    active = g["idx"] >= 0
+    v, vn, m = g["new_v"], g["v"], g["m"]
+    Fn = s["F"].reshape(n, 3, 3).transpose(0, 2, 1)
+    Fv = s["Fneq"].reshape(n, 3, 3).transpose(0, 2, 1)
+    force = np.zeros((len(m), 3))
+    for p in range(n):
+        bw = [_bspline(s["X"][p, d], s["dx"]) for d in range(3)]
+        ks, gs = [], []
+        for a in range(3):
+            for b in range(3):
+                for c in range(3):
+                    k = coords[(bw[0][0] + a, bw[1][0] + b, bw[2][0] + c)]
+                    if active[k]:
+                        ks.append(k)
+                        gs.append(np.array([bw[0][2][a] * bw[1][1][b] * bw[2][1][c],
+                                            bw[0][1][a] * bw[1][2][b] * bw[2][1][c],
+                                            bw[0][1][a] * bw[1][1][b] * bw[2][2][c]]))
+        G = sum(np.outer(v[k], gw) for k, gw in zip(ks, gs))
+        F1 = (np.eye(3) + dt * G) @ Fn[p]
+        F2 = (np.eye(3) + dt * G) @ Fv[p]
+        U, z, Vt = np.linalg.svd(F2)
+        zp, _ = oracle.admm_visco_map(s["mu2"][p], s["lam2"][p], VISCO["viscosity_d"], VISCO["viscosity_v"], dt, z)
+        P1 = oracle.stvk_hencky_first_piola(s["mu"][p], s["lam"][p], F1)
+        P2 = oracle.stvk_hencky_first_piola(s["mu2"][p], s["lam2"][p], U @ np.diag(zp) @ Vt)
+        S = s["vol"][p] * (P1 @ Fn[p].T + P2 @ Fv[p].T)
+        for k, gw in zip(ks, gs):
+            force[k] += dt * S @ gw
+    r = m[:, None] * (v - vn) - dt * m[:, None] * np.array([0.0, -9.8, 0.0]) + force
+    return np.abs(r[active]).max() / np.abs(force[active]).max()
+
+
+def _visco_scene(seed=5):
+    s = _scene(seed)
+    n = len(s["mass"])
+    rng = np.random.default_rng(seed + 100)
+    Fv = np.tile(np.eye(3), (n, 1, 1)) + 0.05 * rng.normal(size=(n, 3, 3))
+    s["Fneq"] = np.ascontiguousarray(Fv.transpose(0, 2, 1).reshape(n, 9))
+    mu2, lam2 = oracle.lame(5e3, 0.3)
+    s["mu2"], s["lam2"] = np.full(n, mu2), np.full(n, lam2)
+    return s
+
+
+def test_admm_visco_converges_to_backward_euler_balance():
+    """AdmmSimulationFoam.h's enable_visco branch: the second F per particle
+    (F2 from Fe_Nonequilibrated) through initStep, FStepSolve, dvStep and
+    uStep; the ADMM fixed point is the backward-Euler balance of both
+    branches' stresses."""
+    s = _visco_scene()
+    res = [_visco_balance_residual(s, k) for k in (5, 40, 300)]
+    assert res[0] > res[1] > res[2]
+    assert res[2] < 1e-10
+
+
+def test_admm_visco_rejects_plasticity():
+    """The reference's viscoelastic ADMM runs with use_elasticity_plasticity
+    off: a plastic particle beside the branch is refused."""
+    s = _visco_scene()
+    n = len(s["mass"])
+    sim = oracle.OracleSim(dx=s["dx"], gravity=(0.0, -9.8, 0.0), nthreads=2)
+    sim.set_particles(s["X"], s["V"], s["C"], s["F"], s["mass"], s["vol"], s["mu"], s["lam"])
+    sim.set_materials(model=1, plastic=np.full(n, VM[0]), pparam=VM[1], pstate=VM[2])
+    sim.set_viscoelastic(s["Fneq"], s["mu2"], s["lam2"], VISCO["viscosity_d"], VISCO["viscosity_v"])
+    flags, _ = sim.step_admm(1e-3, oracle.admm_opts(admm_iterations=2))
+    assert flags == -2

@@ -1,7 +1,8 @@
 // zr_admm.hpp -- the ADMM step of Projects/admm/AdmmSimulationFoam.h
 // (StvkWithHencky elasticity with no / von Mises / Drucker-Prager plasticity,
-// no viscoelastic branch) for the gfx950 engine.  Included by zr_engine.hip
-// inside its anonymous namespace after zr_adjoint.hpp.
+// or elastic with the viscoelastic branch, enable_visco) for the gfx950
+// engine.  Included by zr_engine.hip inside its anonymous namespace after
+// zr_adjoint.hpp.
 //
 // Per step: APIC P2G; initStep (warm-started dv, collision projection,
 // F = (I + dt grad(vn + dv)) Fn, omega from the PD-clamped Hencky Hessians,
@@ -19,6 +20,16 @@ struct AdmmParticles {
   double* y;       // [9][n] scaled dual
   double* om;      // [n] omega
   double* last;    // [3][n] last dv per particle
+  // the viscoelastic branch (the reference's entries i + particles.count):
+  // its ADMM F, dual and omega, the particle's Fe_Nonequilibrated (FFn) and
+  // the branch's StvkWithHencky model and viscosities
+  double* F2;      // [9][n]
+  double* y2;      // [9][n]
+  double* om2;     // [n]
+  const double* fneq;
+  const double* mu2;
+  const double* lam2;
+  double visc_d, visc_v;
 };
 
 struct AdmmNodes {
@@ -252,6 +263,54 @@ __device__ __forceinline__ double admm_omega(const PlasticParams& q, double mu, 
   return sqrt(vol * sqrt(total));
 }
 
+// initOmega for the viscoelastic branch (:254-270): its StvkWithHencky model,
+// no projection
+__device__ __forceinline__ double admm_visco_omega(double mu, double lam, double vol, const double s[3])
+{
+  const PlasticParams none{ ZR_PLASTIC_NONE, 0.0, 0.0, 0.0 };
+  return admm_omega(none, mu, lam, vol, s);
+}
+
+// the viscoelastic branch's return map in FStepSolve (:583-603): relaxed
+// singular values zp = exp((log|z| - c t 1) / (1 + dt alpha)),
+// c = dt beta / (1 + dt (alpha + 3 beta)), t = log|z_0| (Eigen's trace() of
+// the 3x1 log vector is its first entry), and the derivative the reference
+// forms, D = diag(zp) (diag(1/z) - c 1 (1/z)^T) / (1 + dt alpha)
+__device__ __forceinline__ void admm_visco_map(double mu, double lam, double visc_d, double visc_v, double dt,
+                                               const double z[3], double zp[3], double D[3][3])
+{
+  const double alpha = 2.0 * mu / visc_d;
+  const double beta = 2.0 * (2.0 * mu + lam * 3.0) / (9.0 * visc_v) - 2.0 * mu / (visc_d * 3.0);
+  const double c = dt * beta / (1.0 + dt * (alpha + 3.0 * beta));
+  const double h = 1.0 / (1.0 + dt * alpha);
+  double e[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) e[d] = log(fabs(z[d]));
+  const double t = e[0];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) zp[d] = exp(h * (e[d] - c * t));
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) D[a][b] = zp[a] * h * ((a == b ? 1.0 / z[b] : 0.0) - c / z[b]);
+}
+
+__device__ __forceinline__ void inv3(const double H[3][3], double Hi[3][3])
+{
+  const double det = H[0][0] * (H[1][1] * H[2][2] - H[2][1] * H[1][2]) -
+                     H[0][1] * (H[1][0] * H[2][2] - H[2][0] * H[1][2]) +
+                     H[0][2] * (H[1][0] * H[2][1] - H[2][0] * H[1][1]);
+  Hi[0][0] = (H[1][1] * H[2][2] - H[1][2] * H[2][1]) / det;
+  Hi[0][1] = (H[0][2] * H[2][1] - H[0][1] * H[2][2]) / det;
+  Hi[0][2] = (H[0][1] * H[1][2] - H[0][2] * H[1][1]) / det;
+  Hi[1][0] = (H[1][2] * H[2][0] - H[1][0] * H[2][2]) / det;
+  Hi[1][1] = (H[0][0] * H[2][2] - H[0][2] * H[2][0]) / det;
+  Hi[1][2] = (H[0][2] * H[1][0] - H[0][0] * H[1][2]) / det;
+  Hi[2][0] = (H[1][0] * H[2][1] - H[1][1] * H[2][0]) / det;
+  Hi[2][1] = (H[0][1] * H[2][0] - H[0][0] * H[2][1]) / det;
+  Hi[2][2] = (H[0][0] * H[1][1] - H[0][1] * H[1][0]) / det;
+}
+
 // multiObjectCollisionWithFakeSeparate (CollisionObject.cpp:108-160) in the
 // frame at the step's start.
 //
@@ -423,8 +482,74 @@ __device__ __forceinline__ void load_F(const double* F, int64_t S, uint32_t i, d
     for (int r = 0; r < 3; ++r) M[r][c] = F[(c * 3 + r) * S + i];
 }
 
+// FStepSolve's viscoelastic branch (:566-611): F2 = U diag(z) V^T with z the
+// Newton solution (at most 20 iterations) of
+// vol dE(zp(z)) + rho omega2^2 (z - sigma) = 0, sigma the singular values of
+// FFn - y2 / (rho omega2) + dt G FFn, G = grad(vn + dv)
+__device__ __forceinline__ void admm_visco_fstep(const AdmmParticles& A, int64_t S, uint32_t i, double vol,
+                                                 const double G[3][3], double dt, double rho, double local_tol)
+{
+  double Fv[3][3];
+  load_F(A.fneq, S, i, Fv);
+  const double om = A.om2[i];
+  M3<double> fd, u, v;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      fd.a[r][c] = Fv[r][c] - A.y2[(c * 3 + r) * S + i] / (om * rho) +
+                   dt * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+  double sg[3];
+  svd3(fd, u, sg, v);
+  double z[3];
+  {
+    double Fa[3][3];
+    load_F(A.F2, S, i, Fa);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double acc = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) acc += u.a[a][d] * Fa[a][bb] * v.a[bb][d];
+      z[d] = acc;
+    }
+  }
+  const double mu = A.mu2[i], lam = A.lam2[i], coeff = rho * om * om;
+  for (int it = 0; it < 20; ++it) {
+    double zp[3], D[3][3], dE[3], ddE[3][3], gg[3], H[3][3], Hi[3][3];
+    admm_visco_map(mu, lam, A.visc_d, A.visc_v, dt, z, zp, D);
+    hencky_dE(mu, lam, zp, dE);
+    hencky_ddE(mu, lam, zp, ddE);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      gg[a] = vol * dE[a] + coeff * (z[a] - sg[a]);
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+        H[a][bb] = vol * (ddE[a][0] * D[0][bb] + ddE[a][1] * D[1][bb] + ddE[a][2] * D[2][bb]) + (a == bb ? coeff : 0.0);
+    }
+    inv3(H, Hi);
+    double sn = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double st = -(Hi[a][0] * gg[0] + Hi[a][1] * gg[1] + Hi[a][2] * gg[2]);
+      z[a] += st;
+      sn += st * st;
+    }
+    if (sqrt(sn) < local_tol) break;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      A.F2[(c * 3 + r) * S + i] = u.a[r][0] * z[0] * v.a[c][0] + u.a[r][1] * z[1] * v.a[c][1] + u.a[r][2] * z[2] * v.a[c][2];
+}
+
+
 // initStep F and omega (:359-377), the y rescale, and the preconditioner
-// scatter c_p |Fn^T grad w|^2 (:480-500), one staged field
+// scatter c_p |Fn^T grad w|^2 (:480-500), one staged field; VISCO: the same
+// for the viscoelastic branch's F2 from FFn (:372-400, 488-497)
+template <bool VISCO>
 __global__ __launch_bounds__(256) void k_admm_init_particles(ParticlesView P, AdmmParticles A, AdmmNodes N,
                                                              const int32_t* __restrict__ mat,
                                                              const double* __restrict__ pparam,
@@ -482,6 +607,26 @@ __global__ __launch_bounds__(256) void k_admm_init_particles(ParticlesView P, Ad
     for (int c = 0; c < 9; ++c) A.y[c * S + i] = first ? 0.0 : A.y[c * S + i] * old / om;
     A.om[i] = om;
     const double cp = rho * om * om * dt * dt;
+    double Fv[3][3], cp2 = 0.0;
+    if constexpr (VISCO) {
+      load_F(A.fneq, S, i, Fv);
+      M3<double> f2, u2, v2;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          f2.a[r][c] = Fv[r][c] + dt * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+          A.F2[(c * 3 + r) * S + i] = f2.a[r][c];
+        }
+      double s2[3];
+      svd3(f2, u2, s2, v2);
+      const double om2 = admm_visco_omega(A.mu2[i], A.lam2[i], P.vol[i], s2);
+      const double old2 = first ? 0.0 : A.om2[i];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) A.y2[c * S + i] = first ? 0.0 : A.y2[c * S + i] * old2 / om2;
+      A.om2[i] = om2;
+      cp2 = rho * om2 * om2 * dt * dt;
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -493,7 +638,13 @@ __global__ __launch_bounds__(256) void k_admm_init_particles(ParticlesView P, Ad
 #pragma unroll
           for (int d = 0; d < 3; ++d) fg[d] = Fn[0][d] * gw[0] + Fn[1][d] * gw[1] + Fn[2][d] * gw[2];
           const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
-          atomicAdd(&wt[t], cp * (fg[0] * fg[0] + fg[1] * fg[1] + fg[2] * fg[2]));
+          double val = cp * (fg[0] * fg[0] + fg[1] * fg[1] + fg[2] * fg[2]);
+          if constexpr (VISCO) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fg[d] = Fv[0][d] * gw[0] + Fv[1][d] * gw[1] + Fv[2][d] * gw[2];
+            val += cp2 * (fg[0] * fg[0] + fg[1] * fg[1] + fg[2] * fg[2]);
+          }
+          atomicAdd(&wt[t], val);
         }
   }
   __syncthreads();
@@ -521,7 +672,9 @@ __global__ __launch_bounds__(256) void k_admm_diag(uint32_t nn, const int32_t* _
 
 // FStepSolve (:507-615): F = U diag(z) V^T with z the Newton solution of
 // vol dE(proj(z)) + rho omega^2 (z - sigma) = 0, sigma the singular values
-// of Fn - y/(rho omega) + dt grad(vn + dv) Fn
+// of Fn - y/(rho omega) + dt grad(vn + dv) Fn; VISCO: then the viscoelastic
+// branch's F2
+template <bool VISCO>
 __global__ __launch_bounds__(256) void k_admm_fstep(ParticlesView P, AdmmParticles A, AdmmNodes N,
                                                     const int32_t* __restrict__ mat,
                                                     const double* __restrict__ pparam,
@@ -621,12 +774,14 @@ __global__ __launch_bounds__(256) void k_admm_fstep(ParticlesView P, AdmmParticl
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         A.F[(c * 3 + r) * S + i] = u.a[r][0] * z[0] * v.a[c][0] + u.a[r][1] * z[1] * v.a[c][1] + u.a[r][2] * z[2] * v.a[c][2];
+    if constexpr (VISCO) admm_visco_fstep(A, S, i, vol, G, dt, rho, local_tol);
   }
 }
 
 // dvStep rhs scatter (:167-205): dt rho omega^2 (F - Fn + y/(rho omega) - dt grad(vn) Fn) Fn^T grad w
-// MODE 0; the operator's particle term c_p (grad(x) Fn) Fn^T grad w: MODE 1
-template <int MODE>
+// MODE 0; the operator's particle term c_p (grad(x) Fn) Fn^T grad w: MODE 1;
+// VISCO adds the viscoelastic branch's terms with F2, y2, omega2 and FFn
+template <int MODE, bool VISCO>
 __global__ __launch_bounds__(256) void k_admm_scatter(ParticlesView P, AdmmParticles A, AdmmNodes N,
                                                       const double* __restrict__ xvec,
                                                       const uint32_t* __restrict__ order,
@@ -679,6 +834,22 @@ __global__ __launch_bounds__(256) void k_admm_scatter(ParticlesView P, AdmmParti
         else
           Y[r][c] = c2 * gf;
       }
+    double Fv[3][3], Y2[3][3];
+    if constexpr (VISCO) {
+      load_F(A.fneq, S, i, Fv);
+      const double om2 = A.om2[i];
+      const double c22 = MODE == 0 ? dt * rho * om2 * om2 : rho * om2 * om2 * dt * dt;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double gf = G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c];
+          if (MODE == 0)
+            Y2[r][c] = c22 * (A.F2[(c * 3 + r) * S + i] - Fv[r][c] + A.y2[(c * 3 + r) * S + i] / (om2 * rho) - dt * gf);
+          else
+            Y2[r][c] = c22 * gf;
+        }
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -690,8 +861,19 @@ __global__ __launch_bounds__(256) void k_admm_scatter(ParticlesView P, AdmmParti
 #pragma unroll
           for (int d = 0; d < 3; ++d) fg[d] = Fn[0][d] * gw[0] + Fn[1][d] * gw[1] + Fn[2][d] * gw[2];
           const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+          if constexpr (VISCO) {
+            double fg2[3];
 #pragma unroll
-          for (int r = 0; r < 3; ++r) atomicAdd(&wt[r * kTileNodes + t], Y[r][0] * fg[0] + Y[r][1] * fg[1] + Y[r][2] * fg[2]);
+            for (int d = 0; d < 3; ++d) fg2[d] = Fv[0][d] * gw[0] + Fv[1][d] * gw[1] + Fv[2][d] * gw[2];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+              atomicAdd(&wt[r * kTileNodes + t], (Y[r][0] * fg[0] + Y[r][1] * fg[1] + Y[r][2] * fg[2]) +
+                                                     (Y2[r][0] * fg2[0] + Y2[r][1] * fg2[1] + Y2[r][2] * fg2[2]));
+          } else {
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+              atomicAdd(&wt[r * kTileNodes + t], Y[r][0] * fg[0] + Y[r][1] * fg[1] + Y[r][2] * fg[2]);
+          }
         }
   }
   __syncthreads();
@@ -858,7 +1040,9 @@ __global__ __launch_bounds__(256) void k_minres_scale(uint32_t nn, const double*
 }
 
 
-// uStep (:638-671): y += rho omega (F - Fn - dt grad(vn + dv) Fn)
+// uStep (:638-671): y += rho omega (F - Fn - dt grad(vn + dv) Fn); VISCO
+// also y2 += rho omega2 (F2 - FFn - dt grad(vn + dv) FFn)
+template <bool VISCO>
 __global__ __launch_bounds__(256) void k_admm_ustep(ParticlesView P, AdmmParticles A, AdmmNodes N,
                                                     const uint32_t* __restrict__ order,
                                                     const uint32_t* __restrict__ group_start,
@@ -898,6 +1082,19 @@ __global__ __launch_bounds__(256) void k_admm_ustep(ParticlesView P, AdmmParticl
                          dt * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
         A.y[(c * 3 + r) * S + i] += rho * om * t;
       }
+    if constexpr (VISCO) {
+      double Fv[3][3];
+      load_F(A.fneq, S, i, Fv);
+      const double om2 = A.om2[i];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double t = A.F2[(c * 3 + r) * S + i] - Fv[r][c] -
+                           dt * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+          A.y2[(c * 3 + r) * S + i] += rho * om2 * t;
+        }
+    }
   }
 }
 

@@ -413,13 +413,19 @@ struct ParticlesView {
   const int32_t* mat;   // model | plasticity << 4 per particle; null: all fixed-corotated
   const double* gdeg;   // NeoHookeanBorden degradation g
   const double* vtau_add;  // the viscoelastic branch's V0 tau, added to the stress (null: none)
+  double* gradv;           // the APIC G2P writes grad v here when set (9 rows of n)
 };
 
-// constitutive_model.kirchhoff of the particle's model (FBasedMpmForceHelper.cpp:44-64)
+// constitutive_model.kirchhoff of the particle's model (FBasedMpmForceHelper.cpp:44-64).
+// MODEL >= 0: every particle has that model (the kernel instance holds only
+// its stress, so a NeoHookeanBorden scene does not pay the SVD's registers);
+// kAnyModel: the model code of each particle.
+constexpr int kAnyModel = -1;
+template <int MODEL>
 __device__ __forceinline__ void particle_kirchhoff(const ParticlesView& P, uint32_t i, const M3<double>& f,
                                                    M3<double>& tau)
 {
-  const int model = P.mat ? (P.mat[i] & 15) : ZR_MODEL_FIXED_COROTATED;
+  const int model = MODEL != kAnyModel ? MODEL : (P.mat ? (P.mat[i] & 15) : ZR_MODEL_FIXED_COROTATED);
   if (model == ZR_MODEL_STVK_HENCKY)
     hencky_kirchhoff(P.mu[i], P.lam[i], f, tau);
   else if (model == ZR_MODEL_NEO_HOOKEAN_BORDEN)
@@ -430,7 +436,7 @@ __device__ __forceinline__ void particle_kirchhoff(const ParticlesView& P, uint3
 
 // FORCE = true: MLS with force (explicit).  FORCE = false: APIC mass and
 // momentum only, particlesToGridHelper<APIC> (MpmSimulationBase.cpp:563-608).
-template <bool FORCE>
+template <bool FORCE, int MODEL = kAnyModel>
 __global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const uint32_t* __restrict__ order,
                                              const uint32_t* __restrict__ group_start,
                                              const uint32_t* __restrict__ group_page, double* __restrict__ stage,
@@ -468,7 +474,7 @@ __global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
-      particle_kirchhoff(P, i, f, tau);
+      particle_kirchhoff<MODEL>(P, i, f, tau);
       const double vol = P.vol[i];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -702,6 +708,12 @@ __global__ __launch_bounds__(g2p_threads<MLS>()) void k_g2p(ParticlesView P, con
       P.x[d * S + i] = xp[d] + inc;
       inc2 += inc * inc;
     }
+    if constexpr (!MLS)
+      if (P.gradv)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) P.gradv[(c * 3 + r) * S + i] = Gv[r][c];
     if (inc2 > dx2) flags |= 1u;
     if (inc2 > dx2 * 0.25 * (cfl * cfl)) flags |= 2u;
     double Fo[3][3];
@@ -979,10 +991,12 @@ struct zr_engine {
   double *pparam = nullptr, *pstate = nullptr, *gdeg = nullptr;
   int32_t* broken = nullptr;
   bool has_models = false, has_plastic = false, all_hencky = false;
+  int uniform_model = ZR_MODEL_FIXED_COROTATED;  // the model of every particle, or kAnyModel
   // viscoelastic branch: Fe_Nonequilibrated [9][n], its StvkWithHencky mu /
   // lambda, V0 tau of the branch [9][n], the two viscosities
   double *fneq = nullptr, *mu_neq = nullptr, *lam_neq = nullptr, *vtau_neq = nullptr;
   double visc_d = 0.0, visc_v = 0.0;
+  double* g2p_gradv = nullptr;  // set around the APIC G2P whose grad v a step needs
   bool visco = false;
   // phase field: per particle (c, H, l0, 1/sigma_c, pf_Fp, H_max, k, vol, allow)
   // and per node (phase, rhs, D, x, r, mr, s, q) of the damage solve
@@ -1007,9 +1021,10 @@ struct zr_engine {
   std::vector<double> ckpt_time;
   // ADMM: per particle F 9, y 9, omega 1, last dv 3 (= 22 rows of n); per
   // node vn 3, dv 3, proj 9, diag 1, rhs 3 and MINRES q 3x3, z 3, m 3x3
-  double* admm_p = nullptr;
+  double* admm_p = nullptr;  // F 9, y 9, omega, last dv 3 (+ F2 9, y2 9, omega2 with the viscoelastic branch)
   int64_t admm_n = 0;
   int admm_steps = 0;
+  bool admm_visco = false;
   double* admm_node = nullptr;
   int32_t* admm_coll = nullptr;
   size_t admm_cap = 0;
@@ -1177,6 +1192,7 @@ void free_particles(zr_engine* e)
   dfree(e->touch_page); dfree(e->touch_slot); dfree(e->slot_page); dfree(e->group_nslot);
   dfree(e->mat); dfree(e->pparam); dfree(e->pstate); dfree(e->gdeg); dfree(e->broken);
   e->has_models = e->has_plastic = false;
+  e->uniform_model = ZR_MODEL_FIXED_COROTATED;
   dfree(e->fneq); dfree(e->mu_neq); dfree(e->lam_neq); dfree(e->vtau_neq);
   e->visco = false;
   dfree(e->pf_c); dfree(e->pf_H); dfree(e->pf_l0); dfree(e->pf_osc); dfree(e->pf_Fp); dfree(e->pf_Hmax);
@@ -1281,7 +1297,7 @@ ColliderSet collider_set(const zr_engine* e, double dt = 0.0)
 ParticlesView pview(zr_engine* e)
 {
   return ParticlesView{ e->x, e->v, e->C, e->F, e->m, e->vol, e->mu, e->lam, e->n,
-                        e->has_models ? e->mat : nullptr, e->gdeg, e->visco ? e->vtau_neq : nullptr };
+                        e->has_models ? e->mat : nullptr, e->gdeg, e->visco ? e->vtau_neq : nullptr, nullptr };
 }
 
 int ensure_grid(zr_engine* e)
@@ -1388,8 +1404,23 @@ int do_p2g(zr_engine* e, bool mls)
     ZR_LAUNCH_CHECK(e);
   }
   if (mls)
-    k_p2g<true><<<e->A, kTransferThreads, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
-                                             1.0 / dx, 4.0 / (dx * dx));
+  {
+    const ParticlesView pv = pview(e);
+    const double Dinv = 4.0 / (dx * dx);
+    const int model = e->has_models ? e->uniform_model : ZR_MODEL_FIXED_COROTATED;
+    if (model == ZR_MODEL_NEO_HOOKEAN_BORDEN)
+      k_p2g<true, ZR_MODEL_NEO_HOOKEAN_BORDEN><<<e->A, kTransferThreads, 0, e->stream>>>(
+          pv, e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx, Dinv);
+    else if (model == ZR_MODEL_STVK_HENCKY)
+      k_p2g<true, ZR_MODEL_STVK_HENCKY><<<e->A, kTransferThreads, 0, e->stream>>>(
+          pv, e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx, Dinv);
+    else if (model == ZR_MODEL_FIXED_COROTATED)
+      k_p2g<true, ZR_MODEL_FIXED_COROTATED><<<e->A, kTransferThreads, 0, e->stream>>>(
+          pv, e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx, Dinv);
+    else
+      k_p2g<true><<<e->A, kTransferThreads, 0, e->stream>>>(pv, e->order, e->group_start, e->group_page, e->stage, dx,
+                                                             1.0 / dx, Dinv);
+  }
   else
     k_p2g<false><<<e->A, kTransferThreads, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
                                               1.0 / dx, 4.0 / (dx * dx));
@@ -1426,13 +1457,15 @@ int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
   if (!e->updated) return fail(e, ZR_ERR_STATE, "gridToParticles before the grid update");
   const double dx = e->cfg.dx;
   ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_CFL, 0, sizeof(uint32_t), e->stream));
+  ParticlesView pv = pview(e);
+  pv.gradv = e->g2p_gradv;
   prof_mark(e, ZR_PROF_G2P);
   if (mls)
     k_g2p<true><<<e->A, g2p_threads<true>(), 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot, e->gvs,
                                              (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt, e->cfg.cfl,
                                              e->dcount);
   else
-    k_g2p<false><<<e->A, g2p_threads<false>(), 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->page_slot,
+    k_g2p<false><<<e->A, g2p_threads<false>(), 0, e->stream>>>(pv, e->order, e->group_start, e->group_page, e->page_slot,
                                               e->gvs, (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx), dt,
                                               e->cfg.cfl, e->dcount);
   ZR_LAUNCH_CHECK(e);
@@ -1629,12 +1662,15 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
   const size_t N = (size_t)n;
   int rc = ensure_scalars(e);
   if (rc) return rc;
-  const bool first = e->admm_steps == 0 || e->admm_n != n;
-  if (e->admm_n != n) {
+  const bool visco = e->visco;
+  const bool first = e->admm_steps == 0 || e->admm_n != n || e->admm_visco != visco;
+  if (e->admm_n != n || e->admm_visco != visco) {
+    const size_t rows = visco ? 41 : 22;
     dfree(e->admm_p);
-    ZR_HIP(e, hipMalloc(&e->admm_p, 22 * N * sizeof(double)));
-    ZR_HIP(e, hipMemsetAsync(e->admm_p, 0, 22 * N * sizeof(double), s));
+    ZR_HIP(e, hipMalloc(&e->admm_p, rows * N * sizeof(double)));
+    ZR_HIP(e, hipMemsetAsync(e->admm_p, 0, rows * N * sizeof(double), s));
     e->admm_n = n;
+    e->admm_visco = visco;
     e->admm_steps = 0;
   }
   if (e->admm_cap < e->grid_cap) {
@@ -1658,7 +1694,16 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
   double* zb = base + 28 * C;
   double* mb[3] = { base + 31 * C, base + 34 * C, base + 37 * C };
   double* pa = e->admm_p;
-  AdmmParticles Ap{ pa, pa + 9 * N, pa + 18 * N, pa + 19 * N };
+  AdmmParticles Ap{ pa, pa + 9 * N, pa + 18 * N, pa + 19 * N, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                    e->visc_d, e->visc_v };
+  if (visco) {
+    Ap.F2 = pa + 22 * N;
+    Ap.y2 = pa + 31 * N;
+    Ap.om2 = pa + 40 * N;
+    Ap.fneq = e->fneq;
+    Ap.mu2 = e->mu_neq;
+    Ap.lam2 = e->lam_neq;
+  }
   const uint32_t nn = e->S * 32, nbn = blocks_for(nn, 256);
   const double dx = e->cfg.dx, inv_dx = 1.0 / dx, rho = o->rho_scale;
   ColliderSet cs = collider_set(e);
@@ -1669,25 +1714,43 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
     k_admm_lastdv_p2g<<<e->A, 256, 0, s>>>(P, Ap, e->order, e->group_start, e->group_page, e->stage, inv_dx);
   k_admm_init_nodes<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, e->gv, Nd, first ? 0 : 1, dt);
   k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, cs);
-  k_admm_init_particles<<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
-                                              e->group_page, e->group_nslot, e->gm, e->stage, first ? 1 : 0, dt, rho,
-                                              inv_dx);
+  if (visco)
+    k_admm_init_particles<true><<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
+                                                     e->group_page, e->group_nslot, e->gm, e->stage, first ? 1 : 0,
+                                                     dt, rho, inv_dx);
+  else
+    k_admm_init_particles<false><<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
+                                                      e->group_page, e->group_nslot, e->gm, e->stage, first ? 1 : 0,
+                                                      dt, rho, inv_dx);
   k_admm_diag<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd.diag);
   ZR_LAUNCH_CHECK(e);
   int total_minres = 0;
   for (int tim = 0; tim < o->admm_iterations; ++tim) {
     k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, cs);
-    k_admm_fstep<<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
-                                      e->group_page, e->group_nslot, e->gm, dt, rho, o->local_tolerance, inv_dx);
-    k_admm_scatter<0><<<e->A, 256, 0, s>>>(P, Ap, Nd, nullptr, e->order, e->group_start, e->group_page,
-                                           e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+    if (visco) {
+      k_admm_fstep<true><<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
+                                              e->group_page, e->group_nslot, e->gm, dt, rho, o->local_tolerance,
+                                              inv_dx);
+      k_admm_scatter<0, true><<<e->A, 256, 0, s>>>(P, Ap, Nd, nullptr, e->order, e->group_start, e->group_page,
+                                                   e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+    } else {
+      k_admm_fstep<false><<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
+                                               e->group_page, e->group_nslot, e->gm, dt, rho, o->local_tolerance,
+                                               inv_dx);
+      k_admm_scatter<0, false><<<e->A, 256, 0, s>>>(P, Ap, Nd, nullptr, e->order, e->group_start, e->group_page,
+                                                    e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+    }
     k_admm_rhs<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, rhs, C, dt, e->cfg.gravity[0],
                                    e->cfg.gravity[1], e->cfg.gravity[2]);
     // MINRES on the global system, warm-started from dv (Minres.h:69-149),
     // relative tolerance min(0.5, sqrt(max(1, 1e-6))) = 0.5 (:785-787)
     k_admm_reset_flags<<<1, 64, 0, s>>>(e->ci);
-    k_admm_scatter<1><<<e->A, 256, 0, s>>>(P, Ap, Nd, Nd.dv, e->order, e->group_start, e->group_page,
-                                           e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+    if (visco)
+      k_admm_scatter<1, true><<<e->A, 256, 0, s>>>(P, Ap, Nd, Nd.dv, e->order, e->group_start, e->group_page,
+                                                   e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+    else
+      k_admm_scatter<1, false><<<e->A, 256, 0, s>>>(P, Ap, Nd, Nd.dv, e->order, e->group_start, e->group_page,
+                                                    e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
     double *q = qb[2], *qk = qb[1], *qkm1 = qb[0], *z = zb, *mk = mb[2], *mkm1 = mb[1], *mkm2 = mb[0];
     k_minres_init<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, rhs, Nd.dv, q, z, mk, mkm1, qk,
                                       e->partial);
@@ -1708,8 +1771,12 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
       qkm1 = qk;
       qk = q;
       q = t;
-      k_admm_scatter<1><<<e->A, 256, 0, s>>>(P, Ap, Nd, mk, e->order, e->group_start, e->group_page,
-                                             e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+      if (visco)
+        k_admm_scatter<1, true><<<e->A, 256, 0, s>>>(P, Ap, Nd, mk, e->order, e->group_start, e->group_page,
+                                                     e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
+      else
+        k_admm_scatter<1, false><<<e->A, 256, 0, s>>>(P, Ap, Nd, mk, e->order, e->group_start, e->group_page,
+                                                      e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
       k_minres_apply<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, mk, q, e->ci, e->partial);
       k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_ALPHA, 0.0, 0.0, k,
                                         o->minres_max_iterations);
@@ -1728,8 +1795,12 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
     ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     ZR_HIP(e, hipStreamSynchronize(s));
     total_minres += e->hci[AI_ITERS];
-    k_admm_ustep<<<e->A, 256, 0, s>>>(P, Ap, Nd, e->order, e->group_start, e->group_page, e->group_nslot, e->gm, dt,
-                                      rho, inv_dx);
+    if (visco)
+      k_admm_ustep<true><<<e->A, 256, 0, s>>>(P, Ap, Nd, e->order, e->group_start, e->group_page, e->group_nslot,
+                                              e->gm, dt, rho, inv_dx);
+    else
+      k_admm_ustep<false><<<e->A, 256, 0, s>>>(P, Ap, Nd, e->order, e->group_start, e->group_page, e->group_nslot,
+                                               e->gm, dt, rho, inv_dx);
     ZR_LAUNCH_CHECK(e);
   }
   k_admm_finish_nodes<<<nbn, 256, 0, s>>>(nn, e->gm, Nd, e->gvs);
@@ -2534,6 +2605,9 @@ int zr_set_materials(zr_engine* e, const zr_materials* mt)
   e->has_plastic = plastic;
   e->all_hencky = true;
   for (size_t i = 0; i < N; ++i) e->all_hencky = e->all_hencky && (code[i] & 15) == ZR_MODEL_STVK_HENCKY;
+  e->uniform_model = N ? (code[0] & 15) : ZR_MODEL_FIXED_COROTATED;
+  for (size_t i = 0; i < N; ++i)
+    if ((code[i] & 15) != e->uniform_model) e->uniform_model = kAnyModel;
   return ZR_OK;
 }
 
@@ -2630,6 +2704,8 @@ int zr_set_phase_field(zr_engine* e, const zr_phase_field* pf, const zr_phase_fi
   }
   ZR_HIP(e, hipMemcpy(e->mat, code.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
   e->has_models = true;
+  e->uniform_model = ZR_MODEL_NEO_HOOKEAN_BORDEN;
+  e->all_hencky = false;
   double** dst[] = { &e->pf_c, &e->pf_H, &e->pf_l0, &e->pf_osc, &e->pf_Fp, &e->pf_Hmax, &e->pf_k, &e->pf_vol };
   const double* src[] = { pf->c, pf->H, pf->l0, pf->one_over_sigma_c, pf->pf_Fp, pf->H_max, pf->residual_phase,
                           nullptr };
@@ -2809,7 +2885,8 @@ int zr_advance_one_time_step_admm(zr_engine* e, double dt, const zr_admm_opts* o
     return fail(e, ZR_ERR_ARG, "bad dt or ADMM options");
   if (e->n > 0 && (!e->mat || !e->all_hencky || e->pf_on))
     return fail(e, ZR_ERR_ARG, "the ADMM step needs StvkWithHencky particles (zr_set_materials)");
-  if (e->visco) return fail(e, ZR_ERR_ARG, "the ADMM step has no viscoelastic branch");
+  if (e->visco && e->has_plastic)
+    return fail(e, ZR_ERR_ARG, "the ADMM step's viscoelastic branch needs particles without plasticity");
   if (ast) ast->minres_iterations = 0;
   int rc = do_sort(e, st);
   if (rc) return rc;
@@ -2819,8 +2896,18 @@ int zr_advance_one_time_step_admm(zr_engine* e, double dt, const zr_admm_opts* o
   if (rc) return rc;
   rc = do_admm(e, dt, o, ast);
   if (rc) return rc;
+  // gridToParticles: with the viscoelastic branch the APIC G2P also writes
+  // grad v, from which that helper's evolveStrain moves and relaxes Fneq
+  e->g2p_gradv = e->visco ? e->vtau_neq : nullptr;
   rc = do_g2p(e, dt, st, false);
+  e->g2p_gradv = nullptr;
   if (rc < 0) return rc;
+  if (e->visco) {
+    k_visco_strain<<<blocks_for((uint64_t)e->n, 256), 256, 0, e->stream>>>((uint32_t)e->n, e->vtau_neq, e->fneq,
+                                                                           e->mu_neq, e->lam_neq, e->n, dt, e->visc_d,
+                                                                           e->visc_v);
+    ZR_LAUNCH_CHECK(e);
+  }
   e->time += dt;
   return rc;
 }
