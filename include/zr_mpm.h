@@ -390,7 +390,9 @@ int zr_get_admm_state(zr_engine* e, double* F, double* y, double* omega);
 /* Differentiable form of the explicit MLS step.  No counterpart in the
    reference; for gradient-based callers (parameter estimation, control) that
    need d loss / d (state, mu, lambda) of a rollout.  Fixed-corotated
-   particles without plasticity or damage; colliders without friction.
+   particles without plasticity or damage; every collider kind, in moving
+   frames, with Coulomb friction (differentiated on its sliding branch; the
+   sticking branch has a zero derivative).
    zr_adjoint_set: the adjoint of the current state (NULL arrays: zero; mu and
    lambda set the accumulators).  zr_adjoint_step: with the engine holding
    the INPUT state of a step of size dt (and its time), replaces the adjoint

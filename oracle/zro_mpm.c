@@ -1914,8 +1914,17 @@ void zro_phase_field_multiply(zro_sim* s, const double* x, double* b) { pf_multi
 typedef double M3[3][3];
 
 /* Jacobian d v_after / d v_before of the collider handling of one node
-   (multi_object_collision, then trial_collision when nothing collided).
-   Returns 0 when a branch has no supported derivative (friction). */
+   (multi_object_collision in the frame at t, then trial_collision in the
+   frame at t + dt when nothing collided) for objects moving as x = R s X + b.
+   The object's velocity at the node does not depend on v, so each collider
+   acts on v - v_obj through its world normal n = R N: sticky gives 0, a
+   slipping or approaching contact the projection P = I - n n^T.  Coulomb
+   friction on the sliding branch (|t| > -mu (v - v_obj).n, t the tangential
+   part) scales t by k = 1 + mu dot / |t|: d t' / d rel = k P + t (mu n / |t|
+   - mu dot t / |t|^3)^T; the sticking branch gives 0.  The trial collision
+   v -= (phi / dt) n at x + dt v has I - n n^T - phi grad n, with grad n =
+   P / (phi + s r) for a sphere of radius s r.  Returns 0 for a node at a
+   sphere's centre in the trial collision (no normal). */
 static int collide_jacobian(const zro_sim* s, double dt, const double* x, const double* vpre, double J[3][3])
 {
   double v[3] = { vpre[0], vpre[1], vpre[2] };
@@ -1924,50 +1933,64 @@ static int collide_jacobian(const zro_sim* s, double dt, const double* x, const 
   int any = 0;
   for (int k = 0; k < s->ncol; ++k) {
     const zro_collider* c = &s->col[k];
-    double X[3], phi, N[3];
+    double xb[3], phi, N[3], vo[3];
     zro_frame fr;
     collider_frame(c, s->time, &fr);
-    for (int d = 0; d < 3; ++d) X[d] = x[d] - fr.b[d];
-    if (!ls_query_inside(c, X, &phi, N)) continue;
+    if (!collider_query(c, &fr, x, &phi, N, xb)) continue;
     any = 1;
     if (c->type == 1) {
       memset(J, 0, sizeof(double) * 9);
       return 1;
     }
+    collider_velocity(c, &fr, xb, vo);
     double rel[3], dot = 0.0;
-    for (int d = 0; d < 3; ++d) rel[d] = v[d] - c->velocity[d];
+    for (int d = 0; d < 3; ++d) rel[d] = v[d] - vo[d];
     for (int d = 0; d < 3; ++d) dot += rel[d] * N[d];
     if (c->type == 2 || dot < 0) {
-      if (c->friction != 0.0 && dot < 0) return 0;
-      double P[3][3], T[3][3];
+      double P[3][3], Jr[3][3], T[3][3], t[3];
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) P[i][j] = (i == j ? 1.0 : 0.0) - N[i] * N[j];
+      for (int d = 0; d < 3; ++d) t[d] = rel[d] - N[d] * dot;
+      memcpy(Jr, P, sizeof(P));
+      if (c->friction != 0.0 && dot < 0) {
+        const double mu = c->friction;
+        const double tn = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        if (-dot * mu < tn) {
+          const double kf = 1.0 + mu * dot / tn;
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Jr[i][j] = kf * P[i][j] + t[i] * (mu * N[j] / tn - mu * dot * t[j] / (tn * tn * tn));
+          for (int d = 0; d < 3; ++d) t[d] *= kf;
+        } else {
+          memset(Jr, 0, sizeof(Jr));
+          t[0] = t[1] = t[2] = 0.0;
+        }
+      }
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) T[i][j] = P[i][0] * J[0][j] + P[i][1] * J[1][j] + P[i][2] * J[2][j];
+        for (int j = 0; j < 3; ++j) T[i][j] = Jr[i][0] * J[0][j] + Jr[i][1] * J[1][j] + Jr[i][2] * J[2][j];
       memcpy(J, T, sizeof(T));
-      for (int d = 0; d < 3; ++d) rel[d] -= N[d] * dot;
+      for (int d = 0; d < 3; ++d) rel[d] = t[d];
     }
-    for (int d = 0; d < 3; ++d) v[d] = rel[d] + c->velocity[d];
+    for (int d = 0; d < 3; ++d) v[d] = rel[d] + vo[d];
   }
   if (any) return 1;
   for (int k = 0; k < s->ncol; ++k) { /* trial collision at t + dt */
     const zro_collider* c = &s->col[k];
-    double X[3], phi, n[3];
+    double xt[3], xb[3], phi, n[3];
     zro_frame fr;
     collider_frame(c, s->time + dt, &fr);
-    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - fr.b[d];
-    if (!ls_query_inside(c, X, &phi, n)) continue;
+    for (int d = 0; d < 3; ++d) xt[d] = x[d] + dt * v[d];
+    if (!collider_query(c, &fr, xt, &phi, n, xb)) continue;
     double A[3][3], T[3][3];
     if (c->shape == 0) {
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - n[i] * n[j];
     } else {
-      double dist = phi + c->radius;
-      if (dist < 1e-7) return 0;
+      double dist = phi + fr.s * c->radius;
+      if (dist < 1e-7 * fr.s) return 0;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
           double pr = (i == j ? 1.0 : 0.0) - n[i] * n[j];
-          A[i][j] = (i == j ? 1.0 : 0.0) - n[i] * n[j] - phi / dist * pr;
+          A[i][j] = pr - phi / dist * pr;
         }
     }
     for (int i = 0; i < 3; ++i)

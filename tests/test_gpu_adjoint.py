@@ -53,6 +53,39 @@ def test_one_step_adjoint_parity():
     eng.close()
 
 
+def test_one_step_adjoint_parity_moving_frictional_colliders():
+    """The collider Jacobian for objects in moving frames (a rotating,
+    translating ground with Coulomb friction and a spinning, growing
+    sphere) at t = 0.3, against the CPU adjoint oracle (pinned there by
+    central differences)."""
+    s = _scene(seed=7)
+    X = s["X"]
+    lo, hi, c = X.min(axis=0), X.max(axis=0), X.mean(axis=0)
+    th = 0.2
+    cols = [dict(shape=0, type=2, friction=0.4, origin=(0.0, 0.0, 0.0), normal=(0.0, 1.0, 0.0),
+                 translation=(0.0, float(lo[1] + 0.6 * s["dx"]), 0.0), velocity=(0.05, 0.1, 0.0),
+                 rotation=(np.cos(th / 2), 0.0, 0.0, np.sin(th / 2)), omega=(0.0, 0.0, 0.5)),
+            dict(shape=1, type=2, origin=(0.0, 0.0, 0.0), radius=float(0.35 * (hi[0] - lo[0])),
+                 translation=(float(hi[0]), float(c[1]), float(c[2])), velocity=(-0.2, 0.0, 0.0),
+                 rotation=(1.0, 0.0, 0.0, 0.0), omega=(3.0, -2.0, 1.0), scale=1.0, dscale_dt=0.5)]
+    n = len(s["mass"])
+    rng = np.random.default_rng(2)
+    w = dict(X=rng.normal(size=(n, 3)), V=rng.normal(size=(n, 3)), C=rng.normal(size=(n, 9)),
+             F=rng.normal(size=(n, 9)))
+    eng, ora = make_pair(s, colliders=cols)
+    eng.set_time(0.3)
+    ora.set_time(0.3)
+    eng.adjoint_set(**w)
+    eng.adjoint_step(DT)
+    g = eng.adjoint_get()
+    aX, aV, aC, aF = (np.ascontiguousarray(w[k].copy()) for k in ("X", "V", "C", "F"))
+    amu, alam = np.zeros(n), np.zeros(n)
+    ora.adjoint_step(DT, aX, aV, aC, aF, amu, alam)
+    for k, ref in (("X", aX), ("V", aV), ("C", aC), ("F", aF), ("mu", amu), ("lam", alam)):
+        assert rel_err(g[k], ref) < 1e-10, k
+    eng.close()
+
+
 def _loss_and_grad(X, m, target):
     com = (m[:, None] * X).sum(0) / m.sum()
     return float(np.sum((com - target) ** 2)), 2 * (com - target)[None, :] * (m / m.sum())[:, None]

@@ -2,12 +2,12 @@
 angular velocity, scaling, translation; CollisionObject.cpp:158-194,
 279-296,340-405) against the oracle: explicit MLS steps, the implicit step
 (collision projection with world-space normals), calculateDt with a spinning
-sphere, and the adjoint's refusal of frames it has no derivative for."""
+sphere (the adjoint in moving frames is in test_gpu_adjoint.py)."""
 import numpy as np
 import pytest
 
 import oracle
-from ziran2019_amd.engine import ZrError, implicit_opts
+from ziran2019_amd.engine import implicit_opts
 from _helpers import make_pair, perturbed, rel_err
 from test_oracle_collider_frame import _frames, _quat
 
@@ -64,17 +64,6 @@ def test_calculate_dt_with_a_spinning_sphere_bit_exact():
     eng, ora = make_pair(s, colliders=[_spinner(s)])
     assert eng.calculateDt(1.0) == ora.calculate_dt(1.0)
     eng.close()
-
-
-def test_adjoint_refuses_a_spinning_collider():
-    from ziran2019_amd.autodiff import Rollout
-    s = perturbed(44, n_cells=3)
-    n = len(s["mass"])
-    prob = Rollout(s, 2, 1e-4, colliders=[_spinner(s)])
-    X = prob.forward(np.full(n, 1e4), np.zeros((n, 3)))
-    with pytest.raises(ZrError):
-        prob.backward(np.ones_like(X))
-    prob.close()
 
 
 @pytest.mark.parametrize("separating", [None, "plain", "framed"])

@@ -32,8 +32,15 @@ struct AdjointView {
 
 // d v_after / d v_before of the node's collider handling: multi_object_collision
 // in the frame at t, then trial_collision in the frame at t + dt when nothing
-// collided (translating colliders: R = I, s = 1).  Returns false for a branch
-// without a derivative (Coulomb friction, a node at a sphere centre).
+// collided, for objects moving as x = R s X + b.  The object's velocity at
+// the node does not depend on v, so each collider acts on v - v_obj through
+// its world normal n = R N: sticky gives 0, a slipping or approaching
+// contact P = I - n n^T; Coulomb friction on the sliding branch (|t| >
+// -mu dot, t the tangential part) scales t by k = 1 + mu dot / |t|, with
+// d t' / d rel = k P + t (mu n / |t| - mu dot t / |t|^3)^T, and the sticking
+// branch gives 0.  The trial collision v -= (phi / dt) n at x + dt v has
+// I - n n^T - phi grad n, grad n = P / (phi + s r) for a sphere of radius
+// s r.  Returns false for a node at a sphere's centre in the trial collision.
 __device__ bool collide_jacobian(const ColliderSet& cs, double dt, const double x[3], const double vpre[3],
                                  double J[3][3])
 {
@@ -43,43 +50,61 @@ __device__ bool collide_jacobian(const ColliderSet& cs, double dt, const double 
   bool any = false;
   for (int k = 0; k < cs.n; ++k) {
     const DevCollider& c = cs.c[k];
-    double X[3], phi, N[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] - c.f[0].b[d];
-    if (!ls_query_inside(c, X, phi, N)) continue;
+    double xb[3], phi, N[3], vo[3];
+    if (!collider_query(c, 0, x, phi, N, xb)) continue;
     any = true;
     if (c.type == ZR_STICKY) {
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) J[i][j] = 0.0;
       return true;
     }
+    collider_velocity(c, 0, xb, vo);
     double rel[3], dot = 0.0;
-    for (int d = 0; d < 3; ++d) rel[d] = v[d] - c.velocity[d];
+    for (int d = 0; d < 3; ++d) rel[d] = v[d] - vo[d];
     for (int d = 0; d < 3; ++d) dot += rel[d] * N[d];
     if (c.type == ZR_SLIP || dot < 0.0) {
-      if (c.friction != 0.0 && dot < 0.0) return false;
-      double T[3][3];
+      double P[3][3], Jr[3][3], T[3][3], t[3];
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j)
-          T[i][j] = J[i][j] - N[i] * (N[0] * J[0][j] + N[1] * J[1][j] + N[2] * J[2][j]);
+        for (int j = 0; j < 3; ++j) {
+          P[i][j] = (i == j ? 1.0 : 0.0) - N[i] * N[j];
+          Jr[i][j] = P[i][j];
+        }
+      for (int d = 0; d < 3; ++d) t[d] = rel[d] - N[d] * dot;
+      if (c.friction != 0.0 && dot < 0.0) {
+        const double mu = c.friction;
+        const double tn = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        if (-dot * mu < tn) {
+          const double kf = 1.0 + mu * dot / tn;
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Jr[i][j] = kf * P[i][j] + t[i] * (mu * N[j] / tn - mu * dot * t[j] / (tn * tn * tn));
+          for (int d = 0; d < 3; ++d) t[d] *= kf;
+        } else {
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Jr[i][j] = 0.0;
+          t[0] = t[1] = t[2] = 0.0;
+        }
+      }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T[i][j] = Jr[i][0] * J[0][j] + Jr[i][1] * J[1][j] + Jr[i][2] * J[2][j];
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) J[i][j] = T[i][j];
-      for (int d = 0; d < 3; ++d) rel[d] -= N[d] * dot;
+      for (int d = 0; d < 3; ++d) rel[d] = t[d];
     }
-    for (int d = 0; d < 3; ++d) v[d] = rel[d] + c.velocity[d];
+    for (int d = 0; d < 3; ++d) v[d] = rel[d] + vo[d];
   }
   if (any) return true;
   for (int k = 0; k < cs.n; ++k) {
     const DevCollider& c = cs.c[k];
-    double X[3], phi, n[3];
-    for (int d = 0; d < 3; ++d) X[d] = x[d] + dt * v[d] - c.f[1].b[d];
-    if (!ls_query_inside(c, X, phi, n)) continue;
+    double xt[3], xb[3], phi, n[3];
+    for (int d = 0; d < 3; ++d) xt[d] = x[d] + dt * v[d];
+    if (!collider_query(c, 1, xt, phi, n, xb)) continue;
     double A[3][3], T[3][3];
     if (c.shape == ZR_HALF_SPACE) {
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - n[i] * n[j];
     } else {
-      const double dist = phi + c.radius;
-      if (dist < 1e-7) return false;
+      const double dist = phi + c.f[1].s * c.radius;
+      if (dist < 1e-7 * c.f[1].s) return false;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
           const double pr = (i == j ? 1.0 : 0.0) - n[i] * n[j];

@@ -1258,16 +1258,6 @@ void collider_frame(const zr_collider& c, double t, ColliderFrame& f)
   f.inv_s = 1.0 / f.s;
 }
 
-// a collider whose frame neither rotates nor scales (a translation only)
-bool collider_translates_only(const zr_collider& k)
-{
-  ColliderFrame f;
-  collider_frame(k, 0.0, f);
-  for (int i = 0; i < 9; ++i)
-    if (f.R[i] != ((i % 4 == 0) ? 1.0 : 0.0)) return false;
-  return f.s == 1.0 && k.dscale_dt == 0.0 && k.omega[0] == 0.0 && k.omega[1] == 0.0 && k.omega[2] == 0.0;
-}
-
 // the colliders with their frames at the engine time t and at t + dt
 ColliderSet collider_set(const zr_engine* e, double dt = 0.0)
 {
@@ -1408,18 +1398,17 @@ int do_p2g(zr_engine* e, bool mls)
     const ParticlesView pv = pview(e);
     const double Dinv = 4.0 / (dx * dx);
     const int model = e->has_models ? e->uniform_model : ZR_MODEL_FIXED_COROTATED;
+#define ZR_P2G_LAUNCH(M) k_p2g<true, M><<<e->A, kTransferThreads, 0, e->stream>>>( \
+          pv, e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx, Dinv)
     if (model == ZR_MODEL_NEO_HOOKEAN_BORDEN)
-      k_p2g<true, ZR_MODEL_NEO_HOOKEAN_BORDEN><<<e->A, kTransferThreads, 0, e->stream>>>(
-          pv, e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx, Dinv);
+      ZR_P2G_LAUNCH(ZR_MODEL_NEO_HOOKEAN_BORDEN);
     else if (model == ZR_MODEL_STVK_HENCKY)
-      k_p2g<true, ZR_MODEL_STVK_HENCKY><<<e->A, kTransferThreads, 0, e->stream>>>(
-          pv, e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx, Dinv);
+      ZR_P2G_LAUNCH(ZR_MODEL_STVK_HENCKY);
     else if (model == ZR_MODEL_FIXED_COROTATED)
-      k_p2g<true, ZR_MODEL_FIXED_COROTATED><<<e->A, kTransferThreads, 0, e->stream>>>(
-          pv, e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx, Dinv);
+      ZR_P2G_LAUNCH(ZR_MODEL_FIXED_COROTATED);
     else
-      k_p2g<true><<<e->A, kTransferThreads, 0, e->stream>>>(pv, e->order, e->group_start, e->group_page, e->stage, dx,
-                                                             1.0 / dx, Dinv);
+      ZR_P2G_LAUNCH(kAnyModel);
+#undef ZR_P2G_LAUNCH
   }
   else
     k_p2g<false><<<e->A, kTransferThreads, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
@@ -1605,11 +1594,6 @@ bool adjoint_supported(zr_engine* e)
          "the adjoint supports fixed-corotated particles without plasticity, damage or a viscoelastic branch only");
     return false;
   }
-  for (const zr_collider& k : e->colliders)
-    if (!collider_translates_only(k)) {
-      fail(e, ZR_ERR_ARG, "the adjoint supports translating colliders only (no rotation, spin or scaling)");
-      return false;
-    }
   return true;
 }
 
