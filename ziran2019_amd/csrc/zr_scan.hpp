@@ -119,12 +119,13 @@ struct LdPopcSum {
   const uint32_t* bits;
   __device__ uint32_t operator()(uint32_t i) const
   {
-    uint32_t sb = sum[i], c = 0;
-    while (sb) {
-      const uint32_t b = (uint32_t)__builtin_ctz(sb);
-      sb &= sb - 1u;
-      c += (uint32_t)__popc(bits[i * 32u + b]);
-    }
+    // predicated and unrolled so that the flagged words are requested
+    // together rather than one dependent load after another
+    const uint32_t sb = sum[i];
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 32u; ++b)
+      if ((sb >> b) & 1u) c += (uint32_t)__popc(bits[i * 32u + b]);
     return c;
   }
 };
