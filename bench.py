@@ -375,13 +375,15 @@ def run_row_workload(args):
         eng.set_colliders(s["colliders"])
         if wl == "admm-slabs":
             eng.set_materials(**s["materials"])
-            o = admm_opts(admm_iterations=20)
+            fp32 = args.admm_precision == "fp32-refined"
+            o = admm_opts(admm_iterations=20, precision=1 if fp32 else 0)
             minres = []
 
             def step():
                 minres.append(eng.advanceOneTimeStepAdmm(dt, o).minres_iterations)
             workload = ("ADMM step (StvkWithHencky, von Mises slab with yield stress 200 Pa beside a Drucker-Prager "
-                        "slab at 30 deg), 2048000 particles at 8 ppc, dx=1/256, dt=1e-3, 20 ADMM iterations")
+                        "slab at 30 deg), 2048000 particles at 8 ppc, dx=1/256, dt=1e-3, 20 ADMM iterations, "
+                        + ("FP32-bulk global step" if fp32 else "FP64"))
             extra["minres_iterations"] = minres
         else:
             eng.set_materials(model=ZR_MODEL_NEO_HOOKEAN_BORDEN)
@@ -449,6 +451,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--preconditioner", choices=sorted(PRECONDITIONERS), default="mass",
                     help="CG preconditioner of the implicit-boxes workload")
+    ap.add_argument("--admm-precision", choices=["fp64", "fp32-refined"], default="fp64",
+                    help="admm-slabs: the global step in FP64 or in the FP32-bulk form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (5 launches) to scale HBM counters")

@@ -212,13 +212,18 @@ typedef struct zr_particles_adjoint {
 
 /* AdmmSimulation options (Projects/admm/AdmmSimulationFoam.h:55-68):
    admm_max_iterations (reference default 100), Minres(10000), rho_scale 0.5,
-   global tolerance 1e-6, local Newton tolerance 1e-8. */
+   global tolerance 1e-6, local Newton tolerance 1e-8.  precision: ZR_FP64
+   (the reference's arithmetic) or ZR_FP32_REFINED (the global step's
+   operator data and Krylov vectors in FP32; the local step, projections,
+   inner products and the velocity update dv stay FP64). */
 typedef struct zr_admm_opts {
   int32_t admm_iterations;
   int32_t minres_max_iterations;
   double rho_scale;
   double global_tolerance;
   double local_tolerance;
+  int32_t precision;   /* ZR_FP64 */
+  int32_t reserved;
 } zr_admm_opts;
 
 typedef struct zr_admm_stats {

@@ -7,8 +7,8 @@ import numpy as np
 import pytest
 
 import oracle
-from ziran2019_amd.engine import (ZR_MODEL_STVK_HENCKY, ZR_PLASTIC_DRUCKER_PRAGER, ZR_PLASTIC_VON_MISES, ZrError,
-                                  admm_opts, drucker_prager_alpha)
+from ziran2019_amd.engine import (ZR_FP32_REFINED, ZR_MODEL_STVK_HENCKY, ZR_PLASTIC_DRUCKER_PRAGER,
+                                  ZR_PLASTIC_VON_MISES, ZrError, admm_opts, drucker_prager_alpha)
 from _helpers import make_pair, perturbed, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -108,6 +108,38 @@ def test_admm_viscoelastic_parity():
     fg, fo = eng.get_viscoelastic(), ora.get_viscoelastic()
     assert rel_err(fg, fo) < 1e-9
     assert np.abs(fo - fneq).max() > 1e-3  # the branch moved and relaxed
+    eng.close()
+
+
+def test_admm_viscoelastic_fp32_global_step():
+    """The FP32-bulk global step with the viscoelastic branch (the FP32
+    operator adds the FFn term): three steps of 20 ADMM iterations against
+    the FP64 oracle, within the bounds written here."""
+    s, eng, ora, fneq, mu2, lam2 = _visco_pair(45)
+    eng.set_materials(**_materials(len(s["mass"]), None, "elastic"))
+    ora.set_materials(model=ZR_MODEL_STVK_HENCKY)
+    eng.set_viscoelastic(fneq, mu2, lam2, 20.0, 80.0)
+    ora.set_viscoelastic(fneq, mu2, lam2, 20.0, 80.0)
+    dt = 1e-3
+    for step in range(3):
+        sg = eng.advanceOneTimeStepAdmm(dt, admm_opts(admm_iterations=20, precision=ZR_FP32_REFINED))
+        _, so = ora.step_admm(dt, oracle.admm_opts(admm_iterations=20))
+        print("fp32 visco admm step", step, "minres", sg.minres_iterations, so.minres_iterations, flush=True)
+    X, V, C, F = ora.get_particles()
+    out = eng.download()
+    disp = np.abs(X - s["X"]).max()
+    ex, ev, ef = (np.abs(out["X"] - X).max() / disp, rel_err(out["V"], V), rel_err(out["F"], F))
+    efn = rel_err(eng.get_viscoelastic(), ora.get_viscoelastic())
+    print("FP32-bulk visco ADMM: X error / displacement", ex, "V", ev, "F", ef, "Fneq", efn, flush=True)
+    assert ex < 1e-3 and ev < 1e-3 and ef < 1e-5 and efn < 1e-5
+    eng.close()
+
+
+def test_admm_rejects_unknown_precision():
+    s, eng, ora, fneq, mu2, lam2 = _visco_pair(44)
+    eng.set_materials(**_materials(len(s["mass"]), None, "elastic"))
+    with pytest.raises(ZrError):
+        eng.advanceOneTimeStepAdmm(1e-3, admm_opts(admm_iterations=2, precision=7))
     eng.close()
 
 

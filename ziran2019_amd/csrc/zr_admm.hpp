@@ -914,11 +914,13 @@ __global__ __launch_bounds__(256) void k_admm_rhs(uint32_t nn, const int32_t* __
 
 // MINRES start (Minres.h:40-101): q = P(b - A x), z = q / diag, partial z.q;
 // m_k, m_{k-1} and q_k start at zero (Minres::reinitialize)
+// (T: the precision the Krylov vectors are stored in; arithmetic in FP64)
+template <class T>
 __global__ __launch_bounds__(256) void k_minres_init(uint32_t nn, const int32_t* __restrict__ slot_src,
                                                      const double* __restrict__ stage, const double* __restrict__ gm,
                                                      AdmmNodes N, const double* __restrict__ b,
-                                                     const double* __restrict__ x, double* q, double* z,
-                                                     double* zero0, double* zero1, double* zero2, double* partial)
+                                                     const double* __restrict__ x, T* q, T* z, T* zero0, T* zero1,
+                                                     T* zero2, double* partial)
 {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   const int64_t S = N.stride;
@@ -939,21 +941,22 @@ __global__ __launch_bounds__(256) void k_minres_init(uint32_t nn, const int32_t*
       }
     }
     for (int d = 0; d < 3; ++d) {
-      q[d * S + k] = r[d];
-      z[d * S + k] = zz[d];
-      zero0[d * S + k] = 0.0;
-      zero1[d * S + k] = 0.0;
-      zero2[d * S + k] = 0.0;
+      q[d * S + k] = (T)r[d];
+      z[d * S + k] = (T)zz[d];
+      zero0[d * S + k] = (T)0;
+      zero1[d * S + k] = (T)0;
+      zero2[d * S + k] = (T)0;
     }
   }
   block_partial(acc, partial);
 }
 
 // q_{k+1} = P(A z) (z is the new m_k), partial of alpha = z . q
+template <class T>
 __global__ __launch_bounds__(256) void k_minres_apply(uint32_t nn, const int32_t* __restrict__ slot_src,
                                                       const double* __restrict__ stage,
                                                       const double* __restrict__ gm, AdmmNodes N,
-                                                      const double* __restrict__ z, double* q, const int32_t* ai,
+                                                      const T* __restrict__ z, T* q, const int32_t* ai,
                                                       double* partial)
 {
   if (ai[AI_DONE]) return;
@@ -967,20 +970,20 @@ __global__ __launch_bounds__(256) void k_minres_apply(uint32_t nn, const int32_t
       const uint32_t e = k & 31u;
       double f[3];
       sum_tiles_src<3>(stage, slot_src + 8 * (size_t)(k >> 5), (int)(e >> 4), (int)((e >> 2) & 3u), (int)(e & 3u), f);
-      for (int d = 0; d < 3; ++d) r[d] = m * z[d * S + k] + f[d];
+      for (int d = 0; d < 3; ++d) r[d] = m * (double)z[d * S + k] + f[d];
       admm_project_node(N, k, r);
-      for (int d = 0; d < 3; ++d) acc += z[d * S + k] * r[d];
+      for (int d = 0; d < 3; ++d) acc += (double)z[d * S + k] * r[d];
     }
-    for (int d = 0; d < 3; ++d) q[d * S + k] = r[d];
+    for (int d = 0; d < 3; ++d) q[d * S + k] = (T)r[d];
   }
   block_partial(acc, partial);
 }
 
 // q -= alpha q_k + beta_k q_{k-1}; z = q / diag; partial of z . q
+template <class T>
 __global__ __launch_bounds__(256) void k_minres_lanczos(uint32_t nn, const double* __restrict__ gm, AdmmNodes N,
-                                                        double* q, const double* __restrict__ qk,
-                                                        const double* __restrict__ qkm1, double* z, const double* sc,
-                                                        const int32_t* ai, double* partial)
+                                                        T* q, const T* __restrict__ qk, const T* __restrict__ qkm1,
+                                                        T* z, const double* sc, const int32_t* ai, double* partial)
 {
   if (ai[AI_DONE]) return;
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
@@ -990,23 +993,23 @@ __global__ __launch_bounds__(256) void k_minres_lanczos(uint32_t nn, const doubl
     const double alpha = sc[AD_ALPHA], beta = sc[AD_BETA_K];
     const double dg = N.diag[k];
     for (int d = 0; d < 3; ++d) {
-      double v = q[d * S + k];
-      v -= alpha * qk[d * S + k];
-      v -= beta * qkm1[d * S + k];
-      q[d * S + k] = v;
-      const double zz = v / dg;
-      z[d * S + k] = zz;
-      acc += zz * v;
+      double v = (double)q[d * S + k];
+      v -= alpha * (double)qk[d * S + k];
+      v -= beta * (double)qkm1[d * S + k];
+      const T vt = (T)v, zt = (T)(v / dg);
+      q[d * S + k] = vt;
+      z[d * S + k] = zt;
+      acc += (double)zt * (double)vt;
     }
   }
   block_partial(acc, partial);
 }
 
 // normalise q, z by beta_{k+1}; m_k = (z_old - delta m_{k-1} - eps m_{k-2}) / gamma; x += t_k m_k
+template <class T>
 __global__ __launch_bounds__(256) void k_minres_update(uint32_t nn, const double* __restrict__ gm, AdmmNodes N,
-                                                       double* q, double* z, double* mk,
-                                                       const double* __restrict__ mkm1,
-                                                       const double* __restrict__ mkm2, double* x, const double* sc,
+                                                       T* q, T* z, T* mk, const T* __restrict__ mkm1,
+                                                       const T* __restrict__ mkm2, double* x, const double* sc,
                                                        const int32_t* ai)
 {
   if (ai[AI_DONE + 2]) return;  // finished before this iteration's Givens step
@@ -1016,27 +1019,184 @@ __global__ __launch_bounds__(256) void k_minres_update(uint32_t nn, const double
   const double bk = sc[AD_BETA_KP1], gamma = sc[AD_GAMMA], delta = sc[AD_DELTA], eps = sc[AD_EPS], tk = sc[AD_TK];
   for (int d = 0; d < 3; ++d) {
     if (bk > 0.0) {
-      q[d * S + k] /= bk;
-      z[d * S + k] /= bk;
+      q[d * S + k] = (T)((double)q[d * S + k] / bk);
+      z[d * S + k] = (T)((double)z[d * S + k] / bk);
     }
-    const double m = (mk[d * S + k] - delta * mkm1[d * S + k] - eps * mkm2[d * S + k]) / gamma;
+    const T m = (T)(((double)mk[d * S + k] - delta * (double)mkm1[d * S + k] - eps * (double)mkm2[d * S + k]) / gamma);
     mk[d * S + k] = m;
-    x[d * S + k] += tk * m;
+    x[d * S + k] += tk * (double)m;
   }
 }
 
 // the start vectors divided by beta_1 = |r0|_P (Minres.h:95-98)
-__global__ __launch_bounds__(256) void k_minres_scale(uint32_t nn, const double* __restrict__ gm, int64_t S, double* q,
-                                                      double* z, const double* sc, const int32_t* ai)
+template <class T>
+__global__ __launch_bounds__(256) void k_minres_scale(uint32_t nn, const double* __restrict__ gm, int64_t S, T* q,
+                                                      T* z, const double* sc, const int32_t* ai)
 {
   if (ai[AI_DONE]) return;
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   const double b = sc[AD_BETA_KP1];
   if (k >= nn || gm[k] == 0.0 || !(b > 0.0)) return;
   for (int d = 0; d < 3; ++d) {
-    q[d * S + k] /= b;
-    z[d * S + k] /= b;
+    q[d * S + k] = (T)((double)q[d * S + k] / b);
+    z[d * S + k] = (T)((double)z[d * S + k] / b);
   }
+}
+
+// ---- the FP32-bulk global step (configuration 3: FP32 bulk, FP64 SVD and
+// projection): the operator's per-particle data in FP32, in sorted order,
+// formed once per step after initStep; the operator apply reads it
+// coalesced, gathers and forms its terms in FP32, and adds them into the
+// FP64 LDS tiles (FP32 LDS adds are far slower on this part); the Krylov
+// vectors are FP32, their inner products and the solution dv FP64.
+struct AdmmCache32 {
+  float* frac;     // [3][n] x / dx - base node
+  float* F;        // [9][n] Fn, column-major
+  float* cp;       // [n] rho omega^2 dt^2
+  float* F2;       // [9][n] FFn of the viscoelastic branch (VISCO)
+  float* cp2;      // [n]
+  uint8_t* cell;   // [n] element code of the base node in its page
+  int64_t stride;  // n
+};
+
+template <bool VISCO>
+__global__ __launch_bounds__(256) void k_admm_cache32(ParticlesView P, AdmmParticles A,
+                                                      const uint32_t* __restrict__ order, uint32_t n, AdmmCache32 C,
+                                                      double inv_dx, double rho, double dt)
+{
+  const uint32_t ks = blockIdx.x * 256u + threadIdx.x;
+  if (ks >= n) return;
+  const uint32_t i = order[ks];
+  const int64_t S = P.stride, CS = C.stride;
+  int b[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double xi = P.x[d * S + i] * inv_dx;
+    b[d] = int_floor(xi - 0.5);
+    C.frac[d * CS + ks] = (float)(xi - (double)b[d]);
+  }
+  C.cell[ks] = (uint8_t)elem_of(b[0], b[1], b[2]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) C.F[k * CS + ks] = (float)P.F[k * S + i];
+  const double om = A.om[i];
+  C.cp[ks] = (float)(rho * om * om * dt * dt);
+  if constexpr (VISCO) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) C.F2[k * CS + ks] = (float)A.fneq[k * S + i];
+    const double om2 = A.om2[i];
+    C.cp2[ks] = (float)(rho * om2 * om2 * dt * dt);
+  }
+}
+
+// the global operator's particle term in FP32 (k_admm_scatter<1>'s sums):
+// c_p (grad(x) Fn) Fn^T grad w (+ the viscoelastic branch's term with FFn),
+// x given in precision TX
+template <bool VISCO, class TX>
+__global__ __launch_bounds__(256) void k_admm_op32(AdmmCache32 C, const uint32_t* __restrict__ group_start,
+                                                   const int32_t* __restrict__ group_nslot,
+                                                   const double* __restrict__ gm, const TX* __restrict__ xvec,
+                                                   int64_t NS, double* stage, float inv_dx, const int32_t* ai)
+{
+  if (ai[AI_DONE]) return;
+  constexpr int SS = 3 * kTileNodes;
+  __shared__ float tv[3 * kTileNodes];
+  __shared__ double tile[4 * SS];
+  __shared__ int32_t nslot[8];
+  const int tid = threadIdx.x;
+  const uint32_t g = blockIdx.x;
+  for (int k = tid; k < 4 * SS; k += 256) tile[k] = 0.0;
+  if (tid < 8) nslot[tid] = group_nslot[8 * (size_t)g + tid];
+  __syncthreads();
+  for (int t = tid; t < kTileNodes; t += 256) {
+    const int tx = t / (kTileY * kTileZ), ty = (t / kTileZ) % kTileY, tz = t % kTileZ;
+    const int32_t sl = nslot[((tx >> 1) << 2) | ((ty >> 2) << 1) | (tz >> 2)];
+    const int64_t node = (int64_t)sl * 32 + elem_of(tx, ty, tz);
+    const bool on = sl >= 0 && gm[node] != 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) tv[d * kTileNodes + t] = on ? (float)xvec[d * NS + node] : 0.0f;
+  }
+  __syncthreads();
+  const uint32_t start = group_start[g];
+  const uint32_t n = group_start[g + 1] - start;
+  const int64_t CS = C.stride;
+  double* wt = tile + (tid >> 6) * SS;
+  const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
+  for (uint32_t j0 = 0; j0 < n; j0 += 256) {
+    const uint32_t j = j0 + perm;
+    if (j >= n) continue;
+    const uint32_t ks = start + j;
+    const uint32_t e = C.cell[ks];
+    const int lx = (int)(e >> 4), ly = (int)((e >> 2) & 3u), lz = (int)(e & 3u);
+    float w[3][3], dw[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bspline_frac(C.frac[d * CS + ks], w[d], dw[d]);
+    float G[3][3] = { { 0.f } };
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float gw[3];
+          wgradT(w, dw, inv_dx, a, bb, c, gw);
+          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const float v = tv[r * kTileNodes + t];
+            G[r][0] += v * gw[0];
+            G[r][1] += v * gw[1];
+            G[r][2] += v * gw[2];
+          }
+        }
+    float Fn[3][3], Y[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) Fn[r][c] = C.F[(c * 3 + r) * CS + ks];
+    const float cp = C.cp[ks];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Y[r][c] = cp * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
+    float Fv[3][3], Y2[3][3];
+    if constexpr (VISCO) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Fv[r][c] = C.F2[(c * 3 + r) * CS + ks];
+      const float cp2 = C.cp2[ks];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Y2[r][c] = cp2 * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float gw[3], fg[3];
+          wgradT(w, dw, inv_dx, a, bb, c, gw);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) fg[d] = Fn[0][d] * gw[0] + Fn[1][d] * gw[1] + Fn[2][d] * gw[2];
+          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            float val = Y[r][0] * fg[0] + Y[r][1] * fg[1] + Y[r][2] * fg[2];
+            if constexpr (VISCO) {
+              float fg2[3];
+#pragma unroll
+              for (int d = 0; d < 3; ++d) fg2[d] = Fv[0][d] * gw[0] + Fv[1][d] * gw[1] + Fv[2][d] * gw[2];
+              val += Y2[r][0] * fg2[0] + Y2[r][1] * fg2[1] + Y2[r][2] * fg2[2];
+            }
+            atomicAdd(&wt[r * kTileNodes + t], (double)val);
+          }
+        }
+  }
+  __syncthreads();
+  double* out = stage + (size_t)g * SS;
+  for (int k = tid; k < SS; k += 256) out[k] = tile[k] + tile[SS + k] + tile[2 * SS + k] + tile[3 * SS + k];
 }
 
 

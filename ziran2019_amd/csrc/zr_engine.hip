@@ -1025,6 +1025,13 @@ struct zr_engine {
   int64_t admm_n = 0;
   int admm_steps = 0;
   bool admm_visco = false;
+  // the FP32-bulk global step: sorted per-particle operator data (13 or 23
+  // rows of n floats), element codes, and seven FP32 Krylov vectors
+  float* admm_c32 = nullptr;
+  uint8_t* admm_cell32 = nullptr;
+  size_t admm_c32_cap = 0, admm_c32_rows = 0;
+  float* admm_node32 = nullptr;
+  size_t admm_node32_cap = 0;
   double* admm_node = nullptr;
   int32_t* admm_coll = nullptr;
   size_t admm_cap = 0;
@@ -1639,6 +1646,89 @@ int do_adjoint_step(zr_engine* e, double dt)
 // ------------------------------------------------------------------- ADMM
 constexpr int kAdmmNodeRows = 3 + 3 + 9 + 1 + 3 + 9 + 3 + 9;  // vn dv proj diag rhs q(3) z m(3)
 
+// the global operator's particle term of x into the staged tiles: the FP64
+// scatter from the particle arrays, or the FP32 apply over the sorted cache
+struct AdmmOp {
+  bool fp32, visco;
+  ParticlesView P;
+  AdmmParticles Ap;
+  AdmmNodes Nd;
+  AdmmCache32 C;
+  double dt, rho, inv_dx;
+};
+
+template <class TX>
+void admm_apply(zr_engine* e, const AdmmOp& op, const TX* x)
+{
+  hipStream_t s = e->stream;
+  if (op.fp32) {
+    if (op.visco)
+      k_admm_op32<true, TX><<<e->A, 256, 0, s>>>(op.C, e->group_start, e->group_nslot, e->gm, x, op.Nd.stride,
+                                                 e->stage, (float)op.inv_dx, e->ci);
+    else
+      k_admm_op32<false, TX><<<e->A, 256, 0, s>>>(op.C, e->group_start, e->group_nslot, e->gm, x, op.Nd.stride,
+                                                  e->stage, (float)op.inv_dx, e->ci);
+  } else if constexpr (std::is_same<TX, double>::value) {
+    if (op.visco)
+      k_admm_scatter<1, true><<<e->A, 256, 0, s>>>(op.P, op.Ap, op.Nd, x, e->order, e->group_start, e->group_page,
+                                                   e->group_nslot, e->gm, e->stage, op.dt, op.rho, op.inv_dx, e->ci);
+    else
+      k_admm_scatter<1, false><<<e->A, 256, 0, s>>>(op.P, op.Ap, op.Nd, x, e->order, e->group_start, e->group_page,
+                                                    e->group_nslot, e->gm, e->stage, op.dt, op.rho, op.inv_dx, e->ci);
+  }
+}
+
+// Minres::solve (Minres.h:69-149) on the global system for dv (FP64), the
+// Krylov vectors in precision T (the seven vectors rotate by pointer)
+template <class T>
+int admm_minres(zr_engine* e, const AdmmOp& op, const double* rhs, T* const qb[3], T* zb, T* const mb[3],
+                const zr_admm_opts* o, uint32_t nn, uint32_t nbn)
+{
+  hipStream_t s = e->stream;
+  const AdmmNodes& Nd = op.Nd;
+  const int64_t C = Nd.stride;
+  k_admm_reset_flags<<<1, 64, 0, s>>>(e->ci);
+  admm_apply<double>(e, op, Nd.dv);
+  T *q = qb[2], *qk = qb[1], *qkm1 = qb[0], *z = zb, *mk = mb[2], *mkm1 = mb[1], *mkm2 = mb[0];
+  k_minres_init<T><<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, rhs, Nd.dv, q, z, mk, mkm1, qk,
+                                       e->partial);
+  k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_INIT, o->global_tolerance, 0.5, 0,
+                                    o->minres_max_iterations);
+  k_minres_scale<T><<<nbn, 256, 0, s>>>(nn, e->gm, C, q, z, e->sc, e->ci);
+  ZR_LAUNCH_CHECK(e);
+  int next_poll = 1;
+  for (int k = 0; k < o->minres_max_iterations; ++k) {
+    k_minres_begin<<<1, 64, 0, s>>>(e->sc, e->ci);
+    // m_{k-2} <- m_{k-1} <- m_k <- z; q_{k-1} <- q_k <- q_{k+1}
+    T* t = mkm2;
+    mkm2 = mkm1;
+    mkm1 = mk;
+    mk = z;
+    z = t;
+    t = qkm1;
+    qkm1 = qk;
+    qk = q;
+    q = t;
+    admm_apply<T>(e, op, mk);
+    k_minres_apply<T><<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, mk, q, e->ci, e->partial);
+    k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_ALPHA, 0.0, 0.0, k, o->minres_max_iterations);
+    k_minres_lanczos<T><<<nbn, 256, 0, s>>>(nn, e->gm, Nd, q, qk, qkm1, z, e->sc, e->ci, e->partial);
+    k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_GIVENS, 0.0, 0.0, k,
+                                      o->minres_max_iterations);
+    k_minres_update<T><<<nbn, 256, 0, s>>>(nn, e->gm, Nd, q, z, mk, mkm1, mkm2, Nd.dv, e->sc, e->ci);
+    ZR_LAUNCH_CHECK(e);
+    if (k + 1 == next_poll || k + 1 == o->minres_max_iterations) {
+      next_poll *= 2;
+      ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      ZR_HIP(e, hipStreamSynchronize(s));
+      if (e->hci[AI_DONE]) break;
+    }
+  }
+  ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  ZR_HIP(e, hipStreamSynchronize(s));
+  return ZR_OK;
+}
+
 int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
 {
   hipStream_t s = e->stream;
@@ -1708,6 +1798,45 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
                                                       dt, rho, inv_dx);
   k_admm_diag<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd.diag);
   ZR_LAUNCH_CHECK(e);
+  // the FP32-bulk global step: per-particle operator data in FP32, sorted,
+  // and FP32 Krylov vectors
+  const bool fp32 = o->precision == ZR_FP32_REFINED;
+  AdmmOp op{ fp32, visco, P, Ap, Nd, AdmmCache32{}, dt, rho, inv_dx };
+  float *qb32[3] = { nullptr, nullptr, nullptr }, *zb32 = nullptr, *mb32[3] = { nullptr, nullptr, nullptr };
+  if (fp32) {
+    const size_t rows = visco ? 23 : 13;
+    if (e->admm_c32_cap < N || e->admm_c32_rows < rows) {
+      dfree(e->admm_c32);
+      dfree(e->admm_cell32);
+      e->admm_c32_cap = 0;
+      ZR_HIP(e, hipMalloc(&e->admm_c32, rows * N * sizeof(float)));
+      ZR_HIP(e, hipMalloc(&e->admm_cell32, N));
+      e->admm_c32_cap = N;
+      e->admm_c32_rows = rows;
+    }
+    if (e->admm_node32_cap < (size_t)C) {
+      dfree(e->admm_node32);
+      e->admm_node32_cap = 0;
+      ZR_HIP(e, hipMalloc(&e->admm_node32, 21 * (size_t)C * sizeof(float)));
+      e->admm_node32_cap = (size_t)C;
+    }
+    float* c = e->admm_c32;
+    op.C = AdmmCache32{ c, c + 3 * N, c + 12 * N, visco ? c + 13 * N : nullptr, visco ? c + 22 * N : nullptr,
+                        e->admm_cell32, n };
+    float* b32 = e->admm_node32;
+    for (int k = 0; k < 3; ++k) {
+      qb32[k] = b32 + 3 * k * C;
+      mb32[k] = b32 + 3 * (4 + k) * C;
+    }
+    zb32 = b32 + 9 * C;
+    if (visco)
+      k_admm_cache32<true><<<blocks_for((uint64_t)n, 256), 256, 0, s>>>(P, Ap, e->order, (uint32_t)n, op.C, inv_dx,
+                                                                       rho, dt);
+    else
+      k_admm_cache32<false><<<blocks_for((uint64_t)n, 256), 256, 0, s>>>(P, Ap, e->order, (uint32_t)n, op.C, inv_dx,
+                                                                        rho, dt);
+    ZR_LAUNCH_CHECK(e);
+  }
   int total_minres = 0;
   for (int tim = 0; tim < o->admm_iterations; ++tim) {
     k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, cs);
@@ -1728,56 +1857,9 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
                                    e->cfg.gravity[1], e->cfg.gravity[2]);
     // MINRES on the global system, warm-started from dv (Minres.h:69-149),
     // relative tolerance min(0.5, sqrt(max(1, 1e-6))) = 0.5 (:785-787)
-    k_admm_reset_flags<<<1, 64, 0, s>>>(e->ci);
-    if (visco)
-      k_admm_scatter<1, true><<<e->A, 256, 0, s>>>(P, Ap, Nd, Nd.dv, e->order, e->group_start, e->group_page,
-                                                   e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
-    else
-      k_admm_scatter<1, false><<<e->A, 256, 0, s>>>(P, Ap, Nd, Nd.dv, e->order, e->group_start, e->group_page,
-                                                    e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
-    double *q = qb[2], *qk = qb[1], *qkm1 = qb[0], *z = zb, *mk = mb[2], *mkm1 = mb[1], *mkm2 = mb[0];
-    k_minres_init<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, rhs, Nd.dv, q, z, mk, mkm1, qk,
-                                      e->partial);
-    k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_INIT, o->global_tolerance, 0.5, 0,
-                                      o->minres_max_iterations);
-    k_minres_scale<<<nbn, 256, 0, s>>>(nn, e->gm, C, q, z, e->sc, e->ci);
-    ZR_LAUNCH_CHECK(e);
-    int next_poll = 1;
-    for (int k = 0; k < o->minres_max_iterations; ++k) {
-      k_minres_begin<<<1, 64, 0, s>>>(e->sc, e->ci);
-      // m_{k-2} <- m_{k-1} <- m_k <- z; q_{k-1} <- q_k <- q_{k+1}
-      double* t = mkm2;
-      mkm2 = mkm1;
-      mkm1 = mk;
-      mk = z;
-      z = t;
-      t = qkm1;
-      qkm1 = qk;
-      qk = q;
-      q = t;
-      if (visco)
-        k_admm_scatter<1, true><<<e->A, 256, 0, s>>>(P, Ap, Nd, mk, e->order, e->group_start, e->group_page,
-                                                     e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
-      else
-        k_admm_scatter<1, false><<<e->A, 256, 0, s>>>(P, Ap, Nd, mk, e->order, e->group_start, e->group_page,
-                                                      e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
-      k_minres_apply<<<nbn, 256, 0, s>>>(nn, e->slot_src, e->stage, e->gm, Nd, mk, q, e->ci, e->partial);
-      k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_ALPHA, 0.0, 0.0, k,
-                                        o->minres_max_iterations);
-      k_minres_lanczos<<<nbn, 256, 0, s>>>(nn, e->gm, Nd, q, qk, qkm1, z, e->sc, e->ci, e->partial);
-      k_minres_reduce<<<1, 256, 0, s>>>(e->partial, nbn, e->sc, e->ci, MR_GIVENS, 0.0, 0.0, k,
-                                        o->minres_max_iterations);
-      k_minres_update<<<nbn, 256, 0, s>>>(nn, e->gm, Nd, q, z, mk, mkm1, mkm2, Nd.dv, e->sc, e->ci);
-      ZR_LAUNCH_CHECK(e);
-      if (k + 1 == next_poll || k + 1 == o->minres_max_iterations) {
-        next_poll *= 2;
-        ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        ZR_HIP(e, hipStreamSynchronize(s));
-        if (e->hci[AI_DONE]) break;
-      }
-    }
-    ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    ZR_HIP(e, hipStreamSynchronize(s));
+    int rc2 = fp32 ? admm_minres<float>(e, op, rhs, qb32, zb32, mb32, o, nn, nbn)
+                   : admm_minres<double>(e, op, rhs, qb, zb, mb, o, nn, nbn);
+    if (rc2) return rc2;
     total_minres += e->hci[AI_ITERS];
     if (visco)
       k_admm_ustep<true><<<e->A, 256, 0, s>>>(P, Ap, Nd, e->order, e->group_start, e->group_page, e->group_nslot,
@@ -2292,7 +2374,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->scan_tmp); dfree(e->speed); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->jstage); dfree(e->pinv); dfree(e->collided); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
-  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll);
+  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll); dfree(e->admm_c32); dfree(e->admm_cell32); dfree(e->admm_node32);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   dfree(e->mr64); dfree(e->mr32);
   if (e->hsc) (void)hipHostFree(e->hsc);
@@ -2865,7 +2947,8 @@ int zr_advance_one_time_step_admm(zr_engine* e, double dt, const zr_admm_opts* o
                                   zr_step_stats* st)
 {
   if (!e) return ZR_ERR_ARG;
-  if (!(dt > 0.0) || !o || o->admm_iterations < 0 || o->minres_max_iterations < 0 || !(o->rho_scale > 0.0))
+  if (!(dt > 0.0) || !o || o->admm_iterations < 0 || o->minres_max_iterations < 0 || !(o->rho_scale > 0.0) ||
+      (o->precision != ZR_FP64 && o->precision != ZR_FP32_REFINED))
     return fail(e, ZR_ERR_ARG, "bad dt or ADMM options");
   if (e->n > 0 && (!e->mat || !e->all_hencky || e->pf_on))
     return fail(e, ZR_ERR_ARG, "the ADMM step needs StvkWithHencky particles (zr_set_materials)");

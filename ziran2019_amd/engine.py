@@ -28,6 +28,7 @@ ZR_ERR_STATE = -4
 
 ZR_STICKY, ZR_SLIP, ZR_SEPARATE = 1, 2, 3
 ZR_HALF_SPACE, ZR_SPHERE = 0, 1
+ZR_FP64, ZR_FP32_REFINED = 0, 1
 
 DUMP = dict(sorter=0, order=1, base_offset=2, group_first=3, group_last=4, block_offset=5, pages=6,
             grid_m=7, grid_v=8, grid_f=9, grid_vstar=10, node_idx=11, dv=12, hvp_xcd=13, jacobi=14)
@@ -241,7 +242,7 @@ class ZrParticlesAdjoint(ctypes.Structure):
 class ZrAdmmOpts(ctypes.Structure):
     _fields_ = [("admm_iterations", ctypes.c_int32), ("minres_max_iterations", ctypes.c_int32),
                 ("rho_scale", ctypes.c_double), ("global_tolerance", ctypes.c_double),
-                ("local_tolerance", ctypes.c_double)]
+                ("local_tolerance", ctypes.c_double), ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class ZrAdmmStats(ctypes.Structure):
@@ -249,10 +250,11 @@ class ZrAdmmStats(ctypes.Structure):
 
 
 def admm_opts(admm_iterations=100, minres_max_iterations=10000, rho_scale=0.5, global_tolerance=1e-6,
-              local_tolerance=1e-8):
-    """AdmmSimulation defaults (AdmmSimulationFoam.h:55-68)."""
+              local_tolerance=1e-8, precision=ZR_FP64):
+    """AdmmSimulation defaults (AdmmSimulationFoam.h:55-68); ``precision``
+    ZR_FP32_REFINED runs the global step's operator and Krylov vectors in FP32."""
     return ZrAdmmOpts(int(admm_iterations), int(minres_max_iterations), float(rho_scale), float(global_tolerance),
-                      float(local_tolerance))
+                      float(local_tolerance), int(precision), 0)
 
 
 class ZrStepStats(ctypes.Structure):
