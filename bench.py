@@ -370,7 +370,8 @@ def run_row_workload(args):
             s = scenes.fracture_scene()
             dt = 5e-4
         n = len(s["mass"])
-        eng = MpmEngine(dx=s["dx"], gravity=s["gravity"])
+        fp32_transfers = wl == "fracture-sphere" and args.transfer_precision == "fp32"
+        eng = MpmEngine(dx=s["dx"], gravity=s["gravity"], precision=1 if fp32_transfers else 0)
         eng.upload(s)
         eng.set_colliders(s["colliders"])
         if wl == "admm-slabs":
@@ -393,7 +394,8 @@ def run_row_workload(args):
             def step():
                 eng.advanceOneTimeStep(dt)
             workload = ("explicit MLS step with the CD-MPM phase-field solve (CG), 3000000 NeoHookeanBorden "
-                        "particles in a Voronoi-toughened block, sticky sphere at 5 m/s, dx=1/320, dt=5e-4")
+                        "particles in a Voronoi-toughened block, sticky sphere at 5 m/s, dx=1/320, dt=5e-4, "
+                        + ("FP32 P2G terms" if fp32_transfers else "FP64"))
         per = 1
     for _ in range(args.warmup):
         step()
@@ -453,6 +455,8 @@ def main():
                     help="CG preconditioner of the implicit-boxes workload")
     ap.add_argument("--admm-precision", choices=["fp64", "fp32-refined"], default="fp64",
                     help="admm-slabs: the global step in FP64 or in the FP32-bulk form")
+    ap.add_argument("--transfer-precision", choices=["fp64", "fp32"], default="fp64",
+                    help="fracture-sphere: the P2G's per-particle terms in FP64 or FP32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB (5 launches) to scale HBM counters")

@@ -78,7 +78,10 @@ typedef struct zr_config {
   int32_t grid_log2;    /* node coordinates live in [0, 2^grid_log2); 6..12 (12: the
                            reference's 4096^3 SPGrid extent; the dense page tables
                            take 4 x 2^(3 grid_log2 - 3) bytes, 32 GiB at 12) */
-  int32_t precision;    /* ZR_FP64 */
+  int32_t precision;    /* ZR_FP64; ZR_FP32_REFINED: the explicit P2G's per-particle
+                           terms (weights, offsets, products; a NeoHookeanBorden
+                           stress) in FP32, the node sums, grid, G2P, damage and
+                           particle state FP64 */
   int32_t transfer;     /* ZR_TRANSFER_MLS (mls_mpm = true) */
   int32_t reserved;
 } zr_config;

@@ -105,15 +105,16 @@ def test_admm_slabs():
     dp = s["slab"] == 1
     assert rel_err(pg, po) < 1e-8
     assert np.abs(po[dp]).max() > 0.0  # the sand slab flowed
-    # the FP32-bulk global step: every MINRES solve stops at half its start
-    # residual, so FP32 rounding that moves a stopping test by one iteration
-    # changes that solve's dv by a fraction of its correction; 20 ADMM
-    # iterations pull the result back toward the FP64 path's
+    # the FP32-bulk global step (measured on MI355X: 2e-8 of the displacement,
+    # V 3e-8, F 6e-10 relative, with the oracle's MINRES counts); every MINRES
+    # solve stops at half its start residual, so FP32 rounding that moved a
+    # stopping test by one iteration would change that solve's dv by a
+    # fraction of its correction
     o32 = e32.download()
     disp = np.abs(X - s["X"]).max()
     ex, ev, ef = (np.abs(o32["X"] - X).max() / disp, rel_err(o32["V"], V), rel_err(o32["F"], F))
     print("FP32-bulk ADMM: X error / displacement", ex, "V", ev, "F", ef, flush=True)
-    assert ex < 1e-3 and ev < 1e-3 and ef < 1e-5
+    assert ex < 1e-6 and ev < 1e-6 and ef < 1e-7
     eng.close()
     e32.close()
 

@@ -627,6 +627,7 @@ __global__ __launch_bounds__(256) void k_admm_init_particles(ParticlesView P, Ad
       A.om2[i] = om2;
       cp2 = rho * om2 * om2 * dt * dt;
     }
+    opaque_weights(w, dw);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -674,8 +675,10 @@ __global__ __launch_bounds__(256) void k_admm_diag(uint32_t nn, const int32_t* _
 // vol dE(proj(z)) + rho omega^2 (z - sigma) = 0, sigma the singular values
 // of Fn - y/(rho omega) + dt grad(vn + dv) Fn; VISCO: then the viscoelastic
 // branch's F2
+// (two waves per SIMD: the SVD and the local Newton spill a few registers
+// to scratch, which costs less than the latency one wave leaves exposed)
 template <bool VISCO>
-__global__ __launch_bounds__(256) void k_admm_fstep(ParticlesView P, AdmmParticles A, AdmmNodes N,
+__global__ __launch_bounds__(256, 2) void k_admm_fstep(ParticlesView P, AdmmParticles A, AdmmNodes N,
                                                     const int32_t* __restrict__ mat,
                                                     const double* __restrict__ pparam,
                                                     const double* __restrict__ pstate,
@@ -850,6 +853,7 @@ __global__ __launch_bounds__(256) void k_admm_scatter(ParticlesView P, AdmmParti
             Y2[r][c] = c22 * gf;
         }
     }
+    opaque_weights(w, dw);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -1170,6 +1174,7 @@ __global__ __launch_bounds__(256) void k_admm_op32(AdmmCache32 C, const uint32_t
 #pragma unroll
         for (int c = 0; c < 3; ++c) Y2[r][c] = cp2 * (G[r][0] * Fv[0][c] + G[r][1] * Fv[1][c] + G[r][2] * Fv[2][c]);
     }
+    opaque_weights(w, dw);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll

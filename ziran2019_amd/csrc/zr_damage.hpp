@@ -224,6 +224,7 @@ __global__ __launch_bounds__(256) void k_pf_apply(const double* __restrict__ X, 
     const double kap = pf.vol[i] * 4.0 * l0 * l0 * gscale;
 #pragma unroll
     for (int d = 0; d < 3; ++d) gc[d] *= kap;
+    opaque_weights(w, dw);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll

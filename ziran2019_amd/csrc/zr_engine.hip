@@ -27,6 +27,7 @@
 #include <vector>
 #include <array>
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/zr_mpm.h"
 #include "zr_math.hpp"
@@ -258,6 +259,25 @@ __global__ void k_bin(const double* __restrict__ x, int64_t stride, uint32_t n, 
 // order): sum_base[s] is the first group of summary word s; 32 lanes per
 // summary word, grid-stride, each lane one page_bits word, the lanes' page
 // counts prefix-summed across the 32 lanes.
+// the touched pages under each summary word (32 lanes per word, each
+// counting one flagged page_bits word)
+__global__ __launch_bounds__(256) void k_sum_counts(const uint32_t* __restrict__ page_sum, uint32_t nsum,
+                                                    const uint32_t* __restrict__ page_bits,
+                                                    uint32_t* __restrict__ sum_count)
+{
+  const uint32_t lane = threadIdx.x & 31u;
+  for (uint32_t s = blockIdx.x * 8u + (threadIdx.x >> 5); s < nsum; s += gridDim.x * 8u) {
+    const uint32_t sb = page_sum[s];
+    uint32_t c = 0;
+    if (sb) {  // alike in the 32 lanes of s
+      c = ((sb >> lane) & 1u) ? (uint32_t)__popc(page_bits[s * 32u + lane]) : 0u;
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) c += __shfl_xor(c, off, 32);
+    }
+    if (lane == 0) sum_count[s] = c;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_list_groups(const uint32_t* page_sum, uint32_t nsum,
                                                      const uint32_t* page_bits, const uint32_t* sum_base,
                                                      const uint32_t* page_count, uint32_t* group_page,
@@ -300,33 +320,56 @@ __global__ void k_scatter(uint32_t n, const uint32_t* p_page, const uint32_t* p_
   keys[dst] = ((uint64_t)(re & 31u) << 32) | (uint64_t)i;
 }
 
-__global__ __launch_bounds__(256) void k_sort_groups(const uint64_t* keys, const uint32_t* group_start, uint32_t* order)
+// sorts each group's keys (cell << 32 | particle index) ascending: the
+// particles are bucketed by cell (LDS counters), and each particle's place in
+// its cell's bucket is the number of the bucket's indices below its own
+__global__ __launch_bounds__(256) void k_sort_groups(const uint64_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ group_start,
+                                                     uint32_t* __restrict__ order)
 {
-  __shared__ uint64_t a[kSortLds];
+  __shared__ uint32_t bucket[kSortLds];  // particle indices by cell, unordered within a cell
+  __shared__ uint32_t pidx[kSortLds];    // particle index of each key position
+  __shared__ uint16_t code[kSortLds];    // cell | slot << 5 of each key position
+  __shared__ uint32_t cnt[32], off[32];
+  static_assert(kSortLds <= 2048, "slot << 5 fits 16 bits");
+  const int tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
   if (n > (uint32_t)kSortLds) return;
-  uint32_t P = 1;
-  while (P < n) P <<= 1;
-  for (uint32_t k = threadIdx.x; k < P; k += blockDim.x) a[k] = (k < n) ? keys[start + k] : ~0ull;
+  if (tid < 32) cnt[tid] = 0u;
   __syncthreads();
-  for (uint32_t size = 2; size <= P; size <<= 1)
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t k = threadIdx.x; k < P; k += blockDim.x) {
-        uint32_t l = k ^ stride;
-        if (l > k) {
-          bool up = (k & size) == 0;
-          uint64_t x = a[k], y = a[l];
-          if ((x > y) == up) {
-            a[k] = y;
-            a[l] = x;
-          }
-        }
-      }
-      __syncthreads();
+  for (uint32_t k = tid; k < n; k += 256) {
+    const uint64_t key = keys[start + k];
+    const uint32_t c = (uint32_t)(key >> 32) & 31u;
+    const uint32_t slot = atomicAdd(&cnt[c], 1u);
+    code[k] = (uint16_t)(c | (slot << 5));
+    pidx[k] = (uint32_t)key;
+  }
+  __syncthreads();
+  if (tid < 64) {  // exclusive scan of the 32 cell counts
+    const uint32_t v = tid < 32 ? cnt[tid] : 0u;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o, 64);
+      if (tid >= o) incl += y;
     }
-  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) order[start + k] = (uint32_t)(a[k] & 0xffffffffull);
+    if (tid < 32) off[tid] = incl - v;
+  }
+  __syncthreads();
+  for (uint32_t k = tid; k < n; k += 256) {
+    const uint32_t cs = code[k];
+    bucket[off[cs & 31u] + (cs >> 5)] = pidx[k];
+  }
+  __syncthreads();
+  for (uint32_t k = tid; k < n; k += 256) {
+    const uint32_t c = code[k] & 31u, i = pidx[k];
+    const uint32_t b0 = off[c], b1 = b0 + cnt[c];
+    uint32_t rank = 0;
+    for (uint32_t j = b0; j < b1; ++j) rank += bucket[j] < i ? 1u : 0u;
+    order[start + b0 + rank] = i;
+  }
 }
 
 // pages holding more than kSortLds particles: rank sort (keys are unique)
@@ -434,9 +477,37 @@ __device__ __forceinline__ void particle_kirchhoff(const ParticlesView& P, uint3
     fcr_kirchhoff(P.mu[i], P.lam[i], f, tau);
 }
 
+// the stress in precision T: a NeoHookeanBorden instance evaluates it in T,
+// every other model in FP64 (its SVD), rounded to T
+template <int MODEL, class T>
+__device__ __forceinline__ void particle_kirchhoff_t(const ParticlesView& P, uint32_t i, M3<T>& tau)
+{
+  if constexpr (std::is_same<T, float>::value && MODEL == ZR_MODEL_NEO_HOOKEAN_BORDEN) {
+    M3<float> f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) f.a[r][c] = (float)P.F[(c * 3 + r) * P.stride + i];
+    nhb_kirchhoff((float)P.mu[i], (float)P.lam[i], (float)P.gdeg[i], f, tau);
+  } else {
+    M3<double> f, t64;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * P.stride + i];
+    particle_kirchhoff<MODEL>(P, i, f, t64);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tau.a[r][c] = (T)t64.a[r][c];
+  }
+}
+
 // FORCE = true: MLS with force (explicit).  FORCE = false: APIC mass and
 // momentum only, particlesToGridHelper<APIC> (MpmSimulationBase.cpp:563-608).
-template <bool FORCE, int MODEL = kAnyModel>
+// T: the precision of the per-particle terms (stress, weights, products);
+// the node sums are FP64 either way
+template <bool FORCE, int MODEL = kAnyModel, class T = double>
 __global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const uint32_t* __restrict__ order,
                                              const uint32_t* __restrict__ group_start,
                                              const uint32_t* __restrict__ group_page, double* __restrict__ stage,
@@ -465,17 +536,13 @@ __global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const
     if (j >= n) continue;
     const uint32_t i = order[start + j];
     double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
-    const double mass = P.m[i];
-    double mom[3] = { mass * P.v[i], mass * P.v[S + i], mass * P.v[2 * S + i] };
-    double mC[3][3], st[3][3];
+    const T mass = (T)P.m[i];
+    T mom[3] = { mass * (T)P.v[i], mass * (T)P.v[S + i], mass * (T)P.v[2 * S + i] };
+    T mC[3][3], st[3][3];
     if constexpr (FORCE) {
-      M3<double> f, tau;
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
-      particle_kirchhoff<MODEL>(P, i, f, tau);
-      const double vol = P.vol[i];
+      M3<T> tau;
+      particle_kirchhoff_t<MODEL, T>(P, i, tau);
+      const T vol = (T)P.vol[i];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -484,39 +551,50 @@ __global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int r = 0; r < 3; ++r) st[r][c] += P.vtau_add[(c * 3 + r) * S + i];
+          for (int r = 0; r < 3; ++r) st[r][c] += (T)P.vtau_add[(c * 3 + r) * S + i];
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int r = 0; r < 3; ++r) mC[r][c] = mass * P.C[(c * 3 + r) * S + i];
-    double w[3][3];
+      for (int r = 0; r < 3; ++r) mC[r][c] = mass * (T)P.C[(c * 3 + r) * S + i];
+    // base node and the offsets x_i - x_p in FP64, the weights and the
+    // products in T
+    T w[3][3], off[3][3];
     int b[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) b[d] = bspline_w(xp[d] * inv_dx, w[d]);
+    for (int d = 0; d < 3; ++d) {
+      double w64[3];
+      b[d] = bspline_w(xp[d] * inv_dx, w64);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        w[d][a] = (T)w64[a];
+        off[d][a] = (T)((double)(b[d] + a) * dx - xp[d]);
+      }
+    }
     const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
     if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
+    const T Dinv_t = (T)Dinv;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const double dx0 = (double)(b[0] + a) * dx - xp[0];
+      const T dx0 = off[0][a];
 #pragma unroll
       for (int bb = 0; bb < 3; ++bb) {
-        const double wab = w[0][a] * w[1][bb];
-        const double dx1 = (double)(b[1] + bb) * dx - xp[1];
+        const T wab = w[0][a] * w[1][bb];
+        const T dx1 = off[1][bb];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const double wt_ = wab * w[2][c];
-          const double dx2 = (double)(b[2] + c) * dx - xp[2];
+          const T wt_ = wab * w[2][c];
+          const T dx2 = off[2][c];
           const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
-          const double fw = wt_ * Dinv;
-          atomicAdd(&wt[t], mass * wt_);
+          const T fw = wt_ * Dinv_t;
+          atomicAdd(&wt[t], (double)(mass * wt_));
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
-            double vd = mC[r][0] * dx0 + mC[r][1] * dx1 + mC[r][2] * dx2 + mom[r];
-            atomicAdd(&wt[(1 + r) * kTileNodes + t], vd * wt_);
+            T vd = mC[r][0] * dx0 + mC[r][1] * dx1 + mC[r][2] * dx2 + mom[r];
+            atomicAdd(&wt[(1 + r) * kTileNodes + t], (double)(vd * wt_));
             if constexpr (FORCE) {
-              double fd = st[r][0] * dx0 + st[r][1] * dx1 + st[r][2] * dx2;
-              atomicAdd(&wt[(4 + r) * kTileNodes + t], -(fd * fw));
+              T fd = st[r][0] * dx0 + st[r][1] * dx1 + st[r][2] * dx2;
+              atomicAdd(&wt[(4 + r) * kTileNodes + t], (double)(-(fd * fw)));
             }
           }
         }
@@ -979,6 +1057,7 @@ struct zr_engine {
   int32_t* page_group = nullptr;
   uint32_t* page_bits = nullptr;
   uint32_t* page_sum = nullptr;
+  uint32_t* page_sumcnt = nullptr;
   uint32_t* page_touch = nullptr;
   int32_t* page_slot = nullptr;
 
@@ -1337,11 +1416,13 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   k_bin<<<blocks_for(n, 256), 256, 0, s>>>(e->x, e->n, n, 1.0 / e->cfg.dx, (int)e->lim, e->page_count, e->page_bits,
                                            e->page_sum, e->p_page, e->p_re, e->dcount);
   ZR_LAUNCH_CHECK(e);
-  int rc = run_scan(e, LdPopcSum{ e->page_sum, e->page_bits }, e->nsum, e->sum_base, e->dcount + CNT_GROUPS);
+  uint32_t nbs = (e->nsum + 7u) / 8u;
+  if (nbs > 2048u) nbs = 2048u;
+  k_sum_counts<<<nbs, 256, 0, s>>>(e->page_sum, e->nsum, e->page_bits, e->page_sumcnt);
+  int rc = run_scan(e, LdArray{ e->page_sumcnt }, e->nsum, e->sum_base, e->dcount + CNT_GROUPS);
   if (rc) return rc;
   {
-    uint32_t nb = (e->nsum + 7u) / 8u;
-    if (nb > 2048u) nb = 2048u;
+    const uint32_t nb = nbs;
     k_list_groups<<<nb, 256, 0, s>>>(e->page_sum, e->nsum, e->page_bits, e->sum_base, e->page_count, e->group_page,
                                      e->page_group, e->group_count);
   }
@@ -1405,8 +1486,17 @@ int do_p2g(zr_engine* e, bool mls)
     const ParticlesView pv = pview(e);
     const double Dinv = 4.0 / (dx * dx);
     const int model = e->has_models ? e->uniform_model : ZR_MODEL_FIXED_COROTATED;
-#define ZR_P2G_LAUNCH(M) k_p2g<true, M><<<e->A, kTransferThreads, 0, e->stream>>>( \
-          pv, e->order, e->group_start, e->group_page, e->stage, dx, 1.0 / dx, Dinv)
+    const bool fp32 = e->cfg.precision == ZR_FP32_REFINED;
+#define ZR_P2G_LAUNCH(M)                                                                                     \
+  do {                                                                                                       \
+    if (fp32)                                                                                                \
+      k_p2g<true, M, float><<<e->A, kTransferThreads, 0, e->stream>>>(pv, e->order, e->group_start,          \
+                                                                      e->group_page, e->stage, dx, 1.0 / dx, \
+                                                                      Dinv);                                 \
+    else                                                                                                     \
+      k_p2g<true, M><<<e->A, kTransferThreads, 0, e->stream>>>(pv, e->order, e->group_start, e->group_page,  \
+                                                               e->stage, dx, 1.0 / dx, Dinv);                \
+  } while (0)
     if (model == ZR_MODEL_NEO_HOOKEAN_BORDEN)
       ZR_P2G_LAUNCH(ZR_MODEL_NEO_HOOKEAN_BORDEN);
     else if (model == ZR_MODEL_STVK_HENCKY)
@@ -2305,8 +2395,8 @@ int zr_create(const zr_config* cfg, zr_engine** out)
 {
   if (!cfg || !out) return ZR_ERR_ARG;
   *out = nullptr;
-  if (!(cfg->dx > 0.0) || cfg->grid_log2 < 6 || cfg->grid_log2 > 12 || cfg->precision != ZR_FP64 ||
-      cfg->transfer != ZR_TRANSFER_MLS)
+  if (!(cfg->dx > 0.0) || cfg->grid_log2 < 6 || cfg->grid_log2 > 12 ||
+      (cfg->precision != ZR_FP64 && cfg->precision != ZR_FP32_REFINED) || cfg->transfer != ZR_TRANSFER_MLS)
     return ZR_ERR_ARG;
   zr_engine* e = new zr_engine();
   e->cfg = *cfg;
@@ -2344,6 +2434,7 @@ int zr_create(const zr_config* cfg, zr_engine** out)
             hipMalloc(&e->page_slot, e->ndense * sizeof(int32_t)) == hipSuccess &&
             hipMalloc(&e->page_bits, e->nwords * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->page_sum, e->nsum * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc(&e->page_sumcnt, e->nsum * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->sum_base, (e->nsum + 1) * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->dcount, CNT_N * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->ticket, sizeof(uint32_t)) == hipSuccess &&
@@ -2354,6 +2445,7 @@ int zr_create(const zr_config* cfg, zr_engine** out)
        hipMemset(e->page_slot, 0xff, e->ndense * sizeof(int32_t)) == hipSuccess &&
        hipMemset(e->page_bits, 0, e->nwords * sizeof(uint32_t)) == hipSuccess &&
        hipMemset(e->page_sum, 0, e->nsum * sizeof(uint32_t)) == hipSuccess &&
+       hipMemset(e->page_sumcnt, 0, e->nsum * sizeof(uint32_t)) == hipSuccess &&
        hipMemset(e->ticket, 0, sizeof(uint32_t)) == hipSuccess &&
        hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
@@ -2370,7 +2462,7 @@ void zr_destroy(zr_engine* e)
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   free_particles(e);
   dfree(e->page_count); dfree(e->page_group); dfree(e->page_touch); dfree(e->page_slot); dfree(e->page_bits);
-  dfree(e->page_sum); dfree(e->sum_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
+  dfree(e->page_sum); dfree(e->page_sumcnt); dfree(e->sum_base); dfree(e->stage); dfree(e->gm); dfree(e->gv); dfree(e->gf); dfree(e->gvs);
   dfree(e->scan_tmp); dfree(e->speed); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->jstage); dfree(e->pinv); dfree(e->collided); dfree(e->c64); dfree(e->c32); dfree(e->ccell);

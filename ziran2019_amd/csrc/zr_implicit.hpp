@@ -339,6 +339,19 @@ __device__ __forceinline__ T wgradT(const T w[3][3], const T dw[3][3], T inv_dx,
 // holding all 27 gradients live)
 __device__ __forceinline__ void opaque(double& v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void opaque(float& v) { asm volatile("" : "+v"(v)); }
+// a scatter after a gather recomputes the 27 weight gradients from the 1-D
+// weights rather than holding them from the gather (register budget)
+template <class T>
+__device__ __forceinline__ void opaque_weights(T w[3][3], T dw[3][3])
+{
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      opaque(w[d][a]);
+      opaque(dw[d][a]);
+    }
+}
 
 // quotient in the operator's precision: IEEE division in FP64; in FP32 the
 // hardware reciprocal (1 ulp), well inside the FP32 operator's own rounding

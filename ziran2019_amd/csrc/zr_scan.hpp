@@ -112,23 +112,6 @@ struct LdPopc {
   const uint32_t* w;
   __device__ uint32_t operator()(uint32_t i) const { return (uint32_t)__popc(w[i]); }
 };
-// pages flagged under summary word i: the popcounts of the page_bits words
-// its set bits name
-struct LdPopcSum {
-  const uint32_t* sum;
-  const uint32_t* bits;
-  __device__ uint32_t operator()(uint32_t i) const
-  {
-    // predicated and unrolled so that the flagged words are requested
-    // together rather than one dependent load after another
-    const uint32_t sb = sum[i];
-    uint32_t c = 0;
-#pragma unroll
-    for (uint32_t b = 0; b < 32u; ++b)
-      if ((sb >> b) & 1u) c += (uint32_t)__popc(bits[i * 32u + b]);
-    return c;
-  }
-};
 // first touch of a page: the minimum touch index stored for the page is this one
 struct LdFirstTouch {
   const uint32_t* touch_page;

@@ -345,14 +345,14 @@ def grid_log2_for(dx, extent=1.0):
 class MpmEngine:
     """One GPU MPM simulation (FP64 plain path, MLS transfers)."""
 
-    def __init__(self, dx, gravity=(0.0, -9.8, 0.0), cfl=0.6, grid_log2=None):
+    def __init__(self, dx, gravity=(0.0, -9.8, 0.0), cfl=0.6, grid_log2=None, precision=ZR_FP64):
         self.L = load_library()
         cfg = ZrConfig()
         cfg.dx = float(dx)
         cfg.gravity = (ctypes.c_double * 3)(*gravity)
         cfg.cfl = float(cfl)
         cfg.grid_log2 = int(grid_log2 if grid_log2 is not None else grid_log2_for(dx))
-        cfg.precision = 0
+        cfg.precision = int(precision)  # ZR_FP32_REFINED: the explicit P2G's per-particle terms in FP32
         cfg.transfer = 0
         h = ctypes.c_void_p()
         rc = self.L.zr_create(ctypes.byref(cfg), ctypes.byref(h))
