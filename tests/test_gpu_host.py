@@ -9,6 +9,7 @@ import pytest
 
 import oracle
 from ziran2019_amd import build as zbuild
+from ziran2019_amd import partio
 from ziran2019_amd.engine import ZrError
 from _helpers import perturbed, rel_err
 
@@ -142,4 +143,9 @@ def test_simulate_with_automatic_restart_in_both_hosts(tmp_path):
     for f in range(frames + 1):
         a, b = (cdir / f"restart_{f}.dat").read_bytes(), (pdir / f"restart_{f}.dat").read_bytes()
         assert a == b, f
+        # the per-frame positions (write_partio), alike in both hosts
+        a, b = (cdir / f"partio_{f}.bgeo").read_bytes(), (pdir / f"partio_{f}.bgeo").read_bytes()
+        assert a == b, f
+    X32, _ = partio.read_bgeo(pdir / f"partio_{frames}.bgeo")
+    assert np.array_equal(X32, out["X"].astype(np.float32))
     eng.close()

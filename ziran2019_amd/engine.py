@@ -557,6 +557,28 @@ class MpmEngine:
             arrays["phase field"] = state.pack_phase_field(pf)
         state.write(f, n, arrays)
 
+    def write_partio(self, f):
+        """writePartio (Lib/Ziran/Math/Geometry/PartioIO.h:141-181), which
+        MpmSimulationBase::writeState calls when write_partio is set: every
+        particle's position in a .bgeo file (ziran2019_amd/partio.py)."""
+        from . import partio
+        partio.write_bgeo(f, self.download()["X"])
+
+    def write_phase_field_partio(self, f):
+        """The fracture project's frame output (Projects/fracture/
+        FractureSimulation.h:111-144): position, "pf" = c and "sigmaC" =
+        1 / one_over_sigma_c (the reciprocal of the float-rounded value),
+        1 for particles without a phase field."""
+        from . import partio
+        X = self.download()["X"]
+        pf, sc = np.ones(len(X)), np.ones(len(X))
+        if getattr(self, "_pf", None) is not None:
+            pf = self.get_phase_field()["c"]
+            sc = np.asarray(self._pf["one_over_sigma_c"], dtype=np.float64) * np.ones(len(X))
+        with np.errstate(divide="ignore"):
+            sigma_c = 1.0 / sc.astype(np.float32).astype(np.float64)
+        partio.write_bgeo(f, X, [("pf", partio.VECTOR, pf), ("sigmaC", partio.VECTOR, sigma_c)])
+
     def read_state(self, f):
         """MpmSimulationBase::readState: the particles of a state written by
         write_state (or by the C++ host) uploaded to the device, with the

@@ -386,9 +386,44 @@ void MpmSimulationGpu::readState(std::istream& in)
 
 std::string MpmSimulationGpu::outputFileName() const { return "restart_" + std::to_string(frame) + ".dat"; }
 
+namespace {
+void put_be32(std::ostream& out, uint32_t v)
+{
+  const char b[4] = { (char)(v >> 24), (char)(v >> 16), (char)(v >> 8), (char)v };
+  out.write(b, 4);
+}
+void put_be_float(std::ostream& out, float x)
+{
+  uint32_t v;
+  std::memcpy(&v, &x, 4);
+  put_be32(out, v);
+}
+}  // namespace
+
+void MpmSimulationGpu::writePartio(const std::string& path)
+{
+  if (eng_) downloadParticles();
+  std::ofstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  const size_t n = particles.count();
+  put_be32(f, ((uint32_t)'B' << 24) | ((uint32_t)'g' << 16) | ((uint32_t)'e' << 8) | (uint32_t)'o');
+  f.put('V');
+  put_be32(f, 5);
+  put_be32(f, (uint32_t)n);
+  for (int k = 0; k < 7; ++k) put_be32(f, 0);  // primitives, groups, attributes other than position
+  for (size_t i = 0; i < n; ++i) {
+    for (int d = 0; d < 3; ++d) put_be_float(f, (float)particles.X[i][d]);
+    put_be_float(f, 1.0f);
+  }
+  f.put((char)0x00);
+  f.put((char)0xff);
+  if (!f) throw std::runtime_error("partio output: write failed");
+}
+
 void MpmSimulationGpu::write(const std::string& filename)
 {
   if (!write_files) return;
+  if (write_partio) writePartio(output_dir + "/partio_" + std::to_string(frame) + ".bgeo");
   std::ofstream f(output_dir + "/" + filename, std::ios::binary);
   if (!f) throw std::runtime_error("cannot open " + output_dir + "/" + filename);
   writeState(f);

@@ -92,6 +92,9 @@ public:
   int frame = 0, start_frame = 0, end_frame = 0, restart_frame = 0;
   bool restarting = false;
   bool write_files = true;
+  // MpmSimulationBase::write_partio (MpmSimulationBase.cpp:40,736-739): each
+  // written state also writes partio_<frame>.bgeo
+  bool write_partio = true;
   std::string output_dir = ".";
   std::vector<std::function<void(int)>> restart_callbacks;
 
@@ -141,6 +144,12 @@ public:
   // readState uploads what it read, so the device holds that state again.
   virtual void writeState(std::ostream& out);
   virtual void readState(std::istream& in);
+  // writePartio (Lib/Ziran/Math/Geometry/PartioIO.h:141-181): every
+  // particle's position in Houdini's classic binary geometry format as
+  // Partio's BGEO writer lays it out (wdas/partio v1.1.0): big-endian header
+  // 'Bgeo' 'V' 5, the point count and seven zero counts, then x, y, z, 1 as
+  // float32 per point, then the bytes 0x00 0xff
+  void writePartio(const std::string& path);
   // SimulationBase::write / read (SimulationBase.h:149-187): the state in
   // output_dir/filename (and last_written.txt with the frame)
   virtual void write(const std::string& filename);

@@ -21,9 +21,12 @@ from .engine import RestartException, TimeStepping
 
 class Simulation:
     def __init__(self, engine, step=None, output_dir=".", start_frame=0, end_frame=0, autorestart=True,
-                 substep=None, write_files=True):
+                 substep=None, write_files=True, write_partio=True):
         """``substep(dt)`` is the time step (engine.advanceOneTimeStep by
-        default; the implicit or ADMM step with their options otherwise)."""
+        default; the implicit or ADMM step with their options otherwise).
+        ``write_partio``: each written state also writes partio_<frame>.bgeo
+        (MpmSimulationBase::write_partio, true by default,
+        MpmSimulationBase.cpp:40,736-739)."""
         self.engine = engine
         self.step = step if step is not None else TimeStepping()
         self.output_dir = output_dir
@@ -31,6 +34,7 @@ class Simulation:
         self.end_frame = end_frame
         self.autorestart = autorestart
         self.write_files = write_files
+        self.write_partio = write_partio
         self.substep = substep if substep is not None else engine.advanceOneTimeStep
         self.restarting = False
         self.restart_callbacks = []
@@ -77,6 +81,8 @@ class Simulation:
         """SimulationBase::write (SimulationBase.h:149-163)."""
         if not self.write_files:
             return
+        if self.write_partio:
+            self.engine.write_partio(os.path.join(self.output_dir, f"partio_{self.frame}.bgeo"))
         self.engine.write_state(os.path.join(self.output_dir, filename))
         with open(os.path.join(self.output_dir, "last_written.txt"), "w") as f:
             f.write(str(self.frame))
