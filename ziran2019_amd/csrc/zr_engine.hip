@@ -1028,6 +1028,14 @@ __global__ void k_dump_base_offset(uint32_t n, const uint32_t* p_page, const uin
   uint64_t cell = ((uint64_t)p_page[i] << 5) | (p_re[i] & 31u);
   out[i] = cell << 7;
 }
+// the nodes with mass, ascending: act[scan[k]] = k
+__global__ void k_active_list(uint32_t nn, const double* __restrict__ m, const uint32_t* __restrict__ scan,
+                              uint32_t* __restrict__ act)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nn && m[k] != 0.0) act[scan[k]] = k;
+}
+
 __global__ void k_node_idx(uint32_t nn, const double* m, const uint32_t* scan, int64_t* idx)
 {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1151,6 +1159,12 @@ struct zr_engine {
   double* sc = nullptr;
   int32_t* ci = nullptr;
   uint32_t* hvp_ctr = nullptr;     // work counters of the persistent operator apply
+  // the slot nodes with mass in ascending order, listed once per implicit
+  // step for the CG node kernels, and their count and block count
+  uint32_t* act = nullptr;
+  uint32_t* act_scan = nullptr;
+  size_t act_cap = 0;
+  uint32_t nact = 0, ncg_act = 1;
   uint32_t* hvp_bounds = nullptr;  // its XCD group ranges, FP32 then FP64 (2 x 9)
   uint32_t* hvp_order = nullptr;   // their hand-out orders (2 x hvp_order_cap)
   uint8_t* cell_order = nullptr;   // each group's cells by particle count (32 x hvp_order_cap)
@@ -2113,6 +2127,10 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, int32_t* ci, i
   prof_mark(e, ZR_PROF_HVP);
   const uint32_t resident = e->hvp_resident[sizeof(T) == sizeof(double) ? 1 : 0];
   const uint32_t nblk = e->A < resident ? e->A : resident;
+  // the CG step (qvec given) forms A p over the listed nodes with mass; the
+  // plain product writes every slot node (zeros without mass)
+  const uint32_t* act = qvec ? e->act : nullptr;
+  const uint32_t nap = qvec ? e->ncg_act : ncg;
   k_hvp<T><<<nblk, HvpChunk<T>::threads, 0, e->stream>>>(hvp_cache<T>(e), e->group_start, e->group_nslot, qvec, pvec, e->sc,
                                         (int64_t)e->grid_cap, (T)(1.0 / dx), stage, ci,
                                         e->hvp_bounds + (sizeof(T) == sizeof(double) ? 9 : 0),
@@ -2120,9 +2138,9 @@ int be_apply(zr_engine* e, T* qvec, T* pvec, T* out, int project, int32_t* ci, i
                                         e->partial2, ncg, cg_iter, e->sc, e->cell_order);
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_HVP);
-  k_cg_ap<T><<<ncg, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
+  k_cg_ap<T><<<nap, 256, 0, e->stream>>>(nn, e->slot_page, e->slot_src, stage, e->gm, nvT(e, qvec), nvT(e, pvec),
                                          nvT(e, out), dx, dt, cs, project, e->partial, e->sc, ci,
-                                         e->hvp_ctr);
+                                         e->hvp_ctr, act, e->nact);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
 }
@@ -2183,8 +2201,8 @@ int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
     int rc = be_apply<T>(e, V::q(e), V::p(e), V::ap(e), 1, e->ci, k);
     if (rc) return rc;
     k_cg_update<T><<<ncg, 256, 0, s>>>(nn, e->gm, nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
-                                       nvT(e, V::p(e)), nvT(e, V::ap(e)), pinv, e->sc, e->partial, ncg, e->partial2,
-                                       e->ci, k);
+                                       nvT(e, V::p(e)), nvT(e, V::ap(e)), pinv, e->sc, e->partial, e->ncg_act,
+                                       e->partial2, e->ci, k, nullptr, 0);
     ZR_LAUNCH_CHECK(e);
     if ((k + 1) % check == 0 && k + 1 < o->cg_max_iterations) {
       ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -2349,6 +2367,26 @@ int do_be(zr_engine* e, double dt, const zr_implicit_opts* o, zr_implicit_stats*
                                e->collided);
   ZR_HIP(e, hipMemsetAsync(e->ci, 0, CI_N * sizeof(int32_t), s));
   ZR_LAUNCH_CHECK(e);
+  // the nodes with mass, for the CG node kernels (the masses hold for the step)
+  if (e->act_cap < e->grid_cap) {
+    dfree(e->act);
+    dfree(e->act_scan);
+    e->act_cap = 0;
+    ZR_HIP(e, hipMalloc(&e->act, e->grid_cap * sizeof(uint32_t)));
+    ZR_HIP(e, hipMalloc(&e->act_scan, e->grid_cap * sizeof(uint32_t)));
+    e->act_cap = e->grid_cap;
+  }
+  rc = run_scan(e, LdNonzeroMass{ e->gm }, nn, e->act_scan, e->dcount + CNT_TOTAL);
+  if (rc) return rc;
+  if (nn) k_active_list<<<nb, 256, 0, s>>>(nn, e->gm, e->act_scan, e->act);
+  ZR_LAUNCH_CHECK(e);
+  rc = sync_counts(e);
+  if (rc) return rc;
+  e->nact = e->hcount[CNT_TOTAL];
+  {
+    const uint32_t b = blocks_for(e->nact, 256);
+    e->ncg_act = b == 0 ? 1u : (b < kCgBlocks ? b : kCgBlocks);
+  }
   e->be_ready = true;
   rc = o->precision == ZR_FP32_REFINED ? be_newton<CgVecs32>(e, o, local) : be_newton<CgVecs64>(e, o, local);
   if (rc) return rc;
@@ -2466,7 +2504,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->scan_tmp); dfree(e->speed); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->jstage); dfree(e->pinv); dfree(e->collided); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
-  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll); dfree(e->admm_c32); dfree(e->admm_cell32); dfree(e->admm_node32);
+  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll); dfree(e->act); dfree(e->act_scan); dfree(e->admm_c32); dfree(e->admm_cell32); dfree(e->admm_node32);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   dfree(e->mr64); dfree(e->mr32);
   if (e->hsc) (void)hipHostFree(e->hsc);

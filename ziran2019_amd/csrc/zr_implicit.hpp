@@ -1368,17 +1368,21 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
                                                const double* __restrict__ gm, NodeVec<T> q, NodeVec<T> p,
                                                NodeVec<T> ap, double dx, double dt, ColliderSet cs,
                                                int project, double* __restrict__ partial, const double* sc,
-                                               const int32_t* ci, uint32_t* __restrict__ hvp_ctr)
+                                               const int32_t* ci, uint32_t* __restrict__ hvp_ctr,
+                                               const uint32_t* __restrict__ act, uint32_t nact)
 {
   // the operator apply this kernel follows has drained its work counters
   if (blockIdx.x == 0 && threadIdx.x < kHvpCounters) hvp_ctr[kHvpCounterStride * threadIdx.x] = 0u;
+  // act: the nodes with mass (ascending), when the caller has listed them;
+  // otherwise every slot node
   // the first node's operands and its page's source groups are requested
   // before the flag and beta are known, so that the staged sums are one
   // dependent request away from the launch
   // A node without mass holds zeros in every CG vector (the start sets
   // them so and nothing adds to them), so its direction and A p are not
   // touched in a solve; the plain product (no q) still writes its zero A p.
-  const uint32_t k0 = blockIdx.x * 256u + threadIdx.x, kstep = gridDim.x * 256u;
+  const uint32_t t0 = blockIdx.x * 256u + threadIdx.x, tstep = gridDim.x * 256u;
+  const uint32_t count = act ? nact : nn;
   double m = 0.0;
   int32_t src8[8];
   T qv[3], pv0[3];
@@ -1393,12 +1397,13 @@ __global__ __launch_bounds__(256) void k_cg_ap(uint32_t nn, const uint32_t* __re
       }
     }
   };
-  if (k0 < nn) fetch(k0);
+  if (t0 < count) fetch(act ? act[t0] : t0);
   if (ci && ci[CI_DONE]) return;
   double contrib = 0.0;
   const T beta = q.p ? (T)sc[SC_BETA] : (T)0;
-  for (uint32_t k = k0; k < nn; k += kstep) {
-    if (k != k0) fetch(k);
+  for (uint32_t t = t0; t < count; t += tstep) {
+    const uint32_t k = act ? act[t] : t;
+    if (t != t0) fetch(k);
     if (m == 0.0) {
       if (!q.p)
         for (int d = 0; d < 3; ++d) ap(d, k) = (T)0;
@@ -1445,11 +1450,14 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
                                                    NodeVec<T> pinv, double* __restrict__ sc,
                                                    const double* __restrict__ pap_partial,
                                                    uint32_t n_pap, double* __restrict__ upd_partial,
-                                                   const int32_t* ci, int cg_iter)
+                                                   const int32_t* ci, int cg_iter,
+                                                   const uint32_t* __restrict__ act, uint32_t nact)
 {
   // this thread's first node's operands are requested before alpha is
-  // known (they do not depend on it), so that the two waits overlap
-  const uint32_t k0 = blockIdx.x * 256u + threadIdx.x, kstep = gridDim.x * 256u;
+  // known (they do not depend on it), so that the two waits overlap;
+  // act: the nodes with mass (ascending) when listed, else every slot node
+  const uint32_t t0 = blockIdx.x * 256u + threadIdx.x, tstep = gridDim.x * 256u;
+  const uint32_t count = act ? nact : nn;
   double m = 0.0;
   T xv[3], pv[3], rv[3], av[3], jv[3];
   // a node without mass keeps its zeros (see k_cg_ap)
@@ -1466,7 +1474,7 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
       }
     }
   };
-  if (k0 < nn) fetch(k0);
+  if (t0 < count) fetch(act ? act[t0] : t0);
   if (ci[CI_DONE]) return;
   __shared__ double red[5];
   const double pap = block_total256(pap_partial, n_pap, red);
@@ -1477,8 +1485,9 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
   }
   const T alpha = (T)a64;
   double contrib = 0.0;
-  for (uint32_t k = k0; k < nn; k += kstep) {
-    if (k != k0) fetch(k);
+  for (uint32_t t = t0; t < count; t += tstep) {
+    const uint32_t k = act ? act[t] : t;
+    if (t != t0) fetch(k);
     if (m == 0.0) continue;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
