@@ -2202,7 +2202,7 @@ int be_cg(zr_engine* e, double rel, const zr_implicit_opts* o)
     if (rc) return rc;
     k_cg_update<T><<<ncg, 256, 0, s>>>(nn, e->gm, nvT(e, V::x(e)), nvT(e, V::r(e)), nvT(e, V::q(e)),
                                        nvT(e, V::p(e)), nvT(e, V::ap(e)), pinv, e->sc, e->partial, e->ncg_act,
-                                       e->partial2, e->ci, k, nullptr, 0);
+                                       e->partial2, e->ci, k);
     ZR_LAUNCH_CHECK(e);
     if ((k + 1) % check == 0 && k + 1 < o->cg_max_iterations) {
       ZR_HIP(e, hipMemcpyAsync(e->hci, e->ci, CI_N * sizeof(int32_t), hipMemcpyDeviceToHost, s));

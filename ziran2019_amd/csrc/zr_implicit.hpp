@@ -1450,14 +1450,11 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
                                                    NodeVec<T> pinv, double* __restrict__ sc,
                                                    const double* __restrict__ pap_partial,
                                                    uint32_t n_pap, double* __restrict__ upd_partial,
-                                                   const int32_t* ci, int cg_iter,
-                                                   const uint32_t* __restrict__ act, uint32_t nact)
+                                                   const int32_t* ci, int cg_iter)
 {
   // this thread's first node's operands are requested before alpha is
-  // known (they do not depend on it), so that the two waits overlap;
-  // act: the nodes with mass (ascending) when listed, else every slot node
-  const uint32_t t0 = blockIdx.x * 256u + threadIdx.x, tstep = gridDim.x * 256u;
-  const uint32_t count = act ? nact : nn;
+  // known (they do not depend on it), so that the two waits overlap
+  const uint32_t k0 = blockIdx.x * 256u + threadIdx.x, kstep = gridDim.x * 256u;
   double m = 0.0;
   T xv[3], pv[3], rv[3], av[3], jv[3];
   // a node without mass keeps its zeros (see k_cg_ap)
@@ -1474,7 +1471,7 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
       }
     }
   };
-  if (t0 < count) fetch(act ? act[t0] : t0);
+  if (k0 < nn) fetch(k0);
   if (ci[CI_DONE]) return;
   __shared__ double red[5];
   const double pap = block_total256(pap_partial, n_pap, red);
@@ -1485,9 +1482,8 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t nn, const double* __
   }
   const T alpha = (T)a64;
   double contrib = 0.0;
-  for (uint32_t t = t0; t < count; t += tstep) {
-    const uint32_t k = act ? act[t] : t;
-    if (t != t0) fetch(k);
+  for (uint32_t k = k0; k < nn; k += kstep) {
+    if (k != k0) fetch(k);
     if (m == 0.0) continue;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
