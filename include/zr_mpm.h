@@ -412,6 +412,19 @@ int zr_adjoint_get(zr_engine* e, zr_particles_adjoint* a);
 int zr_adjoint_step(zr_engine* e, double dt);
 int zr_checkpoint_save(zr_engine* e, int32_t slot);
 int zr_checkpoint_restore(zr_engine* e, int32_t slot);
+/* The checkpointed rollout's two loops in one call each.  No counterpart in
+   the reference; for rollout drivers (ziran2019_amd/autodiff.py) that would
+   otherwise pay a host round trip per step and per checkpoint.
+   zr_rollout_forward: steps explicit steps of size dt from the current
+   state, saving checkpoint slot0 + k / every before step k when k is a
+   multiple of every; stops at the first error or ZR_CFL_RESTART.
+   zr_rollout_backward: after zr_adjoint_set on the rollout's output, per
+   segment from the last: restore its checkpoint, save the inputs of its
+   steps in slots scratch0 .. scratch0 + every - 1 (recomputing them), then
+   run zr_adjoint_step for each, last first; the adjoint then holds that of
+   the rollout's input. */
+int zr_rollout_forward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0);
+int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0, int32_t scratch0);
 
 /* Introspection for parity tests (reference state after the named call):
    sizes in bytes via zr_debug_size, contents via zr_debug_dump. */

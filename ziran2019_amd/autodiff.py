@@ -48,10 +48,7 @@ class Rollout:
         eng.upload(s)
         eng.set_colliders(self.colliders)
         eng.set_time(0.0)
-        for k in range(self.steps):
-            if k % self.every == 0:
-                eng.checkpoint_save(k // self.every)
-            eng.advanceOneTimeStep(self.dt)
+        eng.rollout_forward(self.steps, self.dt, self.every, 0)
         return eng.download()["X"]
 
     def backward(self, gX, gV=None):
@@ -59,18 +56,10 @@ class Rollout:
         final (X, V) is (gX, gV); runs after forward()."""
         eng = self.eng
         eng.adjoint_set(X=gX, V=gV)
-        scratch = self.nseg
-        for seg in range(self.nseg - 1, -1, -1):
-            k0 = seg * self.every
-            k1 = min(k0 + self.every, self.steps)
-            eng.checkpoint_restore(seg)
-            for k in range(k0, k1):
-                eng.checkpoint_save(scratch + (k - k0))
-                if k < k1 - 1:
-                    eng.advanceOneTimeStep(self.dt)
-            for k in range(k1 - 1, k0 - 1, -1):
-                eng.checkpoint_restore(scratch + (k - k0))
-                eng.adjoint_step(self.dt)
+        # per segment from the last: restore its checkpoint, recompute its
+        # steps' inputs into the scratch slots, then the adjoint steps in
+        # reverse (zr_rollout_backward)
+        eng.rollout_backward(self.steps, self.dt, self.every, 0, self.nseg)
         a = eng.adjoint_get()
         nu = self.nu
         gE = a["mu"] / (2.0 * (1.0 + nu)) + a["lam"] * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))

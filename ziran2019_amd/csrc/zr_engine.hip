@@ -3038,38 +3038,135 @@ int zr_adjoint_step(zr_engine* e, double dt)
   return ZR_OK;
 }
 
-int zr_checkpoint_save(zr_engine* e, int32_t slot)
+namespace {
+// the checkpoint arrays X, V, C, F and a 24n-double checkpoint, copied in
+// one pass (to: state -> checkpoint, else back)
+__global__ __launch_bounds__(256) void k_ckpt_copy(double* x, double* v, double* C, double* F, double* c, uint64_t N,
+                                                   int to)
 {
-  if (!e || slot < 0 || slot > 4096) return fail(e, ZR_ERR_ARG, "bad checkpoint slot");
+  const uint64_t total = 24 * N, stride = (uint64_t)gridDim.x * 256u;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < total; i += stride) {
+    double* a;
+    uint64_t j;
+    if (i < 3 * N) {
+      a = x;
+      j = i;
+    } else if (i < 6 * N) {
+      a = v;
+      j = i - 3 * N;
+    } else if (i < 15 * N) {
+      a = C;
+      j = i - 6 * N;
+    } else {
+      a = F;
+      j = i - 15 * N;
+    }
+    if (to)
+      c[i] = a[j];
+    else
+      a[j] = c[i];
+  }
+}
+
+// checkpoint save / restore, enqueued on the engine's stream
+int ckpt_save(zr_engine* e, int32_t slot)
+{
+  if (slot < 0 || slot > 4096) return fail(e, ZR_ERR_ARG, "bad checkpoint slot");
   const size_t N = (size_t)e->n;
   if ((size_t)slot >= e->ckpt.size()) {
     e->ckpt.resize((size_t)slot + 1, nullptr);
     e->ckpt_time.resize((size_t)slot + 1, 0.0);
   }
   if (!e->ckpt[slot]) ZR_HIP(e, hipMalloc(&e->ckpt[slot], 24 * (N ? N : 1) * sizeof(double)));
-  double* c = e->ckpt[slot];
-  ZR_HIP(e, hipMemcpyAsync(c, e->x, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-  ZR_HIP(e, hipMemcpyAsync(c + 3 * N, e->v, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-  ZR_HIP(e, hipMemcpyAsync(c + 6 * N, e->C, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-  ZR_HIP(e, hipMemcpyAsync(c + 15 * N, e->F, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  if (N) {
+    const uint64_t b = blocks_for(24 * (uint64_t)N, 256);
+    k_ckpt_copy<<<b < 2048 ? (uint32_t)b : 2048u, 256, 0, e->stream>>>(e->x, e->v, e->C, e->F, e->ckpt[slot], N, 1);
+    ZR_LAUNCH_CHECK(e);
+  }
   e->ckpt_time[slot] = e->time;
+  return ZR_OK;
+}
+
+int ckpt_restore(zr_engine* e, int32_t slot)
+{
+  if (slot < 0 || (size_t)slot >= e->ckpt.size() || !e->ckpt[slot])
+    return fail(e, ZR_ERR_ARG, "no such checkpoint");
+  const size_t N = (size_t)e->n;
+  if (N) {
+    const uint64_t b = blocks_for(24 * (uint64_t)N, 256);
+    k_ckpt_copy<<<b < 2048 ? (uint32_t)b : 2048u, 256, 0, e->stream>>>(e->x, e->v, e->C, e->F, e->ckpt[slot], N, 0);
+    ZR_LAUNCH_CHECK(e);
+  }
+  e->time = e->ckpt_time[slot];
+  e->sorted = e->rasterised = e->updated = e->be_ready = false;
+  return ZR_OK;
+}
+}  // namespace
+
+int zr_checkpoint_save(zr_engine* e, int32_t slot)
+{
+  if (!e) return ZR_ERR_ARG;
+  int rc = ckpt_save(e, slot);
+  if (rc) return rc;
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
   return ZR_OK;
 }
 
 int zr_checkpoint_restore(zr_engine* e, int32_t slot)
 {
-  if (!e || slot < 0 || (size_t)slot >= e->ckpt.size() || !e->ckpt[slot])
-    return fail(e, ZR_ERR_ARG, "no such checkpoint");
-  const size_t N = (size_t)e->n;
-  const double* c = e->ckpt[slot];
-  ZR_HIP(e, hipMemcpyAsync(e->x, c, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-  ZR_HIP(e, hipMemcpyAsync(e->v, c + 3 * N, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-  ZR_HIP(e, hipMemcpyAsync(e->C, c + 6 * N, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-  ZR_HIP(e, hipMemcpyAsync(e->F, c + 15 * N, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+  if (!e) return ZR_ERR_ARG;
+  int rc = ckpt_restore(e, slot);
+  if (rc) return rc;
   ZR_HIP(e, hipStreamSynchronize(e->stream));
-  e->time = e->ckpt_time[slot];
-  e->sorted = e->rasterised = e->updated = e->be_ready = false;
+  return ZR_OK;
+}
+
+// the checkpointed rollout's loops (the same calls as the Python driver's,
+// without a host round trip per call)
+int zr_rollout_forward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (steps < 0 || every < 1 || slot0 < 0 || !(dt > 0.0)) return fail(e, ZR_ERR_ARG, "bad rollout arguments");
+  for (int32_t k = 0; k < steps; ++k) {
+    if (k % every == 0) {
+      int rc = ckpt_save(e, slot0 + k / every);
+      if (rc) return rc;
+    }
+    int rc = zr_advance_one_time_step(e, dt, nullptr);
+    if (rc) return rc;  // an error, or ZR_CFL_RESTART: the rollout stops there
+  }
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
+  return ZR_OK;
+}
+
+int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0, int32_t scratch0)
+{
+  if (!e) return ZR_ERR_ARG;
+  if (steps < 0 || every < 1 || slot0 < 0 || scratch0 < 0 || !(dt > 0.0))
+    return fail(e, ZR_ERR_ARG, "bad rollout arguments");
+  if (!adjoint_supported(e)) return ZR_ERR_ARG;
+  if (!e->adj || e->adj_n != e->n) return fail(e, ZR_ERR_STATE, "zr_adjoint_set was not called");
+  const int32_t nseg = (steps + every - 1) / every;
+  for (int32_t seg = nseg - 1; seg >= 0; --seg) {
+    const int32_t k0 = seg * every, k1 = k0 + every < steps ? k0 + every : steps;
+    int rc = ckpt_restore(e, slot0 + seg);
+    if (rc) return rc;
+    for (int32_t k = k0; k < k1; ++k) {  // the segment's step inputs
+      rc = ckpt_save(e, scratch0 + (k - k0));
+      if (rc) return rc;
+      if (k < k1 - 1) {
+        rc = zr_advance_one_time_step(e, dt, nullptr);
+        if (rc) return rc;
+      }
+    }
+    for (int32_t k = k1 - 1; k >= k0; --k) {  // the adjoint of each step, last first
+      rc = ckpt_restore(e, scratch0 + (k - k0));
+      if (rc) return rc;
+      rc = do_adjoint_step(e, dt);
+      if (rc) return rc;
+    }
+  }
+  ZR_HIP(e, hipStreamSynchronize(e->stream));
   return ZR_OK;
 }
 

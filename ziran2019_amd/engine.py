@@ -49,7 +49,7 @@ EXPORTS = (
     "zr_phase_field_last_stats", "zr_adjoint_set", "zr_adjoint_get", "zr_adjoint_step", "zr_checkpoint_save",
     "zr_checkpoint_restore", "zr_advance_one_time_step_admm", "zr_get_admm_state",
     "zr_eval_max_particle_speed", "zr_calculate_dt", "zr_set_viscoelastic", "zr_get_viscoelastic",
-    "zr_eval_max_particle_speed_noalloc",
+    "zr_eval_max_particle_speed_noalloc", "zr_rollout_forward", "zr_rollout_backward",
 )
 ZR_MODEL_FIXED_COROTATED, ZR_MODEL_STVK_HENCKY, ZR_MODEL_NEO_HOOKEAN_BORDEN = 0, 1, 2
 ZR_PLASTIC_NONE, ZR_PLASTIC_VON_MISES, ZR_PLASTIC_DRUCKER_PRAGER, ZR_PLASTIC_CAM_CLAY = 0, 1, 2, 3
@@ -321,6 +321,9 @@ def load_library(path=LIB_PATH):
     L.zr_adjoint_step.argtypes = [vp, ctypes.c_double]
     L.zr_checkpoint_save.argtypes = [vp, ctypes.c_int32]
     L.zr_checkpoint_restore.argtypes = [vp, ctypes.c_int32]
+    L.zr_rollout_forward.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32]
+    L.zr_rollout_backward.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32]
     L.zr_profile_enable.argtypes = [vp, ctypes.c_int32]
     L.zr_profile_read.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_int64)]
@@ -515,6 +518,21 @@ class MpmEngine:
 
     def checkpoint_restore(self, slot):
         self._check(self.L.zr_checkpoint_restore(self.h, int(slot)), "checkpoint_restore")
+
+    def rollout_forward(self, steps, dt, every, slot0=0):
+        """``steps`` explicit steps with a checkpoint (slot slot0 + k // every)
+        before every every-th step, in one library call."""
+        self._check(self.L.zr_rollout_forward(self.h, int(steps), float(dt), int(every), int(slot0)),
+                    "rollout_forward")
+
+    def rollout_backward(self, steps, dt, every, slot0=0, scratch0=None):
+        """The adjoint of a rollout_forward (after adjoint_set on its output):
+        the segments in reverse, each recomputed from its checkpoint into the
+        scratch slots, in one library call."""
+        nseg = (int(steps) + int(every) - 1) // int(every)
+        scratch0 = slot0 + nseg if scratch0 is None else scratch0
+        self._check(self.L.zr_rollout_backward(self.h, int(steps), float(dt), int(every), int(slot0),
+                                               int(scratch0)), "rollout_backward")
 
     def get_materials(self):
         n = self.n
