@@ -186,11 +186,13 @@ __device__ __forceinline__ void trial_collision(const ColliderSet& cs, int fr, c
 // ---------------------------------------------------------------------------
 // binning and sort (MpmSimulationBase.cpp:1032-1103)
 // ---------------------------------------------------------------------------
+// (and zeroes the sort's counters)
 __global__ void k_reset_tables(const uint32_t* group_page, uint32_t ngroups, const uint32_t* slot_page, uint32_t nslots,
                                uint32_t* page_count, int32_t* page_group, uint32_t* page_bits, uint32_t* page_sum,
-                               uint32_t* page_touch, int32_t* page_slot)
+                               uint32_t* page_touch, int32_t* page_slot, uint32_t* counters)
 {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (counters && t < CNT_N) counters[t] = 0u;
   if (t < ngroups) {
     uint32_t p = group_page[t];
     page_count[p] = 0u;
@@ -309,10 +311,12 @@ __global__ __launch_bounds__(256) void k_list_groups(const uint32_t* page_sum, u
   }
 }
 
+// (thread 0 also closes group_start with the particle total)
 __global__ void k_scatter(uint32_t n, const uint32_t* p_page, const uint32_t* p_re, const int32_t* page_group,
-                          const uint32_t* group_start, uint64_t* keys)
+                          uint32_t* group_start, uint64_t* keys, uint32_t ngroups, const uint32_t* total)
 {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) group_start[ngroups] = *total;
   if (i >= n) return;
   uint32_t re = p_re[i];
   int32_t g = page_group[p_page[i]];
@@ -336,7 +340,17 @@ __global__ __launch_bounds__(256) void k_sort_groups(const uint64_t* __restrict_
   const uint32_t g = blockIdx.x;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
-  if (n > (uint32_t)kSortLds) return;
+  if (n > (uint32_t)kSortLds) {
+    // a page holding more than kSortLds particles: rank sort over its keys
+    // in memory (the keys are unique)
+    for (uint32_t k = tid; k < n; k += 256) {
+      const uint64_t key = keys[start + k];
+      uint32_t rank = 0;
+      for (uint32_t j = 0; j < n; ++j) rank += keys[start + j] < key ? 1u : 0u;
+      order[start + rank] = (uint32_t)(key & 0xffffffffull);
+    }
+    return;
+  }
   if (tid < 32) cnt[tid] = 0u;
   __syncthreads();
   for (uint32_t k = tid; k < n; k += 256) {
@@ -369,21 +383,6 @@ __global__ __launch_bounds__(256) void k_sort_groups(const uint64_t* __restrict_
     uint32_t rank = 0;
     for (uint32_t j = b0; j < b1; ++j) rank += bucket[j] < i ? 1u : 0u;
     order[start + b0 + rank] = i;
-  }
-}
-
-// pages holding more than kSortLds particles: rank sort (keys are unique)
-__global__ __launch_bounds__(256) void k_sort_groups_big(const uint64_t* keys, const uint32_t* group_start, uint32_t* order)
-{
-  const uint32_t g = blockIdx.x;
-  const uint32_t start = group_start[g];
-  const uint32_t n = group_start[g + 1] - start;
-  if (n <= (uint32_t)kSortLds) return;
-  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-    uint64_t key = keys[start + k];
-    uint32_t rank = 0;
-    for (uint32_t j = 0; j < n; ++j) rank += keys[start + j] < key ? 1u : 0u;
-    order[start + rank] = (uint32_t)(key & 0xffffffffull);
   }
 }
 
@@ -1249,6 +1248,11 @@ int run_scan(zr_engine* e, Ld ld, uint32_t n, uint32_t* out, uint32_t* total_dev
     ZR_HIP(e, hipMemsetAsync(total_dev, 0, sizeof(uint32_t), e->stream));
     return ZR_OK;
   }
+  if (n <= kScanSingleMax) {
+    k_scan_single<Ld><<<1, 1024, 0, e->stream>>>(ld, n, out, total_dev);
+    ZR_LAUNCH_CHECK(e);
+    return ZR_OK;
+  }
   uint32_t nt = (n + kScanTile - 1) / kScanTile;
   k_scan_reduce<Ld><<<nt, kScanBlock, 0, e->stream>>>(ld, n, e->scan_tmp);
   k_scan_tile_sums<<<1, 1024, 0, e->stream>>>(e->scan_tmp, nt, total_dev);
@@ -1420,13 +1424,12 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   hipStream_t s = e->stream;
   // clear the dense tables touched by the previous sort
   uint32_t rst = e->A > e->S ? e->A : e->S;
-  if (rst)
-    k_reset_tables<<<blocks_for(rst, 256), 256, 0, s>>>(e->group_page, e->A, e->slot_page, e->S, e->page_count,
-                                                       e->page_group, e->page_bits, e->page_sum, e->page_touch,
-                                                       e->page_slot);
+  const uint32_t nrst = rst > (uint32_t)CNT_N ? rst : (uint32_t)CNT_N;
   e->sorted = e->rasterised = e->updated = false;
   prof_mark(e, ZR_PROF_SORT);
-  ZR_HIP(e, hipMemsetAsync(e->dcount, 0, CNT_N * sizeof(uint32_t), s));
+  k_reset_tables<<<blocks_for(nrst, 256), 256, 0, s>>>(e->group_page, e->A, e->slot_page, e->S, e->page_count,
+                                                      e->page_group, e->page_bits, e->page_sum, e->page_touch,
+                                                      e->page_slot, e->dcount);
   k_bin<<<blocks_for(n, 256), 256, 0, s>>>(e->x, e->n, n, 1.0 / e->cfg.dx, (int)e->lim, e->page_count, e->page_bits,
                                            e->page_sum, e->p_page, e->p_re, e->dcount);
   ZR_LAUNCH_CHECK(e);
@@ -1455,10 +1458,9 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   prof_mark(e, ZR_PROF_SORT);
   rc = run_scan(e, LdArray{ e->group_count }, A, e->group_start, e->dcount + CNT_TOTAL);
   if (rc) return rc;
-  ZR_HIP(e, hipMemcpyAsync(e->group_start + A, e->dcount + CNT_TOTAL, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-  k_scatter<<<blocks_for(n, 256), 256, 0, s>>>(n, e->p_page, e->p_re, e->page_group, e->group_start, e->keys);
+  k_scatter<<<blocks_for(n, 256), 256, 0, s>>>(n, e->p_page, e->p_re, e->page_group, e->group_start, e->keys, A,
+                                               e->dcount + CNT_TOTAL);
   k_sort_groups<<<A, 256, 0, s>>>(e->keys, e->group_start, e->order);
-  k_sort_groups_big<<<A, 256, 0, s>>>(e->keys, e->group_start, e->order);
   k_touch<<<blocks_for(8ull * A, 256), 256, 0, s>>>(A, e->group_page, e->touch_page, e->page_touch);
   ZR_LAUNCH_CHECK(e);
   rc = run_scan(e, LdFirstTouch{ e->touch_page, e->page_touch }, 8u * A, e->touch_slot, e->dcount + CNT_SLOTS);
@@ -2671,7 +2673,8 @@ int zr_upload_particles(zr_engine* e, const zr_particles* p)
     uint32_t rst = e->A > e->S ? e->A : e->S;
     k_reset_tables<<<blocks_for(rst, 256), 256, 0, e->stream>>>(e->group_page, e->A, e->slot_page, e->S,
                                                                 e->page_count, e->page_group, e->page_bits,
-                                                                e->page_sum, e->page_touch, e->page_slot);
+                                                                e->page_sum, e->page_touch, e->page_slot,
+                                                                nullptr);
     ZR_LAUNCH_CHECK(e);
     ZR_HIP(e, hipStreamSynchronize(e->stream));
     e->A = e->S = 0;

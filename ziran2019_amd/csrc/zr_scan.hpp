@@ -104,6 +104,35 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(Ld ld, uint32_t n, con
   }
 }
 
+// inputs of at most kScanSingleMax elements: one block of 1024 threads scans
+// them in one launch (each thread a run of <= 16 consecutive elements) and
+// writes the total
+constexpr uint32_t kScanSingleMax = 1024u * 16u;
+template <class Ld>
+__global__ __launch_bounds__(1024) void k_scan_single(Ld ld, uint32_t n, uint32_t* out, uint32_t* total_out)
+{
+  __shared__ uint32_t smem[1024 / 64];
+  const uint32_t chunk = (n + 1023u) / 1024u;
+  const uint32_t b = threadIdx.x * chunk;
+  uint32_t vals[16];
+  uint32_t s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 16u; ++k) {
+    const uint32_t i = b + k;
+    vals[k] = (k < chunk && i < n) ? ld(i) : 0u;
+    s += vals[k];
+  }
+  uint32_t total;
+  uint32_t off = block_exclusive_scan<1024>(s, total, smem);
+#pragma unroll
+  for (uint32_t k = 0; k < 16u; ++k) {
+    const uint32_t i = b + k;
+    if (k < chunk && i < n) out[i] = off;
+    off += vals[k];
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = total;
+}
+
 struct LdArray {
   const uint32_t* a;
   __device__ uint32_t operator()(uint32_t i) const { return a[i]; }
