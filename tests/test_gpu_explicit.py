@@ -81,6 +81,36 @@ def test_big_page_sort_bit_exact():
     assert np.array_equal(eng.dump("pages"), r["pages"])
 
 
+def test_sort_after_the_group_count_jumps():
+    """The sort launches its group kernels for the previous sort's group
+    count with headroom and repeats them at the exact count when there are
+    more groups: the same particle count first in one page, then spread over
+    many, sorted bit-exactly both times."""
+    dx = 1.0 / 64
+    rng = np.random.default_rng(12)
+    n = 4096
+    clustered = (np.array([20.0, 20.0, 20.0]) + rng.uniform(0.5, 1.5, size=(n, 3)) * np.array([1.0, 2.0, 2.0])) * dx
+    spread = (np.array([8.0, 8.0, 8.0]) + rng.uniform(0.0, 40.0, size=(n, 3))) * dx
+    eng = None
+    for X in (clustered, spread):
+        s = scenes._finish(X, dx, 1000.0, 8, *scenes.lame(1e4, 0.3), order=False)
+        s["dx"] = dx
+        if eng is None:
+            eng = MpmEngine(dx=dx)
+        eng.upload(s)
+        ora = oracle.OracleSim(dx=dx, gravity=(0.0, -9.8, 0.0), nthreads=8)
+        ora.set_particles(s["X"], s["V"], s["C"], s["F"], s["mass"], s["vol"], s["mu"], s["lam"])
+        eng.sortParticlesAndPolluteGrid()
+        ora.sort_and_pollute()
+        r = ora.sort_result()
+        print("groups", eng.stats.num_groups, "pages", eng.stats.num_pages, flush=True)
+        assert eng.stats.num_groups == len(r["group_first"])
+        for k in ("sorter", "order", "group_first", "group_last", "block_offset", "pages"):
+            assert np.array_equal(eng.dump(k), r[k]), k
+    assert eng.stats.num_groups > 200  # far above the first sort's count and its headroom
+    eng.close()
+
+
 def test_p2g_grid_parity_and_node_numbering():
     s = perturbed(3)
     eng, ora = make_pair(s)

@@ -311,15 +311,45 @@ __global__ __launch_bounds__(256) void k_list_groups(const uint32_t* page_sum, u
   }
 }
 
+// the group count the kernels after the page listing stop at: the
+// device-side count, or the launch's hint when that is smaller (the host
+// then repeats those kernels at the exact count)
+struct GroupBound {
+  const uint32_t* count;
+  uint32_t hint;
+  __device__ uint32_t get() const
+  {
+    const uint32_t a = *count;
+    return a < hint ? a : hint;
+  }
+};
+struct LdGroupCount {
+  const uint32_t* group_count;
+  GroupBound gb;
+  __device__ uint32_t operator()(uint32_t i) const { return i < gb.get() ? group_count[i] : 0u; }
+};
+// first touch of a page: the minimum touch index stored for the page is this one
+struct LdFirstTouch {
+  const uint32_t* touch_page;
+  const uint32_t* page_touch;
+  GroupBound gb;
+  __device__ uint32_t operator()(uint32_t i) const
+  {
+    return i < 8u * gb.get() && page_touch[touch_page[i]] == i ? 1u : 0u;
+  }
+};
+
 // (thread 0 also closes group_start with the particle total)
 __global__ void k_scatter(uint32_t n, const uint32_t* p_page, const uint32_t* p_re, const int32_t* page_group,
-                          uint32_t* group_start, uint64_t* keys, uint32_t ngroups, const uint32_t* total)
+                          uint32_t* group_start, uint64_t* keys, GroupBound gb, const uint32_t* total)
 {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) group_start[ngroups] = *total;
+  const uint32_t ng = gb.get();
+  if (i == 0) group_start[ng] = *total;
   if (i >= n) return;
   uint32_t re = p_re[i];
   int32_t g = page_group[p_page[i]];
+  if ((uint32_t)g >= ng) return;
   uint32_t dst = group_start[g] + (re >> 5);
   keys[dst] = ((uint64_t)(re & 31u) << 32) | (uint64_t)i;
 }
@@ -329,7 +359,7 @@ __global__ void k_scatter(uint32_t n, const uint32_t* p_page, const uint32_t* p_
 // its cell's bucket is the number of the bucket's indices below its own
 __global__ __launch_bounds__(256) void k_sort_groups(const uint64_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ group_start,
-                                                     uint32_t* __restrict__ order)
+                                                     uint32_t* __restrict__ order, GroupBound gb)
 {
   __shared__ uint32_t bucket[kSortLds];  // particle indices by cell, unordered within a cell
   __shared__ uint32_t pidx[kSortLds];    // particle index of each key position
@@ -338,6 +368,7 @@ __global__ __launch_bounds__(256) void k_sort_groups(const uint64_t* __restrict_
   static_assert(kSortLds <= 2048, "slot << 5 fits 16 bits");
   const int tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
+  if (g >= gb.get()) return;
   const uint32_t start = group_start[g];
   const uint32_t n = group_start[g + 1] - start;
   if (n > (uint32_t)kSortLds) {
@@ -388,10 +419,10 @@ __global__ __launch_bounds__(256) void k_sort_groups(const uint64_t* __restrict_
 
 // SPGrid_Page_Map::Set_Page over the 2x2x2 page neighbourhood of each group in
 // the reference loop order i(x), j(y), k(z) (MpmSimulationBase.cpp:1081-1090)
-__global__ void k_touch(uint32_t ngroups, const uint32_t* group_page, uint32_t* touch_page, uint32_t* page_touch)
+__global__ void k_touch(GroupBound gb, const uint32_t* group_page, uint32_t* touch_page, uint32_t* page_touch)
 {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 8u * ngroups) return;
+  if (t >= 8u * gb.get()) return;
   uint32_t g = t >> 3, o = t & 7u;
   uint32_t px, py, pz;
   page_decode(group_page[g], px, py, pz);
@@ -400,11 +431,11 @@ __global__ void k_touch(uint32_t ngroups, const uint32_t* group_page, uint32_t* 
   atomicMin(&page_touch[q], t);
 }
 
-__global__ void k_assign_slots(uint32_t ntouch, const uint32_t* touch_page, const uint32_t* page_touch,
+__global__ void k_assign_slots(GroupBound gb, const uint32_t* touch_page, const uint32_t* page_touch,
                                const uint32_t* touch_slot, uint32_t* slot_page, int32_t* page_slot)
 {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntouch) return;
+  if (t >= 8u * gb.get()) return;
   uint32_t q = touch_page[t];
   if (page_touch[q] != t) return;
   uint32_t s = touch_slot[t];
@@ -413,11 +444,11 @@ __global__ void k_assign_slots(uint32_t ntouch, const uint32_t* touch_page, cons
 }
 
 // the 8 neighbour page slots of every group, in touch order (o = 4a + 2b + c)
-__global__ void k_group_slots(uint32_t ntouch, const uint32_t* touch_page, const int32_t* page_slot,
+__global__ void k_group_slots(GroupBound gb, const uint32_t* touch_page, const int32_t* page_slot,
                               int32_t* group_nslot)
 {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntouch) return;
+  if (t >= 8u * gb.get()) return;
   group_nslot[t] = page_slot[touch_page[t]];
 }
 
@@ -1164,6 +1195,9 @@ struct zr_engine {
   uint32_t* act_scan = nullptr;
   size_t act_cap = 0;
   uint32_t nact = 0, ncg_act = 1;
+  // launch size of the sort's group kernels: the last sort's group count
+  // with headroom (0: read the count first)
+  uint32_t a_hint = 0;
   uint32_t* hvp_bounds = nullptr;  // its XCD group ranges, FP32 then FP64 (2 x 9)
   uint32_t* hvp_order = nullptr;   // their hand-out orders (2 x hvp_order_cap)
   uint8_t* cell_order = nullptr;   // each group's cells by particle count (32 x hvp_order_cap)
@@ -1417,6 +1451,29 @@ int ensure_grid(zr_engine* e)
   return ZR_OK;
 }
 
+// the sort after the page listing, for at most `hint` groups (GroupBound)
+int sort_groups(zr_engine* e, uint32_t hint)
+{
+  hipStream_t s = e->stream;
+  const uint32_t n = (uint32_t)e->n;
+  const GroupBound gb{ e->dcount + CNT_GROUPS, hint };
+  int rc = run_scan(e, LdGroupCount{ e->group_count, gb }, hint, e->group_start, e->dcount + CNT_TOTAL);
+  if (rc) return rc;
+  k_scatter<<<blocks_for(n, 256), 256, 0, s>>>(n, e->p_page, e->p_re, e->page_group, e->group_start, e->keys, gb,
+                                               e->dcount + CNT_TOTAL);
+  k_sort_groups<<<hint, 256, 0, s>>>(e->keys, e->group_start, e->order, gb);
+  k_touch<<<blocks_for(8ull * hint, 256), 256, 0, s>>>(gb, e->group_page, e->touch_page, e->page_touch);
+  ZR_LAUNCH_CHECK(e);
+  rc = run_scan(e, LdFirstTouch{ e->touch_page, e->page_touch, gb }, 8u * hint, e->touch_slot,
+                e->dcount + CNT_SLOTS);
+  if (rc) return rc;
+  k_assign_slots<<<blocks_for(8ull * hint, 256), 256, 0, s>>>(gb, e->touch_page, e->page_touch, e->touch_slot,
+                                                             e->slot_page, e->page_slot);
+  k_group_slots<<<blocks_for(8ull * hint, 256), 256, 0, s>>>(gb, e->touch_page, e->page_slot, e->group_nslot);
+  ZR_LAUNCH_CHECK(e);
+  return ZR_OK;
+}
+
 int do_sort(zr_engine* e, zr_step_stats* st)
 {
   if (e->n <= 0) return fail(e, ZR_ERR_STATE, "no particles uploaded");
@@ -1444,35 +1501,45 @@ int do_sort(zr_engine* e, zr_step_stats* st)
                                      e->page_group, e->group_count);
   }
   ZR_LAUNCH_CHECK(e);
+  // group sizes, in-page order and slots: launched for a hint of the group
+  // count (the last sort's with headroom), so that the host reads the counts
+  // once, after them; the kernels stop at the device-side count, and a count
+  // above the hint is redone at its exact size
+  e->cell_order_set = false;
+  uint32_t hint = e->a_hint;
+  if (hint == 0) {  // no hint yet: the count first
+    rc = sync_counts(e);
+    if (rc) return rc;
+    hint = e->hcount[CNT_GROUPS];
+    if (e->hcount[CNT_DOMAIN]) {
+      // tables hold clamped entries; keep A so the next sort clears them
+      e->A = hint;
+      e->S = 0;
+      return fail(e, ZR_ERR_DOMAIN, "particle base node outside [0, 2^grid_log2 - 8]");
+    }
+  }
+  if (hint > n) hint = n;
+  if (hint == 0) hint = 1;
+  rc = sort_groups(e, hint);
+  if (rc) return rc;
   prof_mark(e, ZR_PROF_SORT);
   rc = sync_counts(e);
   if (rc) return rc;
-  e->A = e->hcount[CNT_GROUPS];
-  e->cell_order_set = false;
-  e->S = 0;
+  const uint32_t A = e->hcount[CNT_GROUPS];
+  e->A = A;
+  e->S = e->hcount[CNT_SLOTS];
   if (e->hcount[CNT_DOMAIN]) {
-    // tables hold clamped entries; keep A so the next sort clears them
+    // tables hold clamped entries; A and S are kept so the next sort clears them
     return fail(e, ZR_ERR_DOMAIN, "particle base node outside [0, 2^grid_log2 - 8]");
   }
-  const uint32_t A = e->A;
-  prof_mark(e, ZR_PROF_SORT);
-  rc = run_scan(e, LdArray{ e->group_count }, A, e->group_start, e->dcount + CNT_TOTAL);
-  if (rc) return rc;
-  k_scatter<<<blocks_for(n, 256), 256, 0, s>>>(n, e->p_page, e->p_re, e->page_group, e->group_start, e->keys, A,
-                                               e->dcount + CNT_TOTAL);
-  k_sort_groups<<<A, 256, 0, s>>>(e->keys, e->group_start, e->order);
-  k_touch<<<blocks_for(8ull * A, 256), 256, 0, s>>>(A, e->group_page, e->touch_page, e->page_touch);
-  ZR_LAUNCH_CHECK(e);
-  rc = run_scan(e, LdFirstTouch{ e->touch_page, e->page_touch }, 8u * A, e->touch_slot, e->dcount + CNT_SLOTS);
-  if (rc) return rc;
-  k_assign_slots<<<blocks_for(8ull * A, 256), 256, 0, s>>>(8u * A, e->touch_page, e->page_touch, e->touch_slot,
-                                                          e->slot_page, e->page_slot);
-  k_group_slots<<<blocks_for(8ull * A, 256), 256, 0, s>>>(8u * A, e->touch_page, e->page_slot, e->group_nslot);
-  ZR_LAUNCH_CHECK(e);
-  prof_mark(e, ZR_PROF_SORT);
-  rc = sync_counts(e);
-  if (rc) return rc;
-  e->S = e->hcount[CNT_SLOTS];
+  if (A > hint) {
+    rc = sort_groups(e, A);
+    if (rc) return rc;
+    rc = sync_counts(e);
+    if (rc) return rc;
+    e->S = e->hcount[CNT_SLOTS];
+  }
+  e->a_hint = A + A / 4 + 64 < n ? A + A / 4 + 64 : n;
   rc = ensure_grid(e);
   if (rc) return rc;
   if (e->S) {

@@ -141,12 +141,6 @@ struct LdPopc {
   const uint32_t* w;
   __device__ uint32_t operator()(uint32_t i) const { return (uint32_t)__popc(w[i]); }
 };
-// first touch of a page: the minimum touch index stored for the page is this one
-struct LdFirstTouch {
-  const uint32_t* touch_page;
-  const uint32_t* page_touch;
-  __device__ uint32_t operator()(uint32_t i) const { return page_touch[touch_page[i]] == i ? 1u : 0u; }
-};
 struct LdNonzeroMass {
   const double* m;
   __device__ uint32_t operator()(uint32_t i) const { return m[i] != 0.0 ? 1u : 0u; }
