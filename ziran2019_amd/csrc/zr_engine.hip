@@ -536,14 +536,26 @@ __device__ __forceinline__ void particle_kirchhoff_t(const ParticlesView& P, uin
 // FORCE = true: MLS with force (explicit).  FORCE = false: APIC mass and
 // momentum only, particlesToGridHelper<APIC> (MpmSimulationBase.cpp:563-608).
 // T: the precision of the per-particle terms (stress, weights, products);
-// the node sums are FP64 either way
-template <bool FORCE, int MODEL = kAnyModel, class T = double>
-__global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const uint32_t* __restrict__ order,
+// the node sums are FP64 either way.  FUSED (with FORCE, inside a whole
+// explicit step of size fdt): the force term goes into the momentum sum as
+// mv + fdt f, so that the grid update's v + dt (f / m + g) becomes
+// (mv + dt f) / m + dt g, with three fields fewer to add per node
+// waves per SIMD the register budget is held to: three for the FP64 fused
+// NeoHookeanBorden instance, which would otherwise miss that occupancy by
+// one register
+template <int MODEL, class T, bool FUSED>
+constexpr int p2g_min_waves()
+{
+  return MODEL == ZR_MODEL_NEO_HOOKEAN_BORDEN && sizeof(T) == sizeof(double) && FUSED ? 3 : 1;
+}
+template <bool FORCE, int MODEL = kAnyModel, class T = double, bool FUSED = false>
+__global__ __launch_bounds__(kTransferThreads, (p2g_min_waves<MODEL, T, FUSED>())) void k_p2g(ParticlesView P, const uint32_t* __restrict__ order,
                                              const uint32_t* __restrict__ group_start,
                                              const uint32_t* __restrict__ group_page, double* __restrict__ stage,
-                                             double dx, double inv_dx, double Dinv)
+                                             double dx, double inv_dx, double Dinv, double fdt)
 {
-  constexpr int NF = FORCE ? kP2GFields : 4;
+  static_assert(!FUSED || FORCE, "the fused form needs the force");
+  constexpr int NF = FORCE && !FUSED ? kP2GFields : 4;
   constexpr int SS = NF * kTileNodes;
   constexpr int NT = kTransferThreads, NW = NT / 64;
   __shared__ double tile[NW * SS];
@@ -589,42 +601,51 @@ __global__ __launch_bounds__(kTransferThreads) void k_p2g(ParticlesView P, const
       for (int r = 0; r < 3; ++r) mC[r][c] = mass * (T)P.C[(c * 3 + r) * S + i];
     // base node and the offsets x_i - x_p in FP64, the weights and the
     // products in T
-    T w[3][3], off[3][3];
+    T w[3][3];
     int b[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       double w64[3];
       b[d] = bspline_w(xp[d] * inv_dx, w64);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        w[d][a] = (T)w64[a];
-        off[d][a] = (T)((double)(b[d] + a) * dx - xp[d]);
-      }
+      for (int a = 0; a < 3; ++a) w[d][a] = (T)w64[a];
     }
     const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
     if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
     const T Dinv_t = (T)Dinv;
+    if constexpr (FUSED) {  // dt D^-1 V0 tau, so that a node takes (vd - st dx) w
+      const T sc = (T)(fdt * Dinv);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) st[r][c] *= sc;
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const T dx0 = off[0][a];
+      const T dx0 = (T)((double)(b[0] + a) * dx - xp[0]);
 #pragma unroll
       for (int bb = 0; bb < 3; ++bb) {
         const T wab = w[0][a] * w[1][bb];
-        const T dx1 = off[1][bb];
+        const T dx1 = (T)((double)(b[1] + bb) * dx - xp[1]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T wt_ = wab * w[2][c];
-          const T dx2 = off[2][c];
+          const T dx2 = (T)((double)(b[2] + c) * dx - xp[2]);
           const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
           const T fw = wt_ * Dinv_t;
           atomicAdd(&wt[t], (double)(mass * wt_));
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
             T vd = mC[r][0] * dx0 + mC[r][1] * dx1 + mC[r][2] * dx2 + mom[r];
-            atomicAdd(&wt[(1 + r) * kTileNodes + t], (double)(vd * wt_));
-            if constexpr (FORCE) {
+            if constexpr (FUSED) {
               T fd = st[r][0] * dx0 + st[r][1] * dx1 + st[r][2] * dx2;
-              atomicAdd(&wt[(4 + r) * kTileNodes + t], (double)(-(fd * fw)));
+              atomicAdd(&wt[(1 + r) * kTileNodes + t], (double)((vd - fd) * wt_));
+            } else {
+              atomicAdd(&wt[(1 + r) * kTileNodes + t], (double)(vd * wt_));
+              if constexpr (FORCE) {
+                T fd = st[r][0] * dx0 + st[r][1] * dx1 + st[r][2] * dx2;
+                atomicAdd(&wt[(4 + r) * kTileNodes + t], (double)(-(fd * fw)));
+              }
             }
           }
         }
@@ -1180,6 +1201,7 @@ struct zr_engine {
   // implicit backward Euler: node vectors [3][grid_cap], reduction partials,
   // device scalars/flags (+ pinned mirrors), per-particle SVD cache
   bool mls_rasterised = true;  // kind of the last P2G (MLS with force or APIC)
+  int grid_nf = 7;              // fields of its staged tiles (7: m, mv, f; 4: m, mv)
   double *ivn = nullptr, *idv = nullptr, *ir = nullptr, *ix = nullptr, *irc = nullptr, *iq = nullptr,
          *ip = nullptr, *iap = nullptr;
   size_t imp_cap = 0;
@@ -1198,6 +1220,8 @@ struct zr_engine {
   // launch size of the sort's group kernels: the last sort's group count
   // with headroom (0: read the count first)
   uint32_t a_hint = 0;
+  // the CFL flag counter is zero (the sort clears every counter; only G2P sets it)
+  bool cfl_clear = false;
   uint32_t* hvp_bounds = nullptr;  // its XCD group ranges, FP32 then FP64 (2 x 9)
   uint32_t* hvp_order = nullptr;   // their hand-out orders (2 x hvp_order_cap)
   uint8_t* cell_order = nullptr;   // each group's cells by particle count (32 x hvp_order_cap)
@@ -1483,6 +1507,7 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   uint32_t rst = e->A > e->S ? e->A : e->S;
   const uint32_t nrst = rst > (uint32_t)CNT_N ? rst : (uint32_t)CNT_N;
   e->sorted = e->rasterised = e->updated = false;
+  e->cfl_clear = true;
   prof_mark(e, ZR_PROF_SORT);
   k_reset_tables<<<blocks_for(nrst, 256), 256, 0, s>>>(e->group_page, e->A, e->slot_page, e->S, e->page_count,
                                                       e->page_group, e->page_bits, e->page_sum, e->page_touch,
@@ -1554,7 +1579,10 @@ int do_sort(zr_engine* e, zr_step_stats* st)
   return ZR_OK;
 }
 
-int do_p2g(zr_engine* e, bool mls)
+// mls: the MLS P2G with force (explicit) or the APIC one (implicit, ADMM);
+// fuse_dt > 0: a whole explicit step of that size follows, and the force
+// goes into the momentum sum (k_p2g's FUSED form)
+int do_p2g(zr_engine* e, bool mls, double fuse_dt = 0.0)
 {
   if (!e->sorted) return fail(e, ZR_ERR_STATE, "particlesToGrid before sortParticlesAndPolluteGrid");
   const double dx = e->cfg.dx;
@@ -1569,16 +1597,20 @@ int do_p2g(zr_engine* e, bool mls)
     const ParticlesView pv = pview(e);
     const double Dinv = 4.0 / (dx * dx);
     const int model = e->has_models ? e->uniform_model : ZR_MODEL_FIXED_COROTATED;
-    const bool fp32 = e->cfg.precision == ZR_FP32_REFINED;
-#define ZR_P2G_LAUNCH(M)                                                                                     \
-  do {                                                                                                       \
-    if (fp32)                                                                                                \
-      k_p2g<true, M, float><<<e->A, kTransferThreads, 0, e->stream>>>(pv, e->order, e->group_start,          \
-                                                                      e->group_page, e->stage, dx, 1.0 / dx, \
-                                                                      Dinv);                                 \
-    else                                                                                                     \
-      k_p2g<true, M><<<e->A, kTransferThreads, 0, e->stream>>>(pv, e->order, e->group_start, e->group_page,  \
-                                                               e->stage, dx, 1.0 / dx, Dinv);                \
+    const bool fp32 = e->cfg.precision == ZR_FP32_REFINED, fused = fuse_dt > 0.0;
+#define ZR_P2G_LAUNCH_T(M, T, F)                                                                        \
+  k_p2g<true, M, T, F><<<e->A, kTransferThreads, 0, e->stream>>>(pv, e->order, e->group_start, e->group_page, \
+                                                                 e->stage, dx, 1.0 / dx, Dinv, fuse_dt)
+#define ZR_P2G_LAUNCH(M)                       \
+  do {                                         \
+    if (fp32 && fused)                         \
+      ZR_P2G_LAUNCH_T(M, float, true);         \
+    else if (fp32)                             \
+      ZR_P2G_LAUNCH_T(M, float, false);        \
+    else if (fused)                            \
+      ZR_P2G_LAUNCH_T(M, double, true);        \
+    else                                       \
+      ZR_P2G_LAUNCH_T(M, double, false);       \
   } while (0)
     if (model == ZR_MODEL_NEO_HOOKEAN_BORDEN)
       ZR_P2G_LAUNCH(ZR_MODEL_NEO_HOOKEAN_BORDEN);
@@ -1589,10 +1621,15 @@ int do_p2g(zr_engine* e, bool mls)
     else
       ZR_P2G_LAUNCH(kAnyModel);
 #undef ZR_P2G_LAUNCH
+#undef ZR_P2G_LAUNCH_T
+    e->grid_nf = fused ? 4 : kP2GFields;
   }
   else
+  {
     k_p2g<false><<<e->A, kTransferThreads, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, e->stage, dx,
-                                              1.0 / dx, 4.0 / (dx * dx));
+                                              1.0 / dx, 4.0 / (dx * dx), 0.0);
+    e->grid_nf = 4;
+  }
   ZR_LAUNCH_CHECK(e);
   prof_mark(e, ZR_PROF_P2G);
   e->rasterised = true;
@@ -1608,7 +1645,7 @@ int do_grid(zr_engine* e, int mode, double dt)
   prof_mark(e, ZR_PROF_GRID);
   if (e->S) {
     const uint32_t nb = blocks_for((uint64_t)e->S * 32, 256);
-    if (e->mls_rasterised)
+    if (e->grid_nf == kP2GFields)
       k_grid<kP2GFields><<<nb, 256, 0, e->stream>>>(mode, e->S, e->slot_page, e->slot_src, e->stage, G, e->cfg.dx,
                                                     dt, e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2], cs);
     else
@@ -1625,7 +1662,8 @@ int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
 {
   if (!e->updated) return fail(e, ZR_ERR_STATE, "gridToParticles before the grid update");
   const double dx = e->cfg.dx;
-  ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_CFL, 0, sizeof(uint32_t), e->stream));
+  if (!e->cfl_clear) ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_CFL, 0, sizeof(uint32_t), e->stream));
+  e->cfl_clear = false;
   ParticlesView pv = pview(e);
   pv.gradv = e->g2p_gradv;
   prof_mark(e, ZR_PROF_G2P);
@@ -3349,7 +3387,7 @@ int zr_advance_one_time_step(zr_engine* e, double dt, zr_step_stats* st)
   }
   int rc = do_sort(e, st);
   if (rc) return rc;
-  rc = do_p2g(e, true);
+  rc = do_p2g(e, true, dt);
   if (rc) return rc;
   if (e->pf_on) {  // FractureSimulation::particlesToGrid: the damage solve after P2G
     rc = do_grid(e, GRID_COMBINE, dt);
