@@ -105,32 +105,49 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(Ld ld, uint32_t n, con
 }
 
 // inputs of at most kScanSingleMax elements: one block of 1024 threads scans
-// them in one launch (each thread a run of <= 16 consecutive elements) and
-// writes the total
+// them in one launch.  The inputs are read striped (every thread's loads
+// independent and coalesced) into LDS, each thread then scans a run of
+// <= 16 consecutive entries there, and the results go out striped; the
+// total is written too
 constexpr uint32_t kScanSingleMax = 1024u * 16u;
 template <class Ld>
 __global__ __launch_bounds__(1024) void k_scan_single(Ld ld, uint32_t n, uint32_t* out, uint32_t* total_out)
 {
+  __shared__ uint32_t buf[kScanSingleMax];
   __shared__ uint32_t smem[1024 / 64];
-  const uint32_t chunk = (n + 1023u) / 1024u;
-  const uint32_t b = threadIdx.x * chunk;
-  uint32_t vals[16];
-  uint32_t s = 0;
+  const uint32_t tid = threadIdx.x;
+  uint32_t v[16];
 #pragma unroll
   for (uint32_t k = 0; k < 16u; ++k) {
-    const uint32_t i = b + k;
-    vals[k] = (k < chunk && i < n) ? ld(i) : 0u;
-    s += vals[k];
+    const uint32_t i = k * 1024u + tid;
+    v[k] = i < n ? ld(i) : 0u;
   }
+#pragma unroll
+  for (uint32_t k = 0; k < 16u; ++k) {
+    const uint32_t i = k * 1024u + tid;
+    if (i < n) buf[i] = v[k];
+  }
+  __syncthreads();
+  const uint32_t chunk = (n + 1023u) / 1024u;
+  const uint32_t b = tid * chunk;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < chunk; ++k)
+    if (b + k < n) s += buf[b + k];
   uint32_t total;
   uint32_t off = block_exclusive_scan<1024>(s, total, smem);
+  for (uint32_t k = 0; k < chunk; ++k)
+    if (b + k < n) {
+      const uint32_t x = buf[b + k];
+      buf[b + k] = off;
+      off += x;
+    }
+  __syncthreads();
 #pragma unroll
   for (uint32_t k = 0; k < 16u; ++k) {
-    const uint32_t i = b + k;
-    if (k < chunk && i < n) out[i] = off;
-    off += vals[k];
+    const uint32_t i = k * 1024u + tid;
+    if (i < n) out[i] = buf[i];
   }
-  if (threadIdx.x == 0 && total_out) *total_out = total;
+  if (tid == 0 && total_out) *total_out = total;
 }
 
 struct LdArray {
