@@ -154,10 +154,6 @@ struct LdArray {
   const uint32_t* a;
   __device__ uint32_t operator()(uint32_t i) const { return a[i]; }
 };
-struct LdPopc {
-  const uint32_t* w;
-  __device__ uint32_t operator()(uint32_t i) const { return (uint32_t)__popc(w[i]); }
-};
 struct LdNonzeroMass {
   const double* m;
   __device__ uint32_t operator()(uint32_t i) const { return m[i] != 0.0 ? 1u : 0u; }
