@@ -1,6 +1,7 @@
 // zr_scan.hpp -- device-wide exclusive prefix sums of u32 streams (three
-// launches: tile reduce, scan of tile sums, tile scan).  The input is read
-// through a functor so flags can be derived on the fly (bitmap popcounts,
+// launches: tile reduce, scan of tile sums, tile scan; one launch for inputs
+// of at most 16,384 entries).  The input is read through a functor so that
+// values can be derived on the fly (group counts up to a device-side bound,
 // first-touch tests, non-zero masses) without materialising them.
 #pragma once
 #include <hip/hip_runtime.h>
