@@ -1140,7 +1140,10 @@ __global__ __launch_bounds__(HvpChunk<T>::threads, HvpChunk<T>::waves) void k_hv
                                  nn, nsl, nord, P1, tq, tp);
         issued = true;
       }
-      for (int k = tid; k < kTasks; k += NT) {
+      // tasks past the first NT (the smallest cells) go to the last threads,
+      // whose first tasks hold the smaller cells of the first round
+      constexpr int kExtra = kTasks - NT;
+      for (int k = tid; k < kTasks; k = (k < NT && tid >= NT - kExtra) ? tid + kExtra : kTasks) {
         const int rk = k / kAB, ab = k - rk * kAB;
         const int e = cell_rank[rk], task = e * kAB + ab;
         const int a = ab / 3, bb = ab - 3 * a;
