@@ -1217,6 +1217,10 @@ struct zr_engine {
   uint32_t* act_scan = nullptr;
   size_t act_cap = 0;
   uint32_t nact = 0, ncg_act = 1;
+  // F, then V0 P Fn^T, per particle in sorted order between the Newton
+  // state's passes (9 rows of n)
+  double* be_fb = nullptr;
+  size_t be_fb_cap = 0;
   // launch size of the sort's group kernels: the last sort's group count
   // with headroom (0: read the count first)
   uint32_t a_hint = 0;
@@ -2205,10 +2209,23 @@ int be_state_residual(zr_engine* e, bool jac = false)
     k_be_state<T, true><<<e->A, 256, 0, e->stream>>>(pview(e), hvp_cache<T>(e), e->order, e->group_start,
                                                      e->group_page, e->page_slot, e->ivn, e->idv,
                                                      (int64_t)e->grid_cap, dx, 1.0 / dx, dt, e->stage, co, e->jstage);
-  else
-    k_be_state<T, false><<<e->A, 256, 0, e->stream>>>(pview(e), hvp_cache<T>(e), e->order, e->group_start,
-                                                      e->group_page, e->page_slot, e->ivn, e->idv,
-                                                      (int64_t)e->grid_cap, dx, 1.0 / dx, dt, e->stage, co, nullptr);
+  else {
+    // the same state in three passes (k_be_grad, k_be_svd, k_be_scatter):
+    // the SVD at a higher occupancy than the fused kernel's
+    const size_t N = (size_t)e->n;
+    if (e->be_fb_cap < N) {
+      dfree(e->be_fb);
+      e->be_fb_cap = 0;
+      ZR_HIP(e, hipMalloc(&e->be_fb, 9 * N * sizeof(double)));
+      e->be_fb_cap = N;
+    }
+    const HvpCache<T> hc = hvp_cache<T>(e);
+    k_be_grad<T><<<e->A, 256, 0, e->stream>>>(pview(e), hc, e->order, e->group_start, e->group_page, e->page_slot,
+                                              e->ivn, e->idv, (int64_t)e->grid_cap, 1.0 / dx, dt, e->be_fb, co);
+    k_be_svd<T><<<blocks_for(N, 256), 256, 0, e->stream>>>(pview(e), hc, e->order, (uint32_t)N, e->be_fb);
+    k_be_scatter<<<e->A, 256, 0, e->stream>>>(pview(e), e->order, e->group_start, e->group_page, hc.stride,
+                                              1.0 / dx, e->be_fb, e->stage);
+  }
   ZR_LAUNCH_CHECK(e);
   e->cell_order_set = true;  // the cells do not move within a step
   prof_mark(e, ZR_PROF_STATE);
@@ -2611,7 +2628,7 @@ void zr_destroy(zr_engine* e)
   dfree(e->scan_tmp); dfree(e->speed); dfree(e->dcount); dfree(e->ticket); dfree(e->slot_src);
   dfree(e->ivn); dfree(e->idv); dfree(e->ir); dfree(e->ix); dfree(e->irc); dfree(e->iq); dfree(e->ip); dfree(e->iap);
   dfree(e->partial); dfree(e->partial2); dfree(e->sc); dfree(e->ci); dfree(e->hvp_ctr); dfree(e->hvp_bounds); dfree(e->hvp_order); dfree(e->cell_order); dfree(e->jstage); dfree(e->pinv); dfree(e->collided); dfree(e->c64); dfree(e->c32); dfree(e->ccell);
-  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll); dfree(e->act); dfree(e->act_scan); dfree(e->admm_c32); dfree(e->admm_cell32); dfree(e->admm_node32);
+  dfree(e->pf_node); dfree(e->pf_partial); dfree(e->adj_grid); dfree(e->admm_node); dfree(e->admm_coll); dfree(e->act); dfree(e->act_scan); dfree(e->be_fb); dfree(e->admm_c32); dfree(e->admm_cell32); dfree(e->admm_node32);
   dfree(e->fx); dfree(e->frc); dfree(e->fq); dfree(e->fp); dfree(e->fap);
   dfree(e->mr64); dfree(e->mr32);
   if (e->hsc) (void)hipHostFree(e->hsc);

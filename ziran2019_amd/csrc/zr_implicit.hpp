@@ -627,6 +627,220 @@ __global__ __launch_bounds__(256) void k_be_state(ParticlesView P, HvpCache<T> c
   }
 }
 
+// The Newton state of k_be_state<T, false> in three passes, so that the
+// SVD runs in a kernel of its own at a higher occupancy than the fused
+// kernel's one wave per SIMD:
+//   k_be_grad    per group: grad(vn + dv) from the LDS tile, F = Fn + dt G Fn
+//                into fb (9 rows of n, sorted order), the fractions, cells
+//                and cell order of the operator data
+//   k_be_svd     per particle (sorted order): the SVD of F, the operator's
+//                U, Q, S, K, and vPFnT = V0 P(F) Fn^T back into fb
+//   k_be_scatter per group: vPFnT grad w into the staged tiles
+// The arithmetic and its order are those of k_be_state.
+template <class T>
+__global__ __launch_bounds__(256) void k_be_grad(ParticlesView P, HvpCache<T> cache,
+                                                 const uint32_t* __restrict__ order,
+                                                 const uint32_t* __restrict__ group_start,
+                                                 const uint32_t* __restrict__ group_page,
+                                                 const int32_t* __restrict__ page_slot,
+                                                 const double* __restrict__ vn, const double* __restrict__ dv,
+                                                 int64_t gstride, double inv_dx, double dt, double* __restrict__ fb,
+                                                 uint8_t* __restrict__ cell_order)
+{
+  __shared__ double tv[3 * kTileNodes];
+  __shared__ int32_t nslot[8];
+  __shared__ int ccount[kCells];
+  const int tid = threadIdx.x;
+  const uint32_t g = blockIdx.x;
+  if (tid < kCells) ccount[tid] = 0;
+  const uint32_t start = group_start[g];
+  const uint32_t n = group_start[g + 1] - start;
+  uint32_t px, py, pz;
+  page_decode(group_page[g], px, py, pz);
+  load_tile3(px, py, pz, page_slot, vn, dv, gstride, tv, nslot);
+  const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
+  const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
+  const int64_t S = P.stride, CS = cache.stride;
+  for (uint32_t j0 = 0; j0 < n; j0 += 256) {
+    const uint32_t j = j0 + perm;
+    if (j >= n) continue;
+    const uint32_t i = order[start + j];
+    const uint32_t ks = start + j;
+    const double xp[3] = { P.x[i], P.x[S + i], P.x[2 * S + i] };
+    double w[3][3], dw[3][3], d0[3];
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double xi = xp[d] * inv_dx;
+      b[d] = int_floor(xi - 0.5);
+      d0[d] = xi - (double)b[d];
+      bspline_frac(d0[d], w[d], dw[d]);
+    }
+    const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
+    if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
+    if (cell_order && j0 == 0) atomicAdd(&ccount[lx | (ly << 1) | (lz << 3)], 1);
+    const int t0 = (lx * kTileY + ly) * kTileZ + lz;
+    double G[3][3] = { { 0.0 } };
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          wgradT(w, dw, inv_dx, a, bb, c, gw);
+          const int t = t0 + (a * kTileY + bb) * kTileZ + c;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const double vv = tv[r * kTileNodes + t];
+            G[r][0] += vv * gw[0];
+            G[r][1] += vv * gw[1];
+            G[r][2] += vv * gw[2];
+          }
+        }
+    double Fn[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) Fn[r][c] = P.F[(c * 3 + r) * S + i];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double acc = Fn[r][c];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc += dt * G[r][k] * Fn[k][c];
+        fb[(c * 3 + r) * CS + ks] = acc;
+      }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cache.frac[d * CS + ks] = (T)d0[d];
+    cache.cell[ks] = (uint32_t)(lx | (ly << 1) | (lz << 3));
+  }
+  __syncthreads();
+  if (cell_order && tid < kCells) {
+    const int cnt = ccount[tid];
+    int rank = 0;
+#pragma unroll
+    for (int e2 = 0; e2 < kCells; ++e2) {
+      const int c2 = ccount[e2];
+      rank += (c2 > cnt || (c2 == cnt && e2 < tid)) ? 1 : 0;
+    }
+    cell_order[(size_t)g * kCells + rank] = (uint8_t)tid;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_be_svd(ParticlesView P, HvpCache<T> cache,
+                                                const uint32_t* __restrict__ order, uint32_t n,
+                                                double* __restrict__ fb)
+{
+  const uint32_t ks = blockIdx.x * 256u + threadIdx.x;
+  if (ks >= n) return;
+  const uint32_t i = order[ks];
+  const int64_t S = P.stride, CS = cache.stride;
+  M3<double> F, u, v;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) F.a[r][c] = fb[(c * 3 + r) * CS + ks];
+  double s[3];
+  svd3(F, u, s, v);
+  double Fn[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) Fn[r][c] = P.F[(c * 3 + r) * S + i];
+  const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      cache.U[(c * 3 + r) * CS + ks] = (T)u.a[r][c];
+      cache.Q[(c * 3 + r) * CS + ks] = (T)(Fn[r][0] * v.a[0][c] + Fn[r][1] * v.a[1][c] + Fn[r][2] * v.a[2][c]);
+    }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) cache.S[d * CS + ks] = (T)(d == 2 && sizeof(T) == sizeof(float) ? s[1] + s[2] : s[d]);
+  cache.K[ks] = (T)(2.0 * mu * vol);
+  cache.K[CS + ks] = (T)(lam * vol);
+  // P = U diag(2 mu (s - 1) + lam (J - 1) cof(s)) V^T
+  const double J = s[0] * s[1] * s[2];
+  const double cof[3] = { s[1] * s[2], s[0] * s[2], s[0] * s[1] };
+  double ph[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) ph[d] = 2.0 * mu * (s[d] - 1.0) + lam * (J - 1.0) * cof[d];
+  double Pm[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      Pm[r][c] = u.a[r][0] * ph[0] * v.a[c][0] + u.a[r][1] * ph[1] * v.a[c][1] + u.a[r][2] * ph[2] * v.a[c][2];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      fb[(c * 3 + r) * CS + ks] = vol * (Pm[r][0] * Fn[c][0] + Pm[r][1] * Fn[c][1] + Pm[r][2] * Fn[c][2]);
+}
+
+__global__ __launch_bounds__(256) void k_be_scatter(ParticlesView P, const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ group_start,
+                                                    const uint32_t* __restrict__ group_page, int64_t cstride,
+                                                    double inv_dx, const double* __restrict__ fb,
+                                                    double* __restrict__ stage)
+{
+  __shared__ double tile[4 * kImpStage];
+  const int tid = threadIdx.x;
+  const uint32_t g = blockIdx.x;
+  for (int k = tid; k < 4 * kImpStage; k += 256) tile[k] = 0.0;
+  const uint32_t start = group_start[g];
+  const uint32_t n = group_start[g + 1] - start;
+  uint32_t px, py, pz;
+  page_decode(group_page[g], px, py, pz);
+  const int ox = 2 * (int)px, oy = 4 * (int)py, oz = 4 * (int)pz;
+  __syncthreads();
+  double* wt = tile + (tid >> 6) * kImpStage;
+  const uint32_t perm = (uint32_t)((tid & 31) * 8 + (tid >> 5));
+  const int64_t S = P.stride;
+  for (uint32_t j0 = 0; j0 < n; j0 += 256) {
+    const uint32_t j = j0 + perm;
+    if (j >= n) continue;
+    const uint32_t i = order[start + j];
+    const uint32_t ks = start + j;
+    double w[3][3], dw[3][3];
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double xi = P.x[d * S + i] * inv_dx;
+      b[d] = int_floor(xi - 0.5);
+      bspline_frac(xi - (double)b[d], w[d], dw[d]);
+    }
+    const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
+    if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
+    const int t0 = (lx * kTileY + ly) * kTileZ + lz;
+    double vp[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) vp[r][c] = fb[(c * 3 + r) * cstride + ks];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double gw[3];
+          wgradT(w, dw, inv_dx, a, bb, c, gw);
+          const int t = t0 + (a * kTileY + bb) * kTileZ + c;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+            atomicAdd(&wt[r * kTileNodes + t], vp[r][0] * gw[0] + vp[r][1] * gw[1] + vp[r][2] * gw[2]);
+        }
+  }
+  __syncthreads();
+  double* out = stage + (size_t)g * kImpStage;
+  for (int k = tid; k < kImpStage; k += 256)
+    out[k] = tile[k] + tile[kImpStage + k] + tile[2 * kImpStage + k] + tile[3 * kImpStage + k];
+}
+
 constexpr int kAB = 9;        // (a, b) stencil offsets; c is handled inside a task
 
 // Node-major layout of the per-(cell, a, b, c) partial sums of the operator
