@@ -671,23 +671,100 @@ __global__ __launch_bounds__(256) void k_admm_diag(uint32_t nn, const int32_t* _
   diag[k] = d;
 }
 
+// the local step of particle i (FStepSolve, AdmmSimulationFoam.h:507-615)
+// given G = grad(vn + dv) at the particle
+template <bool VISCO>
+__device__ __forceinline__ void admm_local_step(const ParticlesView& P, const AdmmParticles& A,
+                                                const int32_t* __restrict__ mat, const double* __restrict__ pparam,
+                                                const double* __restrict__ pstate, uint32_t i, const double G[3][3],
+                                                double dt, double rho, double local_tol)
+{
+  const int64_t S = P.stride;
+  double Fn[3][3];
+  load_F(P.F, S, i, Fn);
+  const double om = A.om[i];
+  M3<double> fd, u, v;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      fd.a[r][c] = Fn[r][c] - A.y[(c * 3 + r) * S + i] / (om * rho) +
+                   dt * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
+  double sg[3];
+  svd3(fd, u, sg, v);
+  double z[3];
+  {
+    double Fa[3][3];
+    load_F(A.F, S, i, Fa);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double acc = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) acc += u.a[a][d] * Fa[a][bb] * v.a[bb][d];
+      z[d] = acc;
+    }
+  }
+  const PlasticParams q = plastic_params(P, mat, pparam, pstate, i);
+  const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i], coeff = rho * om * om;
+  for (int it = 0; it < 5; ++it) {
+    double zp[3], D[3][3], dE[3], ddE[3][3], gg[3], H[3][3];
+    admm_project_sigma_d(q, mu, lam, z, zp, D);
+    hencky_dE(mu, lam, zp, dE);
+    hencky_ddE(mu, lam, zp, ddE);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      gg[a] = vol * dE[a] + coeff * (z[a] - sg[a]);
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb)
+        H[a][bb] = vol * (ddE[a][0] * D[0][bb] + ddE[a][1] * D[1][bb] + ddE[a][2] * D[2][bb]) +
+                   (a == bb ? coeff : 0.0);
+    }
+    const double det = H[0][0] * (H[1][1] * H[2][2] - H[2][1] * H[1][2]) -
+                       H[0][1] * (H[1][0] * H[2][2] - H[2][0] * H[1][2]) +
+                       H[0][2] * (H[1][0] * H[2][1] - H[2][0] * H[1][1]);
+    double Hi[3][3];
+    Hi[0][0] = (H[1][1] * H[2][2] - H[1][2] * H[2][1]) / det;
+    Hi[0][1] = (H[0][2] * H[2][1] - H[0][1] * H[2][2]) / det;
+    Hi[0][2] = (H[0][1] * H[1][2] - H[0][2] * H[1][1]) / det;
+    Hi[1][0] = (H[1][2] * H[2][0] - H[1][0] * H[2][2]) / det;
+    Hi[1][1] = (H[0][0] * H[2][2] - H[0][2] * H[2][0]) / det;
+    Hi[1][2] = (H[0][2] * H[1][0] - H[0][0] * H[1][2]) / det;
+    Hi[2][0] = (H[1][0] * H[2][1] - H[1][1] * H[2][0]) / det;
+    Hi[2][1] = (H[0][1] * H[2][0] - H[0][0] * H[2][1]) / det;
+    Hi[2][2] = (H[0][0] * H[1][1] - H[0][1] * H[1][0]) / det;
+    double sn = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double st = -(Hi[a][0] * gg[0] + Hi[a][1] * gg[1] + Hi[a][2] * gg[2]);
+      z[a] += st;
+      sn += st * st;
+    }
+    if (sqrt(sn) < local_tol) break;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      A.F[(c * 3 + r) * S + i] = u.a[r][0] * z[0] * v.a[c][0] + u.a[r][1] * z[1] * v.a[c][1] + u.a[r][2] * z[2] * v.a[c][2];
+  if constexpr (VISCO) admm_visco_fstep(A, S, i, vol, G, dt, rho, local_tol);
+}
+
 // FStepSolve (:507-615): F = U diag(z) V^T with z the Newton solution of
 // vol dE(proj(z)) + rho omega^2 (z - sigma) = 0, sigma the singular values
 // of Fn - y/(rho omega) + dt grad(vn + dv) Fn; VISCO: then the viscoelastic
-// branch's F2
-// (two waves per SIMD: the SVD and the local Newton spill a few registers
-// to scratch, which costs less than the latency one wave leaves exposed)
-template <bool VISCO>
-__global__ __launch_bounds__(256, 2) void k_admm_fstep(ParticlesView P, AdmmParticles A, AdmmNodes N,
-                                                    const int32_t* __restrict__ mat,
-                                                    const double* __restrict__ pparam,
-                                                    const double* __restrict__ pstate,
-                                                    const uint32_t* __restrict__ order,
+// branch's F2.  In two passes, so that the SVD and the Newton iterations run
+// in a kernel without the tile (134 VGPRs, 3 waves/SIMD, against one kernel
+// of 256 VGPRs with spills): k_admm_fgrad writes grad(vn + dv) of every
+// particle (9 rows of n, particle order), k_admm_flocal runs the local step
+// per particle from it
+__global__ __launch_bounds__(256) void k_admm_fgrad(ParticlesView P, AdmmNodes N, const uint32_t* __restrict__ order,
                                                     const uint32_t* __restrict__ group_start,
                                                     const uint32_t* __restrict__ group_page,
                                                     const int32_t* __restrict__ group_nslot,
-                                                    const double* __restrict__ gm, double dt, double rho,
-                                                    double local_tol, double inv_dx)
+                                                    const double* __restrict__ gm, double inv_dx,
+                                                    double* __restrict__ gs)
 {
   __shared__ double tv[3 * kTileNodes];
   __shared__ int32_t nslot[8];
@@ -708,77 +785,32 @@ __global__ __launch_bounds__(256, 2) void k_admm_fstep(ParticlesView P, AdmmPart
     for (int d = 0; d < 3; ++d) b[d] = bspline_wdw(xp[d] * inv_dx, w[d], dw[d]);
     const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
     if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
-    double G[3][3], Fn[3][3];
+    double G[3][3];
     admm_grad_tile(tv, w, dw, inv_dx, lx, ly, lz, G);
-    load_F(P.F, S, i, Fn);
-    const double om = A.om[i];
-    M3<double> fd, u, v;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        fd.a[r][c] = Fn[r][c] - A.y[(c * 3 + r) * S + i] / (om * rho) +
-                     dt * (G[r][0] * Fn[0][c] + G[r][1] * Fn[1][c] + G[r][2] * Fn[2][c]);
-    double sg[3];
-    svd3(fd, u, sg, v);
-    double z[3];
-    {
-      double Fa[3][3];
-      load_F(A.F, S, i, Fa);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double acc = 0.0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-          for (int bb = 0; bb < 3; ++bb) acc += u.a[a][d] * Fa[a][bb] * v.a[bb][d];
-        z[d] = acc;
-      }
-    }
-    const PlasticParams q = plastic_params(P, mat, pparam, pstate, i);
-    const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i], coeff = rho * om * om;
-    for (int it = 0; it < 5; ++it) {
-      double zp[3], D[3][3], dE[3], ddE[3][3], gg[3], H[3][3];
-      admm_project_sigma_d(q, mu, lam, z, zp, D);
-      hencky_dE(mu, lam, zp, dE);
-      hencky_ddE(mu, lam, zp, ddE);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        gg[a] = vol * dE[a] + coeff * (z[a] - sg[a]);
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-          H[a][bb] = vol * (ddE[a][0] * D[0][bb] + ddE[a][1] * D[1][bb] + ddE[a][2] * D[2][bb]) +
-                     (a == bb ? coeff : 0.0);
-      }
-      const double det = H[0][0] * (H[1][1] * H[2][2] - H[2][1] * H[1][2]) -
-                         H[0][1] * (H[1][0] * H[2][2] - H[2][0] * H[1][2]) +
-                         H[0][2] * (H[1][0] * H[2][1] - H[2][0] * H[1][1]);
-      double Hi[3][3];
-      Hi[0][0] = (H[1][1] * H[2][2] - H[1][2] * H[2][1]) / det;
-      Hi[0][1] = (H[0][2] * H[2][1] - H[0][1] * H[2][2]) / det;
-      Hi[0][2] = (H[0][1] * H[1][2] - H[0][2] * H[1][1]) / det;
-      Hi[1][0] = (H[1][2] * H[2][0] - H[1][0] * H[2][2]) / det;
-      Hi[1][1] = (H[0][0] * H[2][2] - H[0][2] * H[2][0]) / det;
-      Hi[1][2] = (H[0][2] * H[1][0] - H[0][0] * H[1][2]) / det;
-      Hi[2][0] = (H[1][0] * H[2][1] - H[1][1] * H[2][0]) / det;
-      Hi[2][1] = (H[0][1] * H[2][0] - H[0][0] * H[2][1]) / det;
-      Hi[2][2] = (H[0][0] * H[1][1] - H[0][1] * H[1][0]) / det;
-      double sn = 0.0;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double st = -(Hi[a][0] * gg[0] + Hi[a][1] * gg[1] + Hi[a][2] * gg[2]);
-        z[a] += st;
-        sn += st * st;
-      }
-      if (sqrt(sn) < local_tol) break;
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        A.F[(c * 3 + r) * S + i] = u.a[r][0] * z[0] * v.a[c][0] + u.a[r][1] * z[1] * v.a[c][1] + u.a[r][2] * z[2] * v.a[c][2];
-    if constexpr (VISCO) admm_visco_fstep(A, S, i, vol, G, dt, rho, local_tol);
+      for (int r = 0; r < 3; ++r) gs[(c * 3 + r) * S + i] = G[r][c];
   }
+}
+
+template <bool VISCO>
+__global__ __launch_bounds__(256) void k_admm_flocal(ParticlesView P, AdmmParticles A,
+                                                     const int32_t* __restrict__ mat,
+                                                     const double* __restrict__ pparam,
+                                                     const double* __restrict__ pstate, uint32_t n,
+                                                     const double* __restrict__ gs, double dt, double rho,
+                                                     double local_tol)
+{
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const int64_t S = P.stride;
+  double G[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) G[r][c] = gs[(c * 3 + r) * S + i];
+  admm_local_step<VISCO>(P, A, mat, pparam, pstate, i, G, dt, rho, local_tol);
 }
 
 // dvStep rhs scatter (:167-205): dt rho omega^2 (F - Fn + y/(rho omega) - dt grad(vn) Fn) Fn^T grad w

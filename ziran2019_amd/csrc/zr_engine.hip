@@ -1217,8 +1217,9 @@ struct zr_engine {
   uint32_t* act_scan = nullptr;
   size_t act_cap = 0;
   uint32_t nact = 0, ncg_act = 1;
-  // F, then V0 P Fn^T, per particle in sorted order between the Newton
-  // state's passes (9 rows of n)
+  // per-particle 3x3 scratch (9 rows of n) between the passes of the Newton
+  // state (F, then V0 P Fn^T, sorted order) and of the ADMM local step
+  // (grad(vn + dv), particle order)
   double* be_fb = nullptr;
   size_t be_fb_cap = 0;
   // launch size of the sort's group kernels: the last sort's group count
@@ -2052,19 +2053,27 @@ int do_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* st)
                                                                         rho, dt);
     ZR_LAUNCH_CHECK(e);
   }
+  // the local step's per-particle grad(vn + dv) (9 rows of n)
+  if (e->be_fb_cap < N) {
+    dfree(e->be_fb);
+    e->be_fb_cap = 0;
+    ZR_HIP(e, hipMalloc(&e->be_fb, 9 * N * sizeof(double)));
+    e->be_fb_cap = N;
+  }
+  const uint32_t npb = blocks_for((uint64_t)n, 256);
   int total_minres = 0;
   for (int tim = 0; tim < o->admm_iterations; ++tim) {
     k_admm_collide<<<nbn, 256, 0, s>>>(nn, e->slot_page, e->gm, Nd, dx, cs);
+    k_admm_fgrad<<<e->A, 256, 0, s>>>(P, Nd, e->order, e->group_start, e->group_page, e->group_nslot, e->gm, inv_dx,
+                                      e->be_fb);
     if (visco) {
-      k_admm_fstep<true><<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
-                                              e->group_page, e->group_nslot, e->gm, dt, rho, o->local_tolerance,
-                                              inv_dx);
+      k_admm_flocal<true><<<npb, 256, 0, s>>>(P, Ap, mat, e->pparam, e->pstate, (uint32_t)n, e->be_fb, dt, rho,
+                                              o->local_tolerance);
       k_admm_scatter<0, true><<<e->A, 256, 0, s>>>(P, Ap, Nd, nullptr, e->order, e->group_start, e->group_page,
                                                    e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
     } else {
-      k_admm_fstep<false><<<e->A, 256, 0, s>>>(P, Ap, Nd, mat, e->pparam, e->pstate, e->order, e->group_start,
-                                               e->group_page, e->group_nslot, e->gm, dt, rho, o->local_tolerance,
-                                               inv_dx);
+      k_admm_flocal<false><<<npb, 256, 0, s>>>(P, Ap, mat, e->pparam, e->pstate, (uint32_t)n, e->be_fb, dt, rho,
+                                               o->local_tolerance);
       k_admm_scatter<0, false><<<e->A, 256, 0, s>>>(P, Ap, Nd, nullptr, e->order, e->group_start, e->group_page,
                                                     e->group_nslot, e->gm, e->stage, dt, rho, inv_dx, e->ci);
     }
