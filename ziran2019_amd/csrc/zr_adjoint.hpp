@@ -289,6 +289,7 @@ __global__ __launch_bounds__(256) void k_adj_g2p(ParticlesView P, AdjointView A,
       xb[d] = A.x[d * S + i];
       vb[d] = A.v[d * S + i] + dt * xb[d];
     }
+    opaque_weights(w, dw);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const double d0 = (double)(b[0] + a) * dx - xp[0];
