@@ -417,12 +417,17 @@ int zr_checkpoint_restore(zr_engine* e, int32_t slot);
    otherwise pay a host round trip per step and per checkpoint.
    zr_rollout_forward: steps explicit steps of size dt from the current
    state, saving checkpoint slot0 + k / every before step k when k is a
-   multiple of every; stops at the first error or ZR_CFL_RESTART.
+   multiple of every; stops at the first error or ZR_CFL_RESTART.  A step's
+   CFL flags are read with the next step's sort (one host wait per step in
+   place of two), so ZR_CFL_RESTART leaves the state after the step that
+   asked for it, as a call per step would, and a checkpoint due before the
+   next step may already hold that state.
    zr_rollout_backward: after zr_adjoint_set on the rollout's output, per
    segment from the last: restore its checkpoint, save the inputs of its
    steps in slots scratch0 .. scratch0 + every - 1 (recomputing them), then
    run zr_adjoint_step for each, last first; the adjoint then holds that of
-   the rollout's input. */
+   the rollout's input.  The adjoint steps' collider flags are read once, at
+   the end. */
 int zr_rollout_forward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0);
 int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0, int32_t scratch0);
 

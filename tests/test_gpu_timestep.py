@@ -11,7 +11,8 @@ import pytest
 
 import oracle
 from ziran2019_amd import build as zbuild
-from ziran2019_amd.engine import MpmEngine, TimeStepping, ZrError
+from ziran2019_amd import scenes
+from ziran2019_amd.engine import MpmEngine, RestartException, TimeStepping, ZrError
 from _helpers import perturbed, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -135,3 +136,48 @@ def test_host_frames_match_oracle(tmp_path):
     X, V, C, F = ora.get_particles()
     assert rel_err(out["X"], X) < 1e-10
     assert rel_err(out["V"], V) < 1e-9
+
+
+def test_rollout_stops_after_the_restart_step():
+    """zr_rollout_forward reads each step's CFL flags with the next step's
+    sort: a rollout stops after the step that asked for a restart, with the
+    state that stepping one call at a time leaves (bit for bit), and a
+    rollout that ends before that step runs to its end."""
+    s = scenes.cube_scene_small(seed=5, ncells=4, v0=(0.0, 0.0, 0.0), spin=0.0)
+    dx, dt, g = s["dx"], 1e-3, 9.8
+    # a uniform fall whose step 5 (the speed dx/dt + 0.5 g dt) moves more than a cell
+    s["V"][:, 1] = -(dx / dt - 4.5 * g * dt)
+
+    def engine():
+        e = MpmEngine(dx=dx, gravity=(0.0, -g, 0.0))
+        e.upload(s)
+        e.set_colliders([])
+        return e
+
+    a = engine()
+    states, restart = [], None
+    for k in range(1, 10):
+        try:
+            a.advanceOneTimeStep(dt)
+        except RestartException:
+            restart = k
+            break
+        states.append(a.download())
+    print("one call per step: restart at step", restart, flush=True)
+    assert restart == 5
+    last = a.download()
+
+    b = engine()
+    with pytest.raises(RestartException):
+        b.rollout_forward(9, dt, 3)
+    got = b.download()
+    for key in ("X", "V", "C", "F"):
+        assert np.array_equal(got[key], last[key]), key
+
+    c = engine()
+    c.rollout_forward(4, dt, 3)
+    got = c.download()
+    for key in ("X", "V", "C", "F"):
+        assert np.array_equal(got[key], states[3][key]), key
+    for e in (a, b, c):
+        e.close()

@@ -71,7 +71,20 @@ struct ColliderSet {
   DevCollider c[kMaxColliders];
 };
 
-enum { CNT_GROUPS = 0, CNT_SLOTS = 1, CNT_DOMAIN = 2, CNT_CFL = 3, CNT_TOTAL = 4, CNT_ADJ_BAD = 5, CNT_N = 8 };
+// device counters: the first CNT_SORT are zeroed by every sort; CNT_ADJ_BAD
+// and CNT_CFL_RUN (the CFL flags of every step since it was last cleared)
+// are cleared by the calls that read them
+enum {
+  CNT_GROUPS = 0,
+  CNT_SLOTS = 1,
+  CNT_DOMAIN = 2,
+  CNT_CFL = 3,
+  CNT_TOTAL = 4,
+  CNT_SORT = 5,
+  CNT_ADJ_BAD = 5,
+  CNT_CFL_RUN = 6,
+  CNT_N = 8
+};
 
 // ---------------------------------------------------------------------------
 // collisions: AnalyticCollisionObject moving as x = R s X + b(t)
@@ -192,7 +205,7 @@ __global__ void k_reset_tables(const uint32_t* group_page, uint32_t ngroups, con
                                uint32_t* page_touch, int32_t* page_slot, uint32_t* counters)
 {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (counters && t < CNT_N) counters[t] = 0u;
+  if (counters && t < CNT_SORT) counters[t] = 0u;
   if (t < ngroups) {
     uint32_t p = group_page[t];
     page_count[p] = 0u;
@@ -860,7 +873,10 @@ __global__ __launch_bounds__(g2p_threads<MLS>()) void k_g2p(ParticlesView P, con
         P.F[(c * 3 + r) * S + i] = acc;
       }
   }
-  if (flags) atomicOr(&counters[CNT_CFL], flags);
+  if (flags) {
+    atomicOr(&counters[CNT_CFL], flags);
+    atomicOr(&counters[CNT_CFL_RUN], flags);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1227,6 +1243,9 @@ struct zr_engine {
   uint32_t a_hint = 0;
   // the CFL flag counter is zero (the sort clears every counter; only G2P sets it)
   bool cfl_clear = false;
+  // inside a rollout: a step does not wait for its CFL flags; the next
+  // step's sort reads them (CNT_CFL_RUN) and stops the rollout there
+  bool defer_cfl = false;
   uint32_t* hvp_bounds = nullptr;  // its XCD group ranges, FP32 then FP64 (2 x 9)
   uint32_t* hvp_order = nullptr;   // their hand-out orders (2 x hvp_order_cap)
   uint8_t* cell_order = nullptr;   // each group's cells by particle count (32 x hvp_order_cap)
@@ -1570,6 +1589,9 @@ int do_sort(zr_engine* e, zr_step_stats* st)
     e->S = e->hcount[CNT_SLOTS];
   }
   e->a_hint = A + A / 4 + 64 < n ? A + A / 4 + 64 : n;
+  // a rollout's earlier step asked for a restart: the particles are those
+  // after that step (a sort moves none)
+  if (e->defer_cfl && (e->hcount[CNT_CFL_RUN] & 1u)) return ZR_CFL_RESTART;
   rc = ensure_grid(e);
   if (rc) return rc;
   if (e->S) {
@@ -1692,11 +1714,12 @@ int do_g2p(zr_engine* e, double dt, zr_step_stats* st, bool mls)
     ZR_LAUNCH_CHECK(e);
   }
   prof_mark(e, ZR_PROF_G2P);
+  // particles moved: the sort is stale until the next one
+  e->sorted = e->rasterised = e->updated = e->be_ready = false;
+  if (e->defer_cfl) return ZR_OK;  // the flags are read by the next sort
   int rc = sync_counts(e);
   if (rc) return rc;
   uint32_t flags = e->hcount[CNT_CFL];
-  // particles moved: the sort is stale until the next one
-  e->sorted = e->rasterised = e->updated = e->be_ready = false;
   if (st) st->cfl_flags = (int32_t)flags;
   return (flags & 1u) ? ZR_CFL_RESTART : ZR_OK;
 }
@@ -1821,7 +1844,19 @@ bool adjoint_supported(zr_engine* e)
 }
 
 // reverse of one explicit step from the current (input) particle state
-int do_adjoint_step(zr_engine* e, double dt)
+// check: wait for the step and report a collider branch without a
+// derivative (CNT_ADJ_BAD, which the caller cleared); a rollout reads it once
+// at its end instead
+int adjoint_flags(zr_engine* e)
+{
+  int rc = sync_counts(e);
+  if (rc) return rc;
+  if (e->hcount[CNT_ADJ_BAD])
+    return fail(e, ZR_ERR_ARG, "adjoint: a grid node at a sphere collider's centre in the trial collision (no derivative)");
+  return ZR_OK;
+}
+
+int do_adjoint_step(zr_engine* e, double dt, bool check)
 {
   int rc = do_sort(e, nullptr);
   if (rc) return rc;
@@ -1837,7 +1872,6 @@ int do_adjoint_step(zr_engine* e, double dt)
   const double dx = e->cfg.dx;
   hipStream_t s = e->stream;
   AdjointView A = adj_view(e);
-  ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_ADJ_BAD, 0, sizeof(uint32_t), s));
   k_adj_g2p<<<e->A, 256, 0, s>>>(pview(e), A, e->order, e->group_start, e->group_page, e->group_nslot, e->gvs,
                                  (int64_t)e->grid_cap, e->stage, dx, 1.0 / dx, 4.0 / (dx * dx), dt);
   ZR_LAUNCH_CHECK(e);
@@ -1851,12 +1885,8 @@ int do_adjoint_step(zr_engine* e, double dt)
   k_adj_p2g<<<e->A, 256, 0, s>>>(pview(e), A, e->order, e->group_start, e->group_page, e->group_nslot, e->adj_grid,
                                  (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx));
   ZR_LAUNCH_CHECK(e);
-  rc = sync_counts(e);
-  if (rc) return rc;
   // the particles were not moved: the sort stays valid, the grid does too
-  if (e->hcount[CNT_ADJ_BAD])
-    return fail(e, ZR_ERR_ARG, "adjoint: a collider branch without a derivative (friction) was hit");
-  return ZR_OK;
+  return check ? adjoint_flags(e) : ZR_OK;
 }
 
 // ------------------------------------------------------------------- ADMM
@@ -2610,6 +2640,7 @@ int zr_create(const zr_config* cfg, zr_engine** out)
             hipMalloc(&e->dcount, CNT_N * sizeof(uint32_t)) == hipSuccess &&
             hipMalloc(&e->ticket, sizeof(uint32_t)) == hipSuccess &&
             hipHostMalloc(&e->hcount, CNT_N * sizeof(uint32_t)) == hipSuccess;
+  ok = ok && hipMemset(e->dcount, 0, CNT_N * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipMemset(e->page_count, 0, e->ndense * sizeof(uint32_t)) == hipSuccess &&
        hipMemset(e->page_group, 0xff, e->ndense * sizeof(int32_t)) == hipSuccess &&
        hipMemset(e->page_touch, 0xff, e->ndense * sizeof(uint32_t)) == hipSuccess &&
@@ -3166,10 +3197,8 @@ int zr_adjoint_step(zr_engine* e, double dt)
   if (!(dt > 0.0)) return fail(e, ZR_ERR_ARG, "dt must be positive");
   if (!adjoint_supported(e)) return ZR_ERR_ARG;
   if (!e->adj || e->adj_n != e->n) return fail(e, ZR_ERR_STATE, "zr_adjoint_set was not called");
-  int rc = do_adjoint_step(e, dt);
-  if (rc) return rc;
-  ZR_HIP(e, hipStreamSynchronize(e->stream));
-  return ZR_OK;
+  ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_ADJ_BAD, 0, sizeof(uint32_t), e->stream));
+  return do_adjoint_step(e, dt, true);
 }
 
 namespace {
@@ -3257,20 +3286,40 @@ int zr_checkpoint_restore(zr_engine* e, int32_t slot)
 
 // the checkpointed rollout's loops (the same calls as the Python driver's,
 // without a host round trip per call)
+namespace {
+// the steps of a rollout defer their CFL flags to the next step's sort
+struct DeferCfl {
+  zr_engine* e;
+  explicit DeferCfl(zr_engine* en) : e(en) { e->defer_cfl = true; }
+  ~DeferCfl() { e->defer_cfl = false; }
+};
+
+// the flags of the rollout's last step
+int rollout_cfl(zr_engine* e)
+{
+  int rc = sync_counts(e);
+  if (rc) return rc;
+  return (e->hcount[CNT_CFL_RUN] & 1u) ? ZR_CFL_RESTART : ZR_OK;
+}
+}  // namespace
+
 int zr_rollout_forward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0)
 {
   if (!e) return ZR_ERR_ARG;
   if (steps < 0 || every < 1 || slot0 < 0 || !(dt > 0.0)) return fail(e, ZR_ERR_ARG, "bad rollout arguments");
+  ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_CFL_RUN, 0, sizeof(uint32_t), e->stream));
+  DeferCfl defer(e);
   for (int32_t k = 0; k < steps; ++k) {
     if (k % every == 0) {
       int rc = ckpt_save(e, slot0 + k / every);
       if (rc) return rc;
     }
+    // an error, or ZR_CFL_RESTART from the sort after the step that asked
+    // for it: the rollout stops there
     int rc = zr_advance_one_time_step(e, dt, nullptr);
-    if (rc) return rc;  // an error, or ZR_CFL_RESTART: the rollout stops there
+    if (rc) return rc;
   }
-  ZR_HIP(e, hipStreamSynchronize(e->stream));
-  return ZR_OK;
+  return rollout_cfl(e);
 }
 
 int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0, int32_t scratch0)
@@ -3280,6 +3329,9 @@ int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, i
     return fail(e, ZR_ERR_ARG, "bad rollout arguments");
   if (!adjoint_supported(e)) return ZR_ERR_ARG;
   if (!e->adj || e->adj_n != e->n) return fail(e, ZR_ERR_STATE, "zr_adjoint_set was not called");
+  static_assert(CNT_CFL_RUN == CNT_ADJ_BAD + 1, "cleared together");
+  ZR_HIP(e, hipMemsetAsync(e->dcount + CNT_ADJ_BAD, 0, 2 * sizeof(uint32_t), e->stream));
+  DeferCfl defer(e);
   const int32_t nseg = (steps + every - 1) / every;
   for (int32_t seg = nseg - 1; seg >= 0; --seg) {
     const int32_t k0 = seg * every, k1 = k0 + every < steps ? k0 + every : steps;
@@ -3296,12 +3348,13 @@ int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, i
     for (int32_t k = k1 - 1; k >= k0; --k) {  // the adjoint of each step, last first
       rc = ckpt_restore(e, scratch0 + (k - k0));
       if (rc) return rc;
-      rc = do_adjoint_step(e, dt);
+      rc = do_adjoint_step(e, dt, false);
       if (rc) return rc;
     }
   }
-  ZR_HIP(e, hipStreamSynchronize(e->stream));
-  return ZR_OK;
+  int rc = adjoint_flags(e);
+  if (rc) return rc;
+  return (e->hcount[CNT_CFL_RUN] & 1u) ? ZR_CFL_RESTART : ZR_OK;
 }
 
 int zr_advance_one_time_step_admm(zr_engine* e, double dt, const zr_admm_opts* o, zr_admm_stats* ast,
