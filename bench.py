@@ -63,10 +63,10 @@ def algorithmic_bytes(kernel, n, g):
 # 256 MiB copy) and WRITE_SIZE (x 1.0) on one MI355X for these workloads
 # (profiles/hbm_traffic_counters.md); None where no counter run exists
 TRAFFIC = {
-    ("implicit-boxes", "hvp"): 137.3e6,
-    ("implicit-boxes", "hvp64"): 269.9e6,
-    ("explicit-cube", "p2g"): 42.8e6,
-    ("explicit-cube", "g2p"): 48.9e6,
+    ("implicit-boxes", "hvp"): 139.0e6,
+    ("implicit-boxes", "hvp64"): 275.4e6,
+    ("explicit-cube", "p2g"): 40.6e6,
+    ("explicit-cube", "g2p"): 49.5e6,
 }
 
 IMPLICIT_OPTS = dict(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iterations=500, cg_tolerance=1e30,
@@ -76,7 +76,7 @@ IMPLICIT_OPTS = dict(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iter
 # is quoted with the first, the second is measured beside it
 PRECONDITIONERS = dict(jacobi=1, mass=0)
 # what holds each kernel below its HBM roofline, from its rocprofv3 SQ counters
-LIMITER = dict(hvp="latency and VALU/LDS issue (waves wait ~66% of cycles; HBM traffic 0.95x the algorithmic bytes)")
+LIMITER = dict(hvp="latency and VALU/LDS issue (waves wait ~66% of cycles; HBM traffic 0.96x the algorithmic bytes)")
 # steps after the timed region whose kernel launches are timed with HIP events
 SAMPLE_STEPS = 3
 
