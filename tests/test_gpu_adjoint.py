@@ -91,6 +91,28 @@ def _loss_and_grad(X, m, target):
     return float(np.sum((com - target) ** 2)), 2 * (com - target)[None, :] * (m / m.sum())[:, None]
 
 
+def test_recorded_steps_match_checkpoints_every_step():
+    """The backward rollout's adjoint steps read the sort and grid that the
+    segment's recomputation recorded; with a checkpoint before every step
+    nothing is recomputed and each adjoint step sorts and rasterises its input
+    itself.  The two agree up to the recomputation's rounding."""
+    s = _scene(seed=9)
+    n = len(s["mass"])
+    rng = np.random.default_rng(9)
+    E = 1e4 * np.exp(0.4 * rng.normal(size=n))
+    V0 = s["V"].copy()
+    gX = rng.normal(size=(n, 3))
+    grads = {}
+    for every in (1, 6):
+        prob = Rollout(s, 12, DT, checkpoint_every=every)
+        prob.forward(E, V0)
+        grads[every] = prob.backward(gX)
+        prob.close()
+    eE, eV = rel_err(grads[6][0], grads[1][0]), rel_err(grads[6][1], grads[1][1])
+    print("recorded vs per-step checkpoints: E", eE, "V0", eV, flush=True)
+    assert eE < 1e-11 and eV < 1e-11
+
+
 @pytest.mark.parametrize("every", [4, 16])
 def test_rollout_gradients(every):
     s = _scene(seed=5)

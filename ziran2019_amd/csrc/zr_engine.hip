@@ -1173,6 +1173,14 @@ struct zr_engine {
   size_t adj_grid_cap = 0;
   std::vector<double*> ckpt;
   std::vector<double> ckpt_time;
+  // a backward rollout's recomputed steps: the sort tables and grid of each
+  // (StepRecord), so that its adjoint step need not sort and rasterise again
+  struct StepRecord {
+    void* buf = nullptr;
+    size_t cap = 0;
+    uint32_t A = 0, S = 0;
+  };
+  std::vector<StepRecord> step_rec;
   // ADMM: per particle F 9, y 9, omega 1, last dv 3 (= 22 rows of n); per
   // node vn 3, dv 3, proj 9, diag 1, rhs 3 and MINRES q 3x3, z 3, m 3x3
   double* admm_p = nullptr;  // F 9, y 9, omega, last dv 3 (+ F2 9, y2 9, omega2 with the viscoelastic branch)
@@ -1392,6 +1400,8 @@ void free_particles(zr_engine* e)
   for (auto& c : e->ckpt) dfree(c);
   e->ckpt.clear();
   e->ckpt_time.clear();
+  for (auto& r : e->step_rec) dfree(r.buf);
+  e->step_rec.clear();
   e->n = 0;
 }
 
@@ -1856,6 +1866,59 @@ int adjoint_flags(zr_engine* e)
   return ZR_OK;
 }
 
+// what the adjoint kernels read of a step's sort and grid: the engine's own
+// (after a sort and the seven-field rasterisation of the step's input) or a
+// StepRecord's copy; grid rows of `stride` nodes
+struct AdjTables {
+  uint32_t A, S;
+  const uint32_t* order;
+  const uint32_t* group_start;
+  const uint32_t* group_page;
+  const int32_t* group_nslot;
+  const uint32_t* slot_page;
+  const int32_t* slot_src;
+  double *gm, *gv, *gf, *gvs;
+  int64_t stride;
+};
+
+AdjTables engine_tables(zr_engine* e)
+{
+  return AdjTables{ e->A,        e->S,         e->order, e->group_start, e->group_page, e->group_nslot,
+                    e->slot_page, e->slot_src, e->gm,    e->gv,          e->gf,         e->gvs,
+                    (int64_t)e->grid_cap };
+}
+
+// the reverse of the step whose input the particles hold, on its tables
+int adjoint_kernels(zr_engine* e, double dt, const AdjTables& t)
+{
+  const uint32_t nn = t.S * 32;
+  if (e->adj_grid_cap < (size_t)t.stride) {
+    dfree(e->adj_grid);
+    e->adj_grid_cap = 0;
+    ZR_HIP(e, hipMalloc(&e->adj_grid, kAdjGridFields * (size_t)t.stride * sizeof(double)));
+    e->adj_grid_cap = (size_t)t.stride;
+  }
+  if ((size_t)t.A > e->stage_cap) return fail(e, ZR_ERR_STATE, "adjoint: staging buffer below the step's groups");
+  const double dx = e->cfg.dx;
+  hipStream_t s = e->stream;
+  AdjointView A = adj_view(e);
+  k_adj_g2p<<<t.A, 256, 0, s>>>(pview(e), A, t.order, t.group_start, t.group_page, t.group_nslot, t.gvs, t.stride,
+                                e->stage, dx, 1.0 / dx, 4.0 / (dx * dx), dt);
+  ZR_LAUNCH_CHECK(e);
+  GridView G{ t.gm, t.gv, t.gf, t.gvs, t.stride };
+  // the grid kernel reads the adjoint stride-wise like the grid fields
+  if (nn) {
+    k_adj_grid<<<blocks_for(nn, 256), 256, 0, s>>>(nn, t.slot_page, t.slot_src, e->stage, G, e->adj_grid, dx, dt,
+                                                   e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2],
+                                                   collider_set(e, dt), e->dcount + CNT_ADJ_BAD);
+    ZR_LAUNCH_CHECK(e);
+  }
+  k_adj_p2g<<<t.A, 256, 0, s>>>(pview(e), A, t.order, t.group_start, t.group_page, t.group_nslot, e->adj_grid,
+                                t.stride, dx, 1.0 / dx, 4.0 / (dx * dx));
+  ZR_LAUNCH_CHECK(e);
+  return ZR_OK;
+}
+
 int do_adjoint_step(zr_engine* e, double dt, bool check)
 {
   int rc = do_sort(e, nullptr);
@@ -1864,29 +1927,135 @@ int do_adjoint_step(zr_engine* e, double dt, bool check)
   if (rc) return rc;
   rc = do_grid(e, GRID_COMBINE_UPDATE, dt);
   if (rc) return rc;
-  if (e->adj_grid_cap < e->grid_cap) {
-    dfree(e->adj_grid);
-    ZR_HIP(e, hipMalloc(&e->adj_grid, kAdjGridFields * e->grid_cap * sizeof(double)));
-    e->adj_grid_cap = e->grid_cap;
-  }
-  const double dx = e->cfg.dx;
-  hipStream_t s = e->stream;
-  AdjointView A = adj_view(e);
-  k_adj_g2p<<<e->A, 256, 0, s>>>(pview(e), A, e->order, e->group_start, e->group_page, e->group_nslot, e->gvs,
-                                 (int64_t)e->grid_cap, e->stage, dx, 1.0 / dx, 4.0 / (dx * dx), dt);
-  ZR_LAUNCH_CHECK(e);
-  const uint32_t nn = e->S * 32;
-  GridView G{ e->gm, e->gv, e->gf, e->gvs, (int64_t)e->grid_cap };
-  // the grid kernel reads the adjoint stride-wise like the grid fields
-  k_adj_grid<<<blocks_for(nn, 256), 256, 0, s>>>(nn, e->slot_page, e->slot_src, e->stage, G, e->adj_grid, dx, dt,
-                                                 e->cfg.gravity[0], e->cfg.gravity[1], e->cfg.gravity[2],
-                                                 collider_set(e, dt), e->dcount + CNT_ADJ_BAD);
-  ZR_LAUNCH_CHECK(e);
-  k_adj_p2g<<<e->A, 256, 0, s>>>(pview(e), A, e->order, e->group_start, e->group_page, e->group_nslot, e->adj_grid,
-                                 (int64_t)e->grid_cap, dx, 1.0 / dx, 4.0 / (dx * dx));
-  ZR_LAUNCH_CHECK(e);
+  rc = adjoint_kernels(e, dt, engine_tables(e));
+  if (rc) return rc;
   // the particles were not moved: the sort stays valid, the grid does too
   return check ? adjoint_flags(e) : ZR_OK;
+}
+
+// up to 16 device copies in one launch (sizes in 4-byte words)
+struct CopyList {
+  const uint32_t* src[16];
+  uint32_t* dst[16];
+  uint64_t words[16];
+  int count;
+};
+
+__global__ __launch_bounds__(256) void k_multi_copy(CopyList L)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * 256u;
+  for (int q = 0; q < L.count; ++q) {
+    const uint32_t* __restrict__ src = L.src[q];
+    uint32_t* __restrict__ dst = L.dst[q];
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < L.words[q]; i += stride) dst[i] = src[i];
+  }
+}
+
+// a StepRecord's layout: order [n], group_start [A + 1], group_page [A],
+// group_nslot [8A], slot_page [S], slot_src [8S] (4-byte words), then m,
+// v 3, f 3, v* 3 over the S * 32 nodes (doubles, 8-byte aligned)
+struct RecordLayout {
+  size_t order, gstart, gpage, gnslot, spage, ssrc, grid, bytes;
+};
+
+RecordLayout record_layout(size_t n, size_t A, size_t S)
+{
+  RecordLayout L;
+  size_t w = 0;
+  L.order = w; w += n;
+  L.gstart = w; w += A + 1;
+  L.gpage = w; w += A;
+  L.gnslot = w; w += 8 * A;
+  L.spage = w; w += S;
+  L.ssrc = w; w += 8 * S;
+  w += w & 1u;
+  L.grid = w;
+  w += 2 * 10 * 32 * S;
+  L.bytes = 4 * w;
+  return L;
+}
+
+// copies the engine's sort tables and grid (after the seven-field
+// rasterisation and the grid update of a step) into record r
+int record_step(zr_engine* e, zr_engine::StepRecord& r)
+{
+  const size_t n = (size_t)e->n, A = e->A, S = e->S, nodes = 32 * S;
+  const RecordLayout L = record_layout(n, A, S);
+  if (r.cap < L.bytes) {
+    dfree(r.buf);
+    r.cap = 0;
+    const size_t cap = L.bytes + L.bytes / 4 + 4096;
+    ZR_HIP(e, hipMalloc(&r.buf, cap));
+    r.cap = cap;
+  }
+  r.A = e->A;
+  r.S = e->S;
+  uint32_t* w = static_cast<uint32_t*>(r.buf);
+  CopyList c{};
+  auto add = [&](const void* src, size_t off, size_t words) {
+    c.src[c.count] = static_cast<const uint32_t*>(src);
+    c.dst[c.count] = w + off;
+    c.words[c.count] = words;
+    ++c.count;
+  };
+  add(e->order, L.order, n);
+  add(e->group_start, L.gstart, A + 1);
+  add(e->group_page, L.gpage, A);
+  add(e->group_nslot, L.gnslot, 8 * A);
+  add(e->slot_page, L.spage, S);
+  add(e->slot_src, L.ssrc, 8 * S);
+  add(e->gm, L.grid, 2 * nodes);
+  for (int d = 0; d < 3; ++d) {
+    add(e->gv + d * e->grid_cap, L.grid + 2 * nodes * (1 + d), 2 * nodes);
+    add(e->gf + d * e->grid_cap, L.grid + 2 * nodes * (4 + d), 2 * nodes);
+    add(e->gvs + d * e->grid_cap, L.grid + 2 * nodes * (7 + d), 2 * nodes);
+  }
+  const uint64_t b = blocks_for(L.bytes / 4, 256);
+  k_multi_copy<<<b < 2048 ? (uint32_t)b : 2048u, 256, 0, e->stream>>>(c);
+  ZR_LAUNCH_CHECK(e);
+  return ZR_OK;
+}
+
+AdjTables record_tables(zr_engine* e, const zr_engine::StepRecord& r)
+{
+  const RecordLayout L = record_layout((size_t)e->n, r.A, r.S);
+  uint32_t* w = static_cast<uint32_t*>(r.buf);
+  double* g = reinterpret_cast<double*>(w + L.grid);
+  const size_t nodes = 32 * (size_t)r.S;
+  // grid rows of the record are `nodes` long; at least one node so that the
+  // stride is positive for an empty grid
+  const int64_t st = nodes ? (int64_t)nodes : 1;
+  return AdjTables{ r.A,
+                    r.S,
+                    w + L.order,
+                    w + L.gstart,
+                    w + L.gpage,
+                    reinterpret_cast<const int32_t*>(w + L.gnslot),
+                    w + L.spage,
+                    reinterpret_cast<const int32_t*>(w + L.ssrc),
+                    g,
+                    g + nodes,
+                    g + 4 * nodes,
+                    g + 7 * nodes,
+                    st };
+}
+
+// one explicit step in the rasterisation the adjoint reads (force and
+// momentum in their own fields), its tables and grid copied into r
+int advance_recorded(zr_engine* e, double dt, zr_engine::StepRecord& r)
+{
+  int rc = do_sort(e, nullptr);
+  if (rc) return rc;
+  rc = do_p2g(e, true);
+  if (rc) return rc;
+  rc = do_grid(e, GRID_COMBINE_UPDATE, dt);
+  if (rc) return rc;
+  rc = record_step(e, r);
+  if (rc) return rc;
+  rc = do_g2p(e, dt, nullptr, true);
+  if (rc < 0) return rc;
+  e->time += dt;
+  return rc;
 }
 
 // ------------------------------------------------------------------- ADMM
@@ -3337,18 +3506,25 @@ int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, i
     const int32_t k0 = seg * every, k1 = k0 + every < steps ? k0 + every : steps;
     int rc = ckpt_restore(e, slot0 + seg);
     if (rc) return rc;
-    for (int32_t k = k0; k < k1; ++k) {  // the segment's step inputs
+    // the segment's step inputs, and the sort tables and grid of each
+    // recomputed step, which its adjoint step reads instead of sorting and
+    // rasterising the same input again
+    if (e->step_rec.size() < (size_t)(k1 - k0)) e->step_rec.resize((size_t)(k1 - k0));
+    for (int32_t k = k0; k < k1; ++k) {
       rc = ckpt_save(e, scratch0 + (k - k0));
       if (rc) return rc;
       if (k < k1 - 1) {
-        rc = zr_advance_one_time_step(e, dt, nullptr);
+        rc = advance_recorded(e, dt, e->step_rec[(size_t)(k - k0)]);
         if (rc) return rc;
       }
     }
     for (int32_t k = k1 - 1; k >= k0; --k) {  // the adjoint of each step, last first
       rc = ckpt_restore(e, scratch0 + (k - k0));
       if (rc) return rc;
-      rc = do_adjoint_step(e, dt, false);
+      if (k == k1 - 1)
+        rc = do_adjoint_step(e, dt, false);
+      else
+        rc = adjoint_kernels(e, dt, record_tables(e, e->step_rec[(size_t)(k - k0)]));
       if (rc) return rc;
     }
   }
