@@ -427,8 +427,8 @@ int zr_checkpoint_restore(zr_engine* e, int32_t slot);
    steps in slots scratch0 .. scratch0 + every - 1 (recomputing them), then
    run zr_adjoint_step for each, last first; the adjoint then holds that of
    the rollout's input.  The recomputed steps keep their sort tables and grid
-   on the device (about 24 B per particle and 80 B per grid node each, up to
-   every - 1 of them) for their adjoint steps, which do not sort again.  The
+   on the device (4 B per particle, 40 B per particle group and 36 B per grid
+   page, 80 B per grid node; up to every - 1 records) for their adjoint steps, which do not sort again.  The
    adjoint steps' collider flags are read once, at the end. */
 int zr_rollout_forward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0);
 int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, int32_t slot0, int32_t scratch0);
