@@ -28,6 +28,8 @@ struct AdjointView {
   double* lam;   // [n]
   double* xg;    // [3][n] x-adjoint of the G2P part (scratch)
   double* Fg;    // [9][n] F-adjoint of the F update (scratch)
+  double* vt;    // [9][n] V0 tau of the step's input (scratch)
+  double* tb;    // [9][n] adjoint of V0 tau (scratch)
 };
 
 // d v_after / d v_before of the node's collider handling: multi_object_collision
@@ -366,7 +368,43 @@ __global__ __launch_bounds__(256) void k_adj_grid(uint32_t nn, const uint32_t* _
   for (int q = 0; q < kAdjGridFields; ++q) ag[q * S + k] = out[q];
 }
 
-// P2G + stress reverse (per group)
+// P2G + stress reverse in three passes, so that the pass holding the node
+// tile holds no SVD (one kernel of 256 VGPRs + 134 AGPRs ran at one wave
+// per SIMD): k_adj_stress writes V0 tau per particle, k_adj_p2g reverses
+// the momentum and stress transfer over the tile, k_adj_stress_rev reverses
+// the stress (the SVD again) into the F, mu and lambda adjoints.
+
+// forward V0 tau = V0 (2 mu (F - R) F^T + lam (J - 1) J I), per particle
+__global__ __launch_bounds__(256) void k_adj_stress(ParticlesView P, AdjointView A, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const int64_t S = P.stride;
+  M3<double> f, u, v;
+  double s[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
+  svd3(f, u, s, v);
+  const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i];
+  const double J = det3(f);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double rk = u.a[r][0] * v.a[k][0] + u.a[r][1] * v.a[k][1] + u.a[r][2] * v.a[k][2];
+        acc += (f.a[r][k] - rk) * f.a[c][k];
+      }
+      A.vt[(c * 3 + r) * S + i] = vol * (2.0 * mu * acc + (r == c ? lam * (J - 1.0) * J : 0.0));
+    }
+}
+
+// the transfer's reverse over the group's node tile (per group): x, v and C
+// adjoints, and the adjoint of V0 tau into A.tb
 __global__ __launch_bounds__(256) void k_adj_p2g(ParticlesView P, AdjointView A, const uint32_t* __restrict__ order,
                                                  const uint32_t* __restrict__ group_start,
                                                  const uint32_t* __restrict__ group_page,
@@ -402,36 +440,15 @@ __global__ __launch_bounds__(256) void k_adj_p2g(ParticlesView P, AdjointView A,
     for (int d = 0; d < 3; ++d) b[d] = bspline_wdw(xp[d] * inv_dx, w[d], dw[d]);
     const int lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
     if ((unsigned)lx > 1u || (unsigned)ly > 3u || (unsigned)lz > 3u) continue;
-    M3<double> f, u, v;
-    double s[3];
+    const double mass = P.m[i];
+    double vt[3][3], Cm[3][3], vp[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
-    svd3(f, u, s, v);
-    const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i], mass = P.m[i];
-    // forward V0 tau = V0 (2 mu (F - R) F^T + lam (J - 1) J I)
-    double vt[3][3];
-    {
-      const double J = det3(f);
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          double acc = 0.0;
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            const double rk = u.a[r][0] * v.a[k][0] + u.a[r][1] * v.a[k][1] + u.a[r][2] * v.a[k][2];
-            acc += (f.a[r][k] - rk) * f.a[c][k];
-          }
-          vt[r][c] = vol * (2.0 * mu * acc + (r == c ? lam * (J - 1.0) * J : 0.0));
-        }
-    }
-    double Cm[3][3], vp[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) Cm[r][c] = P.C[(c * 3 + r) * S + i];
+      for (int r = 0; r < 3; ++r) {
+        vt[r][c] = A.vt[(c * 3 + r) * S + i];
+        Cm[r][c] = P.C[(c * 3 + r) * S + i];
+      }
 #pragma unroll
     for (int d = 0; d < 3; ++d) vp[d] = P.v[d * S + i];
     double Cb[3][3] = { { 0.0 } }, tb[3][3] = { { 0.0 } }, vbar[3] = { 0.0, 0.0, 0.0 }, xb[3] = { 0.0, 0.0, 0.0 };
@@ -441,12 +458,16 @@ __global__ __launch_bounds__(256) void k_adj_p2g(ParticlesView P, AdjointView A,
 #pragma unroll
       for (int bb = 0; bb < 3; ++bb) {
         const double d1 = (double)(b[1] + bb) * dx - xp[1];
+        // the row's tile address, opaque so that the row's reads are not
+        // hoisted ahead of the previous rows' sums (register budget)
+        int row = ((lx + a) * kTileY + (ly + bb)) * kTileZ + lz;
+        asm volatile("" : "+v"(row));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const double d2 = (double)(b[2] + c) * dx - xp[2];
           double gw[3];
           const double wt_ = wgrad(w, dw, inv_dx, a, bb, c, gw);
-          const int t = ((lx + a) * kTileY + (ly + bb)) * kTileZ + (lz + c);
+          const int t = row + c;
           const double am = ta[t];
           const double ap[3] = { ta[kTileNodes + t], ta[2 * kTileNodes + t], ta[3 * kTileNodes + t] };
           const double af[3] = { ta[4 * kTileNodes + t], ta[5 * kTileNodes + t], ta[6 * kTileNodes + t] };
@@ -479,22 +500,45 @@ __global__ __launch_bounds__(256) void k_adj_p2g(ParticlesView P, AdjointView A,
       A.x[d * S + i] = A.xg[d * S + i] + xb[d];
       A.v[d * S + i] = vbar[d];
     }
-    double Fb[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         A.C[(c * 3 + r) * S + i] = Cb[r][c];
-        Fb[r][c] = A.Fg[(c * 3 + r) * S + i];
-        tb[r][c] *= vol;
+        A.tb[(c * 3 + r) * S + i] = tb[r][c];
       }
-    double mub, lab;
-    fcr_kirchhoff_adjoint(mu, lam, f, u, s, v, tb, Fb, mub, lab);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) A.F[(c * 3 + r) * S + i] = Fb[r][c];
-    A.mu[i] += mub;
-    A.lam[i] += lab;
   }
+}
+
+// the stress's reverse per particle: F adjoint = that of the F update plus
+// (d V0 tau / dF)^T tb, and the mu and lambda adjoints accumulated
+__global__ __launch_bounds__(256) void k_adj_stress_rev(ParticlesView P, AdjointView A, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const int64_t S = P.stride;
+  M3<double> f, u, v;
+  double s[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) f.a[r][c] = P.F[(c * 3 + r) * S + i];
+  svd3(f, u, s, v);
+  const double mu = P.mu[i], lam = P.lam[i], vol = P.vol[i];
+  double Fb[3][3], tb[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      Fb[r][c] = A.Fg[(c * 3 + r) * S + i];
+      tb[r][c] = A.tb[(c * 3 + r) * S + i] * vol;
+    }
+  double mub, lab;
+  fcr_kirchhoff_adjoint(mu, lam, f, u, s, v, tb, Fb, mub, lab);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) A.F[(c * 3 + r) * S + i] = Fb[r][c];
+  A.mu[i] += mub;
+  A.lam[i] += lab;
 }

@@ -1823,13 +1823,14 @@ int do_pf(zr_engine* e, double dt)
 }
 
 // ------------------------------------------------------ differentiable form
-constexpr int kAdjRows = 38;
+constexpr int kAdjRows = 56;
 
 AdjointView adj_view(zr_engine* e)
 {
   const int64_t n = e->n;
   double* a = e->adj;
-  return AdjointView{ a, a + 3 * n, a + 6 * n, a + 15 * n, a + 24 * n, a + 25 * n, a + 26 * n, a + 29 * n };
+  return AdjointView{ a,          a + 3 * n,  a + 6 * n,  a + 15 * n, a + 24 * n,
+                      a + 25 * n, a + 26 * n, a + 29 * n, a + 38 * n, a + 47 * n };
 }
 
 int ensure_adjoint(zr_engine* e)
@@ -1902,6 +1903,9 @@ int adjoint_kernels(zr_engine* e, double dt, const AdjTables& t)
   const double dx = e->cfg.dx;
   hipStream_t s = e->stream;
   AdjointView A = adj_view(e);
+  const uint32_t n = (uint32_t)e->n;
+  k_adj_stress<<<blocks_for(n, 256), 256, 0, s>>>(pview(e), A, n);
+  ZR_LAUNCH_CHECK(e);
   k_adj_g2p<<<t.A, 256, 0, s>>>(pview(e), A, t.order, t.group_start, t.group_page, t.group_nslot, t.gvs, t.stride,
                                 e->stage, dx, 1.0 / dx, 4.0 / (dx * dx), dt);
   ZR_LAUNCH_CHECK(e);
@@ -1915,6 +1919,8 @@ int adjoint_kernels(zr_engine* e, double dt, const AdjTables& t)
   }
   k_adj_p2g<<<t.A, 256, 0, s>>>(pview(e), A, t.order, t.group_start, t.group_page, t.group_nslot, e->adj_grid,
                                 t.stride, dx, 1.0 / dx, 4.0 / (dx * dx));
+  ZR_LAUNCH_CHECK(e);
+  k_adj_stress_rev<<<blocks_for(n, 256), 256, 0, s>>>(pview(e), A, n);
   ZR_LAUNCH_CHECK(e);
   return ZR_OK;
 }
