@@ -424,7 +424,8 @@ int zr_checkpoint_restore(zr_engine* e, int32_t slot);
    next step may already hold that state.
    zr_rollout_backward: after zr_adjoint_set on the rollout's output, per
    segment from the last: restore its checkpoint, save the inputs of its
-   steps in slots scratch0 .. scratch0 + every - 1 (recomputing them), then
+   steps after the first in slots scratch0 + 1 .. scratch0 + every - 1
+   (recomputing them; the first step's input is the checkpoint), then
    run zr_adjoint_step for each, last first; the adjoint then holds that of
    the rollout's input.  The recomputed steps keep their sort tables and grid
    on the device (4 B per particle, 40 B per particle group and 36 B per grid

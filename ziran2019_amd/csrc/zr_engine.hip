@@ -3517,15 +3517,17 @@ int zr_rollout_backward(zr_engine* e, int32_t steps, double dt, int32_t every, i
     // rasterising the same input again
     if (e->step_rec.size() < (size_t)(k1 - k0)) e->step_rec.resize((size_t)(k1 - k0));
     for (int32_t k = k0; k < k1; ++k) {
-      rc = ckpt_save(e, scratch0 + (k - k0));
-      if (rc) return rc;
+      if (k > k0) {  // the first step's input is the segment's checkpoint
+        rc = ckpt_save(e, scratch0 + (k - k0));
+        if (rc) return rc;
+      }
       if (k < k1 - 1) {
         rc = advance_recorded(e, dt, e->step_rec[(size_t)(k - k0)]);
         if (rc) return rc;
       }
     }
     for (int32_t k = k1 - 1; k >= k0; --k) {  // the adjoint of each step, last first
-      rc = ckpt_restore(e, scratch0 + (k - k0));
+      rc = ckpt_restore(e, k > k0 ? scratch0 + (k - k0) : slot0 + seg);
       if (rc) return rc;
       if (k == k1 - 1)
         rc = do_adjoint_step(e, dt, false);
