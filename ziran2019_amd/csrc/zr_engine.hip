@@ -1939,11 +1939,12 @@ int do_adjoint_step(zr_engine* e, double dt, bool check)
   return check ? adjoint_flags(e) : ZR_OK;
 }
 
-// up to 16 device copies in one launch (sizes in 4-byte words)
+// up to kCopySegs device copies in one launch (sizes in 4-byte words)
+constexpr int kCopySegs = 16;
 struct CopyList {
-  const uint32_t* src[16];
-  uint32_t* dst[16];
-  uint64_t words[16];
+  const uint32_t* src[kCopySegs];
+  uint32_t* dst[kCopySegs];
+  uint64_t words[kCopySegs];
   int count;
 };
 
@@ -1998,6 +1999,8 @@ int record_step(zr_engine* e, zr_engine::StepRecord& r)
   r.S = e->S;
   uint32_t* w = static_cast<uint32_t*>(r.buf);
   CopyList c{};
+  // 16 segments: six tables, the mass and three rows each of v, f and v*
+  static_assert(kCopySegs >= 6 + 1 + 3 * 3, "record segments");
   auto add = [&](const void* src, size_t off, size_t words) {
     c.src[c.count] = static_cast<const uint32_t*>(src);
     c.dst[c.count] = w + off;
