@@ -76,7 +76,8 @@ IMPLICIT_OPTS = dict(newton_max_iterations=3, newton_tolerance=1e-5, cg_max_iter
 # is quoted with the first, the second is measured beside it
 PRECONDITIONERS = dict(jacobi=1, mass=0)
 # what holds each kernel below its HBM roofline, from its rocprofv3 SQ counters
-LIMITER = dict(hvp="latency and VALU/LDS issue (waves wait ~66% of cycles; HBM traffic 0.96x the algorithmic bytes)")
+# (profiles/hvp_issue_counters.md)
+LIMITER = dict(hvp="latency and VALU/LDS issue (waves wait 64% of cycles, LDS bank conflicts 0.60 of LDS-active cycles; HBM traffic 0.96x the algorithmic bytes)")
 # steps after the timed region whose kernel launches are timed with HIP events
 SAMPLE_STEPS = 3
 
